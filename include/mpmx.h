@@ -1,0 +1,211 @@
+/*
+ * libmpmx - C ABI of the MI355X (gfx950) hot path of MPMWarpYDW.
+ *
+ * Every entry point replaces one Warp kernel launch (or one Warp runtime
+ * facility) of the reference; the comment above each declaration cites the
+ * reference file:line it replaces (paths relative to the MPMWarpYDW tree).
+ *
+ * Conventions
+ *  - All pointers are device pointers (hipMalloc'd, or torch CUDA tensors'
+ *    data_ptr()), unless named host_*.
+ *  - Structure-of-arrays layout: a per-particle vec3 array of n items is three
+ *    float planes [3][n]; a mat33 array is nine planes [9][n] in row-major
+ *    component order; grid arrays hold G = gx*gy*gz nodes with node index
+ *    (ix*gy + iy)*gz + iz, vec3 grid arrays are [3][G].
+ *  - Labels are int32; "active" means activeLabel == 1.
+ *  - stream is a hipStream_t (0 = the null stream).  Calls are asynchronous on
+ *    that stream unless stated.  No entry point allocates device memory: the
+ *    ones that need scratch take a caller-provided workspace whose size is
+ *    returned by the matching *_workspace_bytes() query.
+ *  - Return value: 0 on success, a negative MPMX_E* code for an argument error,
+ *    or a positive hipError_t.  mpmx_last_error() returns a thread-local text.
+ *  - Environment MPMX_DEBUG=1 synchronises and checks after every launch (the
+ *    analogue of the reference's wp.config.verify_cuda, runMPMYDW.py:7).
+ */
+#ifndef MPMX_H
+#define MPMX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPMX_EINVAL (-1)   /* bad argument (null pointer, negative size, ...) */
+#define MPMX_ERANGE (-2)   /* size beyond a documented maximum */
+
+/* boundary-condition selector of mpmx_grid_update (simulationRoutines.py:124-228) */
+#define MPMX_BC_NONE 0
+#define MPMX_BC_FRICTION 1
+
+const char* mpmx_last_error(void);
+int mpmx_version(void);
+/* device synchronisation used by the host wrappers (mirrors .numpy() syncs) */
+int mpmx_stream_sync(void* stream);
+/* fill n int32 / float words with a value (wp.array.zero_()/fill_()) */
+int mpmx_fill_i32(int32_t* dst, long n, int32_t value, void* stream);
+int mpmx_fill_f32(float* dst, long n, float value, void* stream);
+/* wp.copy / wp.array.assign on float planes */
+int mpmx_copy_f32(float* dst, const float* src, long n, void* stream);
+
+/* ---- MPM kernels (utils/mpmRoutines.py) ---------------------------------- */
+
+/* compute_mu_lam_bulk_from_E_nu, mpmRoutines.py:206-226 */
+int mpmx_compute_mu_lam_bulk(int n, const float* E, const float* nu, float* mu, float* lam,
+                             float* bulk, void* stream);
+
+/* compute_stress_from_F_trial (+ drucker_prager_return_mapping, kirchoff_stress,
+ * wp.svd3), mpmRoutines.py:229-471; launched at simulationRoutines.py:15-45.
+ * x, v, x_init, v_init: vec3[n]; F, F_trial, C, stress: mat33[n]. */
+int mpmx_compute_stress_from_F_trial(int n, const int32_t* activeLabel, int32_t* materialLabel,
+                                     const float* x, float* v, float* x_init, float* v_init,
+                                     float* F, const float* F_trial, float* mu, float* lam,
+                                     float* ys, const float* hardening, const float* softening,
+                                     const float* density, const float* strainCriteria,
+                                     float eff, const float* eta_shear, const float* eta_bulk,
+                                     const float* C, float* stress, float* acc_strain,
+                                     float* damage, int constitutive_model, float* alpha,
+                                     void* stream);
+
+/* p2g_apic_with_stress, mpmRoutines.py:485-569 (launch simulationRoutines.py:48-69),
+ * as a deterministic binned reduction: particles are counting-sorted by base cell
+ * into 4x4x4-cell tiles, each tile is reduced in LDS in a fixed order and the
+ * 6x6x6-node tile partials are summed per node in a fixed order.
+ * Writes grid_m[G] and grid_v_in[3][G] completely (no prior zeroing needed).
+ * The workspace must hold mpmx_p2g_workspace_bytes(n, gx, gy, gz) bytes.
+ * *host_out_of_grid (optional, synchronising when non-null) receives the number of
+ * active MPM particles whose stencil left the grid; they are skipped (the
+ * reference has no bounds check there). */
+size_t mpmx_p2g_workspace_bytes(int n, int gx, int gy, int gz);
+int mpmx_p2g_apic_with_stress(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                              const float* stress, const float* x, const float* v,
+                              const float* C, const float* vol, const float* mass, float dx,
+                              float inv_dx, float minx, float miny, float minz,
+                              float rpic_damping, int gx, int gy, int gz, float* grid_m,
+                              float* grid_v_in, float dt, void* workspace, size_t ws_bytes,
+                              int* host_out_of_grid, void* stream);
+
+/* p2g_apic_with_stress followed by steps 4-6 of mpmSimulationStep
+ * (simulationRoutines.py:48-140) in the same pass over the grid: also writes
+ * grid_v_out (normalisation + gravity, damping when damping_scale < 1, then the
+ * boundary condition bc), so no separate zeroing or grid launch is needed.
+ * Same workspace as mpmx_p2g_apic_with_stress. */
+int mpmx_p2g_grid_update(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                         const float* stress, const float* x, const float* v, const float* C,
+                         const float* vol, const float* mass, float dx, float inv_dx, float minx,
+                         float miny, float minz, float rpic_damping, int gx, int gy, int gz,
+                         float* grid_m, float* grid_v_in, float* grid_v_out, float gxv, float gyv,
+                         float gzv, float dt, float damping_scale, int bc, float friction,
+                         float gravity_magnitude, int boundary_padding, void* workspace,
+                         size_t ws_bytes, void* stream);
+
+/* grid_normalization_and_gravity, mpmRoutines.py:691-704: writes v_out only
+ * where m > 1e-15 (the caller zeroes v_out first, simulationRoutines.py:13) */
+int mpmx_grid_normalization_and_gravity(int gx, int gy, int gz, const float* grid_m,
+                                        const float* grid_v_in, float* grid_v_out, float gxv,
+                                        float gyv, float gzv, float dt, void* stream);
+/* add_damping_via_grid, mpmRoutines.py:706-711 */
+int mpmx_add_damping_via_grid(int gx, int gy, int gz, float* grid_v_out, float scale,
+                              void* stream);
+/* collideBounds (+ apply_coulomb_friction), mpmRoutines.py:915-993 */
+int mpmx_collide_bounds(float* grid_v_out, int gx, int gy, int gz, float friction, float dt,
+                        float gravity_magnitude, int boundary_padding, void* stream);
+/* steps 4-6 of mpmSimulationStep (simulationRoutines.py:94-140) fused into one
+ * pass: normalisation + gravity, damping when damping_scale < 1, then the
+ * boundary condition bc (MPMX_BC_*). */
+int mpmx_grid_update(int gx, int gy, int gz, const float* grid_m, const float* grid_v_in,
+                     float* grid_v_out, float gxv, float gyv, float gzv, float dt,
+                     float damping_scale, int bc, float friction, float gravity_magnitude,
+                     int boundary_padding, void* stream);
+
+/* g2p (+ update_cov), mpmRoutines.py:715-802; launched simulationRoutines.py:231-250.
+ * cov is float[6*n] packed per particle [xx,xy,xz,yy,yz,zz] (the reference layout). */
+int mpmx_g2p(float dt, int n, const int32_t* activeLabel, const int32_t* materialLabel,
+             float* x, float* v, float* C, const float* F, float* F_trial, float* cov,
+             float inv_dx, const float* grid_v_out, int gx, int gy, int gz, int update_cov,
+             float minx, float miny, float minz, void* stream);
+
+/* ---- reductions (utils/simulationRoutines.py:538-675), FP64 on the device,
+ * two-level and in a fixed order, so the results are run-to-run identical.
+ * All results are written to device memory; read them with a synchronising
+ * copy.  The workspace must hold mpmx_reduce_workspace_bytes(n) bytes. */
+size_t mpmx_reduce_workspace_bytes(int n);
+/* velocityConvergence :538-549 and countActiveParticles :551-558:
+ * out_sum = sum over active of |v|/radius, out_count = number of active */
+int mpmx_velocity_convergence(int n, const int32_t* activeLabel, const float* v,
+                              const float* radius, double* out_sum, int32_t* out_count,
+                              void* workspace, size_t ws_bytes, void* stream);
+/* sum_float_array_active :598-610 (compute_mean_gpu filtered=True, :633-675) */
+int mpmx_sum_float_array_active(int n, const float* arr, const int32_t* activeLabel,
+                                double* out_sum, int32_t* out_count, void* workspace,
+                                size_t ws_bytes, void* stream);
+/* countParticlesByType :560-585: out4 = {activeMPM, activeXPBD, inactiveMPM, inactiveXPBD} */
+int mpmx_count_particles_by_type(int n, const int32_t* activeLabel,
+                                 const int32_t* materialLabel, int32_t* out4, void* workspace,
+                                 size_t ws_bytes, void* stream);
+
+/* ---- XPBD (utils/xpbdRoutines.py) ---------------------------------------- */
+
+/* wp.HashGrid(dx,dy,dz).build(points, cell_width) (simStates.py:174,
+ * simulationRoutines.py:311): point_ids[n] lists the points sorted by hash cell
+ * (stable: ascending id inside a cell); cell_start[dims+1] with cell c holding
+ * point_ids[cell_start[c] .. cell_start[c+1]).  Workspace:
+ * mpmx_hashgrid_workspace_bytes(n, dims). */
+size_t mpmx_hashgrid_workspace_bytes(int n, int dimx, int dimy, int dimz);
+int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+                        int32_t* point_ids, int32_t* cell_start, void* workspace,
+                        size_t ws_bytes, void* stream);
+
+/* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
+int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,
+                                  void* stream);
+/* integrateParticlesXPBD, xpbdRoutines.py:23-48 (x_new/v_new may alias x/v) */
+int mpmx_integrate_particles_xpbd(int n, const int32_t* activeLabel,
+                                  const int32_t* materialLabel, const float* x, const float* v,
+                                  float gxv, float gyv, float gzv, float dt, float* x_new,
+                                  float* v_new, float v_max, void* stream);
+/* my_solve_particle_bound_contacts, xpbdRoutines.py:49-136 (adds into delta) */
+int mpmx_solve_particle_bound_contacts(int n, const int32_t* activeLabel,
+                                       const int32_t* materialLabel, const float* x,
+                                       const float* v, const float* mass, const float* radius,
+                                       float mu, float static_threshold, float static_friction,
+                                       float minx, float miny, float minz, float maxx, float maxy,
+                                       float maxz, float dt, float relaxation, float* delta,
+                                       void* stream);
+/* my_solve_particle_particle_contacts, xpbdRoutines.py:138-219 (adds into deltas);
+ * candidates are visited in wp.hash_grid_query order (cells x fastest, then y,
+ * then z; ids ascending inside a cell). */
+int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel,
+                                          int32_t* materialLabel, const int32_t* point_ids,
+                                          const int32_t* cell_start, float cell_width, int dimx,
+                                          int dimy, int dimz, const float* x, const float* v,
+                                          const float* mass, const float* radius, float k_mu,
+                                          float static_threshold, float static_friction,
+                                          float k_cohesion, float max_radius, float dt,
+                                          float relaxation, int contact_lock, float* deltas,
+                                          void* stream);
+/* my_apply_particle_deltas, xpbdRoutines.py:221-257 (x_out/v_out may alias x_pred) */
+int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                               const float* x_orig, const float* x_pred, const float* delta,
+                               float dt, float v_max, float* x_out, float* v_out, void* stream);
+/* sleepParticles, xpbdRoutines.py:313-334; counters[2] = {total, sleeping} are
+ * incremented (zero them first, simulationRoutines.py:424-425) */
+int mpmx_sleep_particles(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                         float threshold, const float* radius, const float* x_prev, float* x_after,
+                         float* dist_total, float dt, int32_t* counters, void* stream);
+/* apply_mpm_contact_impulses, xpbdRoutines.py:259-310 */
+int mpmx_apply_mpm_contact_impulses(int n, const int32_t* activeLabel,
+                                    const int32_t* materialLabel, const float* x_orig,
+                                    const float* v_orig, float* x_final, const float* v_final,
+                                    float dt, float v_max, float coupling, int xpbd_only,
+                                    float* v_out, void* stream);
+/* swellParticlesType2, xpbdRoutines.py:354-376 */
+int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                               const float* x_init, const float* x, float* radius, float factor,
+                               float factor_max, const float* swell_radius,
+                               const float* base_radius, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPMX_H */

@@ -1,0 +1,100 @@
+"""ctypes binding of libmpmx.so (the C ABI declared in include/mpmx.h).
+
+The library is loaded once.  Every binding declares its argument types, so a
+call with the wrong number or kind of arguments raises instead of corrupting
+the stack, and every non-zero return code raises ``MpmxError`` with the
+library's own message.  There is no fallback: without the built library, or
+without a GPU, the hot path fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpmx.so")
+
+P = C.c_void_p
+I = C.c_int
+L = C.c_long
+F = C.c_float
+S = C.c_size_t
+IP = C.POINTER(C.c_int)
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "mpmx_last_error": [],
+    "mpmx_version": [],
+    "mpmx_stream_sync": [P],
+    "mpmx_fill_i32": [P, L, I, P],
+    "mpmx_fill_f32": [P, L, F, P],
+    "mpmx_copy_f32": [P, P, L, P],
+    "mpmx_compute_mu_lam_bulk": [I, P, P, P, P, P, P],
+    "mpmx_compute_stress_from_F_trial": [I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, F, P, P, P, P, P, P, I, P,
+                                         P],
+    "mpmx_p2g_workspace_bytes": [I, I, I, I],
+    "mpmx_p2g_apic_with_stress": [I, P, P, P, P, P, P, P, P, F, F, F, F, F, F, I, I, I, P, P, F, P, S, IP, P],
+    "mpmx_p2g_grid_update": [I, P, P, P, P, P, P, P, P, F, F, F, F, F, F, I, I, I, P, P, P, F, F, F, F, F, I, F, F,
+                             I, P, S, P],
+    "mpmx_grid_normalization_and_gravity": [I, I, I, P, P, P, F, F, F, F, P],
+    "mpmx_add_damping_via_grid": [I, I, I, P, F, P],
+    "mpmx_collide_bounds": [P, I, I, I, F, F, F, I, P],
+    "mpmx_grid_update": [I, I, I, P, P, P, F, F, F, F, F, I, F, F, I, P],
+    "mpmx_g2p": [F, I, P, P, P, P, P, P, P, P, F, P, I, I, I, I, F, F, F, P],
+    "mpmx_reduce_workspace_bytes": [I],
+    "mpmx_velocity_convergence": [I, P, P, P, P, P, P, S, P],
+    "mpmx_sum_float_array_active": [I, P, P, P, P, P, S, P],
+    "mpmx_count_particles_by_type": [I, P, P, P, P, S, P],
+    "mpmx_hashgrid_workspace_bytes": [I, I, I, I],
+    "mpmx_hashgrid_build": [I, P, F, I, I, I, P, P, P, S, P],
+    "mpmx_reset_mpm_contact_labels": [I, P, P, P],
+    "mpmx_integrate_particles_xpbd": [I, P, P, P, P, F, F, F, F, P, P, F, P],
+    "mpmx_solve_particle_bound_contacts": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
+    "mpmx_solve_particle_particle_contacts": [I, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F, F, I, P, P],
+    "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],
+    "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
+    "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],
+    "mpmx_swell_particles_type2": [I, P, P, P, P, P, F, F, P, P, P],
+}
+_RESTYPE = {
+    "mpmx_last_error": C.c_char_p,
+    "mpmx_p2g_workspace_bytes": S,
+    "mpmx_reduce_workspace_bytes": S,
+    "mpmx_hashgrid_workspace_bytes": S,
+}
+
+
+class MpmxError(RuntimeError):
+    pass
+
+
+class _Lib:
+    def __init__(self):
+        self._dll = None
+
+    def load(self):
+        if self._dll is None:
+            if not os.path.exists(LIB_PATH):
+                raise MpmxError(
+                    f"{LIB_PATH} is missing: build it with `python -m mpmwarpydw_amd.build` "
+                    "(there is no CPU fallback for the hot path)")
+            dll = C.CDLL(LIB_PATH)
+            for name, args in SIGNATURES.items():
+                fn = getattr(dll, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPE.get(name, C.c_int)
+            self._dll = dll
+        return self._dll
+
+    def raw(self, name):
+        return getattr(self.load(), name)
+
+    def __call__(self, name, *args):
+        rc = getattr(self.load(), name)(*args)
+        if rc != 0:
+            msg = self.load().mpmx_last_error().decode(errors="replace")
+            raise MpmxError(f"{name} returned {rc}: {msg}")
+        return rc
+
+
+lib = _Lib()

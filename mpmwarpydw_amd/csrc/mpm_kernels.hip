@@ -1,0 +1,406 @@
+// Per-particle and per-node MPM kernels of the hot path (gfx950):
+//   compute_mu_lam_bulk_from_E_nu, compute_stress_from_F_trial (return map +
+//   Kirchhoff stress + Kelvin-Voigt), grid normalisation / damping /
+//   collideBounds (separate and fused), g2p.
+// The deterministic P2G lives in p2g.hip, the reductions in reduce.hip.
+#include "mpmx_device.hpp"
+#include "mpmx_internal.hpp"
+
+namespace mpmx {
+
+// Warp's scalar min/max builtins are a<b?a:b and a>b?a:b (NaN falls to b).
+__device__ __forceinline__ float wmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float wmax(float a, float b) { return a > b ? a : b; }
+
+// --------------------------------------------------------------------------
+// compute_mu_lam_bulk_from_E_nu (mpmRoutines.py:206-226)
+// --------------------------------------------------------------------------
+__global__ void mu_lam_bulk_kernel(int n, const float* __restrict__ E, const float* __restrict__ nu,
+                                   float* __restrict__ mu, float* __restrict__ lam, float* __restrict__ bulk) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float e = E[p], v = nu[p];
+    float m = e / (2.0f * (1.0f + v));
+    float l = e * v / ((1.0f + v) * (1.0f - 2.0f * v));
+    mu[p] = m;
+    lam[p] = l;
+    bulk[p] = l + (2.0f / 3.0f) * m;
+}
+
+// --------------------------------------------------------------------------
+// compute_stress_from_F_trial (mpmRoutines.py:229-312) with
+// drucker_prager_return_mapping (:314-453) and kirchoff_stress (:456-471).
+//
+// One SVD per particle: the reference's second svd3 of the returned F
+// (mpmRoutines.py:288) is the first one again in the elastic branch and
+// U diag(exp eps) V^T in the plastic branch, and U diag(c) V^T F^T with
+// F = U diag(s) V^T equals U diag(c_k s_k) U^T, so the Kirchhoff stress is
+// assembled directly as U diag(d) U^T (symmetric, as the reference's
+// (tau + tau^T)/2 makes it).
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) stress_kernel(
+    int n, const int32_t* __restrict__ activeLabel, int32_t* __restrict__ materialLabel,
+    const float* __restrict__ x, float* __restrict__ v, float* __restrict__ x_init, float* __restrict__ v_init,
+    float* __restrict__ F, const float* __restrict__ F_trial, float* __restrict__ mu_a, float* __restrict__ lam_a,
+    float* __restrict__ ys_a, const float* __restrict__ hard_a, const float* __restrict__ soft_a,
+    const float* __restrict__ density, const float* __restrict__ sc_a, float eff,
+    const float* __restrict__ eta_s_a, const float* __restrict__ eta_b_a, const float* __restrict__ C_a,
+    float* __restrict__ stress, float* __restrict__ acc_a, float* __restrict__ damage_a, float* __restrict__ alpha_a) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int label = materialLabel[p];
+    if (label > 1 || activeLabel[p] != 1) return;
+
+    float Ft[9];
+    ld9(F_trial, n, p, Ft);
+    Svd3 sv;
+    svd3(Ft, sv);
+
+    float mu = mu_a[p], lam = lam_a[p], ys = ys_a[p], al = alpha_a[p], D = damage_a[p];
+    float eps[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
+    float esum = eps[0] + eps[1] + eps[2];
+    float mean_eps = esum / 3.0f;
+    float edev[3] = {eps[0] - mean_eps, eps[1] - mean_eps, eps[2] - mean_eps};
+    float tau[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) tau[k] = 2.0f * mu * eps[k] + lam * esum;
+    float p_stress = (tau[0] + tau[1] + tau[2]) / 3.0f;
+    float td0 = tau[0] - p_stress, td1 = tau[1] - p_stress, td2 = tau[2] - p_stress;
+    float sigma_eq = sqrtf(td0 * td0 + td1 * td1 + td2 * td2);
+    float yield_eff = (1.0f - D) * (ys - al * p_stress);
+
+    int branch = 0;
+    float epsn[3] = {0.0f, 0.0f, 0.0f};
+    if (sigma_eq > yield_eff || ys < 0.0f || yield_eff < 0.0f) {
+        float edn = sqrtf(edev[0] * edev[0] + edev[1] * edev[1] + edev[2] * edev[2]) + 1.0e-6f;
+        float dgam = edn - yield_eff / (2.0f * mu);
+        float dep = sqrtf(2.0f / 3.0f) * dgam;
+        acc_a[p] = acc_a[p] + dep;
+        float sc = sc_a[p];
+        float Dn = (sc > 0.0f) ? wmin(1.0f, D + dep / sc) : 1.0f;
+        if (ys < 0.0f || yield_eff < 0.0f) Dn = 1.0f;
+        damage_a[p] = Dn;
+        if (Dn >= 1.0f && label == 1) {
+            // phase transition to XPBD (mpmRoutines.py:379-408)
+            branch = 2;
+            materialLabel[p] = 2;
+            f3 xp = ld3(x, n, p);
+            st3(x_init, n, p, xp);
+            float rho = density[p];
+            float u = 0.5f * (tau[0] * eps[0] + tau[1] * eps[1] + tau[2] * eps[2]);
+            float v_expected = sqrtf(2.0f * eff * u / rho);
+            f3 vp = ld3(v, n, p);
+            f3 vdir = normalize3(f3{vp.x + 1.0e-12f, vp.y + 1.0e-12f, vp.z + 1.0e-12f});
+            f3 vn = f3{vp.x + v_expected * vdir.x, vp.y + v_expected * vdir.y, vp.z + v_expected * vdir.z};
+            st3(v, n, p, vn);
+            st3(v_init, n, p, vn);
+            ys_a[p] = 0.0f;
+            mu_a[p] = 0.0f;
+            lam_a[p] = 0.0f;
+            alpha_a[p] = 0.0f;
+            mu = 0.0f;
+            lam = 0.0f;
+        } else {
+            branch = 1;
+            float r = dgam / edn;
+            float volc = (0.3f * al) * dgam;
+#pragma unroll
+            for (int k = 0; k < 3; k++) epsn[k] = (eps[k] - r * edev[k]) + volc;
+            ys_a[p] = wmax(0.0f, ys + 2.0f * mu * (hard_a[p] - soft_a[p]) * dgam);
+        }
+    }
+
+    // ---- deformation gradient and principal Kirchhoff stresses ----
+    float d[3];
+    float Fo[9];
+    if (branch == 1) {
+        float e[3], l[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            e[k] = expf(epsn[k]);
+            l[k] = (e[k] >= 1.0e-6f) ? epsn[k] : logf(1.0e-6f);
+        }
+        float ls = l[0] + l[1] + l[2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float dk = 2.0f * mu * l[k] + lam * ls;
+            d[k] = (e[k] >= 1.0e-6f) ? dk : dk * (e[k] / 1.0e-6f);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                Fo[i * 3 + j] = sv.U[i * 3 + 0] * e[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * e[1] * sv.V[j * 3 + 1] +
+                                sv.U[i * 3 + 2] * e[2] * sv.V[j * 3 + 2];
+    } else {
+        float l[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) l[k] = log_clamped(sv.s[k], sv.lam[k], 1.0e-6f);
+        float ls = l[0] + l[1] + l[2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float dk = 2.0f * mu * l[k] + lam * ls;
+            d[k] = (sv.s[k] >= 1.0e-6f) ? dk : dk * (sv.s[k] / 1.0e-6f);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) Fo[k] = Ft[k];
+    }
+    st9(F, n, p, Fo);
+
+    // tau = U diag(d) U^T, plus Kelvin-Voigt with the APIC C as grad v (mpmRoutines.py:294-310)
+    float G[9];
+    ld9(C_a, n, p, G);
+    float s00 = G[0], s11 = G[4], s22 = G[8];
+    float s01 = 0.5f * (G[1] + G[3]), s02 = 0.5f * (G[2] + G[6]), s12 = 0.5f * (G[5] + G[7]);
+    float vr = s00 + s11 + s22;
+    float es = 2.0f * eta_s_a[p], eb = eta_b_a[p];
+    float third = vr / 3.0f;
+    float T[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i; j < 3; j++) {
+            float t = sv.U[i * 3 + 0] * d[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * d[1] * sv.U[j * 3 + 1] +
+                      sv.U[i * 3 + 2] * d[2] * sv.U[j * 3 + 2];
+            T[i * 3 + j] = t;
+            T[j * 3 + i] = t;
+        }
+    T[0] += es * (s00 - third) + eb * vr;
+    T[4] += es * (s11 - third) + eb * vr;
+    T[8] += es * (s22 - third) + eb * vr;
+    T[1] += es * s01; T[3] += es * s01;
+    T[2] += es * s02; T[6] += es * s02;
+    T[5] += es * s12; T[7] += es * s12;
+    st9(stress, n, p, T);
+}
+
+// --------------------------------------------------------------------------
+// grid kernels (mpmRoutines.py:691-711, 915-993)
+// --------------------------------------------------------------------------
+__global__ void grid_norm_kernel(long G, const float* __restrict__ m, const float* __restrict__ vin,
+                                 float* __restrict__ vout, float gx, float gy, float gz, float dt) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    float mm = m[i];
+    if (mm > 1.0e-15f) {
+        float inv = 1.0f / mm;
+        vout[i] = vin[i] * inv + dt * gx;
+        vout[G + i] = vin[G + i] * inv + dt * gy;
+        vout[2 * G + i] = vin[2 * G + i] * inv + dt * gz;
+    }
+}
+
+__global__ void grid_damp_kernel(long G, float* __restrict__ vout, float scale) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * G) return;
+    vout[i] = vout[i] * scale;
+}
+
+__global__ void collide_kernel(float* __restrict__ vout, int gx, int gy, int gz, float mu, float dt, float g,
+                               int pad) {
+    long G = (long)gx * gy * gz;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    int iz = (int)(i % gz);
+    int iy = (int)((i / gz) % gy);
+    int ix = (int)(i / ((long)gy * gz));
+    // interior nodes are never touched: skip the load
+    if (ix > pad && ix < gx - pad && iy > pad && iy < gy - pad && iz > pad && iz < gz - pad) return;
+    f3 v = ld3(vout, G, i);
+    f3 o = collide_node(v, ix, iy, iz, gx, gy, gz, mu, dt, g, pad);
+    st3(vout, G, i, o);
+}
+
+}  // namespace mpmx
+
+namespace mpmx {
+__global__ void grid_update_kernel(int gx, int gy, int gz, const float* __restrict__ m, const float* __restrict__ vin,
+                                   float* __restrict__ vout, float gxv, float gyv, float gzv, float dt, float damping,
+                                   int bc, float fric, float gmag, int pad) {
+    long G = (long)gx * gy * gz;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    int iz = (int)(i % gz);
+    int iy = (int)((i / gz) % gy);
+    int ix = (int)(i / ((long)gy * gz));
+    f3 v = grid_update_node(m[i], ld3(vin, G, i), ix, iy, iz, gx, gy, gz, gxv, gyv, gzv, dt, damping, bc, fric,
+                            gmag, pad);
+    st3(vout, G, i, v);
+}
+
+// --------------------------------------------------------------------------
+// g2p + update_cov (mpmRoutines.py:715-802)
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) g2p_kernel(
+    float dt, int n, const int32_t* __restrict__ activeLabel, const int32_t* __restrict__ materialLabel,
+    float* __restrict__ x, float* __restrict__ v, float* __restrict__ C, const float* __restrict__ F,
+    float* __restrict__ F_trial, float* __restrict__ cov, float inv_dx, const float* __restrict__ gv, int gx, int gy,
+    int gz, int update_cov, float minx, float miny, float minz) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (activeLabel[p] != 1 || materialLabel[p] > 1) return;
+    f3 xp = ld3(x, n, p);
+    float fx[3];
+    int b0 = base_axis(xp.x, minx, inv_dx, fx[0]);
+    int b1 = base_axis(xp.y, miny, inv_dx, fx[1]);
+    int b2 = base_axis(xp.z, minz, inv_dx, fx[2]);
+    if (b0 < 0 || b1 < 0 || b2 < 0 || b0 > gx - 3 || b1 > gy - 3 || b2 > gz - 3) return;
+    float w[3][3], dw[3][3];
+    bspline(fx[0], w[0], dw[0]);
+    bspline(fx[1], w[1], dw[1]);
+    bspline(fx[2], w[2], dw[2]);
+    long G = (long)gx * gy * gz;
+    float nv[3] = {0.0f, 0.0f, 0.0f};
+    float nB[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float nG[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float wxy = w[0][i] * w[1][j];
+            float gx_ = dw[0][i] * w[1][j] * inv_dx;
+            float gy_ = w[0][i] * dw[1][j] * inv_dx;
+            long row = ((long)(b0 + i) * gy + (b1 + j)) * gz + b2;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                long idx = row + k;
+                float vx = gv[idx], vy = gv[G + idx], vz = gv[2 * G + idx];
+                float wt = wxy * w[2][k];
+                float d0 = (float)i - fx[0], d1 = (float)j - fx[1], d2 = (float)k - fx[2];
+                float g0 = gx_ * w[2][k], g1 = gy_ * w[2][k], g2 = wxy * dw[2][k] * inv_dx;
+                nv[0] += vx * wt; nv[1] += vy * wt; nv[2] += vz * wt;
+                float wvx = vx * wt, wvy = vy * wt, wvz = vz * wt;
+                nB[0] += wvx * d0; nB[1] += wvx * d1; nB[2] += wvx * d2;
+                nB[3] += wvy * d0; nB[4] += wvy * d1; nB[5] += wvy * d2;
+                nB[6] += wvz * d0; nB[7] += wvz * d1; nB[8] += wvz * d2;
+                nG[0] += vx * g0; nG[1] += vx * g1; nG[2] += vx * g2;
+                nG[3] += vy * g0; nG[4] += vy * g1; nG[5] += vy * g2;
+                nG[6] += vz * g0; nG[7] += vz * g1; nG[8] += vz * g2;
+            }
+        }
+    }
+    st3(v, n, p, f3{nv[0], nv[1], nv[2]});
+    st3(x, n, p, f3{xp.x + dt * nv[0], xp.y + dt * nv[1], xp.z + dt * nv[2]});
+    float sC = inv_dx * 4.0f;
+    float Cn[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Cn[k] = nB[k] * sC;
+    st9(C, n, p, Cn);
+    // F_trial = (I + dt G) F, evaluated as F + dt (G F)
+    float Fp[9], Fn[9];
+    ld9(F, n, p, Fp);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            Fn[i * 3 + j] = Fp[i * 3 + j] +
+                            dt * (nG[i * 3 + 0] * Fp[0 * 3 + j] + nG[i * 3 + 1] * Fp[1 * 3 + j] + nG[i * 3 + 2] * Fp[2 * 3 + j]);
+    st9(F_trial, n, p, Fn);
+    if (update_cov) {
+        float* c6 = cov + 6L * p;
+        float cn[9] = {c6[0], c6[1], c6[2], c6[1], c6[3], c6[4], c6[2], c6[4], c6[5]};
+        float c1[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                float gc = nG[i * 3 + 0] * cn[0 * 3 + j] + nG[i * 3 + 1] * cn[1 * 3 + j] + nG[i * 3 + 2] * cn[2 * 3 + j];
+                float cg = cn[i * 3 + 0] * nG[j * 3 + 0] + cn[i * 3 + 1] * nG[j * 3 + 1] + cn[i * 3 + 2] * nG[j * 3 + 2];
+                c1[i * 3 + j] = cn[i * 3 + j] + dt * (gc + cg);
+            }
+        c6[0] = c1[0]; c6[1] = c1[1]; c6[2] = c1[2]; c6[3] = c1[4]; c6[4] = c1[5]; c6[5] = c1[8];
+    }
+}
+
+}  // namespace mpmx
+
+using namespace mpmx;
+
+extern "C" {
+
+int mpmx_compute_mu_lam_bulk(int n, const float* E, const float* nu, float* mu, float* lam, float* bulk,
+                             void* stream) {
+    MPMX_REQUIRE(n >= 0, "mpmx_compute_mu_lam_bulk: n < 0");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(E && nu && mu && lam && bulk, "mpmx_compute_mu_lam_bulk: null pointer");
+    hipLaunchKernelGGL(mu_lam_bulk_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, E, nu,
+                       mu, lam, bulk);
+    return check_launch("compute_mu_lam_bulk", as_stream(stream));
+}
+
+int mpmx_compute_stress_from_F_trial(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x,
+                                     float* v, float* x_init, float* v_init, float* F, const float* F_trial,
+                                     float* mu, float* lam, float* ys, const float* hardening,
+                                     const float* softening, const float* density, const float* strainCriteria,
+                                     float eff, const float* eta_shear, const float* eta_bulk, const float* C,
+                                     float* stress, float* acc_strain, float* damage, int constitutive_model,
+                                     float* alpha, void* stream) {
+    (void)constitutive_model;  // passed but never read by the reference kernel (mpmRoutines.py:253)
+    MPMX_REQUIRE(n >= 0, "mpmx_compute_stress_from_F_trial: n < 0");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(activeLabel && materialLabel && x && v && x_init && v_init && F && F_trial && mu && lam && ys &&
+                     hardening && softening && density && strainCriteria && eta_shear && eta_bulk && C && stress &&
+                     acc_strain && damage && alpha,
+                 "mpmx_compute_stress_from_F_trial: null pointer");
+    hipLaunchKernelGGL(stress_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, activeLabel,
+                       materialLabel, x, v, x_init, v_init, F, F_trial, mu, lam, ys, hardening, softening, density,
+                       strainCriteria, eff, eta_shear, eta_bulk, C, stress, acc_strain, damage, alpha);
+    return check_launch("compute_stress_from_F_trial", as_stream(stream));
+}
+
+int mpmx_grid_normalization_and_gravity(int gx, int gy, int gz, const float* grid_m, const float* grid_v_in,
+                                        float* grid_v_out, float gxv, float gyv, float gzv, float dt, void* stream) {
+    MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_m && grid_v_in && grid_v_out,
+                 "mpmx_grid_normalization_and_gravity: bad arguments");
+    long G = (long)gx * gy * gz;
+    hipLaunchKernelGGL(grid_norm_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), G, grid_m,
+                       grid_v_in, grid_v_out, gxv, gyv, gzv, dt);
+    return check_launch("grid_normalization_and_gravity", as_stream(stream));
+}
+
+int mpmx_add_damping_via_grid(int gx, int gy, int gz, float* grid_v_out, float scale, void* stream) {
+    MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_v_out, "mpmx_add_damping_via_grid: bad arguments");
+    long G = (long)gx * gy * gz;
+    hipLaunchKernelGGL(grid_damp_kernel, dim3(grid_for(3 * G, kBlock)), dim3(kBlock), 0, as_stream(stream), G,
+                       grid_v_out, scale);
+    return check_launch("add_damping_via_grid", as_stream(stream));
+}
+
+int mpmx_collide_bounds(float* grid_v_out, int gx, int gy, int gz, float friction, float dt, float gmag,
+                        int boundary_padding, void* stream) {
+    MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_v_out, "mpmx_collide_bounds: bad arguments");
+    long G = (long)gx * gy * gz;
+    hipLaunchKernelGGL(collide_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), grid_v_out, gx,
+                       gy, gz, friction, dt, gmag, boundary_padding);
+    return check_launch("collideBounds", as_stream(stream));
+}
+
+int mpmx_grid_update(int gx, int gy, int gz, const float* grid_m, const float* grid_v_in, float* grid_v_out,
+                     float gxv, float gyv, float gzv, float dt, float damping_scale, int bc, float friction,
+                     float gmag, int boundary_padding, void* stream) {
+    MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_m && grid_v_in && grid_v_out, "mpmx_grid_update: bad arguments");
+    MPMX_REQUIRE(bc == MPMX_BC_NONE || bc == MPMX_BC_FRICTION, "mpmx_grid_update: unsupported boundary condition %d",
+                 bc);
+    long G = (long)gx * gy * gz;
+    hipLaunchKernelGGL(grid_update_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), gx, gy, gz,
+                       grid_m, grid_v_in, grid_v_out, gxv, gyv, gzv, dt, damping_scale, bc, friction, gmag,
+                       boundary_padding);
+    return check_launch("grid_update", as_stream(stream));
+}
+
+int mpmx_g2p(float dt, int n, const int32_t* activeLabel, const int32_t* materialLabel, float* x, float* v,
+             float* C, const float* F, float* F_trial, float* cov, float inv_dx, const float* grid_v_out, int gx,
+             int gy, int gz, int update_cov, float minx, float miny, float minz, void* stream) {
+    MPMX_REQUIRE(n >= 0 && gx > 0 && gy > 0 && gz > 0, "mpmx_g2p: bad sizes");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(activeLabel && materialLabel && x && v && C && F && F_trial && grid_v_out && (!update_cov || cov),
+                 "mpmx_g2p: null pointer");
+    hipLaunchKernelGGL(g2p_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), dt, n, activeLabel,
+                       materialLabel, x, v, C, F, F_trial, cov, inv_dx, grid_v_out, gx, gy, gz, update_cov, minx,
+                       miny, minz);
+    return check_launch("g2p", as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+// Device-side building blocks shared by the gfx950 kernels of libmpmx.
+//
+// Layout in HBM (structure of arrays): a per-particle vec3 array of N items is
+// three float planes [3][N]; a mat33 array is nine planes [9][N] in row-major
+// component order (r*3+c); a grid vec3 array of G = gx*gy*gz nodes is [3][G]
+// with node index (ix*gy + iy)*gz + iz.  Every particle kernel therefore reads
+// and writes each component with unit stride across a wavefront.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mpmx.h"
+
+namespace mpmx {
+
+constexpr int kBlock = 256;
+
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 ld3(const float* __restrict__ a, long n, long i) {
+    return f3{a[i], a[n + i], a[2 * n + i]};
+}
+__device__ __forceinline__ void st3(float* __restrict__ a, long n, long i, f3 v) {
+    a[i] = v.x;
+    a[n + i] = v.y;
+    a[2 * n + i] = v.z;
+}
+__device__ __forceinline__ void ld9(const float* __restrict__ a, long n, long i, float m[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) m[k] = a[k * n + i];
+}
+__device__ __forceinline__ void st9(float* __restrict__ a, long n, long i, const float m[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k * n + i] = m[k];
+}
+__device__ __forceinline__ float len3(f3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+// Warp's normalize(): the zero vector stays zero, otherwise divide by length.
+__device__ __forceinline__ f3 normalize3(f3 v) {
+    float l = len3(v);
+    if (l > 0.0f) return f3{v.x / l, v.y / l, v.z / l};
+    return f3{0.0f, 0.0f, 0.0f};
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 SVD  F = U diag(s) V^T,  U, V in SO(3), |s| descending, sign(det F) on
+// s[2] (the McAdams et al. 2011 convention of the reference's wp.svd3,
+// mpmRoutines.py:288,339).
+//
+// Method (written for near-identity deformation gradients):
+//  1. A = F^T F - I formed as H + H^T + H^T H with H = F - I.  Entries of H are
+//     exact in FP32 (Sterbenz) for |F_ii - 1| < 1/2, so A carries the strain
+//     with relative rather than absolute FP32 error.
+//  2. Cyclic Jacobi on A with exact rotations (Golub & Van Loan, sym.schur2),
+//     sweeping until the off-diagonal mass is below FP32 resolution.
+//  3. Sort descending with a sign flip per column swap (V stays in SO(3)).
+//  4. U from a Givens QR of B = F V; R00, R11 >= 0 and R22 carries det F.
+// lam[k] = s_k^2 - 1 is returned so that ln s_k = 0.5*log1p(lam[k]) keeps the
+// precision of small principal strains.
+// ---------------------------------------------------------------------------
+struct Svd3 {
+    float U[9], V[9], s[3], lam[3];
+};
+
+__device__ __forceinline__ void jacobi_rot(float A[9], float V[9], int p, int q) {
+    float apq = A[p * 3 + q];
+    if (apq == 0.0f) return;
+    float theta = (A[q * 3 + q] - A[p * 3 + p]) / (2.0f * apq);
+    float at = fabsf(theta);
+    float t = (at > 1.0e10f) ? 0.5f / theta : copysignf(1.0f / (at + sqrtf(1.0f + theta * theta)), theta);
+    float c = 1.0f / sqrtf(1.0f + t * t);
+    float s = t * c;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float akp = A[k * 3 + p], akq = A[k * 3 + q];
+        A[k * 3 + p] = c * akp - s * akq;
+        A[k * 3 + q] = s * akp + c * akq;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float apk = A[p * 3 + k], aqk = A[q * 3 + k];
+        A[p * 3 + k] = c * apk - s * aqk;
+        A[q * 3 + k] = s * apk + c * aqk;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float vkp = V[k * 3 + p], vkq = V[k * 3 + q];
+        V[k * 3 + p] = c * vkp - s * vkq;
+        V[k * 3 + q] = s * vkp + c * vkq;
+    }
+}
+
+__device__ __forceinline__ void givens_rows(float B[9], float Q[9], int p, int q, int col) {
+    float a = B[p * 3 + col], b = B[q * 3 + col];
+    float r = sqrtf(a * a + b * b);
+    float c = 1.0f, s = 0.0f;
+    if (r > 0.0f) {
+        c = a / r;
+        s = b / r;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float bp = B[p * 3 + k], bq = B[q * 3 + k];
+        B[p * 3 + k] = c * bp + s * bq;
+        B[q * 3 + k] = -s * bp + c * bq;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float qp = Q[k * 3 + p], qq = Q[k * 3 + q];
+        Q[k * 3 + p] = c * qp + s * qq;
+        Q[k * 3 + q] = -s * qp + c * qq;
+    }
+}
+
+__device__ __forceinline__ void svd3(const float F[9], Svd3& r) {
+    float H0 = F[0] - 1.0f, H4 = F[4] - 1.0f, H8 = F[8] - 1.0f;
+    float H[9] = {H0, F[1], F[2], F[3], H4, F[5], F[6], F[7], H8};
+    float A[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i; j < 3; j++) {
+            float htH = H[0 * 3 + i] * H[0 * 3 + j] + H[1 * 3 + i] * H[1 * 3 + j] + H[2 * 3 + i] * H[2 * 3 + j];
+            float v = H[i * 3 + j] + H[j * 3 + i] + htH;
+            A[i * 3 + j] = v;
+            A[j * 3 + i] = v;
+        }
+    float* V = r.V;
+    V[0] = 1.0f; V[1] = 0.0f; V[2] = 0.0f;
+    V[3] = 0.0f; V[4] = 1.0f; V[5] = 0.0f;
+    V[6] = 0.0f; V[7] = 0.0f; V[8] = 1.0f;
+    for (int sweep = 0; sweep < 8; sweep++) {
+        float off = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
+        float dia = A[0] * A[0] + A[4] * A[4] + A[8] * A[8];
+        if (!(off > 1.0e-15f * dia)) break;
+        jacobi_rot(A, V, 0, 1);
+        jacobi_rot(A, V, 0, 2);
+        jacobi_rot(A, V, 1, 2);
+    }
+    float l0 = A[0], l1 = A[4], l2 = A[8];
+    // sort descending; each swap negates one column so that det V stays +1
+    if (l0 < l1) {
+        float t = l0; l0 = l1; l1 = t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { float tv = V[k * 3 + 0]; V[k * 3 + 0] = V[k * 3 + 1]; V[k * 3 + 1] = -tv; }
+    }
+    if (l1 < l2) {
+        float t = l1; l1 = l2; l2 = t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { float tv = V[k * 3 + 1]; V[k * 3 + 1] = V[k * 3 + 2]; V[k * 3 + 2] = -tv; }
+    }
+    if (l0 < l1) {
+        float t = l0; l0 = l1; l1 = t;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { float tv = V[k * 3 + 0]; V[k * 3 + 0] = V[k * 3 + 1]; V[k * 3 + 1] = -tv; }
+    }
+    float B[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) B[i * 3 + j] = F[i * 3 + 0] * V[0 * 3 + j] + F[i * 3 + 1] * V[1 * 3 + j] + F[i * 3 + 2] * V[2 * 3 + j];
+    float* U = r.U;
+    U[0] = 1.0f; U[1] = 0.0f; U[2] = 0.0f;
+    U[3] = 0.0f; U[4] = 1.0f; U[5] = 0.0f;
+    U[6] = 0.0f; U[7] = 0.0f; U[8] = 1.0f;
+    givens_rows(B, U, 0, 1, 0);
+    givens_rows(B, U, 0, 2, 0);
+    givens_rows(B, U, 1, 2, 1);
+    r.lam[0] = l0; r.lam[1] = l1; r.lam[2] = l2;
+    r.s[0] = sqrtf(fmaxf(1.0f + l0, 0.0f));
+    r.s[1] = sqrtf(fmaxf(1.0f + l1, 0.0f));
+    float m2 = sqrtf(fmaxf(1.0f + l2, 0.0f));
+    r.s[2] = (B[8] < 0.0f) ? -m2 : m2;
+}
+
+// ln(max(s, floor)) with the precision of log1p for s >= floor.
+__device__ __forceinline__ float log_clamped(float s, float lam, float floorv) {
+    return (s >= floorv) ? 0.5f * log1pf(lam) : logf(floorv);
+}
+
+// ---------------------------------------------------------------------------
+// Quadratic B-spline stencil (mpmRoutines.py:514-530, :733-749): base index by
+// truncation toward zero (wp.int), fx in [0.5, 1.5) for interior particles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bspline(float f, float w[3], float dw[3]) {
+    float wa = 1.5f - f, wb = f - 1.0f, wc = f - 0.5f;
+    w[0] = wa * wa * 0.5f;
+    w[1] = (0.0f - wb * wb) + 0.75f;
+    w[2] = wc * wc * 0.5f;
+    dw[0] = f - 1.5f;
+    dw[1] = -2.0f * (f - 1.0f);
+    dw[2] = f - 0.5f;
+}
+
+// Grid coordinate, base index and fractional offset of a particle on one axis.
+__device__ __forceinline__ int base_axis(float x, float minb, float inv_dx, float& fx) {
+    float gp = (x - minb) * inv_dx;
+    int b = (int)(gp - 0.5f);
+    fx = gp - (float)b;
+    return b;
+}
+
+// Coulomb friction on one tangential component (mpmRoutines.py:915-940).
+__device__ __forceinline__ float coulomb(float v, float mu, float dt, float g) {
+    if (fabsf(v) < 1.0e-12f) return 0.0f;
+    float dv = mu * g * dt;
+    float s = (v < 0.0f) ? -1.0f : 1.0f;
+    return s * fmaxf(fabsf(v) - dv, 0.0f);
+}
+
+
+// collideBounds on one node (mpmRoutines.py:942-993): v is read once, every
+// matching face overwrites the whole vector, the last match wins; lower faces
+// act for index <= pad, upper faces for index >= dim - pad.
+__device__ __forceinline__ f3 collide_node(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, float mu, float dt,
+                                           float g, int pad) {
+    f3 o = v;
+    if (ix <= pad && v.x < 0.0f) o = f3{0.0f, coulomb(v.y, mu, dt, g), coulomb(v.z, mu, dt, g)};
+    if (ix >= gx - pad && v.x > 0.0f) o = f3{0.0f, coulomb(v.y, mu, dt, g), coulomb(v.z, mu, dt, g)};
+    if (iy <= pad && v.y < 0.0f) o = f3{coulomb(v.x, mu, dt, g), 0.0f, coulomb(v.z, mu, dt, g)};
+    if (iy >= gy - pad && v.y > 0.0f) o = f3{coulomb(v.x, mu, dt, g), 0.0f, coulomb(v.z, mu, dt, g)};
+    if (iz <= pad && v.z < 0.0f) o = f3{coulomb(v.x, mu, dt, g), coulomb(v.y, mu, dt, g), 0.0f};
+    if (iz >= gz - pad && v.z > 0.0f) o = f3{coulomb(v.x, mu, dt, g), coulomb(v.y, mu, dt, g), 0.0f};
+    return o;
+}
+
+
+// Steps 4-6 of mpmSimulationStep on one node (simulationRoutines.py:94-140):
+// normalisation + gravity (v = 0 where m <= 1e-15), damping when the scale is
+// below 1, then the boundary condition.
+__device__ __forceinline__ f3 grid_update_node(float m, f3 vin, int ix, int iy, int iz, int gx, int gy, int gz, float gxv, float gyv,
+                               float gzv, float dt, float damping, int bc, float fric, float gmag, int pad) {
+    f3 v = f3{0.0f, 0.0f, 0.0f};
+    if (m > 1.0e-15f) {
+        float inv = 1.0f / m;
+        v = f3{vin.x * inv + dt * gxv, vin.y * inv + dt * gyv, vin.z * inv + dt * gzv};
+    }
+    if (damping < 1.0f) v = f3{v.x * damping, v.y * damping, v.z * damping};
+    if (bc == MPMX_BC_FRICTION) v = collide_node(v, ix, iy, iz, gx, gy, gz, fric, dt, gmag, pad);
+    return v;
+}
+
+}  // namespace mpmx

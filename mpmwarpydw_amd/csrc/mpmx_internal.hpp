@@ -1,0 +1,75 @@
+// Host-side helpers shared by the libmpmx translation units: error reporting,
+// launch checking and the workspace carving used by the entry points that need
+// scratch memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/mpmx.h"
+
+namespace mpmx {
+
+void set_error(const char* fmt, ...);
+int check_launch(const char* what, hipStream_t s);
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Bump allocator over a caller-provided workspace; every slice is 256-byte aligned.
+struct Carver {
+    char* base;
+    size_t cap;
+    size_t off = 0;
+    bool overflow = false;
+    Carver(void* b, size_t c) : base(static_cast<char*>(b)), cap(c) {}
+    template <typename T>
+    T* take(size_t count) {
+        size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        if (off + bytes > cap) {
+            overflow = true;
+            return nullptr;
+        }
+        T* p = reinterpret_cast<T*>(base + off);
+        off += bytes;
+        return p;
+    }
+    static size_t bytes_for(size_t count, size_t elem) { return (count * elem + 255) & ~size_t(255); }
+};
+
+// Exclusive scan of n int32 counts into out[0..n] (out[n] = total), on stream s.
+// partials must hold scan_partials_count(n) ints.
+size_t scan_partials_count(long n);
+int exclusive_scan_i32(const int32_t* counts, int32_t* out, long n, int32_t* partials, hipStream_t s);
+
+// Counting sort of particle ids by int32 key (key < 0 = not binned):
+//   count[key]++ (rank kept per id), cell_start = exclusive scan(count),
+//   sorted[cell_start[key] + rank] = id, then each key's id run is sorted ascending
+// so the result equals a stable sort of ids by key.
+struct BinWork {
+    int32_t* count;      // [nkeys]
+    int32_t* cell_start; // [nkeys + 1]
+    int32_t* rank;       // [n]
+    int32_t* partials;   // scan partials
+};
+size_t bin_workspace_bytes(long n, long nkeys);
+bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w);
+int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s);
+
+}  // namespace mpmx
+
+#define MPMX_REQUIRE(cond, ...)                 \
+    do {                                        \
+        if (!(cond)) {                          \
+            ::mpmx::set_error(__VA_ARGS__);     \
+            return MPMX_EINVAL;                 \
+        }                                       \
+    } while (0)
+
+#define MPMX_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::mpmx::set_error("%s failed: %s", #call, hipGetErrorString(e_));       \
+            return (int)e_;                                                         \
+        }                                                                           \
+    } while (0)

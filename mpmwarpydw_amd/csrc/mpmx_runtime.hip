@@ -1,0 +1,267 @@
+// libmpmx runtime pieces: error text, debug synchronisation, fills and copies,
+// a three-pass exclusive scan and the counting sort by key that both the P2G
+// binning and the XPBD hash grid are built on.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "mpmx_device.hpp"
+#include "mpmx_internal.hpp"
+
+namespace mpmx {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+static int debug_sync_enabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = getenv("MPMX_DEBUG");
+        v = (e && e[0] == '1') ? 1 : 0;
+    }
+    return v;
+}
+
+int check_launch(const char* what, hipStream_t s) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of %s failed: %s", what, hipGetErrorString(e));
+        return (int)e;
+    }
+    if (debug_sync_enabled()) {
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            set_error("%s failed on the device: %s", what, hipGetErrorString(e));
+            return (int)e;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// fills / copies
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ dst, long n, T value) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = value;
+}
+
+__global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ src, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+static unsigned fill_grid(long n) {
+    long g = (n + kBlock - 1) / kBlock;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan: tiles of 2048 ints per 256-thread block
+// ---------------------------------------------------------------------------
+constexpr int kScanPer = 8;
+constexpr int kScanTile = kBlock * kScanPer;
+
+__device__ int block_exclusive_scan(int v, int* sh, int* total) {
+    // Hillis-Steele inclusive scan over the 256 thread values in LDS
+    int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int add = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    int incl = sh[t];
+    *total = sh[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void scan_reduce_kernel(const int32_t* __restrict__ in, long n, int32_t* __restrict__ partials) {
+    __shared__ int sh[kBlock];
+    long base = (long)blockIdx.x * kScanTile;
+    int s = 0;
+    for (int k = 0; k < kScanPer; k++) {
+        long i = base + (long)k * kBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// one block: exclusive scan of partials[0..nb) in place, total in partials[nb]
+__global__ void scan_partials_kernel(int32_t* __restrict__ partials, long nb) {
+    __shared__ int sh[kBlock];
+    int carry = 0;
+    for (long base = 0; base < nb; base += kBlock) {
+        long i = base + threadIdx.x;
+        int v = (i < nb) ? partials[i] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, sh, &tot);
+        if (i < nb) partials[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) partials[nb] = carry;
+}
+
+__global__ void scan_apply_kernel(const int32_t* __restrict__ in, long n, const int32_t* __restrict__ partials,
+                                  int32_t* __restrict__ out, long nb) {
+    __shared__ int sh[kBlock];
+    __shared__ int vals[kScanTile];
+    long base = (long)blockIdx.x * kScanTile;
+    for (int k = 0; k < kScanPer; k++) {
+        long i = base + (long)k * kBlock + threadIdx.x;
+        vals[k * kBlock + threadIdx.x] = (i < n) ? in[i] : 0;
+    }
+    __syncthreads();
+    int local = 0;
+    for (int k = 0; k < kScanPer; k++) local += vals[threadIdx.x * kScanPer + k];
+    int tot;
+    int ex = block_exclusive_scan(local, sh, &tot) + partials[blockIdx.x];
+    for (int k = 0; k < kScanPer; k++) {
+        long i = base + (long)threadIdx.x * kScanPer + k;
+        if (i < n) out[i] = ex;
+        ex += vals[threadIdx.x * kScanPer + k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[nb];
+}
+
+size_t scan_partials_count(long n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+int exclusive_scan_i32(const int32_t* counts, int32_t* out, long n, int32_t* partials, hipStream_t s) {
+    long nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) {
+        hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(kBlock), 0, s, out, 1L, 0);
+        return check_launch("scan(empty)", s);
+    }
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, counts, n, partials);
+    int rc = check_launch("scan_reduce", s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb);
+    rc = check_launch("scan_partials", s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, counts, n, partials, out, nb);
+    return check_launch("scan_apply", s);
+}
+
+// ---------------------------------------------------------------------------
+// counting sort of ids by key
+// ---------------------------------------------------------------------------
+__global__ void bin_count_kernel(const int32_t* __restrict__ keys, long n, int32_t* __restrict__ count,
+                                 int32_t* __restrict__ rank) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = keys[i];
+    if (k >= 0) rank[i] = atomicAdd(&count[k], 1);
+}
+
+__global__ void bin_scatter_kernel(const int32_t* __restrict__ keys, long n, const int32_t* __restrict__ cell_start,
+                                   const int32_t* __restrict__ rank, int32_t* __restrict__ sorted) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = keys[i];
+    if (k >= 0) sorted[cell_start[k] + rank[i]] = (int32_t)i;
+}
+
+// ascending ids inside each key's run: the order a stable sort by key gives
+__global__ void bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start, int32_t* __restrict__ sorted) {
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nkeys) return;
+    int b = cell_start[c], e = cell_start[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        int v = sorted[i];
+        int j = i - 1;
+        while (j >= b && sorted[j] > v) {
+            sorted[j + 1] = sorted[j];
+            j--;
+        }
+        sorted[j + 1] = v;
+    }
+}
+
+size_t bin_workspace_bytes(long n, long nkeys) {
+    return Carver::bytes_for(nkeys, 4) + Carver::bytes_for(nkeys + 1, 4) + Carver::bytes_for(n > 0 ? n : 1, 4) +
+           Carver::bytes_for(scan_partials_count(nkeys), 4);
+}
+
+bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w) {
+    w.count = cv.take<int32_t>(nkeys);
+    w.cell_start = cv.take<int32_t>(nkeys + 1);
+    w.rank = cv.take<int32_t>(n > 0 ? n : 1);
+    w.partials = cv.take<int32_t>(scan_partials_count(nkeys));
+    return !cv.overflow;
+}
+
+int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s) {
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(fill_grid(nkeys)), dim3(kBlock), 0, s, w.count, nkeys, 0);
+    int rc = check_launch("bin_zero", s);
+    if (rc) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(bin_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.count, w.rank);
+        if ((rc = check_launch("bin_count", s))) return rc;
+    }
+    if ((rc = exclusive_scan_i32(w.count, w.cell_start, nkeys, w.partials, s))) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
+                           w.rank, sorted);
+        if ((rc = check_launch("bin_scatter", s))) return rc;
+    }
+    hipLaunchKernelGGL(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
+                       sorted);
+    return check_launch("bin_sort_runs", s);
+}
+
+}  // namespace mpmx
+
+using namespace mpmx;
+
+extern "C" {
+
+const char* mpmx_last_error(void) { return g_err; }
+int mpmx_version(void) { return 1; }
+
+int mpmx_stream_sync(void* stream) {
+    MPMX_HIP(hipStreamSynchronize(as_stream(stream)));
+    return 0;
+}
+
+int mpmx_fill_i32(int32_t* dst, long n, int32_t value, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || dst), "mpmx_fill_i32: bad arguments");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, n, value);
+    return check_launch("fill_i32", as_stream(stream));
+}
+
+int mpmx_fill_f32(float* dst, long n, float value, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || dst), "mpmx_fill_f32: bad arguments");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, n, value);
+    return check_launch("fill_f32", as_stream(stream));
+}
+
+int mpmx_copy_f32(float* dst, const float* src, long n, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (dst && src)), "mpmx_copy_f32: bad arguments");
+    if (n == 0 || dst == src) return 0;
+    hipLaunchKernelGGL(copy_kernel, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, src, n);
+    return check_launch("copy_f32", as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,384 @@
+// XPBD contact step kernels (utils/xpbdRoutines.py) and the spatial hash grid
+// that replaces wp.HashGrid (simStates.py:174, simulationRoutines.py:311).
+//
+// Hash grid: cell coordinates int(p / width) (truncation toward zero), offset by
+// 2^20 and wrapped modulo the hash dims per axis, key = cz*dx*dy + cy*dx + cx.
+// Points are counting-sorted by key with ids ascending inside a cell (the order
+// of a stable radix sort of (key, id) pairs), and thread tid of the contact
+// kernel works on point_ids[tid].  A query box visits cells x fastest, then y,
+// then z, never visiting a wrapped cell twice.
+#include "mpmx_device.hpp"
+#include "mpmx_internal.hpp"
+
+namespace mpmx {
+
+__device__ __forceinline__ int hash_index(int dimx, int dimy, int dimz, int x, int y, int z) {
+    const int origin = 1 << 20;
+    x += origin;
+    y += origin;
+    z += origin;
+    // wp.HashGrid wraps x + 2^20 modulo the dims; the extra "+ dim) % dim" only
+    // matters for coordinates below -2^20 cells, where it keeps the index in range
+    int cx = ((x % dimx) + dimx) % dimx, cy = ((y % dimy) + dimy) % dimy, cz = ((z % dimz) + dimz) % dimz;
+    return cz * (dimx * dimy) + cy * dimx + cx;
+}
+
+__global__ void hash_key_kernel(int n, const float* __restrict__ x, float inv, int dimx, int dimy, int dimz,
+                                int32_t* __restrict__ keys) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    keys[p] = hash_index(dimx, dimy, dimz, (int)(x[p] * inv), (int)(x[n + p] * inv), (int)(x[2L * n + p] * inv));
+}
+
+// reset_mpm_contact_labels (xpbdRoutines.py:3-21)
+__global__ void reset_labels_kernel(int n, const int32_t* __restrict__ active, int32_t* __restrict__ mat) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (active[p] == 1 && mat[p] == 0) mat[p] = 1;
+}
+
+// integrateParticlesXPBD (xpbdRoutines.py:23-48)
+__global__ void integrate_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                                 const float* x, const float* v, float gx, float gy, float gz, float dt, float* x_new,
+                                 float* v_new, float v_max) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (active[p] != 1 || mat[p] != 2) return;
+    f3 x0 = ld3(x, n, p), v0 = ld3(v, n, p);
+    f3 v1 = f3{v0.x + gx * dt, v0.y + gy * dt, v0.z + gz * dt};
+    float m = len3(v1);
+    if (m > v_max) {
+        float s = v_max / m;
+        v1 = f3{v1.x * s, v1.y * s, v1.z * s};
+    }
+    f3 x1 = f3{x0.x + v1.x * dt, x0.y + v1.y * dt, x0.z + v1.z * dt};
+    st3(x_new, n, p, x1);
+    st3(v_new, n, p, v1);
+}
+
+// my_solve_particle_bound_contacts (xpbdRoutines.py:49-136)
+__global__ void bound_contacts_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                                      const float* __restrict__ x, const float* __restrict__ v,
+                                      const float* __restrict__ mass, const float* __restrict__ radius, float mu,
+                                      float thr, float smu, f3 bmin, f3 bmax, float dt, float relax,
+                                      float* __restrict__ delta) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (active[t] != 1 || mat[t] != 2 || mass[t] == 0.0f) return;
+    float wi = 1.0f / mass[t];
+    f3 xp = ld3(x, n, t), vp = ld3(v, n, t);
+    float r = radius[t];
+    f3 d = ld3(delta, n, t);
+    const float xa[3] = {xp.x, xp.y, xp.z}, va[3] = {vp.x, vp.y, vp.z};
+    const float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
+    float da[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            float sg = side == 0 ? 1.0f : -1.0f;
+            float b = side == 0 ? lo[ax] : hi[ax];
+            float c = fminf(sg * (xa[ax] - b) - r, 0.0f);
+            if (c < 0.0f) {
+                float lambda_n = c;
+                float vn = sg * va[ax];
+                float fmu = (fabsf(vn) < thr * r) ? smu : mu;
+                float vt[3] = {va[0], va[1], va[2]};
+                vt[ax] = va[ax] - sg * vn;
+                float vtl = sqrtf(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+                float a_ = fmu * lambda_n, b_ = -vtl * dt;
+                float lf = a_ > b_ ? a_ : b_;  // wp.max
+                float df[3] = {0.0f, 0.0f, 0.0f};
+                if (vtl > 1.0e-8f) {
+                    f3 nt = normalize3(f3{vt[0], vt[1], vt[2]});
+                    df[0] = nt.x * lf; df[1] = nt.y * lf; df[2] = nt.z * lf;
+                }
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    float dn = (a == ax) ? sg * lambda_n : 0.0f;
+                    da[a] += (df[a] - dn) / wi * relax * wi;
+                }
+            }
+        }
+    }
+    st3(delta, n, t, f3{da[0], da[1], da[2]});
+}
+
+// my_solve_particle_particle_contacts (xpbdRoutines.py:138-219)
+__global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
+    int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
+    const int32_t* __restrict__ cell_start, float inv, int dimx, int dimy, int dimz, const float* __restrict__ x,
+    const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
+    float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    if (active[tid] != 1 || mat[tid] < 0) return;
+    int i = point_ids[tid];
+    if (i == -1) return;
+    f3 xi = ld3(x, n, i), vi = ld3(v, n, i);
+    float ri = radius[i];
+    if (mass[tid] == 0.0f) return;  // the reference tests tid here (xpbdRoutines.py:175)
+    float w1 = 1.0f / mass[i];
+    float qr = ri + max_radius + k_coh * ri;
+    int xs = (int)((xi.x - qr) * inv), ys = (int)((xi.y - qr) * inv), zs = (int)((xi.z - qr) * inv);
+    int xe = min((int)((xi.x + qr) * inv), xs + dimx - 1);
+    int ye = min((int)((xi.y + qr) * inv), ys + dimy - 1);
+    int ze = min((int)((xi.z + qr) * inv), zs + dimz - 1);
+    float k_mu = k_mu_in;
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+    const float alpha = 0.0f;  // XPBD compliance of the reference
+    for (int cz = zs; cz <= ze; cz++)
+        for (int cy = ys; cy <= ye; cy++)
+            for (int cx = xs; cx <= xe; cx++) {
+                int cell = hash_index(dimx, dimy, dimz, cx, cy, cz);
+                int b = cell_start[cell], e = cell_start[cell + 1];
+                for (int s = b; s < e; s++) {
+                    int j = point_ids[s];
+                    if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
+                    if (mat[i] <= 1 && mat[j] <= 1) continue;
+                    f3 xj = ld3(x, n, j);
+                    float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
+                    float d = sqrtf(nx * nx + ny * ny + nz * nz);
+                    float err = d - ri - radius[j];
+                    float w2 = 1.0f / mass[j];
+                    float denom = w1 + w2;
+                    if (err <= k_coh && denom > 0.0f) {
+                        if (lock == 1) {
+                            if (mat[i] <= 1 && mat[j] == 2) mat[i] = 0;
+                            if (mat[i] == 2 && mat[j] <= 1) mat[j] = 0;
+                        }
+                        float ux = nx / d, uy = ny / d, uz = nz / d;
+                        f3 vj = ld3(v, n, j);
+                        float rx = vi.x - vj.x, ry = vi.y - vj.y, rz = vi.z - vj.z;
+                        float lambda_n = err / (1.0f + (alpha * dt * dt));
+                        float vn = ux * rx + uy * ry + uz * rz;
+                        if (fabsf(vn) < thr * ri) k_mu = smu;
+                        float tx = rx - ux * vn, ty = ry - uy * vn, tz = rz - uz * vn;
+                        float tl = sqrtf(tx * tx + ty * ty + tz * tz);
+                        float a_ = k_mu * lambda_n, b_ = -tl * dt;
+                        float lf = a_ > b_ ? a_ : b_;  // wp.max
+                        f3 nt = normalize3(f3{tx, ty, tz});
+                        d0 += (nt.x * lf - ux * lambda_n) / denom * w1;
+                        d1 += (nt.y * lf - uy * lambda_n) / denom * w1;
+                        d2 += (nt.z * lf - uz * lambda_n) / denom * w1;
+                    }
+                }
+            }
+    // point_ids is a permutation, so particle i has exactly one writer here
+    deltas[i] += d0 * relax;
+    deltas[n + i] += d1 * relax;
+    deltas[2L * n + i] += d2 * relax;
+}
+
+// my_apply_particle_deltas (xpbdRoutines.py:221-257)
+__global__ void apply_deltas_kernel(int n, const int32_t* __restrict__ active, const float* __restrict__ x_orig,
+                                    const float* x_pred, const float* __restrict__ delta, float dt, float v_max,
+                                    float* x_out, float* v_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (active[t] != 1) return;
+    f3 x0 = ld3(x_orig, n, t), xp = ld3(x_pred, n, t), d = ld3(delta, n, t);
+    if (isnan(d.x) || isnan(d.y) || isnan(d.z)) d = f3{0.0f, 0.0f, 0.0f};
+    f3 xn = f3{xp.x + d.x, xp.y + d.y, xp.z + d.z};
+    f3 vn = f3{(xn.x - x0.x) / dt, (xn.y - x0.y) / dt, (xn.z - x0.z) / dt};
+    float m = len3(vn);
+    if (m > v_max) {
+        float s = v_max / m;
+        vn = f3{vn.x * s, vn.y * s, vn.z * s};
+    }
+    st3(x_out, n, t, xn);
+    st3(v_out, n, t, vn);
+}
+
+// sleepParticles (xpbdRoutines.py:313-334)
+__global__ void sleep_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat, float thr,
+                             const float* __restrict__ radius, const float* __restrict__ x_prev,
+                             float* __restrict__ x_after, float* __restrict__ dist, float dt,
+                             int32_t* __restrict__ counters) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (active[t] != 1 || mat[t] != 2) return;
+    atomicAdd(&counters[0], 1);
+    f3 a = ld3(x_after, n, t), p = ld3(x_prev, n, t);
+    float d = len3(f3{a.x - p.x, a.y - p.y, a.z - p.z});
+    if (d / dt < thr * radius[t]) {
+        st3(x_after, n, t, p);
+        atomicAdd(&counters[1], 1);
+    }
+    dist[t] = dist[t] + d;
+}
+
+// apply_mpm_contact_impulses (xpbdRoutines.py:259-310)
+__global__ void impulse_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                               const float* x_orig, const float* v_orig, float* x_final, float dt, float v_max,
+                               float k, int xpbd_only, float* v_out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (!(active[t] == 1 && mat[t] <= 1)) return;
+    f3 xo = ld3(x_orig, n, t);
+    st3(x_final, n, t, xo);
+    if (xpbd_only == 0) {
+        f3 vo = ld3(v_orig, n, t);
+        // (x_final - x_orig) is identically zero after the reset above
+        f3 vc = f3{vo.x + (xo.x - xo.x) / dt * k, vo.y + (xo.y - xo.y) / dt * k, vo.z + (xo.z - xo.z) / dt * k};
+        float m = len3(vc);
+        if (m > v_max) {
+            float s = v_max / m;
+            vc = f3{vc.x * s, vc.y * s, vc.z * s};
+        }
+        st3(v_out, n, t, vc);
+    }
+}
+
+// swellParticlesType2 (xpbdRoutines.py:354-376)
+__global__ void swell_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                             const float* __restrict__ x_init, const float* __restrict__ x, float* __restrict__ radius,
+                             float factor_in, float factor_max_in, const float* __restrict__ swell_r,
+                             const float* __restrict__ base_r) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (active[t] != 1 || mat[t] != 2) return;
+    f3 a = ld3(x, n, t), b = ld3(x_init, n, t);
+    float br = base_r[t];
+    float dxl = len3(f3{a.x - b.x, a.y - b.y, a.z - b.z}) * br;
+    float factor = factor_in * br;
+    float factor_max = factor_max_in * br;
+    if (dxl > factor) {
+        float nr = br + (swell_r[t] - br) * (dxl - factor) / (factor_max - factor);
+        float r0 = radius[t];
+        nr = nr > r0 ? nr : r0;
+        float sr = swell_r[t];
+        nr = nr < sr ? nr : sr;
+        radius[t] = nr;
+    }
+}
+
+}  // namespace mpmx
+
+using namespace mpmx;
+
+#define LAUNCH1(kernel, n, ...)                                                                         \
+    hipLaunchKernelGGL(kernel, dim3(grid_for((n), kBlock)), dim3(kBlock), 0, as_stream(stream), __VA_ARGS__); \
+    return check_launch(#kernel, as_stream(stream));
+
+extern "C" {
+
+size_t mpmx_hashgrid_workspace_bytes(int n, int dimx, int dimy, int dimz) {
+    long nkeys = (long)dimx * dimy * dimz;
+    long nn = n > 0 ? n : 1;
+    return Carver::bytes_for(nn, 4) + bin_workspace_bytes(nn, nkeys);
+}
+
+int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
+                        int32_t* cell_start, void* workspace, size_t ws_bytes, void* stream) {
+    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f && cell_start,
+                 "mpmx_hashgrid_build: bad arguments");
+    MPMX_REQUIRE(n == 0 || (x && point_ids), "mpmx_hashgrid_build: null pointer");
+    long nkeys = (long)dimx * dimy * dimz;
+    MPMX_REQUIRE(nkeys < (1L << 31), "mpmx_hashgrid_build: too many cells");
+    MPMX_REQUIRE(workspace && ws_bytes >= mpmx_hashgrid_workspace_bytes(n, dimx, dimy, dimz),
+                 "mpmx_hashgrid_build: workspace too small");
+    hipStream_t s = as_stream(stream);
+    Carver cv(workspace, ws_bytes);
+    long nn = n > 0 ? n : 1;
+    int32_t* keys = cv.take<int32_t>(nn);
+    BinWork w;
+    carve_bin(cv, nn, nkeys, w);
+    w.cell_start = cell_start;  // the caller's table receives the cell ranges
+    if (n > 0) {
+        hipLaunchKernelGGL(hash_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, x, 1.0f / cell_width,
+                           dimx, dimy, dimz, keys);
+        int rc = check_launch("hash_key", s);
+        if (rc) return rc;
+    }
+    return counting_sort_ids(keys, n, nkeys, point_ids, w, s);
+}
+
+int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel)), "mpmx_reset_mpm_contact_labels: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(reset_labels_kernel, n, n, activeLabel, materialLabel)
+}
+
+int mpmx_integrate_particles_xpbd(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x,
+                                  const float* v, float gxv, float gyv, float gzv, float dt, float* x_new,
+                                  float* v_new, float v_max, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && x_new && v_new)),
+                 "mpmx_integrate_particles_xpbd: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(integrate_kernel, n, n, activeLabel, materialLabel, x, v, gxv, gyv, gzv, dt, x_new, v_new, v_max)
+}
+
+int mpmx_solve_particle_bound_contacts(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                                       const float* x, const float* v, const float* mass, const float* radius,
+                                       float mu, float static_threshold, float static_friction, float minx,
+                                       float miny, float minz, float maxx, float maxy, float maxz, float dt,
+                                       float relaxation, float* delta, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && mass && radius && delta)),
+                 "mpmx_solve_particle_bound_contacts: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(bound_contacts_kernel, n, n, activeLabel, materialLabel, x, v, mass, radius, mu, static_threshold,
+            static_friction, f3{minx, miny, minz}, f3{maxx, maxy, maxz}, dt, relaxation, delta)
+}
+
+int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel, int32_t* materialLabel,
+                                          const int32_t* point_ids, const int32_t* cell_start, float cell_width,
+                                          int dimx, int dimy, int dimz, const float* x, const float* v,
+                                          const float* mass, const float* radius, float k_mu, float static_threshold,
+                                          float static_friction, float k_cohesion, float max_radius, float dt,
+                                          float relaxation, int contact_lock, float* deltas, void* stream) {
+    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f,
+                 "mpmx_solve_particle_particle_contacts: bad sizes");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && x && v && mass && radius && deltas,
+                 "mpmx_solve_particle_particle_contacts: null pointer");
+    LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, 1.0f / cell_width,
+            dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold, static_friction, k_cohesion, max_radius, dt,
+            relaxation, contact_lock, deltas)
+}
+
+int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,
+                               const float* x_pred, const float* delta, float dt, float v_max, float* x_out,
+                               float* v_out, void* stream) {
+    (void)materialLabel;
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && x_orig && x_pred && delta && x_out && v_out)),
+                 "mpmx_apply_particle_deltas: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(apply_deltas_kernel, n, n, activeLabel, x_orig, x_pred, delta, dt, v_max, x_out, v_out)
+}
+
+int mpmx_sleep_particles(int n, const int32_t* activeLabel, const int32_t* materialLabel, float threshold,
+                         const float* radius, const float* x_prev, float* x_after, float* dist_total, float dt,
+                         int32_t* counters, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && radius && x_prev && x_after && dist_total &&
+                                       counters)),
+                 "mpmx_sleep_particles: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(sleep_kernel, n, n, activeLabel, materialLabel, threshold, radius, x_prev, x_after, dist_total, dt,
+            counters)
+}
+
+int mpmx_apply_mpm_contact_impulses(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                                    const float* x_orig, const float* v_orig, float* x_final, const float* v_final,
+                                    float dt, float v_max, float coupling, int xpbd_only, float* v_out,
+                                    void* stream) {
+    (void)v_final;
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x_orig && v_orig && x_final && v_out)),
+                 "mpmx_apply_mpm_contact_impulses: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(impulse_kernel, n, n, activeLabel, materialLabel, x_orig, v_orig, x_final, dt, v_max, coupling, xpbd_only,
+            v_out)
+}
+
+int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_init,
+                               const float* x, float* radius, float factor, float factor_max,
+                               const float* swell_radius, const float* base_radius, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x_init && x && radius && swell_radius &&
+                                       base_radius)),
+                 "mpmx_swell_particles_type2: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(swell_kernel, n, n, activeLabel, materialLabel, x_init, x, radius, factor, factor_max, swell_radius,
+            base_radius)
+}
+
+}  // extern "C"

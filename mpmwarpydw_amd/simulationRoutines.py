@@ -1,0 +1,251 @@
+"""Substep orchestration of the hot path: the call surface of the reference's
+utils/simulationRoutines.py (mpmSimulationStep :6, xpbdSimulationStep :295,
+reductions :530-675) driving libmpmx's HIP kernels.
+
+``mpmSimulationStep`` issues the reference's launches in the reference's
+order.  ``mpmSimulationStepFused`` computes the same substep with the grid
+update folded into the P2G combine pass (no grid zeroing, no separate
+normalisation/boundary launches); it is what the driver and the benchmark run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import mpmRoutines, xpbdRoutines
+from . import runtime as rt
+from ._lib import MpmxError, lib
+from .runtime import Kernel, Workspace, launch, stream_handle
+
+_BC = {"friction": 1}
+
+
+def _bc_code(mpm):
+    if mpm.boundaryCondition not in _BC:
+        raise MpmxError(f"boundaryCondition {mpm.boundaryCondition!r} is not implemented on the device path "
+                        "(implemented: 'friction', mpmRoutines.py:942)")
+    return _BC[mpm.boundaryCondition]
+
+
+def _check_vlc(mpm):
+    if mpm.volumetric_locking_correction:
+        raise MpmxError("volumetric_locking_correction=1 (simulationRoutines.py:255-293) is not implemented on the "
+                        "device path; set it to 0 in the JSON config")
+
+
+def _cov_flag(mpm):
+    return bool(mpm.update_cov) and not getattr(mpm, "cov_is_zero", False)
+
+
+def mpmSimulationStep(sim, mpm, xpbd, device=None):
+    """One MPM substep in the reference's launch order (simulationRoutines.py:6-293)."""
+    _check_vlc(mpm)
+    mpm.grid_m.zero_()
+    mpm.grid_v_in.zero_()
+    mpm.grid_v_out.zero_()
+    launch(kernel=mpmRoutines.compute_stress_from_F_trial, dim=sim.nPoints,
+           inputs=[sim.activeLabel, sim.materialLabel, sim.particle_x, sim.particle_v, xpbd.particle_x_initial_xpbd,
+                   xpbd.particle_v_initial_xpbd, mpm.particle_F, mpm.particle_F_trial, mpm.mu, mpm.lam, mpm.ys,
+                   mpm.hardening, mpm.softening, sim.particle_density, mpm.strainCriteria, mpm.eff, mpm.eta_shear,
+                   mpm.eta_bulk, mpm.particle_C, mpm.particle_stress, mpm.particle_accumulated_strain,
+                   mpm.particle_damage, mpm.constitutive_model, mpm.alpha])
+    launch(kernel=mpmRoutines.p2g_apic_with_stress, dim=sim.nPoints,
+           inputs=[sim.activeLabel, sim.materialLabel, mpm.particle_stress, sim.particle_x, sim.particle_v,
+                   mpm.particle_C, sim.particle_vol, sim.particle_mass, sim.dx, sim.invdx, sim.minBounds,
+                   mpm.rpic_damping, mpm.grid_m, mpm.grid_v_in, sim.dt])
+    launch(kernel=mpmRoutines.grid_normalization_and_gravity, dim=mpm.gridDims,
+           inputs=[mpm.grid_m, mpm.grid_v_in, mpm.grid_v_out, sim.gravity, sim.dt])
+    if mpm.grid_v_damping_scale < 1.0:
+        launch(kernel=mpmRoutines.add_damping_via_grid, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, mpm.grid_v_damping_scale])
+    _bc_code(mpm)
+    launch(kernel=mpmRoutines.collideBounds, dim=mpm.gridDims,
+           inputs=[mpm.grid_v_out, mpm.gridDims[0], mpm.gridDims[1], mpm.gridDims[2], mpm.boundFriction, sim.dt,
+                   abs(sim.gravity[2]), mpm.boundaryPadding])
+    launch(kernel=mpmRoutines.g2p, dim=sim.nPoints,
+           inputs=[sim.dt, sim.activeLabel, sim.materialLabel, sim.particle_x, sim.particle_v, mpm.particle_C,
+                   mpm.particle_F, mpm.particle_F_trial, mpm.particle_cov, sim.invdx, mpm.grid_v_out,
+                   _cov_flag(mpm), sim.minBounds])
+
+
+def stress_phase(sim, mpm, xpbd):
+    """compute_stress_from_F_trial over all particles (simulationRoutines.py:15-45)."""
+    launch(kernel=mpmRoutines.compute_stress_from_F_trial, dim=sim.nPoints,
+           inputs=[sim.activeLabel, sim.materialLabel, sim.particle_x, sim.particle_v, xpbd.particle_x_initial_xpbd,
+                   xpbd.particle_v_initial_xpbd, mpm.particle_F, mpm.particle_F_trial, mpm.mu, mpm.lam, mpm.ys,
+                   mpm.hardening, mpm.softening, sim.particle_density, mpm.strainCriteria, mpm.eff, mpm.eta_shear,
+                   mpm.eta_bulk, mpm.particle_C, mpm.particle_stress, mpm.particle_accumulated_strain,
+                   mpm.particle_damage, mpm.constitutive_model, mpm.alpha])
+
+
+def p2g_grid_phase(sim, mpm, xpbd):
+    """P2G (simulationRoutines.py:48-69) with the grid update of :94-140 in its combine pass."""
+    bc = _bc_code(mpm)
+    n = sim.nPoints
+    gx, gy, gz = mpm.gridDims
+    ws, nb = mpmRoutines.p2g_workspace(n, mpm.gridDims)
+    g = sim.gravity
+    lib("mpmx_p2g_grid_update", n, sim.activeLabel.ptr, sim.materialLabel.ptr, mpm.particle_stress.ptr,
+        sim.particle_x.ptr, sim.particle_v.ptr, mpm.particle_C.ptr, sim.particle_vol.ptr, sim.particle_mass.ptr,
+        float(sim.dx), float(sim.invdx), *[float(v) for v in sim.minBounds], float(mpm.rpic_damping), gx, gy, gz,
+        mpm.grid_m.ptr, mpm.grid_v_in.ptr, mpm.grid_v_out.ptr, float(g[0]), float(g[1]), float(g[2]), float(sim.dt),
+        float(mpm.grid_v_damping_scale), bc, float(mpm.boundFriction), float(abs(g[2])), int(mpm.boundaryPadding),
+        ws, nb, stream_handle())
+
+
+def g2p_phase(sim, mpm, xpbd):
+    """g2p over all particles (simulationRoutines.py:231-250)."""
+    launch(kernel=mpmRoutines.g2p, dim=sim.nPoints,
+           inputs=[sim.dt, sim.activeLabel, sim.materialLabel, sim.particle_x, sim.particle_v, mpm.particle_C,
+                   mpm.particle_F, mpm.particle_F_trial, mpm.particle_cov, sim.invdx, mpm.grid_v_out,
+                   _cov_flag(mpm), sim.minBounds])
+
+
+def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
+    """The same substep with P2G and the grid update in one device pass (3 kernels + binning)."""
+    _check_vlc(mpm)
+    stress_phase(sim, mpm, xpbd)
+    p2g_grid_phase(sim, mpm, xpbd)
+    g2p_phase(sim, mpm, xpbd)
+
+
+def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False):
+    """The XPBD contact step (simulationRoutines.py:295-526), launch for launch."""
+    n = sim.nPoints
+    xpbd.particle_grid.build(sim.particle_x, sim.dx)
+    rt.copy(xpbd.particle_x_integrated, sim.particle_x)
+    rt.copy(xpbd.particle_v_integrated, sim.particle_v)
+    launch(kernel=xpbdRoutines.integrateParticlesXPBD, dim=n,
+           inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_x_integrated, xpbd.particle_v_integrated,
+                   sim.gravity, sim.dtxpbd, xpbd.particle_x_integrated, xpbd.particle_v_integrated,
+                   xpbd.particle_v_max])
+    if xpbd.mpm_contact_transition_lock and not xpbd_only:
+        launch(kernel=xpbdRoutines.reset_mpm_contact_labels, dim=n, inputs=[sim.activeLabel, sim.materialLabel])
+    contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
+    for _ in range(xpbd.xpbd_iterations):
+        xpbd.particle_delta.zero_()
+        launch(kernel=xpbdRoutines.my_solve_particle_bound_contacts, dim=n,
+               inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_x_integrated, xpbd.particle_v_integrated,
+                       sim.particle_mass, sim.particle_radius, xpbd.dynamicParticleFriction,
+                       xpbd.staticVelocityThreshold, xpbd.staticParticleFriction, xpbd.minBoundsXPBD,
+                       xpbd.maxBoundsXPBD, sim.dtxpbd, xpbd.xpbd_relaxation],
+               outputs=[xpbd.particle_delta])
+        launch(kernel=xpbdRoutines.my_solve_particle_particle_contacts, dim=n,
+               inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, xpbd.particle_x_integrated,
+                       xpbd.particle_v_integrated, sim.particle_mass, sim.particle_radius,
+                       xpbd.dynamicParticleFriction, xpbd.staticVelocityThreshold, xpbd.staticParticleFriction,
+                       xpbd.particle_cohesion, xpbd.max_radius, sim.dtxpbd, xpbd.xpbd_relaxation, contact_lock],
+               outputs=[xpbd.particle_delta])
+        rt.copy(xpbd.particle_v_deltaInt, xpbd.particle_v_integrated)
+        rt.copy(xpbd.particle_x_deltaInt, xpbd.particle_x_integrated)
+        launch(kernel=xpbdRoutines.my_apply_particle_deltas, dim=n,
+               inputs=[sim.activeLabel, sim.materialLabel, sim.particle_x, xpbd.particle_x_integrated,
+                       xpbd.particle_delta, sim.dtxpbd, xpbd.particle_v_max],
+               outputs=[xpbd.particle_x_deltaInt, xpbd.particle_v_deltaInt])
+        rt.copy(xpbd.particle_v_integrated, xpbd.particle_v_deltaInt)
+        rt.copy(xpbd.particle_x_integrated, xpbd.particle_x_deltaInt)
+    sim.numTotalXPBD.zero_()
+    sim.numSleepingXPBD.zero_()
+    launch(kernel=xpbdRoutines.sleepParticles, dim=n,
+           inputs=[sim.activeLabel, sim.materialLabel, xpbd.sleepThreshold, sim.particle_radius, sim.particle_x,
+                   xpbd.particle_x_integrated, xpbd.particle_cumDist_xpbd, sim.dtxpbd, sim.numTotalXPBD,
+                   sim.numSleepingXPBD])
+    rt.copy(xpbd.particle_v_before_contact, sim.particle_v)
+    launch(kernel=xpbdRoutines.apply_mpm_contact_impulses, dim=n,
+           inputs=[sim.activeLabel, sim.materialLabel, sim.particle_x, sim.particle_v, xpbd.particle_x_integrated,
+                   xpbd.particle_v_integrated, sim.dtxpbd, xpbd.particle_v_max, xpbd.xpbd_mpm_coupling_strength,
+                   1 if xpbd_only else 0],
+           outputs=[xpbd.particle_v_integrated])
+    sim.particle_x.assign(xpbd.particle_x_integrated)
+    sim.particle_v.assign(xpbd.particle_v_integrated)
+    if xpbd.swellingRatio > 0:
+        launch(kernel=xpbdRoutines.swellParticlesType2, dim=n,
+               inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_x_initial_xpbd, sim.particle_x,
+                       sim.particle_radius, xpbd.swellingActivationFactor, xpbd.swellingMaxFactor,
+                       xpbd.particleMaxRadius, xpbd.particleBaseRadius])
+
+
+# ---------------------------------------------------------------------------
+# reductions (simulationRoutines.py:530-675), FP64 on the device
+# ---------------------------------------------------------------------------
+_red_ws = Workspace()
+_red_out = {}
+
+
+def _red_buffers():
+    if not _red_out:
+        _red_out["sum"] = torch.zeros(1, dtype=torch.float64, device=rt.device())
+        _red_out["cnt"] = torch.zeros(4, dtype=torch.int32, device=rt.device())
+    return _red_out["sum"], _red_out["cnt"]
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def device_velocity_convergence(n, activeLabel, particle_v, particle_radius):
+    """(sum over active of |v|/r as a float64 device scalar, active count as an int32 device scalar)."""
+    s, c = _red_buffers()
+    ws, nb = _red_ws.get(lib.raw("mpmx_reduce_workspace_bytes")(int(n)))
+    lib("mpmx_velocity_convergence", int(n), activeLabel.ptr, particle_v.ptr, particle_radius.ptr, _ptr(s), _ptr(c),
+        ws, nb, stream_handle())
+    return s, c[:1]
+
+
+def _velocityConvergence(dim, activeLabel, materialLabel, particle_v, particle_radius, residual):
+    s, _ = device_velocity_convergence(dim, activeLabel, particle_v, particle_radius)
+    residual.t.add_(s.to(residual.t.dtype))
+
+
+velocityConvergence = Kernel("velocityConvergence", _velocityConvergence, "simulationRoutines.py:538-549")
+
+
+def _countActiveParticles(dim, activeLabel, out):
+    _, c = device_velocity_convergence_count(dim, activeLabel)
+    out.t.add_(c.to(out.t.dtype))
+
+
+def device_velocity_convergence_count(n, activeLabel):
+    s, c = _red_buffers()
+    ws, nb = _red_ws.get(lib.raw("mpmx_reduce_workspace_bytes")(int(n)))
+    lib("mpmx_count_particles_by_type", int(n), activeLabel.ptr, activeLabel.ptr, _ptr(c), ws, nb, stream_handle())
+    # with materialLabel := activeLabel every particle counts as "MPM": slot 0 = active
+    return s, c[:1]
+
+
+countActiveParticles = Kernel("countActiveParticles", _countActiveParticles, "simulationRoutines.py:551-558")
+
+
+def _countParticlesByType(dim, activeLabel, materialLabel, numActiveMPM, numActiveXPBD, numInactiveMPM,
+                          numInactiveXPBD):
+    _, c = _red_buffers()
+    ws, nb = _red_ws.get(lib.raw("mpmx_reduce_workspace_bytes")(int(dim)))
+    lib("mpmx_count_particles_by_type", int(dim), activeLabel.ptr, materialLabel.ptr, _ptr(c), ws, nb,
+        stream_handle())
+    for k, arr in enumerate((numActiveMPM, numActiveXPBD, numInactiveMPM, numInactiveXPBD)):
+        arr.t.add_(c[k:k + 1].to(arr.t.dtype))
+
+
+countParticlesByType = Kernel("countParticlesByType", _countParticlesByType, "simulationRoutines.py:560-585")
+
+
+def device_sum_active(n, arr, activeLabel):
+    s, c = _red_buffers()
+    ws, nb = _red_ws.get(lib.raw("mpmx_reduce_workspace_bytes")(int(n)))
+    lib("mpmx_sum_float_array_active", int(n), arr.ptr, activeLabel.ptr, _ptr(s), _ptr(c), ws, nb, stream_handle())
+    return s, c[:1]
+
+
+def compute_mean_gpu(arr, sim, device=None, filtered=True):
+    """compute_mean_gpu (simulationRoutines.py:633-675): FP64 device sum, one readback."""
+    if not filtered:
+        ones = rt.Array(sim.nPoints, rt.int32, storage=torch.ones(sim.nPoints, dtype=torch.int32,
+                                                                  device=rt.device()))
+        s, c = device_sum_active(sim.nPoints, arr, ones)
+        return float(s.item()) / sim.nPoints if sim.nPoints > 0 else 0.0
+    s, c = device_sum_active(sim.nPoints, arr, sim.activeLabel)
+    both = torch.cat([s, c.to(torch.float64)]).cpu().numpy()
+    total, count = both[0], int(both[1])
+    return total / count if count > 0 else 0.0

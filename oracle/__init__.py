@@ -1,0 +1,245 @@
+"""CPU oracle for the MPMWarpYDW per-substep hot path (TEST INFRASTRUCTURE).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this package, and only as the checker or the timed CPU baseline.
+The product path (``mpmwarpydw_amd``) never imports it.
+
+``Oracle(64)`` wraps ``liboracle_f64.so`` (FP64, the oracle of record) and
+``Oracle(32)`` wraps ``liboracle_f32.so`` (FP32 restatement of the reference's
+arithmetic).  Every method takes numpy arrays in the reference's AoS layout
+(vec3 ``(N,3)``, mat33 ``(N,3,3)``, grid ``(gx,gy,gz[,3])``) and mutates the
+output arrays in place, exactly as the reference kernels mutate Warp arrays.
+Each wrapped C function cites the reference kernel it restates
+(utils/mpmRoutines.py, utils/xpbdRoutines.py, utils/simulationRoutines.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_I = C.c_int
+_L = C.c_long
+_D = C.c_double
+_P = C.c_void_p
+
+
+def build() -> None:
+    """Compile both oracle precisions with the Makefile in this folder."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """ctypes view of one precision of the C oracle."""
+
+    def __init__(self, bits: int = 64, svd_from_ftf: bool | None = None):
+        path = os.path.join(HERE, f"liboracle_f{bits}.so")
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        self.bits = bits
+        self.dtype = np.float64 if bits == 64 else np.float32
+        self.R = C.c_double if bits == 64 else C.c_float
+        assert self.lib.o_real_size() == np.dtype(self.dtype).itemsize
+        if svd_from_ftf is None:
+            svd_from_ftf = bits == 32  # the FP32 restatement resolves F^T F like an SVD of F
+        self.lib.o_set_svd_mode(int(svd_from_ftf))
+        self.lib.o_velocity_convergence.restype = C.c_double
+        self.lib.o_sum_active.restype = C.c_double
+
+    # ---- helpers ---------------------------------------------------------
+    def f(self, a):
+        """Return ``a`` as a C-contiguous array of the oracle's REAL type (copy if needed)."""
+        return np.ascontiguousarray(a, dtype=self.dtype)
+
+    @staticmethod
+    def i(a):
+        return np.ascontiguousarray(a, dtype=np.int32)
+
+    def r(self, x):
+        return self.R(float(x))
+
+    def _chk(self, *arrs):
+        for a in arrs:
+            if a is None:
+                continue
+            assert a.flags.c_contiguous, "oracle arrays must be C-contiguous"
+            assert a.dtype in (self.dtype, np.int32), a.dtype
+
+    # ---- A5 svd3 ---------------------------------------------------------
+    def svd3(self, F):
+        F = self.f(F).reshape(3, 3)
+        U = np.zeros((3, 3), self.dtype)
+        V = np.zeros((3, 3), self.dtype)
+        s = np.zeros(3, self.dtype)
+        lam = np.zeros(3, self.dtype)
+        self.lib.o_svd3(_ptr(F), _ptr(U), _ptr(s), _ptr(V), _ptr(lam))
+        return U, s, V
+
+    # ---- A1 --------------------------------------------------------------
+    def compute_mu_lam_bulk(self, E, nu, mu, lam, bulk):
+        self._chk(E, nu, mu, lam, bulk)
+        self.lib.o_compute_mu_lam_bulk(_I(len(E)), _ptr(E), _ptr(nu), _ptr(mu), _ptr(lam), _ptr(bulk))
+
+    # ---- A2/A3/A4 --------------------------------------------------------
+    def compute_stress_from_F_trial(self, activeLabel, materialLabel, x, v, x_init, v_init, F, F_trial,
+                                    mu, lam, ys, hardening, softening, density, strainCriteria, eff,
+                                    eta_shear, eta_bulk, C_, stress, acc, damage, constitutive_model, alpha):
+        """Argument order of mpmRoutines.compute_stress_from_F_trial (mpmRoutines.py:229-254)."""
+        self._chk(activeLabel, materialLabel, x, v, x_init, v_init, F, F_trial, mu, lam, ys, hardening,
+                  softening, density, strainCriteria, eta_shear, eta_bulk, C_, stress, acc, damage, alpha)
+        self.lib.o_compute_stress_from_F_trial(
+            _I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel), _ptr(x), _ptr(v), _ptr(x_init),
+            _ptr(v_init), _ptr(F), _ptr(F_trial), _ptr(mu), _ptr(lam), _ptr(ys), _ptr(hardening),
+            _ptr(softening), _ptr(density), _ptr(strainCriteria), self.r(eff), _ptr(eta_shear),
+            _ptr(eta_bulk), _ptr(C_), _ptr(stress), _ptr(acc), _ptr(damage), _I(int(constitutive_model)),
+            _ptr(alpha))
+
+    # ---- A6 --------------------------------------------------------------
+    def p2g_apic_with_stress(self, activeLabel, materialLabel, stress, x, v, C_, vol, mass, dx, inv_dx,
+                             minBounds, rpic_damping, grid_m, grid_v_in, dt):
+        """Argument order of mpmRoutines.p2g_apic_with_stress (mpmRoutines.py:485-501)."""
+        self._chk(activeLabel, materialLabel, stress, x, v, C_, vol, mass, grid_m, grid_v_in)
+        gx, gy, gz = grid_m.shape
+        mb = self.f(minBounds)
+        return self.lib.o_p2g(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel), _ptr(stress),
+                              _ptr(x), _ptr(v), _ptr(C_), _ptr(vol), _ptr(mass), self.r(dx), self.r(inv_dx),
+                              _ptr(mb), self.r(rpic_damping), _I(gx), _I(gy), _I(gz), _ptr(grid_m),
+                              _ptr(grid_v_in), self.r(dt))
+
+    # ---- A7/A8/A9 --------------------------------------------------------
+    def grid_normalization_and_gravity(self, grid_m, grid_v_in, grid_v_out, gravity, dt):
+        self._chk(grid_m, grid_v_in, grid_v_out)
+        g = self.f(gravity)
+        self.lib.o_grid_normalization_and_gravity(_L(grid_m.size), _ptr(grid_m), _ptr(grid_v_in),
+                                                  _ptr(grid_v_out), _ptr(g), self.r(dt))
+
+    def add_damping_via_grid(self, grid_v_out, scale):
+        self._chk(grid_v_out)
+        self.lib.o_add_damping(_L(grid_v_out.size // 3), _ptr(grid_v_out), self.r(scale))
+
+    def collideBounds(self, grid_v_out, gx, gy, gz, friction, dt, gmag, padding):
+        self._chk(grid_v_out)
+        self.lib.o_collide_bounds(_ptr(grid_v_out), _I(gx), _I(gy), _I(gz), self.r(friction), self.r(dt),
+                                  self.r(gmag), _I(padding))
+
+    # ---- A10 -------------------------------------------------------------
+    def g2p(self, dt, activeLabel, materialLabel, x, v, C_, F, F_trial, cov, inv_dx, grid_v_out,
+            update_cov, minBounds):
+        """Argument order of mpmRoutines.g2p (mpmRoutines.py:715-728)."""
+        self._chk(activeLabel, materialLabel, x, v, C_, F, F_trial, cov, grid_v_out)
+        gx, gy, gz = grid_v_out.shape[:3]
+        mb = self.f(minBounds)
+        return self.lib.o_g2p(self.r(dt), _I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel),
+                              _ptr(x), _ptr(v), _ptr(C_), _ptr(F), _ptr(F_trial), _ptr(cov), self.r(inv_dx),
+                              _ptr(grid_v_out), _I(gx), _I(gy), _I(gz), _I(int(bool(update_cov))), _ptr(mb))
+
+    # ---- XPBD ------------------------------------------------------------
+    def reset_mpm_contact_labels(self, activeLabel, materialLabel):
+        self._chk(activeLabel, materialLabel)
+        self.lib.o_reset_mpm_contact_labels(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel))
+
+    def integrateParticlesXPBD(self, activeLabel, materialLabel, x, v, gravity, dt, x_new, v_new, v_max):
+        self._chk(activeLabel, materialLabel, x, v, x_new, v_new)
+        g = self.f(gravity)
+        self.lib.o_integrate_particles_xpbd(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel),
+                                            _ptr(x), _ptr(v), _ptr(g), self.r(dt), _ptr(x_new), _ptr(v_new),
+                                            self.r(v_max))
+
+    def my_solve_particle_bound_contacts(self, activeLabel, materialLabel, x, v, mass, radius, mu, thr,
+                                         static_mu, minb, maxb, dt, relaxation, delta):
+        self._chk(activeLabel, materialLabel, x, v, mass, radius, delta)
+        self.lib.o_solve_particle_bound_contacts(
+            _I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel), _ptr(x), _ptr(v), _ptr(mass),
+            _ptr(radius), self.r(mu), self.r(thr), self.r(static_mu), _ptr(self.f(minb)), _ptr(self.f(maxb)),
+            self.r(dt), self.r(relaxation), _ptr(delta))
+
+    def hashgrid_build(self, x, cell_width, dims=(128, 128, 128)):
+        """wp.HashGrid(dims).build(x, cell_width) -> (point_ids, cell_starts, cell_ends)."""
+        x = self.f(x)
+        n = len(x)
+        ncell = int(np.prod(dims))
+        ids = np.zeros(n, np.int32)
+        cs = np.zeros(ncell, np.int32)
+        ce = np.zeros(ncell, np.int32)
+        self.lib.o_hashgrid_build(_I(n), _ptr(x), self.r(cell_width), _I(dims[0]), _I(dims[1]), _I(dims[2]),
+                                  _ptr(ids), _ptr(cs), _ptr(ce))
+        return ids, cs, ce
+
+    def hashgrid_query(self, grid, pos, radius):
+        """Candidate ids of wp.hash_grid_query(grid, pos, radius) in visiting order."""
+        ids, cs, ce, width, dims = grid
+        out = np.zeros(len(ids) + 1, np.int32)
+        k = self.lib.o_hashgrid_query(_ptr(self.f(pos)), self.r(radius), _ptr(ids), _ptr(cs), _ptr(ce),
+                                      self.r(width), _I(dims[0]), _I(dims[1]), _I(dims[2]), _ptr(out),
+                                      _I(len(out)))
+        return out[:k].tolist()
+
+    def my_solve_particle_particle_contacts(self, activeLabel, materialLabel, grid, x, v, mass, radius, k_mu,
+                                            thr, static_mu, k_cohesion, max_radius, dt, relaxation, lock,
+                                            deltas):
+        """``grid`` = (point_ids, cell_starts, cell_ends, cell_width, dims)."""
+        ids, cs, ce, width, dims = grid
+        self._chk(activeLabel, materialLabel, x, v, mass, radius, deltas)
+        self.lib.o_solve_particle_particle_contacts(
+            _I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel), _ptr(ids), _ptr(cs), _ptr(ce),
+            self.r(width), _I(dims[0]), _I(dims[1]), _I(dims[2]), _ptr(x), _ptr(v), _ptr(mass), _ptr(radius),
+            self.r(k_mu), self.r(thr), self.r(static_mu), self.r(k_cohesion), self.r(max_radius), self.r(dt),
+            self.r(relaxation), _I(int(lock)), _ptr(deltas))
+
+    def my_apply_particle_deltas(self, activeLabel, materialLabel, x_orig, x_pred, delta, dt, v_max, x_out,
+                                 v_out):
+        self._chk(activeLabel, materialLabel, x_orig, x_pred, delta, x_out, v_out)
+        self.lib.o_apply_particle_deltas(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel),
+                                         _ptr(x_orig), _ptr(x_pred), _ptr(delta), self.r(dt), self.r(v_max),
+                                         _ptr(x_out), _ptr(v_out))
+
+    def apply_mpm_contact_impulses(self, activeLabel, materialLabel, x_orig, v_orig, x_final, v_final, dt,
+                                   v_max, coupling, xpbd_only, v_out):
+        self._chk(activeLabel, materialLabel, x_orig, v_orig, x_final, v_final, v_out)
+        self.lib.o_apply_mpm_contact_impulses(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel),
+                                              _ptr(x_orig), _ptr(v_orig), _ptr(x_final), _ptr(v_final),
+                                              self.r(dt), self.r(v_max), self.r(coupling), _I(int(xpbd_only)),
+                                              _ptr(v_out))
+
+    def sleepParticles(self, activeLabel, materialLabel, thr, radius, x_prev, x_after, dist_total, dt):
+        self._chk(activeLabel, materialLabel, radius, x_prev, x_after, dist_total)
+        cnt = np.zeros(2, np.int32)
+        self.lib.o_sleep_particles(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel), self.r(thr),
+                                   _ptr(radius), _ptr(x_prev), _ptr(x_after), _ptr(dist_total), self.r(dt),
+                                   _ptr(cnt))
+        return int(cnt[0]), int(cnt[1])
+
+    def swellParticlesType2(self, activeLabel, materialLabel, x_init, x, radius, factor, factor_max, swell_r,
+                            base_r):
+        self._chk(activeLabel, materialLabel, x_init, x, radius, swell_r, base_r)
+        self.lib.o_swell_particles_type2(_I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel),
+                                         _ptr(x_init), _ptr(x), _ptr(radius), self.r(factor),
+                                         self.r(factor_max), _ptr(swell_r), _ptr(base_r))
+
+    # ---- A12 -------------------------------------------------------------
+    def velocityConvergence(self, activeLabel, v, radius):
+        self._chk(activeLabel, v, radius)
+        return self.lib.o_velocity_convergence(_I(len(activeLabel)), _ptr(activeLabel), _ptr(v), _ptr(radius))
+
+    def countActiveParticles(self, activeLabel):
+        return int(self.lib.o_count_active(_I(len(activeLabel)), _ptr(self.i(activeLabel))))
+
+    def countParticlesByType(self, activeLabel, materialLabel):
+        out = np.zeros(4, np.int32)
+        self.lib.o_count_by_type(_I(len(activeLabel)), _ptr(self.i(activeLabel)), _ptr(self.i(materialLabel)),
+                                 _ptr(out))
+        return tuple(int(v) for v in out)
+
+    def sum_float_array_active(self, arr, activeLabel):
+        cnt = C.c_int(0)
+        s = self.lib.o_sum_active(_I(len(arr)), _ptr(self.f(arr)), _ptr(self.i(activeLabel)), C.byref(cnt))
+        return s, cnt.value

@@ -1,0 +1,196 @@
+"""Pins the oracle's transfers and grid update (mpmRoutines.py:474-569 P2G,
+:691-711 normalisation/damping, :715-802 G2P, :915-993 collideBounds) against
+properties of quadratic B-spline MPM that are independent of the formulas:
+partition of unity, linear-field reproduction, momentum conservation, the
+first moment of the stress impulse, hand-computed single-particle weights and
+per-node closed forms of the Coulomb boundary.  CPU only."""
+import numpy as np
+
+from conftest import rel_err
+
+
+def make_grid(dims):
+    return np.zeros(dims), np.zeros(dims + (3,))
+
+
+def p2g(o, x, v, C, stress, vol, mass, dx, minB, dims, dt, rpic=0.0, active=None, mat=None):
+    n = len(x)
+    active = np.ones(n, np.int32) if active is None else active
+    mat = np.ones(n, np.int32) if mat is None else mat
+    gm, gv = make_grid(dims)
+    bad = o.p2g_apic_with_stress(active, mat, np.ascontiguousarray(stress), np.ascontiguousarray(x),
+                                 np.ascontiguousarray(v), np.ascontiguousarray(C), vol, mass, dx, 1.0 / dx, minB, rpic,
+                                 gm, gv, dt)
+    return gm, gv, bad
+
+
+def test_single_particle_weights(o64):
+    """A particle at grid coordinate base + 1 has per-axis weights (1/8, 3/4, 1/8)."""
+    dx, minB, dims = 0.5, np.zeros(3), (6, 6, 6)
+    x = np.array([[2 * dx, 2 * dx, 2 * dx]])  # gp = 2 -> base = int(1.5) = 1, fx = 1
+    v = np.array([[1.0, -2.0, 3.0]])
+    gm, gv, bad = p2g(o64, x, v, np.zeros((1, 3, 3)), np.zeros((1, 3, 3)), np.array([0.1]), np.array([4.0]), dx,
+                      minB, dims, 1e-3)
+    w = np.array([0.125, 0.75, 0.125])
+    W = np.einsum("i,j,k->ijk", w, w, w)
+    assert bad == 0
+    assert np.allclose(gm[1:4, 1:4, 1:4], 4.0 * W, rtol=1e-15)
+    assert gm.sum() == np.float64(4.0) or np.isclose(gm.sum(), 4.0, rtol=1e-15)
+    assert np.allclose(gv[1:4, 1:4, 1:4], 4.0 * W[..., None] * v[0], rtol=1e-14)
+
+
+def test_p2g_conservation(o64):
+    """Sum of grid mass = sum of particle mass; sum of grid momentum = sum m v
+    (the APIC term and the stress impulse both sum to zero over the stencil)."""
+    rng = np.random.default_rng(0)
+    n, dx, dims = 300, 0.5, (14, 12, 13)
+    minB = np.array([-1.0, 0.3, 2.0])
+    x = minB + dx * (2.0 + rng.random((n, 3)) * (np.array(dims) - 5))
+    v = rng.standard_normal((n, 3))
+    C = rng.standard_normal((n, 3, 3))
+    S = 1e4 * rng.standard_normal((n, 3, 3))
+    m = 1 + rng.random(n)
+    vol = 0.01 + 0.01 * rng.random(n)
+    gm, gv, bad = p2g(o64, x, v, C, S, vol, m, dx, minB, dims, 1e-3, rpic=0.3)
+    assert bad == 0
+    assert np.isclose(gm.sum(), m.sum(), rtol=1e-13)
+    assert np.allclose(gv.reshape(-1, 3).sum(0), (m[:, None] * v).sum(0), rtol=1e-10, atol=1e-9)
+
+
+def test_p2g_stress_first_moment(o64):
+    """With v = C = 0 the grid impulse f_i = -dt vol tau grad w_i satisfies
+    sum_i f_i (x_i - x_p)^T = -dt vol tau (gradient of a linear field)."""
+    rng = np.random.default_rng(1)
+    dx, dims, dt, vol = 0.5, (8, 8, 8), 2e-3, 0.03
+    minB = np.zeros(3)
+    for _ in range(10):
+        x = (2.5 + 2 * rng.random((1, 3))) * dx
+        tau = rng.standard_normal((1, 3, 3))
+        gm, gv, _ = p2g(o64, x, np.zeros((1, 3)), np.zeros((1, 3, 3)), tau, np.array([vol]), np.array([1.0]), dx,
+                        minB, dims, dt)
+        idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1) * dx
+        M = np.einsum("ijka,ijkb->ab", gv, idx - x[0])
+        assert np.allclose(M, -dt * vol * tau[0], rtol=1e-12, atol=1e-15)
+
+
+def test_pic_and_rpic(o64):
+    """rpic < -0.001 drops C (PIC); rpic = 1 keeps only the skew part of C."""
+    rng = np.random.default_rng(2)
+    dx, dims, minB = 0.5, (8, 8, 8), np.zeros(3)
+    x = (3 + rng.random((1, 3))) * dx
+    C = rng.standard_normal((1, 3, 3))
+    g0 = p2g(o64, x, np.zeros((1, 3)), np.zeros((1, 3, 3)), np.zeros((1, 3, 3)), np.ones(1), np.ones(1), dx, minB,
+             dims, 1e-3)[1]
+    gp = p2g(o64, x, np.zeros((1, 3)), C, np.zeros((1, 3, 3)), np.ones(1), np.ones(1), dx, minB, dims, 1e-3,
+             rpic=-1.0)[1]
+    assert np.array_equal(g0, gp)
+    skew = 0.5 * (C - C.transpose(0, 2, 1))
+    g1 = p2g(o64, x, np.zeros((1, 3)), C, np.zeros((1, 3, 3)), np.ones(1), np.ones(1), dx, minB, dims, 1e-3,
+             rpic=1.0)[1]
+    g2 = p2g(o64, x, np.zeros((1, 3)), skew, np.zeros((1, 3, 3)), np.ones(1), np.ones(1), dx, minB, dims, 1e-3)[1]
+    assert np.allclose(g1, g2, rtol=1e-13, atol=1e-16)
+
+
+def test_out_of_grid_particles_are_skipped(o64):
+    """base = int(gp - 0.5) truncates toward zero, so gp = 0.2 still lands on base 0
+    (the reference's quirk, weights still sum to one); stencils past the grid are skipped."""
+    dx, dims, minB = 0.5, (6, 6, 6), np.zeros(3)
+    x = np.array([[0.1, 1.5, 1.5], [2.9, 1.5, 1.5], [-0.3, 1.5, 1.5], [1.5, 1.5, 1.5]])
+    gm, gv, bad = p2g(o64, x, np.zeros((4, 3)), np.zeros((4, 3, 3)), np.zeros((4, 3, 3)), np.ones(4), np.ones(4), dx,
+                      minB, dims, 1e-3)
+    assert bad == 2 and np.isclose(gm.sum(), 2.0, rtol=1e-14)
+    assert np.isclose(gm[0].sum(), 0.845, rtol=1e-12)  # w0(0.2) = 0.5 * 1.3^2 on the x = 0 plane
+
+
+def test_g2p_reproduces_affine_fields(o64):
+    """Grid velocity a + B x gives v_p = a + B x_p, C_p = B, F_trial = (I + dt B) F."""
+    rng = np.random.default_rng(3)
+    n, dx, dims, dt = 50, 0.5, (10, 11, 12), 1e-3
+    minB = np.array([0.2, -0.4, 1.0])
+    a = rng.standard_normal(3)
+    B = rng.standard_normal((3, 3))
+    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1) * dx + minB
+    gv = a + idx @ B.T
+    x = minB + dx * (2 + rng.random((n, 3)) * (np.array(dims) - 5))
+    x0 = x.copy()
+    v, C = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    F = np.array([np.eye(3) + 0.1 * rng.standard_normal((3, 3)) for _ in range(n)])
+    Ft = np.zeros((n, 3, 3))
+    cov = np.zeros(6 * n)
+    bad = o64.g2p(dt, np.ones(n, np.int32), np.ones(n, np.int32), x, v, C, F, Ft, cov, 1.0 / dx,
+                  np.ascontiguousarray(gv), True, minB)
+    assert bad == 0
+    assert np.allclose(v, a + x0 @ B.T, rtol=1e-12, atol=1e-12)
+    assert np.allclose(C, np.broadcast_to(B, C.shape), rtol=1e-11, atol=1e-11)
+    assert np.allclose(x, x0 + dt * v, rtol=1e-15)
+    assert np.allclose(Ft, np.einsum("ij,pjk->pik", np.eye(3) + dt * B, F), rtol=1e-12, atol=1e-14)
+    assert np.all(cov == 0)  # the homogeneous covariance update keeps zero at zero
+
+
+def test_g2p_covariance_update(o64):
+    dx, dims, dt = 0.5, (8, 8, 8), 1e-2
+    B = np.array([[0.1, 0.2, 0.0], [-0.3, 0.0, 0.5], [0.0, 0.1, -0.2]])
+    idx = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1) * dx
+    gv = idx @ B.T
+    c0 = np.array([1.0, 0.1, 0.2, 2.0, 0.3, 3.0])
+    cov = c0.copy()
+    x = np.array([[2.1, 1.9, 2.05]])
+    o64.g2p(dt, np.ones(1, np.int32), np.ones(1, np.int32), x, np.zeros((1, 3)), np.zeros((1, 3, 3)),
+            np.eye(3)[None].copy(), np.zeros((1, 3, 3)), cov, 1 / dx, np.ascontiguousarray(gv), True, np.zeros(3))
+    S = np.array([[1.0, 0.1, 0.2], [0.1, 2.0, 0.3], [0.2, 0.3, 3.0]])
+    S1 = S + dt * (B @ S + S @ B.T)
+    assert np.allclose(cov, S1[np.triu_indices(3)], rtol=1e-12)
+
+
+def test_grid_normalization(o64):
+    rng = np.random.default_rng(4)
+    dims = (3, 4, 5)
+    m = rng.random(dims)
+    m[0, 0, 0] = 1e-16
+    vin = rng.standard_normal(dims + (3,))
+    vout = np.zeros(dims + (3,))
+    o64.grid_normalization_and_gravity(m, vin, vout, np.array([0.0, 0.0, -9.81]), 1e-3)
+    want = vin / m[..., None] + 1e-3 * np.array([0, 0, -9.81])
+    want[0, 0, 0] = 0.0
+    assert np.allclose(vout, want, rtol=1e-14)
+    o64.add_damping_via_grid(vout, 0.5)
+    assert np.allclose(vout, 0.5 * want, rtol=1e-14)
+
+
+def coul(v, mu, dt, g):
+    if abs(v) < 1e-12:
+        return 0.0
+    return np.sign(v) * max(abs(v) - mu * g * dt, 0.0)
+
+
+def test_collide_bounds_cases(o64):
+    """Lower faces act for index <= pad with inward velocity, upper faces for
+    index >= dim - pad; each match rewrites the vector from the original
+    velocity, so at a corner the last matching face (z) wins."""
+    dims, mu, dt, g, pad = (8, 8, 8), 0.5, 1e-2, 9.81, 2
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(dims + (3,))
+    v[4, 4, 4] = [-5, -5, -5]
+    v[0, 0, 0] = [-1.0, -0.02, -3.0]
+    v[7, 6, 3] = [2.0, 0.5, 0.01]
+    v[2, 5, 5] = [-1e-13, 0.3, -0.3]
+    out = v.copy()
+    o64.collideBounds(out, *dims, mu, dt, g, pad)
+    assert np.array_equal(out[4, 4, 4], v[4, 4, 4])
+    assert np.allclose(out[0, 0, 0], [coul(-1.0, mu, dt, g), coul(-0.02, mu, dt, g), 0.0])
+    # x-upper and y-upper both match: the later (y) face wins
+    assert np.allclose(out[7, 6, 3], [coul(2.0, mu, dt, g), 0.0, coul(0.01, mu, dt, g)])
+    # brute force every node against the rule
+    ref = v.copy()
+    for ix in range(8):
+        for iy in range(8):
+            for iz in range(8):
+                o = v[ix, iy, iz]
+                r = o.copy()
+                for ax, i in enumerate((ix, iy, iz)):
+                    for lower in (True, False):
+                        hit = (i <= pad and o[ax] < 0) if lower else (i >= dims[ax] - pad and o[ax] > 0)
+                        if hit:
+                            r = np.array([0.0 if a == ax else coul(o[a], mu, dt, g) for a in range(3)])
+                ref[ix, iy, iz] = r
+    assert np.allclose(out, ref, rtol=0, atol=0)
