@@ -2,22 +2,27 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scene undercut_slab]
 
-A step is one MPM substep of the plain method over the whole scene (stress +
-return map, deterministic P2G with the grid update, G2P), with the XPBD
-contact step every ``mpmStepsPerXpbdStep`` substeps as the reference's loop
-runs it (runMPMYDW.py:677-682).  The default workload is the undercut slab:
-960,000 particles, grid 104x54x31, dt = 5e-5 s, XPBD every 10 substeps.
-Inputs are resident in HBM before the timed region; W untimed substeps, then
-K timed substeps between two device synchronisations.
+A step is one MPM substep of the plain method over the whole scene: the fused
+substep (constitutive update + deterministic P2G, grid update, G2P with
+re-binning), plus the XPBD contact step every ``mpmStepsPerXpbdStep`` substeps
+as the reference's loop runs it (runMPMYDW.py:677-682).  The default workload
+is the undercut slab: 960,000 particles, grid 104x54x31, dt = 5e-5 s, XPBD
+every 10 substeps.  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line with the metric (MPM particle-substeps/s), per-phase
-device times from HIP events recorded on the launch stream inside the timed
-region, the roofline entry of the dominant kernel and a CPU baseline (the
-FP32 restatement under oracle/, one core, on a bounded sample).
+One XPBD period (10 substeps, the XPBD step included) is captured once in a
+HIP graph; W untimed substeps run first, then the graph is replayed K/10
+times between two device synchronisations.  The library's launch profiler is
+captured into the graph (an event after every kernel), so the per-kernel
+durations are those of the last period of the timed region.
+
+Prints one JSON line: the metric (MPM particle-substeps/s), per-kernel device
+times, the roofline entry of the dominant kernel and a CPU baseline (the FP32
+restatement under oracle/, one core, on a bounded sample).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -29,6 +34,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per MPM particle of the fused tile kernel (constitutive
+# update + P2G records): reads F_trial 36, C 36, nine material floats 36,
+# accumulated strain 4, damage 4, label 4, x 12, v 12, vol 4, mass 4, bin id 4;
+# writes F 36, stress 36, accumulated strain / damage / ys 12
+TILE_BYTES_PER_PARTICLE = 156.0 + 84.0
 
 
 def parse():
@@ -38,6 +48,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=40)
     p.add_argument("--scene", default="undercut_slab")
     p.add_argument("--scale", type=float, default=1.0)
+    p.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     return p.parse_args()
@@ -87,8 +98,19 @@ def cpu_baseline(S, budget_s):
         if el > budget_s or steps >= 50:
             break
     return {"value": n_mpm * steps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} MPM substeps (stress+P2G+grid+G2P, FP32, no XPBD) of the same {n_mpm}-particle "
-                      f"state, {el:.1f} s on one host core"}
+            "sample": f"{steps} MPM substeps (stress+P2G+grid+G2P, FP32, no XPBD step) of the same "
+                      f"{n_mpm}-particle state, {el:.1f} s on one host core"}
+
+
+def profile_report():
+    from mpmwarpydw_amd._lib import lib
+    buf = C.create_string_buffer(1 << 16)
+    lib("mpmx_profile_report", buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.rsplit(" ", 2)
+        out[name] = {"launches": int(cnt), "total_ms": float(ms), "avg_us": 1e3 * float(ms) / max(1, int(cnt))}
+    return out
 
 
 def main():
@@ -98,69 +120,89 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from mpmwarpydw_amd import driver, getArgs, scenes, simulationRoutines as SR
+    from mpmwarpydw_amd import driver, getArgs, scenes
+    from mpmwarpydw_amd._lib import lib
+    from mpmwarpydw_amd.runtime import stream_handle
 
     dom, ov = scenes.SCENES[a.scene](scale=a.scale)
     args = getArgs.get_args([], ov)
     S = driver.setup(args, dom)
-    sim, mpm, xpbd = S.sim, S.mpm, S.xpbd
+    sim, mpm = S.sim, S.mpm
     n = sim.nPoints
     ratio = sim.mpmStepsPerXpbdStep
-    print(f"[bench] scene={a.scene} n={n} grid={mpm.gridDims} ratio={ratio} steps={a.steps} warmup={a.warmup}",
+    periods = max(1, a.steps // ratio)
+    steps = periods * ratio
+    print(f"[bench] scene={a.scene} n={n} grid={mpm.gridDims} ratio={ratio} steps={steps} warmup={a.warmup}",
           file=sys.stderr, flush=True)
 
-    counter = 0
-    for _ in range(a.warmup):
-        S.substep(counter)
-        counter += 1
+    counter = 1  # start inside a period so that warmup + replays keep the reference's XPBD phase
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for _ in range(max(a.warmup, ratio)):
+            S.substep(counter)
+            counter += 1
+        while counter % ratio != 1:
+            S.substep(counter)
+            counter += 1
     torch.cuda.synchronize()
 
-    # per-phase events on the launch stream (torch's current stream is the one libmpmx launches on)
-    ev = lambda: torch.cuda.Event(enable_timing=True)
-    phases = {"stress": 0.0, "p2g_grid": 0.0, "g2p": 0.0, "xpbd": 0.0}
-    marks = []
-    n_mpm_total = 0
-    torch.cuda.synchronize()
+    period = [counter + k for k in range(ratio)]  # the XPBD step falls on the multiple of ratio inside
+    graph = None
+    lib("mpmx_profile_begin", 64 * ratio + 64)
+    if not a.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            lib("mpmx_profile_mark", stream_handle())
+            for c in period:
+                S.substep(c)
+        lib("mpmx_profile_end")
+        graph.replay()  # the captured period runs once before timing (an untimed step)
+        torch.cuda.synchronize()
+
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        e = [ev() for _ in range(5)]
-        e[0].record()
-        SR.stress_phase(sim, mpm, xpbd)
-        e[1].record()
-        SR.p2g_grid_phase(sim, mpm, xpbd)
-        e[2].record()
-        SR.g2p_phase(sim, mpm, xpbd)
-        e[3].record()
-        if counter % ratio == 0 and counter > 0:
-            SR.xpbdSimulationStep(sim, mpm, xpbd)
-        e[4].record()
-        marks.append(e)
-        counter += 1
+    with torch.cuda.stream(stream):
+        for _ in range(periods):
+            if graph is not None:
+                graph.replay()
+            else:
+                lib("mpmx_profile_mark", stream_handle())
+                for c in period:
+                    S.substep(c)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    for e in marks:
-        phases["stress"] += e[0].elapsed_time(e[1])
-        phases["p2g_grid"] += e[1].elapsed_time(e[2])
-        phases["g2p"] += e[2].elapsed_time(e[3])
-        phases["xpbd"] += e[3].elapsed_time(e[4])
+    lib("mpmx_profile_end")
+    try:
+        kernels = profile_report()
+        how = "events captured in the graph (last replayed period)" if graph is not None else "eager events"
+    except Exception as e:  # timing of graph-recorded events unsupported: profile one eager period instead
+        print(f"[bench] graph event timing unavailable ({e}); profiling one eager period", file=sys.stderr)
+        lib("mpmx_profile_begin", 64 * ratio + 64)
+        with torch.cuda.stream(stream):
+            lib("mpmx_profile_mark", stream_handle())
+            for c in period:
+                S.substep(c)
+        torch.cuda.synchronize()
+        lib("mpmx_profile_end")
+        kernels = profile_report()
+        how = "eager events (one period after the timed region)"
+
     lab = sim.materialLabel.numpy()
     act = sim.activeLabel.numpy()
     n_mpm = int(((act == 1) & (lab <= 1)).sum())
-    value = n_mpm * a.steps / wall
+    value = n_mpm * steps / wall
     G = int(np.prod(mpm.gridDims))
-    per_phase_ms = {k: v / a.steps for k, v in phases.items()}
-    # stress kernel: 124 B read + 84 B written per MPM particle (F_trial, C, 9 material floats, acc, damage, labels
-    # in; F, stress, acc, damage, ys out)
-    stress_bytes = 208.0 * n_mpm
-    achieved = stress_bytes / (per_phase_ms["stress"] * 1e-3) / 1e9
+    dom_name, dom_k = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
+    tile = kernels.get("substep_tile(stress+p2g)")
+    achieved = TILE_BYTES_PER_PARTICLE * n_mpm / (tile["avg_us"] * 1e-6) / 1e9 if tile else None
+    sub_bytes = 300.0 * n_mpm + 28.0 * G
     out = {
         "metric": "MPM particle-substeps/s (plain method)",
         "value": value,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": wall * 1e3 / a.steps,
+        "ms_per_step": wall * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -168,12 +210,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{a.scene}: {n} particles ({n_mpm} MPM), grid {mpm.gridDims[0]}x{mpm.gridDims[1]}x"
                                f"{mpm.gridDims[2]}, dt {sim.dt:g} s, XPBD every {ratio} substeps",
-                   "particles": n, "grid_nodes": G},
-        "phase_ms": per_phase_ms,
-        "roofline": {"bound": "hbm", "kernel": "stress_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
-        "algorithmic_bytes_per_substep": 300.0 * n_mpm + 28.0 * G,
-        "substep_hbm_frac": (300.0 * n_mpm + 28.0 * G) / (wall / a.steps) / 1e9 / HBM_PEAK_GBS,
+                   "particles": n, "grid_nodes": G, "hip_graph": graph is not None},
+        "roofline": {"bound": "hbm", "kernel": "substep_tile(stress+p2g)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+        "substep_algorithmic_bytes": sub_bytes,
+        "substep_hbm_frac": sub_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+        "dominant_kernel": dom_name,
+        "kernel_timing": how,
+        "kernels_us_per_launch": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(S, a.cpu_seconds)

@@ -38,8 +38,72 @@ extern "C" {
 #define MPMX_BC_NONE 0
 #define MPMX_BC_FRICTION 1
 
+/* Device pointers of the per-particle state arrays (the MPMState/SimState/
+ * XPBDState fields of simStates.py, SoA layout) for the fused substep. */
+typedef struct mpmx_particles {
+    int n;
+    int32_t* activeLabel;
+    int32_t* materialLabel;
+    float* x;            /* vec3 */
+    float* v;            /* vec3 */
+    float* x_init_xpbd;  /* vec3, particle_x_initial_xpbd */
+    float* v_init_xpbd;  /* vec3, particle_v_initial_xpbd */
+    float* F;            /* mat33 */
+    float* F_trial;      /* mat33 */
+    float* C;            /* mat33, APIC affine matrix */
+    float* stress;       /* mat33, Kirchhoff */
+    float* cov;          /* float[6n] */
+    float* mu;
+    float* lam;
+    float* ys;
+    float* alpha;
+    float* hardening;
+    float* softening;
+    float* density;
+    float* strainCriteria;
+    float* eta_shear;
+    float* eta_bulk;
+    float* acc_strain;
+    float* damage;
+    float* vol;
+    float* mass;
+} mpmx_particles;
+
+/* Grid arrays of MPMState (simStates.py:117-121). */
+typedef struct mpmx_grid {
+    int gx, gy, gz;
+    float* m;     /* [G] */
+    float* v_in;  /* [3][G] */
+    float* v_out; /* [3][G] */
+} mpmx_grid;
+
+/* Scalars of one substep (SimState/MPMState fields used by mpmSimulationStep). */
+typedef struct mpmx_step_params {
+    float dt, dx, inv_dx;
+    float minx, miny, minz;
+    float gravity_x, gravity_y, gravity_z;
+    float rpic_damping;
+    float eff;
+    float damping_scale;      /* grid_v_damping_scale (applied when < 1) */
+    float friction;           /* boundFriction */
+    float gravity_magnitude;  /* |gravity_z| */
+    int bc;                   /* MPMX_BC_* */
+    int boundary_padding;
+    int update_cov;
+    int constitutive_model;   /* passed through, never read (mpmRoutines.py:253) */
+} mpmx_step_params;
+
 const char* mpmx_last_error(void);
 int mpmx_version(void);
+/* Launch profiler (no counterpart in the reference; replaces its wall-clock
+ * prints, runMPMYDW.py:762): between begin and end an event is recorded on the
+ * launch stream after every kernel the library issues; a kernel's time is the
+ * gap to the previous event, so call mpmx_profile_mark() on the stream before
+ * a timed sequence.  report() writes "name launches total_ms" lines. */
+int mpmx_profile_begin(int capacity);
+int mpmx_profile_mark(void* stream);
+int mpmx_profile_end(void);
+int mpmx_profile_report(char* buf, size_t buflen);
 /* device synchronisation used by the host wrappers (mirrors .numpy() syncs) */
 int mpmx_stream_sync(void* stream);
 /* fill n int32 / float words with a value (wp.array.zero_()/fill_()) */
@@ -99,6 +163,28 @@ int mpmx_p2g_grid_update(int n, const int32_t* activeLabel, const int32_t* mater
                          float gravity_magnitude, int boundary_padding, void* workspace,
                          size_t ws_bytes, void* stream);
 
+/* mpmSimulationStep (simulationRoutines.py:6-250: stress, P2G, grid update,
+ * G2P) as three passes over the state: a tile pass running the constitutive
+ * update of every particle and its deterministic P2G reduction, a node pass
+ * for the grid, and a tile pass for G2P that also bins each particle for the
+ * next substep.  Results equal compute_stress_from_F_trial + p2g + grid update
+ * + g2p.  Particles whose stencil leaves the grid get the constitutive update
+ * and no transfer.  Call mpmx_substep_bin() once before the first substep and
+ * again after any change of x, labels or activity other than this function's
+ * own (the XPBD step between substeps restores MPM positions and needs none).
+ * Workspace: mpmx_substep_workspace_bytes(n, gx, gy, gz), kept between calls. */
+size_t mpmx_substep_workspace_bytes(int n, int gx, int gy, int gz);
+int mpmx_substep_bin(const mpmx_particles* particles, const mpmx_grid* grid,
+                     const mpmx_step_params* params, void* workspace, size_t ws_bytes,
+                     void* stream);
+int mpmx_mpm_substep(const mpmx_particles* particles, const mpmx_grid* grid,
+                     const mpmx_step_params* params, void* workspace, size_t ws_bytes,
+                     void* stream);
+/* Number of particles found outside their bin's cell since the last
+ * mpmx_substep_bin() (0 when bins were kept valid; synchronising). */
+int mpmx_substep_misbinned(void* workspace, size_t ws_bytes, int gx, int gy, int gz, int n,
+                           int* host_count, void* stream);
+
 /* grid_normalization_and_gravity, mpmRoutines.py:691-704: writes v_out only
  * where m > 1e-15 (the caller zeroes v_out first, simulationRoutines.py:13) */
 int mpmx_grid_normalization_and_gravity(int gx, int gy, int gz, const float* grid_m,
@@ -156,6 +242,14 @@ int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int d
                         int32_t* point_ids, int32_t* cell_start, void* workspace,
                         size_t ws_bytes, void* stream);
 
+/* The same table over the points whose materialLabel equals `label` only (the
+ * others are left out; cell_start[dims] = number of binned points).  With
+ * label 2 this is the XPBD-only table used by the split contact solve below. */
+int mpmx_hashgrid_build_label(int n, const float* x, float cell_width, int dimx, int dimy,
+                              int dimz, const int32_t* materialLabel, int label,
+                              int32_t* point_ids, int32_t* cell_start, void* workspace,
+                              size_t ws_bytes, void* stream);
+
 /* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,
                                   void* stream);
@@ -184,6 +278,19 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel,
                                           float k_cohesion, float max_radius, float dt,
                                           float relaxation, int contact_lock, float* deltas,
                                           void* stream);
+/* my_solve_particle_particle_contacts with a second table holding only the
+ * label-2 points (mpmx_hashgrid_build_label(..., 2, ...) on the same positions):
+ * MPM particles (labels 0/1), which skip every MPM candidate in the reference
+ * (xpbdRoutines.py:185-186), walk the XPBD-only table, whose candidates come in
+ * the same relative order.  Results equal mpmx_solve_particle_particle_contacts. */
+int mpmx_solve_particle_particle_contacts_split(
+    int n, const int32_t* activeLabel, int32_t* materialLabel, const int32_t* point_ids,
+    const int32_t* cell_start, const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
+    float cell_width, int dimx, int dimy, int dimz, const float* x, const float* v,
+    const float* mass, const float* radius, float k_mu, float static_threshold,
+    float static_friction, float k_cohesion, float max_radius, float dt, float relaxation,
+    int contact_lock, float* deltas, void* stream);
+
 /* my_apply_particle_deltas, xpbdRoutines.py:221-257 (x_out/v_out may alias x_pred) */
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel,
                                const float* x_orig, const float* x_pred, const float* delta,

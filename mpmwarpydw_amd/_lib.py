@@ -25,6 +25,14 @@ IP = C.POINTER(C.c_int)
 SIGNATURES = {
     "mpmx_last_error": [],
     "mpmx_version": [],
+    "mpmx_profile_begin": [I],
+    "mpmx_profile_mark": [P],
+    "mpmx_profile_end": [],
+    "mpmx_profile_report": [P, S],
+    "mpmx_substep_workspace_bytes": [I, I, I, I],
+    "mpmx_substep_bin": [P, P, P, P, S, P],
+    "mpmx_mpm_substep": [P, P, P, P, S, P],
+    "mpmx_substep_misbinned": [P, S, I, I, I, I, IP, P],
     "mpmx_stream_sync": [P],
     "mpmx_fill_i32": [P, L, I, P],
     "mpmx_fill_f32": [P, L, F, P],
@@ -47,21 +55,47 @@ SIGNATURES = {
     "mpmx_count_particles_by_type": [I, P, P, P, P, S, P],
     "mpmx_hashgrid_workspace_bytes": [I, I, I, I],
     "mpmx_hashgrid_build": [I, P, F, I, I, I, P, P, P, S, P],
+    "mpmx_hashgrid_build_label": [I, P, F, I, I, I, P, I, P, P, P, S, P],
     "mpmx_reset_mpm_contact_labels": [I, P, P, P],
     "mpmx_integrate_particles_xpbd": [I, P, P, P, P, F, F, F, F, P, P, F, P],
     "mpmx_solve_particle_bound_contacts": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
     "mpmx_solve_particle_particle_contacts": [I, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F, F, I, P, P],
+    "mpmx_solve_particle_particle_contacts_split": [I, P, P, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F,
+                                                    F, I, P, P],
     "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],
     "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
     "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],
     "mpmx_swell_particles_type2": [I, P, P, P, P, P, F, F, P, P, P],
 }
 _RESTYPE = {
+    "mpmx_substep_workspace_bytes": S,
     "mpmx_last_error": C.c_char_p,
     "mpmx_p2g_workspace_bytes": S,
     "mpmx_reduce_workspace_bytes": S,
     "mpmx_hashgrid_workspace_bytes": S,
 }
+
+
+class Particles(C.Structure):
+    """mpmx_particles (include/mpmx.h)"""
+    _fields_ = [("n", C.c_int)] + [(name, C.c_void_p) for name in (
+        "activeLabel", "materialLabel", "x", "v", "x_init_xpbd", "v_init_xpbd", "F", "F_trial", "C", "stress", "cov",
+        "mu", "lam", "ys", "alpha", "hardening", "softening", "density", "strainCriteria", "eta_shear", "eta_bulk",
+        "acc_strain", "damage", "vol", "mass")]
+
+
+class Grid(C.Structure):
+    """mpmx_grid (include/mpmx.h)"""
+    _fields_ = [("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int), ("m", C.c_void_p), ("v_in", C.c_void_p),
+                ("v_out", C.c_void_p)]
+
+
+class StepParams(C.Structure):
+    """mpmx_step_params (include/mpmx.h)"""
+    _fields_ = [(name, C.c_float) for name in (
+        "dt", "dx", "inv_dx", "minx", "miny", "minz", "gravity_x", "gravity_y", "gravity_z", "rpic_damping", "eff",
+        "damping_scale", "friction", "gravity_magnitude")] + [(name, C.c_int) for name in (
+        "bc", "boundary_padding", "update_cov", "constitutive_model")]
 
 
 class MpmxError(RuntimeError):

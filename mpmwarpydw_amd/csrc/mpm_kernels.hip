@@ -3,14 +3,11 @@
 //   Kirchhoff stress + Kelvin-Voigt), grid normalisation / damping /
 //   collideBounds (separate and fused), g2p.
 // The deterministic P2G lives in p2g.hip, the reductions in reduce.hip.
+#include "mpmx_constitutive.hpp"
 #include "mpmx_device.hpp"
 #include "mpmx_internal.hpp"
 
 namespace mpmx {
-
-// Warp's scalar min/max builtins are a<b?a:b and a>b?a:b (NaN falls to b).
-__device__ __forceinline__ float wmin(float a, float b) { return a < b ? a : b; }
-__device__ __forceinline__ float wmax(float a, float b) { return a > b ? a : b; }
 
 // --------------------------------------------------------------------------
 // compute_mu_lam_bulk_from_E_nu (mpmRoutines.py:206-226)
@@ -28,152 +25,14 @@ __global__ void mu_lam_bulk_kernel(int n, const float* __restrict__ E, const flo
 }
 
 // --------------------------------------------------------------------------
-// compute_stress_from_F_trial (mpmRoutines.py:229-312) with
-// drucker_prager_return_mapping (:314-453) and kirchoff_stress (:456-471).
-//
-// One SVD per particle: the reference's second svd3 of the returned F
-// (mpmRoutines.py:288) is the first one again in the elastic branch and
-// U diag(exp eps) V^T in the plastic branch, and U diag(c) V^T F^T with
-// F = U diag(s) V^T equals U diag(c_k s_k) U^T, so the Kirchhoff stress is
-// assembled directly as U diag(d) U^T (symmetric, as the reference's
-// (tau + tau^T)/2 makes it).
+// compute_stress_from_F_trial (mpmRoutines.py:229-312), one thread per particle
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) stress_kernel(
-    int n, const int32_t* __restrict__ activeLabel, int32_t* __restrict__ materialLabel,
-    const float* __restrict__ x, float* __restrict__ v, float* __restrict__ x_init, float* __restrict__ v_init,
-    float* __restrict__ F, const float* __restrict__ F_trial, float* __restrict__ mu_a, float* __restrict__ lam_a,
-    float* __restrict__ ys_a, const float* __restrict__ hard_a, const float* __restrict__ soft_a,
-    const float* __restrict__ density, const float* __restrict__ sc_a, float eff,
-    const float* __restrict__ eta_s_a, const float* __restrict__ eta_b_a, const float* __restrict__ C_a,
-    float* __restrict__ stress, float* __restrict__ acc_a, float* __restrict__ damage_a, float* __restrict__ alpha_a) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int label = materialLabel[p];
-    if (label > 1 || activeLabel[p] != 1) return;
-
-    float Ft[9];
-    ld9(F_trial, n, p, Ft);
-    Svd3 sv;
-    svd3(Ft, sv);
-
-    float mu = mu_a[p], lam = lam_a[p], ys = ys_a[p], al = alpha_a[p], D = damage_a[p];
-    float eps[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
-    float esum = eps[0] + eps[1] + eps[2];
-    float mean_eps = esum / 3.0f;
-    float edev[3] = {eps[0] - mean_eps, eps[1] - mean_eps, eps[2] - mean_eps};
-    float tau[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) tau[k] = 2.0f * mu * eps[k] + lam * esum;
-    float p_stress = (tau[0] + tau[1] + tau[2]) / 3.0f;
-    float td0 = tau[0] - p_stress, td1 = tau[1] - p_stress, td2 = tau[2] - p_stress;
-    float sigma_eq = sqrtf(td0 * td0 + td1 * td1 + td2 * td2);
-    float yield_eff = (1.0f - D) * (ys - al * p_stress);
-
-    int branch = 0;
-    float epsn[3] = {0.0f, 0.0f, 0.0f};
-    if (sigma_eq > yield_eff || ys < 0.0f || yield_eff < 0.0f) {
-        float edn = sqrtf(edev[0] * edev[0] + edev[1] * edev[1] + edev[2] * edev[2]) + 1.0e-6f;
-        float dgam = edn - yield_eff / (2.0f * mu);
-        float dep = sqrtf(2.0f / 3.0f) * dgam;
-        acc_a[p] = acc_a[p] + dep;
-        float sc = sc_a[p];
-        float Dn = (sc > 0.0f) ? wmin(1.0f, D + dep / sc) : 1.0f;
-        if (ys < 0.0f || yield_eff < 0.0f) Dn = 1.0f;
-        damage_a[p] = Dn;
-        if (Dn >= 1.0f && label == 1) {
-            // phase transition to XPBD (mpmRoutines.py:379-408)
-            branch = 2;
-            materialLabel[p] = 2;
-            f3 xp = ld3(x, n, p);
-            st3(x_init, n, p, xp);
-            float rho = density[p];
-            float u = 0.5f * (tau[0] * eps[0] + tau[1] * eps[1] + tau[2] * eps[2]);
-            float v_expected = sqrtf(2.0f * eff * u / rho);
-            f3 vp = ld3(v, n, p);
-            f3 vdir = normalize3(f3{vp.x + 1.0e-12f, vp.y + 1.0e-12f, vp.z + 1.0e-12f});
-            f3 vn = f3{vp.x + v_expected * vdir.x, vp.y + v_expected * vdir.y, vp.z + v_expected * vdir.z};
-            st3(v, n, p, vn);
-            st3(v_init, n, p, vn);
-            ys_a[p] = 0.0f;
-            mu_a[p] = 0.0f;
-            lam_a[p] = 0.0f;
-            alpha_a[p] = 0.0f;
-            mu = 0.0f;
-            lam = 0.0f;
-        } else {
-            branch = 1;
-            float r = dgam / edn;
-            float volc = (0.3f * al) * dgam;
-#pragma unroll
-            for (int k = 0; k < 3; k++) epsn[k] = (eps[k] - r * edev[k]) + volc;
-            ys_a[p] = wmax(0.0f, ys + 2.0f * mu * (hard_a[p] - soft_a[p]) * dgam);
-        }
-    }
-
-    // ---- deformation gradient and principal Kirchhoff stresses ----
-    float d[3];
-    float Fo[9];
-    if (branch == 1) {
-        float e[3], l[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            e[k] = expf(epsn[k]);
-            l[k] = (e[k] >= 1.0e-6f) ? epsn[k] : logf(1.0e-6f);
-        }
-        float ls = l[0] + l[1] + l[2];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float dk = 2.0f * mu * l[k] + lam * ls;
-            d[k] = (e[k] >= 1.0e-6f) ? dk : dk * (e[k] / 1.0e-6f);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                Fo[i * 3 + j] = sv.U[i * 3 + 0] * e[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * e[1] * sv.V[j * 3 + 1] +
-                                sv.U[i * 3 + 2] * e[2] * sv.V[j * 3 + 2];
-    } else {
-        float l[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) l[k] = log_clamped(sv.s[k], sv.lam[k], 1.0e-6f);
-        float ls = l[0] + l[1] + l[2];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float dk = 2.0f * mu * l[k] + lam * ls;
-            d[k] = (sv.s[k] >= 1.0e-6f) ? dk : dk * (sv.s[k] / 1.0e-6f);
-        }
-#pragma unroll
-        for (int k = 0; k < 9; k++) Fo[k] = Ft[k];
-    }
-    st9(F, n, p, Fo);
-
-    // tau = U diag(d) U^T, plus Kelvin-Voigt with the APIC C as grad v (mpmRoutines.py:294-310)
-    float G[9];
-    ld9(C_a, n, p, G);
-    float s00 = G[0], s11 = G[4], s22 = G[8];
-    float s01 = 0.5f * (G[1] + G[3]), s02 = 0.5f * (G[2] + G[6]), s12 = 0.5f * (G[5] + G[7]);
-    float vr = s00 + s11 + s22;
-    float es = 2.0f * eta_s_a[p], eb = eta_b_a[p];
-    float third = vr / 3.0f;
+__global__ void __launch_bounds__(kBlock) stress_kernel(mpmx_particles P, float eff) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    if (P.materialLabel[p] > 1 || P.activeLabel[p] != 1) return;
     float T[9];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = i; j < 3; j++) {
-            float t = sv.U[i * 3 + 0] * d[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * d[1] * sv.U[j * 3 + 1] +
-                      sv.U[i * 3 + 2] * d[2] * sv.U[j * 3 + 2];
-            T[i * 3 + j] = t;
-            T[j * 3 + i] = t;
-        }
-    T[0] += es * (s00 - third) + eb * vr;
-    T[4] += es * (s11 - third) + eb * vr;
-    T[8] += es * (s22 - third) + eb * vr;
-    T[1] += es * s01; T[3] += es * s01;
-    T[2] += es * s02; T[6] += es * s02;
-    T[5] += es * s12; T[7] += es * s12;
-    st9(stress, n, p, T);
+    constitutive_update(P, p, eff, T);
 }
 
 // --------------------------------------------------------------------------
@@ -344,9 +203,31 @@ int mpmx_compute_stress_from_F_trial(int n, const int32_t* activeLabel, int32_t*
                      hardening && softening && density && strainCriteria && eta_shear && eta_bulk && C && stress &&
                      acc_strain && damage && alpha,
                  "mpmx_compute_stress_from_F_trial: null pointer");
-    hipLaunchKernelGGL(stress_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, activeLabel,
-                       materialLabel, x, v, x_init, v_init, F, F_trial, mu, lam, ys, hardening, softening, density,
-                       strainCriteria, eff, eta_shear, eta_bulk, C, stress, acc_strain, damage, alpha);
+    mpmx_particles P{};
+    P.n = n;
+    P.activeLabel = const_cast<int32_t*>(activeLabel);
+    P.materialLabel = materialLabel;
+    P.x = const_cast<float*>(x);
+    P.v = v;
+    P.x_init_xpbd = x_init;
+    P.v_init_xpbd = v_init;
+    P.F = F;
+    P.F_trial = const_cast<float*>(F_trial);
+    P.C = const_cast<float*>(C);
+    P.stress = stress;
+    P.mu = mu;
+    P.lam = lam;
+    P.ys = ys;
+    P.alpha = alpha;
+    P.hardening = const_cast<float*>(hardening);
+    P.softening = const_cast<float*>(softening);
+    P.density = const_cast<float*>(density);
+    P.strainCriteria = const_cast<float*>(strainCriteria);
+    P.eta_shear = const_cast<float*>(eta_shear);
+    P.eta_bulk = const_cast<float*>(eta_bulk);
+    P.acc_strain = acc_strain;
+    P.damage = damage;
+    hipLaunchKernelGGL(stress_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), P, eff);
     return check_launch("compute_stress_from_F_trial", as_stream(stream));
 }
 

@@ -54,6 +54,8 @@ struct BinWork {
 size_t bin_workspace_bytes(long n, long nkeys);
 bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w);
 int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s);
+// scan + scatter + in-run sort only (w.count and w.rank already filled)
+int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s);
 
 }  // namespace mpmx
 

@@ -4,6 +4,11 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
 
 #include "mpmx_device.hpp"
 #include "mpmx_internal.hpp"
@@ -28,12 +33,30 @@ static int debug_sync_enabled() {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// launch profiler: when on, an event is recorded on the launch stream after
+// every kernel; a kernel's time is the gap to the previous event on the stream
+// ---------------------------------------------------------------------------
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<const char*> names;  // nullptr = a mark with no kernel before it
+    size_t used = 0;
+};
+static Profiler g_prof;
+
+static void prof_record(const char* what, hipStream_t s) {
+    if (!g_prof.on || g_prof.used >= g_prof.pool.size()) return;
+    if (hipEventRecord(g_prof.pool[g_prof.used], s) == hipSuccess) g_prof.names[g_prof.used++] = what;
+}
+
 int check_launch(const char* what, hipStream_t s) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_error("launch of %s failed: %s", what, hipGetErrorString(e));
         return (int)e;
     }
+    prof_record(what, s);
     if (debug_sync_enabled()) {
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
@@ -229,6 +252,21 @@ int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, 
     return check_launch("bin_sort_runs", s);
 }
 
+// The scan / scatter / in-run sort half of counting_sort_ids, for callers that
+// produced keys, ranks and counts themselves.
+int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s) {
+    int rc = exclusive_scan_i32(w.count, w.cell_start, nkeys, w.partials, s);
+    if (rc) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
+                           w.rank, sorted);
+        if ((rc = check_launch("bin_scatter", s))) return rc;
+    }
+    hipLaunchKernelGGL(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
+                       sorted);
+    return check_launch("bin_sort_runs", s);
+}
+
 }  // namespace mpmx
 
 using namespace mpmx;
@@ -236,6 +274,55 @@ using namespace mpmx;
 extern "C" {
 
 const char* mpmx_last_error(void) { return g_err; }
+
+int mpmx_profile_begin(int capacity) {
+    MPMX_REQUIRE(capacity > 0 && capacity <= (1 << 22), "mpmx_profile_begin: capacity out of range");
+    if ((int)g_prof.pool.size() < capacity) {
+        size_t have = g_prof.pool.size();
+        g_prof.pool.resize(capacity);
+        for (size_t i = have; i < (size_t)capacity; i++) MPMX_HIP(hipEventCreate(&g_prof.pool[i]));
+        g_prof.names.resize(capacity);
+    }
+    g_prof.used = 0;
+    g_prof.on = true;
+    return 0;
+}
+
+int mpmx_profile_mark(void* stream) {
+    if (g_prof.on && g_prof.used < g_prof.pool.size()) {
+        MPMX_HIP(hipEventRecord(g_prof.pool[g_prof.used], as_stream(stream)));
+        g_prof.names[g_prof.used++] = nullptr;
+    }
+    return 0;
+}
+
+int mpmx_profile_end(void) {
+    g_prof.on = false;
+    return 0;
+}
+
+int mpmx_profile_report(char* buf, size_t buflen) {
+    MPMX_REQUIRE(buf && buflen > 0, "mpmx_profile_report: bad buffer");
+    std::map<std::string, std::pair<long, double>> acc;
+    for (size_t i = 1; i < g_prof.used; i++) {
+        if (!g_prof.names[i]) continue;
+        MPMX_HIP(hipEventSynchronize(g_prof.pool[i]));
+        float ms = 0.0f;
+        MPMX_HIP(hipEventElapsedTime(&ms, g_prof.pool[i - 1], g_prof.pool[i]));
+        auto& e = acc[g_prof.names[i]];
+        e.first += 1;
+        e.second += ms;
+    }
+    size_t off = 0;
+    buf[0] = 0;
+    for (auto& kv : acc) {
+        int w = snprintf(buf + off, buflen - off, "%s %ld %.6f\n", kv.first.c_str(), kv.second.first,
+                         kv.second.second);
+        if (w < 0 || (size_t)w >= buflen - off) break;
+        off += (size_t)w;
+    }
+    return 0;
+}
 int mpmx_version(void) { return 1; }
 
 int mpmx_stream_sync(void* stream) {

@@ -1,0 +1,338 @@
+// Deterministic tile reduction for P2G (p2g_apic_with_stress,
+// mpmRoutines.py:485-569), shared by the standalone P2G entry points and the
+// fused substep.
+//
+// The reference scatters 27 x 4 float atomics per particle into the grid, so its
+// sums depend on thread timing.  Here:
+//   1. bin:   every active MPM particle gets the key of its base cell,
+//             key = tile * 64 + local cell, tiles being 4x4x4 base cells;
+//             a counting sort (ids ascending inside a cell) groups them.
+//   2. tile:  one 256-thread workgroup per tile stages a 25-float record per
+//             particle in LDS (fractional offset, mass, affine momentum terms,
+//             stress impulse), then each (cell, i, j) task sums the cell's
+//             particles in sorted order into its 3 nodes (k = 0..2) and adds
+//             the result to an LDS array indexed by (cell, i, j, k).  Every
+//             (cell, offset) slot has one owner, so no atomics are needed.
+//             Finally each of the 6x6x6 tile nodes sums its <= 27 slots in a
+//             fixed offset order and the tile writes its partial node block.
+//   3. combine: each grid node adds the partial blocks of the <= 8 tiles that
+//             cover it in a fixed tile order, and optionally applies the grid
+//             update (normalisation, gravity, damping, boundary condition).
+// The result does not depend on scheduling: identical inputs give identical bits.
+//
+// In the fused substep the record stage first runs the constitutive update
+// (return map + Kirchhoff stress) of the particle, so the stress goes from
+// registers into the record; particles whose stencil leaves the grid are
+// collected in one extra "orphan" bin after the last tile and get the
+// constitutive update only.
+#pragma once
+#include "mpmx_constitutive.hpp"
+#include "mpmx_device.hpp"
+
+namespace mpmx {
+
+constexpr int kTB = 4;                       // base cells per tile edge
+constexpr int kTN = kTB + 2;                 // nodes per tile edge
+constexpr int kTileCells = kTB * kTB * kTB;  // 64
+constexpr int kTileNodes = kTN * kTN * kTN;  // 216
+constexpr int kRec = 25;                     // floats per staged particle record
+constexpr int kChunk = 256;                  // particles staged per LDS round
+constexpr int kTasks = kTileCells * 9;       // (cell, i, j)
+
+struct TileGeom {
+    int gx, gy, gz;     // grid nodes
+    int ntx, nty, ntz;  // tiles
+    __host__ __device__ long ntiles() const { return (long)ntx * nty * ntz; }
+};
+
+inline TileGeom make_geom(int gx, int gy, int gz) {
+    TileGeom g;
+    g.gx = gx;
+    g.gy = gy;
+    g.gz = gz;
+    g.ntx = (gx - 2 + kTB - 1) / kTB;
+    g.nty = (gy - 2 + kTB - 1) / kTB;
+    g.ntz = (gz - 2 + kTB - 1) / kTB;
+    return g;
+}
+
+// Key of the base cell of a particle at x, or -1 when its 3x3x3 stencil leaves
+// the grid.  Compares against g - 3 (not b + 2 < g) so that a saturated huge
+// index cannot wrap around.
+__device__ __forceinline__ int tile_key(f3 x, float minx, float miny, float minz, float inv_dx, const TileGeom& tg) {
+    float f;
+    int bx = base_axis(x.x, minx, inv_dx, f);
+    int by = base_axis(x.y, miny, inv_dx, f);
+    int bz = base_axis(x.z, minz, inv_dx, f);
+    if (bx < 0 || by < 0 || bz < 0 || bx > tg.gx - 3 || by > tg.gy - 3 || bz > tg.gz - 3) return -1;
+    int tx = bx / kTB, ty = by / kTB, tz = bz / kTB;
+    int lx = bx - tx * kTB, ly = by - ty * kTB, lz = bz - tz * kTB;
+    return ((tx * tg.nty + ty) * tg.ntz + tz) * kTileCells + (lx * kTB + ly) * kTB + lz;
+}
+
+// B-spline weight and derivative for a stencil offset i known only at run time
+__device__ __forceinline__ void wsel(float f, int i, float& w, float& dw) {
+    if (i == 0) {
+        float a = 1.5f - f;
+        w = a * a * 0.5f;
+        dw = f - 1.5f;
+    } else if (i == 1) {
+        float b = f - 1.0f;
+        w = (0.0f - b * b) + 0.75f;
+        dw = -2.0f * b;
+    } else {
+        float c = f - 0.5f;
+        w = c * c * 0.5f;
+        dw = c;
+    }
+}
+
+struct TileArgs {
+    mpmx_particles P;             // particle state (read; written by the constitutive update when fused)
+    const float* stress;          // stress to scatter when not fused
+    const int32_t* sorted;        // particle ids in key order
+    const int32_t* cell_start;    // [nkeys + 1]
+    float dx, inv_dx, minx, miny, minz, rpic, dt, eff;
+    float* partial;               // [ntiles][4][216]
+    int32_t* tile_flag;           // [ntiles]
+    int ntiles;
+    TileGeom tg;
+    int32_t* misbinned;           // optional: counts particles whose base cell is not their bin's cell
+};
+
+// Constitutive update of the particles in the orphan bin (fused substep only).
+__device__ __forceinline__ void orphan_bin(const TileArgs& a) {
+    const long b = a.cell_start[(long)a.ntiles * kTileCells];
+    const long e = a.cell_start[(long)a.ntiles * kTileCells + 1];
+    for (long s = b + threadIdx.x; s < e; s += kBlock) {
+        float T[9];
+        constitutive_update(a.P, a.sorted[s], a.eff, T);
+    }
+}
+
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
+    __shared__ float rec[kRec][kChunk];
+    __shared__ float4 part[kTileCells * 27];
+    __shared__ int cs[kTileCells + 1];
+
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (FUSED && t == a.ntiles) {
+        orphan_bin(a);
+        return;
+    }
+    const long n = a.P.n;
+    if (tid <= kTileCells) cs[tid] = a.cell_start[(long)t * kTileCells + tid];
+    __syncthreads();
+    const int p0 = cs[0];
+    const int np = cs[kTileCells] - p0;
+    if (np == 0) {
+        if (tid == 0) a.tile_flag[t] = 0;
+        return;
+    }
+    for (int s = tid; s < kTileCells * 27; s += kBlock) part[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    const float sfac = -a.dt * a.inv_dx;
+    for (int c0 = 0; c0 < np; c0 += kChunk) {
+        const int cn = min(kChunk, np - c0);
+        __syncthreads();  // the previous round's readers are done with rec
+        for (int s = tid; s < cn; s += kBlock) {
+            const int id = a.sorted[p0 + c0 + s];
+            float T[9];
+            bool mpm = true;
+            if (FUSED) {
+                mpm = constitutive_update(a.P, id, a.eff, T);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 9; k++) T[k] = a.stress[(long)k * n + id];
+            }
+            if (!mpm) {
+                // became XPBD in this substep: it no longer transfers to the grid
+                // (p2g tests the label after the stress pass, mpmRoutines.py:507)
+#pragma unroll
+                for (int k = 0; k < kRec; k++) rec[k][s] = 0.0f;
+                rec[0][s] = 1.0f;
+                rec[1][s] = 1.0f;
+                rec[2][s] = 1.0f;
+                continue;
+            }
+            f3 xp = ld3(a.P.x, n, id);
+            f3 vp = ld3(a.P.v, n, id);
+            if (a.misbinned) {
+                // bins come from the previous substep's G2P: check that the particle is still in its cell
+                int c = -1;
+                for (int q = 0; q < kTileCells; q++)
+                    if (p0 + c0 + s >= cs[q] && p0 + c0 + s < cs[q + 1]) c = q;
+                if (tile_key(xp, a.minx, a.miny, a.minz, a.inv_dx, a.tg) != t * kTileCells + c) atomicAdd(a.misbinned, 1);
+            }
+            float f0, f1, f2;
+            base_axis(xp.x, a.minx, a.inv_dx, f0);
+            base_axis(xp.y, a.miny, a.inv_dx, f1);
+            base_axis(xp.z, a.minz, a.inv_dx, f2);
+            float Cp[9], Cm[9];
+            ld9(a.P.C, n, id, Cp);
+            const float rpic = a.rpic;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    Cm[i * 3 + j] = (1.0f - rpic) * Cp[i * 3 + j] + rpic / 2.0f * (Cp[i * 3 + j] - Cp[j * 3 + i]);
+            if (rpic < -0.001f) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) Cm[k] = 0.0f;
+            }
+            const float m = a.P.mass[id];
+            const float md = m * a.dx;
+            const float sv = sfac * a.P.vol[id];
+            rec[0][s] = f0;
+            rec[1][s] = f1;
+            rec[2][s] = f2;
+            rec[3][s] = m;
+            rec[4][s] = m * vp.x - md * (Cm[0] * f0 + Cm[1] * f1 + Cm[2] * f2);
+            rec[5][s] = m * vp.y - md * (Cm[3] * f0 + Cm[4] * f1 + Cm[5] * f2);
+            rec[6][s] = m * vp.z - md * (Cm[6] * f0 + Cm[7] * f1 + Cm[8] * f2);
+#pragma unroll
+            for (int k = 0; k < 9; k++) rec[7 + k][s] = md * Cm[k];
+#pragma unroll
+            for (int k = 0; k < 9; k++) rec[16 + k][s] = sv * T[k];
+        }
+        __syncthreads();
+        for (int task = tid; task < kTasks; task += kBlock) {
+            const int c = task / 9;
+            const int i = (task % 9) / 3;
+            const int j = task % 3;
+            const int lo = max(cs[c] - p0, c0) - c0;
+            const int hi = min(cs[c + 1] - p0, c0 + cn) - c0;
+            if (lo >= hi) continue;
+            float acc[3][4];
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
+            const float fi = (float)i, fj = (float)j;
+            for (int s = lo; s < hi; s++) {
+                float wx, dwx, wy, dwy;
+                wsel(rec[0][s], i, wx, dwx);
+                wsel(rec[1][s], j, wy, dwy);
+                float wz[3], dwz[3];
+                bspline(rec[2][s], wz, dwz);
+                const float m = rec[3][s];
+                const float wxy = wx * wy, gxw = dwx * wy, gyw = wx * dwy;
+                // affine momentum at k = 0: A + Q (i, j, 0)
+                float bx = rec[4][s] + rec[7][s] * fi + rec[8][s] * fj;
+                float by = rec[5][s] + rec[10][s] * fi + rec[11][s] * fj;
+                float bz = rec[6][s] + rec[13][s] * fi + rec[14][s] * fj;
+                const float qx = rec[9][s], qy = rec[12][s], qz = rec[15][s];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float w = wxy * wz[k];
+                    const float g0 = gxw * wz[k], g1 = gyw * wz[k], g2 = wxy * dwz[k];
+                    const float ax = bx + qx * (float)k, ay = by + qy * (float)k, az = bz + qz * (float)k;
+                    acc[k][0] += w * m;
+                    acc[k][1] += w * ax + (rec[16][s] * g0 + rec[17][s] * g1 + rec[18][s] * g2);
+                    acc[k][2] += w * ay + (rec[19][s] * g0 + rec[20][s] * g1 + rec[21][s] * g2);
+                    acc[k][3] += w * az + (rec[22][s] * g0 + rec[23][s] * g1 + rec[24][s] * g2);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float4& dst = part[c * 27 + i * 9 + j * 3 + k];
+                dst.x += acc[k][0];
+                dst.y += acc[k][1];
+                dst.z += acc[k][2];
+                dst.w += acc[k][3];
+            }
+        }
+    }
+    __syncthreads();
+    float* out = a.partial + (long)t * kTileNodes * 4;
+    for (int node = tid; node < kTileNodes; node += kBlock) {
+        const int nx = node / (kTN * kTN), ny = (node / kTN) % kTN, nz = node % kTN;
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int cx = nx - i;
+            if (cx < 0 || cx >= kTB) continue;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int cy = ny - j;
+                if (cy < 0 || cy >= kTB) continue;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int cz = nz - k;
+                    if (cz < 0 || cz >= kTB) continue;
+                    const float4 q = part[((cx * kTB + cy) * kTB + cz) * 27 + i * 9 + j * 3 + k];
+                    s.x += q.x;
+                    s.y += q.y;
+                    s.z += q.z;
+                    s.w += q.w;
+                }
+            }
+        }
+        out[node] = s.x;
+        out[kTileNodes + node] = s.y;
+        out[2 * kTileNodes + node] = s.z;
+        out[3 * kTileNodes + node] = s.w;
+    }
+    if (tid == 0) a.tile_flag[t] = 1;
+}
+
+// ---- combine ----------------------------------------------------------------
+struct CombineArgs {
+    TileGeom tg;
+    const float* partial;
+    const int32_t* tile_flag;
+    float *grid_m, *grid_v_in, *grid_v_out;
+    int do_update;
+    float gxv, gyv, gzv, dt, damping, fric, gmag;
+    int bc, pad;
+    int32_t* zero_ints;  // optional: the next substep's bin counts are cleared here
+    long n_zero;
+};
+
+// a template so that every translation unit including this header can instantiate it
+template <int UNUSED = 0>
+__global__ void p2g_combine_kernel(CombineArgs a) {
+    const TileGeom& tg = a.tg;
+    const long G = (long)tg.gx * tg.gy * tg.gz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.zero_ints) {
+        for (long z = i; z < a.n_zero; z += (long)gridDim.x * blockDim.x) a.zero_ints[z] = 0;
+    }
+    if (i >= G) return;
+    const int iz = (int)(i % tg.gz);
+    const int iy = (int)((i / tg.gz) % tg.gy);
+    const int ix = (int)(i / ((long)tg.gy * tg.gz));
+    float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    const int tx1 = ix / kTB, ty1 = iy / kTB, tz1 = iz / kTB;
+    for (int tx = tx1 - 1; tx <= tx1; tx++) {
+        const int lx = ix - tx * kTB;
+        if (tx < 0 || tx >= tg.ntx || lx >= kTN) continue;
+        for (int ty = ty1 - 1; ty <= ty1; ty++) {
+            const int ly = iy - ty * kTB;
+            if (ty < 0 || ty >= tg.nty || ly >= kTN) continue;
+            for (int tz = tz1 - 1; tz <= tz1; tz++) {
+                const int lz = iz - tz * kTB;
+                if (tz < 0 || tz >= tg.ntz || lz >= kTN) continue;
+                const long t = ((long)tx * tg.nty + ty) * tg.ntz + tz;
+                if (!a.tile_flag[t]) continue;
+                const float* pb = a.partial + t * kTileNodes * 4;
+                const int node = (lx * kTN + ly) * kTN + lz;
+                m += pb[node];
+                mx += pb[kTileNodes + node];
+                my += pb[2 * kTileNodes + node];
+                mz += pb[3 * kTileNodes + node];
+            }
+        }
+    }
+    a.grid_m[i] = m;
+    a.grid_v_in[i] = mx;
+    a.grid_v_in[G + i] = my;
+    a.grid_v_in[2 * G + i] = mz;
+    if (a.do_update) {
+        f3 vo = grid_update_node(m, f3{mx, my, mz}, ix, iy, iz, tg.gx, tg.gy, tg.gz, a.gxv, a.gyv, a.gzv, a.dt,
+                                 a.damping, a.bc, a.fric, a.gmag, a.pad);
+        st3(a.grid_v_out, G, i, vo);
+    }
+}
+
+}  // namespace mpmx

@@ -1,0 +1,364 @@
+// The fused MPM substep (mpmSimulationStep, simulationRoutines.py:6-250) for
+// gfx950: three passes over HBM instead of the reference's launch chain.
+//
+//   K1 tile pass   per 4x4x4-cell tile: the constitutive update of every binned
+//                  particle (return map + Kirchhoff stress, written back) feeds
+//                  the P2G record straight from registers; the tile is reduced
+//                  in LDS (p2g_tile.hpp).  Out-of-grid particles sit in an
+//                  extra orphan bin and get the constitutive update only.
+//   K2 node pass   tile partials -> grid_m, grid_v_in, and the grid update
+//                  (normalisation, gravity, damping, walls) -> grid_v_out; it
+//                  also clears the bin counts of the next substep.
+//   K3 tile pass   G2P per tile with the tile's 6x6x6 grid velocities staged in
+//                  LDS; each particle then computes its new base cell and
+//                  counts itself into the next substep's bins.
+//   bin            scan of the counts, scatter of ids, ascending ids per cell.
+//
+// Particles keep their bins between substeps: K3 bins them at their new
+// positions, and the XPBD step between substeps restores MPM positions
+// (apply_mpm_contact_impulses, xpbdRoutines.py:293) and turns no particle into
+// MPM.  After any other change of x, labels or activity, call
+// mpmx_substep_bin() again.
+#include "mpmx_internal.hpp"
+#include "p2g_tile.hpp"
+
+namespace mpmx {
+
+struct SubstepWork {
+    int32_t* misbinned;  // [1]
+    int32_t* keys;
+    int32_t* sorted;
+    float* partial;
+    int32_t* tile_flag;
+    BinWork bin;
+};
+
+static long substep_nkeys(const TileGeom& tg) { return tg.ntiles() * kTileCells + 1; }  // + orphan bin
+
+static size_t substep_ws_bytes(long n, const TileGeom& tg) {
+    long nn = n > 0 ? n : 1;
+    return Carver::bytes_for(1, 4) + Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(tg.ntiles() * kTileNodes * 4, 4) +
+           Carver::bytes_for(tg.ntiles(), 4) + bin_workspace_bytes(nn, substep_nkeys(tg));
+}
+
+static bool carve_substep(void* ws, size_t bytes, long n, const TileGeom& tg, SubstepWork& w) {
+    Carver cv(ws, bytes);
+    long nn = n > 0 ? n : 1;
+    w.misbinned = cv.take<int32_t>(1);
+    w.keys = cv.take<int32_t>(nn);
+    w.sorted = cv.take<int32_t>(nn);
+    w.partial = cv.take<float>(tg.ntiles() * kTileNodes * 4);
+    w.tile_flag = cv.take<int32_t>(tg.ntiles());
+    carve_bin(cv, nn, substep_nkeys(tg), w.bin);
+    return !cv.overflow;
+}
+
+__global__ void zero_one_kernel(int32_t* __restrict__ p) { *p = 0; }
+
+// key of an active MPM particle: its base-cell key, or the orphan bin
+__global__ void substep_key_kernel(mpmx_particles P, float minx, float miny, float minz, float inv_dx, TileGeom tg,
+                                   int32_t* __restrict__ keys) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    int key = -1;
+    if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
+        key = tile_key(ld3(P.x, P.n, p), minx, miny, minz, inv_dx, tg);
+        if (key < 0) key = (int)(tg.ntiles() * kTileCells);
+    }
+    keys[p] = key;
+}
+
+struct G2PArgs {
+    mpmx_particles P;
+    const int32_t* sorted;
+    const int32_t* cell_start;
+    const float* v_out;
+    TileGeom tg;
+    float dt, inv_dx, minx, miny, minz;
+    int update_cov;
+    int32_t* keys;   // next substep: key per particle
+    int32_t* count;  // next substep: particles per key (cleared by K2)
+    int32_t* rank;   // next substep: arrival rank inside the key
+    int32_t* misbinned;
+};
+
+__device__ __forceinline__ void bin_next(const G2PArgs& a, long id, int key) {
+    a.keys[id] = key;
+    if (key >= 0) a.rank[id] = atomicAdd(&a.count[key], 1);
+}
+
+// G2P (mpmRoutines.py:715-802) of the particles of one tile, grid velocities
+// from LDS; F_trial = (I + dt G) F is evaluated as F + dt (G F).
+__global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
+    __shared__ float gvs[3][kTileNodes];
+    const mpmx_particles& P = a.P;
+    const long n = P.n;
+    const int t = blockIdx.x;
+    const TileGeom& tg = a.tg;
+    const long ntiles = tg.ntiles();
+    const int orphan_key = (int)(ntiles * kTileCells);
+    if (t == ntiles) {
+        // out-of-grid particles: no transfer (the reference would read out of bounds); they stay orphans
+        const long b = a.cell_start[orphan_key], e = a.cell_start[orphan_key + 1];
+        for (long s = b + threadIdx.x; s < e; s += kBlock) {
+            const int id = a.sorted[s];
+            bin_next(a, id, P.materialLabel[id] <= 1 ? orphan_key : -1);
+        }
+        return;
+    }
+    const long p0 = a.cell_start[(long)t * kTileCells];
+    const long p1 = a.cell_start[(long)t * kTileCells + kTileCells];
+    if (p1 == p0) return;
+    const int tz = (int)(t % tg.ntz), ty = (int)((t / tg.ntz) % tg.nty), tx = (int)(t / ((long)tg.ntz * tg.nty));
+    const int ox = tx * kTB, oy = ty * kTB, oz = tz * kTB;
+    const long G = (long)tg.gx * tg.gy * tg.gz;
+    for (int node = threadIdx.x; node < kTileNodes; node += kBlock) {
+        const int nx = ox + node / (kTN * kTN), ny = oy + (node / kTN) % kTN, nz = oz + node % kTN;
+        float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+        if (nx < tg.gx && ny < tg.gy && nz < tg.gz) {
+            const long gi = ((long)nx * tg.gy + ny) * tg.gz + nz;
+            v0 = a.v_out[gi];
+            v1 = a.v_out[G + gi];
+            v2 = a.v_out[2 * G + gi];
+        }
+        gvs[0][node] = v0;
+        gvs[1][node] = v1;
+        gvs[2][node] = v2;
+    }
+    __syncthreads();
+    const float inv_dx = a.inv_dx, dt = a.dt;
+    for (long s = p0 + threadIdx.x; s < p1; s += kBlock) {
+        const int id = a.sorted[s];
+        if (P.materialLabel[id] > 1) {  // became XPBD in this substep's constitutive update
+            bin_next(a, id, -1);
+            continue;
+        }
+        f3 xp = ld3(P.x, n, id);
+        float fx[3];
+        const int b0 = base_axis(xp.x, a.minx, inv_dx, fx[0]) - ox;
+        const int b1 = base_axis(xp.y, a.miny, inv_dx, fx[1]) - oy;
+        const int b2 = base_axis(xp.z, a.minz, inv_dx, fx[2]) - oz;
+        float w[3][3], dw[3][3];
+        bspline(fx[0], w[0], dw[0]);
+        bspline(fx[1], w[1], dw[1]);
+        bspline(fx[2], w[2], dw[2]);
+        // a particle outside its tile's node block (stale bins) reads the grid from HBM instead
+        const bool in_tile = b0 >= 0 && b1 >= 0 && b2 >= 0 && b0 < kTB && b1 < kTB && b2 < kTB;
+        if (!in_tile) {
+            if (a.misbinned) atomicAdd(a.misbinned, 1);
+            if (b0 + ox < 0 || b1 + oy < 0 || b2 + oz < 0 || b0 + ox > tg.gx - 3 || b1 + oy > tg.gy - 3 ||
+                b2 + oz > tg.gz - 3) {
+                bin_next(a, id, orphan_key);
+                continue;
+            }
+        }
+        float nv[3] = {0.0f, 0.0f, 0.0f};
+        float nB[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float nG[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float wxy = w[0][i] * w[1][j];
+                const float gx_ = dw[0][i] * w[1][j] * inv_dx;
+                const float gy_ = w[0][i] * dw[1][j] * inv_dx;
+                const int row = ((b0 + i) * kTN + (b1 + j)) * kTN + b2;
+                const long grow = ((long)(b0 + ox + i) * tg.gy + (b1 + oy + j)) * tg.gz + (b2 + oz);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    float vx, vy, vz;
+                    if (in_tile) {
+                        vx = gvs[0][row + k];
+                        vy = gvs[1][row + k];
+                        vz = gvs[2][row + k];
+                    } else {
+                        vx = a.v_out[grow + k];
+                        vy = a.v_out[G + grow + k];
+                        vz = a.v_out[2 * G + grow + k];
+                    }
+                    const float wt = wxy * w[2][k];
+                    const float d0 = (float)i - fx[0], d1 = (float)j - fx[1], d2 = (float)k - fx[2];
+                    const float g0 = gx_ * w[2][k], g1 = gy_ * w[2][k], g2 = wxy * dw[2][k] * inv_dx;
+                    nv[0] += vx * wt;
+                    nv[1] += vy * wt;
+                    nv[2] += vz * wt;
+                    const float wvx = vx * wt, wvy = vy * wt, wvz = vz * wt;
+                    nB[0] += wvx * d0; nB[1] += wvx * d1; nB[2] += wvx * d2;
+                    nB[3] += wvy * d0; nB[4] += wvy * d1; nB[5] += wvy * d2;
+                    nB[6] += wvz * d0; nB[7] += wvz * d1; nB[8] += wvz * d2;
+                    nG[0] += vx * g0; nG[1] += vx * g1; nG[2] += vx * g2;
+                    nG[3] += vy * g0; nG[4] += vy * g1; nG[5] += vy * g2;
+                    nG[6] += vz * g0; nG[7] += vz * g1; nG[8] += vz * g2;
+                }
+            }
+        }
+        const f3 xn = f3{xp.x + dt * nv[0], xp.y + dt * nv[1], xp.z + dt * nv[2]};
+        st3(P.v, n, id, f3{nv[0], nv[1], nv[2]});
+        st3(P.x, n, id, xn);
+        const float sC = inv_dx * 4.0f;
+        float Cn[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) Cn[k] = nB[k] * sC;
+        st9(P.C, n, id, Cn);
+        float Fp[9], Fn[9];
+        ld9(P.F, n, id, Fp);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                Fn[i * 3 + j] = Fp[i * 3 + j] + dt * (nG[i * 3 + 0] * Fp[0 * 3 + j] + nG[i * 3 + 1] * Fp[1 * 3 + j] +
+                                                      nG[i * 3 + 2] * Fp[2 * 3 + j]);
+        st9(P.F_trial, n, id, Fn);
+        if (a.update_cov) {
+            float* c6 = P.cov + 6L * id;
+            float cn[9] = {c6[0], c6[1], c6[2], c6[1], c6[3], c6[4], c6[2], c6[4], c6[5]};
+            float c1[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    float gc = nG[i * 3 + 0] * cn[0 * 3 + j] + nG[i * 3 + 1] * cn[1 * 3 + j] + nG[i * 3 + 2] * cn[2 * 3 + j];
+                    float cg = cn[i * 3 + 0] * nG[j * 3 + 0] + cn[i * 3 + 1] * nG[j * 3 + 1] + cn[i * 3 + 2] * nG[j * 3 + 2];
+                    c1[i * 3 + j] = cn[i * 3 + j] + dt * (gc + cg);
+                }
+            c6[0] = c1[0]; c6[1] = c1[1]; c6[2] = c1[2]; c6[3] = c1[4]; c6[4] = c1[5]; c6[5] = c1[8];
+        }
+        int key = tile_key(xn, a.minx, a.miny, a.minz, inv_dx, tg);
+        bin_next(a, id, key < 0 ? orphan_key : key);
+    }
+}
+
+static int check_args(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm) {
+    MPMX_REQUIRE(P && G && prm, "substep: null argument");
+    MPMX_REQUIRE(P->n >= 0 && G->gx >= 3 && G->gy >= 3 && G->gz >= 3, "substep: bad sizes");
+    MPMX_REQUIRE(G->m && G->v_in && G->v_out, "substep: null grid array");
+    MPMX_REQUIRE(P->n == 0 || (P->activeLabel && P->materialLabel && P->x && P->v && P->x_init_xpbd &&
+                               P->v_init_xpbd && P->F && P->F_trial && P->C && P->stress && P->mu && P->lam &&
+                               P->ys && P->alpha && P->hardening && P->softening && P->density &&
+                               P->strainCriteria && P->eta_shear && P->eta_bulk && P->acc_strain && P->damage &&
+                               P->vol && P->mass && (!prm->update_cov || P->cov)),
+                 "substep: null particle array");
+    MPMX_REQUIRE(prm->bc == MPMX_BC_NONE || prm->bc == MPMX_BC_FRICTION, "substep: unsupported boundary condition");
+    return 0;
+}
+
+}  // namespace mpmx
+
+using namespace mpmx;
+
+extern "C" {
+
+size_t mpmx_substep_workspace_bytes(int n, int gx, int gy, int gz) {
+    if (n < 0 || gx < 3 || gy < 3 || gz < 3) return 0;
+    return substep_ws_bytes(n, make_geom(gx, gy, gz));
+}
+
+int mpmx_substep_bin(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm, void* workspace,
+                     size_t ws_bytes, void* stream) {
+    int rc = check_args(P, G, prm);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    TileGeom tg = make_geom(G->gx, G->gy, G->gz);
+    MPMX_REQUIRE(substep_nkeys(tg) < (1L << 31), "substep: grid too large for int32 keys");
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, tg), "substep: workspace too small");
+    SubstepWork w;
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "substep: workspace carve failed");
+    if (P->n > 0) {
+        hipLaunchKernelGGL(substep_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->minx,
+                           prm->miny, prm->minz, prm->inv_dx, tg, w.keys);
+        if ((rc = check_launch("substep_key", s))) return rc;
+    }
+    hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(1), 0, s, w.misbinned);
+    if ((rc = check_launch("substep_clear", s))) return rc;
+    return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), w.sorted, w.bin, s);
+}
+
+int mpmx_substep_misbinned(void* workspace, size_t ws_bytes, int gx, int gy, int gz, int n, int* host_count,
+                           void* stream) {
+    MPMX_REQUIRE(host_count && gx >= 3 && gy >= 3 && gz >= 3, "mpmx_substep_misbinned: bad arguments");
+    TileGeom tg = make_geom(gx, gy, gz);
+    SubstepWork w;
+    MPMX_REQUIRE(workspace && carve_substep(workspace, ws_bytes, n, tg, w), "mpmx_substep_misbinned: workspace");
+    MPMX_HIP(hipMemcpyAsync(host_count, w.misbinned, sizeof(int), hipMemcpyDeviceToHost, as_stream(stream)));
+    MPMX_HIP(hipStreamSynchronize(as_stream(stream)));
+    return 0;
+}
+
+int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm, void* workspace,
+                     size_t ws_bytes, void* stream) {
+    int rc = check_args(P, G, prm);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    TileGeom tg = make_geom(G->gx, G->gy, G->gz);
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, tg), "substep: workspace too small");
+    SubstepWork w;
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "substep: workspace carve failed");
+    const long nkeys = substep_nkeys(tg);
+    const unsigned ntile_blocks = (unsigned)tg.ntiles() + 1;  // + orphan bin
+
+    TileArgs ta{};
+    ta.P = *P;
+    ta.sorted = w.sorted;
+    ta.cell_start = w.bin.cell_start;
+    ta.dx = prm->dx;
+    ta.inv_dx = prm->inv_dx;
+    ta.minx = prm->minx;
+    ta.miny = prm->miny;
+    ta.minz = prm->minz;
+    ta.rpic = prm->rpic_damping;
+    ta.dt = prm->dt;
+    ta.eff = prm->eff;
+    ta.partial = w.partial;
+    ta.tile_flag = w.tile_flag;
+    ta.ntiles = (int)tg.ntiles();
+    ta.tg = tg;
+    ta.misbinned = w.misbinned;
+    hipLaunchKernelGGL(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
+    if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
+
+    const long Gn = (long)G->gx * G->gy * G->gz;
+    CombineArgs ca{};
+    ca.tg = tg;
+    ca.partial = w.partial;
+    ca.tile_flag = w.tile_flag;
+    ca.grid_m = G->m;
+    ca.grid_v_in = G->v_in;
+    ca.grid_v_out = G->v_out;
+    ca.do_update = 1;
+    ca.gxv = prm->gravity_x;
+    ca.gyv = prm->gravity_y;
+    ca.gzv = prm->gravity_z;
+    ca.dt = prm->dt;
+    ca.damping = prm->damping_scale;
+    ca.fric = prm->friction;
+    ca.gmag = prm->gravity_magnitude;
+    ca.bc = prm->bc;
+    ca.pad = prm->boundary_padding;
+    ca.zero_ints = w.bin.count;
+    ca.n_zero = nkeys;
+    hipLaunchKernelGGL(p2g_combine_kernel<>, dim3(grid_for(Gn > nkeys ? Gn : nkeys, kBlock)), dim3(kBlock), 0, s, ca);
+    if ((rc = check_launch("substep_combine(grid)", s))) return rc;
+
+    G2PArgs ga{};
+    ga.P = *P;
+    ga.sorted = w.sorted;
+    ga.cell_start = w.bin.cell_start;
+    ga.v_out = G->v_out;
+    ga.tg = tg;
+    ga.dt = prm->dt;
+    ga.inv_dx = prm->inv_dx;
+    ga.minx = prm->minx;
+    ga.miny = prm->miny;
+    ga.minz = prm->minz;
+    ga.update_cov = prm->update_cov;
+    ga.keys = w.keys;
+    ga.count = w.bin.count;
+    ga.rank = w.bin.rank;
+    ga.misbinned = w.misbinned;
+    hipLaunchKernelGGL(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
+    if ((rc = check_launch("substep_g2p", s))) return rc;
+
+    return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s);
+}
+
+}  // extern "C"

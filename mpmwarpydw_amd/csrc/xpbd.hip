@@ -23,10 +23,16 @@ __device__ __forceinline__ int hash_index(int dimx, int dimy, int dimz, int x, i
     return cz * (dimx * dimy) + cy * dimx + cx;
 }
 
+// key of every point, or only of the points whose label equals `label` when
+// `labels` is given (the others get -1 and are left out of the table)
 __global__ void hash_key_kernel(int n, const float* __restrict__ x, float inv, int dimx, int dimy, int dimz,
-                                int32_t* __restrict__ keys) {
+                                const int32_t* __restrict__ labels, int label, int32_t* __restrict__ keys) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    if (labels && labels[p] != label) {
+        keys[p] = -1;
+        return;
+    }
     keys[p] = hash_index(dimx, dimy, dimz, (int)(x[p] * inv), (int)(x[n + p] * inv), (int)(x[2L * n + p] * inv));
 }
 
@@ -104,10 +110,15 @@ __global__ void bound_contacts_kernel(int n, const int32_t* __restrict__ active,
     st3(delta, n, t, f3{da[0], da[1], da[2]});
 }
 
-// my_solve_particle_particle_contacts (xpbdRoutines.py:138-219)
+// my_solve_particle_particle_contacts (xpbdRoutines.py:138-219).
+// With xp_ids/xp_start (a table of the label-2 points only, same hashing and
+// stable order) an MPM particle i walks that table instead of the full one: it
+// skips every MPM candidate anyway (:185-186), and the XPBD candidates come in
+// the same relative order, so deltas and lock writes are unchanged.
 __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
-    const int32_t* __restrict__ cell_start, float inv, int dimx, int dimy, int dimz, const float* __restrict__ x,
+    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids, const int32_t* __restrict__ xp_start,
+    float inv, int dimx, int dimy, int dimz, const float* __restrict__ x,
     const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
     float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,6 +130,12 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     float ri = radius[i];
     if (mass[tid] == 0.0f) return;  // the reference tests tid here (xpbdRoutines.py:175)
     float w1 = 1.0f / mass[i];
+    const int32_t* ids = point_ids;
+    const int32_t* starts = cell_start;
+    if (xp_ids && mat[i] <= 1) {  // labels 0/1 stay MPM while this kernel runs (lock writes only 0)
+        ids = xp_ids;
+        starts = xp_start;
+    }
     float qr = ri + max_radius + k_coh * ri;
     int xs = (int)((xi.x - qr) * inv), ys = (int)((xi.y - qr) * inv), zs = (int)((xi.z - qr) * inv);
     int xe = min((int)((xi.x + qr) * inv), xs + dimx - 1);
@@ -131,9 +148,9 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
         for (int cy = ys; cy <= ye; cy++)
             for (int cx = xs; cx <= xe; cx++) {
                 int cell = hash_index(dimx, dimy, dimz, cx, cy, cz);
-                int b = cell_start[cell], e = cell_start[cell + 1];
+                int b = starts[cell], e = starts[cell + 1];
                 for (int s = b; s < e; s++) {
-                    int j = point_ids[s];
+                    int j = ids[s];
                     if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
                     if (mat[i] <= 1 && mat[j] <= 1) continue;
                     f3 xj = ld3(x, n, j);
@@ -269,8 +286,9 @@ size_t mpmx_hashgrid_workspace_bytes(int n, int dimx, int dimy, int dimz) {
     return Carver::bytes_for(nn, 4) + bin_workspace_bytes(nn, nkeys);
 }
 
-int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
-                        int32_t* cell_start, void* workspace, size_t ws_bytes, void* stream) {
+static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+                          const int32_t* labels, int label, int32_t* point_ids, int32_t* cell_start, void* workspace,
+                          size_t ws_bytes, void* stream) {
     MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f && cell_start,
                  "mpmx_hashgrid_build: bad arguments");
     MPMX_REQUIRE(n == 0 || (x && point_ids), "mpmx_hashgrid_build: null pointer");
@@ -287,11 +305,25 @@ int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int d
     w.cell_start = cell_start;  // the caller's table receives the cell ranges
     if (n > 0) {
         hipLaunchKernelGGL(hash_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, x, 1.0f / cell_width,
-                           dimx, dimy, dimz, keys);
+                           dimx, dimy, dimz, labels, label, keys);
         int rc = check_launch("hash_key", s);
         if (rc) return rc;
     }
     return counting_sort_ids(keys, n, nkeys, point_ids, w, s);
+}
+
+int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
+                        int32_t* cell_start, void* workspace, size_t ws_bytes, void* stream) {
+    return hashgrid_build(n, x, cell_width, dimx, dimy, dimz, nullptr, 0, point_ids, cell_start, workspace, ws_bytes,
+                          stream);
+}
+
+int mpmx_hashgrid_build_label(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+                              const int32_t* materialLabel, int label, int32_t* point_ids, int32_t* cell_start,
+                              void* workspace, size_t ws_bytes, void* stream) {
+    MPMX_REQUIRE(n == 0 || materialLabel, "mpmx_hashgrid_build_label: null label array");
+    return hashgrid_build(n, x, cell_width, dimx, dimy, dimz, materialLabel, label, point_ids, cell_start, workspace,
+                          ws_bytes, stream);
 }
 
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel, void* stream) {
@@ -332,9 +364,28 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel, int
     if (n == 0) return 0;
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && x && v && mass && radius && deltas,
                  "mpmx_solve_particle_particle_contacts: null pointer");
-    LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, 1.0f / cell_width,
-            dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold, static_friction, k_cohesion, max_radius, dt,
-            relaxation, contact_lock, deltas)
+    LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, nullptr, nullptr,
+            1.0f / cell_width, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold, static_friction,
+            k_cohesion, max_radius, dt, relaxation, contact_lock, deltas)
+}
+
+int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabel, int32_t* materialLabel,
+                                                const int32_t* point_ids, const int32_t* cell_start,
+                                                const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
+                                                float cell_width, int dimx, int dimy, int dimz, const float* x,
+                                                const float* v, const float* mass, const float* radius, float k_mu,
+                                                float static_threshold, float static_friction, float k_cohesion,
+                                                float max_radius, float dt, float relaxation, int contact_lock,
+                                                float* deltas, void* stream) {
+    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f,
+                 "mpmx_solve_particle_particle_contacts_split: bad sizes");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && xpbd_point_ids && xpbd_cell_start && x &&
+                     v && mass && radius && deltas,
+                 "mpmx_solve_particle_particle_contacts_split: null pointer");
+    LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, xpbd_point_ids,
+            xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold,
+            static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas)
 }
 
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,

@@ -61,6 +61,9 @@ class Array:
         if storage is None:
             storage = torch.zeros((nc, self.size) if nc > 1 else (self.size,), dtype=tdt, device=device())
         self.t = storage
+        # bumped by every write issued from the host (assign from numpy, fill_):
+        # the fused substep rebuilds its particle bins when x or the labels change
+        self.version = 0
 
     # --- pointers ---------------------------------------------------------
     @property
@@ -94,12 +97,14 @@ class Array:
             np.ascontiguousarray(a, dtype=npdt).reshape(self.size)
         host = torch.from_numpy(np.ascontiguousarray(a.T) if nc > 1 else a)
         self.t.copy_(host.to(self.t.device, non_blocking=False))
+        self.version += 1
         return self
 
     def zero_(self) -> "Array":
         return self.fill_(0)
 
     def fill_(self, value) -> "Array":
+        self.version += 1
         nc = _NCOMP[self.dtype]
         if nc > 1 and np.ndim(value) > 0:
             vals = np.asarray(value, dtype=np.float32).reshape(-1)
@@ -209,7 +214,8 @@ class HashGrid:
     def id(self):
         return self
 
-    def build(self, points: Array, cell_width: float) -> None:
+    def build(self, points: Array, cell_width: float, labels: Array | None = None, label: int = 2) -> None:
+        """Bin all points, or (with ``labels``) only those whose label equals ``label``."""
         n = points.shape[0]
         if self.point_ids is None or self.point_ids.shape[0] != n:
             self.point_ids = zeros(n, int32)
@@ -217,8 +223,12 @@ class HashGrid:
         self.cell_width = float(cell_width)
         dx, dy, dz = self.dims
         ws, nb = self._ws.get(lib.raw("mpmx_hashgrid_workspace_bytes")(n, dx, dy, dz))
-        lib("mpmx_hashgrid_build", n, points.ptr, self.cell_width, dx, dy, dz, self.point_ids.ptr,
-            self.cell_start.ptr, ws, nb, stream_handle())
+        if labels is None:
+            lib("mpmx_hashgrid_build", n, points.ptr, self.cell_width, dx, dy, dz, self.point_ids.ptr,
+                self.cell_start.ptr, ws, nb, stream_handle())
+        else:
+            lib("mpmx_hashgrid_build_label", n, points.ptr, self.cell_width, dx, dy, dz, labels.ptr, int(label),
+                self.point_ids.ptr, self.cell_start.ptr, ws, nb, stream_handle())
 
     def cell_starts_ends(self):
         cs = self.cell_start.numpy()

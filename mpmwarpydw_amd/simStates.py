@@ -165,6 +165,8 @@ class XPBDState:
         self.particle_delta = rt.zeros(n, rt.vec3)
         self.particle_v_before_contact = rt.zeros(n, rt.vec3)
         self.particle_grid = rt.HashGrid(128, 128, 128)
+        # the label-2 points only, same hashing (the MPM side of the contact solve walks it)
+        self.particle_grid_xpbd = rt.HashGrid(128, 128, 128)
         self.minBoundsXPBD = None
         self.maxBoundsXPBD = None
         self.max_radius = None

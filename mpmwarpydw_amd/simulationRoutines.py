@@ -103,18 +103,92 @@ def g2p_phase(sim, mpm, xpbd):
                    _cov_flag(mpm), sim.minBounds])
 
 
-def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
-    """The same substep with P2G and the grid update in one device pass (3 kernels + binning)."""
+def mpmSimulationStepPhased(sim, mpm, xpbd, device=None):
+    """The substep as three calls: stress, P2G with the grid update, G2P."""
     _check_vlc(mpm)
     stress_phase(sim, mpm, xpbd)
     p2g_grid_phase(sim, mpm, xpbd)
     g2p_phase(sim, mpm, xpbd)
 
 
-def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False):
-    """The XPBD contact step (simulationRoutines.py:295-526), launch for launch."""
+def _ptr(a):
+    return None if a is None else a.t.data_ptr()
+
+
+def step_structs(sim, mpm, xpbd):
+    """The mpmx_particles / mpmx_grid / mpmx_step_params views of the state."""
+    from ._lib import Grid, Particles, StepParams
+    P = Particles(n=sim.nPoints, activeLabel=_ptr(sim.activeLabel), materialLabel=_ptr(sim.materialLabel),
+                  x=_ptr(sim.particle_x), v=_ptr(sim.particle_v), x_init_xpbd=_ptr(xpbd.particle_x_initial_xpbd),
+                  v_init_xpbd=_ptr(xpbd.particle_v_initial_xpbd), F=_ptr(mpm.particle_F),
+                  F_trial=_ptr(mpm.particle_F_trial), C=_ptr(mpm.particle_C), stress=_ptr(mpm.particle_stress),
+                  cov=_ptr(mpm.particle_cov), mu=_ptr(mpm.mu), lam=_ptr(mpm.lam), ys=_ptr(mpm.ys),
+                  alpha=_ptr(mpm.alpha), hardening=_ptr(mpm.hardening), softening=_ptr(mpm.softening),
+                  density=_ptr(sim.particle_density), strainCriteria=_ptr(mpm.strainCriteria),
+                  eta_shear=_ptr(mpm.eta_shear), eta_bulk=_ptr(mpm.eta_bulk),
+                  acc_strain=_ptr(mpm.particle_accumulated_strain), damage=_ptr(mpm.particle_damage),
+                  vol=_ptr(sim.particle_vol), mass=_ptr(sim.particle_mass))
+    gx, gy, gz = mpm.gridDims
+    G = Grid(gx=gx, gy=gy, gz=gz, m=_ptr(mpm.grid_m), v_in=_ptr(mpm.grid_v_in), v_out=_ptr(mpm.grid_v_out))
+    g = sim.gravity
+    mb = sim.minBounds
+    prm = StepParams(dt=sim.dt, dx=sim.dx, inv_dx=sim.invdx, minx=mb[0], miny=mb[1], minz=mb[2], gravity_x=g[0],
+                     gravity_y=g[1], gravity_z=g[2], rpic_damping=mpm.rpic_damping, eff=mpm.eff,
+                     damping_scale=mpm.grid_v_damping_scale, friction=mpm.boundFriction,
+                     gravity_magnitude=abs(g[2]), bc=_bc_code(mpm), boundary_padding=int(mpm.boundaryPadding),
+                     update_cov=int(_cov_flag(mpm)), constitutive_model=int(mpm.constitutive_model))
+    return P, G, prm
+
+
+def _bin_key(sim, mpm):
+    return (sim.nPoints, mpm.gridDims, id(sim.particle_x), sim.particle_x.version, id(sim.materialLabel),
+            sim.materialLabel.version, id(sim.activeLabel), sim.activeLabel.version, tuple(sim.minBounds), sim.invdx)
+
+
+def substep_workspace(sim, mpm):
+    if getattr(mpm, "_substep_ws", None) is None:
+        mpm._substep_ws = Workspace()
+        mpm._bins = None
+    gx, gy, gz = mpm.gridDims
+    return mpm._substep_ws.get(lib.raw("mpmx_substep_workspace_bytes")(sim.nPoints, gx, gy, gz))
+
+
+def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
+    """mpmSimulationStep (simulationRoutines.py:6-293) as libmpmx's fused substep:
+    a tile pass (constitutive update + deterministic P2G), a node pass (grid
+    update) and a tile pass (G2P, which also bins the particles for the next
+    substep).  The bins are rebuilt first whenever x, the labels or the
+    activity were written from the host since the last substep."""
+    _check_vlc(mpm)
+    P, G, prm = step_structs(sim, mpm, xpbd)
+    ws, nb = substep_workspace(sim, mpm)
+    key = _bin_key(sim, mpm)
+    if mpm._bins != key:
+        lib("mpmx_substep_bin", C.byref(P), C.byref(G), C.byref(prm), ws, nb, stream_handle())
+    lib("mpmx_mpm_substep", C.byref(P), C.byref(G), C.byref(prm), ws, nb, stream_handle())
+    mpm._bins = key
+
+
+def misbinned_count(sim, mpm):
+    """Particles found outside their bin since the last rebin (must be 0)."""
+    ws, nb = substep_workspace(sim, mpm)
+    out = C.c_int(-1)
+    gx, gy, gz = mpm.gridDims
+    lib("mpmx_substep_misbinned", ws, nb, gx, gy, gz, sim.nPoints, C.byref(out), stream_handle())
+    return out.value
+
+
+def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=True):
+    """The XPBD contact step (simulationRoutines.py:295-526), launch for launch.
+
+    With ``split_grid`` (default) a second table of the label-2 points is built
+    next to the full one and the particle-contact solve lets MPM particles walk
+    it (same candidates in the same order, so the same results; see
+    mpmx_solve_particle_particle_contacts_split)."""
     n = sim.nPoints
     xpbd.particle_grid.build(sim.particle_x, sim.dx)
+    if split_grid:
+        xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2)
     rt.copy(xpbd.particle_x_integrated, sim.particle_x)
     rt.copy(xpbd.particle_v_integrated, sim.particle_v)
     launch(kernel=xpbdRoutines.integrateParticlesXPBD, dim=n,
@@ -132,12 +206,18 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False):
                        xpbd.staticVelocityThreshold, xpbd.staticParticleFriction, xpbd.minBoundsXPBD,
                        xpbd.maxBoundsXPBD, sim.dtxpbd, xpbd.xpbd_relaxation],
                outputs=[xpbd.particle_delta])
-        launch(kernel=xpbdRoutines.my_solve_particle_particle_contacts, dim=n,
-               inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, xpbd.particle_x_integrated,
-                       xpbd.particle_v_integrated, sim.particle_mass, sim.particle_radius,
-                       xpbd.dynamicParticleFriction, xpbd.staticVelocityThreshold, xpbd.staticParticleFriction,
-                       xpbd.particle_cohesion, xpbd.max_radius, sim.dtxpbd, xpbd.xpbd_relaxation, contact_lock],
-               outputs=[xpbd.particle_delta])
+        contact_args = [xpbd.particle_x_integrated, xpbd.particle_v_integrated, sim.particle_mass,
+                        sim.particle_radius, xpbd.dynamicParticleFriction, xpbd.staticVelocityThreshold,
+                        xpbd.staticParticleFriction, xpbd.particle_cohesion, xpbd.max_radius, sim.dtxpbd,
+                        xpbd.xpbd_relaxation, contact_lock]
+        if split_grid:
+            launch(kernel=xpbdRoutines.solve_particle_particle_contacts_split, dim=n,
+                   inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, xpbd.particle_grid_xpbd.id,
+                           *contact_args], outputs=[xpbd.particle_delta])
+        else:
+            launch(kernel=xpbdRoutines.my_solve_particle_particle_contacts, dim=n,
+                   inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, *contact_args],
+                   outputs=[xpbd.particle_delta])
         rt.copy(xpbd.particle_v_deltaInt, xpbd.particle_v_integrated)
         rt.copy(xpbd.particle_x_deltaInt, xpbd.particle_x_integrated)
         launch(kernel=xpbdRoutines.my_apply_particle_deltas, dim=n,

@@ -62,6 +62,25 @@ my_solve_particle_particle_contacts = Kernel("my_solve_particle_particle_contact
                                              "xpbdRoutines.py:138-219 (hash_grid_query order of wp.HashGrid)")
 
 
+def _solve_particle_particle_contacts_split(dim, activeLabel, materialLabel, grid, xpbd_grid, particle_x, particle_v,
+                                            particle_mass, particle_radius, k_mu,
+                                            staticParticleVelocityThresholdRatio, staticParticleFriction, k_cohesion,
+                                            max_radius, dt, relaxation, mpm_contact_transition_lock, deltas):
+    dx, dy, dz = grid.dims
+    assert xpbd_grid.dims == grid.dims and xpbd_grid.cell_width == grid.cell_width
+    lib("mpmx_solve_particle_particle_contacts_split", int(dim), activeLabel.ptr, materialLabel.ptr,
+        grid.point_ids.ptr, grid.cell_start.ptr, xpbd_grid.point_ids.ptr, xpbd_grid.cell_start.ptr,
+        float(grid.cell_width), dx, dy, dz, particle_x.ptr, particle_v.ptr, particle_mass.ptr, particle_radius.ptr,
+        float(k_mu), float(staticParticleVelocityThresholdRatio), float(staticParticleFriction), float(k_cohesion),
+        float(max_radius), float(dt), float(relaxation), int(mpm_contact_transition_lock), deltas.ptr,
+        stream_handle())
+
+
+solve_particle_particle_contacts_split = Kernel(
+    "solve_particle_particle_contacts_split", _solve_particle_particle_contacts_split,
+    "xpbdRoutines.py:138-219 with MPM particles walking an XPBD-only table (same results)")
+
+
 def _my_apply_particle_deltas(dim, activeLabel, materialLabel, x_orig, x_pred, delta, dt, v_max, x_out, v_out):
     lib("mpmx_apply_particle_deltas", int(dim), activeLabel.ptr, materialLabel.ptr, x_orig.ptr, x_pred.ptr, delta.ptr,
         float(dt), float(v_max), x_out.ptr, v_out.ptr, stream_handle())

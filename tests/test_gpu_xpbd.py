@@ -129,6 +129,44 @@ def test_xpbd_step_parity(o64):
                        xi, dist, f32(sim.dtxpbd))
     o64.apply_mpm_contact_impulses(H["activeLabel"], H["materialLabel"], x0, v0, xi, vi, f32(sim.dtxpbd),
                                    f32(xpbd.particle_v_max), f32(xpbd.xpbd_mpm_coupling_strength), 0, vi)
-    assert rel_err(G["particle_x"] - x0, xi - x0) < 1e-4
-    assert rel_err(G["particle_v"], vi) < TOL
-    assert rel_err(G["particle_cumDist_xpbd"], dist) < TOL
+    # Positions are FP32 coordinates of a several-metre domain: one rounding of x
+    # is ulp(max|x|), and the step's velocities are (x_new - x_orig) / dt
+    # (xpbdRoutines.py:246), so FP32 against FP64 differs by a few ulp(x) in x
+    # and a few ulp(x) / dt in v after 4 iterations -- the resolution of the
+    # reference's own FP32 state.  Bound: 16 ulp.
+    ulp = float(np.spacing(np.float32(np.max(np.abs(x0)))))
+    ex = np.max(np.abs(G["particle_x"] - xi))
+    ev = np.max(np.abs(G["particle_v"] - vi))
+    print(f"  x err {ex:.2e} (ulp {ulp:.2e}), v err {ev:.2e} (ulp/dt {ulp / sim.dtxpbd:.2e})")
+    assert ex <= 16 * ulp
+    assert ev <= 16 * ulp / float(np.float32(sim.dtxpbd)) + TOL * np.max(np.abs(vi))
+    assert np.max(np.abs(G["particle_cumDist_xpbd"] - dist)) <= 16 * ulp
+
+
+@pytest.mark.parametrize("lock", [0, 1])
+def test_split_table_is_bit_identical(lock):
+    """MPM particles walking the XPBD-only table give bit-identical deltas and
+    labels to walking the full table (same candidates, same order)."""
+    from mpmwarpydw_amd import runtime as rt
+    from mpmwarpydw_amd import xpbdRoutines as X
+    from mpmwarpydw_amd.runtime import launch
+    outs = []
+    for split in (False, True):
+        S = xpbd_scene(4)
+        sim, xpbd = S.sim, S.xpbd
+        xpbd.particle_grid.build(sim.particle_x, sim.dx)
+        xpbd.particle_delta.zero_()
+        args = [sim.particle_x, sim.particle_v, sim.particle_mass, sim.particle_radius, xpbd.dynamicParticleFriction,
+                xpbd.staticVelocityThreshold, xpbd.staticParticleFriction, xpbd.particle_cohesion, xpbd.max_radius,
+                sim.dtxpbd, xpbd.xpbd_relaxation, lock]
+        if split:
+            xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2)
+            launch(X.solve_particle_particle_contacts_split, sim.nPoints,
+                   inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, xpbd.particle_grid_xpbd.id,
+                           *args], outputs=[xpbd.particle_delta])
+        else:
+            launch(X.my_solve_particle_particle_contacts, sim.nPoints,
+                   inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, *args],
+                   outputs=[xpbd.particle_delta])
+        outs.append((xpbd.particle_delta.numpy(), sim.materialLabel.numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
