@@ -11,9 +11,10 @@ every 10 substeps.  Inputs are resident in HBM before the timed region.
 
 One XPBD period (10 substeps, the XPBD step included) is captured once in a
 HIP graph; W untimed substeps run first, then the graph is replayed K/10
-times between two device synchronisations.  The library's launch profiler is
-captured into the graph (an event after every kernel), so the per-kernel
-durations are those of the last period of the timed region.
+times between two device synchronisations.  Per-kernel durations come from
+the library's launch profiler (HIP events just before and after every kernel,
+on the launch stream) over one more period launched eagerly right after the
+timed region.
 
 Prints one JSON line: the metric (MPM particle-substeps/s), per-kernel device
 times, the roofline entry of the dominant kernel and a CPU baseline (the FP32
@@ -148,15 +149,12 @@ def main():
 
     period = [counter + k for k in range(ratio)]  # the XPBD step falls on the multiple of ratio inside
     graph = None
-    lib("mpmx_profile_begin", 64 * ratio + 64)
     if not a.no_graph:
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph, stream=stream):
-            lib("mpmx_profile_mark", stream_handle())
             for c in period:
                 S.substep(c)
-        lib("mpmx_profile_end")
-        graph.replay()  # the captured period runs once before timing (an untimed step)
+        graph.replay()  # the captured period runs once before timing (untimed)
         torch.cuda.synchronize()
 
     t0 = time.perf_counter()
@@ -165,26 +163,21 @@ def main():
             if graph is not None:
                 graph.replay()
             else:
-                lib("mpmx_profile_mark", stream_handle())
                 for c in period:
                     S.substep(c)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+
+    # per-kernel device time: one more period launched eagerly on the same stream with
+    # every library kernel bracketed by HIP events (graph-recorded events cannot be timed)
+    lib("mpmx_profile_begin", 2 * (64 * ratio + 64))
+    with torch.cuda.stream(stream):
+        for c in period:
+            S.substep(c)
+    torch.cuda.synchronize()
     lib("mpmx_profile_end")
-    try:
-        kernels = profile_report()
-        how = "events captured in the graph (last replayed period)" if graph is not None else "eager events"
-    except Exception as e:  # timing of graph-recorded events unsupported: profile one eager period instead
-        print(f"[bench] graph event timing unavailable ({e}); profiling one eager period", file=sys.stderr)
-        lib("mpmx_profile_begin", 64 * ratio + 64)
-        with torch.cuda.stream(stream):
-            lib("mpmx_profile_mark", stream_handle())
-            for c in period:
-                S.substep(c)
-        torch.cuda.synchronize()
-        lib("mpmx_profile_end")
-        kernels = profile_report()
-        how = "eager events (one period after the timed region)"
+    kernels = profile_report()
+    how = "HIP events bracketing each kernel, one eager period right after the timed region"
 
     lab = sim.materialLabel.numpy()
     act = sim.activeLabel.numpy()

@@ -96,10 +96,10 @@ typedef struct mpmx_step_params {
 const char* mpmx_last_error(void);
 int mpmx_version(void);
 /* Launch profiler (no counterpart in the reference; replaces its wall-clock
- * prints, runMPMYDW.py:762): between begin and end an event is recorded on the
- * launch stream after every kernel the library issues; a kernel's time is the
- * gap to the previous event, so call mpmx_profile_mark() on the stream before
- * a timed sequence.  report() writes "name launches total_ms" lines. */
+ * prints, runMPMYDW.py:762): between begin and end, events are recorded on the
+ * launch stream just before and just after every kernel the library issues
+ * (not while the stream is being captured into a graph: those events cannot be
+ * timed).  report() writes "name launches total_ms" lines. */
 int mpmx_profile_begin(int capacity);
 int mpmx_profile_mark(void* stream);
 int mpmx_profile_end(void);
@@ -306,6 +306,38 @@ int mpmx_apply_mpm_contact_impulses(int n, const int32_t* activeLabel,
                                     const float* v_orig, float* x_final, const float* v_final,
                                     float dt, float v_max, float coupling, int xpbd_only,
                                     float* v_out, void* stream);
+/* ---- fused XPBD step: xpbdSimulationStep (simulationRoutines.py:295-526) with
+ * its copies folded in.  Sequence: mpmx_xpbd_begin; per iteration
+ * mpmx_xpbd_bound_contacts_write, mpmx_solve_particle_particle_contacts[_split],
+ * mpmx_xpbd_apply_deltas_inplace; then mpmx_xpbd_finish (counters zeroed first).
+ * x, v, labels, the cumulative distance and the counters end as after the launch
+ * sequence; the scratch arrays particle_x_deltaInt/particle_v_deltaInt/
+ * particle_v_before_contact are not written. */
+/* x_int, v_int <- x, v then integrateParticlesXPBD (xpbdRoutines.py:23-48); with
+ * reset_labels also reset_mpm_contact_labels (:3-21) */
+int mpmx_xpbd_begin(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x,
+                    const float* v, float gxv, float gyv, float gzv, float dt, float v_max,
+                    int reset_labels, float* x_int, float* v_int, void* stream);
+/* particle_delta.zero_() + my_solve_particle_bound_contacts (xpbdRoutines.py:49-136):
+ * writes every particle's delta */
+int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel,
+                                   const int32_t* materialLabel, const float* x, const float* v,
+                                   const float* mass, const float* radius, float mu,
+                                   float static_threshold, float static_friction, float minx,
+                                   float miny, float minz, float maxx, float maxy, float maxz,
+                                   float dt, float relaxation, float* delta, void* stream);
+/* the four copies and my_apply_particle_deltas (xpbdRoutines.py:221-257) as one
+ * in-place update of x_int / v_int */
+int mpmx_xpbd_apply_deltas_inplace(int n, const int32_t* activeLabel, const float* x_orig,
+                                   const float* delta, float dt, float v_max, float* x_int,
+                                   float* v_int, void* stream);
+/* sleepParticles (xpbdRoutines.py:313-334), apply_mpm_contact_impulses (:259-310)
+ * and the commit (simulationRoutines.py:469-470); counters = {total, sleeping} */
+int mpmx_xpbd_finish(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                     float sleep_threshold, const float* radius, float* dist_total, float dt,
+                     float v_max, float coupling, int xpbd_only, float* x, float* v, float* x_int,
+                     float* v_int, int32_t* counters, void* stream);
+
 /* swellParticlesType2, xpbdRoutines.py:354-376 */
 int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t* materialLabel,
                                const float* x_init, const float* x, float* radius, float factor,

@@ -184,7 +184,7 @@ int mpmx_compute_mu_lam_bulk(int n, const float* E, const float* nu, float* mu, 
     MPMX_REQUIRE(n >= 0, "mpmx_compute_mu_lam_bulk: n < 0");
     if (n == 0) return 0;
     MPMX_REQUIRE(E && nu && mu && lam && bulk, "mpmx_compute_mu_lam_bulk: null pointer");
-    hipLaunchKernelGGL(mu_lam_bulk_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, E, nu,
+    MPMX_LAUNCH(mu_lam_bulk_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, E, nu,
                        mu, lam, bulk);
     return check_launch("compute_mu_lam_bulk", as_stream(stream));
 }
@@ -227,7 +227,7 @@ int mpmx_compute_stress_from_F_trial(int n, const int32_t* activeLabel, int32_t*
     P.eta_bulk = const_cast<float*>(eta_bulk);
     P.acc_strain = acc_strain;
     P.damage = damage;
-    hipLaunchKernelGGL(stress_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), P, eff);
+    MPMX_LAUNCH(stress_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), P, eff);
     return check_launch("compute_stress_from_F_trial", as_stream(stream));
 }
 
@@ -236,7 +236,7 @@ int mpmx_grid_normalization_and_gravity(int gx, int gy, int gz, const float* gri
     MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_m && grid_v_in && grid_v_out,
                  "mpmx_grid_normalization_and_gravity: bad arguments");
     long G = (long)gx * gy * gz;
-    hipLaunchKernelGGL(grid_norm_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), G, grid_m,
+    MPMX_LAUNCH(grid_norm_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), G, grid_m,
                        grid_v_in, grid_v_out, gxv, gyv, gzv, dt);
     return check_launch("grid_normalization_and_gravity", as_stream(stream));
 }
@@ -244,7 +244,7 @@ int mpmx_grid_normalization_and_gravity(int gx, int gy, int gz, const float* gri
 int mpmx_add_damping_via_grid(int gx, int gy, int gz, float* grid_v_out, float scale, void* stream) {
     MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_v_out, "mpmx_add_damping_via_grid: bad arguments");
     long G = (long)gx * gy * gz;
-    hipLaunchKernelGGL(grid_damp_kernel, dim3(grid_for(3 * G, kBlock)), dim3(kBlock), 0, as_stream(stream), G,
+    MPMX_LAUNCH(grid_damp_kernel, dim3(grid_for(3 * G, kBlock)), dim3(kBlock), 0, as_stream(stream), G,
                        grid_v_out, scale);
     return check_launch("add_damping_via_grid", as_stream(stream));
 }
@@ -253,7 +253,7 @@ int mpmx_collide_bounds(float* grid_v_out, int gx, int gy, int gz, float frictio
                         int boundary_padding, void* stream) {
     MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_v_out, "mpmx_collide_bounds: bad arguments");
     long G = (long)gx * gy * gz;
-    hipLaunchKernelGGL(collide_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), grid_v_out, gx,
+    MPMX_LAUNCH(collide_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), grid_v_out, gx,
                        gy, gz, friction, dt, gmag, boundary_padding);
     return check_launch("collideBounds", as_stream(stream));
 }
@@ -265,7 +265,7 @@ int mpmx_grid_update(int gx, int gy, int gz, const float* grid_m, const float* g
     MPMX_REQUIRE(bc == MPMX_BC_NONE || bc == MPMX_BC_FRICTION, "mpmx_grid_update: unsupported boundary condition %d",
                  bc);
     long G = (long)gx * gy * gz;
-    hipLaunchKernelGGL(grid_update_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), gx, gy, gz,
+    MPMX_LAUNCH(grid_update_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), gx, gy, gz,
                        grid_m, grid_v_in, grid_v_out, gxv, gyv, gzv, dt, damping_scale, bc, friction, gmag,
                        boundary_padding);
     return check_launch("grid_update", as_stream(stream));
@@ -278,7 +278,7 @@ int mpmx_g2p(float dt, int n, const int32_t* activeLabel, const int32_t* materia
     if (n == 0) return 0;
     MPMX_REQUIRE(activeLabel && materialLabel && x && v && C && F && F_trial && grid_v_out && (!update_cov || cov),
                  "mpmx_g2p: null pointer");
-    hipLaunchKernelGGL(g2p_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), dt, n, activeLabel,
+    MPMX_LAUNCH(g2p_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), dt, n, activeLabel,
                        materialLabel, x, v, C, F, F_trial, cov, inv_dx, grid_v_out, gx, gy, gz, update_cov, minx,
                        miny, minz);
     return check_launch("g2p", as_stream(stream));

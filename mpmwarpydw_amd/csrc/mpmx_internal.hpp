@@ -12,6 +12,8 @@ namespace mpmx {
 
 void set_error(const char* fmt, ...);
 int check_launch(const char* what, hipStream_t s);
+// launch profiler hook: records a start event before a kernel when profiling is on
+void prof_start(hipStream_t s);
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
 
@@ -58,6 +60,14 @@ int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, 
 int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s);
 
 }  // namespace mpmx
+
+// every kernel launch of the library goes through this macro so that the
+// profiler can bracket it with events (check_launch records the end event)
+#define MPMX_LAUNCH(kernel, grid, block, shmem, stream, ...)                      \
+    do {                                                                          \
+        ::mpmx::prof_start(stream);                                               \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ##__VA_ARGS__);   \
+    } while (0)
 
 #define MPMX_REQUIRE(cond, ...)                 \
     do {                                        \

@@ -34,8 +34,8 @@ static int debug_sync_enabled() {
 }
 
 // ---------------------------------------------------------------------------
-// launch profiler: when on, an event is recorded on the launch stream after
-// every kernel; a kernel's time is the gap to the previous event on the stream
+// launch profiler: when on, events are recorded on the launch stream just
+// before and just after every kernel (MPMX_LAUNCH + check_launch)
 // ---------------------------------------------------------------------------
 struct Profiler {
     bool on = false;
@@ -44,6 +44,11 @@ struct Profiler {
     size_t used = 0;
 };
 static Profiler g_prof;
+
+void prof_start(hipStream_t s) {
+    if (!g_prof.on || g_prof.used >= g_prof.pool.size()) return;
+    if (hipEventRecord(g_prof.pool[g_prof.used], s) == hipSuccess) g_prof.names[g_prof.used++] = nullptr;
+}
 
 static void prof_record(const char* what, hipStream_t s) {
     if (!g_prof.on || g_prof.used >= g_prof.pool.size()) return;
@@ -172,16 +177,16 @@ size_t scan_partials_count(long n) { return (size_t)((n + kScanTile - 1) / kScan
 int exclusive_scan_i32(const int32_t* counts, int32_t* out, long n, int32_t* partials, hipStream_t s) {
     long nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 0) {
-        hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(kBlock), 0, s, out, 1L, 0);
+        MPMX_LAUNCH(fill_kernel<int32_t>, dim3(1), dim3(kBlock), 0, s, out, 1L, 0);
         return check_launch("scan(empty)", s);
     }
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, counts, n, partials);
+    MPMX_LAUNCH(scan_reduce_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, counts, n, partials);
     int rc = check_launch("scan_reduce", s);
     if (rc) return rc;
-    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb);
+    MPMX_LAUNCH(scan_partials_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb);
     rc = check_launch("scan_partials", s);
     if (rc) return rc;
-    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, counts, n, partials, out, nb);
+    MPMX_LAUNCH(scan_apply_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, counts, n, partials, out, nb);
     return check_launch("scan_apply", s);
 }
 
@@ -234,20 +239,20 @@ bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w) {
 }
 
 int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s) {
-    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(fill_grid(nkeys)), dim3(kBlock), 0, s, w.count, nkeys, 0);
+    MPMX_LAUNCH(fill_kernel<int32_t>, dim3(fill_grid(nkeys)), dim3(kBlock), 0, s, w.count, nkeys, 0);
     int rc = check_launch("bin_zero", s);
     if (rc) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(bin_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.count, w.rank);
+        MPMX_LAUNCH(bin_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.count, w.rank);
         if ((rc = check_launch("bin_count", s))) return rc;
     }
     if ((rc = exclusive_scan_i32(w.count, w.cell_start, nkeys, w.partials, s))) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
+        MPMX_LAUNCH(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
                            w.rank, sorted);
         if ((rc = check_launch("bin_scatter", s))) return rc;
     }
-    hipLaunchKernelGGL(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
+    MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
                        sorted);
     return check_launch("bin_sort_runs", s);
 }
@@ -258,11 +263,11 @@ int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* 
     int rc = exclusive_scan_i32(w.count, w.cell_start, nkeys, w.partials, s);
     if (rc) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
+        MPMX_LAUNCH(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
                            w.rank, sorted);
         if ((rc = check_launch("bin_scatter", s))) return rc;
     }
-    hipLaunchKernelGGL(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
+    MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
                        sorted);
     return check_launch("bin_sort_runs", s);
 }
@@ -306,9 +311,14 @@ int mpmx_profile_report(char* buf, size_t buflen) {
     std::map<std::string, std::pair<long, double>> acc;
     for (size_t i = 1; i < g_prof.used; i++) {
         if (!g_prof.names[i]) continue;
-        MPMX_HIP(hipEventSynchronize(g_prof.pool[i]));
         float ms = 0.0f;
-        MPMX_HIP(hipEventElapsedTime(&ms, g_prof.pool[i - 1], g_prof.pool[i]));
+        hipError_t he = hipEventSynchronize(g_prof.pool[i]);
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, g_prof.pool[i - 1], g_prof.pool[i]);
+        if (he != hipSuccess) {
+            (void)hipGetLastError();  // do not leave the error for the next launch check
+            set_error("mpmx_profile_report: event timing failed: %s", hipGetErrorString(he));
+            return (int)he;
+        }
         auto& e = acc[g_prof.names[i]];
         e.first += 1;
         e.second += ms;
@@ -333,21 +343,21 @@ int mpmx_stream_sync(void* stream) {
 int mpmx_fill_i32(int32_t* dst, long n, int32_t value, void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || dst), "mpmx_fill_i32: bad arguments");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, n, value);
+    MPMX_LAUNCH(fill_kernel<int32_t>, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, n, value);
     return check_launch("fill_i32", as_stream(stream));
 }
 
 int mpmx_fill_f32(float* dst, long n, float value, void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || dst), "mpmx_fill_f32: bad arguments");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(fill_kernel<float>, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, n, value);
+    MPMX_LAUNCH(fill_kernel<float>, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, n, value);
     return check_launch("fill_f32", as_stream(stream));
 }
 
 int mpmx_copy_f32(float* dst, const float* src, long n, void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || (dst && src)), "mpmx_copy_f32: bad arguments");
     if (n == 0 || dst == src) return 0;
-    hipLaunchKernelGGL(copy_kernel, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, src, n);
+    MPMX_LAUNCH(copy_kernel, dim3(fill_grid(n)), dim3(kBlock), 0, as_stream(stream), dst, src, n);
     return check_launch("copy_f32", as_stream(stream));
 }
 

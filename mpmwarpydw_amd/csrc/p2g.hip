@@ -68,10 +68,10 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     P2GWork w;
     MPMX_REQUIRE(carve_p2g(ws, ws_bytes, n, tg, w), "p2g: workspace carve failed");
     int rc;
-    hipLaunchKernelGGL(fill_i32_one, dim3(1), dim3(1), 0, s, w.oob);
+    MPMX_LAUNCH(fill_i32_one, dim3(1), dim3(1), 0, s, w.oob);
     if ((rc = check_launch("p2g_oob_zero", s))) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(p2g_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, activeLabel,
+        MPMX_LAUNCH(p2g_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, activeLabel,
                            materialLabel, x, minx, miny, minz, inv_dx, tg, w.keys, w.oob);
         if ((rc = check_launch("p2g_key", s))) return rc;
     }
@@ -96,7 +96,7 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     ta.partial = w.partial;
     ta.tile_flag = w.tile_flag;
     ta.ntiles = (int)tg.ntiles();
-    hipLaunchKernelGGL(p2g_tile_kernel<false>, dim3((unsigned)tg.ntiles()), dim3(kBlock), 0, s, ta);
+    MPMX_LAUNCH(p2g_tile_kernel<false>, dim3((unsigned)tg.ntiles()), dim3(kBlock), 0, s, ta);
     if ((rc = check_launch("p2g_tile", s))) return rc;
     long G = (long)gx * gy * gz;
     CombineArgs ca{};
@@ -116,7 +116,7 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     ca.gmag = gmag;
     ca.bc = bc;
     ca.pad = pad;
-    hipLaunchKernelGGL(p2g_combine_kernel<>, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, s, ca);
+    MPMX_LAUNCH(p2g_combine_kernel<>, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, s, ca);
     if ((rc = check_launch("p2g_combine", s))) return rc;
     if (host_oob) {
         MPMX_HIP(hipMemcpyAsync(host_oob, w.oob, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -115,10 +115,10 @@ static int run_reduce(int n, const int32_t* active, const int32_t* mat, const fl
     Carver cv(ws, ws_bytes);
     double* psum = cv.take<double>(kRedBlocks);
     int32_t* pcnt = cv.take<int32_t>(kRedBlocks * 4);
-    hipLaunchKernelGGL(reduce_pass1<MODE>, dim3(nb), dim3(kBlock), 0, s, n, active, mat, a, radius, psum, pcnt);
+    MPMX_LAUNCH(reduce_pass1<MODE>, dim3(nb), dim3(kBlock), 0, s, n, active, mat, a, radius, psum, pcnt);
     int rc = check_launch("reduce_pass1", s);
     if (rc) return rc;
-    hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(kBlock), 0, s, nb, psum, pcnt, out_sum, out_cnt, ncnt);
+    MPMX_LAUNCH(reduce_pass2, dim3(1), dim3(kBlock), 0, s, nb, psum, pcnt, out_sum, out_cnt, ncnt);
     return check_launch("reduce_pass2", s);
 }
 

@@ -264,11 +264,11 @@ int mpmx_substep_bin(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     SubstepWork w;
     MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "substep: workspace carve failed");
     if (P->n > 0) {
-        hipLaunchKernelGGL(substep_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->minx,
+        MPMX_LAUNCH(substep_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->minx,
                            prm->miny, prm->minz, prm->inv_dx, tg, w.keys);
         if ((rc = check_launch("substep_key", s))) return rc;
     }
-    hipLaunchKernelGGL(zero_one_kernel, dim3(1), dim3(1), 0, s, w.misbinned);
+    MPMX_LAUNCH(zero_one_kernel, dim3(1), dim3(1), 0, s, w.misbinned);
     if ((rc = check_launch("substep_clear", s))) return rc;
     return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), w.sorted, w.bin, s);
 }
@@ -313,7 +313,7 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     ta.ntiles = (int)tg.ntiles();
     ta.tg = tg;
     ta.misbinned = w.misbinned;
-    hipLaunchKernelGGL(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
+    MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
     if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
 
     const long Gn = (long)G->gx * G->gy * G->gz;
@@ -336,7 +336,7 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     ca.pad = prm->boundary_padding;
     ca.zero_ints = w.bin.count;
     ca.n_zero = nkeys;
-    hipLaunchKernelGGL(p2g_combine_kernel<>, dim3(grid_for(Gn > nkeys ? Gn : nkeys, kBlock)), dim3(kBlock), 0, s, ca);
+    MPMX_LAUNCH(p2g_combine_kernel<>, dim3(grid_for(Gn > nkeys ? Gn : nkeys, kBlock)), dim3(kBlock), 0, s, ca);
     if ((rc = check_launch("substep_combine(grid)", s))) return rc;
 
     G2PArgs ga{};
@@ -355,7 +355,7 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     ga.count = w.bin.count;
     ga.rank = w.bin.rank;
     ga.misbinned = w.misbinned;
-    hipLaunchKernelGGL(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
+    MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
     if ((rc = check_launch("substep_g2p", s))) return rc;
 
     return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s);

@@ -275,7 +275,7 @@ __global__ void swell_kernel(int n, const int32_t* __restrict__ active, const in
 using namespace mpmx;
 
 #define LAUNCH1(kernel, n, ...)                                                                         \
-    hipLaunchKernelGGL(kernel, dim3(grid_for((n), kBlock)), dim3(kBlock), 0, as_stream(stream), __VA_ARGS__); \
+    MPMX_LAUNCH(kernel, dim3(grid_for((n), kBlock)), dim3(kBlock), 0, as_stream(stream), __VA_ARGS__); \
     return check_launch(#kernel, as_stream(stream));
 
 extern "C" {
@@ -304,7 +304,7 @@ static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int
     carve_bin(cv, nn, nkeys, w);
     w.cell_start = cell_start;  // the caller's table receives the cell ranges
     if (n > 0) {
-        hipLaunchKernelGGL(hash_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, x, 1.0f / cell_width,
+        MPMX_LAUNCH(hash_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, x, 1.0f / cell_width,
                            dimx, dimy, dimz, labels, label, keys);
         int rc = check_launch("hash_key", s);
         if (rc) return rc;
@@ -430,6 +430,195 @@ int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t*
     if (n == 0) return 0;
     LAUNCH1(swell_kernel, n, n, activeLabel, materialLabel, x_init, x, radius, factor, factor_max, swell_radius,
             base_radius)
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Fused XPBD step: the same persistent results as the launch sequence of
+// xpbdSimulationStep (simulationRoutines.py:295-526) with the copies folded in.
+// x_int/v_int are updated in place; the scratch arrays particle_x_deltaInt,
+// particle_v_deltaInt and particle_v_before_contact are not written.
+// ===========================================================================
+namespace mpmx {
+
+// x_int, v_int <- x, v; integrateParticlesXPBD on active XPBD particles
+// (xpbdRoutines.py:23-48); reset_mpm_contact_labels when lock (:3-21)
+__global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int32_t* __restrict__ mat,
+                                  const float* __restrict__ x, const float* __restrict__ v, float gx, float gy,
+                                  float gz, float dt, float v_max, int reset_labels, float* __restrict__ x_int,
+                                  float* __restrict__ v_int) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    f3 x0 = ld3(x, n, p), v0 = ld3(v, n, p);
+    const int a = active[p], m = mat[p];
+    if (a == 1 && m == 2) {
+        f3 v1 = f3{v0.x + gx * dt, v0.y + gy * dt, v0.z + gz * dt};
+        float mag = len3(v1);
+        if (mag > v_max) {
+            float s = v_max / mag;
+            v1 = f3{v1.x * s, v1.y * s, v1.z * s};
+        }
+        x0 = f3{x0.x + v1.x * dt, x0.y + v1.y * dt, x0.z + v1.z * dt};
+        v0 = v1;
+    }
+    st3(x_int, n, p, x0);
+    st3(v_int, n, p, v0);
+    if (reset_labels && a == 1 && m == 0) mat[p] = 1;
+}
+
+// my_solve_particle_bound_contacts writing (not adding) the delta of every
+// particle: 0 where the reference leaves the zeroed delta untouched
+__global__ void bound_contacts_write_kernel(int n, const int32_t* __restrict__ active,
+                                            const int32_t* __restrict__ mat, const float* __restrict__ x,
+                                            const float* __restrict__ v, const float* __restrict__ mass,
+                                            const float* __restrict__ radius, float mu, float thr, float smu, f3 bmin,
+                                            f3 bmax, float dt, float relax, float* __restrict__ delta) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float da[3] = {0.0f, 0.0f, 0.0f};
+    if (active[t] == 1 && mat[t] == 2 && mass[t] != 0.0f) {
+        float wi = 1.0f / mass[t];
+        f3 xp = ld3(x, n, t), vp = ld3(v, n, t);
+        float r = radius[t];
+        const float xa[3] = {xp.x, xp.y, xp.z}, va[3] = {vp.x, vp.y, vp.z};
+        const float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                float sg = side == 0 ? 1.0f : -1.0f;
+                float b = side == 0 ? lo[ax] : hi[ax];
+                float c = fminf(sg * (xa[ax] - b) - r, 0.0f);
+                if (c < 0.0f) {
+                    float lambda_n = c;
+                    float vn = sg * va[ax];
+                    float fmu = (fabsf(vn) < thr * r) ? smu : mu;
+                    float vt[3] = {va[0], va[1], va[2]};
+                    vt[ax] = va[ax] - sg * vn;
+                    float vtl = sqrtf(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+                    float a_ = fmu * lambda_n, b_ = -vtl * dt;
+                    float lf = a_ > b_ ? a_ : b_;  // wp.max
+                    float df[3] = {0.0f, 0.0f, 0.0f};
+                    if (vtl > 1.0e-8f) {
+                        f3 nt = normalize3(f3{vt[0], vt[1], vt[2]});
+                        df[0] = nt.x * lf; df[1] = nt.y * lf; df[2] = nt.z * lf;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        float dn = (q == ax) ? sg * lambda_n : 0.0f;
+                        da[q] += (df[q] - dn) / wi * relax * wi;
+                    }
+                }
+            }
+        }
+    }
+    st3(delta, n, t, f3{da[0], da[1], da[2]});
+}
+
+// my_apply_particle_deltas (xpbdRoutines.py:221-257) in place on x_int / v_int
+__global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ active, const float* __restrict__ x_orig,
+                                            const float* __restrict__ delta, float dt, float v_max,
+                                            float* __restrict__ x_int, float* __restrict__ v_int) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (active[t] != 1) return;
+    f3 x0 = ld3(x_orig, n, t), xp = ld3(x_int, n, t), d = ld3(delta, n, t);
+    if (isnan(d.x) || isnan(d.y) || isnan(d.z)) d = f3{0.0f, 0.0f, 0.0f};
+    f3 xn = f3{xp.x + d.x, xp.y + d.y, xp.z + d.z};
+    f3 vn = f3{(xn.x - x0.x) / dt, (xn.y - x0.y) / dt, (xn.z - x0.z) / dt};
+    float m = len3(vn);
+    if (m > v_max) {
+        float s = v_max / m;
+        vn = f3{vn.x * s, vn.y * s, vn.z * s};
+    }
+    st3(x_int, n, t, xn);
+    st3(v_int, n, t, vn);
+}
+
+// sleepParticles (xpbdRoutines.py:313-334), apply_mpm_contact_impulses (:259-310)
+// and the commit x <- x_int, v <- v_int (simulationRoutines.py:469-470) per particle
+__global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                                   float thr, const float* __restrict__ radius, float* __restrict__ dist, float dt,
+                                   float v_max, float k, int xpbd_only, float* __restrict__ x, float* __restrict__ v,
+                                   float* __restrict__ x_int, float* __restrict__ v_int,
+                                   int32_t* __restrict__ counters) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    f3 xi = ld3(x_int, n, t), vi = ld3(v_int, n, t);
+    const f3 x0 = ld3(x, n, t);
+    const int a = active[t], m = mat[t];
+    if (a == 1 && m == 2) {
+        atomicAdd(&counters[0], 1);
+        float d = len3(f3{xi.x - x0.x, xi.y - x0.y, xi.z - x0.z});
+        if (d / dt < thr * radius[t]) {
+            xi = x0;
+            atomicAdd(&counters[1], 1);
+        }
+        dist[t] = dist[t] + d;
+    }
+    if (a == 1 && m <= 1) {
+        xi = x0;
+        if (xpbd_only == 0) {
+            f3 vo = ld3(v, n, t);
+            f3 vc = f3{vo.x + (xi.x - x0.x) / dt * k, vo.y + (xi.y - x0.y) / dt * k, vo.z + (xi.z - x0.z) / dt * k};
+            float mag = len3(vc);
+            if (mag > v_max) {
+                float s = v_max / mag;
+                vc = f3{vc.x * s, vc.y * s, vc.z * s};
+            }
+            vi = vc;
+        }
+    }
+    st3(x_int, n, t, xi);
+    st3(v_int, n, t, vi);
+    st3(x, n, t, xi);
+    st3(v, n, t, vi);
+}
+
+}  // namespace mpmx
+
+extern "C" {
+
+int mpmx_xpbd_begin(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x, const float* v,
+                    float gxv, float gyv, float gzv, float dt, float v_max, int reset_labels, float* x_int,
+                    float* v_int, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && x_int && v_int)),
+                 "mpmx_xpbd_begin: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(xpbd_begin_kernel, n, n, activeLabel, materialLabel, x, v, gxv, gyv, gzv, dt, v_max, reset_labels, x_int,
+            v_int)
+}
+
+int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x,
+                                   const float* v, const float* mass, const float* radius, float mu,
+                                   float static_threshold, float static_friction, float minx, float miny, float minz,
+                                   float maxx, float maxy, float maxz, float dt, float relaxation, float* delta,
+                                   void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && mass && radius && delta)),
+                 "mpmx_xpbd_bound_contacts_write: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(bound_contacts_write_kernel, n, n, activeLabel, materialLabel, x, v, mass, radius, mu, static_threshold,
+            static_friction, f3{minx, miny, minz}, f3{maxx, maxy, maxz}, dt, relaxation, delta)
+}
+
+int mpmx_xpbd_apply_deltas_inplace(int n, const int32_t* activeLabel, const float* x_orig, const float* delta,
+                                   float dt, float v_max, float* x_int, float* v_int, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && x_orig && delta && x_int && v_int)),
+                 "mpmx_xpbd_apply_deltas_inplace: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(apply_deltas_inplace_kernel, n, n, activeLabel, x_orig, delta, dt, v_max, x_int, v_int)
+}
+
+int mpmx_xpbd_finish(int n, const int32_t* activeLabel, const int32_t* materialLabel, float sleep_threshold,
+                     const float* radius, float* dist_total, float dt, float v_max, float coupling, int xpbd_only,
+                     float* x, float* v, float* x_int, float* v_int, int32_t* counters, void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && radius && dist_total && x && v && x_int &&
+                                       v_int && counters)),
+                 "mpmx_xpbd_finish: bad args");
+    if (n == 0) return 0;
+    LAUNCH1(xpbd_finish_kernel, n, n, activeLabel, materialLabel, sleep_threshold, radius, dist_total, dt, v_max,
+            coupling, xpbd_only, x, v, x_int, v_int, counters)
 }
 
 }  // extern "C"

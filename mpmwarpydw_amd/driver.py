@@ -89,10 +89,10 @@ class Simulation:
 
     # ---- one substep of the combined phase (runMPMYDW.py:677-682) ----
     def substep(self, counter: int) -> None:
-        step = simulationRoutines.mpmSimulationStepFused if self.fused else simulationRoutines.mpmSimulationStep
-        step(self.sim, self.mpm, self.xpbd)
+        SR = simulationRoutines
+        (SR.mpmSimulationStepFused if self.fused else SR.mpmSimulationStep)(self.sim, self.mpm, self.xpbd)
         if counter % self.sim.mpmStepsPerXpbdStep == 0 and counter > 0:
-            simulationRoutines.xpbdSimulationStep(self.sim, self.mpm, self.xpbd)
+            (SR.xpbdSimulationStepFused if self.fused else SR.xpbdSimulationStep)(self.sim, self.mpm, self.xpbd)
 
     def residual(self) -> float:
         """sum |v|/r / N_active over active particles (runMPMYDW.py:686-708): one readback."""

@@ -111,25 +111,25 @@ def mpmSimulationStepPhased(sim, mpm, xpbd, device=None):
     g2p_phase(sim, mpm, xpbd)
 
 
-def _ptr(a):
+def _aptr(a):
     return None if a is None else a.t.data_ptr()
 
 
 def step_structs(sim, mpm, xpbd):
     """The mpmx_particles / mpmx_grid / mpmx_step_params views of the state."""
     from ._lib import Grid, Particles, StepParams
-    P = Particles(n=sim.nPoints, activeLabel=_ptr(sim.activeLabel), materialLabel=_ptr(sim.materialLabel),
-                  x=_ptr(sim.particle_x), v=_ptr(sim.particle_v), x_init_xpbd=_ptr(xpbd.particle_x_initial_xpbd),
-                  v_init_xpbd=_ptr(xpbd.particle_v_initial_xpbd), F=_ptr(mpm.particle_F),
-                  F_trial=_ptr(mpm.particle_F_trial), C=_ptr(mpm.particle_C), stress=_ptr(mpm.particle_stress),
-                  cov=_ptr(mpm.particle_cov), mu=_ptr(mpm.mu), lam=_ptr(mpm.lam), ys=_ptr(mpm.ys),
-                  alpha=_ptr(mpm.alpha), hardening=_ptr(mpm.hardening), softening=_ptr(mpm.softening),
-                  density=_ptr(sim.particle_density), strainCriteria=_ptr(mpm.strainCriteria),
-                  eta_shear=_ptr(mpm.eta_shear), eta_bulk=_ptr(mpm.eta_bulk),
-                  acc_strain=_ptr(mpm.particle_accumulated_strain), damage=_ptr(mpm.particle_damage),
-                  vol=_ptr(sim.particle_vol), mass=_ptr(sim.particle_mass))
+    P = Particles(n=sim.nPoints, activeLabel=_aptr(sim.activeLabel), materialLabel=_aptr(sim.materialLabel),
+                  x=_aptr(sim.particle_x), v=_aptr(sim.particle_v), x_init_xpbd=_aptr(xpbd.particle_x_initial_xpbd),
+                  v_init_xpbd=_aptr(xpbd.particle_v_initial_xpbd), F=_aptr(mpm.particle_F),
+                  F_trial=_aptr(mpm.particle_F_trial), C=_aptr(mpm.particle_C), stress=_aptr(mpm.particle_stress),
+                  cov=_aptr(mpm.particle_cov), mu=_aptr(mpm.mu), lam=_aptr(mpm.lam), ys=_aptr(mpm.ys),
+                  alpha=_aptr(mpm.alpha), hardening=_aptr(mpm.hardening), softening=_aptr(mpm.softening),
+                  density=_aptr(sim.particle_density), strainCriteria=_aptr(mpm.strainCriteria),
+                  eta_shear=_aptr(mpm.eta_shear), eta_bulk=_aptr(mpm.eta_bulk),
+                  acc_strain=_aptr(mpm.particle_accumulated_strain), damage=_aptr(mpm.particle_damage),
+                  vol=_aptr(sim.particle_vol), mass=_aptr(sim.particle_mass))
     gx, gy, gz = mpm.gridDims
-    G = Grid(gx=gx, gy=gy, gz=gz, m=_ptr(mpm.grid_m), v_in=_ptr(mpm.grid_v_in), v_out=_ptr(mpm.grid_v_out))
+    G = Grid(gx=gx, gy=gy, gz=gz, m=_aptr(mpm.grid_m), v_in=_aptr(mpm.grid_v_in), v_out=_aptr(mpm.grid_v_out))
     g = sim.gravity
     mb = sim.minBounds
     prm = StepParams(dt=sim.dt, dx=sim.dx, inv_dx=sim.invdx, minx=mb[0], miny=mb[1], minz=mb[2], gravity_x=g[0],
@@ -150,7 +150,11 @@ def substep_workspace(sim, mpm):
         mpm._substep_ws = Workspace()
         mpm._bins = None
     gx, gy, gz = mpm.gridDims
-    return mpm._substep_ws.get(lib.raw("mpmx_substep_workspace_bytes")(sim.nPoints, gx, gy, gz))
+    before = mpm._substep_ws.t
+    ws, nb = mpm._substep_ws.get(lib.raw("mpmx_substep_workspace_bytes")(sim.nPoints, gx, gy, gz))
+    if mpm._substep_ws.t is not before:  # new storage: the bins it held are gone
+        mpm._bins = None
+    return ws, nb
 
 
 def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
@@ -240,6 +244,52 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=
            outputs=[xpbd.particle_v_integrated])
     sim.particle_x.assign(xpbd.particle_x_integrated)
     sim.particle_v.assign(xpbd.particle_v_integrated)
+    if xpbd.swellingRatio > 0:
+        launch(kernel=xpbdRoutines.swellParticlesType2, dim=n,
+               inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_x_initial_xpbd, sim.particle_x,
+                       sim.particle_radius, xpbd.swellingActivationFactor, xpbd.swellingMaxFactor,
+                       xpbd.particleMaxRadius, xpbd.particleBaseRadius])
+
+
+def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
+    """xpbdSimulationStep with the copies folded into its kernels: x, v, the
+    labels, particle_cumDist_xpbd and the sleep counters end exactly as after
+    the launch sequence; the scratch arrays particle_x_deltaInt,
+    particle_v_deltaInt and particle_v_before_contact are not written (nothing
+    outside the step reads them)."""
+    n = sim.nPoints
+    xpbd.particle_grid.build(sim.particle_x, sim.dx)
+    xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2)
+    g = sim.gravity
+    s = stream_handle()
+    reset = 1 if (xpbd.mpm_contact_transition_lock and not xpbd_only) else 0
+    xi, vi = xpbd.particle_x_integrated, xpbd.particle_v_integrated
+    lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
+        float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
+        s)
+    contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
+    mn, mx = xpbd.minBoundsXPBD, xpbd.maxBoundsXPBD
+    grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
+    dims = grid.dims
+    for _ in range(xpbd.xpbd_iterations):
+        lib("mpmx_xpbd_bound_contacts_write", n, sim.activeLabel.ptr, sim.materialLabel.ptr, xi.ptr, vi.ptr,
+            sim.particle_mass.ptr, sim.particle_radius.ptr, float(xpbd.dynamicParticleFriction),
+            float(xpbd.staticVelocityThreshold), float(xpbd.staticParticleFriction), *[float(v) for v in mn],
+            *[float(v) for v in mx], float(sim.dtxpbd), float(xpbd.xpbd_relaxation), xpbd.particle_delta.ptr, s)
+        lib("mpmx_solve_particle_particle_contacts_split", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
+            grid.point_ids.ptr, grid.cell_start.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr,
+            float(grid.cell_width), dims[0], dims[1], dims[2], xi.ptr, vi.ptr, sim.particle_mass.ptr,
+            sim.particle_radius.ptr, float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
+            float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
+            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock), xpbd.particle_delta.ptr, s)
+        lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
+            float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, s)
+    sim.numTotalXPBD.t.zero_()
+    sim.numSleepingXPBD.t.zero_()
+    lib("mpmx_xpbd_finish", n, sim.activeLabel.ptr, sim.materialLabel.ptr, float(xpbd.sleepThreshold),
+        sim.particle_radius.ptr, xpbd.particle_cumDist_xpbd.ptr, float(sim.dtxpbd), float(xpbd.particle_v_max),
+        float(xpbd.xpbd_mpm_coupling_strength), 1 if xpbd_only else 0, sim.particle_x.ptr, sim.particle_v.ptr,
+        xi.ptr, vi.ptr, sim.numTotalXPBD.ptr, s)
     if xpbd.swellingRatio > 0:
         launch(kernel=xpbdRoutines.swellParticlesType2, dim=n,
                inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_x_initial_xpbd, sim.particle_x,

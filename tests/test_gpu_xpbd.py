@@ -13,13 +13,20 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def xpbd_scene(seed=0, scale=0.12):
+def xpbd_scene(seed=0, scale=0.12, radius=None):
+    """Half-XPBD lattice block (contact_heavy at reduced size), jittered.  The
+    default radius 0.25 m at 0.25 m spacing overlaps every neighbour by up to
+    0.25 m (a stress case for single-kernel parity); radius=0.135 gives the
+    few-centimetre overlaps of a settled pile."""
     S = make_sim("contact_heavy", scale=scale, seed=seed, perturb=False)
     rng = np.random.default_rng(seed)
     n = S.sim.nPoints
     x = S.sim.particle_x.numpy().astype(np.float64) + 0.03 * rng.standard_normal((n, 3))
     S.sim.particle_x.assign(x)
     S.sim.particle_v.assign(0.5 * rng.standard_normal((n, 3)))
+    if radius is not None:
+        S.sim.particle_radius.assign(np.full(n, radius))
+        S.xpbd.max_radius = radius
     act = np.ones(n, np.int32)
     act[rng.random(n) < 0.02] = 0
     S.sim.activeLabel.assign(act)
@@ -93,7 +100,7 @@ def test_xpbd_step_parity(o64):
     """The whole xpbdSimulationStep (integrate, 4 iterations of bound + particle
     contacts and delta application, sleep, MPM reset, commit) from one state."""
     from mpmwarpydw_amd import simulationRoutines as SR
-    S = xpbd_scene(3)
+    S = xpbd_scene(3, radius=0.135)
     sim, xpbd = S.sim, S.xpbd
     H = host_state(S)
     print(f"[xpbd step] n={sim.nPoints}", flush=True)
@@ -170,3 +177,21 @@ def test_split_table_is_bit_identical(lock):
                    outputs=[xpbd.particle_delta])
         outs.append((xpbd.particle_delta.numpy(), sim.materialLabel.numpy()))
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("lock", [0, 1])
+def test_fused_xpbd_step_matches_launch_sequence(lock):
+    """The fused XPBD step leaves x, v, labels, the cumulative distance and the
+    sleep counters bit-identical to the launch-for-launch step."""
+    from mpmwarpydw_amd import simulationRoutines as SR
+    outs = []
+    for fused in (False, True):
+        S = xpbd_scene(5)
+        S.xpbd.mpm_contact_transition_lock = lock
+        (SR.xpbdSimulationStepFused if fused else SR.xpbdSimulationStep)(S.sim, S.mpm, S.xpbd)
+        h = host_state(S)
+        outs.append((h, S.sim.numTotalXPBD.numpy()[0], S.sim.numSleepingXPBD.numpy()[0]))
+    (a, ta, sa), (b, tb, sb) = outs
+    for k in ("particle_x", "particle_v", "materialLabel", "particle_cumDist_xpbd"):
+        assert np.array_equal(a[k], b[k]), k
+    assert (ta, sa) == (tb, sb) and ta > 0
