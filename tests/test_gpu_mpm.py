@@ -240,12 +240,15 @@ def test_fused_substep_bitwise_and_bins():
     print(f"[fused x12] n={A.sim.nPoints}")
     # the two G2P kernels round differently at the ulp level (FMA contraction);
     # over 12 substeps of this random plastic state F drifts apart at ~1e-6 and
-    # the stress, a function of the small strain ln(F), by ~1e-6 / strain
+    # the quantities that are functions of the small strain ln(F) (stress, and
+    # the plastic increments summed into accumulated strain, damage and yield
+    # stress) by ~1e-6 / strain
+    small_strain = {"particle_stress", "particle_accumulated_strain", "particle_damage", "ys"}
     for k in a:
         same = np.array_equal(a[k], b[k])
         e = rel_err(b[k], a[k])
         print(f"  {k}: {'bitwise' if same else f'rel {e:.1e}'}")
-        assert e < (2e-3 if k == "particle_stress" else 1e-5), k
+        assert e < (2e-3 if k in small_strain else 1e-5), k
     assert SR.misbinned_count(B.sim, B.mpm) == 0
 
 
