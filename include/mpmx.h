@@ -67,6 +67,12 @@ typedef struct mpmx_particles {
     float* damage;
     float* vol;
     float* mass;
+    /* optional (may be NULL): a particle numbering, e.g. the particle's index
+     * before the arrays were permuted into spatial order (mpmx_reorder_
+     * permutation).  Inside a grid cell particles are summed in ascending
+     * order[id] (ascending id when NULL), which makes the results independent
+     * of where each particle sits in the arrays.  No reference counterpart. */
+    int32_t* order;
 } mpmx_particles;
 
 /* Grid arrays of MPMState (simStates.py:117-121). */
@@ -185,6 +191,25 @@ int mpmx_mpm_substep(const mpmx_particles* particles, const mpmx_grid* grid,
 int mpmx_substep_misbinned(void* workspace, size_t ws_bytes, int gx, int gy, int gz, int n,
                            int* host_count, void* stream);
 
+/* Spatial order of the particle arrays.  perm[n] receives a permutation that
+ * lists the active MPM particles tile by tile and cell by cell of the
+ * substep's binning (ascending particles->order inside a cell, or ascending id
+ * when order is NULL), then the out-of-grid ones, then every other particle in
+ * ascending order (or id).  Gathering every per-particle array with it
+ * (mpmx_gather_words) makes each tile's particles contiguous in HBM, so the
+ * substep's particle loads coalesce; call mpmx_substep_bin() afterwards.  Same
+ * workspace as the substep.  No reference counterpart (the reference keeps
+ * the input order). */
+int mpmx_reorder_permutation(const mpmx_particles* particles, const mpmx_grid* grid,
+                             const mpmx_step_params* params, int32_t* perm, void* workspace,
+                             size_t ws_bytes, void* stream);
+/* dst[(p*n + i)*item_words + k] = src[(p*n + perm[i])*item_words + k] for
+ * planes p < planes, items i < n, 32-bit words k < item_words: one SoA array
+ * (planes = components, item_words = 1) or an AoS one (planes = 1, item_words
+ * = words per item) permuted by perm; dst and src must not overlap. */
+int mpmx_gather_words(void* dst, const void* src, const int32_t* perm, long n, int planes,
+                      int item_words, void* stream);
+
 /* grid_normalization_and_gravity, mpmRoutines.py:691-704: writes v_out only
  * where m > 1e-15 (the caller zeroes v_out first, simulationRoutines.py:13) */
 int mpmx_grid_normalization_and_gravity(int gx, int gy, int gz, const float* grid_m,
@@ -249,6 +274,16 @@ int mpmx_hashgrid_build_label(int n, const float* x, float cell_width, int dimx,
                               int dimz, const int32_t* materialLabel, int label,
                               int32_t* point_ids, int32_t* cell_start, void* workspace,
                               size_t ws_bytes, void* stream);
+
+/* Either table above with the ids inside a cell ascending in order[id]
+ * instead of id (materialLabel NULL = all points).  With order = the
+ * particles' numbering before a spatial permutation of the arrays, the
+ * candidate order of the contact solve, and so its results, are those of the
+ * unpermuted arrays.  No reference counterpart (the reference never permutes). */
+int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dimx, int dimy,
+                                int dimz, const int32_t* materialLabel, int label,
+                                const int32_t* order, int32_t* point_ids, int32_t* cell_start,
+                                void* workspace, size_t ws_bytes, void* stream);
 
 /* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,

@@ -33,6 +33,8 @@ SIGNATURES = {
     "mpmx_substep_bin": [P, P, P, P, S, P],
     "mpmx_mpm_substep": [P, P, P, P, S, P],
     "mpmx_substep_misbinned": [P, S, I, I, I, I, IP, P],
+    "mpmx_reorder_permutation": [P, P, P, P, P, S, P],
+    "mpmx_gather_words": [P, P, P, L, I, I, P],
     "mpmx_stream_sync": [P],
     "mpmx_fill_i32": [P, L, I, P],
     "mpmx_fill_f32": [P, L, F, P],
@@ -56,6 +58,7 @@ SIGNATURES = {
     "mpmx_hashgrid_workspace_bytes": [I, I, I, I],
     "mpmx_hashgrid_build": [I, P, F, I, I, I, P, P, P, S, P],
     "mpmx_hashgrid_build_label": [I, P, F, I, I, I, P, I, P, P, P, S, P],
+    "mpmx_hashgrid_build_ordered": [I, P, F, I, I, I, P, I, P, P, P, P, S, P],
     "mpmx_reset_mpm_contact_labels": [I, P, P, P],
     "mpmx_integrate_particles_xpbd": [I, P, P, P, P, F, F, F, F, P, P, F, P],
     "mpmx_solve_particle_bound_contacts": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],

@@ -11,6 +11,9 @@
 
 #include "../../include/mpmx.h"
 
+// Math helpers are host+device so that a CPU build can exercise them.
+#define MPMX_HD __host__ __device__ __forceinline__
+
 namespace mpmx {
 
 constexpr int kBlock = 256;
@@ -19,26 +22,26 @@ struct f3 {
     float x, y, z;
 };
 
-__device__ __forceinline__ f3 ld3(const float* __restrict__ a, long n, long i) {
+MPMX_HD f3 ld3(const float* __restrict__ a, long n, long i) {
     return f3{a[i], a[n + i], a[2 * n + i]};
 }
-__device__ __forceinline__ void st3(float* __restrict__ a, long n, long i, f3 v) {
+MPMX_HD void st3(float* __restrict__ a, long n, long i, f3 v) {
     a[i] = v.x;
     a[n + i] = v.y;
     a[2 * n + i] = v.z;
 }
-__device__ __forceinline__ void ld9(const float* __restrict__ a, long n, long i, float m[9]) {
+MPMX_HD void ld9(const float* __restrict__ a, long n, long i, float m[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) m[k] = a[k * n + i];
 }
-__device__ __forceinline__ void st9(float* __restrict__ a, long n, long i, const float m[9]) {
+MPMX_HD void st9(float* __restrict__ a, long n, long i, const float m[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k * n + i] = m[k];
 }
-__device__ __forceinline__ float len3(f3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+MPMX_HD float len3(f3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
 
 // Warp's normalize(): the zero vector stays zero, otherwise divide by length.
-__device__ __forceinline__ f3 normalize3(f3 v) {
+MPMX_HD f3 normalize3(f3 v) {
     float l = len3(v);
     if (l > 0.0f) return f3{v.x / l, v.y / l, v.z / l};
     return f3{0.0f, 0.0f, 0.0f};
@@ -50,9 +53,9 @@ __device__ __forceinline__ f3 normalize3(f3 v) {
 // mpmRoutines.py:288,339).
 //
 // Method (written for near-identity deformation gradients):
-//  1. A = F^T F - I formed as H + H^T + H^T H with H = F - I.  Entries of H are
-//     exact in FP32 (Sterbenz) for |F_ii - 1| < 1/2, so A carries the strain
-//     with relative rather than absolute FP32 error.
+//  1. A = F^T F - I formed in FP64 (the products of FP32 entries are exact
+//     there) and rounded once, so A carries the strain with relative rather
+//     than absolute FP32 error even when F holds a large rotation.
 //  2. Cyclic Jacobi on A with exact rotations (Golub & Van Loan, sym.schur2),
 //     sweeping until the off-diagonal mass is below FP32 resolution.
 //  3. Sort descending with a sign flip per column swap (V stays in SO(3)).
@@ -64,7 +67,7 @@ struct Svd3 {
     float U[9], V[9], s[3], lam[3];
 };
 
-__device__ __forceinline__ void jacobi_rot(float A[9], float V[9], int p, int q) {
+MPMX_HD void jacobi_rot(float A[9], float V[9], int p, int q) {
     float apq = A[p * 3 + q];
     if (apq == 0.0f) return;
     float theta = (A[q * 3 + q] - A[p * 3 + p]) / (2.0f * apq);
@@ -92,7 +95,7 @@ __device__ __forceinline__ void jacobi_rot(float A[9], float V[9], int p, int q)
     }
 }
 
-__device__ __forceinline__ void givens_rows(float B[9], float Q[9], int p, int q, int col) {
+MPMX_HD void givens_rows(float B[9], float Q[9], int p, int q, int col) {
     float a = B[p * 3 + col], b = B[q * 3 + col];
     float r = sqrtf(a * a + b * b);
     float c = 1.0f, s = 0.0f;
@@ -114,16 +117,15 @@ __device__ __forceinline__ void givens_rows(float B[9], float Q[9], int p, int q
     }
 }
 
-__device__ __forceinline__ void svd3(const float F[9], Svd3& r) {
-    float H0 = F[0] - 1.0f, H4 = F[4] - 1.0f, H8 = F[8] - 1.0f;
-    float H[9] = {H0, F[1], F[2], F[3], H4, F[5], F[6], F[7], H8};
+MPMX_HD void svd3(const float F[9], Svd3& r) {
     float A[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = i; j < 3; j++) {
-            float htH = H[0 * 3 + i] * H[0 * 3 + j] + H[1 * 3 + i] * H[1 * 3 + j] + H[2 * 3 + i] * H[2 * 3 + j];
-            float v = H[i * 3 + j] + H[j * 3 + i] + htH;
+            const double ftf = (double)F[0 * 3 + i] * (double)F[0 * 3 + j] + (double)F[1 * 3 + i] * (double)F[1 * 3 + j] +
+                               (double)F[2 * 3 + i] * (double)F[2 * 3 + j];
+            const float v = (float)(i == j ? ftf - 1.0 : ftf);
             A[i * 3 + j] = v;
             A[j * 3 + i] = v;
         }
@@ -176,7 +178,7 @@ __device__ __forceinline__ void svd3(const float F[9], Svd3& r) {
 }
 
 // ln(max(s, floor)) with the precision of log1p for s >= floor.
-__device__ __forceinline__ float log_clamped(float s, float lam, float floorv) {
+MPMX_HD float log_clamped(float s, float lam, float floorv) {
     return (s >= floorv) ? 0.5f * log1pf(lam) : logf(floorv);
 }
 
@@ -184,7 +186,7 @@ __device__ __forceinline__ float log_clamped(float s, float lam, float floorv) {
 // Quadratic B-spline stencil (mpmRoutines.py:514-530, :733-749): base index by
 // truncation toward zero (wp.int), fx in [0.5, 1.5) for interior particles.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void bspline(float f, float w[3], float dw[3]) {
+MPMX_HD void bspline(float f, float w[3], float dw[3]) {
     float wa = 1.5f - f, wb = f - 1.0f, wc = f - 0.5f;
     w[0] = wa * wa * 0.5f;
     w[1] = (0.0f - wb * wb) + 0.75f;
@@ -195,7 +197,7 @@ __device__ __forceinline__ void bspline(float f, float w[3], float dw[3]) {
 }
 
 // Grid coordinate, base index and fractional offset of a particle on one axis.
-__device__ __forceinline__ int base_axis(float x, float minb, float inv_dx, float& fx) {
+MPMX_HD int base_axis(float x, float minb, float inv_dx, float& fx) {
     float gp = (x - minb) * inv_dx;
     int b = (int)(gp - 0.5f);
     fx = gp - (float)b;
@@ -203,7 +205,7 @@ __device__ __forceinline__ int base_axis(float x, float minb, float inv_dx, floa
 }
 
 // Coulomb friction on one tangential component (mpmRoutines.py:915-940).
-__device__ __forceinline__ float coulomb(float v, float mu, float dt, float g) {
+MPMX_HD float coulomb(float v, float mu, float dt, float g) {
     if (fabsf(v) < 1.0e-12f) return 0.0f;
     float dv = mu * g * dt;
     float s = (v < 0.0f) ? -1.0f : 1.0f;
@@ -214,7 +216,7 @@ __device__ __forceinline__ float coulomb(float v, float mu, float dt, float g) {
 // collideBounds on one node (mpmRoutines.py:942-993): v is read once, every
 // matching face overwrites the whole vector, the last match wins; lower faces
 // act for index <= pad, upper faces for index >= dim - pad.
-__device__ __forceinline__ f3 collide_node(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, float mu, float dt,
+MPMX_HD f3 collide_node(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, float mu, float dt,
                                            float g, int pad) {
     f3 o = v;
     if (ix <= pad && v.x < 0.0f) o = f3{0.0f, coulomb(v.y, mu, dt, g), coulomb(v.z, mu, dt, g)};
@@ -230,7 +232,7 @@ __device__ __forceinline__ f3 collide_node(f3 v, int ix, int iy, int iz, int gx,
 // Steps 4-6 of mpmSimulationStep on one node (simulationRoutines.py:94-140):
 // normalisation + gravity (v = 0 where m <= 1e-15), damping when the scale is
 // below 1, then the boundary condition.
-__device__ __forceinline__ f3 grid_update_node(float m, f3 vin, int ix, int iy, int iz, int gx, int gy, int gz, float gxv, float gyv,
+MPMX_HD f3 grid_update_node(float m, f3 vin, int ix, int iy, int iz, int gx, int gy, int gz, float gxv, float gyv,
                                float gzv, float dt, float damping, int bc, float fric, float gmag, int pad) {
     f3 v = f3{0.0f, 0.0f, 0.0f};
     if (m > 1.0e-15f) {

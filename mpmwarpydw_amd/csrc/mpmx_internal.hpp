@@ -45,8 +45,9 @@ int exclusive_scan_i32(const int32_t* counts, int32_t* out, long n, int32_t* par
 
 // Counting sort of particle ids by int32 key (key < 0 = not binned):
 //   count[key]++ (rank kept per id), cell_start = exclusive scan(count),
-//   sorted[cell_start[key] + rank] = id, then each key's id run is sorted ascending
-// so the result equals a stable sort of ids by key.
+//   sorted[cell_start[key] + rank] = id, then each key's id run is sorted by
+// order[id] (by id when order is null), so the result equals a stable sort by
+// key of the ids numbered by order.
 struct BinWork {
     int32_t* count;      // [nkeys]
     int32_t* cell_start; // [nkeys + 1]
@@ -55,9 +56,11 @@ struct BinWork {
 };
 size_t bin_workspace_bytes(long n, long nkeys);
 bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w);
-int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s);
+int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
+                      const int32_t* order = nullptr);
 // scan + scatter + in-run sort only (w.count and w.rank already filled)
-int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s);
+int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
+                              const int32_t* order = nullptr);
 
 }  // namespace mpmx
 

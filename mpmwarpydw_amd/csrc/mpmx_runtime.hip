@@ -193,12 +193,26 @@ int exclusive_scan_i32(const int32_t* counts, int32_t* out, long n, int32_t* par
 // ---------------------------------------------------------------------------
 // counting sort of ids by key
 // ---------------------------------------------------------------------------
-__global__ void bin_count_kernel(const int32_t* __restrict__ keys, long n, int32_t* __restrict__ count,
-                                 int32_t* __restrict__ rank) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int k = keys[i];
-    if (k >= 0) rank[i] = atomicAdd(&count[k], 1);
+// Runs of equal keys in consecutive lanes (particles stored in spatial order)
+// take one atomic per run: the run's first lane adds the run length and every
+// lane's rank is the returned base plus its offset in the run.  Ranks only
+// place ids inside their bin, which is sorted by id afterwards.
+__global__ void __launch_bounds__(kBlock) bin_count_kernel(const int32_t* __restrict__ keys, long n,
+                                                           int32_t* __restrict__ count, int32_t* __restrict__ rank) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (i < n) ? keys[i] : -1;  // every lane stays for the cross-lane operations
+    const int lane = (int)(threadIdx.x & 63);
+    const int prev = __shfl_up(k, 1);
+    const bool head = lane == 0 || prev != k;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    const int my_head = 63 - __clzll((long long)(heads & upto));
+    const unsigned long long above = heads & ~upto;
+    const int next_head = above ? __ffsll((long long)above) - 1 : 64;
+    int base = 0;
+    if (head && k >= 0) base = atomicAdd(&count[k], next_head - lane);
+    base = __shfl(base, my_head);
+    if (i < n && k >= 0) rank[i] = base + (lane - my_head);
 }
 
 __global__ void bin_scatter_kernel(const int32_t* __restrict__ keys, long n, const int32_t* __restrict__ cell_start,
@@ -209,19 +223,34 @@ __global__ void bin_scatter_kernel(const int32_t* __restrict__ keys, long n, con
     if (k >= 0) sorted[cell_start[k] + rank[i]] = (int32_t)i;
 }
 
-// ascending ids inside each key's run: the order a stable sort by key gives
-__global__ void bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start, int32_t* __restrict__ sorted) {
+// Ids inside each key's run in ascending order of order[id] (of id itself when
+// order is null): the order a stable sort by key gives when ids are numbered
+// by order.  Runs are short (particles of one cell), so insertion sort.
+__global__ void bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start, int32_t* __restrict__ sorted,
+                                     const int32_t* __restrict__ order) {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nkeys) return;
     int b = cell_start[c], e = cell_start[c + 1];
     for (int i = b + 1; i < e; i++) {
-        int v = sorted[i];
+        const int v = sorted[i];
+        const int ov = order ? order[v] : v;
         int j = i - 1;
-        while (j >= b && sorted[j] > v) {
+        while (j >= b && (order ? order[sorted[j]] : sorted[j]) > ov) {
             sorted[j + 1] = sorted[j];
             j--;
         }
         sorted[j + 1] = v;
+    }
+}
+
+// dst[(p*n + i)*w + k] = src[(p*n + perm[i])*w + k] for planes p, items i, words k
+__global__ void gather_words_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                    const int32_t* __restrict__ perm, long n, int planes, int words) {
+    const long total = n * planes;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long p = t / n, i = t - p * n;
+        const long from = (p * n + perm[i]) * words, to = (p * n + i) * words;
+        for (int k = 0; k < words; k++) dst[to + k] = src[from + k];
     }
 }
 
@@ -238,7 +267,8 @@ bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w) {
     return !cv.overflow;
 }
 
-int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s) {
+int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
+                      const int32_t* order) {
     MPMX_LAUNCH(fill_kernel<int32_t>, dim3(fill_grid(nkeys)), dim3(kBlock), 0, s, w.count, nkeys, 0);
     int rc = check_launch("bin_zero", s);
     if (rc) return rc;
@@ -253,13 +283,14 @@ int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, 
         if ((rc = check_launch("bin_scatter", s))) return rc;
     }
     MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
-                       sorted);
+                       sorted, order);
     return check_launch("bin_sort_runs", s);
 }
 
 // The scan / scatter / in-run sort half of counting_sort_ids, for callers that
 // produced keys, ranks and counts themselves.
-int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s) {
+int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
+                              const int32_t* order) {
     int rc = exclusive_scan_i32(w.count, w.cell_start, nkeys, w.partials, s);
     if (rc) return rc;
     if (n > 0) {
@@ -268,7 +299,7 @@ int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* 
         if ((rc = check_launch("bin_scatter", s))) return rc;
     }
     MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
-                       sorted);
+                       sorted, order);
     return check_launch("bin_sort_runs", s);
 }
 
@@ -279,6 +310,20 @@ using namespace mpmx;
 extern "C" {
 
 const char* mpmx_last_error(void) { return g_err; }
+
+int mpmx_gather_words(void* dst, const void* src, const int32_t* perm, long n, int planes, int item_words,
+                      void* stream) {
+    MPMX_REQUIRE(n >= 0 && planes >= 1 && item_words >= 1, "mpmx_gather_words: bad sizes");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(dst && src && perm && dst != src, "mpmx_gather_words: null or aliased pointer");
+    const long total = n * planes;
+    long nb = (total + kBlock - 1) / kBlock;
+    if (nb > 65536) nb = 65536;
+    hipStream_t s = as_stream(stream);
+    MPMX_LAUNCH(gather_words_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, (uint32_t*)dst, (const uint32_t*)src, perm,
+                n, planes, item_words);
+    return check_launch("gather_words", s);
+}
 
 int mpmx_profile_begin(int capacity) {
     MPMX_REQUIRE(capacity > 0 && capacity <= (1 << 22), "mpmx_profile_begin: capacity out of range");

@@ -39,6 +39,15 @@ constexpr int kRec = 25;                     // floats per staged particle recor
 constexpr int kChunk = 256;                  // particles staged per LDS round
 constexpr int kTasks = kTileCells * 9;       // (cell, i, j)
 
+// Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on
+// XCD b % 8), and every XCD has its own L2.  Neighbouring tiles share cache
+// lines of the particle and grid arrays, so tiles are handed out in 8
+// contiguous ranges, one per XCD: block b works on tile xcd_tile(b, nb).
+MPMX_HD int xcd_tile(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, j = b / 8;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 struct TileGeom {
     int gx, gy, gz;     // grid nodes
     int ntx, nty, ntz;  // tiles
@@ -116,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
     __shared__ float4 part[kTileCells * 27];
     __shared__ int cs[kTileCells + 1];
 
-    const int t = blockIdx.x;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     if (FUSED && t == a.ntiles) {
         orphan_bin(a);

@@ -33,7 +33,9 @@ struct SubstepWork {
     BinWork bin;
 };
 
-static long substep_nkeys(const TileGeom& tg) { return tg.ntiles() * kTileCells + 1; }  // + orphan bin
+// keys: tile * 64 + cell, then the orphan bin (out-of-grid MPM particles), then
+// a bin used only by the spatial reorder for every particle that is not MPM
+static long substep_nkeys(const TileGeom& tg) { return tg.ntiles() * kTileCells + 2; }
 
 static size_t substep_ws_bytes(long n, const TileGeom& tg) {
     long nn = n > 0 ? n : 1;
@@ -68,6 +70,20 @@ __global__ void substep_key_kernel(mpmx_particles P, float minx, float miny, flo
     keys[p] = key;
 }
 
+// key of every particle for the spatial reorder: active MPM particles as in
+// substep_key_kernel, everything else in the last bin
+__global__ void reorder_key_kernel(mpmx_particles P, float minx, float miny, float minz, float inv_dx, TileGeom tg,
+                                   int32_t* __restrict__ keys) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    int key = (int)(tg.ntiles() * kTileCells + 1);
+    if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
+        key = tile_key(ld3(P.x, P.n, p), minx, miny, minz, inv_dx, tg);
+        if (key < 0) key = (int)(tg.ntiles() * kTileCells);
+    }
+    keys[p] = key;
+}
+
 struct G2PArgs {
     mpmx_particles P;
     const int32_t* sorted;
@@ -87,13 +103,109 @@ __device__ __forceinline__ void bin_next(const G2PArgs& a, long id, int key) {
     if (key >= 0) a.rank[id] = atomicAdd(&a.count[key], 1);
 }
 
-// G2P (mpmRoutines.py:715-802) of the particles of one tile, grid velocities
-// from LDS; F_trial = (I + dt G) F is evaluated as F + dt (G F).
-__global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
-    __shared__ float gvs[3][kTileNodes];
+// Grid velocity of node (b0+i, b1+j, b2+k) relative to the tile origin: from the
+// tile's LDS block, or from HBM for a particle outside its tile's node block.
+template <bool LDS>
+__device__ __forceinline__ f3 g2p_node(const G2PArgs& a, const float (*gvs)[kTileNodes], int row, long grow, long G) {
+    if (LDS) return f3{gvs[0][row], gvs[1][row], gvs[2][row]};
+    return f3{a.v_out[grow], a.v_out[G + grow], a.v_out[2 * G + grow]};
+}
+
+// G2P of one particle (mpmRoutines.py:715-802): new v, x, C, F_trial (and the
+// covariance when it is tracked); returns the particle's key for the next substep.
+template <bool LDS>
+__device__ __forceinline__ int g2p_one(const G2PArgs& a, const float (*gvs)[kTileNodes], int id, f3 xp, int b0, int b1,
+                                       int b2, int ox, int oy, int oz, const float fx[3]) {
     const mpmx_particles& P = a.P;
     const long n = P.n;
-    const int t = blockIdx.x;
+    const TileGeom& tg = a.tg;
+    const long G = (long)tg.gx * tg.gy * tg.gz;
+    const float inv_dx = a.inv_dx, dt = a.dt;
+    float w[3][3], dw[3][3];
+    bspline(fx[0], w[0], dw[0]);
+    bspline(fx[1], w[1], dw[1]);
+    bspline(fx[2], w[2], dw[2]);
+    float nv[3] = {0.0f, 0.0f, 0.0f};
+    float nB[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float nG[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float wxy = w[0][i] * w[1][j];
+            const float gx_ = dw[0][i] * w[1][j] * inv_dx;
+            const float gy_ = w[0][i] * dw[1][j] * inv_dx;
+            const int row = ((b0 + i) * kTN + (b1 + j)) * kTN + b2;
+            const long grow = ((long)(b0 + ox + i) * tg.gy + (b1 + oy + j)) * tg.gz + (b2 + oz);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const f3 gv = g2p_node<LDS>(a, gvs, row + k, grow + k, G);
+                const float vx = gv.x, vy = gv.y, vz = gv.z;
+                const float wt = wxy * w[2][k];
+                const float d0 = (float)i - fx[0], d1 = (float)j - fx[1], d2 = (float)k - fx[2];
+                const float g0 = gx_ * w[2][k], g1 = gy_ * w[2][k], g2 = wxy * dw[2][k] * inv_dx;
+                nv[0] += vx * wt;
+                nv[1] += vy * wt;
+                nv[2] += vz * wt;
+                const float wvx = vx * wt, wvy = vy * wt, wvz = vz * wt;
+                nB[0] += wvx * d0; nB[1] += wvx * d1; nB[2] += wvx * d2;
+                nB[3] += wvy * d0; nB[4] += wvy * d1; nB[5] += wvy * d2;
+                nB[6] += wvz * d0; nB[7] += wvz * d1; nB[8] += wvz * d2;
+                nG[0] += vx * g0; nG[1] += vx * g1; nG[2] += vx * g2;
+                nG[3] += vy * g0; nG[4] += vy * g1; nG[5] += vy * g2;
+                nG[6] += vz * g0; nG[7] += vz * g1; nG[8] += vz * g2;
+            }
+        }
+    }
+    const f3 xn = f3{xp.x + dt * nv[0], xp.y + dt * nv[1], xp.z + dt * nv[2]};
+    st3(P.v, n, id, f3{nv[0], nv[1], nv[2]});
+    st3(P.x, n, id, xn);
+    const float sC = inv_dx * 4.0f;
+    float Cn[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) Cn[k] = nB[k] * sC;
+    st9(P.C, n, id, Cn);
+    float Fp[9], Fn[9];
+    ld9(P.F, n, id, Fp);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            Fn[i * 3 + j] = Fp[i * 3 + j] + dt * (nG[i * 3 + 0] * Fp[0 * 3 + j] + nG[i * 3 + 1] * Fp[1 * 3 + j] +
+                                                  nG[i * 3 + 2] * Fp[2 * 3 + j]);
+    st9(P.F_trial, n, id, Fn);
+    if (a.update_cov) {
+        float* c6 = P.cov + 6L * id;
+        float cn[9] = {c6[0], c6[1], c6[2], c6[1], c6[3], c6[4], c6[2], c6[4], c6[5]};
+        float c1[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                float gc = nG[i * 3 + 0] * cn[0 * 3 + j] + nG[i * 3 + 1] * cn[1 * 3 + j] + nG[i * 3 + 2] * cn[2 * 3 + j];
+                float cg = cn[i * 3 + 0] * nG[j * 3 + 0] + cn[i * 3 + 1] * nG[j * 3 + 1] + cn[i * 3 + 2] * nG[j * 3 + 2];
+                c1[i * 3 + j] = cn[i * 3 + j] + dt * (gc + cg);
+            }
+        c6[0] = c1[0]; c6[1] = c1[1]; c6[2] = c1[2]; c6[3] = c1[4]; c6[4] = c1[5]; c6[5] = c1[8];
+    }
+    const int key = tile_key(xn, a.minx, a.miny, a.minz, inv_dx, tg);
+    return key < 0 ? (int)(tg.ntiles() * kTileCells) : key;
+}
+
+// G2P per tile with the tile's grid velocities staged in LDS; F_trial = (I + dt G) F
+// is evaluated as F + dt (G F).  The particles then count themselves into the
+// next substep's bins: a particle that stays in this tile takes its rank from an
+// LDS counter, and once per round of 256 particles each cell of the tile adds
+// its LDS count to the global count with one atomic, whose return value is the
+// base of the round's ranks.  Particles that leave the tile count globally.  The
+// ranks only place ids inside their bin; the bin is sorted by id afterwards.
+__global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
+    __shared__ float gvs[3][kTileNodes];
+    __shared__ int lcnt[kTileCells];
+    __shared__ int lbase[kTileCells];
+    const mpmx_particles& P = a.P;
+    const long n = P.n;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const TileGeom& tg = a.tg;
     const long ntiles = tg.ntiles();
     const int orphan_key = (int)(ntiles * kTileCells);
@@ -125,106 +237,46 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
         gvs[1][node] = v1;
         gvs[2][node] = v2;
     }
+    if (threadIdx.x < kTileCells) lcnt[threadIdx.x] = 0;
     __syncthreads();
-    const float inv_dx = a.inv_dx, dt = a.dt;
-    for (long s = p0 + threadIdx.x; s < p1; s += kBlock) {
-        const int id = a.sorted[s];
-        if (P.materialLabel[id] > 1) {  // became XPBD in this substep's constitutive update
-            bin_next(a, id, -1);
-            continue;
-        }
-        f3 xp = ld3(P.x, n, id);
-        float fx[3];
-        const int b0 = base_axis(xp.x, a.minx, inv_dx, fx[0]) - ox;
-        const int b1 = base_axis(xp.y, a.miny, inv_dx, fx[1]) - oy;
-        const int b2 = base_axis(xp.z, a.minz, inv_dx, fx[2]) - oz;
-        float w[3][3], dw[3][3];
-        bspline(fx[0], w[0], dw[0]);
-        bspline(fx[1], w[1], dw[1]);
-        bspline(fx[2], w[2], dw[2]);
-        // a particle outside its tile's node block (stale bins) reads the grid from HBM instead
-        const bool in_tile = b0 >= 0 && b1 >= 0 && b2 >= 0 && b0 < kTB && b1 < kTB && b2 < kTB;
-        if (!in_tile) {
-            if (a.misbinned) atomicAdd(a.misbinned, 1);
-            if (b0 + ox < 0 || b1 + oy < 0 || b2 + oz < 0 || b0 + ox > tg.gx - 3 || b1 + oy > tg.gy - 3 ||
-                b2 + oz > tg.gz - 3) {
-                bin_next(a, id, orphan_key);
-                continue;
-            }
-        }
-        float nv[3] = {0.0f, 0.0f, 0.0f};
-        float nB[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        float nG[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const float wxy = w[0][i] * w[1][j];
-                const float gx_ = dw[0][i] * w[1][j] * inv_dx;
-                const float gy_ = w[0][i] * dw[1][j] * inv_dx;
-                const int row = ((b0 + i) * kTN + (b1 + j)) * kTN + b2;
-                const long grow = ((long)(b0 + ox + i) * tg.gy + (b1 + oy + j)) * tg.gz + (b2 + oz);
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    float vx, vy, vz;
-                    if (in_tile) {
-                        vx = gvs[0][row + k];
-                        vy = gvs[1][row + k];
-                        vz = gvs[2][row + k];
-                    } else {
-                        vx = a.v_out[grow + k];
-                        vy = a.v_out[G + grow + k];
-                        vz = a.v_out[2 * G + grow + k];
-                    }
-                    const float wt = wxy * w[2][k];
-                    const float d0 = (float)i - fx[0], d1 = (float)j - fx[1], d2 = (float)k - fx[2];
-                    const float g0 = gx_ * w[2][k], g1 = gy_ * w[2][k], g2 = wxy * dw[2][k] * inv_dx;
-                    nv[0] += vx * wt;
-                    nv[1] += vy * wt;
-                    nv[2] += vz * wt;
-                    const float wvx = vx * wt, wvy = vy * wt, wvz = vz * wt;
-                    nB[0] += wvx * d0; nB[1] += wvx * d1; nB[2] += wvx * d2;
-                    nB[3] += wvy * d0; nB[4] += wvy * d1; nB[5] += wvy * d2;
-                    nB[6] += wvz * d0; nB[7] += wvz * d1; nB[8] += wvz * d2;
-                    nG[0] += vx * g0; nG[1] += vx * g1; nG[2] += vx * g2;
-                    nG[3] += vy * g0; nG[4] += vy * g1; nG[5] += vy * g2;
-                    nG[6] += vz * g0; nG[7] += vz * g1; nG[8] += vz * g2;
+    const int key0 = t * kTileCells;
+    for (long base = p0; base < p1; base += kBlock) {  // uniform trip count: the barriers below are block-wide
+        const long s = base + threadIdx.x;
+        int id = -1, key = -1, lrank = -1;
+        if (s < p1) {
+            id = a.sorted[s];
+            if (P.materialLabel[id] <= 1) {  // else it became XPBD in this substep's constitutive update
+                const f3 xp = ld3(P.x, n, id);
+                float fx[3];
+                const int b0 = base_axis(xp.x, a.minx, a.inv_dx, fx[0]) - ox;
+                const int b1 = base_axis(xp.y, a.miny, a.inv_dx, fx[1]) - oy;
+                const int b2 = base_axis(xp.z, a.minz, a.inv_dx, fx[2]) - oz;
+                if (b0 >= 0 && b1 >= 0 && b2 >= 0 && b0 < kTB && b1 < kTB && b2 < kTB) {
+                    key = g2p_one<true>(a, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
+                } else {
+                    // stale bin: the particle is outside its tile's node block
+                    if (a.misbinned) atomicAdd(a.misbinned, 1);
+                    if (b0 + ox < 0 || b1 + oy < 0 || b2 + oz < 0 || b0 + ox > tg.gx - 3 || b1 + oy > tg.gy - 3 ||
+                        b2 + oz > tg.gz - 3)
+                        key = orphan_key;  // its stencil leaves the grid: no transfer
+                    else
+                        key = g2p_one<false>(a, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
                 }
             }
+            a.keys[id] = key;
+            if (key >= key0 && key < key0 + kTileCells)
+                lrank = atomicAdd(&lcnt[key - key0], 1);
+            else if (key >= 0)
+                a.rank[id] = atomicAdd(&a.count[key], 1);
         }
-        const f3 xn = f3{xp.x + dt * nv[0], xp.y + dt * nv[1], xp.z + dt * nv[2]};
-        st3(P.v, n, id, f3{nv[0], nv[1], nv[2]});
-        st3(P.x, n, id, xn);
-        const float sC = inv_dx * 4.0f;
-        float Cn[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) Cn[k] = nB[k] * sC;
-        st9(P.C, n, id, Cn);
-        float Fp[9], Fn[9];
-        ld9(P.F, n, id, Fp);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                Fn[i * 3 + j] = Fp[i * 3 + j] + dt * (nG[i * 3 + 0] * Fp[0 * 3 + j] + nG[i * 3 + 1] * Fp[1 * 3 + j] +
-                                                      nG[i * 3 + 2] * Fp[2 * 3 + j]);
-        st9(P.F_trial, n, id, Fn);
-        if (a.update_cov) {
-            float* c6 = P.cov + 6L * id;
-            float cn[9] = {c6[0], c6[1], c6[2], c6[1], c6[3], c6[4], c6[2], c6[4], c6[5]};
-            float c1[9];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    float gc = nG[i * 3 + 0] * cn[0 * 3 + j] + nG[i * 3 + 1] * cn[1 * 3 + j] + nG[i * 3 + 2] * cn[2 * 3 + j];
-                    float cg = cn[i * 3 + 0] * nG[j * 3 + 0] + cn[i * 3 + 1] * nG[j * 3 + 1] + cn[i * 3 + 2] * nG[j * 3 + 2];
-                    c1[i * 3 + j] = cn[i * 3 + j] + dt * (gc + cg);
-                }
-            c6[0] = c1[0]; c6[1] = c1[1]; c6[2] = c1[2]; c6[3] = c1[4]; c6[4] = c1[5]; c6[5] = c1[8];
+        __syncthreads();
+        if (threadIdx.x < kTileCells) {
+            const int c = lcnt[threadIdx.x];
+            lbase[threadIdx.x] = c ? atomicAdd(&a.count[key0 + threadIdx.x], c) : 0;
+            lcnt[threadIdx.x] = 0;
         }
-        int key = tile_key(xn, a.minx, a.miny, a.minz, inv_dx, tg);
-        bin_next(a, id, key < 0 ? orphan_key : key);
+        __syncthreads();
+        if (lrank >= 0) a.rank[id] = lbase[key - key0] + lrank;
     }
 }
 
@@ -270,7 +322,25 @@ int mpmx_substep_bin(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     }
     MPMX_LAUNCH(zero_one_kernel, dim3(1), dim3(1), 0, s, w.misbinned);
     if ((rc = check_launch("substep_clear", s))) return rc;
-    return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), w.sorted, w.bin, s);
+    return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), w.sorted, w.bin, s, P->order);
+}
+
+int mpmx_reorder_permutation(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm, int32_t* perm,
+                             void* workspace, size_t ws_bytes, void* stream) {
+    int rc = check_args(P, G, prm);
+    if (rc) return rc;
+    MPMX_REQUIRE(P->n == 0 || perm, "mpmx_reorder_permutation: null perm");
+    if (P->n == 0) return 0;
+    hipStream_t s = as_stream(stream);
+    TileGeom tg = make_geom(G->gx, G->gy, G->gz);
+    MPMX_REQUIRE(substep_nkeys(tg) < (1L << 31), "substep: grid too large for int32 keys");
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, tg), "mpmx_reorder_permutation: workspace too small");
+    SubstepWork w;
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "mpmx_reorder_permutation: workspace carve failed");
+    MPMX_LAUNCH(reorder_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->minx, prm->miny, prm->minz,
+                prm->inv_dx, tg, w.keys);
+    if ((rc = check_launch("reorder_key", s))) return rc;
+    return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), perm, w.bin, s, P->order);
 }
 
 int mpmx_substep_misbinned(void* workspace, size_t ws_bytes, int gx, int gy, int gz, int n, int* host_count,
@@ -358,7 +428,7 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
     if ((rc = check_launch("substep_g2p", s))) return rc;
 
-    return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s);
+    return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s, P->order);
 }
 
 }  // extern "C"

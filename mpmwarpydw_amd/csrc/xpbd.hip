@@ -133,6 +133,8 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     const int32_t* ids = point_ids;
     const int32_t* starts = cell_start;
     if (xp_ids && mat[i] <= 1) {  // labels 0/1 stay MPM while this kernel runs (lock writes only 0)
+        // an MPM particle only meets XPBD particles: nothing to do when there are none
+        if (xp_start[(long)dimx * dimy * dimz] == 0) return;
         ids = xp_ids;
         starts = xp_start;
     }
@@ -288,7 +290,7 @@ size_t mpmx_hashgrid_workspace_bytes(int n, int dimx, int dimy, int dimz) {
 
 static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
                           const int32_t* labels, int label, int32_t* point_ids, int32_t* cell_start, void* workspace,
-                          size_t ws_bytes, void* stream) {
+                          size_t ws_bytes, void* stream, const int32_t* order = nullptr) {
     MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f && cell_start,
                  "mpmx_hashgrid_build: bad arguments");
     MPMX_REQUIRE(n == 0 || (x && point_ids), "mpmx_hashgrid_build: null pointer");
@@ -309,7 +311,7 @@ static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int
         int rc = check_launch("hash_key", s);
         if (rc) return rc;
     }
-    return counting_sort_ids(keys, n, nkeys, point_ids, w, s);
+    return counting_sort_ids(keys, n, nkeys, point_ids, w, s, order);
 }
 
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
@@ -324,6 +326,13 @@ int mpmx_hashgrid_build_label(int n, const float* x, float cell_width, int dimx,
     MPMX_REQUIRE(n == 0 || materialLabel, "mpmx_hashgrid_build_label: null label array");
     return hashgrid_build(n, x, cell_width, dimx, dimy, dimz, materialLabel, label, point_ids, cell_start, workspace,
                           ws_bytes, stream);
+}
+
+int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+                                const int32_t* materialLabel, int label, const int32_t* order, int32_t* point_ids,
+                                int32_t* cell_start, void* workspace, size_t ws_bytes, void* stream) {
+    return hashgrid_build(n, x, cell_width, dimx, dimy, dimz, materialLabel, label, point_ids, cell_start, workspace,
+                          ws_bytes, stream, order);
 }
 
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel, void* stream) {

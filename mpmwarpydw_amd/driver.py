@@ -13,6 +13,7 @@ residual, damage stall); its reductions are FP64 on the device.
 """
 from __future__ import annotations
 
+import ctypes as C
 import time
 from dataclasses import dataclass, field
 
@@ -20,6 +21,7 @@ import numpy as np
 
 from . import mpmRoutines, simulationRoutines
 from . import runtime as rt
+from ._lib import lib
 from .simStates import MPMState, SimState, XPBDState
 
 MPM_PADDING = 2  # runMPMYDW.py:36
@@ -93,6 +95,35 @@ class Simulation:
         (SR.mpmSimulationStepFused if self.fused else SR.mpmSimulationStep)(self.sim, self.mpm, self.xpbd)
         if counter % self.sim.mpmStepsPerXpbdStep == 0 and counter > 0:
             (SR.xpbdSimulationStepFused if self.fused else SR.xpbdSimulationStep)(self.sim, self.mpm, self.xpbd)
+
+    def reorder(self) -> None:
+        """Permute every per-particle array into the fused substep's tile order
+        (tile by tile, cell by cell, particle id ascending inside a cell), so
+        that each tile's particles are contiguous in HBM and the substep's loads
+        coalesce.  ``sim.particle_id`` follows along and keeps each particle's
+        index in the input; the bins and hash tables order particles by it, so
+        a reordered run gives the same bits as one that never reorders (the
+        FP64 convergence sums aside, whose order follows the arrays)."""
+        SR = simulationRoutines
+        sim, mpm, xpbd = self.sim, self.mpm, self.xpbd
+        n = sim.nPoints
+        P, G, prm = SR.step_structs(sim, mpm, xpbd)
+        ws, nb = SR.substep_workspace(sim, mpm)
+        perm = rt.zeros(n, rt.int32)
+        lib("mpmx_reorder_permutation", C.byref(P), C.byref(G), C.byref(prm), perm.ptr, ws, nb, rt.stream_handle())
+        seen = set()
+        for obj in (sim, mpm, xpbd):
+            for a in list(vars(obj).values()):
+                if not isinstance(a, rt.Array) or id(a) in seen or a.dtype == rt.float64 or len(a.shape) != 1:
+                    continue
+                if a.shape[0] == n:
+                    a.permute_(perm)
+                elif a.shape[0] == 6 * n:  # the covariance arrays hold 6 floats per particle
+                    a.permute_(perm, 6)
+                else:
+                    continue
+                seen.add(id(a))
+        mpm._bins = None
 
     def residual(self) -> float:
         """sum |v|/r / N_active over active particles (runMPMYDW.py:686-708): one readback."""
@@ -187,6 +218,7 @@ def setup(args, domain: dict, fused: bool = True) -> Simulation:
     sim.particle_vol = rt.array(vol.astype(np.float32))
     sim.particle_mass = rt.zeros(n)
     sim.particleBCMask = rt.zeros(n, rt.int32)
+    sim.particle_id = rt.array(np.arange(n, dtype=np.int32), rt.int32)  # input index (Simulation.reorder)
     rt.launch(mpmRoutines.get_float_array_product, n, inputs=[sim.particle_density, sim.particle_vol,
                                                               sim.particle_mass])
 

@@ -118,6 +118,22 @@ class Array:
             lib("mpmx_fill_f32", self.ptr, self.size * nc, float(value), stream_handle())
         return self
 
+    def permute_(self, perm: "Array", item_words: int = 1) -> "Array":
+        """Reorder the items by a device permutation (new[i] = old[perm[i]]):
+        SoA components as planes, ``item_words`` 32-bit words per item."""
+        if self.dtype == float64:
+            raise MpmxError("permute_: float64 arrays are not per-particle state")
+        nc = _NCOMP[self.dtype]
+        n = self.size // item_words
+        if perm.size != n:
+            raise MpmxError(f"permute_: permutation of {perm.size} items for an array of {n}")
+        new = torch.empty_like(self.t)
+        lib("mpmx_gather_words", C.c_void_p(new.data_ptr()), self.ptr, perm.ptr, n, nc, int(item_words),
+            stream_handle())
+        self.t = new
+        self.version += 1
+        return self
+
     def __repr__(self):
         return f"Array(shape={self.shape}, dtype={self.dtype})"
 
@@ -214,8 +230,10 @@ class HashGrid:
     def id(self):
         return self
 
-    def build(self, points: Array, cell_width: float, labels: Array | None = None, label: int = 2) -> None:
-        """Bin all points, or (with ``labels``) only those whose label equals ``label``."""
+    def build(self, points: Array, cell_width: float, labels: Array | None = None, label: int = 2,
+              order: Array | None = None) -> None:
+        """Bin all points, or (with ``labels``) only those whose label equals ``label``.
+        With ``order`` the ids inside a cell ascend in order[id] instead of id."""
         n = points.shape[0]
         if self.point_ids is None or self.point_ids.shape[0] != n:
             self.point_ids = zeros(n, int32)
@@ -223,7 +241,11 @@ class HashGrid:
         self.cell_width = float(cell_width)
         dx, dy, dz = self.dims
         ws, nb = self._ws.get(lib.raw("mpmx_hashgrid_workspace_bytes")(n, dx, dy, dz))
-        if labels is None:
+        if order is not None:
+            lib("mpmx_hashgrid_build_ordered", n, points.ptr, self.cell_width, dx, dy, dz,
+                None if labels is None else labels.ptr, int(label), order.ptr, self.point_ids.ptr,
+                self.cell_start.ptr, ws, nb, stream_handle())
+        elif labels is None:
             lib("mpmx_hashgrid_build", n, points.ptr, self.cell_width, dx, dy, dz, self.point_ids.ptr,
                 self.cell_start.ptr, ws, nb, stream_handle())
         else:

@@ -127,7 +127,8 @@ def step_structs(sim, mpm, xpbd):
                   density=_aptr(sim.particle_density), strainCriteria=_aptr(mpm.strainCriteria),
                   eta_shear=_aptr(mpm.eta_shear), eta_bulk=_aptr(mpm.eta_bulk),
                   acc_strain=_aptr(mpm.particle_accumulated_strain), damage=_aptr(mpm.particle_damage),
-                  vol=_aptr(sim.particle_vol), mass=_aptr(sim.particle_mass))
+                  vol=_aptr(sim.particle_vol), mass=_aptr(sim.particle_mass),
+                  order=_aptr(getattr(sim, "particle_id", None)))
     gx, gy, gz = mpm.gridDims
     G = Grid(gx=gx, gy=gy, gz=gz, m=_aptr(mpm.grid_m), v_in=_aptr(mpm.grid_v_in), v_out=_aptr(mpm.grid_v_out))
     g = sim.gravity
@@ -190,9 +191,10 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=
     it (same candidates in the same order, so the same results; see
     mpmx_solve_particle_particle_contacts_split)."""
     n = sim.nPoints
-    xpbd.particle_grid.build(sim.particle_x, sim.dx)
+    order = getattr(sim, "particle_id", None)
+    xpbd.particle_grid.build(sim.particle_x, sim.dx, order=order)
     if split_grid:
-        xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2)
+        xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2, order=order)
     rt.copy(xpbd.particle_x_integrated, sim.particle_x)
     rt.copy(xpbd.particle_v_integrated, sim.particle_v)
     launch(kernel=xpbdRoutines.integrateParticlesXPBD, dim=n,
@@ -258,8 +260,9 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     particle_v_deltaInt and particle_v_before_contact are not written (nothing
     outside the step reads them)."""
     n = sim.nPoints
-    xpbd.particle_grid.build(sim.particle_x, sim.dx)
-    xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2)
+    order = getattr(sim, "particle_id", None)
+    xpbd.particle_grid.build(sim.particle_x, sim.dx, order=order)
+    xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2, order=order)
     g = sim.gravity
     s = stream_handle()
     reset = 1 if (xpbd.mpm_contact_transition_lock and not xpbd_only) else 0

@@ -243,3 +243,93 @@ class Oracle:
         cnt = C.c_int(0)
         s = self.lib.o_sum_active(_I(len(arr)), _ptr(self.f(arr)), _ptr(self.i(activeLabel)), C.byref(cnt))
         return s, cnt.value
+
+
+class MLPOracle:
+    """FP64 CPU oracle of the learned return map (oracle/mlp_oracle.c):
+    the 10-64-64-3 GELU network's forward pass, mean-squared-error gradient and
+    Adam step, with optional bfloat16 rounding at the points where the GPU
+    kernels feed their matrix units (``quant=True``), plus the features,
+    exact targets and judge of the learned projection."""
+
+    NI, NH, NO = 10, 64, 3
+
+    def __init__(self):
+        path = os.path.join(HERE, "liboracle_mlp.so")
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        self.lib.o_bf16.restype = C.c_double
+        self.lib.o_bf16.argtypes = [C.c_double]
+        self.lib.o_gelu.restype = C.c_double
+        self.lib.o_gelu.argtypes = [C.c_double]
+        self.lib.o_gelu_grad.restype = C.c_double
+        self.lib.o_gelu_grad.argtypes = [C.c_double]
+        self.lib.o_mlp_loss_grad.restype = C.c_double
+        self.nparams = int(self.lib.o_mlp_nparams())
+
+    @staticmethod
+    def _d(a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def bf16(self, v):
+        return self.lib.o_bf16(float(v))
+
+    def gelu(self, v):
+        return self.lib.o_gelu(float(v))
+
+    def gelu_grad(self, v):
+        return self.lib.o_gelu_grad(float(v))
+
+    def split(self, params):
+        """Views W1 (64,10), b1, W2 (64,64), b2, W3 (3,64), b3 of a flat parameter vector."""
+        p = np.asarray(params)
+        o = 0
+        out = []
+        for shape in ((64, 10), (64,), (64, 64), (64,), (3, 64), (3,)):
+            k = int(np.prod(shape))
+            out.append(p[o:o + k].reshape(shape))
+            o += k
+        return out
+
+    def forward(self, params, X, quant=False):
+        X = self._d(X)
+        params = self._d(params)
+        Y = np.zeros((X.shape[0], 3))
+        self.lib.o_mlp_forward(_I(X.shape[0]), _ptr(params), _ptr(X), _ptr(Y), _I(int(quant)))
+        return Y
+
+    def loss_grad(self, params, X, T, quant=False):
+        X, T, params = self._d(X), self._d(T), self._d(params)
+        g = np.zeros(self.nparams)
+        loss = self.lib.o_mlp_loss_grad(_I(X.shape[0]), _ptr(params), _ptr(X), _ptr(T), _ptr(g), _I(int(quant)))
+        return loss, g
+
+    def adam_step(self, params, grad, m, v, lr, beta1, beta2, eps, t):
+        """In place on params, m, v (float64 arrays)."""
+        for a in (params, m, v):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        grad = self._d(grad)
+        self.lib.o_adam_step(_I(len(params)), _ptr(params), _ptr(grad), _ptr(m), _ptr(v), _D(lr), _D(beta1),
+                             _D(beta2), _D(eps), _I(t))
+
+    def features(self, e, mu, lam, ys, alpha, acc, damage):
+        n = len(mu)
+        X = np.zeros((n, 10))
+        self.lib.o_lrm_features(_I(n), *[_ptr(self._d(a)) for a in (e, mu, lam, ys, alpha, acc, damage)], _ptr(X))
+        return X
+
+    def targets(self, e, mu, lam, ys, alpha, damage):
+        n = len(mu)
+        T = np.zeros((n, 3))
+        plastic = np.zeros(n, np.int32)
+        self.lib.o_lrm_targets(_I(n), *[_ptr(self._d(a)) for a in (e, mu, lam, ys, alpha, damage)], _ptr(T),
+                               _ptr(plastic))
+        return T, plastic.astype(bool)
+
+    def judge(self, e, e_pred, mu, lam, ys, alpha, damage, tol=1e-4):
+        n = len(mu)
+        acc = np.zeros(n, np.int32)
+        self.lib.o_lrm_judge(_I(n), *[_ptr(self._d(a)) for a in (e, e_pred, mu, lam, ys, alpha, damage)], _D(tol),
+                             _ptr(acc))
+        return acc.astype(bool)

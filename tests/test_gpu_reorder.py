@@ -1,0 +1,50 @@
+"""Spatial reorder of the particle arrays (Simulation.reorder) changes where each
+particle sits in HBM and nothing else: bins and hash tables order particles by
+their input index (sim.particle_id), so a run that reorders gives the bits of
+one that never does, once its arrays are put back in input order."""
+import numpy as np
+import pytest
+
+from state_utils import F_FIELDS_MPM, F_FIELDS_SIM, F_FIELDS_XPBD, make_sim
+
+pytestmark = pytest.mark.gpu
+
+GRID = {"grid_m", "grid_v_in", "grid_v_out"}
+
+
+def _arrays(S):
+    out = {}
+    for obj, names in ((S.sim, F_FIELDS_SIM), (S.mpm, F_FIELDS_MPM), (S.xpbd, F_FIELDS_XPBD)):
+        for k in names:
+            out[k] = getattr(obj, k).numpy()
+    return out
+
+
+def test_reorder_is_a_pure_layout_change():
+    A = make_sim(seed=21)
+    B = make_sim(seed=21)
+    n = A.sim.nPoints
+    ratio = A.sim.mpmStepsPerXpbdStep
+    print(f"[reorder] n={n} ratio={ratio}", flush=True)
+    B.reorder()
+    pid = B.sim.particle_id.numpy()
+    assert np.array_equal(np.sort(pid), np.arange(n))
+    assert not np.array_equal(pid, np.arange(n))  # the random state is not in tile order
+    for counter in range(1, ratio + 4):
+        if counter == 7:
+            B.reorder()
+        A.substep(counter)
+        B.substep(counter)
+    a, b = _arrays(A), _arrays(B)
+    pid = B.sim.particle_id.numpy()
+    for k in a:
+        if k in GRID:
+            got = b[k]
+        elif k == "particle_cov":
+            got = np.empty_like(b[k]).reshape(n, 6)
+            got[pid] = b[k].reshape(n, 6)
+            got = got.reshape(-1)
+        else:
+            got = np.empty_like(b[k])
+            got[pid] = b[k]
+        assert np.array_equal(got, a[k]), k
