@@ -101,6 +101,11 @@ typedef struct mpmx_step_params {
 
 const char* mpmx_last_error(void);
 int mpmx_version(void);
+/* Layout of the structs above as this build sees them, for bindings to check
+ * theirs against: out[0..] = sizeof(mpmx_particles), offsetof(mpmx_particles,
+ * order), sizeof(mpmx_grid), sizeof(mpmx_step_params), offsetof(
+ * mpmx_step_params, bc); returns the number of values written (<= cap). */
+int mpmx_struct_layout(long* out, int cap);
 /* Launch profiler (no counterpart in the reference; replaces its wall-clock
  * prints, runMPMYDW.py:762): between begin and end, events are recorded on the
  * launch stream just before and just after every kernel the library issues

@@ -25,6 +25,7 @@ IP = C.POINTER(C.c_int)
 SIGNATURES = {
     "mpmx_last_error": [],
     "mpmx_version": [],
+    "mpmx_struct_layout": [P, I],
     "mpmx_profile_begin": [I],
     "mpmx_profile_mark": [P],
     "mpmx_profile_end": [],
@@ -83,21 +84,34 @@ _RESTYPE = {
 }
 
 
-class Particles(C.Structure):
+class _Strict(C.Structure):
+    """A ctypes structure whose constructor rejects names that are not fields
+    (plain ctypes would keep them as Python attributes and pass a struct
+    without them)."""
+
+    def __init__(self, *args, **kw):
+        names = {f[0] for f in self._fields_}
+        bad = [k for k in kw if k not in names]
+        if bad:
+            raise TypeError(f"{type(self).__name__}: no field(s) {bad}")
+        super().__init__(*args, **kw)
+
+
+class Particles(_Strict):
     """mpmx_particles (include/mpmx.h)"""
     _fields_ = [("n", C.c_int)] + [(name, C.c_void_p) for name in (
         "activeLabel", "materialLabel", "x", "v", "x_init_xpbd", "v_init_xpbd", "F", "F_trial", "C", "stress", "cov",
         "mu", "lam", "ys", "alpha", "hardening", "softening", "density", "strainCriteria", "eta_shear", "eta_bulk",
-        "acc_strain", "damage", "vol", "mass")]
+        "acc_strain", "damage", "vol", "mass", "order")]
 
 
-class Grid(C.Structure):
+class Grid(_Strict):
     """mpmx_grid (include/mpmx.h)"""
     _fields_ = [("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int), ("m", C.c_void_p), ("v_in", C.c_void_p),
                 ("v_out", C.c_void_p)]
 
 
-class StepParams(C.Structure):
+class StepParams(_Strict):
     """mpmx_step_params (include/mpmx.h)"""
     _fields_ = [(name, C.c_float) for name in (
         "dt", "dx", "inv_dx", "minx", "miny", "minz", "gravity_x", "gravity_y", "gravity_z", "rpic_damping", "eff",

@@ -380,6 +380,14 @@ int mpmx_profile_report(char* buf, size_t buflen) {
 }
 int mpmx_version(void) { return 1; }
 
+int mpmx_struct_layout(long* out, int cap) {
+    const long v[5] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order), (long)sizeof(mpmx_grid),
+                       (long)sizeof(mpmx_step_params), (long)offsetof(mpmx_step_params, bc)};
+    int k = 0;
+    for (; out && k < cap && k < 5; k++) out[k] = v[k];
+    return k;
+}
+
 int mpmx_stream_sync(void* stream) {
     MPMX_HIP(hipStreamSynchronize(as_stream(stream)));
     return 0;

@@ -32,6 +32,22 @@ def test_library_exports_every_declared_symbol():
     _lib.lib.load()  # dlopen resolves without a GPU
 
 
+def test_ctypes_structs_match_the_library_layout():
+    """The ctypes mirrors of mpmx_particles / mpmx_grid / mpmx_step_params have
+    the size and field offsets the library was compiled with (a struct passed
+    by pointer that is shorter than the C one makes the library read past it),
+    and unknown field names are refused."""
+    import ctypes as C
+    from mpmwarpydw_amd._lib import Grid, Particles, StepParams, lib
+    out = (C.c_long * 8)()
+    k = lib.raw("mpmx_struct_layout")(out, 8)
+    assert k == 5
+    assert list(out[:5]) == [C.sizeof(Particles), Particles.order.offset, C.sizeof(Grid), C.sizeof(StepParams),
+                             StepParams.bc.offset]
+    with pytest.raises(TypeError):
+        Particles(n=1, not_a_field=1)
+
+
 def test_workspace_queries_without_gpu():
     from mpmwarpydw_amd._lib import lib
     assert lib.raw("mpmx_p2g_workspace_bytes")(1000, 20, 20, 20) > 0

@@ -119,7 +119,7 @@ def _run_host(hm, st, eff):
               "eta_bulk", "damage"):
         A[k] = f(st[k])
     from mpmwarpydw_amd._lib import Particles
-    P = Particles(n, *[_p(A[name]) for name, _ in Particles._fields_[1:]])
+    P = Particles(n, *[_p(A[name]) if name in A else None for name, _ in Particles._fields_[1:]])
     hm.host_constitutive(C.byref(P), C.c_float(eff))
     aos = lambda a: np.moveaxis(a, 0, 1).astype(np.float64)  # noqa: E731
     return {"F": aos(A["F"]).reshape(n, 3, 3), "stress": aos(A["stress"]).reshape(n, 3, 3),
