@@ -214,6 +214,9 @@ int mpmx_reorder_permutation(const mpmx_particles* particles, const mpmx_grid* g
  * = words per item) permuted by perm; dst and src must not overlap. */
 int mpmx_gather_words(void* dst, const void* src, const int32_t* perm, long n, int planes,
                       int item_words, void* stream);
+/* inv[perm[i]] = i for a permutation perm of 0..n-1 (e.g. the array index of
+ * each input index from the particle numbering after a reorder) */
+int mpmx_invert_permutation(const int32_t* perm, int32_t* inv, long n, void* stream);
 
 /* grid_normalization_and_gravity, mpmRoutines.py:691-704: writes v_out only
  * where m > 1e-15 (the caller zeroes v_out first, simulationRoutines.py:13) */
@@ -322,14 +325,18 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel,
  * label-2 points (mpmx_hashgrid_build_label(..., 2, ...) on the same positions):
  * MPM particles (labels 0/1), which skip every MPM candidate in the reference
  * (xpbdRoutines.py:185-186), walk the XPBD-only table, whose candidates come in
- * the same relative order.  Results equal mpmx_solve_particle_particle_contacts. */
+ * the same relative order.  Results equal mpmx_solve_particle_particle_contacts.
+ * slot_of_id (may be NULL): after a spatial permutation of the particle arrays,
+ * slot_of_id[k] is the array index of the particle whose input index is k; the
+ * reference's tests of activeLabel/materialLabel/mass at the thread index
+ * (xpbdRoutines.py:151-175) then read that particle, as without the permutation. */
 int mpmx_solve_particle_particle_contacts_split(
     int n, const int32_t* activeLabel, int32_t* materialLabel, const int32_t* point_ids,
     const int32_t* cell_start, const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
     float cell_width, int dimx, int dimy, int dimz, const float* x, const float* v,
     const float* mass, const float* radius, float k_mu, float static_threshold,
     float static_friction, float k_cohesion, float max_radius, float dt, float relaxation,
-    int contact_lock, float* deltas, void* stream);
+    int contact_lock, float* deltas, const int32_t* slot_of_id, void* stream);
 
 /* my_apply_particle_deltas, xpbdRoutines.py:221-257 (x_out/v_out may alias x_pred) */
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel,

@@ -36,6 +36,7 @@ SIGNATURES = {
     "mpmx_substep_misbinned": [P, S, I, I, I, I, IP, P],
     "mpmx_reorder_permutation": [P, P, P, P, P, S, P],
     "mpmx_gather_words": [P, P, P, L, I, I, P],
+    "mpmx_invert_permutation": [P, P, L, P],
     "mpmx_stream_sync": [P],
     "mpmx_fill_i32": [P, L, I, P],
     "mpmx_fill_f32": [P, L, F, P],
@@ -65,7 +66,7 @@ SIGNATURES = {
     "mpmx_solve_particle_bound_contacts": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
     "mpmx_solve_particle_particle_contacts": [I, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F, F, I, P, P],
     "mpmx_solve_particle_particle_contacts_split": [I, P, P, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F,
-                                                    F, I, P, P],
+                                                    F, I, P, P, P],
     "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],
     "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
     "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],

@@ -225,22 +225,66 @@ __global__ void bin_scatter_kernel(const int32_t* __restrict__ keys, long n, con
 
 // Ids inside each key's run in ascending order of order[id] (of id itself when
 // order is null): the order a stable sort by key gives when ids are numbered
-// by order.  Runs are short (particles of one cell), so insertion sort.
-__global__ void bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start, int32_t* __restrict__ sorted,
-                                     const int32_t* __restrict__ order) {
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// by order.  Runs are short (the particles of one cell): up to kRunReg they are
+// sorted in registers by a bitonic network (static indices, one load and one
+// store per entry, nothing written when the run is already in order); longer
+// runs fall back to insertion sort in place.
+constexpr int kRunReg = 16;
+
+__device__ __forceinline__ void cswap(int& oa, int& va, int& ob, int& vb, bool up) {
+    if ((oa > ob) == up) {
+        int t = oa; oa = ob; ob = t;
+        t = va; va = vb; vb = t;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start,
+                                                               int32_t* __restrict__ sorted,
+                                                               const int32_t* __restrict__ order) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nkeys) return;
-    int b = cell_start[c], e = cell_start[c + 1];
+    const int b = cell_start[c], e = cell_start[c + 1];
+    const int len = e - b;
+    if (len <= 1) return;
+    if (len <= kRunReg) {
+        int v[kRunReg], o[kRunReg];
+        bool in_order = true;
+#pragma unroll
+        for (int k = 0; k < kRunReg; k++) {
+            v[k] = k < len ? sorted[b + k] : 0x7fffffff;
+            o[k] = k < len ? (order ? order[v[k]] : v[k]) : 0x7fffffff;
+            if (k > 0 && k < len && o[k - 1] > o[k]) in_order = false;
+        }
+        if (in_order) return;
+#pragma unroll
+        for (int kk = 2; kk <= kRunReg; kk <<= 1)
+#pragma unroll
+            for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < kRunReg; i++) {
+                    const int l = i ^ j;
+                    if (l > i) cswap(o[i], v[i], o[l], v[l], (i & kk) == 0);
+                }
+#pragma unroll
+        for (int k = 0; k < kRunReg; k++)
+            if (k < len) sorted[b + k] = v[k];
+        return;
+    }
     for (int i = b + 1; i < e; i++) {
-        const int v = sorted[i];
-        const int ov = order ? order[v] : v;
+        const int vi = sorted[i];
+        const int ov = order ? order[vi] : vi;
         int j = i - 1;
         while (j >= b && (order ? order[sorted[j]] : sorted[j]) > ov) {
             sorted[j + 1] = sorted[j];
             j--;
         }
-        sorted[j + 1] = v;
+        sorted[j + 1] = vi;
     }
+}
+
+__global__ void invert_permutation_kernel(const int32_t* __restrict__ perm, int32_t* __restrict__ inv, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = (int32_t)i;
 }
 
 // dst[(p*n + i)*w + k] = src[(p*n + perm[i])*w + k] for planes p, items i, words k
@@ -310,6 +354,15 @@ using namespace mpmx;
 extern "C" {
 
 const char* mpmx_last_error(void) { return g_err; }
+
+int mpmx_invert_permutation(const int32_t* perm, int32_t* inv, long n, void* stream) {
+    MPMX_REQUIRE(n >= 0, "mpmx_invert_permutation: n < 0");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(perm && inv && perm != inv, "mpmx_invert_permutation: null or aliased pointer");
+    hipStream_t s = as_stream(stream);
+    MPMX_LAUNCH(invert_permutation_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, perm, inv, n);
+    return check_launch("invert_permutation", s);
+}
 
 int mpmx_gather_words(void* dst, const void* src, const int32_t* perm, long n, int planes, int item_words,
                       void* stream) {

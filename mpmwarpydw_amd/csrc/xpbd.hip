@@ -120,21 +120,27 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids, const int32_t* __restrict__ xp_start,
     float inv, int dimx, int dimy, int dimz, const float* __restrict__ x,
     const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
-    float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas) {
+    float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
+    const int32_t* __restrict__ slot_of_id) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n) return;
-    if (active[tid] != 1 || mat[tid] < 0) return;
+    // with no XPBD particle at all there is no pair to solve (MPM-MPM pairs are skipped)
+    if (xp_ids && xp_start[(long)dimx * dimy * dimz] == 0) return;
+    // The reference tests the labels and the mass of array index tid, the thread's
+    // position in the hash order, not of the particle i it solves
+    // (xpbdRoutines.py:151-175).  After a spatial permutation of the arrays that
+    // index means the particle whose input index is tid.
+    const int q = slot_of_id ? slot_of_id[tid] : tid;
+    if (active[q] != 1 || mat[q] < 0) return;
     int i = point_ids[tid];
     if (i == -1) return;
     f3 xi = ld3(x, n, i), vi = ld3(v, n, i);
     float ri = radius[i];
-    if (mass[tid] == 0.0f) return;  // the reference tests tid here (xpbdRoutines.py:175)
+    if (mass[q] == 0.0f) return;
     float w1 = 1.0f / mass[i];
     const int32_t* ids = point_ids;
     const int32_t* starts = cell_start;
     if (xp_ids && mat[i] <= 1) {  // labels 0/1 stay MPM while this kernel runs (lock writes only 0)
-        // an MPM particle only meets XPBD particles: nothing to do when there are none
-        if (xp_start[(long)dimx * dimy * dimz] == 0) return;
         ids = xp_ids;
         starts = xp_start;
     }
@@ -375,7 +381,7 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel, int
                  "mpmx_solve_particle_particle_contacts: null pointer");
     LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, nullptr, nullptr,
             1.0f / cell_width, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold, static_friction,
-            k_cohesion, max_radius, dt, relaxation, contact_lock, deltas)
+            k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, nullptr)
 }
 
 int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabel, int32_t* materialLabel,
@@ -385,7 +391,7 @@ int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabe
                                                 const float* v, const float* mass, const float* radius, float k_mu,
                                                 float static_threshold, float static_friction, float k_cohesion,
                                                 float max_radius, float dt, float relaxation, int contact_lock,
-                                                float* deltas, void* stream) {
+                                                float* deltas, const int32_t* slot_of_id, void* stream) {
     MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f,
                  "mpmx_solve_particle_particle_contacts_split: bad sizes");
     if (n == 0) return 0;
@@ -394,7 +400,7 @@ int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabe
                  "mpmx_solve_particle_particle_contacts_split: null pointer");
     LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, xpbd_point_ids,
             xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold,
-            static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas)
+            static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, slot_of_id)
 }
 
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,

@@ -113,8 +113,9 @@ class Simulation:
         lib("mpmx_reorder_permutation", C.byref(P), C.byref(G), C.byref(prm), perm.ptr, ws, nb, rt.stream_handle())
         seen = set()
         for obj in (sim, mpm, xpbd):
-            for a in list(vars(obj).values()):
-                if not isinstance(a, rt.Array) or id(a) in seen or a.dtype == rt.float64 or len(a.shape) != 1:
+            for name, a in list(vars(obj).items()):
+                if (not isinstance(a, rt.Array) or name.startswith("_") or id(a) in seen or a.dtype == rt.float64
+                        or len(a.shape) != 1):
                     continue
                 if a.shape[0] == n:
                     a.permute_(perm)
@@ -123,6 +124,11 @@ class Simulation:
                 else:
                     continue
                 seen.add(id(a))
+        # array index of every input index, for the reference's index-based tests
+        # in the contact solve (not a per-particle array itself: reorder skips it)
+        if getattr(sim, "_slot_of_id", None) is None:
+            sim._slot_of_id = rt.zeros(n, rt.int32)
+        lib("mpmx_invert_permutation", sim.particle_id.ptr, sim._slot_of_id.ptr, n, rt.stream_handle())
         mpm._bins = None
 
     def residual(self) -> float:

@@ -219,7 +219,7 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=
         if split_grid:
             launch(kernel=xpbdRoutines.solve_particle_particle_contacts_split, dim=n,
                    inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, xpbd.particle_grid_xpbd.id,
-                           *contact_args], outputs=[xpbd.particle_delta])
+                           *contact_args], outputs=[xpbd.particle_delta, getattr(sim, "_slot_of_id", None)])
         else:
             launch(kernel=xpbdRoutines.my_solve_particle_particle_contacts, dim=n,
                    inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_grid.id, *contact_args],
@@ -284,7 +284,8 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
             float(grid.cell_width), dims[0], dims[1], dims[2], xi.ptr, vi.ptr, sim.particle_mass.ptr,
             sim.particle_radius.ptr, float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
             float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
-            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock), xpbd.particle_delta.ptr, s)
+            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock), xpbd.particle_delta.ptr,
+            _aptr(getattr(sim, "_slot_of_id", None)), s)
         lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
             float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, s)
     sim.numTotalXPBD.t.zero_()

@@ -65,7 +65,8 @@ my_solve_particle_particle_contacts = Kernel("my_solve_particle_particle_contact
 def _solve_particle_particle_contacts_split(dim, activeLabel, materialLabel, grid, xpbd_grid, particle_x, particle_v,
                                             particle_mass, particle_radius, k_mu,
                                             staticParticleVelocityThresholdRatio, staticParticleFriction, k_cohesion,
-                                            max_radius, dt, relaxation, mpm_contact_transition_lock, deltas):
+                                            max_radius, dt, relaxation, mpm_contact_transition_lock, deltas,
+                                            slot_of_id=None):
     dx, dy, dz = grid.dims
     assert xpbd_grid.dims == grid.dims and xpbd_grid.cell_width == grid.cell_width
     lib("mpmx_solve_particle_particle_contacts_split", int(dim), activeLabel.ptr, materialLabel.ptr,
@@ -73,7 +74,7 @@ def _solve_particle_particle_contacts_split(dim, activeLabel, materialLabel, gri
         float(grid.cell_width), dx, dy, dz, particle_x.ptr, particle_v.ptr, particle_mass.ptr, particle_radius.ptr,
         float(k_mu), float(staticParticleVelocityThresholdRatio), float(staticParticleFriction), float(k_cohesion),
         float(max_radius), float(dt), float(relaxation), int(mpm_contact_transition_lock), deltas.ptr,
-        stream_handle())
+        None if slot_of_id is None else slot_of_id.ptr, stream_handle())
 
 
 solve_particle_particle_contacts_split = Kernel(
