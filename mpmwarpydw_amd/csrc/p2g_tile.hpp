@@ -37,7 +37,12 @@ constexpr int kTileCells = kTB * kTB * kTB;  // 64
 constexpr int kTileNodes = kTN * kTN * kTN;  // 216
 constexpr int kRec = 25;                     // floats per staged particle record
 constexpr int kChunk = 256;                  // particles staged per LDS round
-constexpr int kTasks = kTileCells * 9;       // (cell, i, j)
+constexpr int kTasks = kTileCells * 3;       // (cell, i): each task owns the 9 nodes (i, j, k) of its cell
+// Record slot s lives at column s + s/8: the particles of neighbouring cells
+// (about 8 apart in the list) then fall in different LDS banks when the tasks
+// of a wavefront read their cells' records in step.
+constexpr int kRecCols = kChunk + kChunk / 8;
+__host__ __device__ constexpr int rcol(int s) { return s + (s >> 3); }
 
 // Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on
 // XCD b % 8), and every XCD has its own L2.  Neighbouring tiles share cache
@@ -119,11 +124,18 @@ __device__ __forceinline__ void orphan_bin(const TileArgs& a) {
     }
 }
 
+// LDS of the tile kernel: the staged records of one round, and afterwards (the
+// same bytes) the 27 offset slots of every cell for the final node sums.
+constexpr int kRecFloats = kRec * kRecCols;
+constexpr int kPartFloats = kTileCells * 27 * 4;
+constexpr int kTileLdsFloats = kRecFloats > kPartFloats ? kRecFloats : kPartFloats;
+
 template <bool FUSED>
 __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
-    __shared__ float rec[kRec][kChunk];
-    __shared__ float4 part[kTileCells * 27];
+    __shared__ float4 smem4[kTileLdsFloats / 4];
     __shared__ int cs[kTileCells + 1];
+    float (*rec)[kRecCols] = reinterpret_cast<float (*)[kRecCols]>(smem4);
+    float4* part = smem4;
 
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
@@ -140,7 +152,14 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
         if (tid == 0) a.tile_flag[t] = 0;
         return;
     }
-    for (int s = tid; s < kTileCells * 27; s += kBlock) part[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    // task (cell, i) of this thread: it owns the nodes (i, j, k) of the cell and
+    // keeps their sums in registers over all rounds
+    const int tc = tid / 3, ti = tid % 3;
+    const bool has_task = tid < kTasks;
+    float acc[9][4];
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0f;
 
     const float sfac = -a.dt * a.inv_dx;
     for (int c0 = 0; c0 < np; c0 += kChunk) {
@@ -148,6 +167,7 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
         __syncthreads();  // the previous round's readers are done with rec
         for (int s = tid; s < cn; s += kBlock) {
             const int id = a.sorted[p0 + c0 + s];
+            const int r = rcol(s);
             float T[9];
             bool mpm = true;
             if (FUSED) {
@@ -160,20 +180,20 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
                 // became XPBD in this substep: it no longer transfers to the grid
                 // (p2g tests the label after the stress pass, mpmRoutines.py:507)
 #pragma unroll
-                for (int k = 0; k < kRec; k++) rec[k][s] = 0.0f;
-                rec[0][s] = 1.0f;
-                rec[1][s] = 1.0f;
-                rec[2][s] = 1.0f;
+                for (int k = 0; k < kRec; k++) rec[k][r] = 0.0f;
+                rec[0][r] = 1.0f;
+                rec[1][r] = 1.0f;
+                rec[2][r] = 1.0f;
                 continue;
             }
             f3 xp = ld3(a.P.x, n, id);
             f3 vp = ld3(a.P.v, n, id);
             if (a.misbinned) {
                 // bins come from the previous substep's G2P: check that the particle is still in its cell
-                int c = -1;
-                for (int q = 0; q < kTileCells; q++)
-                    if (p0 + c0 + s >= cs[q] && p0 + c0 + s < cs[q + 1]) c = q;
-                if (tile_key(xp, a.minx, a.miny, a.minz, a.inv_dx, a.tg) != t * kTileCells + c) atomicAdd(a.misbinned, 1);
+                const int key = tile_key(xp, a.minx, a.miny, a.minz, a.inv_dx, a.tg);
+                const int c = key - t * kTileCells;
+                const int slot = p0 + c0 + s;
+                if (c < 0 || c >= kTileCells || slot < cs[c] || slot >= cs[c + 1]) atomicAdd(a.misbinned, 1);
             }
             float f0, f1, f2;
             base_axis(xp.x, a.minx, a.inv_dx, f0);
@@ -194,63 +214,62 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
             const float m = a.P.mass[id];
             const float md = m * a.dx;
             const float sv = sfac * a.P.vol[id];
-            rec[0][s] = f0;
-            rec[1][s] = f1;
-            rec[2][s] = f2;
-            rec[3][s] = m;
-            rec[4][s] = m * vp.x - md * (Cm[0] * f0 + Cm[1] * f1 + Cm[2] * f2);
-            rec[5][s] = m * vp.y - md * (Cm[3] * f0 + Cm[4] * f1 + Cm[5] * f2);
-            rec[6][s] = m * vp.z - md * (Cm[6] * f0 + Cm[7] * f1 + Cm[8] * f2);
+            rec[0][r] = f0;
+            rec[1][r] = f1;
+            rec[2][r] = f2;
+            rec[3][r] = m;
+            rec[4][r] = m * vp.x - md * (Cm[0] * f0 + Cm[1] * f1 + Cm[2] * f2);
+            rec[5][r] = m * vp.y - md * (Cm[3] * f0 + Cm[4] * f1 + Cm[5] * f2);
+            rec[6][r] = m * vp.z - md * (Cm[6] * f0 + Cm[7] * f1 + Cm[8] * f2);
 #pragma unroll
-            for (int k = 0; k < 9; k++) rec[7 + k][s] = md * Cm[k];
+            for (int k = 0; k < 9; k++) rec[7 + k][r] = md * Cm[k];
 #pragma unroll
-            for (int k = 0; k < 9; k++) rec[16 + k][s] = sv * T[k];
+            for (int k = 0; k < 9; k++) rec[16 + k][r] = sv * T[k];
         }
         __syncthreads();
-        for (int task = tid; task < kTasks; task += kBlock) {
-            const int c = task / 9;
-            const int i = (task % 9) / 3;
-            const int j = task % 3;
-            const int lo = max(cs[c] - p0, c0) - c0;
-            const int hi = min(cs[c + 1] - p0, c0 + cn) - c0;
-            if (lo >= hi) continue;
-            float acc[3][4];
-#pragma unroll
-            for (int k = 0; k < 3; k++) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.0f;
-            const float fi = (float)i, fj = (float)j;
+        if (has_task) {
+            const int lo = max(cs[tc] - p0, c0) - c0;
+            const int hi = min(cs[tc + 1] - p0, c0 + cn) - c0;
+            const float fi = (float)ti;
             for (int s = lo; s < hi; s++) {
-                float wx, dwx, wy, dwy;
-                wsel(rec[0][s], i, wx, dwx);
-                wsel(rec[1][s], j, wy, dwy);
-                float wz[3], dwz[3];
-                bspline(rec[2][s], wz, dwz);
-                const float m = rec[3][s];
-                const float wxy = wx * wy, gxw = dwx * wy, gyw = wx * dwy;
-                // affine momentum at k = 0: A + Q (i, j, 0)
-                float bx = rec[4][s] + rec[7][s] * fi + rec[8][s] * fj;
-                float by = rec[5][s] + rec[10][s] * fi + rec[11][s] * fj;
-                float bz = rec[6][s] + rec[13][s] * fi + rec[14][s] * fj;
-                const float qx = rec[9][s], qy = rec[12][s], qz = rec[15][s];
+                const int r = rcol(s);
+                float wx, dwx;
+                wsel(rec[0][r], ti, wx, dwx);
+                float wy[3], dwy[3], wz[3], dwz[3];
+                bspline(rec[1][r], wy, dwy);
+                bspline(rec[2][r], wz, dwz);
+                const float m = rec[3][r];
+                // affine momentum A + Q (i, j, k), Q = m dx Cm
+                const float ax0 = rec[4][r] + rec[7][r] * fi, ay0 = rec[5][r] + rec[10][r] * fi,
+                            az0 = rec[6][r] + rec[13][r] * fi;
+                const float q1x = rec[8][r], q1y = rec[11][r], q1z = rec[14][r];
+                const float q2x = rec[9][r], q2y = rec[12][r], q2z = rec[15][r];
+                const float s00 = rec[16][r], s01 = rec[17][r], s02 = rec[18][r];
+                const float s10 = rec[19][r], s11 = rec[20][r], s12 = rec[21][r];
+                const float s20 = rec[22][r], s21 = rec[23][r], s22 = rec[24][r];
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const float w = wxy * wz[k];
-                    const float g0 = gxw * wz[k], g1 = gyw * wz[k], g2 = wxy * dwz[k];
-                    const float ax = bx + qx * (float)k, ay = by + qy * (float)k, az = bz + qz * (float)k;
-                    acc[k][0] += w * m;
-                    acc[k][1] += w * ax + (rec[16][s] * g0 + rec[17][s] * g1 + rec[18][s] * g2);
-                    acc[k][2] += w * ay + (rec[19][s] * g0 + rec[20][s] * g1 + rec[21][s] * g2);
-                    acc[k][3] += w * az + (rec[22][s] * g0 + rec[23][s] * g1 + rec[24][s] * g2);
+                for (int j = 0; j < 3; j++) {
+                    const float wxy = wx * wy[j], gxw = dwx * wy[j], gyw = wx * dwy[j];
+                    const float bx = ax0 + q1x * (float)j, by = ay0 + q1y * (float)j, bz = az0 + q1z * (float)j;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const float w = wxy * wz[k];
+                        const float g0 = gxw * wz[k], g1 = gyw * wz[k], g2 = wxy * dwz[k];
+                        const float ax = bx + q2x * (float)k, ay = by + q2y * (float)k, az = bz + q2z * (float)k;
+                        float* A = acc[j * 3 + k];
+                        A[0] += w * m;
+                        A[1] += w * ax + (s00 * g0 + s01 * g1 + s02 * g2);
+                        A[2] += w * ay + (s10 * g0 + s11 * g1 + s12 * g2);
+                        A[3] += w * az + (s20 * g0 + s21 * g1 + s22 * g2);
+                    }
                 }
             }
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                float4& dst = part[c * 27 + i * 9 + j * 3 + k];
-                dst.x += acc[k][0];
-                dst.y += acc[k][1];
-                dst.z += acc[k][2];
-                dst.w += acc[k][3];
-            }
         }
+    }
+    __syncthreads();  // every record has been read: the LDS now holds the offset slots
+    if (has_task) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) part[tc * 27 + ti * 9 + q] = make_float4(acc[q][0], acc[q][1], acc[q][2], acc[q][3]);
     }
     __syncthreads();
     float* out = a.partial + (long)t * kTileNodes * 4;
