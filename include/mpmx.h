@@ -391,6 +391,44 @@ int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t*
                                float factor_max, const float* swell_radius,
                                const float* base_radius, void* stream);
 
+/* ---- learned return map (no reference counterpart) ----------------------
+ * A 10-64-64-3 GELU network stands in for the projection step of the
+ * Drucker-Prager return map (drucker_prager_return_mapping, mpmRoutines.py:
+ * 360-413), with bfloat16 weights on the matrix cores and FP32 accumulation.
+ * Parameters: one FP32 vector of mpmx_mlp_nparams() floats, W1[64][10], b1[64],
+ * W2[64][64], b2[64], W3[3][64], b3[3] (out x in).  Samples: rows of
+ * X[B][10]; predictions and targets: rows of Y[B][3], T[B][3].  Features and
+ * targets are defined with mpmx_lrm_harvest below. */
+int mpmx_mlp_nparams(void);
+int mpmx_mlp_forward(int B, const float* params, const float* X, float* Y, void* stream);
+/* mean squared error over the B*3 outputs and its gradient w.r.t. every
+ * parameter (FP32 grad[nparams], FP64 *loss, may be NULL); reproducible bit for
+ * bit for a given B.  Workspace: mpmx_mlp_workspace_bytes(B). */
+size_t mpmx_mlp_workspace_bytes(int B);
+int mpmx_mlp_loss_grad(int B, const float* params, const float* X, const float* T, float* grad,
+                       double* loss, void* workspace, size_t ws_bytes, void* stream);
+/* one Adam step (Kingma & Ba 2015) on n parameters, step number t >= 1 */
+int mpmx_mlp_adam_step(int n, float* params, const float* grad, float* m, float* v, float lr,
+                       float beta1, float beta2, float eps, int t, void* stream);
+/* Training pairs from the simulator's own state after G2P: for every particle,
+ * flag = 1 (active MPM, plastic trial state), 0 (active MPM, elastic) or -1;
+ * for MPM particles X[p] = the 10 features (principal trial log strains over the
+ * yield strain eref = max(ys,1)/(2 mu), log10 E - 8, 4 nu - 1, log10 cohesion
+ * - 5, tan friction angle, tan dilation angle, 20 acc, damage), T[p] = (e' - e)
+ * / eref of the exact projection, e_trial[p] = the principal trial log
+ * strains.  The state is not modified. */
+int mpmx_lrm_harvest(const mpmx_particles* particles, float* X, float* T, float* e_trial,
+                     int32_t* flag, void* stream);
+/* compute_stress_from_F_trial (mpmRoutines.py:229-312) with the projection of
+ * plastic states predicted by the network (params; NULL = exact projection for
+ * all) and judged by the exact map: a prediction whose yield-surface/flow
+ * residual 2 mu |dev e' - Y/(2 mu edn) d| or dilatancy residual
+ * K |tr e' - tr e - 0.9 alpha dgam| exceeds tol * max(cohesion, 1) is replaced
+ * by the exact projection.  counters[3] (may be NULL) += {plastic states,
+ * accepted predictions, particles updated}. */
+int mpmx_compute_stress_learned(const mpmx_particles* particles, float eff, const float* params,
+                                float tol, int32_t* counters, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,13 @@ SIGNATURES = {
     "mpmx_xpbd_bound_contacts_write": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
     "mpmx_xpbd_apply_deltas_inplace": [I, P, P, P, F, F, P, P, P],
     "mpmx_xpbd_finish": [I, P, P, F, P, P, F, F, F, I, P, P, P, P, P, P],
+    "mpmx_mlp_nparams": [],
+    "mpmx_mlp_forward": [I, P, P, P, P],
+    "mpmx_mlp_workspace_bytes": [I],
+    "mpmx_mlp_loss_grad": [I, P, P, P, P, P, P, S, P],
+    "mpmx_mlp_adam_step": [I, P, P, P, P, F, F, F, F, I, P],
+    "mpmx_lrm_harvest": [P, P, P, P, P, P],
+    "mpmx_compute_stress_learned": [P, F, P, F, P, P],
 }
 _RESTYPE = {
     "mpmx_substep_workspace_bytes": S,
@@ -82,6 +89,7 @@ _RESTYPE = {
     "mpmx_p2g_workspace_bytes": S,
     "mpmx_reduce_workspace_bytes": S,
     "mpmx_hashgrid_workspace_bytes": S,
+    "mpmx_mlp_workspace_bytes": S,
 }
 
 

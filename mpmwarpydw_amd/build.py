@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmpmx.so")
 ARCH = os.environ.get("MPMX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mpmx_runtime.hip", "mpm_kernels.hip", "p2g.hip", "substep.hip", "reduce.hip", "xpbd.hip"]
+SOURCES = ["mpmx_runtime.hip", "mpm_kernels.hip", "p2g.hip", "substep.hip", "reduce.hip", "xpbd.hip", "mlp.hip",
+           "learned.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_* FP32 instructions (their results are not trusted

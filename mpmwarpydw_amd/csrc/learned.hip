@@ -1,0 +1,361 @@
+// Learned return map in the constitutive update (compute_stress_from_F_trial,
+// mpmRoutines.py:229-471, with the projection of drucker_prager_return_mapping
+// :360-413 predicted by the network of mlp.hpp).
+//
+// The projection acts on the principal trial log strains e (descending, from
+// the SVD of F_trial).  With d = e - mean(e), edn = |d| + 1e-6, the trial
+// pressure p = mean(2 mu e + lam sum e) and Y = (1 - D)(ys - alpha p), a state
+// is plastic when 2 mu |d| > Y or ys < 0 or Y < 0, and the exact projection is
+//     e' = e - (dgam / edn) d + 0.3 alpha dgam,   dgam = edn - Y / (2 mu).
+// The network sees 10 features (e / eref, log10 E - 8, 4 nu - 1,
+// log10 cohesion - 5, tan of the friction and dilation angles, 20 acc, D, with
+// eref = max(ys, 1) / (2 mu)) and predicts (e' - e) / eref.  The exact map is
+// the judge: a prediction is used only when both residuals
+//     2 mu |dev e' - Y/(2 mu edn) d|              (yield surface and flow direction)
+//     K |tr e' - tr e - 0.9 alpha dgam|           (dilatancy)
+// are within tol * max(cohesion, 1); otherwise the exact projection replaces it.
+// Everything else (plastic strain, damage, hardening, phase transition,
+// Kirchhoff stress, viscosity) is the exact update.
+#include "mlp.hpp"
+#include "mpmx_constitutive.hpp"
+#include "mpmx_internal.hpp"
+
+namespace mpmx {
+
+MPMX_HD float lrm_tan_of_alpha(float a) {
+    const float r3 = 1.7320508075688772f;
+    float s = 3.0f * r3 * a / (2.0f + r3 * a);
+    if (s > 0.999999f) s = 0.999999f;
+    return s / sqrtf(1.0f - s * s);
+}
+
+// principal-space quantities of one trial state
+struct Principal {
+    float e[3], d[3], esum, dn, pr, Y;
+    bool plastic;
+};
+
+MPMX_HD Principal principal_state(const float eps[3], float mu, float lam, float ys, float al, float D) {
+    Principal s;
+    s.esum = eps[0] + eps[1] + eps[2];
+    const float mean = s.esum / 3.0f;
+    for (int k = 0; k < 3; k++) {
+        s.e[k] = eps[k];
+        s.d[k] = eps[k] - mean;
+    }
+    s.dn = sqrtf(s.d[0] * s.d[0] + s.d[1] * s.d[1] + s.d[2] * s.d[2]);
+    s.pr = (2.0f * mu / 3.0f + lam) * s.esum;
+    s.Y = (1.0f - D) * (ys - al * s.pr);
+    s.plastic = (2.0f * mu * s.dn > s.Y) || ys < 0.0f || s.Y < 0.0f;
+    return s;
+}
+
+MPMX_HD void lrm_features(const Principal& s, float mu, float lam, float ys, float al, float acc, float D, float x[10]) {
+    const float E = mu * (3.0f * lam + 2.0f * mu) / (lam + mu);
+    const float nu = lam / (2.0f * (lam + mu));
+    const float c = ys > 1.0f ? ys : 1.0f;
+    const float er = c / (2.0f * mu);
+    for (int k = 0; k < 3; k++) x[k] = s.e[k] / er;
+    x[3] = log10f(E) - 8.0f;
+    x[4] = 4.0f * nu - 1.0f;
+    x[5] = log10f(c) - 5.0f;
+    x[6] = lrm_tan_of_alpha(al);
+    x[7] = lrm_tan_of_alpha(0.3f * al);
+    x[8] = 20.0f * acc;
+    x[9] = D;
+}
+
+// exact projection e' of a plastic state and its dgam
+MPMX_HD void exact_projection(const Principal& s, float mu, float al, float en[3], float& dgam) {
+    const float edn = s.dn + 1.0e-6f;
+    dgam = edn - s.Y / (2.0f * mu);
+    const float r = dgam / edn, volc = (0.3f * al) * dgam;
+    for (int k = 0; k < 3; k++) en[k] = (s.e[k] - r * s.d[k]) + volc;
+}
+
+MPMX_HD bool lrm_judge(const Principal& s, const float q[3], float mu, float lam, float ys, float al, float tol) {
+    const float edn = s.dn + 1.0e-6f;
+    const float c = s.Y / (2.0f * mu * edn);
+    const float qm = (q[0] + q[1] + q[2]) / 3.0f;
+    const float r0 = q[0] - qm - c * s.d[0], r1 = q[1] - qm - c * s.d[1], r2 = q[2] - qm - c * s.d[2];
+    const float res1 = 2.0f * mu * sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
+    const float dgam = edn - s.Y / (2.0f * mu);
+    const float K = lam + 2.0f * mu / 3.0f;
+    const float res2 = K * fabsf((q[0] + q[1] + q[2]) - s.esum - 3.0f * 0.3f * al * dgam);
+    const float lim = tol * (ys > 1.0f ? ys : 1.0f);
+    return res1 <= lim && res2 <= lim;
+}
+
+// Harvest: features, exact targets (e' - e)/eref and the state of every particle
+// (1 plastic MPM, 0 elastic MPM, -1 not an active MPM particle); e_trial gets
+// the principal trial log strains.  Reads the state, writes nothing of it.
+__global__ void lrm_harvest_kernel(mpmx_particles P, float* __restrict__ X, float* __restrict__ Tg,
+                                   float* __restrict__ E3, int32_t* __restrict__ flag) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    if (P.activeLabel[p] != 1 || P.materialLabel[p] > 1) {
+        flag[p] = -1;
+        return;
+    }
+    float Ft[9];
+    ld9(P.F_trial, P.n, p, Ft);
+    Svd3 sv;
+    svd3(Ft, sv);
+    float eps[3];
+    for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
+    const float mu = P.mu[p], lam = P.lam[p], ys = P.ys[p], al = P.alpha[p], D = P.damage[p];
+    const Principal s = principal_state(eps, mu, lam, ys, al, D);
+    float x[10];
+    lrm_features(s, mu, lam, ys, al, P.acc_strain[p], D, x);
+    for (int k = 0; k < 10; k++) X[p * 10 + k] = x[k];
+    const float er = (ys > 1.0f ? ys : 1.0f) / (2.0f * mu);
+    float en[3] = {eps[0], eps[1], eps[2]}, dg = 0.0f;
+    if (s.plastic) exact_projection(s, mu, al, en, dg);
+    for (int k = 0; k < 3; k++) {
+        Tg[p * 3 + k] = (en[k] - eps[k]) / er;
+        E3[p * 3 + k] = eps[k];
+    }
+    flag[p] = s.plastic ? 1 : 0;
+}
+
+// The constitutive update of one particle with the projection supplied: the
+// SVD route of constitutive_update (mpmx_constitutive.hpp) with e' = en.
+__device__ __forceinline__ void finish_update(const mpmx_particles& P, long p, float eff, const Svd3& sv,
+                                              const float eps[3], const Principal& s, bool plastic, const float en_in[3]) {
+    const long n = P.n;
+    const int label = P.materialLabel[p];
+    float mu = P.mu[p], lam = P.lam[p];
+    const float ys = P.ys[p], D = P.damage[p];
+    int branch = 0;
+    float en[3] = {en_in[0], en_in[1], en_in[2]};
+    if (plastic) {
+        const float edn = s.dn + 1.0e-6f;
+        const float dgam = edn - s.Y / (2.0f * mu);
+        const float dep = sqrtf(2.0f / 3.0f) * dgam;
+        P.acc_strain[p] = P.acc_strain[p] + dep;
+        const float sc = P.strainCriteria[p];
+        float Dn = (sc > 0.0f) ? wmin(1.0f, D + dep / sc) : 1.0f;
+        if (ys < 0.0f || s.Y < 0.0f) Dn = 1.0f;
+        P.damage[p] = Dn;
+        if (Dn >= 1.0f && label == 1) {
+            branch = 2;
+            P.materialLabel[p] = 2;
+            f3 xp = ld3(P.x, n, p);
+            st3(P.x_init_xpbd, n, p, xp);
+            const float tau0 = 2.0f * mu * eps[0] + lam * s.esum, tau1 = 2.0f * mu * eps[1] + lam * s.esum,
+                        tau2 = 2.0f * mu * eps[2] + lam * s.esum;
+            const float u = 0.5f * (tau0 * eps[0] + tau1 * eps[1] + tau2 * eps[2]);
+            const float v_expected = sqrtf(2.0f * eff * u / P.density[p]);
+            f3 vp = ld3(P.v, n, p);
+            f3 vdir = normalize3(f3{vp.x + 1.0e-12f, vp.y + 1.0e-12f, vp.z + 1.0e-12f});
+            f3 vn = f3{vp.x + v_expected * vdir.x, vp.y + v_expected * vdir.y, vp.z + v_expected * vdir.z};
+            st3(P.v, n, p, vn);
+            st3(P.v_init_xpbd, n, p, vn);
+            P.ys[p] = 0.0f;
+            P.mu[p] = 0.0f;
+            P.lam[p] = 0.0f;
+            P.alpha[p] = 0.0f;
+            mu = 0.0f;
+            lam = 0.0f;
+        } else {
+            branch = 1;
+            P.ys[p] = wmax(0.0f, ys + 2.0f * mu * (P.hardening[p] - P.softening[p]) * dgam);
+        }
+    }
+    float d[3];
+    float Ft[9];
+    if (branch == 1) {
+        float e[3], l[3];
+        for (int k = 0; k < 3; k++) {
+            e[k] = expf(en[k]);
+            l[k] = (e[k] >= 1.0e-6f) ? en[k] : logf(1.0e-6f);
+        }
+        const float ls = l[0] + l[1] + l[2];
+        for (int k = 0; k < 3; k++) {
+            const float dk = 2.0f * mu * l[k] + lam * ls;
+            d[k] = (e[k] >= 1.0e-6f) ? dk : dk * (e[k] / 1.0e-6f);
+        }
+        float Fo[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                Fo[i * 3 + j] = sv.U[i * 3 + 0] * e[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * e[1] * sv.V[j * 3 + 1] +
+                                sv.U[i * 3 + 2] * e[2] * sv.V[j * 3 + 2];
+        st9(P.F, n, p, Fo);
+    } else {
+        ld9(P.F_trial, n, p, Ft);
+        float l[3];
+        for (int k = 0; k < 3; k++) l[k] = log_clamped(sv.s[k], sv.lam[k], 1.0e-6f);
+        const float ls = l[0] + l[1] + l[2];
+        for (int k = 0; k < 3; k++) {
+            const float dk = 2.0f * mu * l[k] + lam * ls;
+            d[k] = (sv.s[k] >= 1.0e-6f) ? dk : dk * (sv.s[k] / 1.0e-6f);
+        }
+        st9(P.F, n, p, Ft);
+    }
+    float G[9];
+    ld9(P.C, n, p, G);
+    const float s00 = G[0], s11 = G[4], s22 = G[8];
+    const float s01 = 0.5f * (G[1] + G[3]), s02 = 0.5f * (G[2] + G[6]), s12 = 0.5f * (G[5] + G[7]);
+    const float vr = s00 + s11 + s22;
+    const float es = 2.0f * P.eta_shear[p], eb = P.eta_bulk[p];
+    const float third = vr / 3.0f;
+    float T[9];
+    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+    float tau[6];
+    for (int q = 0; q < 6; q++) {
+        const int i = ii[q], j = jj[q];
+        tau[q] = sv.U[i * 3 + 0] * d[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * d[1] * sv.U[j * 3 + 1] +
+                 sv.U[i * 3 + 2] * d[2] * sv.U[j * 3 + 2];
+    }
+    T[0] = tau[0] + (es * (s00 - third) + eb * vr);
+    T[4] = tau[1] + (es * (s11 - third) + eb * vr);
+    T[8] = tau[2] + (es * (s22 - third) + eb * vr);
+    T[1] = T[3] = tau[3] + es * s01;
+    T[2] = T[6] = tau[4] + es * s02;
+    T[5] = T[7] = tau[5] + es * s12;
+    st9(P.stress, n, p, T);
+}
+
+// One wavefront handles 64 consecutive particles per step; the network runs on
+// the matrix cores for the wavefront's 4 tiles of 16 whenever one of its
+// particles is plastic.  counters: [0] += plastic states, [1] += accepted
+// predictions, [2] += particles updated.
+__global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P, float eff, const float* __restrict__ prm,
+                                                                float tol, int use_net, int32_t* __restrict__ counters) {
+    __shared__ MlpWeights w;
+    if (use_net) load_weights(prm, w);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long nwaves = (long)gridDim.x * 4;
+    int n_plastic = 0, n_accept = 0, n_done = 0;
+    for (long base = wave * 64; base < P.n; base += nwaves * 64) {  // uniform per wavefront
+        const long p = base + lane;
+        const bool mpm = p < P.n && P.activeLabel[p] == 1 && P.materialLabel[p] <= 1;
+        Svd3 sv;
+        float eps[3] = {0.0f, 0.0f, 0.0f}, x[10];
+        for (int k = 0; k < 10; k++) x[k] = 0.0f;
+        Principal s;
+        bool plastic = false;
+        float mu = 1.0f, lam = 0.0f, ys = 0.0f, al = 0.0f, D = 0.0f;
+        if (mpm) {
+            float Ft[9];
+            ld9(P.F_trial, P.n, p, Ft);
+            svd3(Ft, sv);
+            for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
+            mu = P.mu[p];
+            lam = P.lam[p];
+            ys = P.ys[p];
+            al = P.alpha[p];
+            D = P.damage[p];
+            s = principal_state(eps, mu, lam, ys, al, D);
+            plastic = s.plastic;
+            if (plastic) lrm_features(s, mu, lam, ys, al, P.acc_strain[p], D, x);
+        }
+        float yo[3] = {0.0f, 0.0f, 0.0f};
+        const unsigned long long pm = __ballot(plastic);
+        if (use_net && pm) {  // uniform: every lane runs the matrix-core tiles
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                // B fragment: lane l takes features 4(l/16)..+3 of particle t*16 + l%16
+                const int src = t * 16 + (lane & 15);
+                const int f0 = 4 * (lane >> 4);
+                float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int q = 0; q < 10; q++) {
+                    const float v = __shfl(x[q], src);
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (f0 + j == q) xv[j] = v;
+                }
+                MlpTile tt;
+                mlp_tile_forward(w, pack_bf16(xv[0], xv[1], xv[2], xv[3]), lane, tt);
+                // lanes 0..15 hold the outputs of particles t*16 + 0..15
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const float v = __shfl(tt.y[j], lane & 15);
+                    if ((lane >> 4) == t) yo[j] = v;
+                }
+            }
+        }
+        if (mpm) {
+            n_done++;
+            float en[3] = {eps[0], eps[1], eps[2]};
+            if (plastic) {
+                n_plastic++;
+                bool ok = false;
+                if (use_net) {
+                    const float er = (ys > 1.0f ? ys : 1.0f) / (2.0f * mu);
+                    float q[3];
+                    for (int k = 0; k < 3; k++) q[k] = eps[k] + er * yo[k];
+                    ok = lrm_judge(s, q, mu, lam, ys, al, tol);
+                    if (ok) {
+                        for (int k = 0; k < 3; k++) en[k] = q[k];
+                        n_accept++;
+                    }
+                }
+                if (!ok) {
+                    float dg;
+                    exact_projection(s, mu, al, en, dg);
+                }
+            }
+            finish_update(P, p, eff, sv, eps, s, plastic, en);
+        }
+    }
+    // one atomic per wavefront and counter
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        n_plastic += __shfl_xor(n_plastic, o);
+        n_accept += __shfl_xor(n_accept, o);
+        n_done += __shfl_xor(n_done, o);
+    }
+    if (lane == 0 && counters) {
+        atomicAdd(&counters[0], n_plastic);
+        atomicAdd(&counters[1], n_accept);
+        atomicAdd(&counters[2], n_done);
+    }
+}
+
+static int check_particles(const mpmx_particles* P) {
+    MPMX_REQUIRE(P && P->n >= 0, "learned: bad particle struct");
+    MPMX_REQUIRE(P->n == 0 || (P->activeLabel && P->materialLabel && P->F_trial && P->mu && P->lam && P->ys &&
+                               P->alpha && P->damage && P->acc_strain),
+                 "learned: null particle array");
+    return 0;
+}
+
+}  // namespace mpmx
+
+using namespace mpmx;
+
+extern "C" {
+
+int mpmx_lrm_harvest(const mpmx_particles* P, float* X, float* T, float* e_trial, int32_t* flag, void* stream) {
+    int rc = check_particles(P);
+    if (rc) return rc;
+    if (P->n == 0) return 0;
+    MPMX_REQUIRE(X && T && e_trial && flag, "mpmx_lrm_harvest: null output");
+    hipStream_t s = as_stream(stream);
+    MPMX_LAUNCH(lrm_harvest_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, X, T, e_trial, flag);
+    return check_launch("lrm_harvest", s);
+}
+
+int mpmx_compute_stress_learned(const mpmx_particles* P, float eff, const float* params, float tol,
+                                int32_t* counters, void* stream) {
+    int rc = check_particles(P);
+    if (rc) return rc;
+    if (P->n == 0) return 0;
+    MPMX_REQUIRE(P->x && P->v && P->x_init_xpbd && P->v_init_xpbd && P->F && P->C && P->stress && P->hardening &&
+                     P->softening && P->density && P->strainCriteria && P->eta_shear && P->eta_bulk,
+                 "mpmx_compute_stress_learned: null particle array");
+    MPMX_REQUIRE(tol >= 0.0f, "mpmx_compute_stress_learned: negative tolerance");
+    hipStream_t s = as_stream(stream);
+    const long waves = ((long)P->n + 63) / 64;
+    long nb = (waves + 3) / 4;
+    if (nb > 8192) nb = 8192;
+    MPMX_LAUNCH(stress_learned_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, *P, eff, params, tol,
+                params ? 1 : 0, counters);
+    return check_launch("stress_learned", s);
+}
+
+}  // extern "C"
