@@ -73,7 +73,18 @@ typedef struct mpmx_particles {
      * order[id] (ascending id when NULL), which makes the results independent
      * of where each particle sits in the arrays.  No reference counterpart. */
     int32_t* order;
+    /* optional (may be NULL = scene 0): the scene of every particle when
+     * several scenes advance together (mpmx_ensemble_*).  No reference
+     * counterpart. */
+    int32_t* scene;
 } mpmx_particles;
+
+/* One scene of an ensemble: its grid (gx*gy*gz nodes from the origin min*,
+ * spacing dx of the step parameters).  No reference counterpart. */
+typedef struct mpmx_scene {
+    int gx, gy, gz;
+    float minx, miny, minz;
+} mpmx_scene;
 
 /* Grid arrays of MPMState (simStates.py:117-121). */
 typedef struct mpmx_grid {
@@ -100,11 +111,45 @@ typedef struct mpmx_step_params {
 } mpmx_step_params;
 
 const char* mpmx_last_error(void);
+
+/* Launch-shape knobs, process-wide (no reference counterpart), set by the
+ * autotuner: MPMX_TUNE_XCD_TILES = 1 deals the tiles of the tile kernels to
+ * the 8 XCDs in contiguous ranges (neighbouring tiles share an L2), 0 in
+ * block order.  Results do not depend on any knob. */
+#define MPMX_TUNE_XCD_TILES 0
+#define MPMX_TUNE_COUNT 1
+int mpmx_set_tuning(int knob, int value);
+int mpmx_get_tuning(int knob);
+
+/* ---- contexts, device buffers, captured substep sequences ----------------
+ * For callers without a device-memory runtime of their own (the Python layer
+ * uses torch tensors and skips these).  A context is one device and one HIP
+ * stream (default priority); buffers are hipMalloc'd and zeroed at allocation
+ * (the analogue of wp.zeros in SimState construction, simStates.py); upload
+ * and download are synchronising (download mirrors wp.array.numpy()).
+ * mpmx_graph_begin/_end capture everything issued on the context's stream in
+ * between (e.g. ten mpmx_mpm_substep calls and one XPBD step) into a graph
+ * that mpmx_graph_launch replays; no entry point allocates while capturing,
+ * and every buffer clear inside the library is a kernel, not a memset. */
+typedef struct mpmx_ctx mpmx_ctx;
+typedef struct mpmx_graph mpmx_graph;
+int mpmx_ctx_create(int device, mpmx_ctx** out);
+int mpmx_ctx_destroy(mpmx_ctx* ctx);
+void* mpmx_ctx_stream(mpmx_ctx* ctx);
+int mpmx_buf_alloc(mpmx_ctx* ctx, size_t bytes, void** out);
+int mpmx_buf_free(mpmx_ctx* ctx, void* p);
+int mpmx_buf_upload(mpmx_ctx* ctx, void* dst, const void* host, size_t bytes);
+int mpmx_buf_download(mpmx_ctx* ctx, void* host, const void* src, size_t bytes);
+int mpmx_graph_begin(mpmx_ctx* ctx);
+int mpmx_graph_end(mpmx_ctx* ctx, mpmx_graph** out);
+int mpmx_graph_launch(mpmx_graph* graph, void* stream);
+int mpmx_graph_destroy(mpmx_graph* graph);
 int mpmx_version(void);
 /* Layout of the structs above as this build sees them, for bindings to check
  * theirs against: out[0..] = sizeof(mpmx_particles), offsetof(mpmx_particles,
  * order), sizeof(mpmx_grid), sizeof(mpmx_step_params), offsetof(
- * mpmx_step_params, bc); returns the number of values written (<= cap). */
+ * mpmx_step_params, bc), offsetof(mpmx_particles, scene), sizeof(mpmx_scene);
+ * returns the number of values written (<= cap). */
 int mpmx_struct_layout(long* out, int cap);
 /* Launch profiler (no counterpart in the reference; replaces its wall-clock
  * prints, runMPMYDW.py:762): between begin and end, events are recorded on the
@@ -195,6 +240,27 @@ int mpmx_mpm_substep(const mpmx_particles* particles, const mpmx_grid* grid,
  * mpmx_substep_bin() (0 when bins were kept valid; synchronising). */
 int mpmx_substep_misbinned(void* workspace, size_t ws_bytes, int gx, int gy, int gz, int n,
                            int* host_count, void* stream);
+
+/* Ensemble: several independent scenes (each with its own grid and walls, the
+ * same dx, dt and gravity) advanced by ONE launch of every kernel of the
+ * fused substep above.  Tiles and grid nodes are numbered scene after scene:
+ * scene s owns the nodes node0(s) .. node0(s) + gx*gy*gz of the grid arrays,
+ * node0(s) being the node count of the scenes before it; particles carry their
+ * scene in particles->scene.  The step parameters' grid origin is not used (each
+ * scene has its own).  Each scene's results equal running it alone through
+ * mpmx_mpm_substep.  Workspace: mpmx_ensemble_workspace_bytes(); call
+ * mpmx_ensemble_bin() first, as mpmx_substep_bin() for one scene. */
+size_t mpmx_ensemble_workspace_bytes(int n, const mpmx_scene* scenes, int nscenes);
+int mpmx_ensemble_bin(const mpmx_particles* particles, const mpmx_scene* scenes, int nscenes,
+                      const mpmx_step_params* params, void* workspace, size_t ws_bytes,
+                      void* stream);
+int mpmx_ensemble_substep(const mpmx_particles* particles, const mpmx_scene* scenes, int nscenes,
+                          float* grid_m, float* grid_v_in, float* grid_v_out,
+                          const mpmx_step_params* params, void* workspace, size_t ws_bytes,
+                          void* stream);
+int mpmx_ensemble_reorder_permutation(const mpmx_particles* particles, const mpmx_scene* scenes,
+                                      int nscenes, const mpmx_step_params* params, int32_t* perm,
+                                      void* workspace, size_t ws_bytes, void* stream);
 
 /* Spatial order of the particle arrays.  perm[n] receives a permutation that
  * lists the active MPM particles tile by tile and cell by cell of the

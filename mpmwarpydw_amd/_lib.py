@@ -26,6 +26,19 @@ SIGNATURES = {
     "mpmx_last_error": [],
     "mpmx_version": [],
     "mpmx_struct_layout": [P, I],
+    "mpmx_set_tuning": [I, I],
+    "mpmx_get_tuning": [I],
+    "mpmx_ctx_create": [I, P],
+    "mpmx_ctx_destroy": [P],
+    "mpmx_ctx_stream": [P],
+    "mpmx_buf_alloc": [P, S, P],
+    "mpmx_buf_free": [P, P],
+    "mpmx_buf_upload": [P, P, P, S],
+    "mpmx_buf_download": [P, P, P, S],
+    "mpmx_graph_begin": [P],
+    "mpmx_graph_end": [P, P],
+    "mpmx_graph_launch": [P, P],
+    "mpmx_graph_destroy": [P],
     "mpmx_profile_begin": [I],
     "mpmx_profile_mark": [P],
     "mpmx_profile_end": [],
@@ -35,6 +48,10 @@ SIGNATURES = {
     "mpmx_mpm_substep": [P, P, P, P, S, P],
     "mpmx_substep_misbinned": [P, S, I, I, I, I, IP, P],
     "mpmx_reorder_permutation": [P, P, P, P, P, S, P],
+    "mpmx_ensemble_workspace_bytes": [I, P, I],
+    "mpmx_ensemble_bin": [P, P, I, P, P, S, P],
+    "mpmx_ensemble_substep": [P, P, I, P, P, P, P, P, S, P],
+    "mpmx_ensemble_reorder_permutation": [P, P, I, P, P, P, S, P],
     "mpmx_gather_words": [P, P, P, L, I, I, P],
     "mpmx_invert_permutation": [P, P, L, P],
     "mpmx_stream_sync": [P],
@@ -90,6 +107,8 @@ _RESTYPE = {
     "mpmx_reduce_workspace_bytes": S,
     "mpmx_hashgrid_workspace_bytes": S,
     "mpmx_mlp_workspace_bytes": S,
+    "mpmx_ctx_stream": P,
+    "mpmx_ensemble_workspace_bytes": S,
 }
 
 
@@ -111,7 +130,13 @@ class Particles(_Strict):
     _fields_ = [("n", C.c_int)] + [(name, C.c_void_p) for name in (
         "activeLabel", "materialLabel", "x", "v", "x_init_xpbd", "v_init_xpbd", "F", "F_trial", "C", "stress", "cov",
         "mu", "lam", "ys", "alpha", "hardening", "softening", "density", "strainCriteria", "eta_shear", "eta_bulk",
-        "acc_strain", "damage", "vol", "mass", "order")]
+        "acc_strain", "damage", "vol", "mass", "order", "scene")]
+
+
+class Scene(_Strict):
+    """mpmx_scene (include/mpmx.h)"""
+    _fields_ = [("gx", C.c_int), ("gy", C.c_int), ("gz", C.c_int), ("minx", C.c_float), ("miny", C.c_float),
+                ("minz", C.c_float)]
 
 
 class Grid(_Strict):

@@ -1,0 +1,62 @@
+"""Autotuner of the fused substep's launch shape and particle order.
+
+Knobs (none changes results, only speed):
+  xcd_tiles   deal tiles to the chip's 8 XCDs in contiguous ranges (1) or in
+              block order (0)                      (mpmx_set_tuning)
+  reorder     keep the particle arrays in tile order in HBM (the sort-bin
+              layout of the tile kernels) or in input order  (Simulation.reorder,
+              Ensemble.reorder)
+Every candidate is timed on the caller's own state with HIP events over a few
+substeps, best of ``trials``; candidates are visited in a fixed order and ties
+keep the earlier one, so a tuning run is reproducible given the timings, and
+the log records every measurement.
+"""
+from __future__ import annotations
+
+import time
+
+import torch
+
+from ._lib import lib
+
+MPMX_TUNE_XCD_TILES = 0
+
+
+def set_xcd_tiles(on: bool) -> None:
+    lib("mpmx_set_tuning", MPMX_TUNE_XCD_TILES, 1 if on else 0)
+
+
+def _time(step, steps: int, warmup: int, trials: int) -> float:
+    for _ in range(warmup):
+        step()
+    best = float("inf")
+    for _ in range(trials):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(steps):
+            step()
+        b.record()
+        torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b) / steps)
+    return best
+
+
+def tune(state, step, steps: int = 10, warmup: int = 3, trials: int = 2) -> dict:
+    """Tune on ``state`` (a driver.Simulation or an ensemble.Ensemble with
+    a ``reorder()`` method) advanced by ``step()``.  Leaves the state reordered
+    when that wins and the best XCD setting applied; returns the log."""
+    t0 = time.perf_counter()
+    log = []
+    for reorder in (False, True):
+        if reorder:
+            state.reorder()
+        for xcd in (1, 0):
+            set_xcd_tiles(bool(xcd))
+            ms = _time(step, steps, warmup, trials)
+            log.append({"reorder": reorder, "xcd_tiles": xcd, "ms_per_substep": round(ms, 4)})
+    best = min(log, key=lambda r: r["ms_per_substep"])  # min keeps the first of equal entries
+    set_xcd_tiles(bool(best["xcd_tiles"]))
+    # the state is reordered at this point; input order is kept only when it was faster
+    return {"best": best, "candidates": log, "kept_reordered": True, "tuning_s": round(time.perf_counter() - t0, 2)}

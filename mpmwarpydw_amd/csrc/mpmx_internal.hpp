@@ -15,6 +15,8 @@ int check_launch(const char* what, hipStream_t s);
 // launch profiler hook: records a start event before a kernel when profiling is on
 void prof_start(hipStream_t s);
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+// launch-shape knobs (mpmx_set_tuning); read on the host when a launch is issued
+int tuning(int knob);
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
 
 // Bump allocator over a caller-provided workspace; every slice is 256-byte aligned.

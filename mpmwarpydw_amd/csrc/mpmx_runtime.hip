@@ -16,6 +16,9 @@
 namespace mpmx {
 
 static thread_local char g_err[512] = "";
+static int g_tuning[MPMX_TUNE_COUNT] = {1};  // MPMX_TUNE_XCD_TILES on
+
+int tuning(int knob) { return (knob >= 0 && knob < MPMX_TUNE_COUNT) ? g_tuning[knob] : 0; }
 
 void set_error(const char* fmt, ...) {
     va_list ap;
@@ -433,12 +436,134 @@ int mpmx_profile_report(char* buf, size_t buflen) {
 }
 int mpmx_version(void) { return 1; }
 
+int mpmx_set_tuning(int knob, int value) {
+    MPMX_REQUIRE(knob >= 0 && knob < MPMX_TUNE_COUNT, "mpmx_set_tuning: unknown knob %d", knob);
+    MPMX_REQUIRE(knob != MPMX_TUNE_XCD_TILES || value == 0 || value == 1, "mpmx_set_tuning: XCD tiles is 0 or 1");
+    g_tuning[knob] = value;
+    return 0;
+}
+
+int mpmx_get_tuning(int knob) { return tuning(knob); }
+
 int mpmx_struct_layout(long* out, int cap) {
-    const long v[5] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order), (long)sizeof(mpmx_grid),
-                       (long)sizeof(mpmx_step_params), (long)offsetof(mpmx_step_params, bc)};
+    const long v[7] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order), (long)sizeof(mpmx_grid),
+                       (long)sizeof(mpmx_step_params), (long)offsetof(mpmx_step_params, bc),
+                       (long)offsetof(mpmx_particles, scene), (long)sizeof(mpmx_scene)};
     int k = 0;
-    for (; out && k < cap && k < 5; k++) out[k] = v[k];
+    for (; out && k < cap && k < 7; k++) out[k] = v[k];
     return k;
+}
+
+// ---- contexts, buffers and captured substep sequences ----------------------
+
+struct mpmx_ctx {
+    int device;
+    hipStream_t stream;
+    bool capturing;
+};
+struct mpmx_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int mpmx_ctx_create(int device, mpmx_ctx** out) {
+    MPMX_REQUIRE(out, "mpmx_ctx_create: null out");
+    int count = 0;
+    MPMX_HIP(hipGetDeviceCount(&count));
+    MPMX_REQUIRE(device >= 0 && device < count, "mpmx_ctx_create: device %d of %d", device, count);
+    MPMX_HIP(hipSetDevice(device));
+    hipStream_t s;
+    MPMX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+    mpmx_ctx* c = new mpmx_ctx{device, s, false};
+    *out = c;
+    return 0;
+}
+
+int mpmx_ctx_destroy(mpmx_ctx* ctx) {
+    if (!ctx) return 0;
+    MPMX_HIP(hipStreamSynchronize(ctx->stream));
+    MPMX_HIP(hipStreamDestroy(ctx->stream));
+    delete ctx;
+    return 0;
+}
+
+void* mpmx_ctx_stream(mpmx_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mpmx_buf_alloc(mpmx_ctx* ctx, size_t bytes, void** out) {
+    MPMX_REQUIRE(ctx && out, "mpmx_buf_alloc: null argument");
+    MPMX_REQUIRE(!ctx->capturing, "mpmx_buf_alloc: not allowed while a substep sequence is being captured");
+    void* p = nullptr;
+    MPMX_HIP(hipMalloc(&p, bytes > 0 ? bytes : 1));
+    hipError_t e = hipMemsetAsync(p, 0, bytes > 0 ? bytes : 1, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        set_error("mpmx_buf_alloc: clearing failed: %s", hipGetErrorString(e));
+        return (int)e;
+    }
+    *out = p;
+    return 0;
+}
+
+int mpmx_buf_free(mpmx_ctx* ctx, void* p) {
+    MPMX_REQUIRE(ctx, "mpmx_buf_free: null context");
+    if (!p) return 0;
+    MPMX_HIP(hipStreamSynchronize(ctx->stream));
+    MPMX_HIP(hipFree(p));
+    return 0;
+}
+
+int mpmx_buf_upload(mpmx_ctx* ctx, void* dst, const void* host, size_t bytes) {
+    MPMX_REQUIRE(ctx && (bytes == 0 || (dst && host)), "mpmx_buf_upload: null argument");
+    if (bytes == 0) return 0;
+    MPMX_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MPMX_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int mpmx_buf_download(mpmx_ctx* ctx, void* host, const void* src, size_t bytes) {
+    MPMX_REQUIRE(ctx && (bytes == 0 || (src && host)), "mpmx_buf_download: null argument");
+    if (bytes == 0) return 0;
+    MPMX_HIP(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MPMX_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int mpmx_graph_begin(mpmx_ctx* ctx) {
+    MPMX_REQUIRE(ctx && !ctx->capturing, "mpmx_graph_begin: no context or already capturing");
+    MPMX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return 0;
+}
+
+int mpmx_graph_end(mpmx_ctx* ctx, mpmx_graph** out) {
+    MPMX_REQUIRE(ctx && ctx->capturing && out, "mpmx_graph_end: not capturing");
+    ctx->capturing = false;
+    hipGraph_t g;
+    MPMX_HIP(hipStreamEndCapture(ctx->stream, &g));
+    hipGraphExec_t ex;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error("mpmx_graph_end: instantiate failed: %s", hipGetErrorString(e));
+        return (int)e;
+    }
+    *out = new mpmx_graph{g, ex};
+    return 0;
+}
+
+int mpmx_graph_launch(mpmx_graph* g, void* stream) {
+    MPMX_REQUIRE(g, "mpmx_graph_launch: null graph");
+    MPMX_HIP(hipGraphLaunch(g->exec, as_stream(stream)));
+    return 0;
+}
+
+int mpmx_graph_destroy(mpmx_graph* g) {
+    if (!g) return 0;
+    MPMX_HIP(hipGraphExecDestroy(g->exec));
+    MPMX_HIP(hipGraphDestroy(g->graph));
+    delete g;
+    return 0;
 }
 
 int mpmx_stream_sync(void* stream) {

@@ -10,13 +10,12 @@ __global__ void fill_i32_one(int32_t* __restrict__ p) { *p = 0; }
 
 __global__ void p2g_key_kernel(int n, const int32_t* __restrict__ activeLabel,
                                const int32_t* __restrict__ materialLabel, const float* __restrict__ x,
-                               float minx, float miny, float minz, float inv_dx, TileGeom tg,
-                               int32_t* __restrict__ keys, int32_t* __restrict__ oob) {
+                               float inv_dx, Geo geo, int32_t* __restrict__ keys, int32_t* __restrict__ oob) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int key = -1;
     if (activeLabel[p] == 1 && materialLabel[p] <= 1) {
-        key = tile_key(ld3(x, n, p), minx, miny, minz, inv_dx, tg);
+        key = tile_key(ld3(x, n, p), inv_dx, geo.one);
         if (key < 0) atomicAdd(oob, 1);
     }
     keys[p] = key;
@@ -31,14 +30,14 @@ struct P2GWork {
     BinWork bin;
 };
 
-static size_t p2g_ws_bytes(long n, const TileGeom& tg) {
+static size_t p2g_ws_bytes(long n, const Geo& tg) {
     long nkeys = tg.ntiles() * kTileCells;
     long nn = n > 0 ? n : 1;
     return Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(tg.ntiles() * kTileNodes * 4, 4) +
            Carver::bytes_for(tg.ntiles(), 4) + Carver::bytes_for(1, 4) + bin_workspace_bytes(nn, nkeys);
 }
 
-static bool carve_p2g(void* ws, size_t bytes, long n, const TileGeom& tg, P2GWork& w) {
+static bool carve_p2g(void* ws, size_t bytes, long n, const Geo& tg, P2GWork& w) {
     Carver cv(ws, bytes);
     long nn = n > 0 ? n : 1;
     w.keys = cv.take<int32_t>(nn);
@@ -61,7 +60,7 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     MPMX_REQUIRE(n == 0 || (activeLabel && materialLabel && stress && x && v && C && vol && mass),
                  "p2g: null particle pointer");
     MPMX_REQUIRE(bc == MPMX_BC_NONE || bc == MPMX_BC_FRICTION, "p2g: unsupported boundary condition %d", bc);
-    TileGeom tg = make_geom(gx, gy, gz);
+    Geo tg = make_geom(gx, gy, gz, minx, miny, minz);
     MPMX_REQUIRE(tg.ntiles() * kTileCells < (1L << 31), "p2g: grid too large for int32 keys");
     size_t need = p2g_ws_bytes(n, tg);
     MPMX_REQUIRE(ws && ws_bytes >= need, "p2g: workspace of %zu bytes given, %zu needed", ws_bytes, need);
@@ -72,7 +71,7 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     if ((rc = check_launch("p2g_oob_zero", s))) return rc;
     if (n > 0) {
         MPMX_LAUNCH(p2g_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, activeLabel,
-                           materialLabel, x, minx, miny, minz, inv_dx, tg, w.keys, w.oob);
+                           materialLabel, x, inv_dx, tg, w.keys, w.oob);
         if ((rc = check_launch("p2g_key", s))) return rc;
     }
     if ((rc = counting_sort_ids(w.keys, n, tg.ntiles() * kTileCells, w.sorted, w.bin, s))) return rc;
@@ -88,19 +87,18 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     ta.cell_start = w.bin.cell_start;
     ta.dx = dx;
     ta.inv_dx = inv_dx;
-    ta.minx = minx;
-    ta.miny = miny;
-    ta.minz = minz;
     ta.rpic = rpic;
     ta.dt = dt;
     ta.partial = w.partial;
     ta.tile_flag = w.tile_flag;
     ta.ntiles = (int)tg.ntiles();
+    ta.geo = tg;
+    ta.xcd = tuning(MPMX_TUNE_XCD_TILES);
     MPMX_LAUNCH(p2g_tile_kernel<false>, dim3((unsigned)tg.ntiles()), dim3(kBlock), 0, s, ta);
     if ((rc = check_launch("p2g_tile", s))) return rc;
     long G = (long)gx * gy * gz;
     CombineArgs ca{};
-    ca.tg = tg;
+    ca.geo = tg;
     ca.partial = w.partial;
     ca.tile_flag = w.tile_flag;
     ca.grid_m = grid_m;

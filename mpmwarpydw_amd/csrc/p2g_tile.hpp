@@ -53,35 +53,90 @@ MPMX_HD int xcd_tile(int b, int nb) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
-struct TileGeom {
+// Grid geometry of one scene: its nodes, its 4x4x4-cell tiles, where its tiles
+// and nodes start in the global numbering, and its grid origin.  Several
+// scenes (an ensemble) share one launch of every substep kernel: tiles are
+// numbered scene after scene, grid nodes likewise (scene s owns the global
+// nodes node0 .. node0 + gx*gy*gz), and each particle belongs to one scene.
+struct SceneDesc {
     int gx, gy, gz;     // grid nodes
     int ntx, nty, ntz;  // tiles
-    __host__ __device__ long ntiles() const { return (long)ntx * nty * ntz; }
+    int tile0;          // first global tile
+    int unused;
+    long node0;         // first global node
+    float minx, miny, minz, unused2;
 };
 
-inline TileGeom make_geom(int gx, int gy, int gz) {
-    TileGeom g;
-    g.gx = gx;
-    g.gy = gy;
-    g.gz = gz;
-    g.ntx = (gx - 2 + kTB - 1) / kTB;
-    g.nty = (gy - 2 + kTB - 1) / kTB;
-    g.ntz = (gz - 2 + kTB - 1) / kTB;
+inline SceneDesc make_scene(int gx, int gy, int gz, float minx, float miny, float minz, int tile0, long node0) {
+    SceneDesc d{};
+    d.gx = gx;
+    d.gy = gy;
+    d.gz = gz;
+    d.ntx = (gx - 2 + kTB - 1) / kTB;
+    d.nty = (gy - 2 + kTB - 1) / kTB;
+    d.ntz = (gz - 2 + kTB - 1) / kTB;
+    d.tile0 = tile0;
+    d.node0 = node0;
+    d.minx = minx;
+    d.miny = miny;
+    d.minz = minz;
+    return d;
+}
+
+// All scenes of a launch.  With one scene its description travels in the
+// kernel arguments; with more, in a device table of nscenes + 1 entries whose
+// last entry holds the totals (tile0 = tiles, node0 = nodes).
+struct Geo {
+    SceneDesc one;
+    const SceneDesc* table;
+    int nscenes;
+    int ntiles_;
+    long nnodes;
+    __host__ __device__ long ntiles() const { return ntiles_; }
+    __device__ __forceinline__ SceneDesc at(int s) const { return nscenes == 1 ? one : table[s]; }
+    // the scene holding global tile t (node i): the last entry with tile0 <= t
+    __device__ __forceinline__ int scene_of_tile(int t) const {
+        if (nscenes == 1) return 0;
+        int lo = 0, hi = nscenes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ int scene_of_node(long i) const {
+        if (nscenes == 1) return 0;
+        int lo = 0, hi = nscenes - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].node0 <= i) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }
+};
+
+inline Geo make_geom(int gx, int gy, int gz, float minx = 0.0f, float miny = 0.0f, float minz = 0.0f) {
+    Geo g{};
+    g.one = make_scene(gx, gy, gz, minx, miny, minz, 0, 0);
+    g.table = nullptr;
+    g.nscenes = 1;
+    g.ntiles_ = g.one.ntx * g.one.nty * g.one.ntz;
+    g.nnodes = (long)gx * gy * gz;
     return g;
 }
 
-// Key of the base cell of a particle at x, or -1 when its 3x3x3 stencil leaves
-// the grid.  Compares against g - 3 (not b + 2 < g) so that a saturated huge
-// index cannot wrap around.
-__device__ __forceinline__ int tile_key(f3 x, float minx, float miny, float minz, float inv_dx, const TileGeom& tg) {
+// Key of the base cell of a particle of scene d at x, or -1 when its 3x3x3
+// stencil leaves the scene's grid.  Compares against g - 3 (not b + 2 < g) so
+// that a saturated huge index cannot wrap around.
+__device__ __forceinline__ int tile_key(f3 x, float inv_dx, const SceneDesc& d) {
     float f;
-    int bx = base_axis(x.x, minx, inv_dx, f);
-    int by = base_axis(x.y, miny, inv_dx, f);
-    int bz = base_axis(x.z, minz, inv_dx, f);
-    if (bx < 0 || by < 0 || bz < 0 || bx > tg.gx - 3 || by > tg.gy - 3 || bz > tg.gz - 3) return -1;
+    int bx = base_axis(x.x, d.minx, inv_dx, f);
+    int by = base_axis(x.y, d.miny, inv_dx, f);
+    int bz = base_axis(x.z, d.minz, inv_dx, f);
+    if (bx < 0 || by < 0 || bz < 0 || bx > d.gx - 3 || by > d.gy - 3 || bz > d.gz - 3) return -1;
     int tx = bx / kTB, ty = by / kTB, tz = bz / kTB;
     int lx = bx - tx * kTB, ly = by - ty * kTB, lz = bz - tz * kTB;
-    return ((tx * tg.nty + ty) * tg.ntz + tz) * kTileCells + (lx * kTB + ly) * kTB + lz;
+    return (d.tile0 + (tx * d.nty + ty) * d.ntz + tz) * kTileCells + (lx * kTB + ly) * kTB + lz;
 }
 
 // B-spline weight and derivative for a stencil offset i known only at run time
@@ -106,11 +161,12 @@ struct TileArgs {
     const float* stress;          // stress to scatter when not fused
     const int32_t* sorted;        // particle ids in key order
     const int32_t* cell_start;    // [nkeys + 1]
-    float dx, inv_dx, minx, miny, minz, rpic, dt, eff;
+    float dx, inv_dx, rpic, dt, eff;
     float* partial;               // [ntiles][4][216]
     int32_t* tile_flag;           // [ntiles]
     int ntiles;
-    TileGeom tg;
+    Geo geo;
+    int xcd;                      // deal tiles to XCDs in contiguous ranges (xcd_tile) or in block order
     int32_t* misbinned;           // optional: counts particles whose base cell is not their bin's cell
 };
 
@@ -137,13 +193,14 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
     float (*rec)[kRecCols] = reinterpret_cast<float (*)[kRecCols]>(smem4);
     float4* part = smem4;
 
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int tid = threadIdx.x;
     if (FUSED && t == a.ntiles) {
         orphan_bin(a);
         return;
     }
     const long n = a.P.n;
+    const SceneDesc sd = a.geo.at(a.geo.scene_of_tile(t));
     if (tid <= kTileCells) cs[tid] = a.cell_start[(long)t * kTileCells + tid];
     __syncthreads();
     const int p0 = cs[0];
@@ -190,15 +247,15 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
             f3 vp = ld3(a.P.v, n, id);
             if (a.misbinned) {
                 // bins come from the previous substep's G2P: check that the particle is still in its cell
-                const int key = tile_key(xp, a.minx, a.miny, a.minz, a.inv_dx, a.tg);
+                const int key = tile_key(xp, a.inv_dx, sd);
                 const int c = key - t * kTileCells;
                 const int slot = p0 + c0 + s;
                 if (c < 0 || c >= kTileCells || slot < cs[c] || slot >= cs[c + 1]) atomicAdd(a.misbinned, 1);
             }
             float f0, f1, f2;
-            base_axis(xp.x, a.minx, a.inv_dx, f0);
-            base_axis(xp.y, a.miny, a.inv_dx, f1);
-            base_axis(xp.z, a.minz, a.inv_dx, f2);
+            base_axis(xp.x, sd.minx, a.inv_dx, f0);
+            base_axis(xp.y, sd.miny, a.inv_dx, f1);
+            base_axis(xp.z, sd.minz, a.inv_dx, f2);
             float Cp[9], Cm[9];
             ld9(a.P.C, n, id, Cp);
             const float rpic = a.rpic;
@@ -306,7 +363,7 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
 
 // ---- combine ----------------------------------------------------------------
 struct CombineArgs {
-    TileGeom tg;
+    Geo geo;
     const float* partial;
     const int32_t* tile_flag;
     float *grid_m, *grid_v_in, *grid_v_out;
@@ -320,28 +377,29 @@ struct CombineArgs {
 // a template so that every translation unit including this header can instantiate it
 template <int UNUSED = 0>
 __global__ void p2g_combine_kernel(CombineArgs a) {
-    const TileGeom& tg = a.tg;
-    const long G = (long)tg.gx * tg.gy * tg.gz;
+    const long G = a.geo.nnodes;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (a.zero_ints) {
         for (long z = i; z < a.n_zero; z += (long)gridDim.x * blockDim.x) a.zero_ints[z] = 0;
     }
     if (i >= G) return;
-    const int iz = (int)(i % tg.gz);
-    const int iy = (int)((i / tg.gz) % tg.gy);
-    const int ix = (int)(i / ((long)tg.gy * tg.gz));
+    const SceneDesc d = a.geo.at(a.geo.scene_of_node(i));
+    const long li = i - d.node0;
+    const int iz = (int)(li % d.gz);
+    const int iy = (int)((li / d.gz) % d.gy);
+    const int ix = (int)(li / ((long)d.gy * d.gz));
     float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
     const int tx1 = ix / kTB, ty1 = iy / kTB, tz1 = iz / kTB;
     for (int tx = tx1 - 1; tx <= tx1; tx++) {
         const int lx = ix - tx * kTB;
-        if (tx < 0 || tx >= tg.ntx || lx >= kTN) continue;
+        if (tx < 0 || tx >= d.ntx || lx >= kTN) continue;
         for (int ty = ty1 - 1; ty <= ty1; ty++) {
             const int ly = iy - ty * kTB;
-            if (ty < 0 || ty >= tg.nty || ly >= kTN) continue;
+            if (ty < 0 || ty >= d.nty || ly >= kTN) continue;
             for (int tz = tz1 - 1; tz <= tz1; tz++) {
                 const int lz = iz - tz * kTB;
-                if (tz < 0 || tz >= tg.ntz || lz >= kTN) continue;
-                const long t = ((long)tx * tg.nty + ty) * tg.ntz + tz;
+                if (tz < 0 || tz >= d.ntz || lz >= kTN) continue;
+                const long t = d.tile0 + ((long)tx * d.nty + ty) * d.ntz + tz;
                 if (!a.tile_flag[t]) continue;
                 const float* pb = a.partial + t * kTileNodes * 4;
                 const int node = (lx * kTN + ly) * kTN + lz;
@@ -357,7 +415,7 @@ __global__ void p2g_combine_kernel(CombineArgs a) {
     a.grid_v_in[G + i] = my;
     a.grid_v_in[2 * G + i] = mz;
     if (a.do_update) {
-        f3 vo = grid_update_node(m, f3{mx, my, mz}, ix, iy, iz, tg.gx, tg.gy, tg.gz, a.gxv, a.gyv, a.gzv, a.dt,
+        f3 vo = grid_update_node(m, f3{mx, my, mz}, ix, iy, iz, d.gx, d.gy, d.gz, a.gxv, a.gyv, a.gzv, a.dt,
                                  a.damping, a.bc, a.fric, a.gmag, a.pad);
         st3(a.grid_v_out, G, i, vo);
     }

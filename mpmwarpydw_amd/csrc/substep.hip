@@ -30,58 +30,75 @@ struct SubstepWork {
     int32_t* sorted;
     float* partial;
     int32_t* tile_flag;
+    SceneDesc* table;    // [nscenes + 1] for an ensemble
     BinWork bin;
 };
 
 // keys: tile * 64 + cell, then the orphan bin (out-of-grid MPM particles), then
 // a bin used only by the spatial reorder for every particle that is not MPM
-static long substep_nkeys(const TileGeom& tg) { return tg.ntiles() * kTileCells + 2; }
+static long substep_nkeys(const Geo& g) { return g.ntiles() * kTileCells + 2; }
 
-static size_t substep_ws_bytes(long n, const TileGeom& tg) {
+static size_t substep_ws_bytes(long n, const Geo& g) {
     long nn = n > 0 ? n : 1;
-    return Carver::bytes_for(1, 4) + Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(tg.ntiles() * kTileNodes * 4, 4) +
-           Carver::bytes_for(tg.ntiles(), 4) + bin_workspace_bytes(nn, substep_nkeys(tg));
+    return Carver::bytes_for(1, 4) + Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(g.ntiles() * kTileNodes * 4, 4) +
+           Carver::bytes_for(g.ntiles(), 4) + Carver::bytes_for(g.nscenes + 1, sizeof(SceneDesc)) +
+           bin_workspace_bytes(nn, substep_nkeys(g));
 }
 
-static bool carve_substep(void* ws, size_t bytes, long n, const TileGeom& tg, SubstepWork& w) {
+static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepWork& w) {
     Carver cv(ws, bytes);
     long nn = n > 0 ? n : 1;
     w.misbinned = cv.take<int32_t>(1);
     w.keys = cv.take<int32_t>(nn);
     w.sorted = cv.take<int32_t>(nn);
-    w.partial = cv.take<float>(tg.ntiles() * kTileNodes * 4);
-    w.tile_flag = cv.take<int32_t>(tg.ntiles());
-    carve_bin(cv, nn, substep_nkeys(tg), w.bin);
+    w.partial = cv.take<float>(g.ntiles() * kTileNodes * 4);
+    w.tile_flag = cv.take<int32_t>(g.ntiles());
+    w.table = cv.take<SceneDesc>(g.nscenes + 1);
+    carve_bin(cv, nn, substep_nkeys(g), w.bin);
     return !cv.overflow;
 }
 
 __global__ void zero_one_kernel(int32_t* __restrict__ p) { *p = 0; }
 
+// the scene of particle p (its scene array, or scene 0 without one)
+__device__ __forceinline__ int scene_of(const mpmx_particles& P, long p) { return P.scene ? P.scene[p] : 0; }
+
 // key of an active MPM particle: its base-cell key, or the orphan bin
-__global__ void substep_key_kernel(mpmx_particles P, float minx, float miny, float minz, float inv_dx, TileGeom tg,
-                                   int32_t* __restrict__ keys) {
+__global__ void substep_key_kernel(mpmx_particles P, float inv_dx, Geo geo, int32_t* __restrict__ keys) {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
     int key = -1;
     if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
-        key = tile_key(ld3(P.x, P.n, p), minx, miny, minz, inv_dx, tg);
-        if (key < 0) key = (int)(tg.ntiles() * kTileCells);
+        const int sc = scene_of(P, p);
+        key = (sc >= 0 && sc < geo.nscenes) ? tile_key(ld3(P.x, P.n, p), inv_dx, geo.at(sc)) : -1;
+        if (key < 0) key = (int)(geo.ntiles() * kTileCells);
     }
     keys[p] = key;
 }
 
 // key of every particle for the spatial reorder: active MPM particles as in
 // substep_key_kernel, everything else in the last bin
-__global__ void reorder_key_kernel(mpmx_particles P, float minx, float miny, float minz, float inv_dx, TileGeom tg,
-                                   int32_t* __restrict__ keys) {
+__global__ void reorder_key_kernel(mpmx_particles P, float inv_dx, Geo geo, int32_t* __restrict__ keys) {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
-    int key = (int)(tg.ntiles() * kTileCells + 1);
+    int key = (int)(geo.ntiles() * kTileCells + 1);
     if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
-        key = tile_key(ld3(P.x, P.n, p), minx, miny, minz, inv_dx, tg);
-        if (key < 0) key = (int)(tg.ntiles() * kTileCells);
+        const int sc = scene_of(P, p);
+        key = (sc >= 0 && sc < geo.nscenes) ? tile_key(ld3(P.x, P.n, p), inv_dx, geo.at(sc)) : -1;
+        if (key < 0) key = (int)(geo.ntiles() * kTileCells);
     }
     keys[p] = key;
+}
+
+// the scene table of an ensemble, written into the workspace from kernel
+// arguments (no host memory is read by a replayed graph)
+constexpr int kSceneChunk = 16;
+struct SceneChunk {
+    SceneDesc d[kSceneChunk];
+};
+__global__ void write_scenes_kernel(SceneChunk c, int count, SceneDesc* __restrict__ dst) {
+    const int k = threadIdx.x;
+    if (k < count) dst[k] = c.d[k];
 }
 
 struct G2PArgs {
@@ -89,13 +106,14 @@ struct G2PArgs {
     const int32_t* sorted;
     const int32_t* cell_start;
     const float* v_out;
-    TileGeom tg;
-    float dt, inv_dx, minx, miny, minz;
+    Geo geo;
+    float dt, inv_dx;
     int update_cov;
     int32_t* keys;   // next substep: key per particle
     int32_t* count;  // next substep: particles per key (cleared by K2)
     int32_t* rank;   // next substep: arrival rank inside the key
     int32_t* misbinned;
+    int xcd;
 };
 
 __device__ __forceinline__ void bin_next(const G2PArgs& a, long id, int key) {
@@ -114,12 +132,11 @@ __device__ __forceinline__ f3 g2p_node(const G2PArgs& a, const float (*gvs)[kTil
 // G2P of one particle (mpmRoutines.py:715-802): new v, x, C, F_trial (and the
 // covariance when it is tracked); returns the particle's key for the next substep.
 template <bool LDS>
-__device__ __forceinline__ int g2p_one(const G2PArgs& a, const float (*gvs)[kTileNodes], int id, f3 xp, int b0, int b1,
-                                       int b2, int ox, int oy, int oz, const float fx[3]) {
+__device__ __forceinline__ int g2p_one(const G2PArgs& a, const SceneDesc& sd, const float (*gvs)[kTileNodes], int id,
+                                       f3 xp, int b0, int b1, int b2, int ox, int oy, int oz, const float fx[3]) {
     const mpmx_particles& P = a.P;
     const long n = P.n;
-    const TileGeom& tg = a.tg;
-    const long G = (long)tg.gx * tg.gy * tg.gz;
+    const long G = a.geo.nnodes;
     const float inv_dx = a.inv_dx, dt = a.dt;
     float w[3][3], dw[3][3];
     bspline(fx[0], w[0], dw[0]);
@@ -136,7 +153,7 @@ __device__ __forceinline__ int g2p_one(const G2PArgs& a, const float (*gvs)[kTil
             const float gx_ = dw[0][i] * w[1][j] * inv_dx;
             const float gy_ = w[0][i] * dw[1][j] * inv_dx;
             const int row = ((b0 + i) * kTN + (b1 + j)) * kTN + b2;
-            const long grow = ((long)(b0 + ox + i) * tg.gy + (b1 + oy + j)) * tg.gz + (b2 + oz);
+            const long grow = sd.node0 + ((long)(b0 + ox + i) * sd.gy + (b1 + oy + j)) * sd.gz + (b2 + oz);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const f3 gv = g2p_node<LDS>(a, gvs, row + k, grow + k, G);
@@ -188,8 +205,8 @@ __device__ __forceinline__ int g2p_one(const G2PArgs& a, const float (*gvs)[kTil
             }
         c6[0] = c1[0]; c6[1] = c1[1]; c6[2] = c1[2]; c6[3] = c1[4]; c6[4] = c1[5]; c6[5] = c1[8];
     }
-    const int key = tile_key(xn, a.minx, a.miny, a.minz, inv_dx, tg);
-    return key < 0 ? (int)(tg.ntiles() * kTileCells) : key;
+    const int key = tile_key(xn, inv_dx, sd);
+    return key < 0 ? (int)(a.geo.ntiles() * kTileCells) : key;
 }
 
 // G2P per tile with the tile's grid velocities staged in LDS; F_trial = (I + dt G) F
@@ -205,9 +222,8 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     __shared__ int lbase[kTileCells];
     const mpmx_particles& P = a.P;
     const long n = P.n;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const TileGeom& tg = a.tg;
-    const long ntiles = tg.ntiles();
+    const int t = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const long ntiles = a.geo.ntiles();
     const int orphan_key = (int)(ntiles * kTileCells);
     if (t == ntiles) {
         // out-of-grid particles: no transfer (the reference would read out of bounds); they stay orphans
@@ -221,14 +237,16 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     const long p0 = a.cell_start[(long)t * kTileCells];
     const long p1 = a.cell_start[(long)t * kTileCells + kTileCells];
     if (p1 == p0) return;
-    const int tz = (int)(t % tg.ntz), ty = (int)((t / tg.ntz) % tg.nty), tx = (int)(t / ((long)tg.ntz * tg.nty));
+    const SceneDesc sd = a.geo.at(a.geo.scene_of_tile(t));
+    const int lt = t - sd.tile0;
+    const int tz = lt % sd.ntz, ty = (lt / sd.ntz) % sd.nty, tx = lt / (sd.ntz * sd.nty);
     const int ox = tx * kTB, oy = ty * kTB, oz = tz * kTB;
-    const long G = (long)tg.gx * tg.gy * tg.gz;
+    const long G = a.geo.nnodes;
     for (int node = threadIdx.x; node < kTileNodes; node += kBlock) {
         const int nx = ox + node / (kTN * kTN), ny = oy + (node / kTN) % kTN, nz = oz + node % kTN;
         float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-        if (nx < tg.gx && ny < tg.gy && nz < tg.gz) {
-            const long gi = ((long)nx * tg.gy + ny) * tg.gz + nz;
+        if (nx < sd.gx && ny < sd.gy && nz < sd.gz) {
+            const long gi = sd.node0 + ((long)nx * sd.gy + ny) * sd.gz + nz;
             v0 = a.v_out[gi];
             v1 = a.v_out[G + gi];
             v2 = a.v_out[2 * G + gi];
@@ -248,19 +266,19 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
             if (P.materialLabel[id] <= 1) {  // else it became XPBD in this substep's constitutive update
                 const f3 xp = ld3(P.x, n, id);
                 float fx[3];
-                const int b0 = base_axis(xp.x, a.minx, a.inv_dx, fx[0]) - ox;
-                const int b1 = base_axis(xp.y, a.miny, a.inv_dx, fx[1]) - oy;
-                const int b2 = base_axis(xp.z, a.minz, a.inv_dx, fx[2]) - oz;
+                const int b0 = base_axis(xp.x, sd.minx, a.inv_dx, fx[0]) - ox;
+                const int b1 = base_axis(xp.y, sd.miny, a.inv_dx, fx[1]) - oy;
+                const int b2 = base_axis(xp.z, sd.minz, a.inv_dx, fx[2]) - oz;
                 if (b0 >= 0 && b1 >= 0 && b2 >= 0 && b0 < kTB && b1 < kTB && b2 < kTB) {
-                    key = g2p_one<true>(a, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
+                    key = g2p_one<true>(a, sd, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
                 } else {
                     // stale bin: the particle is outside its tile's node block
                     if (a.misbinned) atomicAdd(a.misbinned, 1);
-                    if (b0 + ox < 0 || b1 + oy < 0 || b2 + oz < 0 || b0 + ox > tg.gx - 3 || b1 + oy > tg.gy - 3 ||
-                        b2 + oz > tg.gz - 3)
+                    if (b0 + ox < 0 || b1 + oy < 0 || b2 + oz < 0 || b0 + ox > sd.gx - 3 || b1 + oy > sd.gy - 3 ||
+                        b2 + oz > sd.gz - 3)
                         key = orphan_key;  // its stencil leaves the grid: no transfer
                     else
-                        key = g2p_one<false>(a, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
+                        key = g2p_one<false>(a, sd, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
                 }
             }
             a.keys[id] = key;
@@ -280,10 +298,9 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     }
 }
 
-static int check_args(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm) {
-    MPMX_REQUIRE(P && G && prm, "substep: null argument");
-    MPMX_REQUIRE(P->n >= 0 && G->gx >= 3 && G->gy >= 3 && G->gz >= 3, "substep: bad sizes");
-    MPMX_REQUIRE(G->m && G->v_in && G->v_out, "substep: null grid array");
+static int check_particles(const mpmx_particles* P, const mpmx_step_params* prm) {
+    MPMX_REQUIRE(P && prm, "substep: null argument");
+    MPMX_REQUIRE(P->n >= 0, "substep: bad sizes");
     MPMX_REQUIRE(P->n == 0 || (P->activeLabel && P->materialLabel && P->x && P->v && P->x_init_xpbd &&
                                P->v_init_xpbd && P->F && P->F_trial && P->C && P->stress && P->mu && P->lam &&
                                P->ys && P->alpha && P->hardening && P->softening && P->density &&
@@ -292,6 +309,147 @@ static int check_args(const mpmx_particles* P, const mpmx_grid* G, const mpmx_st
                  "substep: null particle array");
     MPMX_REQUIRE(prm->bc == MPMX_BC_NONE || prm->bc == MPMX_BC_FRICTION, "substep: unsupported boundary condition");
     return 0;
+}
+
+static int check_args(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm) {
+    MPMX_REQUIRE(P && G && prm, "substep: null argument");
+    MPMX_REQUIRE(G->gx >= 3 && G->gy >= 3 && G->gz >= 3, "substep: bad sizes");
+    MPMX_REQUIRE(G->m && G->v_in && G->v_out, "substep: null grid array");
+    return check_particles(P, prm);
+}
+
+// Geometry of an ensemble from its scene list: tiles and nodes numbered scene
+// after scene; the device table goes into the workspace (w.table).
+static int ensemble_geo(const mpmx_scene* scenes, int nscenes, Geo& g) {
+    MPMX_REQUIRE(scenes && nscenes >= 1, "ensemble: no scenes");
+    long tiles = 0, nodes = 0;
+    for (int k = 0; k < nscenes; k++) {
+        const mpmx_scene& c = scenes[k];
+        MPMX_REQUIRE(c.gx >= 3 && c.gy >= 3 && c.gz >= 3, "ensemble: scene %d has fewer than 3 nodes on an axis", k);
+        const SceneDesc d = make_scene(c.gx, c.gy, c.gz, c.minx, c.miny, c.minz, (int)tiles, nodes);
+        tiles += (long)d.ntx * d.nty * d.ntz;
+        nodes += (long)c.gx * c.gy * c.gz;
+        if (k == 0) g.one = d;
+    }
+    MPMX_REQUIRE(tiles * kTileCells + 2 < (1L << 31), "ensemble: too many tiles for int32 keys");
+    g.table = nullptr;
+    g.nscenes = nscenes;
+    g.ntiles_ = (int)tiles;
+    g.nnodes = nodes;
+    return 0;
+}
+
+// write the scene table (nscenes + 1 entries, the last with the totals) from kernel arguments
+static int write_scene_table(const mpmx_scene* scenes, int nscenes, const Geo& g, SceneDesc* dst, hipStream_t s) {
+    SceneChunk c;
+    int tile0 = 0;
+    long node0 = 0;
+    for (int base = 0; base <= nscenes; base += kSceneChunk) {
+        const int cnt = nscenes + 1 - base < kSceneChunk ? nscenes + 1 - base : kSceneChunk;
+        for (int k = 0; k < cnt; k++) {
+            const int sidx = base + k;
+            if (sidx < nscenes) {
+                const mpmx_scene& sc = scenes[sidx];
+                c.d[k] = make_scene(sc.gx, sc.gy, sc.gz, sc.minx, sc.miny, sc.minz, tile0, node0);
+                tile0 += c.d[k].ntx * c.d[k].nty * c.d[k].ntz;
+                node0 += (long)sc.gx * sc.gy * sc.gz;
+            } else {
+                c.d[k] = SceneDesc{};
+                c.d[k].tile0 = (int)g.ntiles();
+                c.d[k].node0 = g.nnodes;
+            }
+        }
+        MPMX_LAUNCH(write_scenes_kernel, dim3(1), dim3(kSceneChunk), 0, s, c, cnt, dst + base);
+        int rc = check_launch("write_scenes", s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+static int substep_bin_geo(const mpmx_particles* P, const Geo& geo, float inv_dx, SubstepWork& w, hipStream_t s) {
+    int rc;
+    if (P->n > 0) {
+        MPMX_LAUNCH(substep_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, inv_dx, geo, w.keys);
+        if ((rc = check_launch("substep_key", s))) return rc;
+    }
+    MPMX_LAUNCH(zero_one_kernel, dim3(1), dim3(1), 0, s, w.misbinned);
+    if ((rc = check_launch("substep_clear", s))) return rc;
+    return counting_sort_ids(w.keys, P->n, substep_nkeys(geo), w.sorted, w.bin, s, P->order);
+}
+
+static int reorder_geo(const mpmx_particles* P, const Geo& geo, float inv_dx, SubstepWork& w, int32_t* perm,
+                       hipStream_t s) {
+    MPMX_LAUNCH(reorder_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, inv_dx, geo, w.keys);
+    int rc = check_launch("reorder_key", s);
+    if (rc) return rc;
+    return counting_sort_ids(w.keys, P->n, substep_nkeys(geo), perm, w.bin, s, P->order);
+}
+
+static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in, float* grid_v_out,
+                       const mpmx_step_params* prm, const Geo& geo, SubstepWork& w, hipStream_t s) {
+    int rc;
+    const long nkeys = substep_nkeys(geo);
+    const unsigned ntile_blocks = (unsigned)geo.ntiles() + 1;  // + orphan bin
+
+    TileArgs ta{};
+    ta.P = *P;
+    ta.sorted = w.sorted;
+    ta.cell_start = w.bin.cell_start;
+    ta.dx = prm->dx;
+    ta.inv_dx = prm->inv_dx;
+    ta.rpic = prm->rpic_damping;
+    ta.dt = prm->dt;
+    ta.eff = prm->eff;
+    ta.partial = w.partial;
+    ta.tile_flag = w.tile_flag;
+    ta.ntiles = (int)geo.ntiles();
+    ta.geo = geo;
+    ta.xcd = tuning(MPMX_TUNE_XCD_TILES);
+    ta.misbinned = w.misbinned;
+    MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
+    if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
+
+    CombineArgs ca{};
+    ca.geo = geo;
+    ca.partial = w.partial;
+    ca.tile_flag = w.tile_flag;
+    ca.grid_m = grid_m;
+    ca.grid_v_in = grid_v_in;
+    ca.grid_v_out = grid_v_out;
+    ca.do_update = 1;
+    ca.gxv = prm->gravity_x;
+    ca.gyv = prm->gravity_y;
+    ca.gzv = prm->gravity_z;
+    ca.dt = prm->dt;
+    ca.damping = prm->damping_scale;
+    ca.fric = prm->friction;
+    ca.gmag = prm->gravity_magnitude;
+    ca.bc = prm->bc;
+    ca.pad = prm->boundary_padding;
+    ca.zero_ints = w.bin.count;
+    ca.n_zero = nkeys;
+    const long Gn = geo.nnodes;
+    MPMX_LAUNCH(p2g_combine_kernel<>, dim3(grid_for(Gn > nkeys ? Gn : nkeys, kBlock)), dim3(kBlock), 0, s, ca);
+    if ((rc = check_launch("substep_combine(grid)", s))) return rc;
+
+    G2PArgs ga{};
+    ga.P = *P;
+    ga.sorted = w.sorted;
+    ga.cell_start = w.bin.cell_start;
+    ga.v_out = grid_v_out;
+    ga.geo = geo;
+    ga.dt = prm->dt;
+    ga.inv_dx = prm->inv_dx;
+    ga.update_cov = prm->update_cov;
+    ga.keys = w.keys;
+    ga.count = w.bin.count;
+    ga.rank = w.bin.rank;
+    ga.misbinned = w.misbinned;
+    ga.xcd = tuning(MPMX_TUNE_XCD_TILES);
+    MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
+    if ((rc = check_launch("substep_g2p", s))) return rc;
+
+    return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s, P->order);
 }
 
 }  // namespace mpmx
@@ -309,20 +467,12 @@ int mpmx_substep_bin(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
                      size_t ws_bytes, void* stream) {
     int rc = check_args(P, G, prm);
     if (rc) return rc;
-    hipStream_t s = as_stream(stream);
-    TileGeom tg = make_geom(G->gx, G->gy, G->gz);
-    MPMX_REQUIRE(substep_nkeys(tg) < (1L << 31), "substep: grid too large for int32 keys");
-    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, tg), "substep: workspace too small");
+    Geo geo = make_geom(G->gx, G->gy, G->gz, prm->minx, prm->miny, prm->minz);
+    MPMX_REQUIRE(substep_nkeys(geo) < (1L << 31), "substep: grid too large for int32 keys");
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, geo), "substep: workspace too small");
     SubstepWork w;
-    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "substep: workspace carve failed");
-    if (P->n > 0) {
-        MPMX_LAUNCH(substep_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->minx,
-                           prm->miny, prm->minz, prm->inv_dx, tg, w.keys);
-        if ((rc = check_launch("substep_key", s))) return rc;
-    }
-    MPMX_LAUNCH(zero_one_kernel, dim3(1), dim3(1), 0, s, w.misbinned);
-    if ((rc = check_launch("substep_clear", s))) return rc;
-    return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), w.sorted, w.bin, s, P->order);
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, geo, w), "substep: workspace carve failed");
+    return substep_bin_geo(P, geo, prm->inv_dx, w, as_stream(stream));
 }
 
 int mpmx_reorder_permutation(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm, int32_t* perm,
@@ -331,24 +481,20 @@ int mpmx_reorder_permutation(const mpmx_particles* P, const mpmx_grid* G, const 
     if (rc) return rc;
     MPMX_REQUIRE(P->n == 0 || perm, "mpmx_reorder_permutation: null perm");
     if (P->n == 0) return 0;
-    hipStream_t s = as_stream(stream);
-    TileGeom tg = make_geom(G->gx, G->gy, G->gz);
-    MPMX_REQUIRE(substep_nkeys(tg) < (1L << 31), "substep: grid too large for int32 keys");
-    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, tg), "mpmx_reorder_permutation: workspace too small");
+    Geo geo = make_geom(G->gx, G->gy, G->gz, prm->minx, prm->miny, prm->minz);
+    MPMX_REQUIRE(substep_nkeys(geo) < (1L << 31), "substep: grid too large for int32 keys");
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, geo), "mpmx_reorder_permutation: workspace too small");
     SubstepWork w;
-    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "mpmx_reorder_permutation: workspace carve failed");
-    MPMX_LAUNCH(reorder_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->minx, prm->miny, prm->minz,
-                prm->inv_dx, tg, w.keys);
-    if ((rc = check_launch("reorder_key", s))) return rc;
-    return counting_sort_ids(w.keys, P->n, substep_nkeys(tg), perm, w.bin, s, P->order);
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, geo, w), "mpmx_reorder_permutation: workspace carve failed");
+    return reorder_geo(P, geo, prm->inv_dx, w, perm, as_stream(stream));
 }
 
 int mpmx_substep_misbinned(void* workspace, size_t ws_bytes, int gx, int gy, int gz, int n, int* host_count,
                            void* stream) {
     MPMX_REQUIRE(host_count && gx >= 3 && gy >= 3 && gz >= 3, "mpmx_substep_misbinned: bad arguments");
-    TileGeom tg = make_geom(gx, gy, gz);
+    Geo geo = make_geom(gx, gy, gz);
     SubstepWork w;
-    MPMX_REQUIRE(workspace && carve_substep(workspace, ws_bytes, n, tg, w), "mpmx_substep_misbinned: workspace");
+    MPMX_REQUIRE(workspace && carve_substep(workspace, ws_bytes, n, geo, w), "mpmx_substep_misbinned: workspace");
     MPMX_HIP(hipMemcpyAsync(host_count, w.misbinned, sizeof(int), hipMemcpyDeviceToHost, as_stream(stream)));
     MPMX_HIP(hipStreamSynchronize(as_stream(stream)));
     return 0;
@@ -358,77 +504,66 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
                      size_t ws_bytes, void* stream) {
     int rc = check_args(P, G, prm);
     if (rc) return rc;
-    hipStream_t s = as_stream(stream);
-    TileGeom tg = make_geom(G->gx, G->gy, G->gz);
-    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, tg), "substep: workspace too small");
+    Geo geo = make_geom(G->gx, G->gy, G->gz, prm->minx, prm->miny, prm->minz);
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, geo), "substep: workspace too small");
     SubstepWork w;
-    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, tg, w), "substep: workspace carve failed");
-    const long nkeys = substep_nkeys(tg);
-    const unsigned ntile_blocks = (unsigned)tg.ntiles() + 1;  // + orphan bin
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, geo, w), "substep: workspace carve failed");
+    return substep_geo(P, G->m, G->v_in, G->v_out, prm, geo, w, as_stream(stream));
+}
 
-    TileArgs ta{};
-    ta.P = *P;
-    ta.sorted = w.sorted;
-    ta.cell_start = w.bin.cell_start;
-    ta.dx = prm->dx;
-    ta.inv_dx = prm->inv_dx;
-    ta.minx = prm->minx;
-    ta.miny = prm->miny;
-    ta.minz = prm->minz;
-    ta.rpic = prm->rpic_damping;
-    ta.dt = prm->dt;
-    ta.eff = prm->eff;
-    ta.partial = w.partial;
-    ta.tile_flag = w.tile_flag;
-    ta.ntiles = (int)tg.ntiles();
-    ta.tg = tg;
-    ta.misbinned = w.misbinned;
-    MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
-    if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
+size_t mpmx_ensemble_workspace_bytes(int n, const mpmx_scene* scenes, int nscenes) {
+    Geo geo;
+    if (n < 0 || ensemble_geo(scenes, nscenes, geo)) return 0;
+    return substep_ws_bytes(n, geo);
+}
 
-    const long Gn = (long)G->gx * G->gy * G->gz;
-    CombineArgs ca{};
-    ca.tg = tg;
-    ca.partial = w.partial;
-    ca.tile_flag = w.tile_flag;
-    ca.grid_m = G->m;
-    ca.grid_v_in = G->v_in;
-    ca.grid_v_out = G->v_out;
-    ca.do_update = 1;
-    ca.gxv = prm->gravity_x;
-    ca.gyv = prm->gravity_y;
-    ca.gzv = prm->gravity_z;
-    ca.dt = prm->dt;
-    ca.damping = prm->damping_scale;
-    ca.fric = prm->friction;
-    ca.gmag = prm->gravity_magnitude;
-    ca.bc = prm->bc;
-    ca.pad = prm->boundary_padding;
-    ca.zero_ints = w.bin.count;
-    ca.n_zero = nkeys;
-    MPMX_LAUNCH(p2g_combine_kernel<>, dim3(grid_for(Gn > nkeys ? Gn : nkeys, kBlock)), dim3(kBlock), 0, s, ca);
-    if ((rc = check_launch("substep_combine(grid)", s))) return rc;
+// shared set-up of the ensemble entry points: geometry, workspace, scene table
+static int ensemble_setup(const mpmx_particles* P, const mpmx_scene* scenes, int nscenes,
+                          const mpmx_step_params* prm, void* workspace, size_t ws_bytes, hipStream_t s, Geo& geo,
+                          SubstepWork& w) {
+    int rc = check_particles(P, prm);
+    if (rc) return rc;
+    MPMX_REQUIRE(P->n == 0 || P->scene, "ensemble: particles need a scene array");
+    if ((rc = ensemble_geo(scenes, nscenes, geo))) return rc;
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, geo), "ensemble: workspace too small");
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, geo, w), "ensemble: workspace carve failed");
+    geo.table = w.table;
+    return write_scene_table(scenes, nscenes, geo, w.table, s);
+}
 
-    G2PArgs ga{};
-    ga.P = *P;
-    ga.sorted = w.sorted;
-    ga.cell_start = w.bin.cell_start;
-    ga.v_out = G->v_out;
-    ga.tg = tg;
-    ga.dt = prm->dt;
-    ga.inv_dx = prm->inv_dx;
-    ga.minx = prm->minx;
-    ga.miny = prm->miny;
-    ga.minz = prm->minz;
-    ga.update_cov = prm->update_cov;
-    ga.keys = w.keys;
-    ga.count = w.bin.count;
-    ga.rank = w.bin.rank;
-    ga.misbinned = w.misbinned;
-    MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
-    if ((rc = check_launch("substep_g2p", s))) return rc;
+int mpmx_ensemble_bin(const mpmx_particles* P, const mpmx_scene* scenes, int nscenes, const mpmx_step_params* prm,
+                      void* workspace, size_t ws_bytes, void* stream) {
+    Geo geo;
+    SubstepWork w;
+    hipStream_t s = as_stream(stream);
+    int rc = ensemble_setup(P, scenes, nscenes, prm, workspace, ws_bytes, s, geo, w);
+    if (rc) return rc;
+    return substep_bin_geo(P, geo, prm->inv_dx, w, s);
+}
 
-    return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s, P->order);
+int mpmx_ensemble_reorder_permutation(const mpmx_particles* P, const mpmx_scene* scenes, int nscenes,
+                                      const mpmx_step_params* prm, int32_t* perm, void* workspace, size_t ws_bytes,
+                                      void* stream) {
+    MPMX_REQUIRE(P && (P->n == 0 || perm), "mpmx_ensemble_reorder_permutation: null argument");
+    Geo geo;
+    SubstepWork w;
+    hipStream_t s = as_stream(stream);
+    int rc = ensemble_setup(P, scenes, nscenes, prm, workspace, ws_bytes, s, geo, w);
+    if (rc) return rc;
+    if (P->n == 0) return 0;
+    return reorder_geo(P, geo, prm->inv_dx, w, perm, s);
+}
+
+int mpmx_ensemble_substep(const mpmx_particles* P, const mpmx_scene* scenes, int nscenes, float* grid_m,
+                          float* grid_v_in, float* grid_v_out, const mpmx_step_params* prm, void* workspace,
+                          size_t ws_bytes, void* stream) {
+    MPMX_REQUIRE(grid_m && grid_v_in && grid_v_out, "ensemble: null grid array");
+    Geo geo;
+    SubstepWork w;
+    hipStream_t s = as_stream(stream);
+    int rc = ensemble_setup(P, scenes, nscenes, prm, workspace, ws_bytes, s, geo, w);
+    if (rc) return rc;
+    return substep_geo(P, grid_m, grid_v_in, grid_v_out, prm, geo, w, s);
 }
 
 }  // extern "C"

@@ -97,13 +97,14 @@ class Simulation:
             (SR.xpbdSimulationStepFused if self.fused else SR.xpbdSimulationStep)(self.sim, self.mpm, self.xpbd)
 
     def reorder(self) -> None:
-        """Permute every per-particle array into the fused substep's tile order
-        (tile by tile, cell by cell, particle id ascending inside a cell), so
-        that each tile's particles are contiguous in HBM and the substep's loads
-        coalesce.  ``sim.particle_id`` follows along and keeps each particle's
-        index in the input; the bins and hash tables order particles by it, so
-        a reordered run gives the same bits as one that never reorders (the
-        FP64 convergence sums aside, whose order follows the arrays)."""
+        """Permute every per-particle array in HBM into the fused substep's tile
+        order (tile by tile, cell by cell, input index ascending inside a cell),
+        so that each tile's particles are contiguous and the substep's loads
+        coalesce.  ``sim.particle_id`` holds each slot's input index; the bins
+        and hash tables order particles by it, so a reordered run gives the same
+        bits as one that never reorders (the FP64 convergence sums aside, whose
+        order follows the slots), and the host views (numpy, assign) keep the
+        input order: the permutation is invisible outside the device."""
         SR = simulationRoutines
         sim, mpm, xpbd = self.sim, self.mpm, self.xpbd
         n = sim.nPoints
@@ -118,9 +119,9 @@ class Simulation:
                         or len(a.shape) != 1):
                     continue
                 if a.shape[0] == n:
-                    a.permute_(perm)
+                    a.permute_(perm, 1, None if a is sim.particle_id else sim.particle_id)
                 elif a.shape[0] == 6 * n:  # the covariance arrays hold 6 floats per particle
-                    a.permute_(perm, 6)
+                    a.permute_(perm, 6, sim.particle_id)
                 else:
                     continue
                 seen.add(id(a))

@@ -64,6 +64,12 @@ class Array:
         # bumped by every write issued from the host (assign from numpy, fill_):
         # the fused substep rebuilds its particle bins when x or the labels change
         self.version = 0
+        # set by permute_: the items sit in HBM in a permuted order, and
+        # numpy()/assign() translate from and to the input order, so the
+        # permutation stays invisible to the host (``_order[i]`` is the input
+        # index of item i, ``_words`` the 32-bit words per item)
+        self._order = None
+        self._words = 1
 
     # --- pointers ---------------------------------------------------------
     @property
@@ -74,9 +80,32 @@ class Array:
         return self.shape[0]
 
     # --- wp.array API used by the reference -------------------------------
+    def _to_input_order(self, h: np.ndarray) -> np.ndarray:
+        """host copy of the storage (SoA planes) -> same layout in input order"""
+        if self._order is None:
+            return h
+        order = self._order.t.detach().cpu().numpy()
+        w = self._words
+        if h.ndim == 2:  # planes
+            out = np.empty_like(h)
+            out[:, order] = h
+            return out
+        out = np.empty_like(h).reshape(-1, w)
+        out[order] = h.reshape(-1, w)
+        return out.reshape(h.shape)
+
+    def _from_input_order(self, h: np.ndarray) -> np.ndarray:
+        if self._order is None:
+            return h
+        order = self._order.t.detach().cpu().numpy()
+        w = self._words
+        if h.ndim == 2:
+            return np.ascontiguousarray(h[:, order])
+        return np.ascontiguousarray(h.reshape(-1, w)[order].reshape(h.shape))
+
     def numpy(self) -> np.ndarray:
-        """Synchronising download in the reference's AoS shape."""
-        h = self.t.detach().cpu().numpy()
+        """Synchronising download in the reference's AoS shape (input order)."""
+        h = self._to_input_order(self.t.detach().cpu().numpy())
         nc = _NCOMP[self.dtype]
         if nc == 1:
             return h.reshape(self.shape)
@@ -95,8 +124,8 @@ class Array:
         npdt = _NUMPY[self.dtype]
         a = np.ascontiguousarray(a, dtype=npdt).reshape(self.size, nc) if nc > 1 else \
             np.ascontiguousarray(a, dtype=npdt).reshape(self.size)
-        host = torch.from_numpy(np.ascontiguousarray(a.T) if nc > 1 else a)
-        self.t.copy_(host.to(self.t.device, non_blocking=False))
+        host = self._from_input_order(np.ascontiguousarray(a.T) if nc > 1 else a)
+        self.t.copy_(torch.from_numpy(host).to(self.t.device, non_blocking=False))
         self.version += 1
         return self
 
@@ -118,9 +147,11 @@ class Array:
             lib("mpmx_fill_f32", self.ptr, self.size * nc, float(value), stream_handle())
         return self
 
-    def permute_(self, perm: "Array", item_words: int = 1) -> "Array":
+    def permute_(self, perm: "Array", item_words: int = 1, order: "Array | None" = None) -> "Array":
         """Reorder the items by a device permutation (new[i] = old[perm[i]]):
-        SoA components as planes, ``item_words`` 32-bit words per item."""
+        SoA components as planes, ``item_words`` 32-bit words per item.  With
+        ``order`` (the input index of every item after the permutation) the
+        host views keep showing the input order."""
         if self.dtype == float64:
             raise MpmxError("permute_: float64 arrays are not per-particle state")
         nc = _NCOMP[self.dtype]
@@ -132,6 +163,8 @@ class Array:
             stream_handle())
         self.t = new
         self.version += 1
+        self._order = order
+        self._words = int(item_words)
         return self
 
     def __repr__(self):
