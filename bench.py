@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     p.add_argument("--ensemble-scenes", type=int, default=64, help="scenes of the batched ensemble (--scene ensemble)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run the 256 MiB copy that calibrates the HBM byte counters")
     return p.parse_args()
 
 
@@ -120,8 +122,38 @@ def profile_report():
     return out
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_substep_tile.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None, None
+    if d.get("kernel") != kernel:
+        return None, None
+    return float(d["bytes_per_launch"]), d.get("source")
+
+
+def copy_calibration():
+    """A plain copy of 256 MiB through the library's copy kernel, three times:
+    the reference point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    from mpmwarpydw_amd._lib import lib
+    n = 64 << 20
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty(n, device="cuda")
+    for _ in range(3):
+        lib("mpmx_copy_f32", dst.data_ptr(), src.data_ptr(), n, None)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        return copy_calibration()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (every box here has one GPU).",
               file=sys.stderr)
@@ -198,6 +230,7 @@ def main():
     tile = kernels.get("substep_tile(stress+p2g)")
     achieved = TILE_BYTES_PER_PARTICLE * n_mpm / (tile["avg_us"] * 1e-6) / 1e9 if tile else None
     sub_bytes = 300.0 * n_mpm + 28.0 * G
+    traffic, traffic_src = measured_traffic("substep_tile(stress+p2g)") if a.scene == "undercut_slab" else (None, None)
     out = {
         "metric": "MPM particle-substeps/s (plain method)",
         "value": value,
@@ -217,7 +250,9 @@ def main():
                    "particle_order": "input order" if a.no_reorder else "tile order (reordered once at setup)"},
         "roofline": {"bound": "hbm", "kernel": "substep_tile(stress+p2g)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None},
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": TILE_BYTES_PER_PARTICLE * n_mpm},
         "substep_algorithmic_bytes": sub_bytes,
         "substep_hbm_frac": sub_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
         "dominant_kernel": dom_name,
