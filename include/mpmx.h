@@ -34,9 +34,16 @@ extern "C" {
 #define MPMX_EINVAL (-1)   /* bad argument (null pointer, negative size, ...) */
 #define MPMX_ERANGE (-2)   /* size beyond a documented maximum */
 
-/* boundary-condition selector of mpmx_grid_update (simulationRoutines.py:124-228) */
+/* boundary-condition selector of mpmx_grid_update (simulationRoutines.py:124-228):
+ * boundaryCondition "friction", "friction_gradual", "restitution", "absorbing",
+ * "plane_strain_y", "plane_strain_y_global" */
 #define MPMX_BC_NONE 0
 #define MPMX_BC_FRICTION 1
+#define MPMX_BC_FRICTION_GRADUAL 2
+#define MPMX_BC_RESTITUTION 3
+#define MPMX_BC_ABSORBING 4
+#define MPMX_BC_PLANE_STRAIN_Y 5
+#define MPMX_BC_PLANE_STRAIN_Y_GLOBAL 6
 
 /* Device pointers of the per-particle state arrays (the MPMState/SimState/
  * XPBDState fields of simStates.py, SoA layout) for the fused substep. */
@@ -108,6 +115,9 @@ typedef struct mpmx_step_params {
     int boundary_padding;
     int update_cov;
     int constitutive_model;   /* passed through, never read (mpmRoutines.py:253) */
+    float restitution;        /* boundRestitution (MPMX_BC_RESTITUTION) */
+    int vlc;                  /* volumetric_locking_correction after G2P; the fused
+                                 substep keeps the cell sums in its workspace */
 } mpmx_step_params;
 
 const char* mpmx_last_error(void);
@@ -216,7 +226,7 @@ int mpmx_p2g_grid_update(int n, const int32_t* activeLabel, const int32_t* mater
                          float miny, float minz, float rpic_damping, int gx, int gy, int gz,
                          float* grid_m, float* grid_v_in, float* grid_v_out, float gxv, float gyv,
                          float gzv, float dt, float damping_scale, int bc, float friction,
-                         float gravity_magnitude, int boundary_padding, void* workspace,
+                         float gravity_magnitude, int boundary_padding, float restitution, void* workspace,
                          size_t ws_bytes, void* stream);
 
 /* mpmSimulationStep (simulationRoutines.py:6-250: stress, P2G, grid update,
@@ -301,7 +311,34 @@ int mpmx_collide_bounds(float* grid_v_out, int gx, int gy, int gz, float frictio
 int mpmx_grid_update(int gx, int gy, int gz, const float* grid_m, const float* grid_v_in,
                      float* grid_v_out, float gxv, float gyv, float gzv, float dt,
                      float damping_scale, int bc, float friction, float gravity_magnitude,
-                     int boundary_padding, void* stream);
+                     int boundary_padding, float restitution, void* stream);
+/* the other boundary conditions of simulationRoutines.py:141-228, one kernel
+ * each in the reference: collideBoundsGradualFriction (mpmRoutines.py:1249-1340),
+ * collideBoundsRestitution (:1179-1247), collideBoundsAbsorbing (:1126-1176),
+ * collideBoundsPlaneStrainY (:996-1062), collideBoundsPlaneStrainYGlobal
+ * (:1065-1123); bc selects one (MPMX_BC_*, MPMX_BC_FRICTION = collideBounds) */
+int mpmx_collide_bounds_variant(float* grid_v_out, int gx, int gy, int gz, int bc, float friction,
+                                float dt, float gravity_magnitude, int boundary_padding,
+                                float restitution, void* stream);
+/* Volumetric locking correction (mpmRoutines.py:805-911, after G2P,
+ * simulationRoutines.py:255-293).  accumulate_cell_divergence: per grid cell
+ * int((x - min) inv_dx) the sums of tr(C) vol and vol over active MPM particles,
+ * into cell_div / cell_vol [gx*gy*gz] (overwritten; they are summed in 64-bit
+ * fixed point, so the result does not depend on the order of the atomics).
+ * apply: C += (cell_div/cell_vol - tr C) / 3 I.  Particles whose cell is outside
+ * the grid are skipped (the reference would index out of bounds).  Workspace:
+ * mpmx_vlc_workspace_bytes(gx, gy, gz). */
+size_t mpmx_vlc_workspace_bytes(int gx, int gy, int gz);
+int mpmx_accumulate_cell_divergence(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                                    const float* x, const float* C, const float* vol, float inv_dx,
+                                    float minx, float miny, float minz, int gx, int gy, int gz,
+                                    float* cell_div, float* cell_vol, void* workspace,
+                                    size_t ws_bytes, void* stream);
+int mpmx_apply_volumetric_locking_correction(int n, const int32_t* activeLabel,
+                                             const int32_t* materialLabel, const float* x, float* C,
+                                             float inv_dx, float minx, float miny, float minz,
+                                             int gx, int gy, int gz, const float* cell_div,
+                                             const float* cell_vol, void* stream);
 
 /* g2p (+ update_cov), mpmRoutines.py:715-802; launched simulationRoutines.py:231-250.
  * cov is float[6*n] packed per particle [xx,xy,xz,yy,yz,zz] (the reference layout). */

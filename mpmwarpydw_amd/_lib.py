@@ -64,11 +64,15 @@ SIGNATURES = {
     "mpmx_p2g_workspace_bytes": [I, I, I, I],
     "mpmx_p2g_apic_with_stress": [I, P, P, P, P, P, P, P, P, F, F, F, F, F, F, I, I, I, P, P, F, P, S, IP, P],
     "mpmx_p2g_grid_update": [I, P, P, P, P, P, P, P, P, F, F, F, F, F, F, I, I, I, P, P, P, F, F, F, F, F, I, F, F,
-                             I, P, S, P],
+                             I, F, P, S, P],
     "mpmx_grid_normalization_and_gravity": [I, I, I, P, P, P, F, F, F, F, P],
     "mpmx_add_damping_via_grid": [I, I, I, P, F, P],
     "mpmx_collide_bounds": [P, I, I, I, F, F, F, I, P],
-    "mpmx_grid_update": [I, I, I, P, P, P, F, F, F, F, F, I, F, F, I, P],
+    "mpmx_grid_update": [I, I, I, P, P, P, F, F, F, F, F, I, F, F, I, F, P],
+    "mpmx_collide_bounds_variant": [P, I, I, I, I, F, F, F, I, F, P],
+    "mpmx_vlc_workspace_bytes": [I, I, I],
+    "mpmx_accumulate_cell_divergence": [I, P, P, P, P, P, F, F, F, F, I, I, I, P, P, P, S, P],
+    "mpmx_apply_volumetric_locking_correction": [I, P, P, P, P, F, F, F, F, I, I, I, P, P, P],
     "mpmx_g2p": [F, I, P, P, P, P, P, P, P, P, F, P, I, I, I, I, F, F, F, P],
     "mpmx_reduce_workspace_bytes": [I],
     "mpmx_velocity_convergence": [I, P, P, P, P, P, P, S, P],
@@ -102,6 +106,7 @@ SIGNATURES = {
 }
 _RESTYPE = {
     "mpmx_substep_workspace_bytes": S,
+    "mpmx_vlc_workspace_bytes": S,
     "mpmx_last_error": C.c_char_p,
     "mpmx_p2g_workspace_bytes": S,
     "mpmx_reduce_workspace_bytes": S,
@@ -150,7 +155,7 @@ class StepParams(_Strict):
     _fields_ = [(name, C.c_float) for name in (
         "dt", "dx", "inv_dx", "minx", "miny", "minz", "gravity_x", "gravity_y", "gravity_z", "rpic_damping", "eff",
         "damping_scale", "friction", "gravity_magnitude")] + [(name, C.c_int) for name in (
-        "bc", "boundary_padding", "update_cov", "constitutive_model")]
+        "bc", "boundary_padding", "update_cov", "constitutive_model")] + [("restitution", C.c_float), ("vlc", C.c_int)]
 
 
 class MpmxError(RuntimeError):

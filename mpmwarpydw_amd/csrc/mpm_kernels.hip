@@ -77,7 +77,7 @@ __global__ void collide_kernel(float* __restrict__ vout, int gx, int gy, int gz,
 namespace mpmx {
 __global__ void grid_update_kernel(int gx, int gy, int gz, const float* __restrict__ m, const float* __restrict__ vin,
                                    float* __restrict__ vout, float gxv, float gyv, float gzv, float dt, float damping,
-                                   int bc, float fric, float gmag, int pad) {
+                                   int bc, float fric, float gmag, int pad, float restitution) {
     long G = (long)gx * gy * gz;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G) return;
@@ -85,8 +85,65 @@ __global__ void grid_update_kernel(int gx, int gy, int gz, const float* __restri
     int iy = (int)((i / gz) % gy);
     int ix = (int)(i / ((long)gy * gz));
     f3 v = grid_update_node(m[i], ld3(vin, G, i), ix, iy, iz, gx, gy, gz, gxv, gyv, gzv, dt, damping, bc, fric,
-                            gmag, pad);
+                            gmag, pad, restitution);
     st3(vout, G, i, v);
+}
+
+// the boundary-condition kernels other than collideBounds, on every node
+__global__ void collide_variant_kernel(float* __restrict__ vout, int gx, int gy, int gz, int bc, float mu, float dt,
+                                       float g, int pad, float restitution) {
+    long G = (long)gx * gy * gz;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    int iz = (int)(i % gz);
+    int iy = (int)((i / gz) % gy);
+    int ix = (int)(i / ((long)gy * gz));
+    st3(vout, G, i, apply_bc(ld3(vout, G, i), ix, iy, iz, gx, gy, gz, bc, mu, dt, g, pad, restitution));
+}
+
+// --------------------------------------------------------------------------
+// volumetric locking correction (mpmRoutines.py:805-911)
+// --------------------------------------------------------------------------
+__global__ void vlc_accumulate_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ label,
+                                      const float* __restrict__ x, const float* __restrict__ C,
+                                      const float* __restrict__ vol, float inv_dx, float minx, float miny, float minz,
+                                      int gx, int gy, int gz, unsigned long long* __restrict__ acc) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || active[p] != 1 || label[p] > 1) return;
+    long k;
+    if (!vlc_cell(ld3(x, n, p), minx, miny, minz, inv_dx, gx, gy, gz, k)) return;
+    const float div = (C[p] + C[4L * n + p]) + C[8L * n + p];
+    const long G = (long)gx * gy * gz;
+    atomicAdd(&acc[k], vlc_fixed((double)div * (double)vol[p]));
+    atomicAdd(&acc[G + k], vlc_fixed((double)vol[p]));
+}
+
+__global__ void vlc_finalize_kernel(long G, unsigned long long* __restrict__ acc, float* __restrict__ cell_div,
+                                    float* __restrict__ cell_vol) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= G) return;
+    cell_div[k] = vlc_unfixed(acc[k]);
+    cell_vol[k] = vlc_unfixed(acc[G + k]);
+}
+
+__global__ void vlc_apply_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ label,
+                                 const float* __restrict__ x, float* __restrict__ C, float inv_dx, float minx,
+                                 float miny, float minz, int gx, int gy, int gz, const float* __restrict__ cell_div,
+                                 const float* __restrict__ cell_vol) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || active[p] != 1 || label[p] > 1) return;
+    long k;
+    if (!vlc_cell(ld3(x, n, p), minx, miny, minz, inv_dx, gx, gy, gz, k)) return;
+    const float c00 = C[p], c11 = C[4L * n + p], c22 = C[8L * n + p];
+    const float corr = vlc_shift(c00, c11, c22, cell_div[k], cell_vol[k]);
+    C[p] = c00 + corr;
+    C[4L * n + p] = c11 + corr;
+    C[8L * n + p] = c22 + corr;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 // --------------------------------------------------------------------------
@@ -260,15 +317,70 @@ int mpmx_collide_bounds(float* grid_v_out, int gx, int gy, int gz, float frictio
 
 int mpmx_grid_update(int gx, int gy, int gz, const float* grid_m, const float* grid_v_in, float* grid_v_out,
                      float gxv, float gyv, float gzv, float dt, float damping_scale, int bc, float friction,
-                     float gmag, int boundary_padding, void* stream) {
+                     float gmag, int boundary_padding, float restitution, void* stream) {
     MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_m && grid_v_in && grid_v_out, "mpmx_grid_update: bad arguments");
-    MPMX_REQUIRE(bc == MPMX_BC_NONE || bc == MPMX_BC_FRICTION, "mpmx_grid_update: unsupported boundary condition %d",
-                 bc);
+    MPMX_REQUIRE(bc >= MPMX_BC_NONE && bc <= MPMX_BC_PLANE_STRAIN_Y_GLOBAL,
+                 "mpmx_grid_update: unsupported boundary condition %d", bc);
     long G = (long)gx * gy * gz;
     MPMX_LAUNCH(grid_update_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), gx, gy, gz,
                        grid_m, grid_v_in, grid_v_out, gxv, gyv, gzv, dt, damping_scale, bc, friction, gmag,
-                       boundary_padding);
+                       boundary_padding, restitution);
     return check_launch("grid_update", as_stream(stream));
+}
+
+int mpmx_collide_bounds_variant(float* grid_v_out, int gx, int gy, int gz, int bc, float friction, float dt,
+                                float gmag, int boundary_padding, float restitution, void* stream) {
+    MPMX_REQUIRE(gx > 0 && gy > 0 && gz > 0 && grid_v_out, "mpmx_collide_bounds_variant: bad arguments");
+    MPMX_REQUIRE(bc >= MPMX_BC_FRICTION && bc <= MPMX_BC_PLANE_STRAIN_Y_GLOBAL,
+                 "mpmx_collide_bounds_variant: unsupported boundary condition %d", bc);
+    long G = (long)gx * gy * gz;
+    MPMX_LAUNCH(collide_variant_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, as_stream(stream), grid_v_out, gx,
+                gy, gz, bc, friction, dt, gmag, boundary_padding, restitution);
+    return check_launch("collide_bounds_variant", as_stream(stream));
+}
+
+size_t mpmx_vlc_workspace_bytes(int gx, int gy, int gz) {
+    if (gx <= 0 || gy <= 0 || gz <= 0) return 0;
+    return Carver::bytes_for(2L * gx * gy * gz, sizeof(unsigned long long));
+}
+
+int mpmx_accumulate_cell_divergence(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x,
+                                    const float* C, const float* vol, float inv_dx, float minx, float miny,
+                                    float minz, int gx, int gy, int gz, float* cell_div, float* cell_vol,
+                                    void* workspace, size_t ws_bytes, void* stream) {
+    MPMX_REQUIRE(n >= 0 && gx > 0 && gy > 0 && gz > 0 && cell_div && cell_vol,
+                 "mpmx_accumulate_cell_divergence: bad arguments");
+    MPMX_REQUIRE(n == 0 || (activeLabel && materialLabel && x && C && vol),
+                 "mpmx_accumulate_cell_divergence: null particle array");
+    MPMX_REQUIRE(workspace && ws_bytes >= mpmx_vlc_workspace_bytes(gx, gy, gz),
+                 "mpmx_accumulate_cell_divergence: workspace too small");
+    hipStream_t s = as_stream(stream);
+    const long G = (long)gx * gy * gz;
+    unsigned long long* acc = static_cast<unsigned long long*>(workspace);
+    MPMX_LAUNCH(zero_u64_kernel, dim3(grid_for(2 * G, kBlock)), dim3(kBlock), 0, s, acc, 2 * G);
+    int rc = check_launch("vlc_zero", s);
+    if (rc) return rc;
+    if (n > 0) {
+        MPMX_LAUNCH(vlc_accumulate_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, activeLabel,
+                    materialLabel, x, C, vol, inv_dx, minx, miny, minz, gx, gy, gz, acc);
+        if ((rc = check_launch("vlc_accumulate", s))) return rc;
+    }
+    MPMX_LAUNCH(vlc_finalize_kernel, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, s, G, acc, cell_div, cell_vol);
+    return check_launch("vlc_finalize", s);
+}
+
+int mpmx_apply_volumetric_locking_correction(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                                             const float* x, float* C, float inv_dx, float minx, float miny,
+                                             float minz, int gx, int gy, int gz, const float* cell_div,
+                                             const float* cell_vol, void* stream) {
+    MPMX_REQUIRE(n >= 0 && gx > 0 && gy > 0 && gz > 0 && cell_div && cell_vol,
+                 "mpmx_apply_volumetric_locking_correction: bad arguments");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(activeLabel && materialLabel && x && C, "mpmx_apply_volumetric_locking_correction: null pointer");
+    hipStream_t s = as_stream(stream);
+    MPMX_LAUNCH(vlc_apply_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, activeLabel, materialLabel, x, C,
+                inv_dx, minx, miny, minz, gx, gy, gz, cell_div, cell_vol);
+    return check_launch("vlc_apply", s);
 }
 
 int mpmx_g2p(float dt, int n, const int32_t* activeLabel, const int32_t* materialLabel, float* x, float* v,

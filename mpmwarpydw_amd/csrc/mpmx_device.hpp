@@ -229,19 +229,162 @@ MPMX_HD f3 collide_node(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, fl
 }
 
 
+MPMX_HD float& comp(f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+MPMX_HD float compc(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// collideBoundsPlaneStrainY (mpmRoutines.py:996-1062): x and z faces as
+// collideBounds; on a y face the node is read again (it becomes the v that the
+// z faces test) and its v_y is zeroed whatever its sign.
+MPMX_HD f3 collide_plane_strain_y(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, float mu, float dt, float g,
+                                  int pad) {
+    f3 o = v;
+    if (ix <= pad && v.x < 0.0f) o = f3{0.0f, coulomb(v.y, mu, dt, g), coulomb(v.z, mu, dt, g)};
+    if (ix >= gx - pad && v.x > 0.0f) o = f3{0.0f, coulomb(v.y, mu, dt, g), coulomb(v.z, mu, dt, g)};
+    if (iy <= pad) {
+        v = o;
+        o = f3{v.x, 0.0f, v.z};
+    }
+    if (iy >= gy - pad) {
+        v = o;
+        o = f3{v.x, 0.0f, v.z};
+    }
+    if (iz <= pad && v.z < 0.0f) o = f3{coulomb(v.x, mu, dt, g), coulomb(v.y, mu, dt, g), 0.0f};
+    if (iz >= gz - pad && v.z > 0.0f) o = f3{coulomb(v.x, mu, dt, g), coulomb(v.y, mu, dt, g), 0.0f};
+    return o;
+}
+
+// collideBoundsPlaneStrainYGlobal (mpmRoutines.py:1065-1123): v_y = 0 on every
+// node, then x and z faces as collideBounds on that velocity.
+MPMX_HD f3 collide_plane_strain_y_global(f3 v, int ix, int iz, int gx, int gz, float mu, float dt, float g, int pad) {
+    v.y = 0.0f;
+    f3 o = v;
+    if (ix <= pad && v.x < 0.0f) o = f3{0.0f, 0.0f, coulomb(v.z, mu, dt, g)};
+    if (ix >= gx - pad && v.x > 0.0f) o = f3{0.0f, 0.0f, coulomb(v.z, mu, dt, g)};
+    if (iz <= pad && v.z < 0.0f) o = f3{coulomb(v.x, mu, dt, g), 0.0f, 0.0f};
+    if (iz >= gz - pad && v.z > 0.0f) o = f3{coulomb(v.x, mu, dt, g), 0.0f, 0.0f};
+    return o;
+}
+
+// collideBoundsAbsorbing (mpmRoutines.py:1126-1176): the three outer node
+// layers of every face (a literal 3) scale the node's original v by
+// 0 * lf + 1 * (1 - lf), lf = (pad - distance) / pad; the last matching face wins.
+MPMX_HD f3 collide_absorbing(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, int pad) {
+    const float P = (float)pad;
+    const int idx[3] = {ix, iy, iz}, dim[3] = {gx, gy, gz};
+    f3 o = v;
+    for (int a = 0; a < 3; a++) {
+        for (int side = 0; side < 2; side++) {
+            const bool hit = side == 0 ? idx[a] < 3 : idx[a] >= dim[a] - 3;
+            if (!hit) continue;
+            const int dist = side == 0 ? idx[a] : dim[a] - 1 - idx[a];
+            const float lf = (P - (float)dist) / P;
+            const float d = 0.0f * lf + 1.0f * (1.0f - lf);
+            o = f3{v.x * d, v.y * d, v.z * d};
+        }
+    }
+    return o;
+}
+
+// collideBoundsRestitution (mpmRoutines.py:1179-1247): the normal component of
+// an approaching node is reversed and scaled by e, the tangential ones get
+// Coulomb friction; all faces from the original v, the last match wins.
+MPMX_HD f3 collide_restitution(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, float e, float mu, float dt,
+                               float g, int pad) {
+    f3 o = v;
+    const int idx[3] = {ix, iy, iz}, dim[3] = {gx, gy, gz};
+    for (int a = 0; a < 3; a++) {
+        const float vn = compc(v, a);
+        for (int side = 0; side < 2; side++) {
+            const bool hit = side == 0 ? (idx[a] <= pad && vn < 0.0f) : (idx[a] >= dim[a] - pad && vn > 0.0f);
+            if (!hit) continue;
+            for (int b = 0; b < 3; b++) comp(o, b) = b == a ? -e * vn : coulomb(compc(v, b), mu, dt, g);
+        }
+    }
+    return o;
+}
+
+// collideBoundsGradualFriction (mpmRoutines.py:1249-1340): inside the padding an
+// approaching node keeps fade = distance / pad of its normal velocity and gets
+// friction mu (1 - fade) on the others; per axis the upper face is tested only
+// when the lower one does not apply (elif); axes in turn, all from the original v.
+MPMX_HD f3 collide_gradual(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, float mu, float dt, float g,
+                           int pad) {
+    const float P = (float)pad;
+    const int idx[3] = {ix, iy, iz}, dim[3] = {gx, gy, gz};
+    f3 o = v;
+    for (int a = 0; a < 3; a++) {
+        const float vn = compc(v, a);
+        float fade;
+        if (idx[a] < pad && vn < 0.0f)
+            fade = (float)idx[a] / P;
+        else if (idx[a] >= dim[a] - pad && vn > 0.0f)
+            fade = (float)(dim[a] - 1 - idx[a]) / P;
+        else
+            continue;
+        const float m = mu * (1.0f - fade);
+        for (int b = 0; b < 3; b++)
+            comp(o, b) = b == a ? 0.0f * (1.0f - fade) + vn * fade : coulomb(compc(v, b), m, dt, g);
+    }
+    return o;
+}
+
+MPMX_HD f3 apply_bc(f3 v, int ix, int iy, int iz, int gx, int gy, int gz, int bc, float fric, float dt, float gmag,
+                    int pad, float restitution) {
+    switch (bc) {
+        case MPMX_BC_FRICTION: return collide_node(v, ix, iy, iz, gx, gy, gz, fric, dt, gmag, pad);
+        case MPMX_BC_FRICTION_GRADUAL: return collide_gradual(v, ix, iy, iz, gx, gy, gz, fric, dt, gmag, pad);
+        case MPMX_BC_RESTITUTION:
+            return collide_restitution(v, ix, iy, iz, gx, gy, gz, restitution, fric, dt, gmag, pad);
+        case MPMX_BC_ABSORBING: return collide_absorbing(v, ix, iy, iz, gx, gy, gz, pad);
+        case MPMX_BC_PLANE_STRAIN_Y: return collide_plane_strain_y(v, ix, iy, iz, gx, gy, gz, fric, dt, gmag, pad);
+        case MPMX_BC_PLANE_STRAIN_Y_GLOBAL: return collide_plane_strain_y_global(v, ix, iz, gx, gz, fric, dt, gmag, pad);
+        default: return v;
+    }
+}
+
 // Steps 4-6 of mpmSimulationStep on one node (simulationRoutines.py:94-140):
 // normalisation + gravity (v = 0 where m <= 1e-15), damping when the scale is
 // below 1, then the boundary condition.
 MPMX_HD f3 grid_update_node(float m, f3 vin, int ix, int iy, int iz, int gx, int gy, int gz, float gxv, float gyv,
-                               float gzv, float dt, float damping, int bc, float fric, float gmag, int pad) {
+                               float gzv, float dt, float damping, int bc, float fric, float gmag, int pad,
+                               float restitution) {
     f3 v = f3{0.0f, 0.0f, 0.0f};
     if (m > 1.0e-15f) {
         float inv = 1.0f / m;
         v = f3{vin.x * inv + dt * gxv, vin.y * inv + dt * gyv, vin.z * inv + dt * gzv};
     }
     if (damping < 1.0f) v = f3{v.x * damping, v.y * damping, v.z * damping};
-    if (bc == MPMX_BC_FRICTION) v = collide_node(v, ix, iy, iz, gx, gy, gz, fric, dt, gmag, pad);
-    return v;
+    return apply_bc(v, ix, iy, iz, gx, gy, gz, bc, fric, dt, gmag, pad, restitution);
+}
+
+// ---------------------------------------------------------------------------
+// volumetric locking correction (mpmRoutines.py:818-911).  Per cell the sums
+// of tr(C) V and of V are kept in 64-bit fixed point with 2^-40 units, so that
+// integer atomics make them independent of the order of the additions.
+// ---------------------------------------------------------------------------
+constexpr double kVlcScale = 1099511627776.0;  // 2^40
+
+MPMX_HD unsigned long long vlc_fixed(double v) {
+    double q = rint(v * kVlcScale);
+    const double lim = 4.0e18;
+    q = q > lim ? lim : (q < -lim ? -lim : q);
+    return (unsigned long long)(long long)q;
+}
+
+MPMX_HD float vlc_unfixed(unsigned long long q) { return (float)((double)(long long)q / kVlcScale); }
+
+// the particle's cell: wp.int truncation of (x - min) / dx, as :841-844
+MPMX_HD bool vlc_cell(f3 x, float minx, float miny, float minz, float inv_dx, int gx, int gy, int gz, long& k) {
+    const int cx = (int)((x.x - minx) * inv_dx), cy = (int)((x.y - miny) * inv_dx), cz = (int)((x.z - minz) * inv_dx);
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= gx || cy >= gy || cz >= gz) return false;
+    k = ((long)cx * gy + cy) * gz + cz;
+    return true;
+}
+
+// the diagonal shift of C toward the cell's volume-weighted divergence (:888-908)
+MPMX_HD float vlc_shift(float c00, float c11, float c22, float cell_div, float cell_vol) {
+    const float dc = cell_vol > 1e-20f ? cell_div / cell_vol : 0.0f;
+    return (dc - ((c00 + c11) + c22)) / 3.0f;
 }
 
 }  // namespace mpmx

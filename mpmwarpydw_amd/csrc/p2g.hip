@@ -53,13 +53,14 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
                    const float* x, const float* v, const float* C, const float* vol, const float* mass, float dx,
                    float inv_dx, float minx, float miny, float minz, float rpic, int gx, int gy, int gz,
                    float* grid_m, float* grid_v_in, float* grid_v_out, int do_update, float gxv, float gyv,
-                   float gzv, float dt, float damping, int bc, float fric, float gmag, int pad, void* ws,
-                   size_t ws_bytes, int* host_oob, hipStream_t s) {
+                   float gzv, float dt, float damping, int bc, float fric, float gmag, int pad, float restitution,
+                   void* ws, size_t ws_bytes, int* host_oob, hipStream_t s) {
     MPMX_REQUIRE(n >= 0 && gx >= 3 && gy >= 3 && gz >= 3, "p2g: grid must have at least 3 nodes per axis");
     MPMX_REQUIRE(grid_m && grid_v_in && (!do_update || grid_v_out), "p2g: null grid pointer");
     MPMX_REQUIRE(n == 0 || (activeLabel && materialLabel && stress && x && v && C && vol && mass),
                  "p2g: null particle pointer");
-    MPMX_REQUIRE(bc == MPMX_BC_NONE || bc == MPMX_BC_FRICTION, "p2g: unsupported boundary condition %d", bc);
+    MPMX_REQUIRE(bc >= MPMX_BC_NONE && bc <= MPMX_BC_PLANE_STRAIN_Y_GLOBAL, "p2g: unsupported boundary condition %d",
+                 bc);
     Geo tg = make_geom(gx, gy, gz, minx, miny, minz);
     MPMX_REQUIRE(tg.ntiles() * kTileCells < (1L << 31), "p2g: grid too large for int32 keys");
     size_t need = p2g_ws_bytes(n, tg);
@@ -114,6 +115,7 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     ca.gmag = gmag;
     ca.bc = bc;
     ca.pad = pad;
+    ca.restitution = restitution;
     MPMX_LAUNCH(p2g_combine_kernel<>, dim3(grid_for(G, kBlock)), dim3(kBlock), 0, s, ca);
     if ((rc = check_launch("p2g_combine", s))) return rc;
     if (host_oob) {
@@ -141,7 +143,7 @@ int mpmx_p2g_apic_with_stress(int n, const int32_t* activeLabel, const int32_t* 
                               size_t ws_bytes, int* host_out_of_grid, void* stream) {
     return p2g_run(n, activeLabel, materialLabel, stress, x, v, C, vol, mass, dx, inv_dx, minx, miny, minz,
                    rpic_damping, gx, gy, gz, grid_m, grid_v_in, nullptr, 0, 0.f, 0.f, 0.f, dt, 1.0f, MPMX_BC_NONE,
-                   0.f, 0.f, 0, workspace, ws_bytes, host_out_of_grid, as_stream(stream));
+                   0.f, 0.f, 0, 0.f, workspace, ws_bytes, host_out_of_grid, as_stream(stream));
 }
 
 int mpmx_p2g_grid_update(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* stress,
@@ -149,10 +151,11 @@ int mpmx_p2g_grid_update(int n, const int32_t* activeLabel, const int32_t* mater
                          float dx, float inv_dx, float minx, float miny, float minz, float rpic_damping, int gx,
                          int gy, int gz, float* grid_m, float* grid_v_in, float* grid_v_out, float gxv, float gyv,
                          float gzv, float dt, float damping_scale, int bc, float friction, float gravity_magnitude,
-                         int boundary_padding, void* workspace, size_t ws_bytes, void* stream) {
+                         int boundary_padding, float restitution, void* workspace, size_t ws_bytes, void* stream) {
     return p2g_run(n, activeLabel, materialLabel, stress, x, v, C, vol, mass, dx, inv_dx, minx, miny, minz,
                    rpic_damping, gx, gy, gz, grid_m, grid_v_in, grid_v_out, 1, gxv, gyv, gzv, dt, damping_scale, bc,
-                   friction, gravity_magnitude, boundary_padding, workspace, ws_bytes, nullptr, as_stream(stream));
+                   friction, gravity_magnitude, boundary_padding, restitution, workspace, ws_bytes, nullptr,
+                   as_stream(stream));
 }
 
 }  // extern "C"

@@ -368,10 +368,12 @@ struct CombineArgs {
     const int32_t* tile_flag;
     float *grid_m, *grid_v_in, *grid_v_out;
     int do_update;
-    float gxv, gyv, gzv, dt, damping, fric, gmag;
+    float gxv, gyv, gzv, dt, damping, fric, gmag, restitution;
     int bc, pad;
     int32_t* zero_ints;  // optional: the next substep's bin counts are cleared here
     long n_zero;
+    unsigned long long* zero_u64;  // optional: cell sums cleared before G2P
+    long n_zero_u64;
 };
 
 // a template so that every translation unit including this header can instantiate it
@@ -381,6 +383,9 @@ __global__ void p2g_combine_kernel(CombineArgs a) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (a.zero_ints) {
         for (long z = i; z < a.n_zero; z += (long)gridDim.x * blockDim.x) a.zero_ints[z] = 0;
+    }
+    if (a.zero_u64) {
+        for (long z = i; z < a.n_zero_u64; z += (long)gridDim.x * blockDim.x) a.zero_u64[z] = 0ull;
     }
     if (i >= G) return;
     const SceneDesc d = a.geo.at(a.geo.scene_of_node(i));
@@ -416,7 +421,7 @@ __global__ void p2g_combine_kernel(CombineArgs a) {
     a.grid_v_in[2 * G + i] = mz;
     if (a.do_update) {
         f3 vo = grid_update_node(m, f3{mx, my, mz}, ix, iy, iz, d.gx, d.gy, d.gz, a.gxv, a.gyv, a.gzv, a.dt,
-                                 a.damping, a.bc, a.fric, a.gmag, a.pad);
+                                 a.damping, a.bc, a.fric, a.gmag, a.pad, a.restitution);
         st3(a.grid_v_out, G, i, vo);
     }
 }

@@ -31,6 +31,7 @@ struct SubstepWork {
     float* partial;
     int32_t* tile_flag;
     SceneDesc* table;    // [nscenes + 1] for an ensemble
+    unsigned long long* vlc;  // [2 * nodes] cell sums of the volumetric locking correction
     BinWork bin;
 };
 
@@ -42,7 +43,7 @@ static size_t substep_ws_bytes(long n, const Geo& g) {
     long nn = n > 0 ? n : 1;
     return Carver::bytes_for(1, 4) + Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(g.ntiles() * kTileNodes * 4, 4) +
            Carver::bytes_for(g.ntiles(), 4) + Carver::bytes_for(g.nscenes + 1, sizeof(SceneDesc)) +
-           bin_workspace_bytes(nn, substep_nkeys(g));
+           Carver::bytes_for(2 * g.nnodes, sizeof(unsigned long long)) + bin_workspace_bytes(nn, substep_nkeys(g));
 }
 
 static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepWork& w) {
@@ -54,6 +55,7 @@ static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepW
     w.partial = cv.take<float>(g.ntiles() * kTileNodes * 4);
     w.tile_flag = cv.take<int32_t>(g.ntiles());
     w.table = cv.take<SceneDesc>(g.nscenes + 1);
+    w.vlc = cv.take<unsigned long long>(2 * g.nnodes);
     carve_bin(cv, nn, substep_nkeys(g), w.bin);
     return !cv.overflow;
 }
@@ -298,6 +300,37 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     }
 }
 
+// volumetric locking correction after G2P (simulationRoutines.py:252-293), per
+// scene: a scene's cells are its grid's nodes, numbered from its node0
+__global__ void vlc_accumulate_substep(mpmx_particles P, Geo geo, float inv_dx, unsigned long long* __restrict__ acc) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n || P.activeLabel[p] != 1 || P.materialLabel[p] > 1) return;
+    const SceneDesc d = geo.at(scene_of(P, p));
+    long k;
+    if (!vlc_cell(ld3(P.x, P.n, p), d.minx, d.miny, d.minz, inv_dx, d.gx, d.gy, d.gz, k)) return;
+    k += d.node0;
+    const long n = P.n;
+    const float div = (P.C[p] + P.C[4 * n + p]) + P.C[8 * n + p];
+    atomicAdd(&acc[k], vlc_fixed((double)div * (double)P.vol[p]));
+    atomicAdd(&acc[geo.nnodes + k], vlc_fixed((double)P.vol[p]));
+}
+
+__global__ void vlc_apply_substep(mpmx_particles P, Geo geo, float inv_dx, const unsigned long long* __restrict__ acc) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n || P.activeLabel[p] != 1 || P.materialLabel[p] > 1) return;
+    const SceneDesc d = geo.at(scene_of(P, p));
+    long k;
+    if (!vlc_cell(ld3(P.x, P.n, p), d.minx, d.miny, d.minz, inv_dx, d.gx, d.gy, d.gz, k)) return;
+    k += d.node0;
+    const long n = P.n;
+    float* C = P.C;
+    const float c00 = C[p], c11 = C[4 * n + p], c22 = C[8 * n + p];
+    const float corr = vlc_shift(c00, c11, c22, vlc_unfixed(acc[k]), vlc_unfixed(acc[geo.nnodes + k]));
+    C[p] = c00 + corr;
+    C[4 * n + p] = c11 + corr;
+    C[8 * n + p] = c22 + corr;
+}
+
 static int check_particles(const mpmx_particles* P, const mpmx_step_params* prm) {
     MPMX_REQUIRE(P && prm, "substep: null argument");
     MPMX_REQUIRE(P->n >= 0, "substep: bad sizes");
@@ -307,7 +340,8 @@ static int check_particles(const mpmx_particles* P, const mpmx_step_params* prm)
                                P->strainCriteria && P->eta_shear && P->eta_bulk && P->acc_strain && P->damage &&
                                P->vol && P->mass && (!prm->update_cov || P->cov)),
                  "substep: null particle array");
-    MPMX_REQUIRE(prm->bc == MPMX_BC_NONE || prm->bc == MPMX_BC_FRICTION, "substep: unsupported boundary condition");
+    MPMX_REQUIRE(prm->bc >= MPMX_BC_NONE && prm->bc <= MPMX_BC_PLANE_STRAIN_Y_GLOBAL,
+                 "substep: unsupported boundary condition %d", prm->bc);
     return 0;
 }
 
@@ -426,8 +460,13 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ca.gmag = prm->gravity_magnitude;
     ca.bc = prm->bc;
     ca.pad = prm->boundary_padding;
+    ca.restitution = prm->restitution;
     ca.zero_ints = w.bin.count;
     ca.n_zero = nkeys;
+    if (prm->vlc) {
+        ca.zero_u64 = w.vlc;
+        ca.n_zero_u64 = 2 * geo.nnodes;
+    }
     const long Gn = geo.nnodes;
     MPMX_LAUNCH(p2g_combine_kernel<>, dim3(grid_for(Gn > nkeys ? Gn : nkeys, kBlock)), dim3(kBlock), 0, s, ca);
     if ((rc = check_launch("substep_combine(grid)", s))) return rc;
@@ -448,6 +487,14 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ga.xcd = tuning(MPMX_TUNE_XCD_TILES);
     MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
     if ((rc = check_launch("substep_g2p", s))) return rc;
+
+    if (prm->vlc && P->n > 0) {
+        MPMX_LAUNCH(vlc_accumulate_substep, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, geo, prm->inv_dx,
+                    w.vlc);
+        if ((rc = check_launch("substep_vlc_accumulate", s))) return rc;
+        MPMX_LAUNCH(vlc_apply_substep, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, geo, prm->inv_dx, w.vlc);
+        if ((rc = check_launch("substep_vlc_apply", s))) return rc;
+    }
 
     return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s, P->order);
 }

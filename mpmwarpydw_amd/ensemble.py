@@ -28,7 +28,7 @@ from .simulationRoutines import step_structs
 
 SHARED = ("dt", "dx", "invdx", "gravity")
 SHARED_MPM = ("rpic_damping", "eff", "grid_v_damping_scale", "boundFriction", "boundaryPadding", "boundaryCondition",
-              "update_cov")
+              "boundRestitution", "update_cov", "volumetric_locking_correction")
 
 
 class Ensemble:

@@ -36,7 +36,10 @@ FLAGS = [
     ("sleepThreshold", float, 0.5), ("swellingRatio", float, 0.2), ("swellingActivationFactor", float, 1.0),
     ("swellingMaxFactor", float, 2.0), ("particle_v_max", float, 1000.0),
 ]
-BOUNDARY_CONDITIONS = ["friction", "friction_gradual", "restitution", "absorbing"]
+# the reference's --boundaryCondition choices (getArgs.py:96); its JSON configs
+# also select the two plane-strain modes (simulationRoutines.py:190-228)
+BOUNDARY_CONDITIONS = ["friction", "friction_gradual", "restitution", "absorbing", "plane_strain_y",
+                       "plane_strain_y_global"]
 
 
 def make_parser() -> argparse.ArgumentParser:

@@ -106,6 +106,86 @@ def _collideBounds(dim, grid_v_out, grid_dim_x, grid_dim_y, grid_dim_z, friction
 
 collideBounds = Kernel("collideBounds", _collideBounds, "mpmRoutines.py:942-993 (apply_coulomb_friction :915)")
 
+# boundary-condition selectors of the C ABI (include/mpmx.h MPMX_BC_*)
+BC_CODES = {"friction": 1, "friction_gradual": 2, "restitution": 3, "absorbing": 4, "plane_strain_y": 5,
+            "plane_strain_y_global": 6}
+
+
+def _variant(bc, grid_v_out, gx, gy, gz, mu=0.0, dt=0.0, g=0.0, pad=0, restitution=0.0):
+    lib("mpmx_collide_bounds_variant", grid_v_out.ptr, int(gx), int(gy), int(gz), BC_CODES[bc], float(mu), float(dt),
+        float(g), int(pad), float(restitution), stream_handle())
+
+
+def _collideBoundsGradualFriction(dim, grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude,
+                                  boundary_padding):
+    _variant("friction_gradual", grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude,
+             boundary_padding)
+
+
+collideBoundsGradualFriction = Kernel("collideBoundsGradualFriction", _collideBoundsGradualFriction,
+                                      "mpmRoutines.py:1264-1340")
+
+
+def _collideBoundsRestitution(dim, grid_v_out, gx, gy, gz, restitution, friction_coefficient, dt, gravity_magnitude,
+                              boundary_padding):
+    _variant("restitution", grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude, boundary_padding,
+             restitution)
+
+
+collideBoundsRestitution = Kernel("collideBoundsRestitution", _collideBoundsRestitution, "mpmRoutines.py:1182-1261")
+
+
+def _collideBoundsAbsorbing(dim, grid_v_out, gx, gy, gz, boundary_padding):
+    _variant("absorbing", grid_v_out, gx, gy, gz, pad=boundary_padding)
+
+
+collideBoundsAbsorbing = Kernel("collideBoundsAbsorbing", _collideBoundsAbsorbing, "mpmRoutines.py:1129-1179")
+
+
+def _collideBoundsPlaneStrainY(dim, grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude,
+                               boundary_padding):
+    _variant("plane_strain_y", grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude, boundary_padding)
+
+
+collideBoundsPlaneStrainY = Kernel("collideBoundsPlaneStrainY", _collideBoundsPlaneStrainY, "mpmRoutines.py:997-1064")
+
+
+def _collideBoundsPlaneStrainYGlobal(dim, grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude,
+                                     boundary_padding):
+    _variant("plane_strain_y_global", grid_v_out, gx, gy, gz, friction_coefficient, dt, gravity_magnitude,
+             boundary_padding)
+
+
+collideBoundsPlaneStrainYGlobal = Kernel("collideBoundsPlaneStrainYGlobal", _collideBoundsPlaneStrainYGlobal,
+                                         "mpmRoutines.py:1067-1126")
+
+_vlc_ws = Workspace()
+
+
+def _accumulate_cell_divergence(dim, activeLabel, materialLabel, particle_x, particle_C, particle_vol, inv_dx,
+                                minBounds, cell_div_v_weighted, cell_vol_sum):
+    gx, gy, gz = cell_vol_sum.shape
+    ws, nb = _vlc_ws.get(lib.raw("mpmx_vlc_workspace_bytes")(gx, gy, gz))
+    lib("mpmx_accumulate_cell_divergence", int(dim), activeLabel.ptr, materialLabel.ptr, particle_x.ptr,
+        particle_C.ptr, particle_vol.ptr, float(inv_dx), *_v3(minBounds), gx, gy, gz, cell_div_v_weighted.ptr,
+        cell_vol_sum.ptr, ws, nb, stream_handle())
+
+
+accumulate_cell_divergence = Kernel("accumulate_cell_divergence", _accumulate_cell_divergence,
+                                    "mpmRoutines.py:818-857 (cell sums in 64-bit fixed point, order-independent)")
+
+
+def _apply_volumetric_locking_correction(dim, activeLabel, materialLabel, particle_x, particle_C, inv_dx, minBounds,
+                                         cell_div_v_weighted, cell_vol_sum):
+    gx, gy, gz = cell_vol_sum.shape
+    lib("mpmx_apply_volumetric_locking_correction", int(dim), activeLabel.ptr, materialLabel.ptr, particle_x.ptr,
+        particle_C.ptr, float(inv_dx), *_v3(minBounds), gx, gy, gz, cell_div_v_weighted.ptr, cell_vol_sum.ptr,
+        stream_handle())
+
+
+apply_volumetric_locking_correction = Kernel("apply_volumetric_locking_correction",
+                                             _apply_volumetric_locking_correction, "mpmRoutines.py:860-911")
+
 
 def _g2p(dim, dt, activeLabel, materialLabel, particle_x, particle_v, particle_C, particle_F, particle_F_trial,
          particle_cov, inv_dx, grid_v_out, update_cov_with_F, minBounds):

@@ -122,9 +122,12 @@ class MPMState:
         self.grid_v_out = rt.zeros(self.gridDims, rt.vec3)
         self.gridBCIndex = rt.zeros(self.gridDims, rt.int32)
 
+        # volumetric locking correction (simStates.py:123-126): cell sums of the
+        # reference's launch-by-launch step; the fused substep keeps its own in
+        # its workspace
         self.volumetric_locking_correction = args.volumetric_locking_correction
-        self.cell_div_v_weighted = None  # allocated on first use (VLC is outside the measured path)
-        self.cell_vol_sum = None
+        self.cell_div_v_weighted = rt.zeros(self.gridDims, rt.float32)
+        self.cell_vol_sum = rt.zeros(self.gridDims, rt.float32)
 
         self.particle_x_initial = rt.zeros(n, rt.vec3)
         self.eff = args.eff

@@ -19,20 +19,53 @@ from . import runtime as rt
 from ._lib import MpmxError, lib
 from .runtime import Kernel, Workspace, launch, stream_handle
 
-_BC = {"friction": 1}
-
-
 def _bc_code(mpm):
-    if mpm.boundaryCondition not in _BC:
-        raise MpmxError(f"boundaryCondition {mpm.boundaryCondition!r} is not implemented on the device path "
-                        "(implemented: 'friction', mpmRoutines.py:942)")
-    return _BC[mpm.boundaryCondition]
+    """The MPMX_BC_* code of mpm.boundaryCondition (simulationRoutines.py:124-228).
+    An unknown name raises: the reference silently applies no boundary then."""
+    if mpm.boundaryCondition not in mpmRoutines.BC_CODES:
+        raise MpmxError(f"boundaryCondition {mpm.boundaryCondition!r} is not one of "
+                        f"{sorted(mpmRoutines.BC_CODES)} (simulationRoutines.py:124-228)")
+    return mpmRoutines.BC_CODES[mpm.boundaryCondition]
 
 
-def _check_vlc(mpm):
-    if mpm.volumetric_locking_correction:
-        raise MpmxError("volumetric_locking_correction=1 (simulationRoutines.py:255-293) is not implemented on the "
-                        "device path; set it to 0 in the JSON config")
+def _collide(sim, mpm):
+    """Step 6 of mpmSimulationStep (simulationRoutines.py:122-228): the boundary
+    kernel that mpm.boundaryCondition names."""
+    bc = mpm.boundaryCondition
+    _bc_code(mpm)
+    dims = list(mpm.gridDims)
+    g = abs(sim.gravity[2])
+    if bc == "friction":
+        launch(kernel=mpmRoutines.collideBounds, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, *dims, mpm.boundFriction, sim.dt, g, mpm.boundaryPadding])
+    elif bc == "friction_gradual":
+        launch(kernel=mpmRoutines.collideBoundsGradualFriction, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, *dims, mpm.boundFriction, sim.dt, g, mpm.boundaryPadding])
+    elif bc == "restitution":
+        launch(kernel=mpmRoutines.collideBoundsRestitution, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, *dims, mpm.boundRestitution, mpm.boundFriction, sim.dt, g,
+                       mpm.boundaryPadding])
+    elif bc == "absorbing":
+        launch(kernel=mpmRoutines.collideBoundsAbsorbing, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, *dims, mpm.boundaryPadding])
+    elif bc == "plane_strain_y":
+        launch(kernel=mpmRoutines.collideBoundsPlaneStrainY, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, *dims, mpm.boundFriction, sim.dt, g, mpm.boundaryPadding])
+    else:
+        launch(kernel=mpmRoutines.collideBoundsPlaneStrainYGlobal, dim=mpm.gridDims,
+               inputs=[mpm.grid_v_out, *dims, mpm.boundFriction, sim.dt, g, mpm.boundaryPadding])
+
+
+def volumetric_locking_phase(sim, mpm, xpbd=None):
+    """Step 8 of mpmSimulationStep (simulationRoutines.py:252-293): the cell sums
+    (both arrays are overwritten, which is the reference's zero_() + atomics)
+    and the correction of C."""
+    launch(kernel=mpmRoutines.accumulate_cell_divergence, dim=sim.nPoints,
+           inputs=[sim.activeLabel, sim.materialLabel, sim.particle_x, mpm.particle_C, sim.particle_vol, sim.invdx,
+                   sim.minBounds, mpm.cell_div_v_weighted, mpm.cell_vol_sum])
+    launch(kernel=mpmRoutines.apply_volumetric_locking_correction, dim=sim.nPoints,
+           inputs=[sim.activeLabel, sim.materialLabel, sim.particle_x, mpm.particle_C, sim.invdx, sim.minBounds,
+                   mpm.cell_div_v_weighted, mpm.cell_vol_sum])
 
 
 def _cov_flag(mpm):
@@ -41,7 +74,7 @@ def _cov_flag(mpm):
 
 def mpmSimulationStep(sim, mpm, xpbd, device=None):
     """One MPM substep in the reference's launch order (simulationRoutines.py:6-293)."""
-    _check_vlc(mpm)
+    _bc_code(mpm)
     mpm.grid_m.zero_()
     mpm.grid_v_in.zero_()
     mpm.grid_v_out.zero_()
@@ -60,14 +93,13 @@ def mpmSimulationStep(sim, mpm, xpbd, device=None):
     if mpm.grid_v_damping_scale < 1.0:
         launch(kernel=mpmRoutines.add_damping_via_grid, dim=mpm.gridDims,
                inputs=[mpm.grid_v_out, mpm.grid_v_damping_scale])
-    _bc_code(mpm)
-    launch(kernel=mpmRoutines.collideBounds, dim=mpm.gridDims,
-           inputs=[mpm.grid_v_out, mpm.gridDims[0], mpm.gridDims[1], mpm.gridDims[2], mpm.boundFriction, sim.dt,
-                   abs(sim.gravity[2]), mpm.boundaryPadding])
+    _collide(sim, mpm)
     launch(kernel=mpmRoutines.g2p, dim=sim.nPoints,
            inputs=[sim.dt, sim.activeLabel, sim.materialLabel, sim.particle_x, sim.particle_v, mpm.particle_C,
                    mpm.particle_F, mpm.particle_F_trial, mpm.particle_cov, sim.invdx, mpm.grid_v_out,
                    _cov_flag(mpm), sim.minBounds])
+    if mpm.volumetric_locking_correction:
+        volumetric_locking_phase(sim, mpm)
 
 
 def stress_phase(sim, mpm, xpbd):
@@ -92,7 +124,7 @@ def p2g_grid_phase(sim, mpm, xpbd):
         float(sim.dx), float(sim.invdx), *[float(v) for v in sim.minBounds], float(mpm.rpic_damping), gx, gy, gz,
         mpm.grid_m.ptr, mpm.grid_v_in.ptr, mpm.grid_v_out.ptr, float(g[0]), float(g[1]), float(g[2]), float(sim.dt),
         float(mpm.grid_v_damping_scale), bc, float(mpm.boundFriction), float(abs(g[2])), int(mpm.boundaryPadding),
-        ws, nb, stream_handle())
+        float(mpm.boundRestitution), ws, nb, stream_handle())
 
 
 def g2p_phase(sim, mpm, xpbd):
@@ -104,11 +136,13 @@ def g2p_phase(sim, mpm, xpbd):
 
 
 def mpmSimulationStepPhased(sim, mpm, xpbd, device=None):
-    """The substep as three calls: stress, P2G with the grid update, G2P."""
-    _check_vlc(mpm)
+    """The substep as three calls: stress, P2G with the grid update, G2P (and the
+    volumetric locking correction when it is on)."""
     stress_phase(sim, mpm, xpbd)
     p2g_grid_phase(sim, mpm, xpbd)
     g2p_phase(sim, mpm, xpbd)
+    if mpm.volumetric_locking_correction:
+        volumetric_locking_phase(sim, mpm)
 
 
 def _aptr(a):
@@ -137,7 +171,8 @@ def step_structs(sim, mpm, xpbd):
                      gravity_y=g[1], gravity_z=g[2], rpic_damping=mpm.rpic_damping, eff=mpm.eff,
                      damping_scale=mpm.grid_v_damping_scale, friction=mpm.boundFriction,
                      gravity_magnitude=abs(g[2]), bc=_bc_code(mpm), boundary_padding=int(mpm.boundaryPadding),
-                     update_cov=int(_cov_flag(mpm)), constitutive_model=int(mpm.constitutive_model))
+                     update_cov=int(_cov_flag(mpm)), constitutive_model=int(mpm.constitutive_model),
+                     restitution=float(mpm.boundRestitution), vlc=int(bool(mpm.volumetric_locking_correction)))
     return P, G, prm
 
 
@@ -162,9 +197,9 @@ def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
     """mpmSimulationStep (simulationRoutines.py:6-293) as libmpmx's fused substep:
     a tile pass (constitutive update + deterministic P2G), a node pass (grid
     update) and a tile pass (G2P, which also bins the particles for the next
-    substep).  The bins are rebuilt first whenever x, the labels or the
-    activity were written from the host since the last substep."""
-    _check_vlc(mpm)
+    substep), then the volumetric locking correction when it is on.  The bins
+    are rebuilt first whenever x, the labels or the activity were written from
+    the host since the last substep."""
     P, G, prm = step_structs(sim, mpm, xpbd)
     ws, nb = substep_workspace(sim, mpm)
     key = _bin_key(sim, mpm)

@@ -131,6 +131,38 @@ class Oracle:
         self.lib.o_collide_bounds(_ptr(grid_v_out), _I(gx), _I(gy), _I(gz), self.r(friction), self.r(dt),
                                   self.r(gmag), _I(padding))
 
+    def collide_bounds_variant(self, kind, grid_v_out, gx, gy, gz, friction, dt, gmag, padding, restitution=0.0):
+        """The grid boundary conditions other than "friction" (simulationRoutines.py:141-228)."""
+        self._chk(grid_v_out)
+        v = _ptr(grid_v_out)
+        if kind == "friction_gradual":
+            self.lib.o_collide_bounds_gradual(v, _I(gx), _I(gy), _I(gz), self.r(friction), self.r(dt), self.r(gmag),
+                                              _I(padding))
+        elif kind == "restitution":
+            self.lib.o_collide_bounds_restitution(v, _I(gx), _I(gy), _I(gz), self.r(restitution), self.r(friction),
+                                                  self.r(dt), self.r(gmag), _I(padding))
+        elif kind == "absorbing":
+            self.lib.o_collide_bounds_absorbing(v, _I(gx), _I(gy), _I(gz), _I(padding))
+        elif kind == "plane_strain_y":
+            self.lib.o_collide_bounds_plane_strain_y(v, _I(gx), _I(gy), _I(gz), self.r(friction), self.r(dt),
+                                                     self.r(gmag), _I(padding))
+        elif kind == "plane_strain_y_global":
+            self.lib.o_collide_bounds_plane_strain_y_global(v, _I(gx), _I(gy), _I(gz), self.r(friction),
+                                                            self.r(dt), self.r(gmag), _I(padding))
+        else:
+            raise ValueError(kind)
+
+    def volumetric_locking_correction(self, activeLabel, materialLabel, x, C_, vol, inv_dx, minBounds, gx, gy, gz):
+        """Both VLC phases (mpmRoutines.py:805-911); returns (cell_div, cell_vol) and updates C in place."""
+        self._chk(x, C_, vol)
+        G = gx * gy * gz
+        cd = np.zeros(G, self.dtype)
+        cv = np.zeros(G, self.dtype)
+        mb = self.f(minBounds)
+        self.lib.o_vlc(_I(len(vol)), _ptr(self.i(activeLabel)), _ptr(self.i(materialLabel)), _ptr(x), _ptr(C_),
+                       _ptr(vol), self.r(inv_dx), _ptr(mb), _I(gx), _I(gy), _I(gz), _ptr(cd), _ptr(cv))
+        return cd.reshape(gx, gy, gz), cv.reshape(gx, gy, gz)
+
     # ---- A10 -------------------------------------------------------------
     def g2p(self, dt, activeLabel, materialLabel, x, v, C_, F, F_trial, cov, inv_dx, grid_v_out,
             update_cov, minBounds):

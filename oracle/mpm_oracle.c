@@ -438,6 +438,139 @@ EXPORT void o_collide_bounds(R *v_out, int gx, int gy, int gz, R mu, R dt, R gma
             }
 }
 
+/* the other boundary conditions of the grid update (simulationRoutines.py:
+ * 141-228), each kernel restated per node */
+
+/* collideBoundsPlaneStrainY (mpmRoutines.py:996-1062): x and z faces as
+ * collideBounds; on the y faces v_y is zeroed whatever its sign, from a fresh
+ * read of the node (which also becomes the v the z faces test) */
+EXPORT void o_collide_bounds_plane_strain_y(R *v_out, int gx, int gy, int gz, R mu, R dt, R g, int pad) {
+    for (int ix = 0; ix < gx; ix++)
+        for (int iy = 0; iy < gy; iy++)
+            for (int iz = 0; iz < gz; iz++) {
+                R *o = v_out + 3 * (((long)ix * gy + iy) * gz + iz);
+                R v[3] = {o[0], o[1], o[2]};
+                if (ix <= pad && v[0] < 0) { o[0] = 0; o[1] = coulomb(v[1], mu, dt, g); o[2] = coulomb(v[2], mu, dt, g); }
+                if (ix >= gx - pad && v[0] > 0) { o[0] = 0; o[1] = coulomb(v[1], mu, dt, g); o[2] = coulomb(v[2], mu, dt, g); }
+                if (iy <= pad) { v[0] = o[0]; v[1] = o[1]; v[2] = o[2]; o[1] = 0; }
+                if (iy >= gy - pad) { v[0] = o[0]; v[1] = o[1]; v[2] = o[2]; o[1] = 0; }
+                if (iz <= pad && v[2] < 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = coulomb(v[1], mu, dt, g); o[2] = 0; }
+                if (iz >= gz - pad && v[2] > 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = coulomb(v[1], mu, dt, g); o[2] = 0; }
+            }
+}
+
+/* collideBoundsPlaneStrainYGlobal (mpmRoutines.py:1065-1123): v_y = 0 on every
+ * node first, then x and z faces on that velocity */
+EXPORT void o_collide_bounds_plane_strain_y_global(R *v_out, int gx, int gy, int gz, R mu, R dt, R g, int pad) {
+    for (int ix = 0; ix < gx; ix++)
+        for (int iy = 0; iy < gy; iy++)
+            for (int iz = 0; iz < gz; iz++) {
+                R *o = v_out + 3 * (((long)ix * gy + iy) * gz + iz);
+                o[1] = 0;
+                R v[3] = {o[0], o[1], o[2]};
+                if (ix <= pad && v[0] < 0) { o[0] = 0; o[1] = 0; o[2] = coulomb(v[2], mu, dt, g); }
+                if (ix >= gx - pad && v[0] > 0) { o[0] = 0; o[1] = 0; o[2] = coulomb(v[2], mu, dt, g); }
+                if (iz <= pad && v[2] < 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = 0; o[2] = 0; }
+                if (iz >= gz - pad && v[2] > 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = 0; o[2] = 0; }
+            }
+}
+
+/* collideBoundsAbsorbing (mpmRoutines.py:1126-1176): the three outer node
+ * layers of each face (a literal 3, whatever the padding) scale v by
+ * 1 - (pad - distance) / pad; every face scales the node's original v and the
+ * last matching face wins */
+EXPORT void o_collide_bounds_absorbing(R *v_out, int gx, int gy, int gz, int pad) {
+    const R P = (R)pad;
+    for (int ix = 0; ix < gx; ix++)
+        for (int iy = 0; iy < gy; iy++)
+            for (int iz = 0; iz < gz; iz++) {
+                R *o = v_out + 3 * (((long)ix * gy + iy) * gz + iz);
+                R v[3] = {o[0], o[1], o[2]};
+                int dist[6] = {ix, gx - 1 - ix, iy, gy - 1 - iy, iz, gz - 1 - iz};
+                int hit[6] = {ix < 3, ix >= gx - 3, iy < 3, iy >= gy - 3, iz < 3, iz >= gz - 3};
+                for (int f = 0; f < 6; f++) {
+                    if (!hit[f]) continue;
+                    R lf = (P - (R)dist[f]) / P;
+                    R d = (R)0 * lf + (R)1 * ((R)1 - lf);
+                    o[0] = v[0] * d; o[1] = v[1] * d; o[2] = v[2] * d;
+                }
+            }
+}
+
+/* collideBoundsRestitution (mpmRoutines.py:1179-1247): the normal component of
+ * an approaching node is reversed and scaled by e, the tangential ones get
+ * Coulomb friction; last matching face wins, all from the original v */
+EXPORT void o_collide_bounds_restitution(R *v_out, int gx, int gy, int gz, R e, R mu, R dt, R g, int pad) {
+    for (int ix = 0; ix < gx; ix++)
+        for (int iy = 0; iy < gy; iy++)
+            for (int iz = 0; iz < gz; iz++) {
+                R *o = v_out + 3 * (((long)ix * gy + iy) * gz + iz);
+                R v[3] = {o[0], o[1], o[2]};
+                if (ix <= pad && v[0] < 0) { o[0] = -e * v[0]; o[1] = coulomb(v[1], mu, dt, g); o[2] = coulomb(v[2], mu, dt, g); }
+                if (ix >= gx - pad && v[0] > 0) { o[0] = -e * v[0]; o[1] = coulomb(v[1], mu, dt, g); o[2] = coulomb(v[2], mu, dt, g); }
+                if (iy <= pad && v[1] < 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = -e * v[1]; o[2] = coulomb(v[2], mu, dt, g); }
+                if (iy >= gy - pad && v[1] > 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = -e * v[1]; o[2] = coulomb(v[2], mu, dt, g); }
+                if (iz <= pad && v[2] < 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = coulomb(v[1], mu, dt, g); o[2] = -e * v[2]; }
+                if (iz >= gz - pad && v[2] > 0) { o[0] = coulomb(v[0], mu, dt, g); o[1] = coulomb(v[1], mu, dt, g); o[2] = -e * v[2]; }
+            }
+}
+
+/* collideBoundsGradualFriction (mpmRoutines.py:1249-1340): inside the padding
+ * (index < pad, or > dim - 1 - pad) an approaching node keeps fade = distance
+ * / pad of its normal velocity and gets friction mu (1 - fade) on the tangential
+ * ones; per axis the lower face is tested first and the upper one only when the
+ * lower does not apply (elif), the three axes in turn from the original v */
+EXPORT void o_collide_bounds_gradual(R *v_out, int gx, int gy, int gz, R mu, R dt, R g, int pad) {
+    const R P = (R)pad;
+    for (int ix = 0; ix < gx; ix++)
+        for (int iy = 0; iy < gy; iy++)
+            for (int iz = 0; iz < gz; iz++) {
+                R *o = v_out + 3 * (((long)ix * gy + iy) * gz + iz);
+                R v[3] = {o[0], o[1], o[2]};
+                int idx[3] = {ix, iy, iz}, dim[3] = {gx, gy, gz};
+                for (int a = 0; a < 3; a++) {
+                    R fade;
+                    if (idx[a] < pad && v[a] < 0) fade = (R)idx[a] / P;
+                    else if (idx[a] >= dim[a] - pad && v[a] > 0) fade = (R)(dim[a] - 1 - idx[a]) / P;
+                    else continue;
+                    R m = mu * ((R)1 - fade);
+                    for (int b = 0; b < 3; b++)
+                        o[b] = (b == a) ? (R)0 * ((R)1 - fade) + v[a] * fade : coulomb(v[b], m, dt, g);
+                }
+            }
+}
+
+/* Volumetric locking correction (mpmRoutines.py:805-911, run after G2P by
+ * simulationRoutines.py:255-293): per cell int((x - minB) inv_dx) the sums of
+ * tr(C) V and V over active MPM particles, then C += (div_cell - tr C) / 3 I
+ * with div_cell = sum(tr C V) / sum(V) (0 when sum(V) <= 1e-20).  Particles
+ * whose cell lies outside the grid are skipped (the reference would index out
+ * of bounds).  cell_div and cell_vol [gx*gy*gz] are overwritten. */
+EXPORT void o_vlc(int n, const int *active, const int *label, const R *x, R *C, const R *vol, R inv_dx,
+                  const R *minB, int gx, int gy, int gz, R *cell_div, R *cell_vol) {
+    long G = (long)gx * gy * gz;
+    memset(cell_div, 0, sizeof(R) * G);
+    memset(cell_vol, 0, sizeof(R) * G);
+    for (int pass = 0; pass < 2; pass++)
+        for (int p = 0; p < n; p++) {
+            if (!(active[p] == 1 && label[p] <= 1)) continue;
+            int c[3];
+            for (int a = 0; a < 3; a++) c[a] = (int)((x[3 * p + a] - minB[a]) * inv_dx);
+            if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= gx || c[1] >= gy || c[2] >= gz) continue;
+            long k = ((long)c[0] * gy + c[1]) * gz + c[2];
+            R *Cp = C + 9 * p;
+            R div = Cp[0] + Cp[4] + Cp[8];
+            if (pass == 0) {
+                cell_div[k] += div * vol[p];
+                cell_vol[k] += vol[p];
+            } else {
+                R dc = cell_vol[k] > (R)1e-20 ? cell_div[k] / cell_vol[k] : (R)0;
+                R corr = (dc - div) / (R)3;
+                Cp[0] += corr; Cp[4] += corr; Cp[8] += corr;
+            }
+        }
+}
+
 /* ------------------------------------------------------------------ */
 /* A10 g2p + update_cov (mpmRoutines.py:715-802)                       */
 /* ------------------------------------------------------------------ */

@@ -106,8 +106,22 @@ def oracle_grid(o, H, S):
                                      np.asarray(sim.gravity, np.float32), np.float32(sim.dt))
     if mpm.grid_v_damping_scale < 1.0:
         o.add_damping_via_grid(H["grid_v_out"], mpm.grid_v_damping_scale)
-    o.collideBounds(H["grid_v_out"], *mpm.gridDims, np.float32(mpm.boundFriction), np.float32(sim.dt),
-                    np.float32(abs(sim.gravity[2])), mpm.boundaryPadding)
+    args = (*mpm.gridDims, np.float32(mpm.boundFriction), np.float32(sim.dt), np.float32(abs(sim.gravity[2])),
+            mpm.boundaryPadding)
+    if mpm.boundaryCondition == "friction":
+        o.collideBounds(H["grid_v_out"], *args)
+    else:
+        o.collide_bounds_variant(mpm.boundaryCondition, H["grid_v_out"], *args,
+                                 restitution=np.float32(mpm.boundRestitution))
+
+
+def oracle_vlc(o, H, S):
+    """Step 8 of the substep (simulationRoutines.py:252-293) when it is on."""
+    sim, mpm = S.sim, S.mpm
+    if mpm.volumetric_locking_correction:
+        o.volumetric_locking_correction(H["activeLabel"], H["materialLabel"], H["particle_x"], H["particle_C"],
+                                        H["particle_vol"], np.float32(sim.invdx), np.float32(sim.minBounds),
+                                        *mpm.gridDims)
 
 
 def oracle_g2p(o, H, S):

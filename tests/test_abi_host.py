@@ -61,7 +61,10 @@ def test_argument_errors_are_reported():
     with pytest.raises(MpmxError, match="n < 0"):
         lib("mpmx_compute_mu_lam_bulk", -1, None, None, None, None, None, None)
     with pytest.raises(MpmxError, match="unsupported boundary"):
-        lib("mpmx_grid_update", 4, 4, 4, 1, 1, 1, 0.0, 0.0, 0.0, 1.0, 1.0, 7, 0.0, 0.0, 2, None)
+        lib("mpmx_grid_update", 4, 4, 4, 1, 1, 1, 0.0, 0.0, 0.0, 1.0, 1.0, 7, 0.0, 0.0, 2, 0.0, None)
+    with pytest.raises(MpmxError, match="unsupported boundary"):
+        lib("mpmx_collide_bounds_variant", 1, 4, 4, 4, 0, 0.0, 0.0, 0.0, 2, 0.0, None)
+    assert lib.raw("mpmx_vlc_workspace_bytes")(4, 4, 4) >= 2 * 64 * 8
 
 
 def test_get_args_defaults_and_precedence(tmp_path):

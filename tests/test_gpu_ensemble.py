@@ -13,12 +13,13 @@ FIELDS = [("sim", "particle_x"), ("sim", "particle_v"), ("sim", "materialLabel")
           ("mpm", "particle_accumulated_strain"), ("mpm", "particle_damage"), ("mpm", "ys"), ("mpm", "mu")]
 
 
-def _scenes(perturb):
+def _scenes(perturb, **extra):
     from mpmwarpydw_amd import driver, getArgs
     from mpmwarpydw_amd.ensemble import block_scene
     out = []
     for k, spec in enumerate(SPECS):
         dom, ov = block_scene(*spec)
+        ov.update(extra)
         S = driver.setup(getArgs.get_args([], ov), dom)
         if perturb:
             perturb_state(S, 60 + k)
@@ -26,13 +27,18 @@ def _scenes(perturb):
     return out
 
 
-@pytest.mark.parametrize("reorder", [False, True])
-def test_ensemble_equals_scenes_alone(reorder):
+VARIANTS = {"friction": {}, "restitution+vlc": {"boundaryCondition": "restitution", "boundRestitution": 0.4,
+                                                "volumetric_locking_correction": 1}}
+
+
+@pytest.mark.parametrize("reorder,variant", [(False, "friction"), (True, "friction"), (True, "restitution+vlc")])
+def test_ensemble_equals_scenes_alone(reorder, variant):
     from mpmwarpydw_amd import simulationRoutines as SR
     from mpmwarpydw_amd.ensemble import Ensemble
-    alone = _scenes(True)
-    ens = Ensemble(_scenes(True))
-    print(f"[ensemble] {len(SPECS)} scenes, {ens.n} particles, {ens.node_offsets[-1]} nodes", flush=True)
+    alone = _scenes(True, **VARIANTS[variant])
+    ens = Ensemble(_scenes(True, **VARIANTS[variant]))
+    print(f"[ensemble] {variant} reorder={reorder}: {len(SPECS)} scenes, {ens.n} particles, "
+          f"{ens.node_offsets[-1]} nodes", flush=True)
     if reorder:
         ens.reorder()
     for _ in range(6):

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_err
-from state_utils import host_state, make_sim, oracle_g2p, oracle_grid, oracle_p2g, oracle_stress
+from state_utils import host_state, make_sim, oracle_g2p, oracle_grid, oracle_p2g, oracle_stress, oracle_vlc
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
@@ -144,19 +144,27 @@ def test_g2p_parity(o64):
         assert rel_err(G[k], H[k]) < TOL, k
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_substep_teacher_forced(o64, fused):
-    """One full MPM substep from an identical state: every updated array agrees."""
+SUBSTEP_CASES = [(False, "friction", 0), (True, "friction", 0), (False, "friction_gradual", 1),
+                 (True, "friction_gradual", 1), (True, "restitution", 0), (True, "absorbing", 1),
+                 (True, "plane_strain_y", 0), (False, "plane_strain_y_global", 1), (True, "plane_strain_y_global", 1)]
+
+
+@pytest.mark.parametrize("fused,bc,vlc", SUBSTEP_CASES)
+def test_substep_teacher_forced(o64, fused, bc, vlc):
+    """One full MPM substep from an identical state, under every boundary
+    condition and with the volumetric locking correction: every updated array
+    agrees with the oracle."""
     from mpmwarpydw_amd import simulationRoutines as SR
-    S = make_sim(seed=8)
+    S = make_sim(seed=8, boundaryCondition=bc, boundRestitution=0.6, volumetric_locking_correction=vlc)
     H = host_state(S)
-    print(f"[substep parity] fused={fused} n={S.sim.nPoints}", flush=True)
+    print(f"[substep parity] fused={fused} bc={bc} vlc={vlc} n={S.sim.nPoints}", flush=True)
     (SR.mpmSimulationStepFused if fused else SR.mpmSimulationStep)(S.sim, S.mpm, S.xpbd)
     G = host_state(S)
     oracle_stress(o64, H, S.mpm.eff)
     oracle_p2g(o64, H, S)
     oracle_grid(o64, H, S)
     oracle_g2p(o64, H, S)
+    oracle_vlc(o64, H, S)
     for k in ("particle_stress", "particle_F", "grid_m", "grid_v_in", "grid_v_out", "particle_v", "particle_x",
               "particle_C", "particle_F_trial", "particle_damage", "ys"):
         e = rel_err(G[k], H[k])
@@ -165,10 +173,12 @@ def test_substep_teacher_forced(o64, fused):
     assert np.array_equal(G["materialLabel"], H["materialLabel"])
 
 
-def test_fused_matches_reference_order():
+@pytest.mark.parametrize("bc,vlc", [("friction", 0), ("restitution", 1), ("plane_strain_y", 1)])
+def test_fused_matches_reference_order(bc, vlc):
     from mpmwarpydw_amd import simulationRoutines as SR
-    A = make_sim(seed=9)
-    B = make_sim(seed=9)
+    A = make_sim(seed=9, boundaryCondition=bc, boundRestitution=0.3, volumetric_locking_correction=vlc)
+    B = make_sim(seed=9, boundaryCondition=bc, boundRestitution=0.3, volumetric_locking_correction=vlc)
+    print(f"[fused vs launch order] bc={bc} vlc={vlc}", flush=True)
     SR.mpmSimulationStep(A.sim, A.mpm, A.xpbd)
     SR.mpmSimulationStepFused(B.sim, B.mpm, B.xpbd)
     a, b = host_state(A), host_state(B)

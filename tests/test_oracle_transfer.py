@@ -194,3 +194,82 @@ def test_collide_bounds_cases(o64):
                             r = np.array([0.0 if a == ax else coul(o[a], mu, dt, g) for a in range(3)])
                 ref[ix, iy, iz] = r
     assert np.allclose(out, ref, rtol=0, atol=0)
+
+
+# ---- the other grid boundary conditions and the volumetric locking correction ----
+
+def _grid(gx, gy, gz, v):
+    g = np.zeros((gx, gy, gz, 3))
+    g[...] = v
+    return g
+
+
+def test_bc_variants_hand_cases(o64):
+    mu, dt, gm, pad = 0.5, 1e-3, 9.81, 2
+    dv = mu * gm * dt
+    # restitution: x face, approaching, tangential friction
+    g = _grid(8, 8, 8, (-2.0, 0.5, 0.0))
+    o64.collide_bounds_variant("restitution", g, 8, 8, 8, mu, dt, gm, pad, restitution=0.5)
+    assert np.allclose(g[0, 4, 4], (1.0, 0.5 - dv, 0.0))
+    assert np.allclose(g[4, 4, 4], (-2.0, 0.5, 0.0))  # interior untouched
+    # gradual friction: fade = index / pad on the lower x face
+    g = _grid(8, 8, 8, (-1.0, 0.5, 0.0))
+    o64.collide_bounds_variant("friction_gradual", g, 8, 8, 8, mu, dt, gm, pad)
+    assert np.allclose(g[1, 4, 4], (-0.5, 0.5 - mu * 0.5 * gm * dt, 0.0))
+    assert np.allclose(g[0, 4, 4], (0.0, 0.5 - dv, 0.0))
+    assert np.allclose(g[2, 4, 4], (-1.0, 0.5, 0.0))  # index 2 is not < pad
+    # absorbing: a literal 3 outer layers; the last matching face scales the original v
+    g = _grid(8, 8, 8, (2.0, 2.0, 2.0))
+    o64.collide_bounds_variant("absorbing", g, 8, 8, 8, mu, dt, gm, pad)
+    assert np.allclose(g[0, 4, 4], 0.0) and np.allclose(g[1, 4, 4], 1.0) and np.allclose(g[2, 4, 4], 2.0)
+    assert np.allclose(g[0, 1, 4], 1.0)  # x says 0, y (tested later) says 0.5: y wins
+    assert np.allclose(g[7, 4, 4], 0.0) and np.allclose(g[5, 4, 4], 2.0)
+    # plane strain: v_y zeroed on the y faces whatever its sign; everywhere in the global form
+    g = _grid(8, 8, 8, (1.0, 2.0, 3.0))
+    o64.collide_bounds_variant("plane_strain_y", g, 8, 8, 8, mu, dt, gm, pad)
+    assert np.allclose(g[4, 0, 4], (1.0, 0.0, 3.0)) and np.allclose(g[4, 4, 4], (1.0, 2.0, 3.0))
+    g = _grid(8, 8, 8, (-1.0, 2.0, 3.0))
+    o64.collide_bounds_variant("plane_strain_y_global", g, 8, 8, 8, mu, dt, gm, pad)
+    assert np.allclose(g[4, 4, 4], (-1.0, 0.0, 3.0))
+    assert np.allclose(g[0, 4, 4], (0.0, 0.0, 3.0 - dv))
+
+
+def test_plane_strain_y_reread_feeds_the_z_faces(o64):
+    """On a y face the node is read again after the x faces: the z faces then
+    test the velocity the x faces produced (mpmRoutines.py:1035-1061)."""
+    mu, dt, gm, pad = 0.5, 1e-3, 9.81, 2
+    g = _grid(8, 8, 8, (-1.0, 2.0, -3.0))
+    o64.collide_bounds_variant("plane_strain_y", g, 8, 8, 8, mu, dt, gm, pad)
+    dv = mu * gm * dt
+    # corner x=0, y=0, z=0: x face -> (0, f(2), f(-3)); the y face re-reads that node into v and
+    # writes (0, 0, f(-3)); the z face then writes (f(0), f(v_y), 0) from the re-read v, so v_y
+    # comes back as f(f(2)) (a quirk of the reference kept for parity)
+    assert np.allclose(g[0, 0, 0], (0.0, 2.0 - 2 * dv, 0.0))
+    # x=0, z=0, interior y: no re-read, z face on the original v
+    assert np.allclose(g[0, 4, 0], (-1.0 + dv, 2.0 - dv, 0.0))
+
+
+def test_volumetric_locking_correction_closed_form(o64):
+    rng = np.random.default_rng(5)
+    n = 400
+    x = rng.uniform(0.1, 3.9, (n, 3))
+    C = rng.normal(0, 1, (n, 3, 3))
+    vol = rng.uniform(0.5, 2.0, n)
+    act = np.ones(n, np.int32)
+    lab = np.ones(n, np.int32)
+    lab[:20] = 2  # XPBD particles neither contribute nor change
+    C0 = C.copy()
+    cd, cv = o64.volumetric_locking_correction(act, lab, x, C, vol, 2.0, np.zeros(3), 8, 8, 8)
+    cell = (x * 2.0).astype(int)
+    key = (cell[:, 0] * 8 + cell[:, 1]) * 8 + cell[:, 2]
+    mpm = lab <= 1
+    assert np.array_equal(C[~mpm], C0[~mpm])
+    eye = np.eye(3)[None]
+    dev = lambda M: M - np.trace(M, axis1=1, axis2=2)[:, None, None] / 3 * eye  # noqa: E731
+    assert np.allclose(dev(C[mpm]), dev(C0[mpm]), atol=1e-12)  # shear part untouched
+    tr, tr0 = np.trace(C, axis1=1, axis2=2), np.trace(C0, axis1=1, axis2=2)
+    for k in np.unique(key[mpm]):
+        sel = mpm & (key == k)
+        want = np.sum(tr0[sel] * vol[sel]) / np.sum(vol[sel])
+        assert np.allclose(tr[sel], want, atol=1e-12)  # every particle carries the cell's mean divergence
+        assert np.isclose(cv.reshape(-1)[k], vol[sel].sum())
