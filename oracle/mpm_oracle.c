@@ -554,8 +554,16 @@ EXPORT void o_vlc(int n, const int *active, const int *label, const R *x, R *C, 
     for (int pass = 0; pass < 2; pass++)
         for (int p = 0; p < n; p++) {
             if (!(active[p] == 1 && label[p] <= 1)) continue;
+            /* the cell is an index, fixed by the reference's float32 arithmetic
+             * ((x - minBounds) * inv_dx in wp.vec3, :840-844): particles placed
+             * exactly at grid_padding cells from minBounds sit on a cell face,
+             * where FP64 would pick the other side */
             int c[3];
-            for (int a = 0; a < 3; a++) c[a] = (int)((x[3 * p + a] - minB[a]) * inv_dx);
+            for (int a = 0; a < 3; a++) {
+                volatile float d = (float)x[3 * p + a] - (float)minB[a];
+                volatile float g = d * (float)inv_dx;
+                c[a] = (int)g;
+            }
             if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= gx || c[1] >= gy || c[2] >= gz) continue;
             long k = ((long)c[0] * gy + c[1]) * gz + c[2];
             R *Cp = C + 9 * p;

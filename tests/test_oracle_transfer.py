@@ -273,3 +273,19 @@ def test_volumetric_locking_correction_closed_form(o64):
         want = np.sum(tr0[sel] * vol[sel]) / np.sum(vol[sel])
         assert np.allclose(tr[sel], want, atol=1e-12)  # every particle carries the cell's mean divergence
         assert np.isclose(cv.reshape(-1)[k], vol[sel].sum())
+
+
+def test_volumetric_locking_cell_follows_float32(o64):
+    """The cell is int((x - minBounds) * inv_dx) in float32 (mpmRoutines.py:840-844):
+    a particle sitting on a cell face by construction (the lowest particle is
+    grid_padding cells above minBounds) goes where float32 rounding puts it."""
+    o = o64
+    x0, mb = np.float32(0.07561769), np.float32(-3.9243823)
+    c32 = int((x0 - mb) * np.float32(2.0))
+    c64 = int((np.float64(x0) - np.float64(mb)) * 2.0)
+    assert (c32, c64) == (8, 7)  # the two precisions disagree on this face
+    x = np.array([[x0, 1.0, 1.0]], np.float64)
+    C = np.zeros((1, 3, 3))
+    cd, cv = o.volumetric_locking_correction(np.ones(1, np.int32), np.ones(1, np.int32), x, C, np.ones(1), 2.0,
+                                             np.array([mb, 0.0, 0.0]), 16, 16, 16)
+    assert cv[8, 2, 2] == 1.0 and cv.sum() == 1.0
