@@ -180,6 +180,18 @@ int mpmx_copy_f32(float* dst, const float* src, long n, void* stream);
 
 /* ---- MPM kernels (utils/mpmRoutines.py) ---------------------------------- */
 
+/* initialize_geostatic_F (mpmRoutines.py:52-124) or, with plane_strain = 1,
+ * initialize_geostatic_F_planestrain (:128-203), as launched at runMPMYDW.py:415-438:
+ * F = diag(exp eps) with eps the linear-elastic strain of the geostatic stress
+ * sigma_v = -rho g (z_top - z), sigma_h = K0 sigma_v.  Overwrites F [9][n]. */
+int mpmx_initialize_geostatic_F(int n, const float* x, float* F, const float* mu, const float* lam,
+                                const float* density, float gravity, float z_top, float K0,
+                                int plane_strain, void* stream);
+/* deactivate_and_teleport_xpbd_below_datum, mpmRoutines.py:1446-1475 (launched at
+ * the end of every big step, runMPMYDW.py:864-877) */
+int mpmx_deactivate_and_teleport_xpbd_below_datum(int n, float* x, float* v, int32_t* activeLabel,
+                                                  const int32_t* materialLabel, float z_datum,
+                                                  float teleport_distance, void* stream);
 /* compute_mu_lam_bulk_from_E_nu, mpmRoutines.py:206-226 */
 int mpmx_compute_mu_lam_bulk(int n, const float* E, const float* nu, float* mu, float* lam,
                              float* bulk, void* stream);

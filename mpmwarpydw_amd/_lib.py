@@ -59,6 +59,8 @@ SIGNATURES = {
     "mpmx_fill_f32": [P, L, F, P],
     "mpmx_copy_f32": [P, P, L, P],
     "mpmx_compute_mu_lam_bulk": [I, P, P, P, P, P, P],
+    "mpmx_initialize_geostatic_F": [I, P, P, P, P, P, F, F, F, I, P],
+    "mpmx_deactivate_and_teleport_xpbd_below_datum": [I, P, P, P, P, F, F, P],
     "mpmx_compute_stress_from_F_trial": [I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, F, P, P, P, P, P, P, I, P,
                                          P],
     "mpmx_p2g_workspace_bytes": [I, I, I, I],

@@ -25,6 +25,65 @@ __global__ void mu_lam_bulk_kernel(int n, const float* __restrict__ E, const flo
 }
 
 // --------------------------------------------------------------------------
+// initialize_geostatic_F (mpmRoutines.py:52-124) and, with plane_strain,
+// initialize_geostatic_F_planestrain (:128-203): F = diag(exp eps) of the
+// linear-elastic strain of the geostatic stress at the particle's depth
+// --------------------------------------------------------------------------
+__global__ void geostatic_F_kernel(int n, const float* __restrict__ x, float* __restrict__ F,
+                                   const float* __restrict__ mu, const float* __restrict__ lam,
+                                   const float* __restrict__ density, float g, float z_top, float K0,
+                                   int plane_strain) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float depth = z_top - x[2L * n + p];
+    float exx = 0.0f, eyy = 0.0f, ezz = 0.0f;
+    if (!plane_strain) {
+        const float sv = -density[p] * g * depth;
+        const float sh = K0 * sv;
+        const float m = mu[p], l = lam[p];
+        const float a11 = 2.0f * l + 2.0f * m, a12 = l, a21 = 2.0f * l, a22 = l + 2.0f * m;
+        const float det = a11 * a22 - a12 * a21;
+        if (fabsf(det) > 1e-12f) {
+            exx = eyy = (sh * a22 - sv * a12) / det;  // Cramer's rule on the (eps_h, eps_v) system
+            ezz = (a11 * sv - a21 * sh) / det;
+        }
+    } else if (depth > 0.0f) {
+        const float m = mu[p], l = lam[p];
+        const float nu = l / (2.0f * (l + m));
+        const float E = m * (3.0f * l + 2.0f * m) / (l + m);
+        const float sv = -density[p] * g * depth;
+        const float sh = K0 * sv;
+        const float syy = nu * (sh + sv);  // eps_yy = 0
+        exx = (sh - nu * (syy + sv)) / E;
+        ezz = (sv - nu * (sh + syy)) / E;
+    }
+    const float d[3] = {expf(exx), expf(eyy), expf(ezz)};
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) F[(long)(3 * r + c) * n + p] = r == c ? d[r] : 0.0f;
+}
+
+// --------------------------------------------------------------------------
+// deactivate_and_teleport_xpbd_below_datum (mpmRoutines.py:1446-1475)
+// --------------------------------------------------------------------------
+__global__ void deactivate_teleport_kernel(int n, float* __restrict__ x, float* __restrict__ v,
+                                           int32_t* __restrict__ activeLabel,
+                                           const int32_t* __restrict__ materialLabel, float z_datum,
+                                           float teleport) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || activeLabel[p] != 1 || materialLabel[p] != 2) return;
+    const float z = x[2L * n + p];
+    if (z < z_datum) {
+        activeLabel[p] = 0;
+        x[2L * n + p] = z - teleport;
+        v[p] = 0.0f;
+        v[(long)n + p] = 0.0f;
+        v[2L * n + p] = 0.0f;
+    }
+}
+
+// --------------------------------------------------------------------------
 // compute_stress_from_F_trial (mpmRoutines.py:229-312), one thread per particle
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) stress_kernel(mpmx_particles P, float eff) {
@@ -244,6 +303,28 @@ int mpmx_compute_mu_lam_bulk(int n, const float* E, const float* nu, float* mu, 
     MPMX_LAUNCH(mu_lam_bulk_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, E, nu,
                        mu, lam, bulk);
     return check_launch("compute_mu_lam_bulk", as_stream(stream));
+}
+
+int mpmx_initialize_geostatic_F(int n, const float* x, float* F, const float* mu, const float* lam,
+                                const float* density, float gravity, float z_top, float K0, int plane_strain,
+                                void* stream) {
+    MPMX_REQUIRE(n >= 0, "mpmx_initialize_geostatic_F: n < 0");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(x && F && mu && lam && density, "mpmx_initialize_geostatic_F: null pointer");
+    MPMX_LAUNCH(geostatic_F_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, x, F, mu, lam,
+                density, gravity, z_top, K0, plane_strain ? 1 : 0);
+    return check_launch("initialize_geostatic_F", as_stream(stream));
+}
+
+int mpmx_deactivate_and_teleport_xpbd_below_datum(int n, float* x, float* v, int32_t* activeLabel,
+                                                  const int32_t* materialLabel, float z_datum,
+                                                  float teleport_distance, void* stream) {
+    MPMX_REQUIRE(n >= 0, "mpmx_deactivate_and_teleport_xpbd_below_datum: n < 0");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(x && v && activeLabel && materialLabel, "mpmx_deactivate_and_teleport_xpbd_below_datum: null pointer");
+    MPMX_LAUNCH(deactivate_teleport_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, as_stream(stream), n, x, v,
+                activeLabel, materialLabel, z_datum, teleport_distance);
+    return check_launch("deactivate_and_teleport_xpbd_below_datum", as_stream(stream));
 }
 
 int mpmx_compute_stress_from_F_trial(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x,

@@ -7,9 +7,11 @@ XPBD ratio, :278-312 material upload, Lame constants and mass, :413-470
 geostatic/identity F, damage and radius, :472-495 the XPBD box) on a domain
 given as arrays instead of an HDF5 file (h5py is not part of this build).
 
-``Simulation.run_phase1`` is the combined MPM+XPBD loop of
-runMPMYDW.py:638-770 with its termination rules (minSteps = 1000, velocity
-residual, damage stall); its reductions are FP64 on the device.
+``run`` is the big-step loop of runMPMYDW.py:638-877: the combined MPM+XPBD
+phase with its gravity pulse and termination rules (minSteps = 1000, velocity
+residual, damage stall), the XPBD-only phase with its sleep test, and the
+deactivation below the datum; its reductions are FP64 on the device with one
+readback per substep.
 """
 from __future__ import annotations
 
@@ -132,42 +134,169 @@ class Simulation:
         lib("mpmx_invert_permutation", sim.particle_id.ptr, sim._slot_of_id.ptr, n, rt.stream_handle())
         mpm._bins = None
 
-    def residual(self) -> float:
-        """sum |v|/r / N_active over active particles (runMPMYDW.py:686-708): one readback."""
-        s, c = simulationRoutines.device_velocity_convergence(self.sim.nPoints, self.sim.activeLabel,
-                                                              self.sim.particle_v, self.sim.particle_radius)
+    # ---- what the loop reads back (runMPMYDW.py:686-722) ----
+    def loop_scalars(self) -> tuple[float, float]:
+        """(velocity residual, mean damage) after a substep: sum |v|/r over
+        active particles / active count (runMPMYDW.py:686-708) and the mean of
+        particle_damage over active particles (compute_mean_gpu, filtered),
+        both FP64 on the device, in one readback."""
         import torch
-        both = torch.cat([s, c.to(torch.float64)]).cpu().numpy()
-        return float(both[0] / both[1]) if both[1] > 0 else float("inf")
+        sim = self.sim
+        if getattr(self, "_scal", None) is None:
+            dev = rt.device()
+            self._scal = (torch.zeros(2, dtype=torch.float64, device=dev), torch.zeros(2, dtype=torch.int32,
+                                                                                        device=dev),
+                          rt.Workspace())
+        s, c, ws_ = self._scal
+        n = sim.nPoints
+        ws, nb = ws_.get(lib.raw("mpmx_reduce_workspace_bytes")(int(n)))
+        p = lambda t, k: C.c_void_p(t.data_ptr() + k * t.element_size())  # noqa: E731
+        lib("mpmx_velocity_convergence", n, sim.activeLabel.ptr, sim.particle_v.ptr, sim.particle_radius.ptr,
+            p(s, 0), p(c, 0), ws, nb, rt.stream_handle())
+        lib("mpmx_sum_float_array_active", n, self.mpm.particle_damage.ptr, sim.activeLabel.ptr, p(s, 1), p(c, 1), ws,
+            nb, rt.stream_handle())
+        h = torch.cat([s, c.to(torch.float64)]).cpu().numpy()
+        residual = float(h[0] / h[2]) if h[2] > 0 else float("inf")
+        damage = float(h[1] / h[3]) if h[3] > 0 else 0.0
+        return residual, damage
 
-    def run_phase1(self, max_steps: int | None = None, verbose: bool = False) -> int:
-        """The combined loop with the reference's termination rules; returns the substep count."""
-        sim, mpm = self.sim, self.mpm
-        nSteps = sim.nSteps if max_steps is None else min(sim.nSteps, max_steps)
-        minSteps = 1000
-        counter = 0
-        residual = 1e10
-        prev_damage = simulationRoutines.compute_mean_gpu(mpm.particle_damage, sim)
-        stall = 0
-        stall_steps = int(sim.damage_stall_duration / sim.dt)
-        t0 = time.time()
-        while counter < nSteps and (counter < minSteps or residual > sim.residualThreshold):
-            self.substep(counter)
-            residual = self.residual()
-            counter += 1
-            sim.t += sim.dt
-            dmg = simulationRoutines.compute_mean_gpu(mpm.particle_damage, sim)
-            if dmg - prev_damage < sim.damage_stall_threshold and counter > minSteps:
-                stall += 1
-            else:
-                stall = 0
-            prev_damage = dmg
-            if stall >= stall_steps and counter > minSteps:
-                break
-            if verbose and counter % 100 == 0:
-                print(f"  Step: {counter}, t: {sim.t:.4f}s, residual: {residual:.4e}, mean_damage: {dmg:.6f}, "
-                      f"t_real: {time.time() - t0:.2f}s", flush=True)
-        return counter
+    def residual(self) -> float:
+        return self.loop_scalars()[0]
+
+    def mean_damage(self) -> float:
+        return simulationRoutines.compute_mean_gpu(self.mpm.particle_damage, self.sim)
+
+    def xpbd_only_step(self) -> None:
+        """One step of the XPBD-only phase (runMPMYDW.py:784): MPM frozen."""
+        SR = simulationRoutines
+        (SR.xpbdSimulationStepFused if self.fused else SR.xpbdSimulationStep)(self.sim, self.mpm, self.xpbd,
+                                                                               xpbd_only=True)
+
+    def sleep_counts(self) -> tuple[int, int]:
+        """(XPBD particles, sleeping ones) from the last XPBD step's sleepParticles."""
+        return int(self.sim.numTotalXPBD.numpy()[0]), int(self.sim.numSleepingXPBD.numpy()[0])
+
+    def deactivate_below_datum(self) -> None:
+        """End of a big step (runMPMYDW.py:849-877): XPBD particles below the datum
+        (xpbd_deactivation_z_datum, else the grid's lower z bound) are switched
+        off and moved down by twice the domain height."""
+        sim, g = self.sim, self.geom
+        zd = getattr(self.args, "xpbd_deactivation_z_datum", None)
+        z_datum = float(g.minBounds[2]) if zd is None else float(zd)
+        teleport = 2.0 * float(g.maxBounds[2] - g.minBounds[2])
+        rt.launch(mpmRoutines.deactivate_and_teleport_xpbd_below_datum, sim.nPoints,
+                  inputs=[sim.particle_x, sim.particle_v, sim.activeLabel, sim.materialLabel, z_datum, teleport])
+        self.mpm._bins = None  # activity changed
+
+    # ---- the loop ----
+    def run_phase1(self, max_steps: int | None = None, verbose: bool = False, base_gravity=None) -> tuple:
+        return run_phase1(self, max_steps=max_steps, verbose=verbose, base_gravity=base_gravity)
+
+    def run(self, start_big_step: int = 0, big_steps: int | None = None, max_phase1_steps: int | None = None,
+            checkpoint=None, verbose: bool = False) -> list:
+        return run(self, start_big_step=start_big_step, big_steps=big_steps, max_phase1_steps=max_phase1_steps,
+                   checkpoint=checkpoint, verbose=verbose)
+
+
+MIN_STEPS = 1000  # runMPMYDW.py:662
+
+
+def run_phase1(S, max_steps: int | None = None, verbose: bool = False, base_gravity=None) -> tuple:
+    """The combined MPM+XPBD phase of one big step (runMPMYDW.py:643-770) with
+    its gravity pulse and termination rules; ``S`` is a Simulation (or any
+    object with its substep / loop_scalars / mean_damage methods).  Returns
+    (substeps, reason) with reason "nSteps", "residual" or "damage_stall".
+
+    The loop runs while counter < nSteps and (counter < 1000 or residual >
+    residualThreshold); after each substep the mean damage is compared with
+    the previous one and a run of damage_stall_duration/dt substeps past the
+    first 1000 with a change below damage_stall_threshold ends it early.  The
+    gravity pulse (gravity_pulse_factor for gravity_pulse_duration) is set at
+    the start and restored before the substep whose counter equals its length,
+    or on an early stop inside it; a phase that ends otherwise inside the pulse
+    keeps the pulsed gravity, as the reference does."""
+    sim, args = S.sim, S.args
+    base = tuple(sim.gravity) if base_gravity is None else tuple(base_gravity)
+    factor = float(getattr(args, "gravity_pulse_factor", 1.0))
+    dur = float(getattr(args, "gravity_pulse_duration", 0.0))
+    pulse_steps = int(dur / sim.dt) if dur > 0 else 0
+    pulsing = factor != 1.0 and pulse_steps > 0
+    if pulsing:
+        sim.gravity = tuple(g * factor for g in base)
+    nSteps = sim.nSteps if max_steps is None else min(sim.nSteps, max_steps)
+    counter = 0
+    residual = 1e10
+    prev = S.mean_damage()
+    stall = 0
+    stall_steps = int(sim.damage_stall_duration / sim.dt)
+    reason = None
+    t0 = time.time()
+    while counter < nSteps and (counter < MIN_STEPS or residual > sim.residualThreshold):
+        if pulsing and counter == pulse_steps:
+            sim.gravity = base
+        S.substep(counter)
+        residual, dmg = S.loop_scalars()
+        counter += 1
+        sim.t += sim.dt
+        if dmg - prev < sim.damage_stall_threshold and counter > MIN_STEPS:
+            stall += 1
+        else:
+            stall = 0
+        prev = dmg
+        if stall >= stall_steps and counter > MIN_STEPS:
+            if pulsing and counter < pulse_steps:
+                sim.gravity = base
+            reason = "damage_stall"
+            break
+        if verbose and counter % 100 == 0:
+            print(f"  Step: {counter}, t: {sim.t:.4f}s, residual: {residual:.4e}, mean_damage: {dmg:.6f}, "
+                  f"t_real: {time.time() - t0:.2f}s", flush=True)
+    if reason is None:
+        reason = "nSteps" if counter >= nSteps else "residual"
+    return counter, reason
+
+
+def run_phase2(S, verbose: bool = False) -> tuple:
+    """The XPBD-only debris phase (runMPMYDW.py:775-839): up to xpbdOnlySteps
+    steps of dtxpbd, ending early once the sleeping fraction of the XPBD
+    particles reaches xpbd_sleep_termination_ratio (when that is below 1).
+    Returns (steps, reason) with reason "steps", "asleep" or "off"."""
+    sim = S.sim
+    if sim.xpbdOnlySteps <= 0:
+        return 0, "off"
+    steps = 0
+    for k in range(1, sim.xpbdOnlySteps + 1):
+        S.xpbd_only_step()
+        sim.t += sim.dtxpbd
+        steps = k
+        if sim.xpbd_sleep_termination_ratio < 1.0:
+            total, asleep = S.sleep_counts()
+            if total > 0 and asleep / total >= sim.xpbd_sleep_termination_ratio:
+                return steps, "asleep"
+        if verbose and k % 100 == 0:
+            print(f"    XPBD Step: {k}/{sim.xpbdOnlySteps}, t: {sim.t:.4f}s", flush=True)
+    return steps, "steps"
+
+
+def run(S, start_big_step: int = 0, big_steps: int | None = None, max_phase1_steps: int | None = None,
+        checkpoint=None, verbose: bool = False) -> list:
+    """The big-step loop of runMPMYDW.py:638-877: for each big step the combined
+    phase, the XPBD-only phase, ``checkpoint(S, bigStep)`` (where the reference
+    saves its restart file) and the deactivation below the datum.  Returns one
+    record per big step."""
+    sim = S.sim
+    base = tuple(sim.gravity)  # the pulse scales the gravity the run started with
+    end = sim.bigSteps if big_steps is None else min(sim.bigSteps, start_big_step + big_steps)
+    out = []
+    for big in range(start_big_step, end):
+        n1, why1 = run_phase1(S, max_steps=max_phase1_steps, verbose=verbose, base_gravity=base)
+        n2, why2 = run_phase2(S, verbose=verbose)
+        if checkpoint is not None:
+            checkpoint(S, big)
+        S.deactivate_below_datum()
+        out.append({"bigStep": big, "phase1_steps": n1, "phase1_end": why1, "phase2_steps": n2, "phase2_end": why2,
+                    "t": sim.t})
+    return out
 
 
 def setup(args, domain: dict, fused: bool = True) -> Simulation:
@@ -229,10 +358,15 @@ def setup(args, domain: dict, fused: bool = True) -> Simulation:
     rt.launch(mpmRoutines.get_float_array_product, n, inputs=[sim.particle_density, sim.particle_vol,
                                                               sim.particle_mass])
 
+    # geostatic or identity F (runMPMYDW.py:407-445); z_top defaults to the highest particle
+    z_top = float(np.max(x[:, 2])) if getattr(args, "z_top", None) is None else float(args.z_top)
     if args.initialise_geostatic:
-        raise NotImplementedError("initialise_geostatic=1 (mpmRoutines.py:51-124) is not on the device path yet; "
-                                  "set it to 0")
-    rt.launch(mpmRoutines.set_mat33_to_identity, n, inputs=[mpm.particle_F])
+        k = (mpmRoutines.initialize_geostatic_F_planestrain if args.boundaryCondition == "plane_strain_y_global"
+             else mpmRoutines.initialize_geostatic_F)
+        rt.launch(k, n, inputs=[sim.particle_x, mpm.particle_F, mpm.mu, mpm.lam, sim.particle_density,
+                                abs(float(args.gravity)), z_top, mpm.K0])
+    else:
+        rt.launch(mpmRoutines.set_mat33_to_identity, n, inputs=[mpm.particle_F])
     rt.launch(mpmRoutines.set_mat33_to_copy, n, inputs=[mpm.particle_F, mpm.particle_F_trial])
     damage = prop("damage", 0.0)
     mpm.particle_damage = rt.array(damage)

@@ -30,6 +30,29 @@ compute_mu_lam_bulk_from_E_nu = Kernel("compute_mu_lam_bulk_from_E_nu", _compute
                                        "mpmRoutines.py:206-226")
 
 
+def _geostatic(plane_strain):
+    def fn(dim, particle_x, particle_F, mu, lam, density, gravity, z_top, K0):
+        lib("mpmx_initialize_geostatic_F", int(dim), particle_x.ptr, particle_F.ptr, mu.ptr, lam.ptr, density.ptr,
+            float(gravity), float(z_top), float(K0), int(plane_strain), stream_handle())
+    return fn
+
+
+initialize_geostatic_F = Kernel("initialize_geostatic_F", _geostatic(False), "mpmRoutines.py:52-124")
+initialize_geostatic_F_planestrain = Kernel("initialize_geostatic_F_planestrain", _geostatic(True),
+                                            "mpmRoutines.py:128-203")
+
+
+def _deactivate_and_teleport_xpbd_below_datum(dim, particle_x, particle_v, activeLabel, materialLabel, z_datum,
+                                              teleport_distance):
+    lib("mpmx_deactivate_and_teleport_xpbd_below_datum", int(dim), particle_x.ptr, particle_v.ptr, activeLabel.ptr,
+        materialLabel.ptr, float(z_datum), float(teleport_distance), stream_handle())
+
+
+deactivate_and_teleport_xpbd_below_datum = Kernel("deactivate_and_teleport_xpbd_below_datum",
+                                                  _deactivate_and_teleport_xpbd_below_datum,
+                                                  "mpmRoutines.py:1446-1475")
+
+
 def _compute_stress_from_F_trial(dim, activeLabel, materialLabel, particlePosition, particleVelocity,
                                  initialPhaseChangePosition, initialPhaseChangeVelocity, particle_F, particle_F_trial,
                                  mu, lam, yield_stress, hardening, softening, particle_density, strainCriteria,

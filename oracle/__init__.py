@@ -89,6 +89,18 @@ class Oracle:
         self._chk(E, nu, mu, lam, bulk)
         self.lib.o_compute_mu_lam_bulk(_I(len(E)), _ptr(E), _ptr(nu), _ptr(mu), _ptr(lam), _ptr(bulk))
 
+    def initialize_geostatic_F(self, x, F, mu, lam, density, g, z_top, K0, plane_strain=False):
+        """initialize_geostatic_F (mpmRoutines.py:52-124) or its plane-strain form (:128-203)."""
+        self._chk(x, F, mu, lam, density)
+        self.lib.o_initialize_geostatic_F(_I(len(mu)), _ptr(x), _ptr(F), _ptr(mu), _ptr(lam), _ptr(density),
+                                          self.r(g), self.r(z_top), self.r(K0), _I(int(bool(plane_strain))))
+
+    def deactivate_and_teleport(self, x, v, activeLabel, materialLabel, z_datum, teleport):
+        """deactivate_and_teleport_xpbd_below_datum (mpmRoutines.py:1446-1475); labels in place."""
+        self._chk(x, v, activeLabel, materialLabel)
+        self.lib.o_deactivate_and_teleport(_I(len(activeLabel)), _ptr(x), _ptr(v), _ptr(activeLabel),
+                                           _ptr(materialLabel), self.r(z_datum), self.r(teleport))
+
     # ---- A2/A3/A4 --------------------------------------------------------
     def compute_stress_from_F_trial(self, activeLabel, materialLabel, x, v, x_init, v_init, F, F_trial,
                                     mu, lam, ys, hardening, softening, density, strainCriteria, eff,

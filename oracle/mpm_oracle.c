@@ -204,6 +204,48 @@ EXPORT void o_compute_mu_lam_bulk(int n, const R *E, const R *nu, R *mu, R *lam,
 }
 
 /* ------------------------------------------------------------------ */
+/* initialize_geostatic_F (mpmRoutines.py:52-124) and                  */
+/* initialize_geostatic_F_planestrain (:128-203); F is AoS 9 per particle */
+/* ------------------------------------------------------------------ */
+EXPORT void o_initialize_geostatic_F(int n, const R *x, R *F, const R *mu, const R *lam, const R *density, R g,
+                                     R z_top, R K0, int plane_strain) {
+    for (int p = 0; p < n; p++) {
+        R depth = z_top - x[3 * p + 2];
+        R e[3] = {0, 0, 0};
+        if (!plane_strain) {
+            /* solve [2l+2m, l; 2l, l+2m] (eps_h, eps_v) = (sigma_h, sigma_v) by Cramer (:104-110) */
+            R sv = -density[p] * g * depth, sh = K0 * sv;
+            R a11 = (R)2 * lam[p] + (R)2 * mu[p], a12 = lam[p], a21 = (R)2 * lam[p], a22 = lam[p] + (R)2 * mu[p];
+            R det = a11 * a22 - a12 * a21;
+            if (fabs((double)det) > 1e-12) {
+                e[0] = e[1] = (sh * a22 - sv * a12) / det;
+                e[2] = (a11 * sv - a21 * sh) / det;
+            }
+        } else if (depth > 0) {
+            /* Hooke's law with eps_yy = 0, so sigma_yy = nu (sigma_xx + sigma_zz) (:172-191) */
+            R nu = lam[p] / ((R)2 * (lam[p] + mu[p]));
+            R E = mu[p] * ((R)3 * lam[p] + (R)2 * mu[p]) / (lam[p] + mu[p]);
+            R sv = -density[p] * g * depth, sh = K0 * sv, syy = nu * (sh + sv);
+            e[0] = (sh - nu * (syy + sv)) / E;
+            e[2] = (sv - nu * (sh + syy)) / E;
+        }
+        R *Fp = F + 9 * p;
+        for (int k = 0; k < 9; k++) Fp[k] = 0;
+        for (int k = 0; k < 3; k++) Fp[4 * k] = (R)exp((double)e[k]);
+    }
+}
+
+/* deactivate_and_teleport_xpbd_below_datum (mpmRoutines.py:1446-1475) */
+EXPORT void o_deactivate_and_teleport(int n, R *x, R *v, int *active, const int *label, R z_datum, R teleport) {
+    for (int p = 0; p < n; p++) {
+        if (active[p] != 1 || label[p] != 2 || !(x[3 * p + 2] < z_datum)) continue;
+        active[p] = 0;
+        x[3 * p + 2] -= teleport;
+        v[3 * p] = v[3 * p + 1] = v[3 * p + 2] = 0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* A4 kirchoff_stress (mpmRoutines.py:456-471):  U diag(c) V^T F^T      */
 /* ------------------------------------------------------------------ */
 static void kirchoff_stress(const R *F, const R *U, const R *V, const R *sig, const R *lam_sq,

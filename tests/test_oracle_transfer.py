@@ -289,3 +289,67 @@ def test_volumetric_locking_cell_follows_float32(o64):
     cd, cv = o.volumetric_locking_correction(np.ones(1, np.int32), np.ones(1, np.int32), x, C, np.ones(1), 2.0,
                                              np.array([mb, 0.0, 0.0]), 16, 16, 16)
     assert cv[8, 2, 2] == 1.0 and cv.sum() == 1.0
+
+
+def _kirchhoff_diag(F, mu, lam):
+    """Closed-form Kirchhoff stress of a diagonal F: 2 mu ln F_ii + lam sum ln F_jj."""
+    e = np.log(np.stack([F[:, 0], F[:, 4], F[:, 8]], axis=1))
+    return 2 * mu[:, None] * e + lam[:, None] * e.sum(axis=1, keepdims=True)
+
+
+def test_geostatic_F_solves_the_stress_inverse_problem(o64):
+    """initialize_geostatic_F (mpmRoutines.py:52-124): the Kirchhoff stress of the
+    F it returns is (K0 sv, K0 sv, sv) with sv = -rho g (z_top - z), at every
+    depth (tension above z_top included), and F is diagonal."""
+    rng = np.random.default_rng(11)
+    n = 300
+    E = np.exp(rng.uniform(np.log(1e7), np.log(1e9), n))
+    nu = rng.uniform(0.05, 0.45, n)
+    mu, lam = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    rho = rng.uniform(1500, 3000, n)
+    x = rng.uniform(-20, 20, (n, 3))
+    F = np.zeros((n, 9))
+    g, z_top, K0 = 9.81, 15.0, 0.6
+    o64.initialize_geostatic_F(x, F, mu, lam, rho, g, z_top, K0)
+    sv = -rho * g * (z_top - x[:, 2])
+    tau = _kirchhoff_diag(F, mu, lam)
+    want = np.stack([K0 * sv, K0 * sv, sv], axis=1)
+    assert np.allclose(tau, want, rtol=1e-9, atol=1e-9 * np.abs(sv).max())
+    assert np.array_equal(F[:, [1, 2, 3, 5, 6, 7]], np.zeros((n, 6)))
+
+
+def test_geostatic_F_plane_strain(o64):
+    """initialize_geostatic_F_planestrain (mpmRoutines.py:128-203): eps_yy = 0,
+    tau_xx = K0 sv, tau_zz = sv, tau_yy = nu (tau_xx + tau_zz) below z_top, and
+    F = I at or above it."""
+    rng = np.random.default_rng(12)
+    n = 300
+    E = np.exp(rng.uniform(np.log(1e7), np.log(1e9), n))
+    nu = rng.uniform(0.05, 0.45, n)
+    mu, lam = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    rho = rng.uniform(1500, 3000, n)
+    x = rng.uniform(-20, 20, (n, 3))
+    F = np.full((n, 9), 7.0)
+    g, z_top, K0 = 9.81, 10.0, 0.45
+    o64.initialize_geostatic_F(x, F, mu, lam, rho, g, z_top, K0, plane_strain=True)
+    below = x[:, 2] < z_top
+    assert np.array_equal(F[~below], np.tile(np.eye(3).ravel(), ((~below).sum(), 1)))
+    assert np.array_equal(F[:, 4], np.ones(n))
+    sv = -rho * g * (z_top - x[:, 2])
+    tau = _kirchhoff_diag(F, mu, lam)[below]
+    s = np.abs(sv).max()
+    assert np.allclose(tau[:, 0], (K0 * sv)[below], rtol=1e-9, atol=1e-9 * s)
+    assert np.allclose(tau[:, 2], sv[below], rtol=1e-9, atol=1e-9 * s)
+    assert np.allclose(tau[:, 1], nu[below] * (tau[:, 0] + tau[:, 2]), rtol=1e-9, atol=1e-9 * s)
+
+
+def test_deactivate_and_teleport_hand_case(o64):
+    x = np.array([[0, 0, -1.0], [0, 0, -1.0], [0, 0, -1.0], [0, 0, 1.0], [0, 0, -1.0]])
+    v = np.ones((5, 3))
+    act = np.array([1, 1, 0, 1, 1], np.int32)
+    lab = np.array([2, 1, 2, 2, 2], np.int32)
+    o64.deactivate_and_teleport(x, v, act, lab, 0.0, 30.0)
+    # only active XPBD particles below the datum move, stop and switch off
+    assert act.tolist() == [0, 1, 0, 1, 0]
+    assert x[:, 2].tolist() == [-31.0, -1.0, -1.0, 1.0, -31.0]
+    assert v.tolist() == [[0, 0, 0], [1, 1, 1], [1, 1, 1], [1, 1, 1], [0, 0, 0]]
