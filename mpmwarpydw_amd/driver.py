@@ -198,10 +198,50 @@ class Simulation:
                    checkpoint=checkpoint, verbose=verbose)
 
 
+def apply_restart(S, restart: dict) -> int:
+    """Restore a checkpoint (fs5PlotUtils.load_restart_from_vtp) into a state
+    set up from the same domain, as runMPMYDW.py:496-576 does: time, positions,
+    velocities, labels, F (and F_trial = F), C, accumulated strain, damage,
+    stress, yield stresses, material and physical properties, radius and the
+    XPBD state that the file holds.  Returns the big step to start from."""
+    sim, mpm, xpbd = S.sim, S.mpm, S.xpbd
+    n = sim.nPoints
+    if len(restart["particle_x"]) != n:
+        raise ValueError(f"Restart file has {len(restart['particle_x'])} particles, but domain has {n}. "
+                         "Cannot restart with different particle count.")
+    sim.t = float(restart["sim_time"])
+    sim.particle_x.assign(restart["particle_x"])
+    sim.particle_v.assign(restart["particle_v"])
+    for key, arr in (("materialLabel", sim.materialLabel), ("activeLabel", sim.activeLabel)):
+        if restart.get(key) is not None:
+            arr.assign(restart[key])
+    mpm.particle_F.assign(restart["particle_F"])
+    mpm.particle_C.assign(restart["particle_C"])
+    rt.launch(mpmRoutines.set_mat33_to_copy, n, inputs=[mpm.particle_F, mpm.particle_F_trial])
+    for key, arr in (("particle_accumulated_strain", mpm.particle_accumulated_strain),
+                     ("particle_damage", mpm.particle_damage), ("particle_stress", mpm.particle_stress),
+                     ("ys", mpm.ys), ("ys_base", mpm.ys_base)):
+        if restart.get(key) is not None:
+            arr.assign(restart[key])
+    for prop in ("mu", "lam", "bulk", "alpha", "hardening", "softening", "eta_shear", "eta_bulk", "strainCriteria"):
+        if restart.get(prop) is not None:
+            getattr(mpm, prop).assign(restart[prop])
+    for key, arr in (("particle_vol", sim.particle_vol), ("particle_mass", sim.particle_mass),
+                     ("particle_density", sim.particle_density), ("particle_radius", sim.particle_radius)):
+        if restart.get(key) is not None:
+            arr.assign(restart[key])
+    if restart.get("particle_x_initial_xpbd") is not None:
+        xpbd.particle_x_initial_xpbd.assign(restart["particle_x_initial_xpbd"])
+    if restart.get("particle_cumDist_xpbd") is not None:
+        xpbd.particle_cumDist_xpbd.assign(restart["particle_cumDist_xpbd"])
+    mpm._bins = None
+    return int(restart["bigStep"]) + 1
+
+
 MIN_STEPS = 1000  # runMPMYDW.py:662
 
 
-def run_phase1(S, max_steps: int | None = None, verbose: bool = False, base_gravity=None) -> tuple:
+def run_phase1(S, max_steps: int | None = None, verbose: bool = False, base_gravity=None, after_step=None) -> tuple:
     """The combined MPM+XPBD phase of one big step (runMPMYDW.py:643-770) with
     its gravity pulse and termination rules; ``S`` is a Simulation (or any
     object with its substep / loop_scalars / mean_damage methods).  Returns
@@ -214,7 +254,9 @@ def run_phase1(S, max_steps: int | None = None, verbose: bool = False, base_grav
     gravity pulse (gravity_pulse_factor for gravity_pulse_duration) is set at
     the start and restored before the substep whose counter equals its length,
     or on an early stop inside it; a phase that ends otherwise inside the pulse
-    keeps the pulsed gravity, as the reference does."""
+    keeps the pulsed gravity, as the reference does.  ``after_step(counter)``
+    runs at the end of every substep that did not stop the loop early (where
+    the reference renders and saves, :765-769)."""
     sim, args = S.sim, S.args
     base = tuple(sim.gravity) if base_gravity is None else tuple(base_gravity)
     factor = float(getattr(args, "gravity_pulse_factor", 1.0))
@@ -251,16 +293,19 @@ def run_phase1(S, max_steps: int | None = None, verbose: bool = False, base_grav
         if verbose and counter % 100 == 0:
             print(f"  Step: {counter}, t: {sim.t:.4f}s, residual: {residual:.4e}, mean_damage: {dmg:.6f}, "
                   f"t_real: {time.time() - t0:.2f}s", flush=True)
+        if after_step is not None:
+            after_step(counter)
     if reason is None:
         reason = "nSteps" if counter >= nSteps else "residual"
     return counter, reason
 
 
-def run_phase2(S, verbose: bool = False) -> tuple:
+def run_phase2(S, verbose: bool = False, after_step=None) -> tuple:
     """The XPBD-only debris phase (runMPMYDW.py:775-839): up to xpbdOnlySteps
     steps of dtxpbd, ending early once the sleeping fraction of the XPBD
     particles reaches xpbd_sleep_termination_ratio (when that is below 1).
-    Returns (steps, reason) with reason "steps", "asleep" or "off"."""
+    Returns (steps, reason) with reason "steps", "asleep" or "off";
+    ``after_step(k)`` runs after every step that did not end the phase."""
     sim = S.sim
     if sim.xpbdOnlySteps <= 0:
         return 0, "off"
@@ -275,6 +320,8 @@ def run_phase2(S, verbose: bool = False) -> tuple:
                 return steps, "asleep"
         if verbose and k % 100 == 0:
             print(f"    XPBD Step: {k}/{sim.xpbdOnlySteps}, t: {sim.t:.4f}s", flush=True)
+        if after_step is not None:
+            after_step(k)
     return steps, "steps"
 
 
@@ -283,14 +330,48 @@ def run(S, start_big_step: int = 0, big_steps: int | None = None, max_phase1_ste
     """The big-step loop of runMPMYDW.py:638-877: for each big step the combined
     phase, the XPBD-only phase, ``checkpoint(S, bigStep)`` (where the reference
     saves its restart file) and the deactivation below the datum.  Returns one
-    record per big step."""
+    record per big step.
+
+    With sim.saveFlag set, snapshots follow check_render_save (:609-628): one
+    whenever sim.t passes the next multiple of render_interval (within half a
+    step), one forced at the end of the combined phase, and a checkpoint with
+    the restart data at the end of every big step (fs5PlotUtils.save_mpm)."""
     sim = S.sim
     base = tuple(sim.gravity)  # the pulse scales the gravity the run started with
     end = sim.bigSteps if big_steps is None else min(sim.bigSteps, start_big_step + big_steps)
+    saving = bool(getattr(sim, "saveFlag", 0))
+    interval = float(getattr(S.args, "render_interval", 0.1)) if saving else 0.0
+    state = {"next": sim.t + interval, "counter": 0, "big": start_big_step}
+
+    def save(dt_tol, force=False, restart=False):
+        if not saving:
+            return
+        if force or sim.t >= state["next"] - 0.5 * dt_tol:
+            from . import fs5PlotUtils
+            fs5PlotUtils.save_mpm(sim, S.mpm, state["big"], state["counter"], include_restart_data=restart,
+                                  xpbd=S.xpbd if restart else None)
+            if not force:
+                state["next"] += interval
+
+    def after_mpm(counter):
+        state["counter"] = counter
+        save(sim.dt)
+
     out = []
     for big in range(start_big_step, end):
-        n1, why1 = run_phase1(S, max_steps=max_phase1_steps, verbose=verbose, base_gravity=base)
-        n2, why2 = run_phase2(S, verbose=verbose)
+        state["big"] = big
+        n1, why1 = run_phase1(S, max_steps=max_phase1_steps, verbose=verbose, base_gravity=base,
+                              after_step=after_mpm if saving else None)
+        state["counter"] = n1
+        save(sim.dt, force=True)
+
+        def after_xpbd(k, n1=n1):
+            state["counter"] = n1 + k
+            save(sim.dtxpbd)
+
+        n2, why2 = run_phase2(S, verbose=verbose, after_step=after_xpbd if saving else None)
+        state["counter"] = n1 + n2
+        save(sim.dt, force=True, restart=True)
         if checkpoint is not None:
             checkpoint(S, big)
         S.deactivate_below_datum()

@@ -4,6 +4,8 @@ FP64 oracle, and one short run of the big-step loop on the device.
 
 Tolerance for F: FP32 stores exp(eps) near 1, so |F_gpu - F_oracle| may reach
 one float32 ulp of 1 (1.2e-7) on top of 1e-5 of |F - I|."""
+import os
+
 import numpy as np
 import pytest
 
@@ -90,3 +92,39 @@ def test_big_step_loop_on_device():
     assert np.isfinite(res) and res > 0 and 0 <= dmg <= 1
     x = sim.particle_x.numpy()
     assert np.isfinite(x[sim.activeLabel.numpy() == 1]).all()
+
+
+def test_checkpoint_restart_continues_bit_for_bit(tmp_path):
+    """save_mpm with the restart data, load_restart_from_vtp and apply_restart
+    into a state set up from the same domain: every restored array equals the
+    saved one, and the next substep gives the same bits as the run that was
+    saved (after applying the restart's F_trial = F to it too)."""
+    from mpmwarpydw_amd import driver, fs5PlotUtils
+    from state_utils import F_FIELDS_MPM, F_FIELDS_SIM, F_FIELDS_XPBD
+    A = make_sim(seed=44, outputFolder=str(tmp_path))
+    for counter in range(1, 4):
+        A.substep(counter)
+    A.sim.t = 0.0123456789
+    print(f"[restart] n={A.sim.nPoints}", flush=True)
+    prefix = fs5PlotUtils.save_mpm(A.sim, A.mpm, 3, 7, include_restart_data=True, xpbd=A.xpbd)
+    assert os.path.exists(prefix + "_grid.vti")
+    r = fs5PlotUtils.load_restart_from_vtp(prefix + "_particles.vtp")
+    B = make_sim(seed=44)
+    assert driver.apply_restart(B, r) == 4
+    assert B.sim.t == A.sim.t
+    # what the reference's checkpoint leaves out comes over by hand
+    B.xpbd.particle_v_initial_xpbd.assign(A.xpbd.particle_v_initial_xpbd.numpy())
+    B.mpm.particle_cov.assign(A.mpm.particle_cov.numpy())
+    A.mpm.particle_F_trial.assign(A.mpm.particle_F.numpy())
+    grid = {"grid_m", "grid_v_in", "grid_v_out"}
+    for objs, names in (((A.sim, B.sim), F_FIELDS_SIM), ((A.mpm, B.mpm), F_FIELDS_MPM),
+                        ((A.xpbd, B.xpbd), F_FIELDS_XPBD)):
+        for k in names:
+            if k in grid or k == "particle_stress":
+                continue
+            assert np.array_equal(getattr(objs[1], k).numpy(), getattr(objs[0], k).numpy()), k
+    A.substep(4)
+    B.substep(4)
+    a, b = host_state(A), host_state(B)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
