@@ -5,7 +5,8 @@ hot path.
 (runMPMYDW.py:22-67 grid spacing and bounds, :110-169 CFL dt and the integer
 XPBD ratio, :278-312 material upload, Lame constants and mass, :413-470
 geostatic/identity F, damage and radius, :472-495 the XPBD box) on a domain
-given as arrays instead of an HDF5 file (h5py is not part of this build).
+given as arrays; ``load_domain`` reads the reference's HDF5 domain files (with
+this package's own reader, h5py is not part of this build) or .vtp domains.
 
 ``run`` is the big-step loop of runMPMYDW.py:638-877: the combined MPM+XPBD
 phase with its gravity pulse and termination rules (minSteps = 1000, velocity
@@ -473,3 +474,33 @@ def setup(args, domain: dict, fused: bool = True) -> Simulation:
             arr.assign(h)
     geom = Geometry(dx, 1.0 / dx, minB, maxB, dims, dt, ratio)
     return Simulation(args, sim, mpm, xpbd, geom, fused=fused)
+
+
+def load_domain(path: str) -> dict:
+    """A particle domain from the reference's HDF5 format (runMPMYDW.py:19-23,
+    70-81) or from a .vtp holding the same arrays."""
+    if path.lower().endswith(".vtp"):
+        from .fs5PlotUtils import load_domain_vtp
+        return load_domain_vtp(path)
+    from .hdf5 import load_domain_h5
+    return load_domain_h5(path)
+
+
+def main(argv=None, verbose: bool = True) -> list:
+    """runMPMYDW.py end to end: flags and JSON config, the domain file, setup,
+    an optional restart (--restart a checkpoint .vtp), then the big-step loop
+    with snapshots and checkpoints under outputFolder when saveFlag is set."""
+    from . import getArgs
+    args = getArgs.get_args(argv)
+    S = setup(args, load_domain(args.domainFile))
+    start = 0
+    if args.restart is not None:
+        from .fs5PlotUtils import load_restart_from_vtp
+        start = apply_restart(S, load_restart_from_vtp(args.restart))
+    return S.run(start_big_step=start, verbose=verbose)
+
+
+if __name__ == "__main__":
+    import sys
+    for rec in main(sys.argv[1:]):
+        print(rec)

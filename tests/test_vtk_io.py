@@ -164,3 +164,30 @@ def test_restart_defaults_of_a_snapshot_without_restart_data(tmp_path):
     assert np.array_equal(got["particle_C"], np.zeros((n, 3, 3)))
     assert np.array_equal(got["ys_base"], got["ys"]) and got["bigStep"] == 0 and got["sim_time"] == 0.0
     assert np.array_equal(got["particle_radius"], np.full(n, 0.5, np.float32))
+
+
+def test_hdf5_domain_matches_its_vtp_twin(tmp_path):
+    """The reference's HDF5 domain (h5py-written) and its .vtp twin (VTK-written)
+    hold the same particles: the two readers agree bit for bit."""
+    import gzip
+
+    from mpmwarpydw_amd import hdf5
+    h5 = tmp_path / "d.h5"
+    h5.write_bytes(gzip.decompress(open(GOLDEN.replace(".vtp", ".h5.gz"), "rb").read()))
+    ds = hdf5.read_datasets(str(h5))
+    assert ds["x"].shape == (3, 32000) and ds["x"].dtype == np.float64
+    assert set(ds) == {"x", "particle_volume", "density", "E", "nu", "ys", "alpha", "hardening", "softening",
+                       "eta_shear", "eta_bulk", "strainCriteria"}
+    a = hdf5.load_domain_h5(str(h5))
+    b = fs5PlotUtils.load_domain_vtp(GOLDEN)
+    assert set(a) == set(b)
+    for k in a:
+        assert a[k].dtype == b[k].dtype and np.array_equal(a[k], b[k]), k
+
+
+def test_hdf5_rejects_what_it_does_not_read(tmp_path):
+    from mpmwarpydw_amd import hdf5
+    p = tmp_path / "x.h5"
+    p.write_bytes(b"not an hdf5 file" * 100)
+    with pytest.raises(hdf5.HDF5FormatError):
+        hdf5.read_datasets(str(p))
