@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # accumulated strain 4, damage 4, label 4, x 12, v 12, vol 4, mass 4, bin id 4;
 # writes F 36, stress 36, accumulated strain / damage / ys 12
 TILE_BYTES_PER_PARTICLE = 156.0 + 84.0
+PROFILE_HOLD_US = 50000  # long enough for the host to queue a whole profiled period
 
 
 def parse():
@@ -209,21 +210,27 @@ def main():
                     S.substep(c)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    lab = sim.materialLabel.numpy()
+    act = sim.activeLabel.numpy()
+    n_mpm = int(((act == 1) & (lab <= 1)).sum())  # MPM particles at the end of the timed region
 
-    # per-kernel device time: one more period launched eagerly on the same stream with
-    # every library kernel bracketed by HIP events (graph-recorded events cannot be timed)
-    lib("mpmx_profile_begin", 2 * (64 * ratio + 64))
+    # per-kernel device time: one more period launched eagerly on the same stream, every
+    # library kernel carrying dispatch-stamped events (graph launches cannot carry them)
     with torch.cuda.stream(stream):
+        # the periods queue up behind the hold and then run back to back; the
+        # first brings the clocks back up after the idle hold, the second is timed
+        lib("mpmx_profile_hold", stream_handle(), PROFILE_HOLD_US)
+        for c in period:
+            S.substep(c)
+        lib("mpmx_profile_begin", 2 * (64 * ratio + 64))
         for c in period:
             S.substep(c)
     torch.cuda.synchronize()
     lib("mpmx_profile_end")
     kernels = profile_report()
-    how = "HIP events bracketing each kernel, one eager period right after the timed region"
+    how = ("HIP start/stop events stamped by each kernel's own dispatch (hipExtLaunchKernelGGL) over one eager "
+           "period after the timed region; it and a warm-up period are queued behind a hold kernel")
 
-    lab = sim.materialLabel.numpy()
-    act = sim.activeLabel.numpy()
-    n_mpm = int(((act == 1) & (lab <= 1)).sum())
     value = n_mpm * steps / wall
     G = int(np.prod(mpm.gridDims))
     dom_name, dom_k = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
@@ -274,6 +281,7 @@ def bench_ensemble(a):
     from mpmwarpydw_amd import autotune, driver, getArgs
     from mpmwarpydw_amd._lib import lib
     from mpmwarpydw_amd.ensemble import Ensemble, c4_scene_list
+    from mpmwarpydw_amd.runtime import stream_handle
 
     t0 = time.perf_counter()
     specs = c4_scene_list(count=a.ensemble_scenes)
@@ -307,16 +315,19 @@ def bench_ensemble(a):
             graph.replay()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
-    lib("mpmx_profile_begin", 4096)
+    lab = E.sim.materialLabel.numpy()
+    act = E.sim.activeLabel.numpy()
+    n_mpm = int(((act == 1) & (lab <= 1)).sum())  # MPM particles at the end of the timed region
     with torch.cuda.stream(stream):
+        lib("mpmx_profile_hold", stream_handle(), PROFILE_HOLD_US)
+        for _ in range(per_graph):
+            E.substep()
+        lib("mpmx_profile_begin", 4096)
         for _ in range(per_graph):
             E.substep()
     torch.cuda.synchronize()
     lib("mpmx_profile_end")
     kernels = profile_report()
-    lab = E.sim.materialLabel.numpy()
-    act = E.sim.activeLabel.numpy()
-    n_mpm = int(((act == 1) & (lab <= 1)).sum())
     tile = kernels.get("substep_tile(stress+p2g)")
     achieved = TILE_BYTES_PER_PARTICLE * n_mpm / (tile["avg_us"] * 1e-6) / 1e9 if tile else None
     sub_bytes = 300.0 * n_mpm + 28.0 * G

@@ -162,12 +162,17 @@ int mpmx_version(void);
  * returns the number of values written (<= cap). */
 int mpmx_struct_layout(long* out, int cap);
 /* Launch profiler (no counterpart in the reference; replaces its wall-clock
- * prints, runMPMYDW.py:762): between begin and end, events are recorded on the
- * launch stream just before and just after every kernel the library issues
- * (not while the stream is being captured into a graph: those events cannot be
- * timed).  report() writes "name launches total_ms" lines. */
+ * prints, runMPMYDW.py:762): between begin and end, every kernel the library
+ * issues carries a start/stop event pair that its own dispatch timestamps
+ * (hipExtLaunchKernelGGL), so the times are the kernels' own (not while the
+ * stream is being captured into a graph: those launches are not timed).
+ * report() writes "name launches total_ms" lines. */
 int mpmx_profile_begin(int capacity);
 int mpmx_profile_mark(void* stream);
+/* Queue a one-thread kernel that keeps the stream busy for the given wall
+ * time (at most 1 s), so that the launches issued behind it reach the GPU
+ * back to back and their event brackets time the kernels alone. */
+int mpmx_profile_hold(void* stream, int microseconds);
 int mpmx_profile_end(void);
 int mpmx_profile_report(char* buf, size_t buflen);
 /* device synchronisation used by the host wrappers (mirrors .numpy() syncs) */

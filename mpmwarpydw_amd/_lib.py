@@ -42,6 +42,7 @@ SIGNATURES = {
     "mpmx_profile_begin": [I],
     "mpmx_profile_mark": [P],
     "mpmx_profile_end": [],
+    "mpmx_profile_hold": [P, I],
     "mpmx_profile_report": [P, S],
     "mpmx_substep_workspace_bytes": [I, I, I, I],
     "mpmx_substep_bin": [P, P, P, P, S, P],

@@ -2,6 +2,7 @@
 // launch checking and the workspace carving used by the entry points that need
 // scratch memory.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -13,7 +14,8 @@ namespace mpmx {
 void set_error(const char* fmt, ...);
 int check_launch(const char* what, hipStream_t s);
 // launch profiler hook: records a start event before a kernel when profiling is on
-void prof_start(hipStream_t s);
+// when the profiler is on, the events that time the next launch (true)
+bool prof_pair(hipStream_t s, hipEvent_t* start, hipEvent_t* stop);
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // launch-shape knobs (mpmx_set_tuning); read on the host when a launch is issued
 int tuning(int knob);
@@ -66,12 +68,17 @@ int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* 
 
 }  // namespace mpmx
 
-// every kernel launch of the library goes through this macro so that the
-// profiler can bracket it with events (check_launch records the end event)
-#define MPMX_LAUNCH(kernel, grid, block, shmem, stream, ...)                      \
-    do {                                                                          \
-        ::mpmx::prof_start(stream);                                               \
-        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ##__VA_ARGS__);   \
+// every kernel launch of the library goes through this macro: with the
+// profiler on, the launch carries a start and a stop event that the dispatch
+// itself timestamps (the kernel's own begin and end, as a profiler sees them);
+// check_launch then names the pair
+#define MPMX_LAUNCH(kernel, grid, block, shmem, stream, ...)                                               \
+    do {                                                                                                   \
+        hipEvent_t mpmx_ev0_, mpmx_ev1_;                                                                   \
+        if (::mpmx::prof_pair((stream), &mpmx_ev0_, &mpmx_ev1_))                                           \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, mpmx_ev0_, mpmx_ev1_, 0, ##__VA_ARGS__); \
+        else                                                                                               \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ##__VA_ARGS__);                         \
     } while (0)
 
 #define MPMX_REQUIRE(cond, ...)                 \

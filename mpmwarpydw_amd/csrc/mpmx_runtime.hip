@@ -37,25 +37,36 @@ static int debug_sync_enabled() {
 }
 
 // ---------------------------------------------------------------------------
-// launch profiler: when on, events are recorded on the launch stream just
-// before and just after every kernel (MPMX_LAUNCH + check_launch)
+// launch profiler: when on, every kernel launched through MPMX_LAUNCH carries
+// a start/stop event pair stamped by its own dispatch (hipExtLaunchKernelGGL);
+// check_launch names the pair.  Pairs sit at (2k, 2k + 1) in the pool.
 // ---------------------------------------------------------------------------
 struct Profiler {
     bool on = false;
     std::vector<hipEvent_t> pool;
-    std::vector<const char*> names;  // nullptr = a mark with no kernel before it
+    std::vector<const char*> names;  // name of the kernel whose stop event this is, else nullptr
     size_t used = 0;
+    long pending = -1;  // stop-event index of the last pair not yet named
 };
 static Profiler g_prof;
 
-void prof_start(hipStream_t s) {
-    if (!g_prof.on || g_prof.used >= g_prof.pool.size()) return;
-    if (hipEventRecord(g_prof.pool[g_prof.used], s) == hipSuccess) g_prof.names[g_prof.used++] = nullptr;
+bool prof_pair(hipStream_t s, hipEvent_t* start, hipEvent_t* stop) {
+    if (!g_prof.on || g_prof.used + 2 > g_prof.pool.size()) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    *start = g_prof.pool[g_prof.used];
+    *stop = g_prof.pool[g_prof.used + 1];
+    g_prof.names[g_prof.used] = nullptr;
+    g_prof.names[g_prof.used + 1] = nullptr;
+    g_prof.pending = (long)g_prof.used + 1;
+    g_prof.used += 2;
+    return true;
 }
 
-static void prof_record(const char* what, hipStream_t s) {
-    if (!g_prof.on || g_prof.used >= g_prof.pool.size()) return;
-    if (hipEventRecord(g_prof.pool[g_prof.used], s) == hipSuccess) g_prof.names[g_prof.used++] = what;
+static void prof_record(const char* what, hipStream_t) {
+    if (!g_prof.on || g_prof.pending < 0) return;
+    g_prof.names[g_prof.pending] = what;
+    g_prof.pending = -1;
 }
 
 int check_launch(const char* what, hipStream_t s) {
@@ -390,14 +401,40 @@ int mpmx_profile_begin(int capacity) {
         g_prof.names.resize(capacity);
     }
     g_prof.used = 0;
+    g_prof.pending = -1;
     g_prof.on = true;
     return 0;
 }
 
 int mpmx_profile_mark(void* stream) {
-    if (g_prof.on && g_prof.used < g_prof.pool.size()) {
+    // an unnamed pair: a point in the stream that no kernel's time is taken from
+    if (g_prof.on && g_prof.used + 2 <= g_prof.pool.size()) {
         MPMX_HIP(hipEventRecord(g_prof.pool[g_prof.used], as_stream(stream)));
-        g_prof.names[g_prof.used++] = nullptr;
+        MPMX_HIP(hipEventRecord(g_prof.pool[g_prof.used + 1], as_stream(stream)));
+        g_prof.names[g_prof.used] = g_prof.names[g_prof.used + 1] = nullptr;
+        g_prof.used += 2;
+    }
+    return 0;
+}
+
+// one thread waits until the wall clock has advanced by `ticks` (always ends)
+__global__ void hold_kernel(unsigned long long ticks) {
+    const unsigned long long t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(127);
+}
+
+int mpmx_profile_hold(void* stream, int microseconds) {
+    MPMX_REQUIRE(microseconds >= 0 && microseconds <= 1000000, "mpmx_profile_hold: 0..1e6 microseconds");
+    int dev = 0, khz = 0;
+    MPMX_HIP(hipGetDevice(&dev));
+    MPMX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    MPMX_REQUIRE(khz > 0, "mpmx_profile_hold: no wall clock rate");
+    const unsigned long long ticks = (unsigned long long)khz * (unsigned long long)microseconds / 1000ull;
+    hipLaunchKernelGGL(hold_kernel, dim3(1), dim3(1), 0, as_stream(stream), ticks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("launch of profile hold failed: %s", hipGetErrorString(e));
+        return (int)e;
     }
     return 0;
 }
@@ -410,7 +447,7 @@ int mpmx_profile_end(void) {
 int mpmx_profile_report(char* buf, size_t buflen) {
     MPMX_REQUIRE(buf && buflen > 0, "mpmx_profile_report: bad buffer");
     std::map<std::string, std::pair<long, double>> acc;
-    for (size_t i = 1; i < g_prof.used; i++) {
+    for (size_t i = 1; i < g_prof.used; i += 2) {
         if (!g_prof.names[i]) continue;
         float ms = 0.0f;
         hipError_t he = hipEventSynchronize(g_prof.pool[i]);
