@@ -123,11 +123,14 @@ typedef struct mpmx_step_params {
 const char* mpmx_last_error(void);
 
 /* Launch-shape knobs, process-wide (no reference counterpart), set by the
- * autotuner: MPMX_TUNE_XCD_TILES = 1 deals the tiles of the tile kernels to
+ * autotuner.  MPMX_TUNE_XCD_TILES = 1 deals the tiles of the tile kernels to
  * the 8 XCDs in contiguous ranges (neighbouring tiles share an L2), 0 in
- * block order.  Results do not depend on any knob. */
+ * block order.  MPMX_TUNE_SPLIT_STRESS = 1 runs the constitutive update of the
+ * fused substep as its own particle-parallel kernel before a P2G-only tile
+ * kernel, 0 inside the tile kernel.  Results do not depend on any knob. */
 #define MPMX_TUNE_XCD_TILES 0
-#define MPMX_TUNE_COUNT 1
+#define MPMX_TUNE_SPLIT_STRESS 1
+#define MPMX_TUNE_COUNT 2
 int mpmx_set_tuning(int knob, int value);
 int mpmx_get_tuning(int knob);
 

@@ -3,6 +3,9 @@
 Knobs (none changes results, only speed):
   xcd_tiles   deal tiles to the chip's 8 XCDs in contiguous ranges (1) or in
               block order (0)                      (mpmx_set_tuning)
+  split_stress run the constitutive update as its own particle kernel before
+              a P2G-only tile kernel (1) or inside the tile kernel (0)
+                                                   (mpmx_set_tuning)
   reorder     keep the particle arrays in tile order in HBM (the sort-bin
               layout of the tile kernels) or in input order  (Simulation.reorder,
               Ensemble.reorder)
@@ -20,10 +23,15 @@ import torch
 from ._lib import lib
 
 MPMX_TUNE_XCD_TILES = 0
+MPMX_TUNE_SPLIT_STRESS = 1
 
 
 def set_xcd_tiles(on: bool) -> None:
     lib("mpmx_set_tuning", MPMX_TUNE_XCD_TILES, 1 if on else 0)
+
+
+def set_split_stress(on: bool) -> None:
+    lib("mpmx_set_tuning", MPMX_TUNE_SPLIT_STRESS, 1 if on else 0)
 
 
 def _time(step, steps: int, warmup: int, trials: int) -> float:
@@ -46,17 +54,21 @@ def _time(step, steps: int, warmup: int, trials: int) -> float:
 def tune(state, step, steps: int = 10, warmup: int = 3, trials: int = 2) -> dict:
     """Tune on ``state`` (a driver.Simulation or an ensemble.Ensemble with
     a ``reorder()`` method) advanced by ``step()``.  Leaves the state reordered
-    when that wins and the best XCD setting applied; returns the log."""
+    and the best knob settings applied; returns the log."""
     t0 = time.perf_counter()
     log = []
     for reorder in (False, True):
         if reorder:
             state.reorder()
-        for xcd in (1, 0):
-            set_xcd_tiles(bool(xcd))
-            ms = _time(step, steps, warmup, trials)
-            log.append({"reorder": reorder, "xcd_tiles": xcd, "ms_per_substep": round(ms, 4)})
+        for split in (1, 0):
+            set_split_stress(bool(split))
+            for xcd in (1, 0):
+                set_xcd_tiles(bool(xcd))
+                ms = _time(step, steps, warmup, trials)
+                log.append({"reorder": reorder, "split_stress": split, "xcd_tiles": xcd,
+                            "ms_per_substep": round(ms, 4)})
     best = min(log, key=lambda r: r["ms_per_substep"])  # min keeps the first of equal entries
     set_xcd_tiles(bool(best["xcd_tiles"]))
+    set_split_stress(bool(best["split_stress"]))
     # the state is reordered at this point; input order is kept only when it was faster
     return {"best": best, "candidates": log, "kept_reordered": True, "tuning_s": round(time.perf_counter() - t0, 2)}

@@ -16,7 +16,7 @@
 namespace mpmx {
 
 static thread_local char g_err[512] = "";
-static int g_tuning[MPMX_TUNE_COUNT] = {1};  // MPMX_TUNE_XCD_TILES on
+static int g_tuning[MPMX_TUNE_COUNT] = {1, 1};  // XCD tile dealing on, split constitutive update on
 
 int tuning(int knob) { return (knob >= 0 && knob < MPMX_TUNE_COUNT) ? g_tuning[knob] : 0; }
 
@@ -475,7 +475,7 @@ int mpmx_version(void) { return 1; }
 
 int mpmx_set_tuning(int knob, int value) {
     MPMX_REQUIRE(knob >= 0 && knob < MPMX_TUNE_COUNT, "mpmx_set_tuning: unknown knob %d", knob);
-    MPMX_REQUIRE(knob != MPMX_TUNE_XCD_TILES || value == 0 || value == 1, "mpmx_set_tuning: XCD tiles is 0 or 1");
+    MPMX_REQUIRE(value == 0 || value == 1, "mpmx_set_tuning: knob %d takes 0 or 1", knob);
     g_tuning[knob] = value;
     return 0;
 }

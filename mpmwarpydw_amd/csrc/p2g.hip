@@ -78,6 +78,8 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
     if ((rc = counting_sort_ids(w.keys, n, tg.ntiles() * kTileCells, w.sorted, w.bin, s))) return rc;
     TileArgs ta{};
     ta.P.n = n;
+    ta.P.activeLabel = const_cast<int32_t*>(activeLabel);
+    ta.P.materialLabel = const_cast<int32_t*>(materialLabel);  // the tile kernel tests the label
     ta.P.x = const_cast<float*>(x);
     ta.P.v = const_cast<float*>(v);
     ta.P.C = const_cast<float*>(C);

@@ -230,6 +230,8 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
             if (FUSED) {
                 mpm = constitutive_update(a.P, id, a.eff, T);
             } else {
+                // the constitutive update ran before this kernel and may have turned it XPBD
+                mpm = a.P.materialLabel[id] <= 1;
 #pragma unroll
                 for (int k = 0; k < 9; k++) T[k] = a.stress[(long)k * n + id];
             }
@@ -296,28 +298,40 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
                 bspline(rec[1][r], wy, dwy);
                 bspline(rec[2][r], wz, dwz);
                 const float m = rec[3][r];
-                // affine momentum A + Q (i, j, k), Q = m dx Cm
-                const float ax0 = rec[4][r] + rec[7][r] * fi, ay0 = rec[5][r] + rec[10][r] * fi,
-                            az0 = rec[6][r] + rec[13][r] * fi;
-                const float q1x = rec[8][r], q1y = rec[11][r], q1z = rec[14][r];
-                const float q2x = rec[9][r], q2y = rec[12][r], q2z = rec[15][r];
+                // momentum of node (i, j, k): w (A + Q (i, j, k)) with Q = m dx Cm, plus the
+                // stress impulse S grad w.  With a = wx wy_j, b = dwx wy_j, c = wx dwy_j the
+                // node gets wz_k (E + k F) + dwz_k G, E = a (A + Q (i, j, 0)) + S_0 b + S_1 c,
+                // F = a Q_2, G = a S_2 (S_d: column d of S), and the mass wz_k a m.
+                const float b0 = rec[4][r] + rec[7][r] * fi, b1 = rec[5][r] + rec[10][r] * fi,
+                            b2 = rec[6][r] + rec[13][r] * fi;
+                const float q10 = rec[8][r], q11 = rec[11][r], q12 = rec[14][r];
+                const float q20 = rec[9][r], q21 = rec[12][r], q22 = rec[15][r];
                 const float s00 = rec[16][r], s01 = rec[17][r], s02 = rec[18][r];
                 const float s10 = rec[19][r], s11 = rec[20][r], s12 = rec[21][r];
                 const float s20 = rec[22][r], s21 = rec[23][r], s22 = rec[24][r];
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    const float wxy = wx * wy[j], gxw = dwx * wy[j], gyw = wx * dwy[j];
-                    const float bx = ax0 + q1x * (float)j, by = ay0 + q1y * (float)j, bz = az0 + q1z * (float)j;
+                    const float a = wx * wy[j], b = dwx * wy[j], c = wx * dwy[j];
+                    const float M = a * m;
+                    const float c0 = j == 0 ? b0 : (j == 1 ? b0 + q10 : b0 + 2.0f * q10);
+                    const float c1 = j == 0 ? b1 : (j == 1 ? b1 + q11 : b1 + 2.0f * q11);
+                    const float c2 = j == 0 ? b2 : (j == 1 ? b2 + q12 : b2 + 2.0f * q12);
+                    const float E0 = a * c0 + s00 * b + s01 * c;
+                    const float E1 = a * c1 + s10 * b + s11 * c;
+                    const float E2 = a * c2 + s20 * b + s21 * c;
+                    const float F0 = a * q20, F1 = a * q21, F2 = a * q22;
+                    const float G0 = a * s02, G1 = a * s12, G2 = a * s22;
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        const float w = wxy * wz[k];
-                        const float g0 = gxw * wz[k], g1 = gyw * wz[k], g2 = wxy * dwz[k];
-                        const float ax = bx + q2x * (float)k, ay = by + q2y * (float)k, az = bz + q2z * (float)k;
+                        // E + k F, with the k = 0 and k = 1 products folded away
+                        const float t0 = k == 0 ? E0 : (k == 1 ? E0 + F0 : E0 + 2.0f * F0);
+                        const float t1 = k == 0 ? E1 : (k == 1 ? E1 + F1 : E1 + 2.0f * F1);
+                        const float t2 = k == 0 ? E2 : (k == 1 ? E2 + F2 : E2 + 2.0f * F2);
                         float* A = acc[j * 3 + k];
-                        A[0] += w * m;
-                        A[1] += w * ax + (s00 * g0 + s01 * g1 + s02 * g2);
-                        A[2] += w * ay + (s10 * g0 + s11 * g1 + s12 * g2);
-                        A[3] += w * az + (s20 * g0 + s21 * g1 + s22 * g2);
+                        A[0] += wz[k] * M;
+                        A[1] += wz[k] * t0 + dwz[k] * G0;
+                        A[2] += wz[k] * t1 + dwz[k] * G1;
+                        A[3] += wz[k] * t2 + dwz[k] * G2;
                     }
                 }
             }

@@ -331,6 +331,15 @@ __global__ void vlc_apply_substep(mpmx_particles P, Geo geo, float inv_dx, const
     C[8 * n + p] = c22 + corr;
 }
 
+// the constitutive update of every active MPM particle (compute_stress_from_F_trial,
+// mpmRoutines.py:229-312) when the substep runs it as its own kernel
+__global__ void __launch_bounds__(kBlock) substep_stress_kernel(mpmx_particles P, float eff) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n || P.materialLabel[p] > 1 || P.activeLabel[p] != 1) return;
+    float T[9];
+    constitutive_update(P, p, eff, T);
+}
+
 static int check_particles(const mpmx_particles* P, const mpmx_step_params* prm) {
     MPMX_REQUIRE(P && prm, "substep: null argument");
     MPMX_REQUIRE(P->n >= 0, "substep: bad sizes");
@@ -440,8 +449,18 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ta.geo = geo;
     ta.xcd = tuning(MPMX_TUNE_XCD_TILES);
     ta.misbinned = w.misbinned;
-    MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
-    if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
+    if (tuning(MPMX_TUNE_SPLIT_STRESS)) {
+        if (P->n > 0) {
+            MPMX_LAUNCH(substep_stress_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->eff);
+            if ((rc = check_launch("substep_stress", s))) return rc;
+        }
+        ta.stress = P->stress;
+        MPMX_LAUNCH(p2g_tile_kernel<false>, dim3(ntile_blocks - 1), dim3(kBlock), 0, s, ta);  // no orphan block
+        if ((rc = check_launch("substep_tile(p2g)", s))) return rc;
+    } else {
+        MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
+        if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
+    }
 
     CombineArgs ca{};
     ca.geo = geo;
