@@ -222,40 +222,14 @@ __global__ void __launch_bounds__(kBlock) g2p_kernel(
     int b1 = base_axis(xp.y, miny, inv_dx, fx[1]);
     int b2 = base_axis(xp.z, minz, inv_dx, fx[2]);
     if (b0 < 0 || b1 < 0 || b2 < 0 || b0 > gx - 3 || b1 > gy - 3 || b2 > gz - 3) return;
-    float w[3][3], dw[3][3];
-    bspline(fx[0], w[0], dw[0]);
-    bspline(fx[1], w[1], dw[1]);
-    bspline(fx[2], w[2], dw[2]);
-    long G = (long)gx * gy * gz;
-    float nv[3] = {0.0f, 0.0f, 0.0f};
-    float nB[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float nG[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            float wxy = w[0][i] * w[1][j];
-            float gx_ = dw[0][i] * w[1][j] * inv_dx;
-            float gy_ = w[0][i] * dw[1][j] * inv_dx;
-            long row = ((long)(b0 + i) * gy + (b1 + j)) * gz + b2;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                long idx = row + k;
-                float vx = gv[idx], vy = gv[G + idx], vz = gv[2 * G + idx];
-                float wt = wxy * w[2][k];
-                float d0 = (float)i - fx[0], d1 = (float)j - fx[1], d2 = (float)k - fx[2];
-                float g0 = gx_ * w[2][k], g1 = gy_ * w[2][k], g2 = wxy * dw[2][k] * inv_dx;
-                nv[0] += vx * wt; nv[1] += vy * wt; nv[2] += vz * wt;
-                float wvx = vx * wt, wvy = vy * wt, wvz = vz * wt;
-                nB[0] += wvx * d0; nB[1] += wvx * d1; nB[2] += wvx * d2;
-                nB[3] += wvy * d0; nB[4] += wvy * d1; nB[5] += wvy * d2;
-                nB[6] += wvz * d0; nB[7] += wvz * d1; nB[8] += wvz * d2;
-                nG[0] += vx * g0; nG[1] += vx * g1; nG[2] += vx * g2;
-                nG[3] += vy * g0; nG[4] += vy * g1; nG[5] += vy * g2;
-                nG[6] += vz * g0; nG[7] += vz * g1; nG[8] += vz * g2;
-            }
-        }
-    }
+    const long G = (long)gx * gy * gz;
+    float nv[3], nB[9], nG[9];
+    g2p_gather(fx, inv_dx,
+               [&](int i, int j, int k) {
+                   const long idx = ((long)(b0 + i) * gy + (b1 + j)) * gz + b2 + k;
+                   return f3{gv[idx], gv[G + idx], gv[2 * G + idx]};
+               },
+               nv, nB, nG);
     st3(v, n, p, f3{nv[0], nv[1], nv[2]});
     st3(x, n, p, f3{xp.x + dt * nv[0], xp.y + dt * nv[1], xp.z + dt * nv[2]});
     float sC = inv_dx * 4.0f;

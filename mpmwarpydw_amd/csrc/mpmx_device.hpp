@@ -205,6 +205,75 @@ MPMX_HD int base_axis(float x, float minb, float inv_dx, float& fx) {
 }
 
 // Coulomb friction on one tangential component (mpmRoutines.py:915-940).
+// G2P sums of one particle (mpmRoutines.py:730-776) over its 27 stencil nodes:
+//   nv = sum w v,  nB = sum w v (x) (node - fx),  nG = sum v (x) grad w,
+// with w = wx_i wy_j wz_k and grad w = (dwx_i wy_j wz_k, wx_i dwy_j wz_k,
+// wx_i wy_j dwz_k) inv_dx.  The weights are products along the axes, so the
+// sums are taken over k first, then j, then i: about half the arithmetic of
+// the 27 separate node terms.  fetch(i, j, k) returns the node velocity.
+template <class Fetch>
+MPMX_HD void g2p_gather(const float fx[3], float inv_dx, Fetch fetch, float nv[3], float nB[9], float nG[9]) {
+    float w[3][3], dw[3][3], wd[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        bspline(fx[a], w[a], dw[a]);
+#pragma unroll
+        for (int q = 0; q < 3; q++) wd[a][q] = w[a][q] * ((float)q - fx[a]);  // weight times node offset
+    }
+    float sv[3] = {0.0f, 0.0f, 0.0f}, sx[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
+    float sz[3] = {0.0f, 0.0f, 0.0f}, gx[3] = {0.0f, 0.0f, 0.0f}, gy[3] = {0.0f, 0.0f, 0.0f};
+    float gz[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        // sums over j and k at fixed i
+        float a_[3] = {0.0f, 0.0f, 0.0f}, ay[3] = {0.0f, 0.0f, 0.0f}, az[3] = {0.0f, 0.0f, 0.0f};
+        float ady[3] = {0.0f, 0.0f, 0.0f}, adz[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float b[3] = {0.0f, 0.0f, 0.0f}, bz[3] = {0.0f, 0.0f, 0.0f}, bdz[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const f3 v = fetch(i, j, k);
+                const float vv[3] = {v.x, v.y, v.z};
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    b[r] += w[2][k] * vv[r];
+                    bz[r] += wd[2][k] * vv[r];
+                    bdz[r] += dw[2][k] * vv[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                a_[r] += w[1][j] * b[r];
+                ay[r] += wd[1][j] * b[r];
+                az[r] += w[1][j] * bz[r];
+                ady[r] += dw[1][j] * b[r];
+                adz[r] += w[1][j] * bdz[r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            sv[r] += w[0][i] * a_[r];
+            sx[r] += wd[0][i] * a_[r];
+            sy[r] += w[0][i] * ay[r];
+            sz[r] += w[0][i] * az[r];
+            gx[r] += dw[0][i] * a_[r];
+            gy[r] += w[0][i] * ady[r];
+            gz[r] += w[0][i] * adz[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        nv[r] = sv[r];
+        nB[r * 3 + 0] = sx[r];
+        nB[r * 3 + 1] = sy[r];
+        nB[r * 3 + 2] = sz[r];
+        nG[r * 3 + 0] = gx[r] * inv_dx;
+        nG[r * 3 + 1] = gy[r] * inv_dx;
+        nG[r * 3 + 2] = gz[r] * inv_dx;
+    }
+}
+
 MPMX_HD float coulomb(float v, float mu, float dt, float g) {
     if (fabsf(v) < 1.0e-12f) return 0.0f;
     float dv = mu * g * dt;

@@ -186,8 +186,12 @@ constexpr int kRecFloats = kRec * kRecCols;
 constexpr int kPartFloats = kTileCells * 27 * 4;
 constexpr int kTileLdsFloats = kRecFloats > kPartFloats ? kRecFloats : kPartFloats;
 
-template <bool FUSED>
-__global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
+// FUSED: the constitutive update runs here (stress from registers); else the
+// stress is read.  SYM: the stress is symmetric by construction (the substep's
+// own constitutive update writes T[i][j] = T[j][i]), so 6 components are read
+// and staged; the standalone P2G reads all 9 of whatever it is given.
+template <bool FUSED, bool SYM>
+__device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     __shared__ float4 smem4[kTileLdsFloats / 4];
     __shared__ int cs[kTileCells + 1];
     float (*rec)[kRecCols] = reinterpret_cast<float (*)[kRecCols]>(smem4);
@@ -229,8 +233,19 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
             bool mpm = true;
             if (FUSED) {
                 mpm = constitutive_update(a.P, id, a.eff, T);
-            } else {
+            } else if (SYM) {
                 // the constitutive update ran before this kernel and may have turned it XPBD
+                mpm = a.P.materialLabel[id] <= 1;
+                T[0] = a.stress[id];
+                T[1] = a.stress[1L * n + id];
+                T[2] = a.stress[2L * n + id];
+                T[4] = a.stress[4L * n + id];
+                T[5] = a.stress[5L * n + id];
+                T[8] = a.stress[8L * n + id];
+                T[3] = T[1];
+                T[6] = T[2];
+                T[7] = T[5];
+            } else {
                 mpm = a.P.materialLabel[id] <= 1;
 #pragma unroll
                 for (int k = 0; k < 9; k++) T[k] = a.stress[(long)k * n + id];
@@ -282,8 +297,17 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
             rec[6][r] = m * vp.z - md * (Cm[6] * f0 + Cm[7] * f1 + Cm[8] * f2);
 #pragma unroll
             for (int k = 0; k < 9; k++) rec[7 + k][r] = md * Cm[k];
+            if (SYM || FUSED) {  // symmetric: xx yy zz xy xz yz
+                rec[16][r] = sv * T[0];
+                rec[17][r] = sv * T[4];
+                rec[18][r] = sv * T[8];
+                rec[19][r] = sv * T[1];
+                rec[20][r] = sv * T[2];
+                rec[21][r] = sv * T[5];
+            } else {
 #pragma unroll
-            for (int k = 0; k < 9; k++) rec[16 + k][r] = sv * T[k];
+                for (int k = 0; k < 9; k++) rec[16 + k][r] = sv * T[k];
+            }
         }
         __syncthreads();
         if (has_task) {
@@ -306,9 +330,19 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
                             b2 = rec[6][r] + rec[13][r] * fi;
                 const float q10 = rec[8][r], q11 = rec[11][r], q12 = rec[14][r];
                 const float q20 = rec[9][r], q21 = rec[12][r], q22 = rec[15][r];
-                const float s00 = rec[16][r], s01 = rec[17][r], s02 = rec[18][r];
-                const float s10 = rec[19][r], s11 = rec[20][r], s12 = rec[21][r];
-                const float s20 = rec[22][r], s21 = rec[23][r], s22 = rec[24][r];
+                float s00, s01, s02, s10, s11, s12, s20, s21, s22;
+                if (SYM || FUSED) {
+                    s00 = rec[16][r];
+                    s11 = rec[17][r];
+                    s22 = rec[18][r];
+                    s01 = s10 = rec[19][r];
+                    s02 = s20 = rec[20][r];
+                    s12 = s21 = rec[21][r];
+                } else {
+                    s00 = rec[16][r], s01 = rec[17][r], s02 = rec[18][r];
+                    s10 = rec[19][r], s11 = rec[20][r], s12 = rec[21][r];
+                    s20 = rec[22][r], s21 = rec[23][r], s22 = rec[24][r];
+                }
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
                     const float a = wx * wy[j], b = dwx * wy[j], c = wx * dwy[j];
@@ -373,6 +407,18 @@ __global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
         out[3 * kTileNodes + node] = s.w;
     }
     if (tid == 0) a.tile_flag[t] = 1;
+}
+
+// P2G only (the stress is read); SYM as for p2g_tile_body
+template <bool SYM>
+__global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
+    p2g_tile_body<false, SYM>(a);
+}
+
+// the constitutive update of every particle inside the tile pass
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(kBlock) p2g_tile_fused_kernel(TileArgs a) {
+    p2g_tile_body<true, true>(a);
 }
 
 // ---- combine ----------------------------------------------------------------

@@ -140,42 +140,14 @@ __device__ __forceinline__ int g2p_one(const G2PArgs& a, const SceneDesc& sd, co
     const long n = P.n;
     const long G = a.geo.nnodes;
     const float inv_dx = a.inv_dx, dt = a.dt;
-    float w[3][3], dw[3][3];
-    bspline(fx[0], w[0], dw[0]);
-    bspline(fx[1], w[1], dw[1]);
-    bspline(fx[2], w[2], dw[2]);
-    float nv[3] = {0.0f, 0.0f, 0.0f};
-    float nB[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    float nG[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const float wxy = w[0][i] * w[1][j];
-            const float gx_ = dw[0][i] * w[1][j] * inv_dx;
-            const float gy_ = w[0][i] * dw[1][j] * inv_dx;
-            const int row = ((b0 + i) * kTN + (b1 + j)) * kTN + b2;
-            const long grow = sd.node0 + ((long)(b0 + ox + i) * sd.gy + (b1 + oy + j)) * sd.gz + (b2 + oz);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const f3 gv = g2p_node<LDS>(a, gvs, row + k, grow + k, G);
-                const float vx = gv.x, vy = gv.y, vz = gv.z;
-                const float wt = wxy * w[2][k];
-                const float d0 = (float)i - fx[0], d1 = (float)j - fx[1], d2 = (float)k - fx[2];
-                const float g0 = gx_ * w[2][k], g1 = gy_ * w[2][k], g2 = wxy * dw[2][k] * inv_dx;
-                nv[0] += vx * wt;
-                nv[1] += vy * wt;
-                nv[2] += vz * wt;
-                const float wvx = vx * wt, wvy = vy * wt, wvz = vz * wt;
-                nB[0] += wvx * d0; nB[1] += wvx * d1; nB[2] += wvx * d2;
-                nB[3] += wvy * d0; nB[4] += wvy * d1; nB[5] += wvy * d2;
-                nB[6] += wvz * d0; nB[7] += wvz * d1; nB[8] += wvz * d2;
-                nG[0] += vx * g0; nG[1] += vx * g1; nG[2] += vx * g2;
-                nG[3] += vy * g0; nG[4] += vy * g1; nG[5] += vy * g2;
-                nG[6] += vz * g0; nG[7] += vz * g1; nG[8] += vz * g2;
-            }
-        }
-    }
+    float nv[3], nB[9], nG[9];
+    g2p_gather(fx, inv_dx,
+               [&](int i, int j, int k) {
+                   const int row = ((b0 + i) * kTN + (b1 + j)) * kTN + b2 + k;
+                   const long grow = sd.node0 + ((long)(b0 + ox + i) * sd.gy + (b1 + oy + j)) * sd.gz + (b2 + oz + k);
+                   return g2p_node<LDS>(a, gvs, row, grow, G);
+               },
+               nv, nB, nG);
     const f3 xn = f3{xp.x + dt * nv[0], xp.y + dt * nv[1], xp.z + dt * nv[2]};
     st3(P.v, n, id, f3{nv[0], nv[1], nv[2]});
     st3(P.x, n, id, xn);
@@ -455,10 +427,10 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
             if ((rc = check_launch("substep_stress", s))) return rc;
         }
         ta.stress = P->stress;
-        MPMX_LAUNCH(p2g_tile_kernel<false>, dim3(ntile_blocks - 1), dim3(kBlock), 0, s, ta);  // no orphan block
+        MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks - 1), dim3(kBlock), 0, s, ta);  // no orphan block
         if ((rc = check_launch("substep_tile(p2g)", s))) return rc;
     } else {
-        MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
+        MPMX_LAUNCH(p2g_tile_fused_kernel<>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);
         if ((rc = check_launch("substep_tile(stress+p2g)", s))) return rc;
     }
 

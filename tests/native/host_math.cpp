@@ -32,4 +32,20 @@ void host_sym_exp(long n, const float* S, float* X) {
     for (long p = 0; p < n; p++) mpmx::sym_exp(S + 6 * p, mpmx::sym_norm(S + 6 * p), X + 6 * p);
 }
 
+// the G2P sums of particles with fractional offsets fx [n][3] over node
+// velocities v [n][27][3] (node (i, j, k) at 9 i + 3 j + k): nv [n][3],
+// nB [n][9], nG [n][9]
+void host_g2p_gather(long n, const float* fx, const float* v, float inv_dx, float* nv, float* nB, float* nG) {
+    for (long p = 0; p < n; p++) {
+        const float* vp = v + 81 * p;
+        mpmx::g2p_gather(
+            fx + 3 * p, inv_dx,
+            [&](int i, int j, int k) {
+                const float* q = vp + 3 * (9 * i + 3 * j + k);
+                return mpmx::f3{q[0], q[1], q[2]};
+            },
+            nv + 3 * p, nB + 9 * p, nG + 9 * p);
+    }
+}
+
 }  // extern "C"

@@ -174,3 +174,38 @@ def test_constitutive_update_matches_oracle(hm, kind):
     assert np.max(np.abs(h["ys"] - o["ys"]) / np.maximum(np.abs(o["ys"]), 1.0)) < 1e-5
     # the phase transition adds sqrt(2 eff u / rho) along v: relative FP32 precision
     assert np.max(np.abs(h["v"] - o["v"]) / np.maximum(np.abs(o["v"]), 1.0)) < 1e-5
+
+
+def test_g2p_gather_matches_the_27_node_sums(hm):
+    """The axis-factored G2P sums (mpmx_device.hpp g2p_gather) equal the
+    reference's 27 separate node terms (mpmRoutines.py:738-770) evaluated in FP64."""
+    rng = np.random.default_rng(9)
+    n = 2000
+    fx = rng.uniform(0.5, 1.5, (n, 3)).astype(np.float32)
+    v = rng.standard_normal((n, 27, 3)).astype(np.float32)
+    v[: n // 2] += 20.0  # a large uniform part on half of them
+    inv_dx = 2.0
+    nv = np.zeros((n, 3), np.float32)
+    nB = np.zeros((n, 9), np.float32)
+    nG = np.zeros((n, 9), np.float32)
+    hm.host_g2p_gather(C.c_long(n), _p(fx), _p(v), C.c_float(inv_dx), _p(nv), _p(nB), _p(nG))
+    f = fx.astype(np.float64)
+    w = np.stack([0.5 * (1.5 - f) ** 2, 0.75 - (f - 1.0) ** 2, 0.5 * (f - 0.5) ** 2], axis=2)  # [n][axis][q]
+    dw = np.stack([f - 1.5, -2.0 * (f - 1.0), f - 0.5], axis=2)
+    ev = np.zeros((n, 3)); eB = np.zeros((n, 3, 3)); eG = np.zeros((n, 3, 3))
+    vd = v.astype(np.float64)
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                wt = w[:, 0, i] * w[:, 1, j] * w[:, 2, k]
+                g = np.stack([dw[:, 0, i] * w[:, 1, j] * w[:, 2, k], w[:, 0, i] * dw[:, 1, j] * w[:, 2, k],
+                              w[:, 0, i] * w[:, 1, j] * dw[:, 2, k]], axis=1) * inv_dx
+                d = np.stack([i - f[:, 0], j - f[:, 1], k - f[:, 2]], axis=1)
+                vn = vd[:, 9 * i + 3 * j + k]
+                ev += wt[:, None] * vn
+                eB += wt[:, None, None] * vn[:, :, None] * d[:, None, :]
+                eG += vn[:, :, None] * g[:, None, :]
+    # the uniform part cancels in nB; 27 separate FP32 terms in the reference's
+    # order miss the FP64 value by 1.8e-6 of the array's scale on these inputs
+    for got, want in ((nv, ev), (nB, eB.reshape(n, 9)), (nG, eG.reshape(n, 9))):
+        assert np.abs(got - want).max() <= 4e-6 * np.abs(want).max()
