@@ -44,6 +44,8 @@ struct Carver {
 
 // Exclusive scan of n int32 counts into out[0..n] (out[n] = total), on stream s.
 // partials must hold scan_partials_count(n) ints.
+// exclusive scans cut their input into chunks of kScanChunk values
+constexpr int kScanChunk = 2048;
 size_t scan_partials_count(long n);
 int exclusive_scan_i32(const int32_t* counts, int32_t* out, long n, int32_t* partials, hipStream_t s);
 
@@ -63,6 +65,10 @@ bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w);
 int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
                       const int32_t* order = nullptr);
 // scan + scatter + in-run sort only (w.count and w.rank already filled)
+// as counting_sort_from_counts, for producers that also accumulated the total
+// of every chunk of kScanChunk keys into w.partials (one scan pass fewer)
+int counting_sort_from_chunk_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w,
+                                    hipStream_t s, const int32_t* order = nullptr);
 int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
                               const int32_t* order = nullptr);
 

@@ -114,6 +114,7 @@ static unsigned fill_grid(long n) {
 // ---------------------------------------------------------------------------
 constexpr int kScanPer = 8;
 constexpr int kScanTile = kBlock * kScanPer;
+static_assert(kScanTile == kScanChunk, "scan chunk size");
 
 __device__ int block_exclusive_scan(int v, int* sh, int* total) {
     // Hillis-Steele inclusive scan over the 256 thread values in LDS
@@ -184,6 +185,42 @@ __global__ void scan_apply_kernel(const int32_t* __restrict__ in, long n, const 
         ex += vals[threadIdx.x * kScanPer + k];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partials[nb];
+}
+
+// scan_apply when partials[] holds the plain total of every chunk: each block
+// first sums the totals of the chunks before it (in a fixed order), and block 0
+// also writes the grand total to out[n]
+__global__ void scan_apply_chunks_kernel(const int32_t* __restrict__ in, long n, const int32_t* __restrict__ totals,
+                                         int32_t* __restrict__ out, long nb) {
+    __shared__ int sh[kBlock];
+    __shared__ int vals[kScanTile];
+    const long lim = blockIdx.x == 0 ? nb : (long)blockIdx.x;
+    int part = 0;
+    for (long c = threadIdx.x; c < lim; c += kBlock) part += totals[c];
+    sh[threadIdx.x] = part;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    const int before = blockIdx.x == 0 ? 0 : sh[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = sh[0];
+    __syncthreads();
+    const long base = (long)blockIdx.x * kScanTile;
+    for (int k = 0; k < kScanPer; k++) {
+        long i = base + (long)k * kBlock + threadIdx.x;
+        vals[k * kBlock + threadIdx.x] = (i < n) ? in[i] : 0;
+    }
+    __syncthreads();
+    int local = 0;
+    for (int k = 0; k < kScanPer; k++) local += vals[threadIdx.x * kScanPer + k];
+    int tot;
+    int ex = block_exclusive_scan(local, sh, &tot) + before;
+    for (int k = 0; k < kScanPer; k++) {
+        long i = base + (long)threadIdx.x * kScanPer + k;
+        if (i < n) out[i] = ex;
+        ex += vals[threadIdx.x * kScanPer + k];
+    }
 }
 
 size_t scan_partials_count(long n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
@@ -342,6 +379,28 @@ int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, 
     }
     MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
                        sorted, order);
+    return check_launch("bin_sort_runs", s);
+}
+
+int counting_sort_from_chunk_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w,
+                                    hipStream_t s, const int32_t* order) {
+    const long nb = (nkeys + kScanTile - 1) / kScanTile;
+    int rc;
+    if (nb == 0) {
+        MPMX_LAUNCH(fill_kernel<int32_t>, dim3(1), dim3(kBlock), 0, s, w.cell_start, 1L, 0);
+        if ((rc = check_launch("scan(empty)", s))) return rc;
+    } else {
+        MPMX_LAUNCH(scan_apply_chunks_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, w.count, nkeys, w.partials,
+                    w.cell_start, nb);
+        if ((rc = check_launch("scan_apply", s))) return rc;
+    }
+    if (n > 0) {
+        MPMX_LAUNCH(bin_scatter_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, keys, n, w.cell_start,
+                    w.rank, sorted);
+        if ((rc = check_launch("bin_scatter", s))) return rc;
+    }
+    MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
+                sorted, order);
     return check_launch("bin_sort_runs", s);
 }
 

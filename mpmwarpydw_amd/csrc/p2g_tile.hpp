@@ -434,6 +434,8 @@ struct CombineArgs {
     long n_zero;
     unsigned long long* zero_u64;  // optional: cell sums cleared before G2P
     long n_zero_u64;
+    int32_t* zero_chunks;          // optional: the next substep's per-chunk bin totals
+    long n_chunks;
 };
 
 // a template so that every translation unit including this header can instantiate it
@@ -446,6 +448,9 @@ __global__ void p2g_combine_kernel(CombineArgs a) {
     }
     if (a.zero_u64) {
         for (long z = i; z < a.n_zero_u64; z += (long)gridDim.x * blockDim.x) a.zero_u64[z] = 0ull;
+    }
+    if (a.zero_chunks) {
+        for (long z = i; z < a.n_chunks; z += (long)gridDim.x * blockDim.x) a.zero_chunks[z] = 0;
     }
     if (i >= G) return;
     const SceneDesc d = a.geo.at(a.geo.scene_of_node(i));

@@ -114,13 +114,17 @@ struct G2PArgs {
     int32_t* keys;   // next substep: key per particle
     int32_t* count;  // next substep: particles per key (cleared by K2)
     int32_t* rank;   // next substep: arrival rank inside the key
+    int32_t* chunks; // next substep: particles per chunk of kScanChunk keys (cleared by K2)
     int32_t* misbinned;
     int xcd;
 };
 
 __device__ __forceinline__ void bin_next(const G2PArgs& a, long id, int key) {
     a.keys[id] = key;
-    if (key >= 0) a.rank[id] = atomicAdd(&a.count[key], 1);
+    if (key >= 0) {
+        a.rank[id] = atomicAdd(&a.count[key], 1);
+        atomicAdd(&a.chunks[key / kScanChunk], 1);
+    }
 }
 
 // Grid velocity of node (b0+i, b1+j, b2+k) relative to the tile origin: from the
@@ -256,16 +260,22 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
                 }
             }
             a.keys[id] = key;
-            if (key >= key0 && key < key0 + kTileCells)
+            if (key >= key0 && key < key0 + kTileCells) {
                 lrank = atomicAdd(&lcnt[key - key0], 1);
-            else if (key >= 0)
+            } else if (key >= 0) {
                 a.rank[id] = atomicAdd(&a.count[key], 1);
+                atomicAdd(&a.chunks[key / kScanChunk], 1);
+            }
         }
         __syncthreads();
-        if (threadIdx.x < kTileCells) {
+        if (threadIdx.x < kTileCells) {  // one wavefront: the tile's cells
             const int c = lcnt[threadIdx.x];
             lbase[threadIdx.x] = c ? atomicAdd(&a.count[key0 + threadIdx.x], c) : 0;
             lcnt[threadIdx.x] = 0;
+            int tot = c;  // the tile's keys all lie in one chunk (kScanChunk is a multiple of 64)
+#pragma unroll
+            for (int off = kTileCells / 2; off > 0; off >>= 1) tot += __shfl_xor(tot, off, kTileCells);
+            if (threadIdx.x == 0 && tot) atomicAdd(&a.chunks[key0 / kScanChunk], tot);
         }
         __syncthreads();
         if (lrank >= 0) a.rank[id] = lbase[key - key0] + lrank;
@@ -454,6 +464,8 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ca.restitution = prm->restitution;
     ca.zero_ints = w.bin.count;
     ca.n_zero = nkeys;
+    ca.zero_chunks = w.bin.partials;
+    ca.n_chunks = (nkeys + kScanChunk - 1) / kScanChunk;
     if (prm->vlc) {
         ca.zero_u64 = w.vlc;
         ca.n_zero_u64 = 2 * geo.nnodes;
@@ -474,6 +486,7 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ga.keys = w.keys;
     ga.count = w.bin.count;
     ga.rank = w.bin.rank;
+    ga.chunks = w.bin.partials;
     ga.misbinned = w.misbinned;
     ga.xcd = tuning(MPMX_TUNE_XCD_TILES);
     MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
@@ -487,7 +500,7 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
         if ((rc = check_launch("substep_vlc_apply", s))) return rc;
     }
 
-    return counting_sort_from_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s, P->order);
+    return counting_sort_from_chunk_counts(w.keys, P->n, nkeys, w.sorted, w.bin, s, P->order);
 }
 
 }  // namespace mpmx
