@@ -127,10 +127,13 @@ const char* mpmx_last_error(void);
  * the 8 XCDs in contiguous ranges (neighbouring tiles share an L2), 0 in
  * block order.  MPMX_TUNE_SPLIT_STRESS = 1 runs the constitutive update of the
  * fused substep as its own particle-parallel kernel before a P2G-only tile
- * kernel, 0 inside the tile kernel.  Results do not depend on any knob. */
+ * kernel, 0 inside the tile kernel.  MPMX_TUNE_G2P_TILES = 1 runs G2P per tile
+ * with the tile's grid velocities staged in LDS, 0 one thread per particle with
+ * the grid read through the caches.  Results do not depend on any knob. */
 #define MPMX_TUNE_XCD_TILES 0
 #define MPMX_TUNE_SPLIT_STRESS 1
-#define MPMX_TUNE_COUNT 2
+#define MPMX_TUNE_G2P_TILES 2
+#define MPMX_TUNE_COUNT 3
 int mpmx_set_tuning(int knob, int value);
 int mpmx_get_tuning(int knob);
 

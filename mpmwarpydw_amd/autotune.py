@@ -6,6 +6,9 @@ Knobs (none changes results, only speed):
   split_stress run the constitutive update as its own particle kernel before
               a P2G-only tile kernel (1) or inside the tile kernel (0)
                                                    (mpmx_set_tuning)
+  g2p_tiles   G2P per tile with the grid velocities in LDS (1) or one thread
+              per particle reading the grid through the caches (0)
+                                                   (mpmx_set_tuning)
   reorder     keep the particle arrays in tile order in HBM (the sort-bin
               layout of the tile kernels) or in input order  (Simulation.reorder,
               Ensemble.reorder)
@@ -24,6 +27,7 @@ from ._lib import lib
 
 MPMX_TUNE_XCD_TILES = 0
 MPMX_TUNE_SPLIT_STRESS = 1
+MPMX_TUNE_G2P_TILES = 2
 
 
 def set_xcd_tiles(on: bool) -> None:
@@ -32,6 +36,10 @@ def set_xcd_tiles(on: bool) -> None:
 
 def set_split_stress(on: bool) -> None:
     lib("mpmx_set_tuning", MPMX_TUNE_SPLIT_STRESS, 1 if on else 0)
+
+
+def set_g2p_tiles(on: bool) -> None:
+    lib("mpmx_set_tuning", MPMX_TUNE_G2P_TILES, 1 if on else 0)
 
 
 def _time(step, steps: int, warmup: int, trials: int) -> float:
@@ -62,13 +70,16 @@ def tune(state, step, steps: int = 10, warmup: int = 3, trials: int = 2) -> dict
             state.reorder()
         for split in (1, 0):
             set_split_stress(bool(split))
-            for xcd in (1, 0):
-                set_xcd_tiles(bool(xcd))
-                ms = _time(step, steps, warmup, trials)
-                log.append({"reorder": reorder, "split_stress": split, "xcd_tiles": xcd,
-                            "ms_per_substep": round(ms, 4)})
+            for g2p in (1, 0):
+                set_g2p_tiles(bool(g2p))
+                for xcd in (1, 0):
+                    set_xcd_tiles(bool(xcd))
+                    ms = _time(step, steps, warmup, trials)
+                    log.append({"reorder": reorder, "split_stress": split, "g2p_tiles": g2p, "xcd_tiles": xcd,
+                                "ms_per_substep": round(ms, 4)})
     best = min(log, key=lambda r: r["ms_per_substep"])  # min keeps the first of equal entries
     set_xcd_tiles(bool(best["xcd_tiles"]))
     set_split_stress(bool(best["split_stress"]))
+    set_g2p_tiles(bool(best["g2p_tiles"]))
     # the state is reordered at this point; input order is kept only when it was faster
     return {"best": best, "candidates": log, "kept_reordered": True, "tuning_s": round(time.perf_counter() - t0, 2)}

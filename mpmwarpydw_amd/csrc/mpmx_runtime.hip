@@ -16,7 +16,8 @@
 namespace mpmx {
 
 static thread_local char g_err[512] = "";
-static int g_tuning[MPMX_TUNE_COUNT] = {1, 1};  // XCD tile dealing on, split constitutive update on
+// XCD tile dealing on, split constitutive update on, G2P per tile
+static int g_tuning[MPMX_TUNE_COUNT] = {1, 1, 1};
 
 int tuning(int knob) { return (knob >= 0 && knob < MPMX_TUNE_COUNT) ? g_tuning[knob] : 0; }
 

@@ -56,6 +56,7 @@ static int p2g_run(int n, const int32_t* activeLabel, const int32_t* materialLab
                    float gzv, float dt, float damping, int bc, float fric, float gmag, int pad, float restitution,
                    void* ws, size_t ws_bytes, int* host_oob, hipStream_t s) {
     MPMX_REQUIRE(n >= 0 && gx >= 3 && gy >= 3 && gz >= 3, "p2g: grid must have at least 3 nodes per axis");
+    MPMX_REQUIRE((long)gx * gy * gz < (1L << 31), "p2g: a grid needs fewer than 2^31 nodes");
     MPMX_REQUIRE(grid_m && grid_v_in && (!do_update || grid_v_out), "p2g: null grid pointer");
     MPMX_REQUIRE(n == 0 || (activeLabel && materialLabel && stress && x && v && C && vol && mass),
                  "p2g: null particle pointer");

@@ -381,19 +381,14 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     for (int node = tid; node < kTileNodes; node += kBlock) {
         const int nx = node / (kTN * kTN), ny = (node / kTN) % kTN, nz = node % kTN;
         float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int cx = nx - i;
-            if (cx < 0 || cx >= kTB) continue;
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int cy = ny - j;
-                if (cy < 0 || cy >= kTB) continue;
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const int cz = nz - k;
-                    if (cz < 0 || cz >= kTB) continue;
-                    const float4 q = part[((cx * kTB + cy) * kTB + cz) * 27 + i * 9 + j * 3 + k];
+        // offsets (i, j, k) whose cell (n - offset) lies in the tile, i ascending, then j, then k
+        const int i0 = max(0, nx - (kTB - 1)), i1 = min(2, nx);
+        const int j0 = max(0, ny - (kTB - 1)), j1 = min(2, ny);
+        const int k0 = max(0, nz - (kTB - 1)), k1 = min(2, nz);
+        for (int i = i0; i <= i1; i++) {
+            for (int j = j0; j <= j1; j++) {
+                for (int k = k0; k <= k1; k++) {
+                    const float4 q = part[(((nx - i) * kTB + (ny - j)) * kTB + (nz - k)) * 27 + i * 9 + j * 3 + k];
                     s.x += q.x;
                     s.y += q.y;
                     s.z += q.z;
@@ -454,10 +449,10 @@ __global__ void p2g_combine_kernel(CombineArgs a) {
     }
     if (i >= G) return;
     const SceneDesc d = a.geo.at(a.geo.scene_of_node(i));
-    const long li = i - d.node0;
-    const int iz = (int)(li % d.gz);
-    const int iy = (int)((li / d.gz) % d.gy);
-    const int ix = (int)(li / ((long)d.gy * d.gz));
+    const int li = (int)(i - d.node0);  // a scene has fewer than 2^31 nodes: 32-bit division
+    const int iz = li % d.gz;
+    const int iy = (li / d.gz) % d.gy;
+    const int ix = li / (d.gy * d.gz);
     float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
     const int tx1 = ix / kTB, ty1 = iy / kTB, tz1 = iz / kTB;
     for (int tx = tx1 - 1; tx <= tx1; tx++) {

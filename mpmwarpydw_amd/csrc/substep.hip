@@ -313,6 +313,64 @@ __global__ void vlc_apply_substep(mpmx_particles P, Geo geo, float inv_dx, const
     C[8 * n + p] = c22 + corr;
 }
 
+// G2P over the sorted particle list, one thread per particle, the grid
+// velocities read through the caches.  The list is in bin order and most
+// particles keep their cell, so consecutive lanes mostly share their next key:
+// each run of equal keys takes one atomic for its ranks, and each run of
+// equal key chunks one atomic for the chunk totals.
+__global__ void __launch_bounds__(kBlock) g2p_particles_kernel(G2PArgs a) {
+    const mpmx_particles& P = a.P;
+    const long n = P.n;
+    const int orphan_key = (int)(a.geo.ntiles() * kTileCells);
+    const long total = a.cell_start[orphan_key + 1];  // particles in the bins (tiles and orphans)
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    int id = -1;
+    if (s < total) {
+        id = a.sorted[s];
+        if (P.materialLabel[id] <= 1) {  // else it became XPBD in this substep's constitutive update
+            const SceneDesc sd = a.geo.at(scene_of(P, id));
+            const f3 xp = ld3(P.x, n, id);
+            float fx[3];
+            const int b0 = base_axis(xp.x, sd.minx, a.inv_dx, fx[0]);
+            const int b1 = base_axis(xp.y, sd.miny, a.inv_dx, fx[1]);
+            const int b2 = base_axis(xp.z, sd.minz, a.inv_dx, fx[2]);
+            const bool out = b0 < 0 || b1 < 0 || b2 < 0 || b0 > sd.gx - 3 || b1 > sd.gy - 3 || b2 > sd.gz - 3;
+            if (a.misbinned) {
+                // the bins come from the key written last time: is the particle still there?
+                const int now = out ? orphan_key : tile_key(xp, a.inv_dx, sd);
+                if (now != a.keys[id]) atomicAdd(a.misbinned, 1);
+            }
+            key = out ? orphan_key : g2p_one<false>(a, sd, nullptr, id, xp, b0, b1, b2, 0, 0, 0, fx);
+        }
+        a.keys[id] = key;
+    }
+    // every lane stays for the cross-lane operations
+    const int lane = (int)(threadIdx.x & 63);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    {
+        const int prev = __shfl_up(key, 1);
+        const bool head = lane == 0 || prev != key;
+        const unsigned long long heads = __ballot(head);
+        const int my_head = 63 - __clzll((long long)(heads & upto));
+        const unsigned long long above = heads & ~upto;
+        const int next_head = above ? __ffsll((long long)above) - 1 : 64;
+        int base = 0;
+        if (head && key >= 0) base = atomicAdd(&a.count[key], next_head - lane);
+        base = __shfl(base, my_head);
+        if (key >= 0) a.rank[id] = base + (lane - my_head);
+    }
+    {
+        const int ch = key >= 0 ? key / kScanChunk : -1;
+        const int prev = __shfl_up(ch, 1);
+        const bool head = lane == 0 || prev != ch;
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long above = heads & ~upto;
+        const int next_head = above ? __ffsll((long long)above) - 1 : 64;
+        if (head && ch >= 0) atomicAdd(&a.chunks[ch], next_head - lane);
+    }
+}
+
 // the constitutive update of every active MPM particle (compute_stress_from_F_trial,
 // mpmRoutines.py:229-312) when the substep runs it as its own kernel
 __global__ void __launch_bounds__(kBlock) substep_stress_kernel(mpmx_particles P, float eff) {
@@ -339,6 +397,7 @@ static int check_particles(const mpmx_particles* P, const mpmx_step_params* prm)
 static int check_args(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm) {
     MPMX_REQUIRE(P && G && prm, "substep: null argument");
     MPMX_REQUIRE(G->gx >= 3 && G->gy >= 3 && G->gz >= 3, "substep: bad sizes");
+    MPMX_REQUIRE((long)G->gx * G->gy * G->gz < (1L << 31), "substep: a grid needs fewer than 2^31 nodes");
     MPMX_REQUIRE(G->m && G->v_in && G->v_out, "substep: null grid array");
     return check_particles(P, prm);
 }
@@ -351,6 +410,7 @@ static int ensemble_geo(const mpmx_scene* scenes, int nscenes, Geo& g) {
     for (int k = 0; k < nscenes; k++) {
         const mpmx_scene& c = scenes[k];
         MPMX_REQUIRE(c.gx >= 3 && c.gy >= 3 && c.gz >= 3, "ensemble: scene %d has fewer than 3 nodes on an axis", k);
+        MPMX_REQUIRE((long)c.gx * c.gy * c.gz < (1L << 31), "ensemble: scene %d has 2^31 nodes or more", k);
         const SceneDesc d = make_scene(c.gx, c.gy, c.gz, c.minx, c.miny, c.minz, (int)tiles, nodes);
         tiles += (long)d.ntx * d.nty * d.ntz;
         nodes += (long)c.gx * c.gy * c.gz;
@@ -489,7 +549,11 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ga.chunks = w.bin.partials;
     ga.misbinned = w.misbinned;
     ga.xcd = tuning(MPMX_TUNE_XCD_TILES);
-    MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
+    if (tuning(MPMX_TUNE_G2P_TILES)) {
+        MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
+    } else if (P->n > 0) {
+        MPMX_LAUNCH(g2p_particles_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, ga);
+    }
     if ((rc = check_launch("substep_g2p", s))) return rc;
 
     if (prm->vlc && P->n > 0) {
