@@ -36,7 +36,7 @@ constexpr int kTN = kTB + 2;                 // nodes per tile edge
 constexpr int kTileCells = kTB * kTB * kTB;  // 64
 constexpr int kTileNodes = kTN * kTN * kTN;  // 216
 constexpr int kRec = 25;                     // floats per staged particle record
-constexpr int kChunk = 256;                  // particles staged per LDS round
+constexpr int kChunk = 512;                  // particles staged per LDS round (two per thread)
 constexpr int kTasks = kTileCells * 3;       // (cell, i): each task owns the 9 nodes (i, j, k) of its cell
 // Record slot s lives at column s + s/8: the particles of neighbouring cells
 // (about 8 apart in the list) then fall in different LDS banks when the tasks
@@ -182,9 +182,7 @@ __device__ __forceinline__ void orphan_bin(const TileArgs& a) {
 
 // LDS of the tile kernel: the staged records of one round, and afterwards (the
 // same bytes) the 27 offset slots of every cell for the final node sums.
-constexpr int kRecFloats = kRec * kRecCols;
 constexpr int kPartFloats = kTileCells * 27 * 4;
-constexpr int kTileLdsFloats = kRecFloats > kPartFloats ? kRecFloats : kPartFloats;
 
 // FUSED: the constitutive update runs here (stress from registers); else the
 // stress is read.  SYM: the stress is symmetric by construction (the substep's
@@ -192,7 +190,11 @@ constexpr int kTileLdsFloats = kRecFloats > kPartFloats ? kRecFloats : kPartFloa
 // and staged; the standalone P2G reads all 9 of whatever it is given.
 template <bool FUSED, bool SYM>
 __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
-    __shared__ float4 smem4[kTileLdsFloats / 4];
+    // a symmetric stress takes 6 record rows instead of 9: 22 rows x 576 columns
+    // (51 KB) stage a whole tile of up to 512 particles in one round
+    constexpr int kRows = (SYM || FUSED) ? kRec - 3 : kRec;
+    constexpr int kLds = kRows * kRecCols > kPartFloats ? kRows * kRecCols : kPartFloats;
+    __shared__ float4 smem4[(kLds + 3) / 4];
     __shared__ int cs[kTileCells + 1];
     float (*rec)[kRecCols] = reinterpret_cast<float (*)[kRecCols]>(smem4);
     float4* part = smem4;
@@ -254,7 +256,7 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
                 // became XPBD in this substep: it no longer transfers to the grid
                 // (p2g tests the label after the stress pass, mpmRoutines.py:507)
 #pragma unroll
-                for (int k = 0; k < kRec; k++) rec[k][r] = 0.0f;
+                for (int k = 0; k < kRows; k++) rec[k][r] = 0.0f;
                 rec[0][r] = 1.0f;
                 rec[1][r] = 1.0f;
                 rec[2][r] = 1.0f;
