@@ -47,13 +47,13 @@ HBM_PEAK_GBS = 8000.0
 #     material floats 36, accumulated strain 4, damage 4, two labels 8; writes
 #     F 36, stress 36, accumulated strain / damage / ys 12
 #   substep_tile(p2g), the P2G tile pass after it: reads bin id 4, label 4, x 12,
-#     v 12, C 36, vol 4, mass 4, stress 36
+#     v 12, C 36, vol 4, mass 4, stress 24 (symmetric: 6 of its 9 planes)
 #   substep_g2p: reads bin id 4, labels 8, x 12, F 36; writes x 12, v 12, C 36,
 #     F_trial 36, next bin key 4, rank 4
 KERNEL_BYTES_PER_PARTICLE = {
     "substep_tile(stress+p2g)": 156.0 + 84.0,
     "substep_stress": 124.0 + 84.0,
-    "substep_tile(p2g)": 112.0,
+    "substep_tile(p2g)": 100.0,
     "substep_g2p": 60.0 + 104.0,
 }
 PROFILE_HOLD_US = 50000  # long enough for the host to queue a whole profiled period
@@ -150,15 +150,15 @@ def roofline_kernel(kernels, n_mpm):
 def measured_traffic(kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_substep_tile.json")
+    path = os.path.join(ROOT, "profiles", "traffic_substep_kernels.json")
     try:
         with open(path) as f:
             d = json.load(f)
     except OSError:
         return None, None
-    if d.get("kernel") != kernel:
+    if kernel not in d.get("bytes_per_launch", {}):
         return None, None
-    return float(d["bytes_per_launch"]), d.get("source")
+    return float(d["bytes_per_launch"][kernel]), d.get("source")
 
 
 def copy_calibration():
