@@ -128,8 +128,10 @@ const char* mpmx_last_error(void);
  * block order.  MPMX_TUNE_SPLIT_STRESS = 1 runs the constitutive update of the
  * fused substep as its own particle-parallel kernel before a P2G-only tile
  * kernel, 0 inside the tile kernel.  MPMX_TUNE_G2P_TILES = 1 runs G2P per tile
- * with the tile's grid velocities staged in LDS, 0 one thread per particle with
- * the grid read through the caches.  Results do not depend on any knob. */
+ * with the tile's grid velocities staged in LDS, 2 likewise with two blocks
+ * sharing each tile's particles, 0 one thread per particle with the grid read
+ * through the caches.  The other knobs take 0 or 1.  Results do not depend on
+ * any knob. */
 #define MPMX_TUNE_XCD_TILES 0
 #define MPMX_TUNE_SPLIT_STRESS 1
 #define MPMX_TUNE_G2P_TILES 2
@@ -418,6 +420,18 @@ int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dim
                                 int dimz, const int32_t* materialLabel, int label,
                                 const int32_t* order, int32_t* point_ids, int32_t* cell_start,
                                 void* workspace, size_t ws_bytes, void* stream);
+
+/* The table of all points (simulationRoutines.py:311, the grid of the XPBD
+ * step) and the table of the points whose materialLabel equals `label` (the
+ * split contact solve's), in one set of launches: the same contents as
+ * mpmx_hashgrid_build_ordered called once with materialLabel NULL and once with
+ * it (order may be NULL).  Workspace: mpmx_hashgrid_pair_workspace_bytes. */
+size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz);
+int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy,
+                             int dimz, const int32_t* materialLabel, int label,
+                             const int32_t* order, int32_t* point_ids, int32_t* cell_start,
+                             int32_t* label_point_ids, int32_t* label_cell_start,
+                             void* workspace, size_t ws_bytes, void* stream);
 
 /* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,

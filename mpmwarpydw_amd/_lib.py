@@ -85,6 +85,8 @@ SIGNATURES = {
     "mpmx_hashgrid_build": [I, P, F, I, I, I, P, P, P, S, P],
     "mpmx_hashgrid_build_label": [I, P, F, I, I, I, P, I, P, P, P, S, P],
     "mpmx_hashgrid_build_ordered": [I, P, F, I, I, I, P, I, P, P, P, P, S, P],
+    "mpmx_hashgrid_pair_workspace_bytes": [I, I, I, I],
+    "mpmx_hashgrid_build_pair": [I, P, F, I, I, I, P, I, P, P, P, P, P, P, S, P],
     "mpmx_reset_mpm_contact_labels": [I, P, P, P],
     "mpmx_integrate_particles_xpbd": [I, P, P, P, P, F, F, F, F, P, P, F, P],
     "mpmx_solve_particle_bound_contacts": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
@@ -114,6 +116,7 @@ _RESTYPE = {
     "mpmx_p2g_workspace_bytes": S,
     "mpmx_reduce_workspace_bytes": S,
     "mpmx_hashgrid_workspace_bytes": S,
+    "mpmx_hashgrid_pair_workspace_bytes": S,
     "mpmx_mlp_workspace_bytes": S,
     "mpmx_ctx_stream": P,
     "mpmx_ensemble_workspace_bytes": S,

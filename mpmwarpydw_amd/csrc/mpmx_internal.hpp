@@ -65,12 +65,31 @@ bool carve_bin(Carver& cv, long n, long nkeys, BinWork& w);
 int counting_sort_ids(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
                       const int32_t* order = nullptr);
 // scan + scatter + in-run sort only (w.count and w.rank already filled)
+int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
+                              const int32_t* order = nullptr);
 // as counting_sort_from_counts, for producers that also accumulated the total
 // of every chunk of kScanChunk keys into w.partials (one scan pass fewer)
 int counting_sort_from_chunk_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w,
                                     hipStream_t s, const int32_t* order = nullptr);
-int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w, hipStream_t s,
-                              const int32_t* order = nullptr);
+// Several tables sorted over the same n items and nkeys keys in one set of
+// launches: table g has the key of every item (-1: not in the table), its rank
+// inside its key, the counts in key order and their totals per chunk of
+// kScanChunk keys; it receives its cell ranges ([nkeys + 1]) and sorted ids.
+constexpr int kMaxSortSegs = 2;
+struct SortSeg {
+    const int32_t* keys;
+    const int32_t* rank;
+    const int32_t* count;
+    const int32_t* totals;
+    int32_t* cell_start;
+    int32_t* sorted;
+};
+struct SortSegs {
+    SortSeg seg[kMaxSortSegs];
+};
+int counting_sort_segments(const SortSegs& t, int nseg, long n, long nkeys, hipStream_t s, const int32_t* order);
+// dst[0..n) = value on stream s (a kernel: safe inside a captured graph)
+int fill_i32(int32_t* dst, long n, int32_t value, hipStream_t s);
 
 }  // namespace mpmx
 

@@ -191,8 +191,9 @@ __global__ void scan_apply_kernel(const int32_t* __restrict__ in, long n, const 
 // scan_apply when partials[] holds the plain total of every chunk: each block
 // first sums the totals of the chunks before it (in a fixed order), and block 0
 // also writes the grand total to out[n]
-__global__ void scan_apply_chunks_kernel(const int32_t* __restrict__ in, long n, const int32_t* __restrict__ totals,
-                                         int32_t* __restrict__ out, long nb) {
+__device__ __forceinline__ void scan_apply_chunks_tile(const int32_t* __restrict__ in, long n,
+                                                       const int32_t* __restrict__ totals, int32_t* __restrict__ out,
+                                                       long nb) {
     __shared__ int sh[kBlock];
     __shared__ int vals[kScanTile];
     const long lim = blockIdx.x == 0 ? nb : (long)blockIdx.x;
@@ -222,6 +223,19 @@ __global__ void scan_apply_chunks_kernel(const int32_t* __restrict__ in, long n,
         if (i < n) out[i] = ex;
         ex += vals[threadIdx.x * kScanPer + k];
     }
+}
+
+__global__ void scan_apply_chunks_kernel(const int32_t* __restrict__ in, long n, const int32_t* __restrict__ totals,
+                                         int32_t* __restrict__ out, long nb) {
+    scan_apply_chunks_tile(in, n, totals, out, nb);
+}
+
+static_assert(kMaxSortSegs == 2, "the table kernels select seg[0] or seg[1]");
+
+// the same for each table of a multi-table sort (blockIdx.y = table)
+__global__ void scan_apply_chunks_seg_kernel(SortSegs t, long n, long nb) {
+    const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
+    scan_apply_chunks_tile(g.count, n, g.totals, g.cell_start, nb);
 }
 
 size_t scan_partials_count(long n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
@@ -290,11 +304,8 @@ __device__ __forceinline__ void cswap(int& oa, int& va, int& ob, int& vb, bool u
     }
 }
 
-__global__ void __launch_bounds__(kBlock) bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start,
-                                                               int32_t* __restrict__ sorted,
-                                                               const int32_t* __restrict__ order) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nkeys) return;
+__device__ __forceinline__ void sort_run(long c, const int32_t* __restrict__ cell_start, int32_t* __restrict__ sorted,
+                                         const int32_t* __restrict__ order) {
     const int b = cell_start[c], e = cell_start[c + 1];
     const int len = e - b;
     if (len <= 1) return;
@@ -331,6 +342,31 @@ __global__ void __launch_bounds__(kBlock) bin_sort_runs_kernel(long nkeys, const
             j--;
         }
         sorted[j + 1] = vi;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start,
+                                                               int32_t* __restrict__ sorted,
+                                                               const int32_t* __restrict__ order) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nkeys) sort_run(c, cell_start, sorted, order);
+}
+
+__global__ void __launch_bounds__(kBlock) bin_sort_runs_seg_kernel(SortSegs t, long nkeys,
+                                                                   const int32_t* __restrict__ order) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
+    if (c < nkeys) sort_run(c, g.cell_start, g.sorted, order);
+}
+
+__global__ void bin_scatter_seg_kernel(SortSegs t, int nseg, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int g = 0; g < kMaxSortSegs; g++) {
+        if (g >= nseg) break;
+        const int k = t.seg[g].keys[i];
+        if (k >= 0) t.seg[g].sorted[t.seg[g].cell_start[k] + t.seg[g].rank[i]] = (int32_t)i;
     }
 }
 
@@ -403,6 +439,33 @@ int counting_sort_from_chunk_counts(const int32_t* keys, long n, long nkeys, int
     MPMX_LAUNCH(bin_sort_runs_kernel, dim3(grid_for(nkeys, kBlock)), dim3(kBlock), 0, s, nkeys, w.cell_start,
                 sorted, order);
     return check_launch("bin_sort_runs", s);
+}
+
+int fill_i32(int32_t* dst, long n, int32_t value, hipStream_t s) {
+    if (n <= 0) return 0;
+    MPMX_LAUNCH(fill_kernel<int32_t>, dim3(fill_grid(n)), dim3(kBlock), 0, s, dst, n, value);
+    return check_launch("fill_i32", s);
+}
+
+int counting_sort_segments(const SortSegs& t, int nseg, long n, long nkeys, hipStream_t s, const int32_t* order) {
+    MPMX_REQUIRE(nseg >= 1 && nseg <= kMaxSortSegs, "counting_sort_segments: %d tables", nseg);
+    const long nb = (nkeys + kScanTile - 1) / kScanTile;
+    int rc;
+    if (nb == 0) {
+        for (int g = 0; g < nseg; g++) {
+            MPMX_LAUNCH(fill_kernel<int32_t>, dim3(1), dim3(kBlock), 0, s, t.seg[g].cell_start, 1L, 0);
+            if ((rc = check_launch("scan(empty)", s))) return rc;
+        }
+        return 0;
+    }
+    MPMX_LAUNCH(scan_apply_chunks_seg_kernel, dim3((unsigned)nb, nseg), dim3(kBlock), 0, s, t, nkeys, nb);
+    if ((rc = check_launch("scan_apply(tables)", s))) return rc;
+    if (n > 0) {
+        MPMX_LAUNCH(bin_scatter_seg_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, t, nseg, n);
+        if ((rc = check_launch("bin_scatter(tables)", s))) return rc;
+    }
+    MPMX_LAUNCH(bin_sort_runs_seg_kernel, dim3(grid_for(nkeys, kBlock), nseg), dim3(kBlock), 0, s, t, nkeys, order);
+    return check_launch("bin_sort_runs(tables)", s);
 }
 
 // The scan / scatter / in-run sort half of counting_sort_ids, for callers that
@@ -535,7 +598,8 @@ int mpmx_version(void) { return 1; }
 
 int mpmx_set_tuning(int knob, int value) {
     MPMX_REQUIRE(knob >= 0 && knob < MPMX_TUNE_COUNT, "mpmx_set_tuning: unknown knob %d", knob);
-    MPMX_REQUIRE(value == 0 || value == 1, "mpmx_set_tuning: knob %d takes 0 or 1", knob);
+    const int vmax = knob == MPMX_TUNE_G2P_TILES ? 2 : 1;
+    MPMX_REQUIRE(value >= 0 && value <= vmax, "mpmx_set_tuning: knob %d takes 0 to %d", knob, vmax);
     g_tuning[knob] = value;
     return 0;
 }

@@ -139,23 +139,6 @@ __device__ __forceinline__ int tile_key(f3 x, float inv_dx, const SceneDesc& d) 
     return (d.tile0 + (tx * d.nty + ty) * d.ntz + tz) * kTileCells + (lx * kTB + ly) * kTB + lz;
 }
 
-// B-spline weight and derivative for a stencil offset i known only at run time
-__device__ __forceinline__ void wsel(float f, int i, float& w, float& dw) {
-    if (i == 0) {
-        float a = 1.5f - f;
-        w = a * a * 0.5f;
-        dw = f - 1.5f;
-    } else if (i == 1) {
-        float b = f - 1.0f;
-        w = (0.0f - b * b) + 0.75f;
-        dw = -2.0f * b;
-    } else {
-        float c = f - 0.5f;
-        w = c * c * 0.5f;
-        dw = c;
-    }
-}
-
 struct TileArgs {
     mpmx_particles P;             // particle state (read; written by the constitutive update when fused)
     const float* stress;          // stress to scatter when not fused
@@ -316,10 +299,17 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
             const int lo = max(cs[tc] - p0, c0) - c0;
             const int hi = min(cs[tc + 1] - p0, c0 + cn) - c0;
             const float fi = (float)ti;
+            // the x weight and derivative of offset ti without branches (the lanes of a
+            // wavefront hold all three offsets): with u = f - (1.5, 1, 0.5)[ti],
+            // w = u^2 / 2, 0.75 - u^2, u^2 / 2 and dw = u, -2u, u (bspline's products:
+            // 1.5 - f is -u exactly)
+            const float wshift = ti == 0 ? 1.5f : (ti == 1 ? 1.0f : 0.5f);
+            const bool wmid = ti == 1;
             for (int s = lo; s < hi; s++) {
                 const int r = rcol(s);
-                float wx, dwx;
-                wsel(rec[0][r], ti, wx, dwx);
+                const float u = rec[0][r] - wshift, uu = u * u;
+                const float wx = wmid ? (0.0f - uu) + 0.75f : uu * 0.5f;
+                const float dwx = wmid ? -2.0f * u : u;
                 float wy[3], dwy[3], wz[3], dwz[3];
                 bspline(rec[1][r], wy, dwy);
                 bspline(rec[2][r], wz, dwz);

@@ -117,6 +117,7 @@ struct G2PArgs {
     int32_t* chunks; // next substep: particles per chunk of kScanChunk keys (cleared by K2)
     int32_t* misbinned;
     int xcd;
+    int segs;  // tile kernel: blocks per tile
 };
 
 __device__ __forceinline__ void bin_next(const G2PArgs& a, long id, int key) {
@@ -139,7 +140,8 @@ __device__ __forceinline__ f3 g2p_node(const G2PArgs& a, const float (*gvs)[kTil
 // covariance when it is tracked); returns the particle's key for the next substep.
 template <bool LDS>
 __device__ __forceinline__ int g2p_one(const G2PArgs& a, const SceneDesc& sd, const float (*gvs)[kTileNodes], int id,
-                                       f3 xp, int b0, int b1, int b2, int ox, int oy, int oz, const float fx[3]) {
+                                       f3 xp, const float Fp[9], int b0, int b1, int b2, int ox, int oy, int oz,
+                                       const float fx[3]) {
     const mpmx_particles& P = a.P;
     const long n = P.n;
     const long G = a.geo.nnodes;
@@ -160,8 +162,7 @@ __device__ __forceinline__ int g2p_one(const G2PArgs& a, const SceneDesc& sd, co
 #pragma unroll
     for (int k = 0; k < 9; k++) Cn[k] = nB[k] * sC;
     st9(P.C, n, id, Cn);
-    float Fp[9], Fn[9];
-    ld9(P.F, n, id, Fp);
+    float Fn[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -187,20 +188,44 @@ __device__ __forceinline__ int g2p_one(const G2PArgs& a, const SceneDesc& sd, co
     return key < 0 ? (int)(a.geo.ntiles() * kTileCells) : key;
 }
 
+constexpr int kDeferRounds = 4;  // rounds of kBlock particles whose ranks wait for one count flush
+
+// One wavefront adds the tile's LDS cell counts to the global counts (the
+// returned bases go to lbase), clears them, and adds their total to the
+// tile's scan chunk (the tile's keys all lie in one chunk: kScanChunk is a
+// multiple of 64).
+__device__ __forceinline__ void tile_counts_to_global(const G2PArgs& a, int* lcnt, int* lbase, int key0) {
+    if (threadIdx.x < kTileCells) {
+        const int c = lcnt[threadIdx.x];
+        lbase[threadIdx.x] = c ? atomicAdd(&a.count[key0 + threadIdx.x], c) : 0;
+        lcnt[threadIdx.x] = 0;
+        int tot = c;
+#pragma unroll
+        for (int off = kTileCells / 2; off > 0; off >>= 1) tot += __shfl_xor(tot, off, kTileCells);
+        if (threadIdx.x == 0 && tot) atomicAdd(&a.chunks[key0 / kScanChunk], tot);
+    }
+}
+
 // G2P per tile with the tile's grid velocities staged in LDS; F_trial = (I + dt G) F
 // is evaluated as F + dt (G F).  The particles then count themselves into the
 // next substep's bins: a particle that stays in this tile takes its rank from an
-// LDS counter, and once per round of 256 particles each cell of the tile adds
-// its LDS count to the global count with one atomic, whose return value is the
-// base of the round's ranks.  Particles that leave the tile count globally.  The
-// ranks only place ids inside their bin; the bin is sorted by id afterwards.
+// LDS counter, and each cell of the tile adds its LDS count to the global count
+// with one atomic, whose return value is the base of those ranks: once after the
+// block's last round when it has at most kDeferRounds rounds (the ranks wait in
+// LDS), else after every round.  Particles that leave the tile count globally.
+// The ranks only place ids inside their bin; the bin is sorted by id afterwards.
+// The first round's id and position are loaded before the grid staging.
 __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     __shared__ float gvs[3][kTileNodes];
     __shared__ int lcnt[kTileCells];
     __shared__ int lbase[kTileCells];
+    __shared__ int lslot[kDeferRounds][kBlock];  // (local cell << 16) | rank in the LDS count, or -1
+    __shared__ int lid[kDeferRounds][kBlock];
     const mpmx_particles& P = a.P;
     const long n = P.n;
-    const int t = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    // a.segs blocks share a tile, block h of them taking rounds h, h + segs, ...
+    const int b = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int t = b / a.segs, h = b - t * a.segs;
     const long ntiles = a.geo.ntiles();
     const int orphan_key = (int)(ntiles * kTileCells);
     if (t == ntiles) {
@@ -212,14 +237,29 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
         }
         return;
     }
-    const long p0 = a.cell_start[(long)t * kTileCells];
+    const long p0 = a.cell_start[(long)t * kTileCells] + (long)h * kBlock;
     const long p1 = a.cell_start[(long)t * kTileCells + kTileCells];
-    if (p1 == p0) return;
+    if (p1 <= p0) return;
     const SceneDesc sd = a.geo.at(a.geo.scene_of_tile(t));
     const int lt = t - sd.tile0;
     const int tz = lt % sd.ntz, ty = (lt / sd.ntz) % sd.nty, tx = lt / (sd.ntz * sd.nty);
     const int ox = tx * kTB, oy = ty * kTB, oz = tz * kTB;
     const long G = a.geo.nnodes;
+    const long step = (long)a.segs * kBlock;
+    const int rounds = (int)((p1 - p0 + step - 1) / step);
+    // the first round's particle: its loads overlap the grid staging
+    int id = -1, label = 2;
+    f3 xp{};
+    float Fp[9];
+    {
+        const long s = p0 + threadIdx.x;
+        if (s < p1) {
+            id = a.sorted[s];
+            label = P.materialLabel[id];
+            xp = ld3(P.x, n, id);
+            ld9(P.F, n, id, Fp);
+        }
+    }
     for (int node = threadIdx.x; node < kTileNodes; node += kBlock) {
         const int nx = ox + node / (kTN * kTN), ny = oy + (node / kTN) % kTN, nz = oz + node % kTN;
         float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
@@ -236,19 +276,29 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     if (threadIdx.x < kTileCells) lcnt[threadIdx.x] = 0;
     __syncthreads();
     const int key0 = t * kTileCells;
-    for (long base = p0; base < p1; base += kBlock) {  // uniform trip count: the barriers below are block-wide
-        const long s = base + threadIdx.x;
-        int id = -1, key = -1, lrank = -1;
-        if (s < p1) {
-            id = a.sorted[s];
-            if (P.materialLabel[id] <= 1) {  // else it became XPBD in this substep's constitutive update
-                const f3 xp = ld3(P.x, n, id);
+    // up to kDeferRounds rounds, the tile's cells add their LDS counts to the
+    // global counts once, after the last round; longer tiles do it every round
+    const bool defer = rounds <= kDeferRounds;
+    for (int r = 0; r < rounds; r++) {  // uniform trip count: the barriers below are block-wide
+        const long s = p0 + r * step + threadIdx.x;
+        if (r > 0) {
+            id = -1;
+            if (s < p1) {
+                id = a.sorted[s];
+                label = P.materialLabel[id];
+                xp = ld3(P.x, n, id);
+                ld9(P.F, n, id, Fp);
+            }
+        }
+        int key = -1, lrank = -1;
+        if (id >= 0) {
+            if (label <= 1) {  // else it became XPBD in this substep's constitutive update
                 float fx[3];
                 const int b0 = base_axis(xp.x, sd.minx, a.inv_dx, fx[0]) - ox;
                 const int b1 = base_axis(xp.y, sd.miny, a.inv_dx, fx[1]) - oy;
                 const int b2 = base_axis(xp.z, sd.minz, a.inv_dx, fx[2]) - oz;
                 if (b0 >= 0 && b1 >= 0 && b2 >= 0 && b0 < kTB && b1 < kTB && b2 < kTB) {
-                    key = g2p_one<true>(a, sd, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
+                    key = g2p_one<true>(a, sd, gvs, id, xp, Fp, b0, b1, b2, ox, oy, oz, fx);
                 } else {
                     // stale bin: the particle is outside its tile's node block
                     if (a.misbinned) atomicAdd(a.misbinned, 1);
@@ -256,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
                         b2 + oz > sd.gz - 3)
                         key = orphan_key;  // its stencil leaves the grid: no transfer
                     else
-                        key = g2p_one<false>(a, sd, gvs, id, xp, b0, b1, b2, ox, oy, oz, fx);
+                        key = g2p_one<false>(a, sd, gvs, id, xp, Fp, b0, b1, b2, ox, oy, oz, fx);
                 }
             }
             a.keys[id] = key;
@@ -267,18 +317,24 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
                 atomicAdd(&a.chunks[key / kScanChunk], 1);
             }
         }
-        __syncthreads();
-        if (threadIdx.x < kTileCells) {  // one wavefront: the tile's cells
-            const int c = lcnt[threadIdx.x];
-            lbase[threadIdx.x] = c ? atomicAdd(&a.count[key0 + threadIdx.x], c) : 0;
-            lcnt[threadIdx.x] = 0;
-            int tot = c;  // the tile's keys all lie in one chunk (kScanChunk is a multiple of 64)
-#pragma unroll
-            for (int off = kTileCells / 2; off > 0; off >>= 1) tot += __shfl_xor(tot, off, kTileCells);
-            if (threadIdx.x == 0 && tot) atomicAdd(&a.chunks[key0 / kScanChunk], tot);
+        if (defer) {
+            lslot[r][threadIdx.x] = lrank >= 0 ? ((key - key0) << 16) | lrank : -1;
+            lid[r][threadIdx.x] = id;
+        } else {
+            __syncthreads();
+            tile_counts_to_global(a, lcnt, lbase, key0);
+            __syncthreads();
+            if (lrank >= 0) a.rank[id] = lbase[key - key0] + lrank;
         }
+    }
+    if (defer) {
         __syncthreads();
-        if (lrank >= 0) a.rank[id] = lbase[key - key0] + lrank;
+        tile_counts_to_global(a, lcnt, lbase, key0);
+        __syncthreads();
+        for (int r = 0; r < rounds; r++) {
+            const int v = lslot[r][threadIdx.x];
+            if (v >= 0) a.rank[lid[r][threadIdx.x]] = lbase[v >> 16] + (v & 0xffff);
+        }
     }
 }
 
@@ -341,7 +397,13 @@ __global__ void __launch_bounds__(kBlock) g2p_particles_kernel(G2PArgs a) {
                 const int now = out ? orphan_key : tile_key(xp, a.inv_dx, sd);
                 if (now != a.keys[id]) atomicAdd(a.misbinned, 1);
             }
-            key = out ? orphan_key : g2p_one<false>(a, sd, nullptr, id, xp, b0, b1, b2, 0, 0, 0, fx);
+            if (out) {
+                key = orphan_key;
+            } else {
+                float Fp[9];
+                ld9(P.F, n, id, Fp);
+                key = g2p_one<false>(a, sd, nullptr, id, xp, Fp, b0, b1, b2, 0, 0, 0, fx);
+            }
         }
         a.keys[id] = key;
     }
@@ -549,8 +611,10 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ga.chunks = w.bin.partials;
     ga.misbinned = w.misbinned;
     ga.xcd = tuning(MPMX_TUNE_XCD_TILES);
-    if (tuning(MPMX_TUNE_G2P_TILES)) {
-        MPMX_LAUNCH(g2p_tile_kernel, dim3(ntile_blocks), dim3(kBlock), 0, s, ga);
+    ga.segs = tuning(MPMX_TUNE_G2P_TILES);
+    if (ga.segs) {
+        // segs blocks per tile, then the orphan block
+        MPMX_LAUNCH(g2p_tile_kernel, dim3(ga.segs * geo.ntiles() + 1), dim3(kBlock), 0, s, ga);
     } else if (P->n > 0) {
         MPMX_LAUNCH(g2p_particles_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, ga);
     }

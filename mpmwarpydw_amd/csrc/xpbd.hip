@@ -12,28 +12,128 @@
 
 namespace mpmx {
 
-__device__ __forceinline__ int hash_index(int dimx, int dimy, int dimz, int x, int y, int z) {
+// wrapped cell coordinates of cell (x, y, z)
+__device__ __forceinline__ void hash_cell(int dimx, int dimy, int dimz, int x, int y, int z, int& cx, int& cy,
+                                          int& cz) {
     const int origin = 1 << 20;
     x += origin;
     y += origin;
     z += origin;
     // wp.HashGrid wraps x + 2^20 modulo the dims; the extra "+ dim) % dim" only
     // matters for coordinates below -2^20 cells, where it keeps the index in range
-    int cx = ((x % dimx) + dimx) % dimx, cy = ((y % dimy) + dimy) % dimy, cz = ((z % dimz) + dimz) % dimz;
+    cx = ((x % dimx) + dimx) % dimx;
+    cy = ((y % dimy) + dimy) % dimy;
+    cz = ((z % dimz) + dimz) % dimz;
+}
+
+__device__ __forceinline__ int hash_index(int dimx, int dimy, int dimz, int x, int y, int z) {
+    int cx, cy, cz;
+    hash_cell(dimx, dimy, dimz, x, y, z, cx, cy, cz);
     return cz * (dimx * dimy) + cy * dimx + cx;
 }
 
-// key of every point, or only of the points whose label equals `label` when
-// `labels` is given (the others get -1 and are left out of the table)
-__global__ void hash_key_kernel(int n, const float* __restrict__ x, float inv, int dimx, int dimy, int dimz,
-                                const int32_t* __restrict__ labels, int label, int32_t* __restrict__ keys) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    if (labels && labels[p] != label) {
-        keys[p] = -1;
-        return;
+// One or two hash tables over the same points, built in one set of launches.
+// Counting: points stored in tile order step through z fastest, so the keys of
+// neighbouring points lie dimx*dimy apart and their count atomics land on
+// scattered cache lines (about 3x slower, measured).  The counts are taken in
+// the transposed numbering t = (cx*dimy + cy)*dimz + cz, where those keys are
+// adjacent, and hash_counts_transpose_kernel brings them back to key order
+// together with the per-chunk totals of the scan.
+struct HashTables {
+    int n;
+    const float* x;
+    float inv;
+    int dimx, dimy, dimz;
+    int ntab;
+    const int32_t* labels[kMaxSortSegs];  // table filter (null: every point)
+    int label[kMaxSortSegs];
+    int32_t* keys[kMaxSortSegs];
+    int32_t* rank[kMaxSortSegs];
+    int32_t* tcount;  // [ntab][nkeys], transposed numbering
+    int32_t* count;   // [ntab][nkeys], key order
+    int32_t* totals;  // [ntab][nchunks]
+    long nkeys, nchunks;
+};
+
+// key of each point per table (-1 when the table leaves it out) and its rank
+// among the points of its cell: runs of equal keys in a wavefront take one
+// atomic, whose return value is the base of the run's ranks
+__global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1, tk = -1;
+    if (i < h.n) {
+        const long n = h.n;
+        int cx, cy, cz;
+        hash_cell(h.dimx, h.dimy, h.dimz, (int)(h.x[i] * h.inv), (int)(h.x[n + i] * h.inv),
+                  (int)(h.x[2 * n + i] * h.inv), cx, cy, cz);
+        key = cz * (h.dimx * h.dimy) + cy * h.dimx + cx;
+        tk = (cx * h.dimy + cy) * h.dimz + cz;
     }
-    keys[p] = hash_index(dimx, dimy, dimz, (int)(x[p] * inv), (int)(x[n + p] * inv), (int)(x[2L * n + p] * inv));
+    const int lane = (int)(threadIdx.x & 63);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+#pragma unroll
+    for (int g = 0; g < kMaxSortSegs; g++) {  // every lane stays for the cross-lane operations
+        if (g >= h.ntab) break;
+        const bool in = i < h.n && (h.labels[g] == nullptr || h.labels[g][i] == h.label[g]);
+        const int k = in ? tk : -1;
+        if (i < h.n) h.keys[g][i] = in ? key : -1;
+        const int prev = __shfl_up(k, 1);
+        const bool head = lane == 0 || prev != k;
+        const unsigned long long heads = __ballot(head);
+        const int my_head = 63 - __clzll((long long)(heads & upto));
+        const unsigned long long above = heads & ~upto;
+        const int next_head = above ? __ffsll((long long)above) - 1 : 64;
+        int base = 0;
+        if (head && k >= 0) base = atomicAdd(&h.tcount[g * h.nkeys + k], next_head - lane);
+        base = __shfl(base, my_head);
+        if (in) h.rank[g][i] = base + (lane - my_head);
+    }
+}
+
+// tcount -> count, per table and y plane, through a 64 x 64 (x, z) LDS tile;
+// each wavefront then adds a 64-key row of the output to the totals of the
+// (at most two) scan chunks the row touches
+__global__ void __launch_bounds__(kBlock) hash_counts_transpose_kernel(HashTables h) {
+    __shared__ int tile[64][65];
+    const int nbx = (h.dimx + 63) / 64, nbz = (h.dimz + 63) / 64;
+    long b = blockIdx.x;
+    const int bx = (int)(b % nbx);
+    b /= nbx;
+    const int bz = (int)(b % nbz);
+    b /= nbz;
+    const int cy = (int)(b % h.dimy);
+    const int g = (int)(b / h.dimy);
+    const int cx0 = bx * 64, cz0 = bz * 64;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int32_t* tc = h.tcount + g * h.nkeys;
+    for (int r = w; r < 64; r += kBlock / 64) {
+        const int cx = cx0 + r, cz = cz0 + lane;
+        tile[r][lane] = (cx < h.dimx && cz < h.dimz) ? tc[((long)cx * h.dimy + cy) * h.dimz + cz] : 0;
+    }
+    __syncthreads();
+    int32_t* out = h.count + g * h.nkeys;
+    int32_t* tot = h.totals + g * h.nchunks;
+    for (int r = w; r < 64; r += kBlock / 64) {
+        const int cz = cz0 + r;
+        if (cz >= h.dimz) break;  // uniform over the wavefront
+        const int cx = cx0 + lane;
+        const bool valid = cx < h.dimx;
+        const int v = valid ? tile[lane][r] : 0;
+        const long row = ((long)cz * h.dimy + cy) * h.dimx;
+        if (valid) out[row + cx] = v;
+        const long c0 = (row + cx0) / kScanChunk;
+        const bool second = valid && (row + cx) / kScanChunk != c0;
+        int a = second ? 0 : v, s2 = second ? v : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off);
+            s2 += __shfl_xor(s2, off);
+        }
+        if (lane == 0) {
+            if (a) atomicAdd(&tot[c0], a);
+            if (s2) atomicAdd(&tot[c0 + 1], s2);
+        }
+    }
 }
 
 // reset_mpm_contact_labels (xpbdRoutines.py:3-21)
@@ -288,36 +388,100 @@ using namespace mpmx;
 
 extern "C" {
 
+static size_t hash_tables_bytes(int ntab, int n, int dimx, int dimy, int dimz) {
+    if (n < 0 || dimx <= 0 || dimy <= 0 || dimz <= 0) return 0;
+    const long nkeys = (long)dimx * dimy * dimz, nn = n > 0 ? n : 1;
+    const long nch = (nkeys + kScanChunk - 1) / kScanChunk;
+    return 2 * ntab * Carver::bytes_for(nn, 4) + Carver::bytes_for(ntab * (nkeys + nch), 4) +
+           Carver::bytes_for(ntab * nkeys, 4);
+}
+
 size_t mpmx_hashgrid_workspace_bytes(int n, int dimx, int dimy, int dimz) {
-    long nkeys = (long)dimx * dimy * dimz;
-    long nn = n > 0 ? n : 1;
-    return Carver::bytes_for(nn, 4) + bin_workspace_bytes(nn, nkeys);
+    return hash_tables_bytes(1, n, dimx, dimy, dimz);
+}
+
+size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz) {
+    return hash_tables_bytes(2, n, dimx, dimy, dimz);
+}
+
+// ntab tables: table g holds the points whose labels[g] entry equals label[g]
+// (every point when labels[g] is null), with ids ascending in order[id] (in id
+// when order is null) inside each cell
+static int hash_tables_build(int ntab, int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+                             const int32_t* const* labels, const int* label, int32_t* const* point_ids,
+                             int32_t* const* cell_start, void* workspace, size_t ws_bytes, void* stream,
+                             const int32_t* order) {
+    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f,
+                 "mpmx_hashgrid_build: bad arguments");
+    const long nkeys = (long)dimx * dimy * dimz;
+    MPMX_REQUIRE(nkeys * ntab < (1L << 31), "mpmx_hashgrid_build: too many cells");
+    MPMX_REQUIRE(workspace && ws_bytes >= hash_tables_bytes(ntab, n, dimx, dimy, dimz),
+                 "mpmx_hashgrid_build: workspace too small");
+    for (int g = 0; g < ntab; g++)
+        MPMX_REQUIRE(cell_start[g] && (n == 0 || (x && point_ids[g])), "mpmx_hashgrid_build: null pointer");
+    hipStream_t s = as_stream(stream);
+    const long nn = n > 0 ? n : 1;
+    const long nch = (nkeys + kScanChunk - 1) / kScanChunk;
+    HashTables h{};
+    h.n = n;
+    h.x = x;
+    h.inv = 1.0f / cell_width;
+    h.dimx = dimx;
+    h.dimy = dimy;
+    h.dimz = dimz;
+    h.ntab = ntab;
+    h.nkeys = nkeys;
+    h.nchunks = nch;
+    Carver cv(workspace, ws_bytes);
+    for (int g = 0; g < ntab; g++) {
+        h.labels[g] = labels[g];
+        h.label[g] = label[g];
+        h.keys[g] = cv.take<int32_t>(nn);
+        h.rank[g] = cv.take<int32_t>(nn);
+    }
+    h.tcount = cv.take<int32_t>(ntab * (nkeys + nch));
+    h.totals = h.tcount + ntab * nkeys;
+    h.count = cv.take<int32_t>(ntab * nkeys);
+    MPMX_REQUIRE(!cv.overflow, "mpmx_hashgrid_build: workspace carve failed");
+    int rc = fill_i32(h.tcount, ntab * (nkeys + nch), 0, s);
+    if (rc) return rc;
+    if (n > 0) {
+        MPMX_LAUNCH(hash_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, h);
+        if ((rc = check_launch("hash_count", s))) return rc;
+    }
+    const long nblk = (long)ntab * dimy * ((dimz + 63) / 64) * ((dimx + 63) / 64);
+    MPMX_LAUNCH(hash_counts_transpose_kernel, dim3((unsigned)nblk), dim3(kBlock), 0, s, h);
+    if ((rc = check_launch("hash_counts_transpose", s))) return rc;
+    SortSegs t{};
+    for (int g = 0; g < ntab; g++) {
+        t.seg[g].keys = h.keys[g];
+        t.seg[g].rank = h.rank[g];
+        t.seg[g].count = h.count + g * nkeys;
+        t.seg[g].totals = h.totals + g * nch;
+        t.seg[g].cell_start = cell_start[g];
+        t.seg[g].sorted = point_ids[g];
+    }
+    return counting_sort_segments(t, ntab, n, nkeys, s, order);
 }
 
 static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
                           const int32_t* labels, int label, int32_t* point_ids, int32_t* cell_start, void* workspace,
                           size_t ws_bytes, void* stream, const int32_t* order = nullptr) {
-    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f && cell_start,
-                 "mpmx_hashgrid_build: bad arguments");
-    MPMX_REQUIRE(n == 0 || (x && point_ids), "mpmx_hashgrid_build: null pointer");
-    long nkeys = (long)dimx * dimy * dimz;
-    MPMX_REQUIRE(nkeys < (1L << 31), "mpmx_hashgrid_build: too many cells");
-    MPMX_REQUIRE(workspace && ws_bytes >= mpmx_hashgrid_workspace_bytes(n, dimx, dimy, dimz),
-                 "mpmx_hashgrid_build: workspace too small");
-    hipStream_t s = as_stream(stream);
-    Carver cv(workspace, ws_bytes);
-    long nn = n > 0 ? n : 1;
-    int32_t* keys = cv.take<int32_t>(nn);
-    BinWork w;
-    carve_bin(cv, nn, nkeys, w);
-    w.cell_start = cell_start;  // the caller's table receives the cell ranges
-    if (n > 0) {
-        MPMX_LAUNCH(hash_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, x, 1.0f / cell_width,
-                           dimx, dimy, dimz, labels, label, keys);
-        int rc = check_launch("hash_key", s);
-        if (rc) return rc;
-    }
-    return counting_sort_ids(keys, n, nkeys, point_ids, w, s, order);
+    return hash_tables_build(1, n, x, cell_width, dimx, dimy, dimz, &labels, &label, &point_ids, &cell_start,
+                             workspace, ws_bytes, stream, order);
+}
+
+int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+                             const int32_t* materialLabel, int label, const int32_t* order, int32_t* point_ids,
+                             int32_t* cell_start, int32_t* label_point_ids, int32_t* label_cell_start,
+                             void* workspace, size_t ws_bytes, void* stream) {
+    MPMX_REQUIRE(n == 0 || materialLabel, "mpmx_hashgrid_build_pair: null label array");
+    const int32_t* labels[2] = {nullptr, materialLabel};
+    const int lab[2] = {0, label};
+    int32_t* ids[2] = {point_ids, label_point_ids};
+    int32_t* starts[2] = {cell_start, label_cell_start};
+    return hash_tables_build(2, n, x, cell_width, dimx, dimy, dimz, labels, lab, ids, starts, workspace, ws_bytes,
+                             stream, order);
 }
 
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,

@@ -296,8 +296,7 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     outside the step reads them)."""
     n = sim.nPoints
     order = getattr(sim, "particle_id", None)
-    xpbd.particle_grid.build(sim.particle_x, sim.dx, order=order)
-    xpbd.particle_grid_xpbd.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2, order=order)
+    xpbd.particle_grid.build_pair(xpbd.particle_grid_xpbd, sim.particle_x, sim.dx, sim.materialLabel, 2, order)
     g = sim.gravity
     s = stream_handle()
     reset = 1 if (xpbd.mpm_contact_transition_lock and not xpbd_only) else 0

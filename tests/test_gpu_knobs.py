@@ -8,7 +8,7 @@ from state_utils import host_state, make_sim
 
 pytestmark = pytest.mark.gpu
 
-COMBOS = [(1, 1, 1), (1, 0, 1), (1, 0, 0), (0, 1, 1), (0, 0, 1)]  # (split_stress, g2p_tiles, xcd_tiles)
+COMBOS = [(1, 1, 1), (1, 2, 1), (1, 2, 0), (1, 0, 1), (1, 0, 0), (0, 1, 1), (0, 2, 1), (0, 0, 1)]  # (split_stress, g2p_tiles, xcd_tiles)
 
 
 def test_knobs_leave_results_unchanged():
@@ -18,7 +18,7 @@ def test_knobs_leave_results_unchanged():
     try:
         for split, g2p, xcd in COMBOS:
             autotune.set_split_stress(bool(split))
-            autotune.set_g2p_tiles(bool(g2p))
+            autotune.set_g2p_tiles(g2p)
             autotune.set_xcd_tiles(bool(xcd))
             S = make_sim(seed=50)
             print(f"[knobs] split={split} g2p_tiles={g2p} xcd={xcd}", flush=True)

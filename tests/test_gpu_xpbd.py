@@ -52,6 +52,51 @@ def test_hashgrid_bit_exact(o64):
     assert np.all(ce[~nonempty] == cs[~nonempty])
 
 
+@pytest.mark.parametrize("dims", [(37, 5, 70), (64, 1, 65), (3, 130, 2)])
+def test_hashgrid_odd_dims_bit_exact(o64, dims):
+    """Dims that are not multiples of the 64-cell transpose tiles, points on
+    both sides of 0 (wrapped cells), and cells shared by many points."""
+    from mpmwarpydw_amd import runtime as rt
+    rng = np.random.default_rng(sum(dims))
+    n = 20000
+    x = rng.uniform(-9.0, 11.0, (n, 3)).astype(np.float32)
+    x[: n // 4] = np.round(x[: n // 4])  # a quarter on few cells
+    pts = rt.array(x, rt.vec3)
+    g = rt.HashGrid(*dims)
+    print(f"[hashgrid] dims={dims} n={n}", flush=True)
+    g.build(pts, 0.7)
+    ids, cs, ce, _, _ = grid_tuple(g)
+    rid, rcs, rce = o64.hashgrid_build(x.astype(np.float64), np.float32(0.7), dims)
+    assert np.array_equal(ids, rid)
+    nonempty = rce > rcs
+    assert np.array_equal(cs[nonempty], rcs[nonempty]) and np.array_equal(ce[nonempty], rce[nonempty])
+    assert np.all(ce[~nonempty] == cs[~nonempty]) and g.cell_start.numpy()[-1] == n
+
+
+@pytest.mark.parametrize("ordered", [False, True])
+def test_hashgrid_pair_matches_single_builds(ordered):
+    """build_pair gives both tables bit for bit as two single builds."""
+    from mpmwarpydw_amd import runtime as rt
+    S = xpbd_scene(6)
+    sim = S.sim
+    n = sim.nPoints
+    order = None
+    if ordered:
+        order = rt.array(np.random.default_rng(6).permutation(n).astype(np.int32), rt.int32)
+    print(f"[hashgrid pair] n={n} ordered={ordered}", flush=True)
+    a, b = rt.HashGrid(128, 128, 128), rt.HashGrid(128, 128, 128)
+    a.build(sim.particle_x, sim.dx, order=order)
+    b.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2, order=order)
+    c, d = rt.HashGrid(128, 128, 128), rt.HashGrid(128, 128, 128)
+    c.build_pair(d, sim.particle_x, sim.dx, sim.materialLabel, 2, order)
+    lab = sim.materialLabel.numpy()
+    assert 0 < (lab == 2).sum() < n
+    for single, pair in ((a, c), (b, d)):
+        assert np.array_equal(single.point_ids.numpy(), pair.point_ids.numpy())
+        assert np.array_equal(single.cell_start.numpy(), pair.cell_start.numpy())
+    assert d.cell_start.numpy()[-1] == (lab == 2).sum()
+
+
 @pytest.mark.parametrize("lock", [0, 1])
 def test_contact_kernels_parity(o64, lock):
     from mpmwarpydw_amd import xpbdRoutines as X
