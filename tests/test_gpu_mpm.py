@@ -173,6 +173,68 @@ def test_substep_teacher_forced(o64, fused, bc, vlc):
     assert np.array_equal(G["materialLabel"], H["materialLabel"])
 
 
+def dense_sim(seed, squeeze=0.2):
+    """block_collapse squeezed toward its centre: hundreds of particles per
+    cell and thousands per tile, so the tile kernels take several rounds
+    (P2G stages at most 512 particles at a time, G2P counts in LDS for up to
+    1024 per block before it falls back to a flush per round)."""
+    from mpmwarpydw_amd import driver, getArgs, scenes
+    from state_utils import perturb_state
+    dom, ov = scenes.block_collapse(scale=0.75, seed=seed)  # 13,824 particles within about 1.2 m
+    c = dom["x"].mean(axis=0)
+    dom["x"] = c + (dom["x"] - c) * squeeze
+    S = driver.setup(getArgs.get_args([], ov), dom)
+    perturb_state(S, seed)
+    return S
+
+
+def test_substep_dense_tiles(o64):
+    """One fused substep with several thousand particles per tile: the
+    multi-round paths of both tile kernels agree with the oracle."""
+    from mpmwarpydw_amd import simulationRoutines as SR
+    S = dense_sim(10)
+    H = host_state(S)
+    x = H["particle_x"]
+    cells = np.floor((x - np.asarray(S.sim.minBounds)) / S.sim.dx).astype(np.int64)
+    _, per_tile = np.unique(cells // 4, axis=0, return_counts=True)
+    print(f"[dense substep] n={S.sim.nPoints} max per tile={per_tile.max()}", flush=True)
+    assert per_tile.max() > 1024
+    SR.mpmSimulationStepFused(S.sim, S.mpm, S.xpbd)
+    G = host_state(S)
+    oracle_stress(o64, H, S.mpm.eff)
+    oracle_p2g(o64, H, S)
+    oracle_grid(o64, H, S)
+    oracle_g2p(o64, H, S)
+    for k in ("particle_stress", "particle_F", "grid_m", "grid_v_in", "grid_v_out", "particle_v", "particle_x",
+              "particle_C", "particle_F_trial"):
+        e = rel_err(G[k], H[k])
+        print(f"  {k}: rel err {e:.2e}")
+        assert e < TOL, k
+
+
+def test_dense_tiles_knobs_bitwise():
+    """Dense tiles over three substeps: the G2P launch shapes (one or two
+    blocks per tile, particle-parallel) give the same bits."""
+    from mpmwarpydw_amd import autotune
+    from mpmwarpydw_amd import simulationRoutines as SR
+    ref = None
+    try:
+        for g2p in (1, 2, 0):
+            autotune.set_g2p_tiles(g2p)
+            S = dense_sim(11)
+            print(f"[dense knobs] g2p_tiles={g2p}", flush=True)
+            for _ in range(3):
+                SR.mpmSimulationStepFused(S.sim, S.mpm, S.xpbd)
+            st = host_state(S)
+            if ref is None:
+                ref = st
+                continue
+            for k in ref:
+                assert np.array_equal(st[k], ref[k]), (g2p, k)
+    finally:
+        autotune.set_g2p_tiles(1)
+
+
 @pytest.mark.parametrize("bc,vlc", [("friction", 0), ("restitution", 1), ("plane_strain_y", 1)])
 def test_fused_matches_reference_order(bc, vlc):
     from mpmwarpydw_amd import simulationRoutines as SR
