@@ -445,26 +445,38 @@ __global__ void p2g_combine_kernel(CombineArgs a) {
     const int iz = li % d.gz;
     const int iy = (li / d.gz) % d.gy;
     const int ix = li / (d.gy * d.gz);
-    float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    // the (up to 8) tiles whose node blocks hold this node, tx ascending, then ty,
+    // then tz; every flag and partial is loaded before the first sum, and a tile
+    // whose flag is clear adds nothing
     const int tx1 = ix / kTB, ty1 = iy / kTB, tz1 = iz / kTB;
-    for (int tx = tx1 - 1; tx <= tx1; tx++) {
-        const int lx = ix - tx * kTB;
-        if (tx < 0 || tx >= d.ntx || lx >= kTN) continue;
-        for (int ty = ty1 - 1; ty <= ty1; ty++) {
-            const int ly = iy - ty * kTB;
-            if (ty < 0 || ty >= d.nty || ly >= kTN) continue;
-            for (int tz = tz1 - 1; tz <= tz1; tz++) {
-                const int lz = iz - tz * kTB;
-                if (tz < 0 || tz >= d.ntz || lz >= kTN) continue;
-                const long t = d.tile0 + ((long)tx * d.nty + ty) * d.ntz + tz;
-                if (!a.tile_flag[t]) continue;
-                const float* pb = a.partial + t * kTileNodes * 4;
-                const int node = (lx * kTN + ly) * kTN + lz;
-                m += pb[node];
-                mx += pb[kTileNodes + node];
-                my += pb[2 * kTileNodes + node];
-                mz += pb[3 * kTileNodes + node];
-            }
+    float pv[8][4];
+    int on[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int tx = tx1 - 1 + (q >> 2), ty = ty1 - 1 + ((q >> 1) & 1), tz = tz1 - 1 + (q & 1);
+        const int lx = ix - tx * kTB, ly = iy - ty * kTB, lz = iz - tz * kTB;
+        on[q] = 0;
+        pv[q][0] = pv[q][1] = pv[q][2] = pv[q][3] = 0.0f;
+        if (tx >= 0 && tx < d.ntx && lx < kTN && ty >= 0 && ty < d.nty && ly < kTN && tz >= 0 && tz < d.ntz &&
+            lz < kTN) {
+            const long t = d.tile0 + ((long)tx * d.nty + ty) * d.ntz + tz;
+            const float* pb = a.partial + t * kTileNodes * 4;
+            const int node = (lx * kTN + ly) * kTN + lz;
+            on[q] = a.tile_flag[t];
+            pv[q][0] = pb[node];
+            pv[q][1] = pb[kTileNodes + node];
+            pv[q][2] = pb[2 * kTileNodes + node];
+            pv[q][3] = pb[3 * kTileNodes + node];
+        }
+    }
+    float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (on[q]) {
+            m += pv[q][0];
+            mx += pv[q][1];
+            my += pv[q][2];
+            mz += pv[q][3];
         }
     }
     a.grid_m[i] = m;
