@@ -48,13 +48,13 @@ HBM_PEAK_GBS = 8000.0
 #     F 36, stress 36, accumulated strain / damage / ys 12
 #   substep_tile(p2g), the P2G tile pass after it: reads bin id 4, label 4, x 12,
 #     v 12, C 36, vol 4, mass 4, stress 24 (symmetric: 6 of its 9 planes)
-#   substep_g2p: reads bin id 4, labels 8, x 12, F 36; writes x 12, v 12, C 36,
+#   substep_g2p: reads bin id 4, label 4, x 12, F 36; writes x 12, v 12, C 36,
 #     F_trial 36, next bin key 4, rank 4
 KERNEL_BYTES_PER_PARTICLE = {
     "substep_tile(stress+p2g)": 156.0 + 84.0,
     "substep_stress": 124.0 + 84.0,
     "substep_tile(p2g)": 100.0,
-    "substep_g2p": 60.0 + 104.0,
+    "substep_g2p": 56.0 + 104.0,
 }
 PROFILE_HOLD_US = 50000  # long enough for the host to queue a whole profiled period
 
