@@ -171,6 +171,107 @@ constexpr int kPartFloats = kTileCells * 27 * 4;
 // stress is read.  SYM: the stress is symmetric by construction (the substep's
 // own constitutive update writes T[i][j] = T[j][i]), so 6 components are read
 // and staged; the standalone P2G reads all 9 of whatever it is given.
+// P2G inputs of one particle
+struct P2GIn {
+    float T[9];  // stress
+    f3 x, v;
+    float C[9];
+    float m, vol;
+    bool mpm;
+};
+
+template <bool FUSED, bool SYM>
+__device__ __forceinline__ void p2g_load(const TileArgs& a, long n, int id, P2GIn& q) {
+    if (FUSED) {
+        q.mpm = constitutive_update(a.P, id, a.eff, q.T);
+    } else if (SYM) {
+        // the constitutive update ran before this kernel and may have turned it XPBD
+        q.mpm = a.P.materialLabel[id] <= 1;
+        q.T[0] = a.stress[id];
+        q.T[1] = a.stress[1L * n + id];
+        q.T[2] = a.stress[2L * n + id];
+        q.T[4] = a.stress[4L * n + id];
+        q.T[5] = a.stress[5L * n + id];
+        q.T[8] = a.stress[8L * n + id];
+        q.T[3] = q.T[1];
+        q.T[6] = q.T[2];
+        q.T[7] = q.T[5];
+    } else {
+        q.mpm = a.P.materialLabel[id] <= 1;
+#pragma unroll
+        for (int k = 0; k < 9; k++) q.T[k] = a.stress[(long)k * n + id];
+    }
+    q.x = ld3(a.P.x, n, id);
+    q.v = ld3(a.P.v, n, id);
+    ld9(a.P.C, n, id, q.C);
+    q.m = a.P.mass[id];
+    q.vol = a.P.vol[id];
+}
+
+// the P2G record of one particle in LDS column r (slot: its place in the bins)
+template <bool FUSED, bool SYM>
+__device__ __forceinline__ void p2g_record(const TileArgs& a, const P2GIn& q, float (*rec)[kRecCols], int r, int slot,
+                                           int t, const int* cs, const SceneDesc& sd, float sfac) {
+    constexpr int kRows = (SYM || FUSED) ? kRec - 3 : kRec;
+    if (!q.mpm) {
+        // became XPBD in this substep: it no longer transfers to the grid
+        // (p2g tests the label after the stress pass, mpmRoutines.py:507)
+#pragma unroll
+        for (int k = 0; k < kRows; k++) rec[k][r] = 0.0f;
+        rec[0][r] = 1.0f;
+        rec[1][r] = 1.0f;
+        rec[2][r] = 1.0f;
+        return;
+    }
+    const f3 xp = q.x, vp = q.v;
+    if (a.misbinned) {
+        // bins come from the previous substep's G2P: check that the particle is still in its cell
+        const int key = tile_key(xp, a.inv_dx, sd);
+        const int c = key - t * kTileCells;
+        if (c < 0 || c >= kTileCells || slot < cs[c] || slot >= cs[c + 1]) atomicAdd(a.misbinned, 1);
+    }
+    float f0, f1, f2;
+    base_axis(xp.x, sd.minx, a.inv_dx, f0);
+    base_axis(xp.y, sd.miny, a.inv_dx, f1);
+    base_axis(xp.z, sd.minz, a.inv_dx, f2);
+    const float* Cp = q.C;
+    float Cm[9];
+    const float rpic = a.rpic;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            Cm[i * 3 + j] = (1.0f - rpic) * Cp[i * 3 + j] + rpic / 2.0f * (Cp[i * 3 + j] - Cp[j * 3 + i]);
+    if (rpic < -0.001f) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) Cm[k] = 0.0f;
+    }
+    const float m = q.m;
+    const float md = m * a.dx;
+    const float sv = sfac * q.vol;
+    const float* T = q.T;
+    rec[0][r] = f0;
+    rec[1][r] = f1;
+    rec[2][r] = f2;
+    rec[3][r] = m;
+    rec[4][r] = m * vp.x - md * (Cm[0] * f0 + Cm[1] * f1 + Cm[2] * f2);
+    rec[5][r] = m * vp.y - md * (Cm[3] * f0 + Cm[4] * f1 + Cm[5] * f2);
+    rec[6][r] = m * vp.z - md * (Cm[6] * f0 + Cm[7] * f1 + Cm[8] * f2);
+#pragma unroll
+    for (int k = 0; k < 9; k++) rec[7 + k][r] = md * Cm[k];
+    if (SYM || FUSED) {  // symmetric: xx yy zz xy xz yz
+        rec[16][r] = sv * T[0];
+        rec[17][r] = sv * T[4];
+        rec[18][r] = sv * T[8];
+        rec[19][r] = sv * T[1];
+        rec[20][r] = sv * T[2];
+        rec[21][r] = sv * T[5];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) rec[16 + k][r] = sv * T[k];
+    }
+}
+
 template <bool FUSED, bool SYM>
 __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     // a symmetric stress takes 6 record rows instead of 9: 22 rows x 576 columns
@@ -211,87 +312,24 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     for (int c0 = 0; c0 < np; c0 += kChunk) {
         const int cn = min(kChunk, np - c0);
         __syncthreads();  // the previous round's readers are done with rec
-        for (int s = tid; s < cn; s += kBlock) {
-            const int id = a.sorted[p0 + c0 + s];
-            const int r = rcol(s);
-            float T[9];
-            bool mpm = true;
-            if (FUSED) {
-                mpm = constitutive_update(a.P, id, a.eff, T);
-            } else if (SYM) {
-                // the constitutive update ran before this kernel and may have turned it XPBD
-                mpm = a.P.materialLabel[id] <= 1;
-                T[0] = a.stress[id];
-                T[1] = a.stress[1L * n + id];
-                T[2] = a.stress[2L * n + id];
-                T[4] = a.stress[4L * n + id];
-                T[5] = a.stress[5L * n + id];
-                T[8] = a.stress[8L * n + id];
-                T[3] = T[1];
-                T[6] = T[2];
-                T[7] = T[5];
-            } else {
-                mpm = a.P.materialLabel[id] <= 1;
-#pragma unroll
-                for (int k = 0; k < 9; k++) T[k] = a.stress[(long)k * n + id];
+        if constexpr (FUSED) {
+            for (int s = tid; s < cn; s += kBlock) {
+                P2GIn q;
+                p2g_load<FUSED, SYM>(a, n, a.sorted[p0 + c0 + s], q);
+                p2g_record<FUSED, SYM>(a, q, rec, rcol(s), p0 + c0 + s, t, cs, sd, sfac);
             }
-            if (!mpm) {
-                // became XPBD in this substep: it no longer transfers to the grid
-                // (p2g tests the label after the stress pass, mpmRoutines.py:507)
-#pragma unroll
-                for (int k = 0; k < kRows; k++) rec[k][r] = 0.0f;
-                rec[0][r] = 1.0f;
-                rec[1][r] = 1.0f;
-                rec[2][r] = 1.0f;
-                continue;
-            }
-            f3 xp = ld3(a.P.x, n, id);
-            f3 vp = ld3(a.P.v, n, id);
-            if (a.misbinned) {
-                // bins come from the previous substep's G2P: check that the particle is still in its cell
-                const int key = tile_key(xp, a.inv_dx, sd);
-                const int c = key - t * kTileCells;
-                const int slot = p0 + c0 + s;
-                if (c < 0 || c >= kTileCells || slot < cs[c] || slot >= cs[c + 1]) atomicAdd(a.misbinned, 1);
-            }
-            float f0, f1, f2;
-            base_axis(xp.x, sd.minx, a.inv_dx, f0);
-            base_axis(xp.y, sd.miny, a.inv_dx, f1);
-            base_axis(xp.z, sd.minz, a.inv_dx, f2);
-            float Cp[9], Cm[9];
-            ld9(a.P.C, n, id, Cp);
-            const float rpic = a.rpic;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    Cm[i * 3 + j] = (1.0f - rpic) * Cp[i * 3 + j] + rpic / 2.0f * (Cp[i * 3 + j] - Cp[j * 3 + i]);
-            if (rpic < -0.001f) {
-#pragma unroll
-                for (int k = 0; k < 9; k++) Cm[k] = 0.0f;
-            }
-            const float m = a.P.mass[id];
-            const float md = m * a.dx;
-            const float sv = sfac * a.P.vol[id];
-            rec[0][r] = f0;
-            rec[1][r] = f1;
-            rec[2][r] = f2;
-            rec[3][r] = m;
-            rec[4][r] = m * vp.x - md * (Cm[0] * f0 + Cm[1] * f1 + Cm[2] * f2);
-            rec[5][r] = m * vp.y - md * (Cm[3] * f0 + Cm[4] * f1 + Cm[5] * f2);
-            rec[6][r] = m * vp.z - md * (Cm[6] * f0 + Cm[7] * f1 + Cm[8] * f2);
-#pragma unroll
-            for (int k = 0; k < 9; k++) rec[7 + k][r] = md * Cm[k];
-            if (SYM || FUSED) {  // symmetric: xx yy zz xy xz yz
-                rec[16][r] = sv * T[0];
-                rec[17][r] = sv * T[4];
-                rec[18][r] = sv * T[8];
-                rec[19][r] = sv * T[1];
-                rec[20][r] = sv * T[2];
-                rec[21][r] = sv * T[5];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 9; k++) rec[16 + k][r] = sv * T[k];
+        } else {
+            // two particles per thread and pass: both load before either is used
+            for (int s = tid; s < cn; s += 2 * kBlock) {
+                const int s2 = s + kBlock;
+                const bool two = s2 < cn;
+                const int id0 = a.sorted[p0 + c0 + s];
+                const int id1 = a.sorted[p0 + c0 + (two ? s2 : s)];
+                P2GIn q0, q1;
+                p2g_load<FUSED, SYM>(a, n, id0, q0);
+                p2g_load<FUSED, SYM>(a, n, id1, q1);
+                p2g_record<FUSED, SYM>(a, q0, rec, rcol(s), p0 + c0 + s, t, cs, sd, sfac);
+                if (two) p2g_record<FUSED, SYM>(a, q1, rec, rcol(s2), p0 + c0 + s2, t, cs, sd, sfac);
             }
         }
         __syncthreads();
