@@ -291,10 +291,11 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     }
     const long n = a.P.n;
     const SceneDesc sd = a.geo.at(a.geo.scene_of_tile(t));
+    // the tile's range straight from memory, so that the first round's loads need
+    // not wait for the cell table in LDS (the barrier after staging publishes it)
     if (tid <= kTileCells) cs[tid] = a.cell_start[(long)t * kTileCells + tid];
-    __syncthreads();
-    const int p0 = cs[0];
-    const int np = cs[kTileCells] - p0;
+    const int p0 = a.cell_start[(long)t * kTileCells];
+    const int np = a.cell_start[(long)t * kTileCells + kTileCells] - p0;
     if (np == 0) {
         if (tid == 0) a.tile_flag[t] = 0;
         return;
@@ -311,7 +312,8 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     const float sfac = -a.dt * a.inv_dx;
     for (int c0 = 0; c0 < np; c0 += kChunk) {
         const int cn = min(kChunk, np - c0);
-        __syncthreads();  // the previous round's readers are done with rec
+        // the previous round's readers are done with rec; the misbinned check reads cs
+        if (c0 > 0 || a.misbinned) __syncthreads();
         if constexpr (FUSED) {
             for (int s = tid; s < cn; s += kBlock) {
                 P2GIn q;
