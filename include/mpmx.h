@@ -496,15 +496,19 @@ int mpmx_apply_mpm_contact_impulses(int n, const int32_t* activeLabel,
 /* ---- fused XPBD step: xpbdSimulationStep (simulationRoutines.py:295-526) with
  * its copies folded in.  Sequence: mpmx_xpbd_begin; per iteration
  * mpmx_xpbd_bound_contacts_write, mpmx_solve_particle_particle_contacts[_split],
- * mpmx_xpbd_apply_deltas_inplace; then mpmx_xpbd_finish (counters zeroed first).
+ * mpmx_xpbd_apply_deltas_inplace; then mpmx_xpbd_finish (its two counters are
+ * zeroed by mpmx_xpbd_begin when it is given them, else by the caller).
  * x, v, labels, the cumulative distance and the counters end as after the launch
  * sequence; the scratch arrays particle_x_deltaInt/particle_v_deltaInt/
  * particle_v_before_contact are not written. */
 /* x_int, v_int <- x, v then integrateParticlesXPBD (xpbdRoutines.py:23-48); with
- * reset_labels also reset_mpm_contact_labels (:3-21) */
+ * reset_labels also reset_mpm_contact_labels (:3-21); counters[0..1] (the sleep
+ * counts of mpmx_xpbd_finish, simulationRoutines.py:424-425) are zeroed when
+ * counters is not NULL */
 int mpmx_xpbd_begin(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x,
                     const float* v, float gxv, float gyv, float gzv, float dt, float v_max,
-                    int reset_labels, float* x_int, float* v_int, void* stream);
+                    int reset_labels, float* x_int, float* v_int, int32_t* counters,
+                    void* stream);
 /* particle_delta.zero_() + my_solve_particle_bound_contacts (xpbdRoutines.py:49-136):
  * writes every particle's delta */
 int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel,

@@ -97,7 +97,7 @@ SIGNATURES = {
     "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
     "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],
     "mpmx_swell_particles_type2": [I, P, P, P, P, P, F, F, P, P, P],
-    "mpmx_xpbd_begin": [I, P, P, P, P, F, F, F, F, F, I, P, P, P],
+    "mpmx_xpbd_begin": [I, P, P, P, P, F, F, F, F, F, I, P, P, P, P],
     "mpmx_xpbd_bound_contacts_write": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
     "mpmx_xpbd_apply_deltas_inplace": [I, P, P, P, F, F, P, P, P],
     "mpmx_xpbd_finish": [I, P, P, F, P, P, F, F, F, I, P, P, P, P, P, P],

@@ -626,8 +626,9 @@ namespace mpmx {
 __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int32_t* __restrict__ mat,
                                   const float* __restrict__ x, const float* __restrict__ v, float gx, float gy,
                                   float gz, float dt, float v_max, int reset_labels, float* __restrict__ x_int,
-                                  float* __restrict__ v_int) {
+                                  float* __restrict__ v_int, int32_t* __restrict__ counters) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (counters && p < 2) counters[p] = 0;  // for xpbd_finish's sleep counts
     if (p >= n) return;
     f3 x0 = ld3(x, n, p), v0 = ld3(v, n, p);
     const int a = active[p], m = mat[p];
@@ -761,12 +762,12 @@ extern "C" {
 
 int mpmx_xpbd_begin(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x, const float* v,
                     float gxv, float gyv, float gzv, float dt, float v_max, int reset_labels, float* x_int,
-                    float* v_int, void* stream) {
+                    float* v_int, int32_t* counters, void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && x_int && v_int)),
                  "mpmx_xpbd_begin: bad args");
-    if (n == 0) return 0;
-    LAUNCH1(xpbd_begin_kernel, n, n, activeLabel, materialLabel, x, v, gxv, gyv, gzv, dt, v_max, reset_labels, x_int,
-            v_int)
+    const int threads = n > 2 ? n : 2;  // the counters are cleared even with no particle
+    LAUNCH1(xpbd_begin_kernel, threads, n, activeLabel, materialLabel, x, v, gxv, gyv, gzv, dt, v_max, reset_labels,
+            x_int, v_int, counters)
 }
 
 int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x,

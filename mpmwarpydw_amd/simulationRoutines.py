@@ -303,7 +303,7 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     xi, vi = xpbd.particle_x_integrated, xpbd.particle_v_integrated
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
         float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
-        s)
+        sim.numTotalXPBD.ptr, s)  # also zeroes numTotalXPBD and numSleepingXPBD (adjacent)
     contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
     mn, mx = xpbd.minBoundsXPBD, xpbd.maxBoundsXPBD
     grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
@@ -322,8 +322,6 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
             _aptr(getattr(sim, "_slot_of_id", None)), s)
         lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
             float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, s)
-    sim.numTotalXPBD.t.zero_()
-    sim.numSleepingXPBD.t.zero_()
     lib("mpmx_xpbd_finish", n, sim.activeLabel.ptr, sim.materialLabel.ptr, float(xpbd.sleepThreshold),
         sim.particle_radius.ptr, xpbd.particle_cumDist_xpbd.ptr, float(sim.dtxpbd), float(xpbd.particle_v_max),
         float(xpbd.xpbd_mpm_coupling_strength), 1 if xpbd_only else 0, sim.particle_x.ptr, sim.particle_v.ptr,
