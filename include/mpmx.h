@@ -163,11 +163,12 @@ int mpmx_graph_end(mpmx_ctx* ctx, mpmx_graph** out);
 int mpmx_graph_launch(mpmx_graph* graph, void* stream);
 int mpmx_graph_destroy(mpmx_graph* graph);
 int mpmx_version(void);
-/* Layout of the structs above as this build sees them, for bindings to check
- * theirs against: out[0..] = sizeof(mpmx_particles), offsetof(mpmx_particles,
- * order), sizeof(mpmx_grid), sizeof(mpmx_step_params), offsetof(
- * mpmx_step_params, bc), offsetof(mpmx_particles, scene), sizeof(mpmx_scene);
- * returns the number of values written (<= cap). */
+/* Layout of the structs of this header as this build sees them, for bindings
+ * to check theirs against: out[0..] = sizeof(mpmx_particles), offsetof(
+ * mpmx_particles, order), sizeof(mpmx_grid), sizeof(mpmx_step_params),
+ * offsetof(mpmx_step_params, bc), offsetof(mpmx_particles, scene),
+ * sizeof(mpmx_scene), sizeof(mpmx_xpbd_bounds), offsetof(mpmx_xpbd_bounds,
+ * delta); returns the number of values written (<= cap). */
 int mpmx_struct_layout(long* out, int cap);
 /* Launch profiler (no counterpart in the reference; replaces its wall-clock
  * prints, runMPMYDW.py:762): between begin and end, every kernel the library
@@ -501,14 +502,27 @@ int mpmx_apply_mpm_contact_impulses(int n, const int32_t* activeLabel,
  * x, v, labels, the cumulative distance and the counters end as after the launch
  * sequence; the scratch arrays particle_x_deltaInt/particle_v_deltaInt/
  * particle_v_before_contact are not written. */
+/* The parameters of my_solve_particle_bound_contacts (xpbdRoutines.py:49-136)
+ * and the delta array it writes, for the kernels below that fold it in. */
+typedef struct {
+    const float* mass;
+    const float* radius;
+    float mu, static_threshold, static_friction;
+    float minx, miny, minz, maxx, maxy, maxz; /* the XPBD box */
+    float dt, relaxation;
+    float* delta; /* [3][n]: every particle's delta is written */
+} mpmx_xpbd_bounds;
+
 /* x_int, v_int <- x, v then integrateParticlesXPBD (xpbdRoutines.py:23-48); with
  * reset_labels also reset_mpm_contact_labels (:3-21); counters[0..1] (the sleep
  * counts of mpmx_xpbd_finish, simulationRoutines.py:424-425) are zeroed when
- * counters is not NULL */
+ * counters is not NULL.  With bounds (may be NULL), also the first iteration's
+ * particle_delta.zero_() + my_solve_particle_bound_contacts from the new
+ * x_int / v_int, as mpmx_xpbd_bound_contacts_write would give. */
 int mpmx_xpbd_begin(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x,
                     const float* v, float gxv, float gyv, float gzv, float dt, float v_max,
                     int reset_labels, float* x_int, float* v_int, int32_t* counters,
-                    void* stream);
+                    const mpmx_xpbd_bounds* bounds, void* stream);
 /* particle_delta.zero_() + my_solve_particle_bound_contacts (xpbdRoutines.py:49-136):
  * writes every particle's delta */
 int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel,
@@ -518,16 +532,21 @@ int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel,
                                    float miny, float minz, float maxx, float maxy, float maxz,
                                    float dt, float relaxation, float* delta, void* stream);
 /* the four copies and my_apply_particle_deltas (xpbdRoutines.py:221-257) as one
- * in-place update of x_int / v_int */
+ * in-place update of x_int / v_int.  With next_bounds (may be NULL; it then
+ * needs materialLabel), also the next iteration's bound contacts from the new
+ * x_int / v_int into next_bounds->delta (which may be delta itself). */
 int mpmx_xpbd_apply_deltas_inplace(int n, const int32_t* activeLabel, const float* x_orig,
                                    const float* delta, float dt, float v_max, float* x_int,
-                                   float* v_int, void* stream);
+                                   float* v_int, const int32_t* materialLabel,
+                                   const mpmx_xpbd_bounds* next_bounds, void* stream);
 /* sleepParticles (xpbdRoutines.py:313-334), apply_mpm_contact_impulses (:259-310)
- * and the commit (simulationRoutines.py:469-470); counters = {total, sleeping} */
+ * and the commit (simulationRoutines.py:469-470); counters = {total, sleeping}.
+ * With delta (may be NULL), the last iteration's my_apply_particle_deltas
+ * (x_orig = x, the same dt and v_max) comes first. */
 int mpmx_xpbd_finish(int n, const int32_t* activeLabel, const int32_t* materialLabel,
                      float sleep_threshold, const float* radius, float* dist_total, float dt,
                      float v_max, float coupling, int xpbd_only, float* x, float* v, float* x_int,
-                     float* v_int, int32_t* counters, void* stream);
+                     float* v_int, int32_t* counters, const float* delta, void* stream);
 
 /* swellParticlesType2, xpbdRoutines.py:354-376 */
 int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t* materialLabel,

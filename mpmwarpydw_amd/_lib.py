@@ -97,10 +97,10 @@ SIGNATURES = {
     "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
     "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],
     "mpmx_swell_particles_type2": [I, P, P, P, P, P, F, F, P, P, P],
-    "mpmx_xpbd_begin": [I, P, P, P, P, F, F, F, F, F, I, P, P, P, P],
+    "mpmx_xpbd_begin": [I, P, P, P, P, F, F, F, F, F, I, P, P, P, P, P],
     "mpmx_xpbd_bound_contacts_write": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],
-    "mpmx_xpbd_apply_deltas_inplace": [I, P, P, P, F, F, P, P, P],
-    "mpmx_xpbd_finish": [I, P, P, F, P, P, F, F, F, I, P, P, P, P, P, P],
+    "mpmx_xpbd_apply_deltas_inplace": [I, P, P, P, F, F, P, P, P, P, P],
+    "mpmx_xpbd_finish": [I, P, P, F, P, P, F, F, F, I, P, P, P, P, P, P, P],
     "mpmx_mlp_nparams": [],
     "mpmx_mlp_forward": [I, P, P, P, P],
     "mpmx_mlp_workspace_bytes": [I],
@@ -162,6 +162,13 @@ class StepParams(_Strict):
         "dt", "dx", "inv_dx", "minx", "miny", "minz", "gravity_x", "gravity_y", "gravity_z", "rpic_damping", "eff",
         "damping_scale", "friction", "gravity_magnitude")] + [(name, C.c_int) for name in (
         "bc", "boundary_padding", "update_cov", "constitutive_model")] + [("restitution", C.c_float), ("vlc", C.c_int)]
+
+
+class XpbdBounds(_Strict):
+    """mpmx_xpbd_bounds (include/mpmx.h)"""
+    _fields_ = [("mass", C.c_void_p), ("radius", C.c_void_p)] + [(name, C.c_float) for name in (
+        "mu", "static_threshold", "static_friction", "minx", "miny", "minz", "maxx", "maxy", "maxz", "dt",
+        "relaxation")] + [("delta", C.c_void_p)]
 
 
 class MpmxError(RuntimeError):

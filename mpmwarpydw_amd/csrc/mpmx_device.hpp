@@ -38,7 +38,10 @@ MPMX_HD void st9(float* __restrict__ a, long n, long i, const float m[9]) {
 #pragma unroll
     for (int k = 0; k < 9; k++) a[k * n + i] = m[k];
 }
-MPMX_HD float len3(f3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+// |v| with the squares summed as fma(z, z, fma(y, y, x x)) wherever it is inlined:
+// left to the compiler, whether the sum is contracted depends on the code around
+// it, and kernels that split the same work differently would not agree bit for bit
+MPMX_HD float len3(f3 v) { return sqrtf(fmaf(v.z, v.z, fmaf(v.y, v.y, v.x * v.x))); }
 
 // Warp's normalize(): the zero vector stays zero, otherwise divide by length.
 MPMX_HD f3 normalize3(f3 v) {

@@ -607,11 +607,12 @@ int mpmx_set_tuning(int knob, int value) {
 int mpmx_get_tuning(int knob) { return tuning(knob); }
 
 int mpmx_struct_layout(long* out, int cap) {
-    const long v[7] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order), (long)sizeof(mpmx_grid),
+    const long v[9] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order), (long)sizeof(mpmx_grid),
                        (long)sizeof(mpmx_step_params), (long)offsetof(mpmx_step_params, bc),
-                       (long)offsetof(mpmx_particles, scene), (long)sizeof(mpmx_scene)};
+                       (long)offsetof(mpmx_particles, scene), (long)sizeof(mpmx_scene),
+                       (long)sizeof(mpmx_xpbd_bounds), (long)offsetof(mpmx_xpbd_bounds, delta)};
     int k = 0;
-    for (; out && k < cap && k < 7; k++) out[k] = v[k];
+    for (; out && k < cap && k < 9; k++) out[k] = v[k];
     return k;
 }
 

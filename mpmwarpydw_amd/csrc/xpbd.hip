@@ -191,7 +191,7 @@ __global__ void bound_contacts_kernel(int n, const int32_t* __restrict__ active,
                 float fmu = (fabsf(vn) < thr * r) ? smu : mu;
                 float vt[3] = {va[0], va[1], va[2]};
                 vt[ax] = va[ax] - sg * vn;
-                float vtl = sqrtf(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+                float vtl = len3(f3{vt[0], vt[1], vt[2]});
                 float a_ = fmu * lambda_n, b_ = -vtl * dt;
                 float lf = a_ > b_ ? a_ : b_;  // wp.max
                 float df[3] = {0.0f, 0.0f, 0.0f};
@@ -621,12 +621,76 @@ int mpmx_swell_particles_type2(int n, const int32_t* activeLabel, const int32_t*
 // ===========================================================================
 namespace mpmx {
 
+// the particle-wall delta of my_solve_particle_bound_contacts (xpbdRoutines.py:49-136)
+// for particle t at (xp, vp); zero unless it is an active XPBD particle with mass
+__device__ __forceinline__ f3 bound_delta(const mpmx_xpbd_bounds& B, int n, int t, int a, int m, f3 xp, f3 vp) {
+    float da[3] = {0.0f, 0.0f, 0.0f};
+    if (a == 1 && m == 2 && B.mass[t] != 0.0f) {
+        const float wi = 1.0f / B.mass[t];
+        const float r = B.radius[t];
+        const float xa[3] = {xp.x, xp.y, xp.z}, va[3] = {vp.x, vp.y, vp.z};
+        const float lo[3] = {B.minx, B.miny, B.minz}, hi[3] = {B.maxx, B.maxy, B.maxz};
+        const float thr = B.static_threshold, dt = B.dt, relax = B.relaxation;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                float sg = side == 0 ? 1.0f : -1.0f;
+                float b = side == 0 ? lo[ax] : hi[ax];
+                float c = fminf(sg * (xa[ax] - b) - r, 0.0f);
+                if (c < 0.0f) {
+                    float lambda_n = c;
+                    float vn = sg * va[ax];
+                    float fmu = (fabsf(vn) < thr * r) ? B.static_friction : B.mu;
+                    float vt[3] = {va[0], va[1], va[2]};
+                    vt[ax] = va[ax] - sg * vn;
+                    float vtl = len3(f3{vt[0], vt[1], vt[2]});
+                    float a_ = fmu * lambda_n, b_ = -vtl * dt;
+                    float lf = a_ > b_ ? a_ : b_;  // wp.max
+                    float df[3] = {0.0f, 0.0f, 0.0f};
+                    if (vtl > 1.0e-8f) {
+                        f3 nt = normalize3(f3{vt[0], vt[1], vt[2]});
+                        df[0] = nt.x * lf; df[1] = nt.y * lf; df[2] = nt.z * lf;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        float dn = (q == ax) ? sg * lambda_n : 0.0f;
+                        da[q] += (df[q] - dn) / wi * relax * wi;
+                    }
+                }
+            }
+        }
+    }
+    return f3{da[0], da[1], da[2]};
+}
+
+// v as a kernel that loads it from memory sees it: the empty asm hides how it was
+// computed, so no multiply before it is contracted with an add after it (a fused
+// multiply-add would round once where the stored value rounded twice)
+__device__ __forceinline__ f3 as_stored(f3 v) {
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z));
+    return v;
+}
+
+// my_apply_particle_deltas (xpbdRoutines.py:221-257) of one particle
+__device__ __forceinline__ void apply_delta(f3 x0, f3 xp, f3 d, float dt, float v_max, f3& xn, f3& vn) {
+    if (isnan(d.x) || isnan(d.y) || isnan(d.z)) d = f3{0.0f, 0.0f, 0.0f};
+    xn = f3{xp.x + d.x, xp.y + d.y, xp.z + d.z};
+    vn = f3{(xn.x - x0.x) / dt, (xn.y - x0.y) / dt, (xn.z - x0.z) / dt};
+    float m = len3(vn);
+    if (m > v_max) {
+        float s = v_max / m;
+        vn = f3{vn.x * s, vn.y * s, vn.z * s};
+    }
+}
+
 // x_int, v_int <- x, v; integrateParticlesXPBD on active XPBD particles
 // (xpbdRoutines.py:23-48); reset_mpm_contact_labels when lock (:3-21)
 __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int32_t* __restrict__ mat,
                                   const float* __restrict__ x, const float* __restrict__ v, float gx, float gy,
                                   float gz, float dt, float v_max, int reset_labels, float* __restrict__ x_int,
-                                  float* __restrict__ v_int, int32_t* __restrict__ counters) {
+                                  float* __restrict__ v_int, int32_t* __restrict__ counters, int bound,
+                                  mpmx_xpbd_bounds B) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (counters && p < 2) counters[p] = 0;  // for xpbd_finish's sleep counts
     if (p >= n) return;
@@ -645,75 +709,47 @@ __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int
     st3(x_int, n, p, x0);
     st3(v_int, n, p, v0);
     if (reset_labels && a == 1 && m == 0) mat[p] = 1;
+    // the first iteration's particle-wall deltas (the label reset never makes an XPBD particle)
+    if (bound)
+        st3(B.delta, n, p,
+            (a == 1 && m == 2) ? bound_delta(B, n, p, a, m, as_stored(x0), as_stored(v0)) : f3{0.0f, 0.0f, 0.0f});
 }
 
 // my_solve_particle_bound_contacts writing (not adding) the delta of every
 // particle: 0 where the reference leaves the zeroed delta untouched
 __global__ void bound_contacts_write_kernel(int n, const int32_t* __restrict__ active,
                                             const int32_t* __restrict__ mat, const float* __restrict__ x,
-                                            const float* __restrict__ v, const float* __restrict__ mass,
-                                            const float* __restrict__ radius, float mu, float thr, float smu, f3 bmin,
-                                            f3 bmax, float dt, float relax, float* __restrict__ delta) {
+                                            const float* __restrict__ v, mpmx_xpbd_bounds B) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    float da[3] = {0.0f, 0.0f, 0.0f};
-    if (active[t] == 1 && mat[t] == 2 && mass[t] != 0.0f) {
-        float wi = 1.0f / mass[t];
-        f3 xp = ld3(x, n, t), vp = ld3(v, n, t);
-        float r = radius[t];
-        const float xa[3] = {xp.x, xp.y, xp.z}, va[3] = {vp.x, vp.y, vp.z};
-        const float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-#pragma unroll
-            for (int side = 0; side < 2; side++) {
-                float sg = side == 0 ? 1.0f : -1.0f;
-                float b = side == 0 ? lo[ax] : hi[ax];
-                float c = fminf(sg * (xa[ax] - b) - r, 0.0f);
-                if (c < 0.0f) {
-                    float lambda_n = c;
-                    float vn = sg * va[ax];
-                    float fmu = (fabsf(vn) < thr * r) ? smu : mu;
-                    float vt[3] = {va[0], va[1], va[2]};
-                    vt[ax] = va[ax] - sg * vn;
-                    float vtl = sqrtf(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
-                    float a_ = fmu * lambda_n, b_ = -vtl * dt;
-                    float lf = a_ > b_ ? a_ : b_;  // wp.max
-                    float df[3] = {0.0f, 0.0f, 0.0f};
-                    if (vtl > 1.0e-8f) {
-                        f3 nt = normalize3(f3{vt[0], vt[1], vt[2]});
-                        df[0] = nt.x * lf; df[1] = nt.y * lf; df[2] = nt.z * lf;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        float dn = (q == ax) ? sg * lambda_n : 0.0f;
-                        da[q] += (df[q] - dn) / wi * relax * wi;
-                    }
-                }
-            }
-        }
-    }
-    st3(delta, n, t, f3{da[0], da[1], da[2]});
+    const int a = active[t], m = mat[t];
+    f3 d{0.0f, 0.0f, 0.0f};
+    if (a == 1 && m == 2) d = bound_delta(B, n, t, a, m, ld3(x, n, t), ld3(v, n, t));
+    st3(B.delta, n, t, d);
 }
 
-// my_apply_particle_deltas (xpbdRoutines.py:221-257) in place on x_int / v_int
-__global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ active, const float* __restrict__ x_orig,
-                                            const float* __restrict__ delta, float dt, float v_max,
-                                            float* __restrict__ x_int, float* __restrict__ v_int) {
+// my_apply_particle_deltas (xpbdRoutines.py:221-257) in place on x_int / v_int; with
+// bound, then the next iteration's particle-wall deltas from the new x_int / v_int
+// (each particle's bound delta depends on that particle alone)
+__global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                                            const float* __restrict__ x_orig, const float* delta, float dt,
+                                            float v_max, float* __restrict__ x_int, float* __restrict__ v_int,
+                                            int bound, mpmx_xpbd_bounds B) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    if (active[t] != 1) return;
-    f3 x0 = ld3(x_orig, n, t), xp = ld3(x_int, n, t), d = ld3(delta, n, t);
-    if (isnan(d.x) || isnan(d.y) || isnan(d.z)) d = f3{0.0f, 0.0f, 0.0f};
-    f3 xn = f3{xp.x + d.x, xp.y + d.y, xp.z + d.z};
-    f3 vn = f3{(xn.x - x0.x) / dt, (xn.y - x0.y) / dt, (xn.z - x0.z) / dt};
-    float m = len3(vn);
-    if (m > v_max) {
-        float s = v_max / m;
-        vn = f3{vn.x * s, vn.y * s, vn.z * s};
+    const int a = active[t];
+    f3 nd{0.0f, 0.0f, 0.0f};
+    if (a == 1) {
+        f3 xn, vn;
+        apply_delta(ld3(x_orig, n, t), ld3(x_int, n, t), ld3(delta, n, t), dt, v_max, xn, vn);
+        st3(x_int, n, t, xn);
+        st3(v_int, n, t, vn);
+        if (bound) {
+            const int m = mat[t];
+            if (m == 2) nd = bound_delta(B, n, t, a, m, as_stored(xn), as_stored(vn));
+        }
     }
-    st3(x_int, n, t, xn);
-    st3(v_int, n, t, vn);
+    if (bound) st3(B.delta, n, t, nd);
 }
 
 // sleepParticles (xpbdRoutines.py:313-334), apply_mpm_contact_impulses (:259-310)
@@ -722,12 +758,20 @@ __global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, co
                                    float thr, const float* __restrict__ radius, float* __restrict__ dist, float dt,
                                    float v_max, float k, int xpbd_only, float* __restrict__ x, float* __restrict__ v,
                                    float* __restrict__ x_int, float* __restrict__ v_int,
-                                   int32_t* __restrict__ counters) {
+                                   int32_t* __restrict__ counters, const float* __restrict__ delta) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    f3 xi = ld3(x_int, n, t), vi = ld3(v_int, n, t);
+    f3 xi = ld3(x_int, n, t), vi;
     const f3 x0 = ld3(x, n, t);
     const int a = active[t], m = mat[t];
+    if (delta && a == 1) {
+        // the last iteration's my_apply_particle_deltas (x_orig is x)
+        apply_delta(x0, xi, ld3(delta, n, t), dt, v_max, xi, vi);
+        xi = as_stored(xi);
+        vi = as_stored(vi);
+    } else {
+        vi = ld3(v_int, n, t);
+    }
     if (a == 1 && m == 2) {
         atomicAdd(&counters[0], 1);
         float d = len3(f3{xi.x - x0.x, xi.y - x0.y, xi.z - x0.z});
@@ -760,14 +804,25 @@ __global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, co
 
 extern "C" {
 
+static int check_bounds(int n, const mpmx_xpbd_bounds* B, const char* who) {
+    MPMX_REQUIRE(n == 0 || (B->mass && B->radius && B->delta), "%s: null array in the bounds", who);
+    return 0;
+}
+
 int mpmx_xpbd_begin(int n, const int32_t* activeLabel, int32_t* materialLabel, const float* x, const float* v,
                     float gxv, float gyv, float gzv, float dt, float v_max, int reset_labels, float* x_int,
-                    float* v_int, int32_t* counters, void* stream) {
+                    float* v_int, int32_t* counters, const mpmx_xpbd_bounds* bounds, void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && x_int && v_int)),
                  "mpmx_xpbd_begin: bad args");
+    mpmx_xpbd_bounds B{};
+    if (bounds) {
+        int rc = check_bounds(n, bounds, "mpmx_xpbd_begin");
+        if (rc) return rc;
+        B = *bounds;
+    }
     const int threads = n > 2 ? n : 2;  // the counters are cleared even with no particle
     LAUNCH1(xpbd_begin_kernel, threads, n, activeLabel, materialLabel, x, v, gxv, gyv, gzv, dt, v_max, reset_labels,
-            x_int, v_int, counters)
+            x_int, v_int, counters, bounds ? 1 : 0, B)
 }
 
 int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x,
@@ -778,27 +833,38 @@ int mpmx_xpbd_bound_contacts_write(int n, const int32_t* activeLabel, const int3
     MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && x && v && mass && radius && delta)),
                  "mpmx_xpbd_bound_contacts_write: bad args");
     if (n == 0) return 0;
-    LAUNCH1(bound_contacts_write_kernel, n, n, activeLabel, materialLabel, x, v, mass, radius, mu, static_threshold,
-            static_friction, f3{minx, miny, minz}, f3{maxx, maxy, maxz}, dt, relaxation, delta)
+    const mpmx_xpbd_bounds B{mass, radius, mu, static_threshold, static_friction, minx, miny, minz,
+                             maxx, maxy, maxz, dt, relaxation, delta};
+    LAUNCH1(bound_contacts_write_kernel, n, n, activeLabel, materialLabel, x, v, B)
 }
 
 int mpmx_xpbd_apply_deltas_inplace(int n, const int32_t* activeLabel, const float* x_orig, const float* delta,
-                                   float dt, float v_max, float* x_int, float* v_int, void* stream) {
+                                   float dt, float v_max, float* x_int, float* v_int, const int32_t* materialLabel,
+                                   const mpmx_xpbd_bounds* next_bounds, void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && x_orig && delta && x_int && v_int)),
                  "mpmx_xpbd_apply_deltas_inplace: bad args");
     if (n == 0) return 0;
-    LAUNCH1(apply_deltas_inplace_kernel, n, n, activeLabel, x_orig, delta, dt, v_max, x_int, v_int)
+    mpmx_xpbd_bounds B{};
+    if (next_bounds) {
+        MPMX_REQUIRE(materialLabel, "mpmx_xpbd_apply_deltas_inplace: the bounds need materialLabel");
+        int rc = check_bounds(n, next_bounds, "mpmx_xpbd_apply_deltas_inplace");
+        if (rc) return rc;
+        B = *next_bounds;
+    }
+    LAUNCH1(apply_deltas_inplace_kernel, n, n, activeLabel, materialLabel, x_orig, delta, dt, v_max, x_int, v_int,
+            next_bounds ? 1 : 0, B)
 }
 
 int mpmx_xpbd_finish(int n, const int32_t* activeLabel, const int32_t* materialLabel, float sleep_threshold,
                      const float* radius, float* dist_total, float dt, float v_max, float coupling, int xpbd_only,
-                     float* x, float* v, float* x_int, float* v_int, int32_t* counters, void* stream) {
+                     float* x, float* v, float* x_int, float* v_int, int32_t* counters, const float* delta,
+                     void* stream) {
     MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && radius && dist_total && x && v && x_int &&
                                        v_int && counters)),
                  "mpmx_xpbd_finish: bad args");
     if (n == 0) return 0;
     LAUNCH1(xpbd_finish_kernel, n, n, activeLabel, materialLabel, sleep_threshold, radius, dist_total, dt, v_max,
-            coupling, xpbd_only, x, v, x_int, v_int, counters)
+            coupling, xpbd_only, x, v, x_int, v_int, counters, delta)
 }
 
 }  // extern "C"

@@ -16,7 +16,7 @@ import torch
 
 from . import mpmRoutines, xpbdRoutines
 from . import runtime as rt
-from ._lib import MpmxError, lib
+from ._lib import MpmxError, XpbdBounds, lib
 from .runtime import Kernel, Workspace, launch, stream_handle
 
 def _bc_code(mpm):
@@ -301,18 +301,24 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     s = stream_handle()
     reset = 1 if (xpbd.mpm_contact_transition_lock and not xpbd_only) else 0
     xi, vi = xpbd.particle_x_integrated, xpbd.particle_v_integrated
+    mn, mx = xpbd.minBoundsXPBD, xpbd.maxBoundsXPBD
+    iters = int(xpbd.xpbd_iterations)
+    # each iteration's particle-wall deltas are folded into the kernel before it (begin, or
+    # the previous iteration's delta application) and the last application into finish
+    bounds = XpbdBounds(mass=sim.particle_mass.ptr.value, radius=sim.particle_radius.ptr.value,
+                        mu=float(xpbd.dynamicParticleFriction), static_threshold=float(xpbd.staticVelocityThreshold),
+                        static_friction=float(xpbd.staticParticleFriction), minx=float(mn[0]), miny=float(mn[1]),
+                        minz=float(mn[2]), maxx=float(mx[0]), maxy=float(mx[1]), maxz=float(mx[2]),
+                        dt=float(sim.dtxpbd), relaxation=float(xpbd.xpbd_relaxation),
+                        delta=xpbd.particle_delta.ptr.value)
+    bref = C.byref(bounds)
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
         float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
-        sim.numTotalXPBD.ptr, s)  # also zeroes numTotalXPBD and numSleepingXPBD (adjacent)
+        sim.numTotalXPBD.ptr, bref if iters > 0 else None, s)  # also zeroes numTotalXPBD and numSleepingXPBD
     contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
-    mn, mx = xpbd.minBoundsXPBD, xpbd.maxBoundsXPBD
     grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
     dims = grid.dims
-    for _ in range(xpbd.xpbd_iterations):
-        lib("mpmx_xpbd_bound_contacts_write", n, sim.activeLabel.ptr, sim.materialLabel.ptr, xi.ptr, vi.ptr,
-            sim.particle_mass.ptr, sim.particle_radius.ptr, float(xpbd.dynamicParticleFriction),
-            float(xpbd.staticVelocityThreshold), float(xpbd.staticParticleFriction), *[float(v) for v in mn],
-            *[float(v) for v in mx], float(sim.dtxpbd), float(xpbd.xpbd_relaxation), xpbd.particle_delta.ptr, s)
+    for it in range(iters):
         lib("mpmx_solve_particle_particle_contacts_split", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
             grid.point_ids.ptr, grid.cell_start.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr,
             float(grid.cell_width), dims[0], dims[1], dims[2], xi.ptr, vi.ptr, sim.particle_mass.ptr,
@@ -320,12 +326,13 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
             float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
             float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock), xpbd.particle_delta.ptr,
             _aptr(getattr(sim, "_slot_of_id", None)), s)
-        lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
-            float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, s)
+        if it + 1 < iters:
+            lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
+                float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, sim.materialLabel.ptr, bref, s)
     lib("mpmx_xpbd_finish", n, sim.activeLabel.ptr, sim.materialLabel.ptr, float(xpbd.sleepThreshold),
         sim.particle_radius.ptr, xpbd.particle_cumDist_xpbd.ptr, float(sim.dtxpbd), float(xpbd.particle_v_max),
         float(xpbd.xpbd_mpm_coupling_strength), 1 if xpbd_only else 0, sim.particle_x.ptr, sim.particle_v.ptr,
-        xi.ptr, vi.ptr, sim.numTotalXPBD.ptr, s)
+        xi.ptr, vi.ptr, sim.numTotalXPBD.ptr, xpbd.particle_delta.ptr if iters > 0 else None, s)
     if xpbd.swellingRatio > 0:
         launch(kernel=xpbdRoutines.swellParticlesType2, dim=n,
                inputs=[sim.activeLabel, sim.materialLabel, xpbd.particle_x_initial_xpbd, sim.particle_x,

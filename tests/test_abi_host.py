@@ -38,12 +38,13 @@ def test_ctypes_structs_match_the_library_layout():
     by pointer that is shorter than the C one makes the library read past it),
     and unknown field names are refused."""
     import ctypes as C
-    from mpmwarpydw_amd._lib import Grid, Particles, Scene, StepParams, lib
-    out = (C.c_long * 8)()
-    k = lib.raw("mpmx_struct_layout")(out, 8)
-    assert k == 7
-    assert list(out[:7]) == [C.sizeof(Particles), Particles.order.offset, C.sizeof(Grid), C.sizeof(StepParams),
-                             StepParams.bc.offset, Particles.scene.offset, C.sizeof(Scene)]
+    from mpmwarpydw_amd._lib import Grid, Particles, Scene, StepParams, XpbdBounds, lib
+    out = (C.c_long * 10)()
+    k = lib.raw("mpmx_struct_layout")(out, 10)
+    assert k == 9
+    assert list(out[:9]) == [C.sizeof(Particles), Particles.order.offset, C.sizeof(Grid), C.sizeof(StepParams),
+                             StepParams.bc.offset, Particles.scene.offset, C.sizeof(Scene), C.sizeof(XpbdBounds),
+                             XpbdBounds.delta.offset]
     with pytest.raises(TypeError):
         Particles(n=1, not_a_field=1)
 
