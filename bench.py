@@ -43,16 +43,17 @@ HBM_PEAK_GBS = 8000.0
 #     F_trial 36, C 36, nine material floats 36, accumulated strain 4, damage 4,
 #     label 4, x 12, v 12, vol 4, mass 4, bin id 4; writes F 36, stress 36,
 #     accumulated strain / damage / ys 12
-#   substep_stress, the constitutive update alone: reads F_trial 36, C 36, nine
-#     material floats 36, accumulated strain 4, damage 4, two labels 8; writes
-#     F 36, stress 36, accumulated strain / damage / ys 12
+#   substep_stress, the constitutive update alone: reads F_trial 36, nine
+#     material floats 36, accumulated strain 4, damage 4, two labels 8 (and C 36
+#     when a viscosity is nonzero; the scenes here have none); writes F 36,
+#     stress 36, accumulated strain / damage / ys 12
 #   substep_tile(p2g), the P2G tile pass after it: reads bin id 4, label 4, x 12,
 #     v 12, C 36, vol 4, mass 4, stress 24 (symmetric: 6 of its 9 planes)
 #   substep_g2p: reads bin id 4, label 4, x 12, F 36; writes x 12, v 12, C 36,
 #     F_trial 36, next bin key 4, rank 4
 KERNEL_BYTES_PER_PARTICLE = {
     "substep_tile(stress+p2g)": 156.0 + 84.0,
-    "substep_stress": 124.0 + 84.0,
+    "substep_stress": 88.0 + 84.0,
     "substep_tile(p2g)": 100.0,
     "substep_g2p": 56.0 + 104.0,
 }

@@ -275,11 +275,20 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
 
     // plus Kelvin-Voigt with the APIC C as grad v (mpmRoutines.py:294-310)
     float G[9];
-    ld9(P.C, n, p, G);
-    const float s00 = G[0], s11 = G[4], s22 = G[8];
-    const float s01 = 0.5f * (G[1] + G[3]), s02 = 0.5f * (G[2] + G[6]), s12 = 0.5f * (G[5] + G[7]);
-    const float vr = s00 + s11 + s22;
     const float es = 2.0f * P.eta_shear[p], eb = P.eta_bulk[p];
+    float s00 = 0.0f, s11 = 0.0f, s22 = 0.0f, s01 = 0.0f, s02 = 0.0f, s12 = 0.0f;
+    if (es != 0.0f || eb != 0.0f) {
+        // with both viscosities zero the viscous term is zero for any finite C, and C
+        // (36 bytes per particle) is not read
+        ld9(P.C, n, p, G);
+        s00 = G[0];
+        s11 = G[4];
+        s22 = G[8];
+        s01 = 0.5f * (G[1] + G[3]);
+        s02 = 0.5f * (G[2] + G[6]);
+        s12 = 0.5f * (G[5] + G[7]);
+    }
+    const float vr = s00 + s11 + s22;
     const float third = vr / 3.0f;
     T[0] = tau[0] + (es * (s00 - third) + eb * vr);
     T[4] = tau[1] + (es * (s11 - third) + eb * vr);

@@ -49,6 +49,27 @@ def test_stress_return_map_parity(o64, seed):
     assert (G["materialLabel"] == 2).sum() > (host_state(make_sim(seed=seed))["materialLabel"] == 2).sum()
 
 
+def test_fused_substep_mixed_viscosity(o64):
+    """Viscosities zero on some particles (the update then skips reading C) and
+    not on others (shear only, bulk only, both), in one fused substep."""
+    from mpmwarpydw_amd import simulationRoutines as SR
+    S = make_sim(seed=12)
+    n = S.sim.nPoints
+    kind = np.arange(n) % 4
+    S.mpm.eta_shear.assign(np.where((kind == 1) | (kind == 3), 50.0, 0.0))
+    S.mpm.eta_bulk.assign(np.where((kind == 2) | (kind == 3), 80.0, 0.0))
+    H = host_state(S)
+    print(f"[mixed viscosity] n={n}", flush=True)
+    SR.mpmSimulationStepFused(S.sim, S.mpm, S.xpbd)
+    G = host_state(S)
+    oracle_stress(o64, H, S.mpm.eff)
+    oracle_p2g(o64, H, S)
+    oracle_grid(o64, H, S)
+    oracle_g2p(o64, H, S)
+    for k in ("particle_stress", "particle_F", "grid_v_out", "particle_v", "particle_x", "particle_C"):
+        assert rel_err(G[k], H[k]) < TOL, k
+
+
 def test_stress_small_strain_accuracy(o64):
     """Strains of 1e-7 (first substeps from rest) keep 1e-5 relative accuracy
     versus FP64: the SVD works on F^T F - I formed from F - I."""
