@@ -423,16 +423,19 @@ int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dim
                                 void* workspace, size_t ws_bytes, void* stream);
 
 /* The table of all points (simulationRoutines.py:311, the grid of the XPBD
- * step) and the table of the points whose materialLabel equals `label` (the
- * split contact solve's), in one set of launches: the same contents as
- * mpmx_hashgrid_build_ordered called once with materialLabel NULL and once with
- * it (order may be NULL).  Workspace: mpmx_hashgrid_pair_workspace_bytes. */
-size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz);
+ * step, dims dimx x dimy x dimz) and the table of the points whose
+ * materialLabel equals `label` (the split contact solve's, dims label_dim*), in
+ * one set of launches: the same contents as mpmx_hashgrid_build_ordered called
+ * once per table with its dims, with materialLabel NULL for the first (order may
+ * be NULL).  Workspace: mpmx_hashgrid_pair_workspace_bytes. */
+size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz, int label_dimx,
+                                          int label_dimy, int label_dimz);
 int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy,
-                             int dimz, const int32_t* materialLabel, int label,
-                             const int32_t* order, int32_t* point_ids, int32_t* cell_start,
-                             int32_t* label_point_ids, int32_t* label_cell_start,
-                             void* workspace, size_t ws_bytes, void* stream);
+                             int dimz, const int32_t* materialLabel, int label, int label_dimx,
+                             int label_dimy, int label_dimz, const int32_t* order,
+                             int32_t* point_ids, int32_t* cell_start, int32_t* label_point_ids,
+                             int32_t* label_cell_start, void* workspace, size_t ws_bytes,
+                             void* stream);
 
 /* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,
@@ -463,10 +466,13 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel,
                                           float relaxation, int contact_lock, float* deltas,
                                           void* stream);
 /* my_solve_particle_particle_contacts with a second table holding only the
- * label-2 points (mpmx_hashgrid_build_label(..., 2, ...) on the same positions):
- * MPM particles (labels 0/1), which skip every MPM candidate in the reference
- * (xpbdRoutines.py:185-186), walk the XPBD-only table, whose candidates come in
- * the same relative order.  Results equal mpmx_solve_particle_particle_contacts.
+ * label-2 points (mpmx_hashgrid_build_label(..., 2, ...) on the same positions,
+ * dims xpbd_dim*, which may be smaller than the first table's): MPM particles
+ * (labels 0/1), which skip every MPM candidate in the reference
+ * (xpbdRoutines.py:185-186), walk the XPBD-only table, whose candidates in
+ * contact reach come in the same relative order.  A particle whose query box
+ * comes within its contact reach of wrapping in the smaller table walks the
+ * full one.  Results equal mpmx_solve_particle_particle_contacts.
  * slot_of_id (may be NULL): after a spatial permutation of the particle arrays,
  * slot_of_id[k] is the array index of the particle whose input index is k; the
  * reference's tests of activeLabel/materialLabel/mass at the thread index
@@ -474,10 +480,10 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel,
 int mpmx_solve_particle_particle_contacts_split(
     int n, const int32_t* activeLabel, int32_t* materialLabel, const int32_t* point_ids,
     const int32_t* cell_start, const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
-    float cell_width, int dimx, int dimy, int dimz, const float* x, const float* v,
-    const float* mass, const float* radius, float k_mu, float static_threshold,
-    float static_friction, float k_cohesion, float max_radius, float dt, float relaxation,
-    int contact_lock, float* deltas, const int32_t* slot_of_id, void* stream);
+    float cell_width, int dimx, int dimy, int dimz, int xpbd_dimx, int xpbd_dimy, int xpbd_dimz,
+    const float* x, const float* v, const float* mass, const float* radius, float k_mu,
+    float static_threshold, float static_friction, float k_cohesion, float max_radius, float dt,
+    float relaxation, int contact_lock, float* deltas, const int32_t* slot_of_id, void* stream);
 
 /* my_apply_particle_deltas, xpbdRoutines.py:221-257 (x_out/v_out may alias x_pred) */
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel,

@@ -71,8 +71,8 @@ int counting_sort_from_counts(const int32_t* keys, long n, long nkeys, int32_t* 
 // of every chunk of kScanChunk keys into w.partials (one scan pass fewer)
 int counting_sort_from_chunk_counts(const int32_t* keys, long n, long nkeys, int32_t* sorted, BinWork& w,
                                     hipStream_t s, const int32_t* order = nullptr);
-// Several tables sorted over the same n items and nkeys keys in one set of
-// launches: table g has the key of every item (-1: not in the table), its rank
+// Several tables sorted over the same n items in one set of launches (nkeys:
+// the most keys of any table): table g has the key of every item (-1: not in the table), its rank
 // inside its key, the counts in key order and their totals per chunk of
 // kScanChunk keys; it receives its cell ranges ([nkeys + 1]) and sorted ids.
 constexpr int kMaxSortSegs = 2;
@@ -83,6 +83,7 @@ struct SortSeg {
     const int32_t* totals;
     int32_t* cell_start;
     int32_t* sorted;
+    long nkeys;  // keys of this table (nkeys of counting_sort_segments when 0)
 };
 struct SortSegs {
     SortSeg seg[kMaxSortSegs];

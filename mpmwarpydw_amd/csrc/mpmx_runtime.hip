@@ -235,7 +235,8 @@ static_assert(kMaxSortSegs == 2, "the table kernels select seg[0] or seg[1]");
 // the same for each table of a multi-table sort (blockIdx.y = table)
 __global__ void scan_apply_chunks_seg_kernel(SortSegs t, long n, long nb) {
     const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
-    scan_apply_chunks_tile(g.count, n, g.totals, g.cell_start, nb);
+    const long gn = g.nkeys ? g.nkeys : n, gnb = (gn + kScanTile - 1) / kScanTile;
+    if ((long)blockIdx.x < gnb) scan_apply_chunks_tile(g.count, gn, g.totals, g.cell_start, gnb);
 }
 
 size_t scan_partials_count(long n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
@@ -356,7 +357,7 @@ __global__ void __launch_bounds__(kBlock) bin_sort_runs_seg_kernel(SortSegs t, l
                                                                    const int32_t* __restrict__ order) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
-    if (c < nkeys) sort_run(c, g.cell_start, g.sorted, order);
+    if (c < (g.nkeys ? g.nkeys : nkeys)) sort_run(c, g.cell_start, g.sorted, order);
 }
 
 __global__ void bin_scatter_seg_kernel(SortSegs t, int nseg, long n) {

@@ -43,16 +43,17 @@ struct HashTables {
     int n;
     const float* x;
     float inv;
-    int dimx, dimy, dimz;
     int ntab;
+    int dimx[kMaxSortSegs], dimy[kMaxSortSegs], dimz[kMaxSortSegs];  // per table
+    long nkeys[kMaxSortSegs], nchunks[kMaxSortSegs];
+    long nblk0;  // transpose blocks of table 0 (table 1's follow)
     const int32_t* labels[kMaxSortSegs];  // table filter (null: every point)
     int label[kMaxSortSegs];
     int32_t* keys[kMaxSortSegs];
     int32_t* rank[kMaxSortSegs];
-    int32_t* tcount;  // [ntab][nkeys], transposed numbering
-    int32_t* count;   // [ntab][nkeys], key order
-    int32_t* totals;  // [ntab][nchunks]
-    long nkeys, nchunks;
+    int32_t* tcount[kMaxSortSegs];  // [nkeys], transposed numbering
+    int32_t* count[kMaxSortSegs];   // [nkeys], key order
+    int32_t* totals[kMaxSortSegs];  // [nchunks]
 };
 
 // key of each point per table (-1 when the table leaves it out) and its rank
@@ -60,23 +61,24 @@ struct HashTables {
 // atomic, whose return value is the base of the run's ranks
 __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int key = -1, tk = -1;
+    int ux = 0, uy = 0, uz = 0;  // the point's cell before wrapping
     if (i < h.n) {
         const long n = h.n;
-        int cx, cy, cz;
-        hash_cell(h.dimx, h.dimy, h.dimz, (int)(h.x[i] * h.inv), (int)(h.x[n + i] * h.inv),
-                  (int)(h.x[2 * n + i] * h.inv), cx, cy, cz);
-        key = cz * (h.dimx * h.dimy) + cy * h.dimx + cx;
-        tk = (cx * h.dimy + cy) * h.dimz + cz;
+        ux = (int)(h.x[i] * h.inv);
+        uy = (int)(h.x[n + i] * h.inv);
+        uz = (int)(h.x[2 * n + i] * h.inv);
     }
     const int lane = (int)(threadIdx.x & 63);
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
 #pragma unroll
     for (int g = 0; g < kMaxSortSegs; g++) {  // every lane stays for the cross-lane operations
         if (g >= h.ntab) break;
+        const int dx = h.dimx[g], dy = h.dimy[g], dz = h.dimz[g];
+        int cx, cy, cz;
+        hash_cell(dx, dy, dz, ux, uy, uz, cx, cy, cz);
         const bool in = i < h.n && (h.labels[g] == nullptr || h.labels[g][i] == h.label[g]);
-        const int k = in ? tk : -1;
-        if (i < h.n) h.keys[g][i] = in ? key : -1;
+        const int k = in ? (cx * dy + cy) * dz + cz : -1;
+        if (i < h.n) h.keys[g][i] = in ? cz * (dx * dy) + cy * dx + cx : -1;
         const int prev = __shfl_up(k, 1);
         const bool head = lane == 0 || prev != k;
         const unsigned long long heads = __ballot(head);
@@ -84,7 +86,7 @@ __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
         const unsigned long long above = heads & ~upto;
         const int next_head = above ? __ffsll((long long)above) - 1 : 64;
         int base = 0;
-        if (head && k >= 0) base = atomicAdd(&h.tcount[g * h.nkeys + k], next_head - lane);
+        if (head && k >= 0) base = atomicAdd(&h.tcount[g][k], next_head - lane);
         base = __shfl(base, my_head);
         if (in) h.rank[g][i] = base + (lane - my_head);
     }
@@ -95,31 +97,32 @@ __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
 // (at most two) scan chunks the row touches
 __global__ void __launch_bounds__(kBlock) hash_counts_transpose_kernel(HashTables h) {
     __shared__ int tile[64][65];
-    const int nbx = (h.dimx + 63) / 64, nbz = (h.dimz + 63) / 64;
     long b = blockIdx.x;
+    const int g = b < h.nblk0 ? 0 : 1;
+    if (g == 1) b -= h.nblk0;
+    const int dimx = h.dimx[g], dimy = h.dimy[g], dimz = h.dimz[g];
+    const int nbx = (dimx + 63) / 64, nbz = (dimz + 63) / 64;
     const int bx = (int)(b % nbx);
     b /= nbx;
     const int bz = (int)(b % nbz);
-    b /= nbz;
-    const int cy = (int)(b % h.dimy);
-    const int g = (int)(b / h.dimy);
+    const int cy = (int)(b / nbz);
     const int cx0 = bx * 64, cz0 = bz * 64;
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    const int32_t* tc = h.tcount + g * h.nkeys;
+    const int32_t* tc = h.tcount[g];
     for (int r = w; r < 64; r += kBlock / 64) {
         const int cx = cx0 + r, cz = cz0 + lane;
-        tile[r][lane] = (cx < h.dimx && cz < h.dimz) ? tc[((long)cx * h.dimy + cy) * h.dimz + cz] : 0;
+        tile[r][lane] = (cx < dimx && cz < dimz) ? tc[((long)cx * dimy + cy) * dimz + cz] : 0;
     }
     __syncthreads();
-    int32_t* out = h.count + g * h.nkeys;
-    int32_t* tot = h.totals + g * h.nchunks;
+    int32_t* out = h.count[g];
+    int32_t* tot = h.totals[g];
     for (int r = w; r < 64; r += kBlock / 64) {
         const int cz = cz0 + r;
-        if (cz >= h.dimz) break;  // uniform over the wavefront
+        if (cz >= dimz) break;  // uniform over the wavefront
         const int cx = cx0 + lane;
-        const bool valid = cx < h.dimx;
+        const bool valid = cx < dimx;
         const int v = valid ? tile[lane][r] : 0;
-        const long row = ((long)cz * h.dimy + cy) * h.dimx;
+        const long row = ((long)cz * dimy + cy) * dimx;
         if (valid) out[row + cx] = v;
         const long c0 = (row + cx0) / kScanChunk;
         const bool second = valid && (row + cx) / kScanChunk != c0;
@@ -218,14 +221,14 @@ __global__ void bound_contacts_kernel(int n, const int32_t* __restrict__ active,
 __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids, const int32_t* __restrict__ xp_start,
-    float inv, int dimx, int dimy, int dimz, const float* __restrict__ x,
+    float inv, int dimx, int dimy, int dimz, int xdimx, int xdimy, int xdimz, const float* __restrict__ x,
     const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
     float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
     const int32_t* __restrict__ slot_of_id) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n) return;
     // with no XPBD particle at all there is no pair to solve (MPM-MPM pairs are skipped)
-    if (xp_ids && xp_start[(long)dimx * dimy * dimz] == 0) return;
+    if (xp_ids && xp_start[(long)xdimx * xdimy * xdimz] == 0) return;
     // The reference tests the labels and the mass of array index tid, the thread's
     // position in the hash order, not of the particle i it solves
     // (xpbdRoutines.py:151-175).  After a spatial permutation of the arrays that
@@ -238,24 +241,36 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     float ri = radius[i];
     if (mass[q] == 0.0f) return;
     float w1 = 1.0f / mass[i];
-    const int32_t* ids = point_ids;
-    const int32_t* starts = cell_start;
-    if (xp_ids && mat[i] <= 1) {  // labels 0/1 stay MPM while this kernel runs (lock writes only 0)
-        ids = xp_ids;
-        starts = xp_start;
-    }
     float qr = ri + max_radius + k_coh * ri;
     int xs = (int)((xi.x - qr) * inv), ys = (int)((xi.y - qr) * inv), zs = (int)((xi.z - qr) * inv);
     int xe = min((int)((xi.x + qr) * inv), xs + dimx - 1);
     int ye = min((int)((xi.y + qr) * inv), ys + dimy - 1);
     int ze = min((int)((xi.z + qr) * inv), zs + dimz - 1);
+    const int32_t* ids = point_ids;
+    const int32_t* starts = cell_start;
+    int hx = dimx, hy = dimy, hz = dimz;
+    // labels 0/1 stay MPM while this kernel runs (lock writes only 0).  The XPBD table
+    // may have fewer cells: a cell there also holds the XPBD points of the cells that
+    // wrap onto it.  Those lie more than `gap` cells beyond the query box in some axis,
+    // so while gap cells are longer than the contact reach (ri + max_radius + k_coh)
+    // they fail the contact test below, and the candidates in reach, and their order,
+    // are those of the full table.  Otherwise the particle walks the full table.
+    const float reach = (ri + max_radius + k_coh) * inv;  // in cells
+    const int gapx = xdimx - (xe - xs + 1) - 1, gapy = xdimy - (ye - ys + 1) - 1, gapz = xdimz - (ze - zs + 1) - 1;
+    if (xp_ids && mat[i] <= 1 && (float)gapx > reach && (float)gapy > reach && (float)gapz > reach) {
+        ids = xp_ids;
+        starts = xp_start;
+        hx = xdimx;
+        hy = xdimy;
+        hz = xdimz;
+    }
     float k_mu = k_mu_in;
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
     const float alpha = 0.0f;  // XPBD compliance of the reference
     for (int cz = zs; cz <= ze; cz++)
         for (int cy = ys; cy <= ye; cy++)
             for (int cx = xs; cx <= xe; cx++) {
-                int cell = hash_index(dimx, dimy, dimz, cx, cy, cz);
+                int cell = hash_index(hx, hy, hz, cx, cy, cz);
                 int b = starts[cell], e = starts[cell + 1];
                 for (int s = b; s < e; s++) {
                     int j = ids[s];
@@ -388,100 +403,118 @@ using namespace mpmx;
 
 extern "C" {
 
-static size_t hash_tables_bytes(int ntab, int n, int dimx, int dimy, int dimz) {
-    if (n < 0 || dimx <= 0 || dimy <= 0 || dimz <= 0) return 0;
-    const long nkeys = (long)dimx * dimy * dimz, nn = n > 0 ? n : 1;
-    const long nch = (nkeys + kScanChunk - 1) / kScanChunk;
-    return 2 * ntab * Carver::bytes_for(nn, 4) + Carver::bytes_for(ntab * (nkeys + nch), 4) +
-           Carver::bytes_for(ntab * nkeys, 4);
+static long table_keys(const int* d) { return (long)d[0] * d[1] * d[2]; }
+static long table_chunks(const int* d) { return (table_keys(d) + kScanChunk - 1) / kScanChunk; }
+
+// dims[g] = {dimx, dimy, dimz} of table g
+static size_t hash_tables_bytes(int ntab, int n, const int (*dims)[3]) {
+    if (n < 0) return 0;
+    const long nn = n > 0 ? n : 1;
+    size_t bytes = 2 * ntab * Carver::bytes_for(nn, 4);
+    for (int g = 0; g < ntab; g++) {
+        if (dims[g][0] <= 0 || dims[g][1] <= 0 || dims[g][2] <= 0) return 0;
+        bytes += Carver::bytes_for(table_keys(dims[g]) + table_chunks(dims[g]), 4) +
+                 Carver::bytes_for(table_keys(dims[g]), 4);
+    }
+    return bytes;
 }
 
 size_t mpmx_hashgrid_workspace_bytes(int n, int dimx, int dimy, int dimz) {
-    return hash_tables_bytes(1, n, dimx, dimy, dimz);
+    const int dims[1][3] = {{dimx, dimy, dimz}};
+    return hash_tables_bytes(1, n, dims);
 }
 
-size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz) {
-    return hash_tables_bytes(2, n, dimx, dimy, dimz);
+size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz, int label_dimx, int label_dimy,
+                                          int label_dimz) {
+    const int dims[2][3] = {{dimx, dimy, dimz}, {label_dimx, label_dimy, label_dimz}};
+    return hash_tables_bytes(2, n, dims);
 }
 
-// ntab tables: table g holds the points whose labels[g] entry equals label[g]
-// (every point when labels[g] is null), with ids ascending in order[id] (in id
-// when order is null) inside each cell
-static int hash_tables_build(int ntab, int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
+// ntab tables: table g has dims[g] and holds the points whose labels[g] entry
+// equals label[g] (every point when labels[g] is null), with ids ascending in
+// order[id] (in id when order is null) inside each cell
+static int hash_tables_build(int ntab, int n, const float* x, float cell_width, const int (*dims)[3],
                              const int32_t* const* labels, const int* label, int32_t* const* point_ids,
                              int32_t* const* cell_start, void* workspace, size_t ws_bytes, void* stream,
                              const int32_t* order) {
-    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f,
-                 "mpmx_hashgrid_build: bad arguments");
-    const long nkeys = (long)dimx * dimy * dimz;
-    MPMX_REQUIRE(nkeys * ntab < (1L << 31), "mpmx_hashgrid_build: too many cells");
-    MPMX_REQUIRE(workspace && ws_bytes >= hash_tables_bytes(ntab, n, dimx, dimy, dimz),
-                 "mpmx_hashgrid_build: workspace too small");
-    for (int g = 0; g < ntab; g++)
+    MPMX_REQUIRE(n >= 0 && cell_width > 0.0f, "mpmx_hashgrid_build: bad arguments");
+    for (int g = 0; g < ntab; g++) {
+        MPMX_REQUIRE(dims[g][0] > 0 && dims[g][1] > 0 && dims[g][2] > 0, "mpmx_hashgrid_build: bad dims");
+        MPMX_REQUIRE(table_keys(dims[g]) < (1L << 31), "mpmx_hashgrid_build: too many cells");
         MPMX_REQUIRE(cell_start[g] && (n == 0 || (x && point_ids[g])), "mpmx_hashgrid_build: null pointer");
+    }
+    MPMX_REQUIRE(workspace && ws_bytes >= hash_tables_bytes(ntab, n, dims), "mpmx_hashgrid_build: workspace too small");
     hipStream_t s = as_stream(stream);
     const long nn = n > 0 ? n : 1;
-    const long nch = (nkeys + kScanChunk - 1) / kScanChunk;
     HashTables h{};
     h.n = n;
     h.x = x;
     h.inv = 1.0f / cell_width;
-    h.dimx = dimx;
-    h.dimy = dimy;
-    h.dimz = dimz;
     h.ntab = ntab;
-    h.nkeys = nkeys;
-    h.nchunks = nch;
     Carver cv(workspace, ws_bytes);
+    long nblk = 0, nkeys_max = 0;
     for (int g = 0; g < ntab; g++) {
+        h.dimx[g] = dims[g][0];
+        h.dimy[g] = dims[g][1];
+        h.dimz[g] = dims[g][2];
+        h.nkeys[g] = table_keys(dims[g]);
+        h.nchunks[g] = table_chunks(dims[g]);
         h.labels[g] = labels[g];
         h.label[g] = label[g];
         h.keys[g] = cv.take<int32_t>(nn);
         h.rank[g] = cv.take<int32_t>(nn);
+        const long blk = (long)dims[g][1] * ((dims[g][2] + 63) / 64) * ((dims[g][0] + 63) / 64);
+        if (g == 0) h.nblk0 = blk;
+        nblk += blk;
+        if (h.nkeys[g] > nkeys_max) nkeys_max = h.nkeys[g];
     }
-    h.tcount = cv.take<int32_t>(ntab * (nkeys + nch));
-    h.totals = h.tcount + ntab * nkeys;
-    h.count = cv.take<int32_t>(ntab * nkeys);
+    for (int g = 0; g < ntab; g++) {
+        h.tcount[g] = cv.take<int32_t>(h.nkeys[g] + h.nchunks[g]);  // the chunk totals follow the counts
+        h.totals[g] = h.tcount[g] + h.nkeys[g];
+        h.count[g] = cv.take<int32_t>(h.nkeys[g]);
+    }
     MPMX_REQUIRE(!cv.overflow, "mpmx_hashgrid_build: workspace carve failed");
-    int rc = fill_i32(h.tcount, ntab * (nkeys + nch), 0, s);
-    if (rc) return rc;
+    int rc;
+    for (int g = 0; g < ntab; g++)
+        if ((rc = fill_i32(h.tcount[g], h.nkeys[g] + h.nchunks[g], 0, s))) return rc;
     if (n > 0) {
         MPMX_LAUNCH(hash_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, h);
         if ((rc = check_launch("hash_count", s))) return rc;
     }
-    const long nblk = (long)ntab * dimy * ((dimz + 63) / 64) * ((dimx + 63) / 64);
     MPMX_LAUNCH(hash_counts_transpose_kernel, dim3((unsigned)nblk), dim3(kBlock), 0, s, h);
     if ((rc = check_launch("hash_counts_transpose", s))) return rc;
     SortSegs t{};
     for (int g = 0; g < ntab; g++) {
         t.seg[g].keys = h.keys[g];
         t.seg[g].rank = h.rank[g];
-        t.seg[g].count = h.count + g * nkeys;
-        t.seg[g].totals = h.totals + g * nch;
+        t.seg[g].count = h.count[g];
+        t.seg[g].totals = h.totals[g];
         t.seg[g].cell_start = cell_start[g];
         t.seg[g].sorted = point_ids[g];
+        t.seg[g].nkeys = h.nkeys[g];
     }
-    return counting_sort_segments(t, ntab, n, nkeys, s, order);
+    return counting_sort_segments(t, ntab, n, nkeys_max, s, order);
 }
 
 static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
                           const int32_t* labels, int label, int32_t* point_ids, int32_t* cell_start, void* workspace,
                           size_t ws_bytes, void* stream, const int32_t* order = nullptr) {
-    return hash_tables_build(1, n, x, cell_width, dimx, dimy, dimz, &labels, &label, &point_ids, &cell_start,
-                             workspace, ws_bytes, stream, order);
+    const int dims[1][3] = {{dimx, dimy, dimz}};
+    return hash_tables_build(1, n, x, cell_width, dims, &labels, &label, &point_ids, &cell_start, workspace, ws_bytes,
+                             stream, order);
 }
 
 int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
-                             const int32_t* materialLabel, int label, const int32_t* order, int32_t* point_ids,
-                             int32_t* cell_start, int32_t* label_point_ids, int32_t* label_cell_start,
-                             void* workspace, size_t ws_bytes, void* stream) {
+                             const int32_t* materialLabel, int label, int label_dimx, int label_dimy, int label_dimz,
+                             const int32_t* order, int32_t* point_ids, int32_t* cell_start, int32_t* label_point_ids,
+                             int32_t* label_cell_start, void* workspace, size_t ws_bytes, void* stream) {
     MPMX_REQUIRE(n == 0 || materialLabel, "mpmx_hashgrid_build_pair: null label array");
+    const int dims[2][3] = {{dimx, dimy, dimz}, {label_dimx, label_dimy, label_dimz}};
     const int32_t* labels[2] = {nullptr, materialLabel};
     const int lab[2] = {0, label};
     int32_t* ids[2] = {point_ids, label_point_ids};
     int32_t* starts[2] = {cell_start, label_cell_start};
-    return hash_tables_build(2, n, x, cell_width, dimx, dimy, dimz, labels, lab, ids, starts, workspace, ws_bytes,
-                             stream, order);
+    return hash_tables_build(2, n, x, cell_width, dims, labels, lab, ids, starts, workspace, ws_bytes, stream, order);
 }
 
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
@@ -544,27 +577,30 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel, int
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && x && v && mass && radius && deltas,
                  "mpmx_solve_particle_particle_contacts: null pointer");
     LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, nullptr, nullptr,
-            1.0f / cell_width, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold, static_friction,
-            k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, nullptr)
+            1.0f / cell_width, dimx, dimy, dimz, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold,
+            static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, nullptr)
 }
 
 int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabel, int32_t* materialLabel,
                                                 const int32_t* point_ids, const int32_t* cell_start,
                                                 const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
-                                                float cell_width, int dimx, int dimy, int dimz, const float* x,
+                                                float cell_width, int dimx, int dimy, int dimz, int xpbd_dimx,
+                                                int xpbd_dimy, int xpbd_dimz, const float* x,
                                                 const float* v, const float* mass, const float* radius, float k_mu,
                                                 float static_threshold, float static_friction, float k_cohesion,
                                                 float max_radius, float dt, float relaxation, int contact_lock,
                                                 float* deltas, const int32_t* slot_of_id, void* stream) {
-    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && cell_width > 0.0f,
+    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0 &&
+                     cell_width > 0.0f,
                  "mpmx_solve_particle_particle_contacts_split: bad sizes");
     if (n == 0) return 0;
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && xpbd_point_ids && xpbd_cell_start && x &&
                      v && mass && radius && deltas,
                  "mpmx_solve_particle_particle_contacts_split: null pointer");
     LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, xpbd_point_ids,
-            xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold,
-            static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, slot_of_id)
+            xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, v, mass, radius,
+            k_mu, static_threshold, static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas,
+            slot_of_id)
 }
 
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,

@@ -293,9 +293,7 @@ class HashGrid:
                    order: Array | None = None) -> None:
         """``self.build(points, cell_width, order=order)`` and
         ``other.build(points, cell_width, labels=labels, label=label, order=order)``
-        in one set of launches (mpmx_hashgrid_build_pair); both grids need the same dims."""
-        if other.dims != self.dims:
-            raise ValueError(f"build_pair: dims {self.dims} and {other.dims} differ")
+        in one set of launches (mpmx_hashgrid_build_pair); each grid keeps its own dims."""
         n = points.shape[0]
         for g in (self, other):
             if g.point_ids is None or g.point_ids.shape[0] != n:
@@ -303,7 +301,8 @@ class HashGrid:
             g.n = n
             g.cell_width = float(cell_width)
         dx, dy, dz = self.dims
-        ws, nb = self._ws.get(lib.raw("mpmx_hashgrid_pair_workspace_bytes")(n, dx, dy, dz))
-        lib("mpmx_hashgrid_build_pair", n, points.ptr, self.cell_width, dx, dy, dz, labels.ptr, int(label),
+        lx, ly, lz = other.dims
+        ws, nb = self._ws.get(lib.raw("mpmx_hashgrid_pair_workspace_bytes")(n, dx, dy, dz, lx, ly, lz))
+        lib("mpmx_hashgrid_build_pair", n, points.ptr, self.cell_width, dx, dy, dz, labels.ptr, int(label), lx, ly, lz,
             None if order is None else order.ptr, self.point_ids.ptr, self.cell_start.ptr, other.point_ids.ptr,
             other.cell_start.ptr, ws, nb, stream_handle())

@@ -321,7 +321,7 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     for it in range(iters):
         lib("mpmx_solve_particle_particle_contacts_split", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
             grid.point_ids.ptr, grid.cell_start.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr,
-            float(grid.cell_width), dims[0], dims[1], dims[2], xi.ptr, vi.ptr, sim.particle_mass.ptr,
+            float(grid.cell_width), dims[0], dims[1], dims[2], *xgrid.dims, xi.ptr, vi.ptr, sim.particle_mass.ptr,
             sim.particle_radius.ptr, float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
             float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
             float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock), xpbd.particle_delta.ptr,

@@ -68,10 +68,12 @@ def _solve_particle_particle_contacts_split(dim, activeLabel, materialLabel, gri
                                             max_radius, dt, relaxation, mpm_contact_transition_lock, deltas,
                                             slot_of_id=None):
     dx, dy, dz = grid.dims
-    assert xpbd_grid.dims == grid.dims and xpbd_grid.cell_width == grid.cell_width
+    if xpbd_grid.cell_width != grid.cell_width:
+        raise ValueError("the XPBD-only table needs the full table's cell width")
     lib("mpmx_solve_particle_particle_contacts_split", int(dim), activeLabel.ptr, materialLabel.ptr,
         grid.point_ids.ptr, grid.cell_start.ptr, xpbd_grid.point_ids.ptr, xpbd_grid.cell_start.ptr,
-        float(grid.cell_width), dx, dy, dz, particle_x.ptr, particle_v.ptr, particle_mass.ptr, particle_radius.ptr,
+        float(grid.cell_width), dx, dy, dz, *xpbd_grid.dims, particle_x.ptr, particle_v.ptr, particle_mass.ptr,
+        particle_radius.ptr,
         float(k_mu), float(staticParticleVelocityThresholdRatio), float(staticParticleFriction), float(k_cohesion),
         float(max_radius), float(dt), float(relaxation), int(mpm_contact_transition_lock), deltas.ptr,
         None if slot_of_id is None else slot_of_id.ptr, stream_handle())

@@ -75,7 +75,8 @@ def test_hashgrid_odd_dims_bit_exact(o64, dims):
 
 @pytest.mark.parametrize("ordered", [False, True])
 def test_hashgrid_pair_matches_single_builds(ordered):
-    """build_pair gives both tables bit for bit as two single builds."""
+    """build_pair gives both tables (with different dims) bit for bit as two
+    single builds."""
     from mpmwarpydw_amd import runtime as rt
     S = xpbd_scene(6)
     sim = S.sim
@@ -84,10 +85,10 @@ def test_hashgrid_pair_matches_single_builds(ordered):
     if ordered:
         order = rt.array(np.random.default_rng(6).permutation(n).astype(np.int32), rt.int32)
     print(f"[hashgrid pair] n={n} ordered={ordered}", flush=True)
-    a, b = rt.HashGrid(128, 128, 128), rt.HashGrid(128, 128, 128)
+    a, b = rt.HashGrid(128, 128, 128), rt.HashGrid(64, 64, 64)
     a.build(sim.particle_x, sim.dx, order=order)
     b.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2, order=order)
-    c, d = rt.HashGrid(128, 128, 128), rt.HashGrid(128, 128, 128)
+    c, d = rt.HashGrid(128, 128, 128), rt.HashGrid(64, 64, 64)
     c.build_pair(d, sim.particle_x, sim.dx, sim.materialLabel, 2, order)
     lab = sim.materialLabel.numpy()
     assert 0 < (lab == 2).sum() < n
@@ -195,10 +196,13 @@ def test_xpbd_step_parity(o64):
     assert np.max(np.abs(G["particle_cumDist_xpbd"] - dist)) <= 16 * ulp
 
 
-@pytest.mark.parametrize("lock", [0, 1])
-def test_split_table_is_bit_identical(lock):
+@pytest.mark.parametrize("lock,xdims", [(0, 64), (1, 64), (1, 8), (0, 4)])
+def test_split_table_is_bit_identical(lock, xdims):
     """MPM particles walking the XPBD-only table give bit-identical deltas and
-    labels to walking the full table (same candidates, same order)."""
+    labels to walking the full table (same candidates, same order).  The scene
+    spans more than 8 cells per axis: with 8^3 XPBD cells, cells wrap onto each
+    other and the far candidates must fail the contact test; with 4^3 the gap is
+    shorter than the contact reach and the particles walk the full table."""
     from mpmwarpydw_amd import runtime as rt
     from mpmwarpydw_amd import xpbdRoutines as X
     from mpmwarpydw_amd.runtime import launch
@@ -206,6 +210,10 @@ def test_split_table_is_bit_identical(lock):
     for split in (False, True):
         S = xpbd_scene(4)
         sim, xpbd = S.sim, S.xpbd
+        xpbd.particle_grid_xpbd = rt.HashGrid(xdims, xdims, xdims)
+        ext = np.ptp(sim.particle_x.numpy(), axis=0) / sim.dx
+        assert ext.max() > 8  # wrapping happens in the 8^3 table
+        print(f"[split table] lock={lock} xdims={xdims} extent cells={ext.round(1)}", flush=True)
         xpbd.particle_grid.build(sim.particle_x, sim.dx)
         xpbd.particle_delta.zero_()
         args = [sim.particle_x, sim.particle_v, sim.particle_mass, sim.particle_radius, xpbd.dynamicParticleFriction,
