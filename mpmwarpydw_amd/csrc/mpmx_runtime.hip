@@ -295,7 +295,7 @@ __global__ void bin_scatter_kernel(const int32_t* __restrict__ keys, long n, con
 // by order.  Runs are short (the particles of one cell): up to kRunReg they are
 // sorted in registers by a bitonic network (static indices, one load and one
 // store per entry, nothing written when the run is already in order); longer
-// runs fall back to insertion sort in place.
+// runs are heap-sorted in place.
 constexpr int kRunReg = 16;
 
 __device__ __forceinline__ void cswap(int& oa, int& va, int& ob, int& vb, bool up) {
@@ -334,15 +334,34 @@ __device__ __forceinline__ void sort_run(long c, const int32_t* __restrict__ cel
             if (k < len) sorted[b + k] = v[k];
         return;
     }
-    for (int i = b + 1; i < e; i++) {
-        const int vi = sorted[i];
-        const int ov = order ? order[vi] : vi;
-        int j = i - 1;
-        while (j >= b && (order ? order[sorted[j]] : sorted[j]) > ov) {
-            sorted[j + 1] = sorted[j];
-            j--;
+    // heapsort in place: O(len log len) for the rare long run (an orphan bin, a
+    // crowded hash cell); the keys are distinct, so the result is that of any sort
+    int32_t* r = sorted + b;
+    auto key = [&](int v) { return order ? order[v] : v; };
+    auto sift = [&](int root, int m) {
+        const int v = r[root], kv = key(v);
+        while (2 * root + 1 < m) {
+            int c = 2 * root + 1;
+            int kc = key(r[c]);
+            if (c + 1 < m) {
+                const int k2 = key(r[c + 1]);
+                if (k2 > kc) {
+                    c++;
+                    kc = k2;
+                }
+            }
+            if (kc <= kv) break;
+            r[root] = r[c];
+            root = c;
         }
-        sorted[j + 1] = vi;
+        r[root] = v;
+    };
+    for (int i = len / 2 - 1; i >= 0; i--) sift(i, len);
+    for (int m = len - 1; m > 0; m--) {
+        const int top = r[0];
+        r[0] = r[m];
+        r[m] = top;
+        sift(0, m);
     }
 }
 

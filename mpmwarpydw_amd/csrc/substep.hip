@@ -39,11 +39,21 @@ struct SubstepWork {
 // a bin used only by the spatial reorder for every particle that is not MPM
 static long substep_nkeys(const Geo& g) { return g.ntiles() * kTileCells + 2; }
 
+// The spatial reorder gives the particles that are not active MPM particles
+// bins of kRestRun consecutive input indices instead of that one bin, so that no
+// bin holds more than kRestRun of them (a bin is sorted by one thread).
+constexpr long kRestRun = 16;
+static long reorder_nkeys(const Geo& g, long n) { return g.ntiles() * kTileCells + 1 + (n + kRestRun - 1) / kRestRun; }
+static long bin_nkeys(const Geo& g, long n) {
+    const long a = substep_nkeys(g), b = reorder_nkeys(g, n);
+    return a > b ? a : b;
+}
+
 static size_t substep_ws_bytes(long n, const Geo& g) {
     long nn = n > 0 ? n : 1;
     return Carver::bytes_for(1, 4) + Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(g.ntiles() * kTileNodes * 4, 4) +
            Carver::bytes_for(g.ntiles(), 4) + Carver::bytes_for(g.nscenes + 1, sizeof(SceneDesc)) +
-           Carver::bytes_for(2 * g.nnodes, sizeof(unsigned long long)) + bin_workspace_bytes(nn, substep_nkeys(g));
+           Carver::bytes_for(2 * g.nnodes, sizeof(unsigned long long)) + bin_workspace_bytes(nn, bin_nkeys(g, nn));
 }
 
 static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepWork& w) {
@@ -56,7 +66,7 @@ static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepW
     w.tile_flag = cv.take<int32_t>(g.ntiles());
     w.table = cv.take<SceneDesc>(g.nscenes + 1);
     w.vlc = cv.take<unsigned long long>(2 * g.nnodes);
-    carve_bin(cv, nn, substep_nkeys(g), w.bin);
+    carve_bin(cv, nn, bin_nkeys(g, nn), w.bin);
     return !cv.overflow;
 }
 
@@ -79,11 +89,13 @@ __global__ void substep_key_kernel(mpmx_particles P, float inv_dx, Geo geo, int3
 }
 
 // key of every particle for the spatial reorder: active MPM particles as in
-// substep_key_kernel, everything else in the last bin
+// substep_key_kernel, everything else after them in input order (in bins of
+// kRestRun input indices)
 __global__ void reorder_key_kernel(mpmx_particles P, float inv_dx, Geo geo, int32_t* __restrict__ keys) {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
-    int key = (int)(geo.ntiles() * kTileCells + 1);
+    const long input_index = P.order ? P.order[p] : p;
+    int key = (int)(geo.ntiles() * kTileCells + 1 + input_index / kRestRun);
     if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
         const int sc = scene_of(P, p);
         key = (sc >= 0 && sc < geo.nscenes) ? tile_key(ld3(P.x, P.n, p), inv_dx, geo.at(sc)) : -1;
@@ -529,7 +541,7 @@ static int reorder_geo(const mpmx_particles* P, const Geo& geo, float inv_dx, Su
     MPMX_LAUNCH(reorder_key_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, inv_dx, geo, w.keys);
     int rc = check_launch("reorder_key", s);
     if (rc) return rc;
-    return counting_sort_ids(w.keys, P->n, substep_nkeys(geo), perm, w.bin, s, P->order);
+    return counting_sort_ids(w.keys, P->n, reorder_nkeys(geo, P->n), perm, w.bin, s, P->order);
 }
 
 static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in, float* grid_v_out,
@@ -661,7 +673,7 @@ int mpmx_reorder_permutation(const mpmx_particles* P, const mpmx_grid* G, const 
     MPMX_REQUIRE(P->n == 0 || perm, "mpmx_reorder_permutation: null perm");
     if (P->n == 0) return 0;
     Geo geo = make_geom(G->gx, G->gy, G->gz, prm->minx, prm->miny, prm->minz);
-    MPMX_REQUIRE(substep_nkeys(geo) < (1L << 31), "substep: grid too large for int32 keys");
+    MPMX_REQUIRE(reorder_nkeys(geo, P->n) < (1L << 31), "substep: grid too large for int32 keys");
     MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, geo), "mpmx_reorder_permutation: workspace too small");
     SubstepWork w;
     MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, geo, w), "mpmx_reorder_permutation: workspace carve failed");

@@ -38,3 +38,33 @@ def test_reorder_is_a_pure_layout_change():
     a, b = _arrays(A), _arrays(B)
     for k in a:
         assert np.array_equal(b[k], a[k]), k
+
+
+def test_reorder_with_many_non_mpm_particles():
+    """Particles that are not active MPM particles (XPBD label 2, inactive)
+    follow the MPM ones in input order; with most of a scene among them the
+    reorder stays a pure layout change."""
+    A = make_sim(seed=22)
+    B = make_sim(seed=22)
+    n = A.sim.nPoints
+    rng = np.random.default_rng(22)
+    lab = A.sim.materialLabel.numpy()
+    lab[rng.random(n) < 0.8] = 2
+    for S in (A, B):
+        S.sim.materialLabel.assign(lab)
+    print(f"[reorder-rest] n={n} non_mpm={(lab == 2).sum()}", flush=True)
+    B.reorder()
+    pid = B.sim.particle_id.t.cpu().numpy()
+    assert np.array_equal(np.sort(pid), np.arange(n))
+    act = A.sim.activeLabel.numpy()
+    mpm = (act == 1) & (lab <= 1)
+    slot_is_mpm = mpm[pid]
+    k = int(mpm.sum())
+    assert slot_is_mpm[:k].all() and not slot_is_mpm[k:].any()
+    assert np.array_equal(pid[k:], np.flatnonzero(~mpm))  # the rest keep input order
+    for counter in range(1, A.sim.mpmStepsPerXpbdStep + 2):
+        A.substep(counter)
+        B.substep(counter)
+    a, b = _arrays(A), _arrays(B)
+    for key in a:
+        assert np.array_equal(b[key], a[key]), key
