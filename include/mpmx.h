@@ -168,7 +168,8 @@ int mpmx_version(void);
  * mpmx_particles, order), sizeof(mpmx_grid), sizeof(mpmx_step_params),
  * offsetof(mpmx_step_params, bc), offsetof(mpmx_particles, scene),
  * sizeof(mpmx_scene), sizeof(mpmx_xpbd_bounds), offsetof(mpmx_xpbd_bounds,
- * delta); returns the number of values written (<= cap). */
+ * delta), offsetof(mpmx_xpbd_bounds, xpbd_count); returns the number of
+ * values written (<= cap). */
 int mpmx_struct_layout(long* out, int cap);
 /* Launch profiler (no counterpart in the reference; replaces its wall-clock
  * prints, runMPMYDW.py:762): between begin and end, every kernel the library
@@ -485,6 +486,31 @@ int mpmx_solve_particle_particle_contacts_split(
     float static_threshold, float static_friction, float k_cohesion, float max_radius, float dt,
     float relaxation, int contact_lock, float* deltas, const int32_t* slot_of_id, void* stream);
 
+/* Candidate records for mpmx_solve_particle_particle_contacts_packed, one per
+ * particle: xr[4n] = (x, y, z, radius), vw[4n] = (vx, vy, vz, 1/mass),
+ * flags[n] = 1 (activeLabel == 1 and mass > 0) | 2 (materialLabel <= 1) |
+ * 4 (materialLabel == 2); x and v are the SoA arrays the contact solve reads.
+ * xr and vw need 16-byte alignment.  No counterpart in the reference: a layout
+ * of the data that my_solve_particle_particle_contacts gathers per candidate
+ * (xpbdRoutines.py:178-196). */
+int mpmx_xpbd_contact_pack(int n, const int32_t* activeLabel, const int32_t* materialLabel,
+                           const float* mass, const float* radius, const float* x, const float* v,
+                           float* xr, float* vw, int32_t* flags, void* stream);
+
+/* mpmx_solve_particle_particle_contacts_split reading each candidate from the
+ * records of mpmx_xpbd_contact_pack, packed from the same x, v, labels, mass
+ * and radius after their last change (my_solve_particle_particle_contacts,
+ * xpbdRoutines.py:138-219).  The results equal those of the split form: lock
+ * writes turn label 1 into 0 only, which changes no flag. */
+int mpmx_solve_particle_particle_contacts_packed(
+    int n, const int32_t* activeLabel, int32_t* materialLabel, const int32_t* point_ids,
+    const int32_t* cell_start, const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
+    float cell_width, int dimx, int dimy, int dimz, int xpbd_dimx, int xpbd_dimy, int xpbd_dimz,
+    const float* x, const float* v, const float* mass, const float* radius, const float* xr,
+    const float* vw, const int32_t* flags, float k_mu, float static_threshold, float static_friction,
+    float k_cohesion, float max_radius, float dt, float relaxation, int contact_lock, float* deltas,
+    const int32_t* slot_of_id, void* stream);
+
 /* my_apply_particle_deltas, xpbdRoutines.py:221-257 (x_out/v_out may alias x_pred) */
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel,
                                const float* x_orig, const float* x_pred, const float* delta,
@@ -517,6 +543,15 @@ typedef struct {
     float minx, miny, minz, maxx, maxy, maxz; /* the XPBD box */
     float dt, relaxation;
     float* delta; /* [3][n]: every particle's delta is written */
+    /* The candidate records of the next contact solve (the layout of
+     * mpmx_xpbd_contact_pack), written from the new x_int / v_int and labels by
+     * the kernel that takes these bounds; not written when xr is NULL, nor
+     * while xpbd_count (may be NULL: the XPBD-only table's point count,
+     * cell_start[cells]) reads 0, when the packed contact solve reads none. */
+    float* xr;
+    float* vw;
+    int32_t* flags;
+    const int32_t* xpbd_count;
 } mpmx_xpbd_bounds;
 
 /* x_int, v_int <- x, v then integrateParticlesXPBD (xpbdRoutines.py:23-48); with

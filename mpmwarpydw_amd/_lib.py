@@ -93,6 +93,9 @@ SIGNATURES = {
     "mpmx_solve_particle_particle_contacts": [I, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F, F, I, P, P],
     "mpmx_solve_particle_particle_contacts_split": [I, P, P, P, P, P, P, F, I, I, I, I, I, I, P, P, P, P, F, F, F,
                                                     F, F, F, F, I, P, P, P],
+    "mpmx_xpbd_contact_pack": [I, P, P, P, P, P, P, P, P, P, P],
+    "mpmx_solve_particle_particle_contacts_packed": [I, P, P, P, P, P, P, F, I, I, I, I, I, I, P, P, P, P, P, P, P,
+                                                     F, F, F, F, F, F, F, I, P, P, P],
     "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],
     "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
     "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],
@@ -168,7 +171,7 @@ class XpbdBounds(_Strict):
     """mpmx_xpbd_bounds (include/mpmx.h)"""
     _fields_ = [("mass", C.c_void_p), ("radius", C.c_void_p)] + [(name, C.c_float) for name in (
         "mu", "static_threshold", "static_friction", "minx", "miny", "minz", "maxx", "maxy", "maxz", "dt",
-        "relaxation")] + [("delta", C.c_void_p)]
+        "relaxation")] + [(name, C.c_void_p) for name in ("delta", "xr", "vw", "flags", "xpbd_count")]
 
 
 class MpmxError(RuntimeError):

@@ -16,6 +16,16 @@
 
 namespace mpmx {
 
+// Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on
+// XCD b % 8), and every XCD has its own L2.  Neighbouring tiles (or ranges of
+// a table) share cache lines of the particle and grid arrays, so they are
+// handed out in 8 contiguous ranges, one per XCD: block b works on tile
+// xcd_tile(b, nb).  A bijection of [0, nb) whatever the dispatch order.
+MPMX_HD int xcd_tile(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, j = b / 8;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 constexpr int kBlock = 256;
 
 struct f3 {

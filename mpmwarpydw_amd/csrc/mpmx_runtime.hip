@@ -627,12 +627,13 @@ int mpmx_set_tuning(int knob, int value) {
 int mpmx_get_tuning(int knob) { return tuning(knob); }
 
 int mpmx_struct_layout(long* out, int cap) {
-    const long v[9] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order), (long)sizeof(mpmx_grid),
-                       (long)sizeof(mpmx_step_params), (long)offsetof(mpmx_step_params, bc),
-                       (long)offsetof(mpmx_particles, scene), (long)sizeof(mpmx_scene),
-                       (long)sizeof(mpmx_xpbd_bounds), (long)offsetof(mpmx_xpbd_bounds, delta)};
+    const long v[10] = {(long)sizeof(mpmx_particles), (long)offsetof(mpmx_particles, order),
+                        (long)sizeof(mpmx_grid), (long)sizeof(mpmx_step_params), (long)offsetof(mpmx_step_params, bc),
+                        (long)offsetof(mpmx_particles, scene), (long)sizeof(mpmx_scene),
+                        (long)sizeof(mpmx_xpbd_bounds), (long)offsetof(mpmx_xpbd_bounds, delta),
+                        (long)offsetof(mpmx_xpbd_bounds, xpbd_count)};
     int k = 0;
-    for (; out && k < cap && k < 9; k++) out[k] = v[k];
+    for (; out && k < cap && k < 10; k++) out[k] = v[k];
     return k;
 }
 

@@ -44,15 +44,6 @@ constexpr int kTasks = kTileCells * 3;       // (cell, i): each task owns the 9 
 constexpr int kRecCols = kChunk + kChunk / 8;
 __host__ __device__ constexpr int rcol(int s) { return s + (s >> 3); }
 
-// Workgroups are dealt to the 8 XCDs of the chip round-robin (block b runs on
-// XCD b % 8), and every XCD has its own L2.  Neighbouring tiles share cache
-// lines of the particle and grid arrays, so tiles are handed out in 8
-// contiguous ranges, one per XCD: block b works on tile xcd_tile(b, nb).
-MPMX_HD int xcd_tile(int b, int nb) {
-    const int q = nb / 8, r = nb % 8, x = b % 8, j = b / 8;
-    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
-}
-
 // Grid geometry of one scene: its nodes, its 4x4x4-cell tiles, where its tiles
 // and nodes start in the global numbering, and its grid origin.  Several
 // scenes (an ensemble) share one launch of every substep kernel: tiles are

@@ -39,11 +39,15 @@ struct SubstepWork {
 // a bin used only by the spatial reorder for every particle that is not MPM
 static long substep_nkeys(const Geo& g) { return g.ntiles() * kTileCells + 2; }
 
-// The spatial reorder gives the particles that are not active MPM particles
-// bins of kRestRun consecutive input indices instead of that one bin, so that no
-// bin holds more than kRestRun of them (a bin is sorted by one thread).
+// The spatial reorder's keys: active MPM particles as in the substep (tile
+// keys, then the orphan bin), then active XPBD particles by the same tile keys
+// (their contact walk gathers the particles of neighbouring cells), then every
+// other particle in bins of kRestRun consecutive input indices, so that no bin of
+// them holds more than kRestRun (a bin is sorted by one thread).
 constexpr long kRestRun = 16;
-static long reorder_nkeys(const Geo& g, long n) { return g.ntiles() * kTileCells + 1 + (n + kRestRun - 1) / kRestRun; }
+static long reorder_nkeys(const Geo& g, long n) {
+    return 2 * g.ntiles() * kTileCells + 1 + (n + kRestRun - 1) / kRestRun;
+}
 static long bin_nkeys(const Geo& g, long n) {
     const long a = substep_nkeys(g), b = reorder_nkeys(g, n);
     return a > b ? a : b;
@@ -88,20 +92,23 @@ __global__ void substep_key_kernel(mpmx_particles P, float inv_dx, Geo geo, int3
     keys[p] = key;
 }
 
-// key of every particle for the spatial reorder: active MPM particles as in
-// substep_key_kernel, everything else after them in input order (in bins of
-// kRestRun input indices)
+// key of every particle for the spatial reorder (the key ranges of reorder_nkeys)
 __global__ void reorder_key_kernel(mpmx_particles P, float inv_dx, Geo geo, int32_t* __restrict__ keys) {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
+    const long tiles = geo.ntiles() * kTileCells;
     const long input_index = P.order ? P.order[p] : p;
-    int key = (int)(geo.ntiles() * kTileCells + 1 + input_index / kRestRun);
-    if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
+    long key = 2 * tiles + 1 + input_index / kRestRun;
+    const int label = P.materialLabel[p];
+    if (P.activeLabel[p] == 1 && (label <= 1 || label == 2)) {
         const int sc = scene_of(P, p);
-        key = (sc >= 0 && sc < geo.nscenes) ? tile_key(ld3(P.x, P.n, p), inv_dx, geo.at(sc)) : -1;
-        if (key < 0) key = (int)(geo.ntiles() * kTileCells);
+        const int t = (sc >= 0 && sc < geo.nscenes) ? tile_key(ld3(P.x, P.n, p), inv_dx, geo.at(sc)) : -1;
+        if (label <= 1)
+            key = t >= 0 ? t : tiles;
+        else if (t >= 0)
+            key = tiles + 1 + t;
     }
-    keys[p] = key;
+    keys[p] = (int)key;
 }
 
 // the scene table of an ensemble, written into the workspace from kernel

@@ -213,19 +213,53 @@ __global__ void bound_contacts_kernel(int n, const int32_t* __restrict__ active,
     st3(delta, n, t, f3{da[0], da[1], da[2]});
 }
 
+// Candidate records of the contact walk, one per particle slot: a candidate
+// costs one 16-byte load of its position and radius and one flag word; its
+// velocity and inverse mass load only for a contact.  The flags hold what the
+// walk tests of a candidate's labels.  Lock writes turn label 1 into 0 only,
+// so "label <= 1" and "label == 2" cannot change while a contact solve runs,
+// and records packed before it give the tests of the live labels.
+struct ContactPack {
+    const float4* xr;       // x, y, z, radius
+    const float4* vw;       // vx, vy, vz, 1 / mass
+    const int32_t* flags;   // kCand* bits
+};
+constexpr int kCandValid = 1;  // active == 1 and mass > 0
+constexpr int kCandMpm = 2;    // label <= 1
+constexpr int kCandXpbd = 4;   // label == 2
+
+__global__ void contact_pack_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
+                                    const float* __restrict__ mass, const float* __restrict__ radius,
+                                    const float* __restrict__ x, const float* __restrict__ v,
+                                    float4* __restrict__ xr, float4* __restrict__ vw, int32_t* __restrict__ flags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float m = mass[t];
+    const int lab = mat[t];
+    const f3 xp = ld3(x, n, t), vp = ld3(v, n, t);
+    xr[t] = make_float4(xp.x, xp.y, xp.z, radius[t]);
+    vw[t] = make_float4(vp.x, vp.y, vp.z, 1.0f / m);
+    flags[t] = ((active[t] == 1 && m > 0.0f) ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) |
+               (lab == 2 ? kCandXpbd : 0);
+}
+
 // my_solve_particle_particle_contacts (xpbdRoutines.py:138-219).
 // With xp_ids/xp_start (a table of the label-2 points only, same hashing and
 // stable order) an MPM particle i walks that table instead of the full one: it
 // skips every MPM candidate anyway (:185-186), and the XPBD candidates come in
-// the same relative order, so deltas and lock writes are unchanged.
+// the same relative order, so deltas and lock writes are unchanged.  PACKED
+// reads the candidates from ContactPack records instead of the particle arrays.
+template <bool PACKED>
 __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids, const int32_t* __restrict__ xp_start,
     float inv, int dimx, int dimy, int dimz, int xdimx, int xdimy, int xdimz, const float* __restrict__ x,
     const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
     float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
-    const int32_t* __restrict__ slot_of_id) {
-    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t* __restrict__ slot_of_id, ContactPack pk) {
+    // table positions in XCD-contiguous ranges: the cells a block walks are
+    // those of the blocks beside it in the table, so they share one L2
+    int tid = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (tid >= n) return;
     // with no XPBD particle at all there is no pair to solve (MPM-MPM pairs are skipped)
     if (xp_ids && xp_start[(long)xdimx * xdimy * xdimz] == 0) return;
@@ -249,15 +283,18 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     const int32_t* ids = point_ids;
     const int32_t* starts = cell_start;
     int hx = dimx, hy = dimy, hz = dimz;
-    // labels 0/1 stay MPM while this kernel runs (lock writes only 0).  The XPBD table
-    // may have fewer cells: a cell there also holds the XPBD points of the cells that
-    // wrap onto it.  Those lie more than `gap` cells beyond the query box in some axis,
-    // so while gap cells are longer than the contact reach (ri + max_radius + k_coh)
-    // they fail the contact test below, and the candidates in reach, and their order,
-    // are those of the full table.  Otherwise the particle walks the full table.
+    // labels <= 1 and == 2 do not change while this kernel runs (lock turns 1 into 0)
+    const int lab_i = mat[i];
+    const bool i_mpm = lab_i <= 1, i_xpbd = lab_i == 2;
+    // The XPBD table may have fewer cells: a cell there also holds the XPBD points of
+    // the cells that wrap onto it.  Those lie more than `gap` cells beyond the query
+    // box in some axis, so while gap cells are longer than the contact reach
+    // (ri + max_radius + k_coh) they fail the contact test below, and the candidates
+    // in reach, and their order, are those of the full table.  Otherwise the
+    // particle walks the full table.
     const float reach = (ri + max_radius + k_coh) * inv;  // in cells
     const int gapx = xdimx - (xe - xs + 1) - 1, gapy = xdimy - (ye - ys + 1) - 1, gapz = xdimz - (ze - zs + 1) - 1;
-    if (xp_ids && mat[i] <= 1 && (float)gapx > reach && (float)gapy > reach && (float)gapz > reach) {
+    if (xp_ids && i_mpm && (float)gapx > reach && (float)gapy > reach && (float)gapz > reach) {
         ids = xp_ids;
         starts = xp_start;
         hx = xdimx;
@@ -274,21 +311,46 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
                 int b = starts[cell], e = starts[cell + 1];
                 for (int s = b; s < e; s++) {
                     int j = ids[s];
-                    if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
-                    if (mat[i] <= 1 && mat[j] <= 1) continue;
-                    f3 xj = ld3(x, n, j);
+                    f3 xj, vj;
+                    float rj, w2;
+                    bool j_mpm, j_xpbd;
+                    if constexpr (PACKED) {
+                        const int fj = pk.flags[j];
+                        if (!(fj & kCandValid) || j == i) continue;
+                        j_mpm = (fj & kCandMpm) != 0;
+                        if (i_mpm && j_mpm) continue;
+                        j_xpbd = (fj & kCandXpbd) != 0;
+                        const float4 pj = pk.xr[j];
+                        xj = f3{pj.x, pj.y, pj.z};
+                        rj = pj.w;
+                    } else {
+                        if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
+                        const int lab_j = mat[j];
+                        j_mpm = lab_j <= 1;
+                        if (i_mpm && j_mpm) continue;
+                        j_xpbd = lab_j == 2;
+                        xj = ld3(x, n, j);
+                        rj = radius[j];
+                    }
                     float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
                     float d = sqrtf(nx * nx + ny * ny + nz * nz);
-                    float err = d - ri - radius[j];
-                    float w2 = 1.0f / mass[j];
+                    float err = d - ri - rj;
+                    if (!(err <= k_coh)) continue;
+                    if constexpr (PACKED) {
+                        const float4 qj = pk.vw[j];
+                        vj = f3{qj.x, qj.y, qj.z};
+                        w2 = qj.w;
+                    } else {
+                        vj = ld3(v, n, j);
+                        w2 = 1.0f / mass[j];
+                    }
                     float denom = w1 + w2;
-                    if (err <= k_coh && denom > 0.0f) {
+                    if (denom > 0.0f) {
                         if (lock == 1) {
-                            if (mat[i] <= 1 && mat[j] == 2) mat[i] = 0;
-                            if (mat[i] == 2 && mat[j] <= 1) mat[j] = 0;
+                            if (i_mpm && j_xpbd) mat[i] = 0;
+                            if (i_xpbd && j_mpm) mat[j] = 0;
                         }
                         float ux = nx / d, uy = ny / d, uz = nz / d;
-                        f3 vj = ld3(v, n, j);
                         float rx = vi.x - vj.x, ry = vi.y - vj.y, rz = vi.z - vj.z;
                         float lambda_n = err / (1.0f + (alpha * dt * dt));
                         float vn = ux * rx + uy * ry + uz * rz;
@@ -576,9 +638,10 @@ int mpmx_solve_particle_particle_contacts(int n, const int32_t* activeLabel, int
     if (n == 0) return 0;
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && x && v && mass && radius && deltas,
                  "mpmx_solve_particle_particle_contacts: null pointer");
-    LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, nullptr, nullptr,
-            1.0f / cell_width, dimx, dimy, dimz, dimx, dimy, dimz, x, v, mass, radius, k_mu, static_threshold,
-            static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, nullptr)
+    LAUNCH1(particle_contacts_kernel<false>, n, n, activeLabel, materialLabel, point_ids, cell_start, nullptr,
+            nullptr, 1.0f / cell_width, dimx, dimy, dimz, dimx, dimy, dimz, x, v, mass, radius, k_mu,
+            static_threshold, static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, nullptr,
+            ContactPack{})
 }
 
 int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabel, int32_t* materialLabel,
@@ -597,10 +660,42 @@ int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabe
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && xpbd_point_ids && xpbd_cell_start && x &&
                      v && mass && radius && deltas,
                  "mpmx_solve_particle_particle_contacts_split: null pointer");
-    LAUNCH1(particle_contacts_kernel, n, n, activeLabel, materialLabel, point_ids, cell_start, xpbd_point_ids,
+    LAUNCH1(particle_contacts_kernel<false>, n, n, activeLabel, materialLabel, point_ids, cell_start,
+            xpbd_point_ids, xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, xpbd_dimx, xpbd_dimy, xpbd_dimz, x,
+            v, mass, radius, k_mu, static_threshold, static_friction, k_cohesion, max_radius, dt, relaxation,
+            contact_lock, deltas, slot_of_id, ContactPack{})
+}
+
+int mpmx_xpbd_contact_pack(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* mass,
+                           const float* radius, const float* x, const float* v, float* xr, float* vw, int32_t* flags,
+                           void* stream) {
+    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && mass && radius && x && v && xr && vw && flags)),
+                 "mpmx_xpbd_contact_pack: bad args");
+    MPMX_REQUIRE(((uintptr_t)xr | (uintptr_t)vw) % 16 == 0, "mpmx_xpbd_contact_pack: records need 16-byte alignment");
+    if (n == 0) return 0;
+    LAUNCH1(contact_pack_kernel, n, n, activeLabel, materialLabel, mass, radius, x, v, (float4*)xr, (float4*)vw, flags)
+}
+
+int mpmx_solve_particle_particle_contacts_packed(
+    int n, const int32_t* activeLabel, int32_t* materialLabel, const int32_t* point_ids, const int32_t* cell_start,
+    const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start, float cell_width, int dimx, int dimy, int dimz,
+    int xpbd_dimx, int xpbd_dimy, int xpbd_dimz, const float* x, const float* v, const float* mass,
+    const float* radius, const float* xr, const float* vw, const int32_t* flags, float k_mu, float static_threshold,
+    float static_friction, float k_cohesion, float max_radius, float dt, float relaxation, int contact_lock,
+    float* deltas, const int32_t* slot_of_id, void* stream) {
+    MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0 &&
+                     cell_width > 0.0f,
+                 "mpmx_solve_particle_particle_contacts_packed: bad sizes");
+    if (n == 0) return 0;
+    MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && xpbd_point_ids && xpbd_cell_start && x &&
+                     v && mass && radius && xr && vw && flags && deltas,
+                 "mpmx_solve_particle_particle_contacts_packed: null pointer");
+    MPMX_REQUIRE(((uintptr_t)xr | (uintptr_t)vw) % 16 == 0,
+                 "mpmx_solve_particle_particle_contacts_packed: records need 16-byte alignment");
+    LAUNCH1(particle_contacts_kernel<true>, n, n, activeLabel, materialLabel, point_ids, cell_start, xpbd_point_ids,
             xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, v, mass, radius,
             k_mu, static_threshold, static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas,
-            slot_of_id)
+            slot_of_id, (ContactPack{(const float4*)xr, (const float4*)vw, flags}))
 }
 
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,
@@ -720,6 +815,11 @@ __device__ __forceinline__ void apply_delta(f3 x0, f3 xp, f3 d, float dt, float 
     }
 }
 
+// whether the kernel taking B writes the contact records (mpmx_xpbd_bounds)
+__device__ __forceinline__ bool records_wanted(const mpmx_xpbd_bounds& B) {
+    return B.xr && !(B.xpbd_count && *B.xpbd_count == 0);
+}
+
 // x_int, v_int <- x, v; integrateParticlesXPBD on active XPBD particles
 // (xpbdRoutines.py:23-48); reset_mpm_contact_labels when lock (:3-21)
 __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int32_t* __restrict__ mat,
@@ -744,11 +844,23 @@ __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int
     }
     st3(x_int, n, p, x0);
     st3(v_int, n, p, v0);
-    if (reset_labels && a == 1 && m == 0) mat[p] = 1;
+    const bool reset = reset_labels && a == 1 && m == 0;
+    if (reset) mat[p] = 1;
     // the first iteration's particle-wall deltas (the label reset never makes an XPBD particle)
-    if (bound)
+    if (bound) {
         st3(B.delta, n, p,
             (a == 1 && m == 2) ? bound_delta(B, n, p, a, m, as_stored(x0), as_stored(v0)) : f3{0.0f, 0.0f, 0.0f});
+        if (records_wanted(B)) {
+            const int lab = reset ? 1 : m;
+            const float pm = B.mass[p];
+            const bool valid = a == 1 && pm > 0.0f;
+            B.flags[p] = (valid ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) | (lab == 2 ? kCandXpbd : 0);
+            if (valid) {  // the walk reads no record of an invalid candidate
+                ((float4*)B.xr)[p] = make_float4(x0.x, x0.y, x0.z, B.radius[p]);
+                ((float4*)B.vw)[p] = make_float4(v0.x, v0.y, v0.z, 1.0f / pm);
+            }
+        }
+    }
 }
 
 // my_solve_particle_bound_contacts writing (not adding) the delta of every
@@ -783,6 +895,14 @@ __global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ a
         if (bound) {
             const int m = mat[t];
             if (m == 2) nd = bound_delta(B, n, t, a, m, as_stored(xn), as_stored(vn));
+            // labels, mass and radius are those the records were packed from (lock
+            // writes change no flag): only the positions and velocities are new
+            if (records_wanted(B)) {
+                float* r = B.xr + 4L * t;
+                float* w = B.vw + 4L * t;
+                r[0] = xn.x, r[1] = xn.y, r[2] = xn.z;
+                w[0] = vn.x, w[1] = vn.y, w[2] = vn.z;
+            }
         }
     }
     if (bound) st3(B.delta, n, t, nd);

@@ -103,7 +103,9 @@ class Simulation:
         """Permute every per-particle array in HBM into the fused substep's tile
         order (tile by tile, cell by cell, input index ascending inside a cell),
         so that each tile's particles are contiguous and the substep's loads
-        coalesce.  ``sim.particle_id`` holds each slot's input index; the bins
+        coalesce.  Active XPBD particles follow in the same tile order (the
+        contact walk gathers the particles of neighbouring cells), then the
+        rest in input order.  ``sim.particle_id`` holds each slot's input index; the bins
         and hash tables order particles by it, so a reordered run gives the same
         bits as one that never reorders (the FP64 convergence sums aside, whose
         order follows the slots), and the host views (numpy, assign) keep the

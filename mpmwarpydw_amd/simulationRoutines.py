@@ -288,6 +288,16 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=
                        xpbd.particleMaxRadius, xpbd.particleBaseRadius])
 
 
+def _contact_records(xpbd, n):
+    """The candidate records of the packed contact solve (mpmx_xpbd_contact_pack),
+    kept on the XPBD state between steps."""
+    rec = getattr(xpbd, "_contact_records", None)
+    if rec is None or rec[2].shape[0] != n:
+        rec = (rt.empty(4 * n, rt.float32), rt.empty(4 * n, rt.float32), rt.empty(n, rt.int32))
+        xpbd._contact_records = rec
+    return rec
+
+
 def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     """xpbdSimulationStep with the copies folded into its kernels: x, v, the
     labels, particle_cumDist_xpbd and the sleep counters end exactly as after
@@ -311,21 +321,27 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
                         minz=float(mn[2]), maxx=float(mx[0]), maxy=float(mx[1]), maxz=float(mx[2]),
                         dt=float(sim.dtxpbd), relaxation=float(xpbd.xpbd_relaxation),
                         delta=xpbd.particle_delta.ptr.value)
+    # the packed contact solve's candidate records, written by begin and each delta
+    # application from the x_int / v_int they produce (none while no XPBD point exists)
+    xgrid = xpbd.particle_grid_xpbd
+    xr, vw, flags = _contact_records(xpbd, n)
+    bounds.xr, bounds.vw, bounds.flags = xr.ptr.value, vw.ptr.value, flags.ptr.value
+    bounds.xpbd_count = xgrid.cell_start.ptr.value + 4 * int(np.prod(xgrid.dims))
     bref = C.byref(bounds)
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
         float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
         sim.numTotalXPBD.ptr, bref if iters > 0 else None, s)  # also zeroes numTotalXPBD and numSleepingXPBD
     contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
-    grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
+    grid = xpbd.particle_grid
     dims = grid.dims
     for it in range(iters):
-        lib("mpmx_solve_particle_particle_contacts_split", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
+        lib("mpmx_solve_particle_particle_contacts_packed", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
             grid.point_ids.ptr, grid.cell_start.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr,
             float(grid.cell_width), dims[0], dims[1], dims[2], *xgrid.dims, xi.ptr, vi.ptr, sim.particle_mass.ptr,
-            sim.particle_radius.ptr, float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
-            float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
-            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock), xpbd.particle_delta.ptr,
-            _aptr(getattr(sim, "_slot_of_id", None)), s)
+            sim.particle_radius.ptr, xr.ptr, vw.ptr, flags.ptr, float(xpbd.dynamicParticleFriction),
+            float(xpbd.staticVelocityThreshold), float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion),
+            float(xpbd.max_radius), float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock),
+            xpbd.particle_delta.ptr, _aptr(getattr(sim, "_slot_of_id", None)), s)
         if it + 1 < iters:
             lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
                 float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, sim.materialLabel.ptr, bref, s)

@@ -39,12 +39,12 @@ def test_ctypes_structs_match_the_library_layout():
     and unknown field names are refused."""
     import ctypes as C
     from mpmwarpydw_amd._lib import Grid, Particles, Scene, StepParams, XpbdBounds, lib
-    out = (C.c_long * 10)()
-    k = lib.raw("mpmx_struct_layout")(out, 10)
-    assert k == 9
-    assert list(out[:9]) == [C.sizeof(Particles), Particles.order.offset, C.sizeof(Grid), C.sizeof(StepParams),
-                             StepParams.bc.offset, Particles.scene.offset, C.sizeof(Scene), C.sizeof(XpbdBounds),
-                             XpbdBounds.delta.offset]
+    out = (C.c_long * 12)()
+    k = lib.raw("mpmx_struct_layout")(out, 12)
+    assert k == 10
+    assert list(out[:10]) == [C.sizeof(Particles), Particles.order.offset, C.sizeof(Grid), C.sizeof(StepParams),
+                              StepParams.bc.offset, Particles.scene.offset, C.sizeof(Scene), C.sizeof(XpbdBounds),
+                              XpbdBounds.delta.offset, XpbdBounds.xpbd_count.offset]
     with pytest.raises(TypeError):
         Particles(n=1, not_a_field=1)
 

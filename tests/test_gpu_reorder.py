@@ -41,9 +41,9 @@ def test_reorder_is_a_pure_layout_change():
 
 
 def test_reorder_with_many_non_mpm_particles():
-    """Particles that are not active MPM particles (XPBD label 2, inactive)
-    follow the MPM ones in input order; with most of a scene among them the
-    reorder stays a pure layout change."""
+    """Active MPM particles come first, then active XPBD particles (label 2) in
+    tile order, then the rest (inactive particles) in input order; with most of
+    a scene in the XPBD phase the reorder stays a pure layout change."""
     A = make_sim(seed=22)
     B = make_sim(seed=22)
     n = A.sim.nPoints
@@ -58,10 +58,11 @@ def test_reorder_with_many_non_mpm_particles():
     assert np.array_equal(np.sort(pid), np.arange(n))
     act = A.sim.activeLabel.numpy()
     mpm = (act == 1) & (lab <= 1)
-    slot_is_mpm = mpm[pid]
-    k = int(mpm.sum())
-    assert slot_is_mpm[:k].all() and not slot_is_mpm[k:].any()
-    assert np.array_equal(pid[k:], np.flatnonzero(~mpm))  # the rest keep input order
+    xp = (act == 1) & (lab == 2)
+    k, m = int(mpm.sum()), int(xp.sum())
+    assert mpm[pid[:k]].all() and xp[pid[k:k + m]].all()  # the block collapse lies inside the grid
+    assert np.array_equal(pid[k + m:], np.flatnonzero(~mpm & ~xp))  # the rest keep input order
+    assert not np.array_equal(pid[k:k + m], np.flatnonzero(xp))  # XPBD particles in tile order
     for counter in range(1, A.sim.mpmStepsPerXpbdStep + 2):
         A.substep(counter)
         B.substep(counter)
