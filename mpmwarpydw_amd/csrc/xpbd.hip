@@ -243,6 +243,24 @@ __global__ void contact_pack_kernel(int n, const int32_t* __restrict__ active, c
                (lab == 2 ? kCandXpbd : 0);
 }
 
+// The contact walk runs in two passes over each lane's candidates.  The scan
+// applies the reference's filters and keeps the candidates within a margin of
+// contact reach, queued per lane in LDS; the drain takes them in the same order,
+// makes the exact contact test and solves the contacts.  The lanes of a wave
+// scan in step, and the drain's divisions run for near candidates only (solved
+// inline, they ran for the whole wave whenever one of its 64 lanes had a contact).
+constexpr int kContactQueue = 24;  // queued candidates per lane
+constexpr int kContactFlush = 12;  // a wave drains once one of its lanes holds this many
+
+// False only where the exact test d - ri - rj <= k_coh fails: the squared distance
+// d2 exceeds (1.001 R)^2, R = |ri| + |rj| + |k_coh|, so d exceeds R by far more
+// than the rounding of that test (a few float ulps of R); the 1e-30 floor keeps
+// the bound where the square of a tiny R would underflow.  NaN keeps a candidate.
+__device__ __forceinline__ bool maybe_in_reach(float d2, float base, float rj) {
+    const float r = 1.001f * (base + fabsf(rj));
+    return !(d2 > fmaxf(r * r, 1.0e-30f));
+}
+
 // my_solve_particle_particle_contacts (xpbdRoutines.py:138-219).
 // With xp_ids/xp_start (a table of the label-2 points only, same hashing and
 // stable order) an MPM particle i walks that table instead of the full one: it
@@ -257,6 +275,7 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
     float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
     const int32_t* __restrict__ slot_of_id, ContactPack pk) {
+    __shared__ int32_t queue[kContactQueue][kBlock];
     // table positions in XCD-contiguous ranges: the cells a block walks are
     // those of the blocks beside it in the table, so they share one L2
     int tid = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -301,9 +320,77 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
         hy = xdimy;
         hz = xdimz;
     }
+    const float base = fabsf(ri) + fabsf(k_coh);
     float k_mu = k_mu_in;
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
     const float alpha = 0.0f;  // XPBD compliance of the reference
+    int queued = 0;
+    auto position = [&](int j) {
+        if constexpr (PACKED) {
+            const float4 pj = pk.xr[j];
+            return f3{pj.x, pj.y, pj.z};
+        } else {
+            return ld3(x, n, j);
+        }
+    };
+    // the exact contact test and the contact of every queued candidate, in order
+    auto drain = [&]() {
+        for (int k = 0; k < queued; k++) {
+            const int j = queue[k][threadIdx.x];
+            f3 xj, vj;
+            float rj, w2;
+            bool j_mpm, j_xpbd;
+            if constexpr (PACKED) {
+                const float4 pj = pk.xr[j];
+                xj = f3{pj.x, pj.y, pj.z};
+                rj = pj.w;
+            } else {
+                xj = ld3(x, n, j);
+                rj = radius[j];
+            }
+            const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
+            const float d = len3(f3{nx, ny, nz});
+            const float err = d - ri - rj;
+            if (!(err <= k_coh)) continue;
+            if constexpr (PACKED) {
+                const float4 qj = pk.vw[j];
+                vj = f3{qj.x, qj.y, qj.z};
+                w2 = qj.w;
+            } else {
+                vj = ld3(v, n, j);
+                w2 = 1.0f / mass[j];
+            }
+            const float denom = w1 + w2;
+            if (!(denom > 0.0f)) continue;
+            if (lock == 1) {
+                if constexpr (PACKED) {
+                    const int fj = pk.flags[j];
+                    j_mpm = (fj & kCandMpm) != 0;
+                    j_xpbd = (fj & kCandXpbd) != 0;
+                } else {
+                    const int lab_j = mat[j];
+                    j_mpm = lab_j <= 1;
+                    j_xpbd = lab_j == 2;
+                }
+                if (i_mpm && j_xpbd) mat[i] = 0;
+                if (i_xpbd && j_mpm) mat[j] = 0;
+            }
+            const float ux = nx / d, uy = ny / d, uz = nz / d;
+            const float rx = vi.x - vj.x, ry = vi.y - vj.y, rz = vi.z - vj.z;
+            const float lambda_n = err / (1.0f + (alpha * dt * dt));
+            const float vn = ux * rx + uy * ry + uz * rz;
+            if (fabsf(vn) < thr * ri) k_mu = smu;
+            const float tx = rx - ux * vn, ty = ry - uy * vn, tz = rz - uz * vn;
+            const float tl = len3(f3{tx, ty, tz});
+            const float a_ = k_mu * lambda_n, b_ = -tl * dt;
+            const float lf = a_ > b_ ? a_ : b_;  // wp.max
+            const f3 nt = normalize3(f3{tx, ty, tz});
+            d0 += (nt.x * lf - ux * lambda_n) / denom * w1;
+            d1 += (nt.y * lf - uy * lambda_n) / denom * w1;
+            d2 += (nt.z * lf - uz * lambda_n) / denom * w1;
+        }
+        queued = 0;
+    };
     for (int cz = zs; cz <= ze; cz++)
         for (int cy = ys; cy <= ye; cy++)
             for (int cx = xs; cx <= xe; cx++) {
@@ -311,61 +398,26 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
                 int b = starts[cell], e = starts[cell + 1];
                 for (int s = b; s < e; s++) {
                     int j = ids[s];
-                    f3 xj, vj;
-                    float rj, w2;
-                    bool j_mpm, j_xpbd;
+                    float rj;
                     if constexpr (PACKED) {
                         const int fj = pk.flags[j];
                         if (!(fj & kCandValid) || j == i) continue;
-                        j_mpm = (fj & kCandMpm) != 0;
-                        if (i_mpm && j_mpm) continue;
-                        j_xpbd = (fj & kCandXpbd) != 0;
-                        const float4 pj = pk.xr[j];
-                        xj = f3{pj.x, pj.y, pj.z};
-                        rj = pj.w;
+                        if (i_mpm && (fj & kCandMpm)) continue;
+                        rj = pk.xr[j].w;
                     } else {
                         if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
-                        const int lab_j = mat[j];
-                        j_mpm = lab_j <= 1;
-                        if (i_mpm && j_mpm) continue;
-                        j_xpbd = lab_j == 2;
-                        xj = ld3(x, n, j);
+                        if (i_mpm && mat[j] <= 1) continue;
                         rj = radius[j];
                     }
-                    float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
-                    float d = sqrtf(nx * nx + ny * ny + nz * nz);
-                    float err = d - ri - rj;
-                    if (!(err <= k_coh)) continue;
-                    if constexpr (PACKED) {
-                        const float4 qj = pk.vw[j];
-                        vj = f3{qj.x, qj.y, qj.z};
-                        w2 = qj.w;
-                    } else {
-                        vj = ld3(v, n, j);
-                        w2 = 1.0f / mass[j];
-                    }
-                    float denom = w1 + w2;
-                    if (denom > 0.0f) {
-                        if (lock == 1) {
-                            if (i_mpm && j_xpbd) mat[i] = 0;
-                            if (i_xpbd && j_mpm) mat[j] = 0;
-                        }
-                        float ux = nx / d, uy = ny / d, uz = nz / d;
-                        float rx = vi.x - vj.x, ry = vi.y - vj.y, rz = vi.z - vj.z;
-                        float lambda_n = err / (1.0f + (alpha * dt * dt));
-                        float vn = ux * rx + uy * ry + uz * rz;
-                        if (fabsf(vn) < thr * ri) k_mu = smu;
-                        float tx = rx - ux * vn, ty = ry - uy * vn, tz = rz - uz * vn;
-                        float tl = sqrtf(tx * tx + ty * ty + tz * tz);
-                        float a_ = k_mu * lambda_n, b_ = -tl * dt;
-                        float lf = a_ > b_ ? a_ : b_;  // wp.max
-                        f3 nt = normalize3(f3{tx, ty, tz});
-                        d0 += (nt.x * lf - ux * lambda_n) / denom * w1;
-                        d1 += (nt.y * lf - uy * lambda_n) / denom * w1;
-                        d2 += (nt.z * lf - uz * lambda_n) / denom * w1;
-                    }
+                    const f3 xj = position(j);
+                    const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
+                    if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
+                    if (queued == kContactQueue) drain();  // a lane with many near candidates
+                    queue[queued++][threadIdx.x] = j;
                 }
+                if (__any(queued >= kContactFlush)) drain();
             }
+    drain();
     // point_ids is a permutation, so particle i has exactly one writer here
     deltas[i] += d0 * relax;
     deltas[n + i] += d1 * relax;
@@ -392,6 +444,14 @@ __global__ void apply_deltas_kernel(int n, const int32_t* __restrict__ active, c
     st3(v_out, n, t, vn);
 }
 
+// counter += the number of lanes of this wave with pred set, in one atomic (the
+// counts are integers, so the total does not depend on the order); every lane
+// of the wave that has not returned must reach it
+__device__ __forceinline__ void wave_count_add(int32_t* counter, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(counter, (int)__popcll(m));
+}
+
 // sleepParticles (xpbdRoutines.py:313-334)
 __global__ void sleep_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat, float thr,
                              const float* __restrict__ radius, const float* __restrict__ x_prev,
@@ -399,15 +459,19 @@ __global__ void sleep_kernel(int n, const int32_t* __restrict__ active, const in
                              int32_t* __restrict__ counters) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    if (active[t] != 1 || mat[t] != 2) return;
-    atomicAdd(&counters[0], 1);
-    f3 a = ld3(x_after, n, t), p = ld3(x_prev, n, t);
-    float d = len3(f3{a.x - p.x, a.y - p.y, a.z - p.z});
-    if (d / dt < thr * radius[t]) {
-        st3(x_after, n, t, p);
-        atomicAdd(&counters[1], 1);
+    const bool xp = active[t] == 1 && mat[t] == 2;
+    bool slept = false;
+    if (xp) {
+        f3 a = ld3(x_after, n, t), p = ld3(x_prev, n, t);
+        float d = len3(f3{a.x - p.x, a.y - p.y, a.z - p.z});
+        if (d / dt < thr * radius[t]) {
+            st3(x_after, n, t, p);
+            slept = true;
+        }
+        dist[t] = dist[t] + d;
     }
-    dist[t] = dist[t] + d;
+    wave_count_add(&counters[0], xp);
+    wave_count_add(&counters[1], slept);
 }
 
 // apply_mpm_contact_impulses (xpbdRoutines.py:259-310)
@@ -928,15 +992,18 @@ __global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, co
     } else {
         vi = ld3(v_int, n, t);
     }
-    if (a == 1 && m == 2) {
-        atomicAdd(&counters[0], 1);
+    const bool xp = a == 1 && m == 2;
+    bool slept = false;
+    if (xp) {
         float d = len3(f3{xi.x - x0.x, xi.y - x0.y, xi.z - x0.z});
         if (d / dt < thr * radius[t]) {
             xi = x0;
-            atomicAdd(&counters[1], 1);
+            slept = true;
         }
         dist[t] = dist[t] + d;
     }
+    wave_count_add(&counters[0], xp);
+    wave_count_add(&counters[1], slept);
     if (a == 1 && m <= 1) {
         xi = x0;
         if (xpbd_only == 0) {

@@ -13,15 +13,22 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def xpbd_scene(seed=0, scale=0.12, radius=None):
+def xpbd_scene(seed=0, scale=0.12, radius=None, cluster=0):
     """Half-XPBD lattice block (contact_heavy at reduced size), jittered.  The
     default radius 0.25 m at 0.25 m spacing overlaps every neighbour by up to
     0.25 m (a stress case for single-kernel parity); radius=0.135 gives the
-    few-centimetre overlaps of a settled pile."""
+    few-centimetre overlaps of a settled pile.  cluster=k packs k XPBD particles
+    into a 0.1 m cube inside one hash cell (far more near candidates per cell
+    than a lattice has)."""
     S = make_sim("contact_heavy", scale=scale, seed=seed, perturb=False)
     rng = np.random.default_rng(seed)
     n = S.sim.nPoints
     x = S.sim.particle_x.numpy().astype(np.float64) + 0.03 * rng.standard_normal((n, 3))
+    if cluster:
+        xp = np.flatnonzero(S.sim.materialLabel.numpy() == 2)
+        pick = rng.choice(xp, cluster, replace=False)
+        centre = (np.floor(x[pick[0]] / S.sim.dx) + 0.5) * S.sim.dx
+        x[pick] = centre + rng.uniform(-0.05, 0.05, (cluster, 3))
     S.sim.particle_x.assign(x)
     S.sim.particle_v.assign(0.5 * rng.standard_normal((n, 3)))
     if radius is not None:
@@ -98,11 +105,12 @@ def test_hashgrid_pair_matches_single_builds(ordered):
     assert d.cell_start.numpy()[-1] == (lab == 2).sum()
 
 
-@pytest.mark.parametrize("lock", [0, 1])
-def test_contact_kernels_parity(o64, lock):
+@pytest.mark.parametrize("lock,cluster", [(0, 0), (1, 0), (0, 64)])
+def test_contact_kernels_parity(o64, lock, cluster):
     from mpmwarpydw_amd import xpbdRoutines as X
     from mpmwarpydw_amd.runtime import launch
-    S = xpbd_scene(2)
+    S = xpbd_scene(2, cluster=cluster)
+    print(f"[contacts] n={S.sim.nPoints} lock={lock} cluster={cluster}", flush=True)
     sim, xpbd = S.sim, S.xpbd
     xpbd.particle_grid.build(sim.particle_x, sim.dx)
     H = host_state(S)
@@ -232,14 +240,15 @@ def test_split_table_is_bit_identical(lock, xdims):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("lock", [0, 1])
-def test_fused_xpbd_step_matches_launch_sequence(lock):
-    """The fused XPBD step leaves x, v, labels, the cumulative distance and the
-    sleep counters bit-identical to the launch-for-launch step."""
+@pytest.mark.parametrize("lock,cluster", [(0, 0), (1, 0), (1, 64)])
+def test_fused_xpbd_step_matches_launch_sequence(lock, cluster):
+    """The fused XPBD step (packed candidate records) leaves x, v, labels, the
+    cumulative distance and the sleep counters bit-identical to the
+    launch-for-launch step."""
     from mpmwarpydw_amd import simulationRoutines as SR
     outs = []
     for fused in (False, True):
-        S = xpbd_scene(5)
+        S = xpbd_scene(5, cluster=cluster)
         S.xpbd.mpm_contact_transition_lock = lock
         (SR.xpbdSimulationStepFused if fused else SR.xpbdSimulationStep)(S.sim, S.mpm, S.xpbd)
         h = host_state(S)
