@@ -325,14 +325,6 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
     const float alpha = 0.0f;  // XPBD compliance of the reference
     int queued = 0;
-    auto position = [&](int j) {
-        if constexpr (PACKED) {
-            const float4 pj = pk.xr[j];
-            return f3{pj.x, pj.y, pj.z};
-        } else {
-            return ld3(x, n, j);
-        }
-    };
     // the exact contact test and the contact of every queued candidate, in order
     auto drain = [&]() {
         for (int k = 0; k < queued; k++) {
@@ -396,20 +388,27 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
             for (int cx = xs; cx <= xe; cx++) {
                 int cell = hash_index(hx, hy, hz, cx, cy, cz);
                 int b = starts[cell], e = starts[cell + 1];
+                // the next candidate's id loads while this one is tested, and a
+                // candidate's record loads with its flags (every id is a particle)
+                int j_next = b < e ? ids[b] : 0;
                 for (int s = b; s < e; s++) {
-                    int j = ids[s];
+                    const int j = j_next;
+                    if (s + 1 < e) j_next = ids[s + 1];
                     float rj;
+                    f3 xj;
                     if constexpr (PACKED) {
                         const int fj = pk.flags[j];
+                        const float4 pj = pk.xr[j];
                         if (!(fj & kCandValid) || j == i) continue;
                         if (i_mpm && (fj & kCandMpm)) continue;
-                        rj = pk.xr[j].w;
+                        rj = pj.w;
+                        xj = f3{pj.x, pj.y, pj.z};
                     } else {
                         if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
                         if (i_mpm && mat[j] <= 1) continue;
                         rj = radius[j];
+                        xj = ld3(x, n, j);
                     }
-                    const f3 xj = position(j);
                     const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
                     if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
                     if (queued == kContactQueue) drain();  // a lane with many near candidates
@@ -973,14 +972,15 @@ __global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ a
 }
 
 // sleepParticles (xpbdRoutines.py:313-334), apply_mpm_contact_impulses (:259-310)
-// and the commit x <- x_int, v <- v_int (simulationRoutines.py:469-470) per particle
-__global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
-                                   float thr, const float* __restrict__ radius, float* __restrict__ dist, float dt,
-                                   float v_max, float k, int xpbd_only, float* __restrict__ x, float* __restrict__ v,
-                                   float* __restrict__ x_int, float* __restrict__ v_int,
-                                   int32_t* __restrict__ counters, const float* __restrict__ delta) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+// and the commit x <- x_int, v <- v_int (simulationRoutines.py:469-470) of particle t;
+// returns whether it is an active XPBD particle and whether it went to sleep
+__device__ __forceinline__ void finish_one(int n, int t, const int32_t* __restrict__ active,
+                                           const int32_t* __restrict__ mat, float thr,
+                                           const float* __restrict__ radius, float* __restrict__ dist, float dt,
+                                           float v_max, float k, int xpbd_only, float* __restrict__ x,
+                                           float* __restrict__ v, float* __restrict__ x_int,
+                                           float* __restrict__ v_int, const float* __restrict__ delta, bool& xp,
+                                           bool& slept) {
     f3 xi = ld3(x_int, n, t), vi;
     const f3 x0 = ld3(x, n, t);
     const int a = active[t], m = mat[t];
@@ -992,8 +992,8 @@ __global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, co
     } else {
         vi = ld3(v_int, n, t);
     }
-    const bool xp = a == 1 && m == 2;
-    bool slept = false;
+    xp = a == 1 && m == 2;
+    slept = false;
     if (xp) {
         float d = len3(f3{xi.x - x0.x, xi.y - x0.y, xi.z - x0.z});
         if (d / dt < thr * radius[t]) {
@@ -1002,8 +1002,6 @@ __global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, co
         }
         dist[t] = dist[t] + d;
     }
-    wave_count_add(&counters[0], xp);
-    wave_count_add(&counters[1], slept);
     if (a == 1 && m <= 1) {
         xi = x0;
         if (xpbd_only == 0) {
@@ -1021,6 +1019,39 @@ __global__ void xpbd_finish_kernel(int n, const int32_t* __restrict__ active, co
     st3(v_int, n, t, vi);
     st3(x, n, t, xi);
     st3(v, n, t, vi);
+}
+
+// finish_one over every particle, a bounded grid striding over them, and the two
+// sleep counts summed per block into one atomic each: atomics on one address
+// serialise in L2, so one per wave cost ~0.2 ms at 2M particles
+constexpr int kFinishBlocks = 2048;
+__global__ void __launch_bounds__(kBlock) xpbd_finish_kernel(
+    int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat, float thr,
+    const float* __restrict__ radius, float* __restrict__ dist, float dt, float v_max, float k, int xpbd_only,
+    float* __restrict__ x, float* __restrict__ v, float* __restrict__ x_int, float* __restrict__ v_int,
+    int32_t* __restrict__ counters, const float* __restrict__ delta) {
+    __shared__ int32_t block_counts[2];
+    if (threadIdx.x < 2) block_counts[threadIdx.x] = 0;
+    __syncthreads();
+    int nxp = 0, nslept = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        bool xp, slept;
+        finish_one(n, t, active, mat, thr, radius, dist, dt, v_max, k, xpbd_only, x, v, x_int, v_int, delta, xp,
+                   slept);
+        nxp += xp;
+        nslept += slept;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        nxp += __shfl_xor(nxp, off);
+        nslept += __shfl_xor(nslept, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (nxp) atomicAdd(&block_counts[0], nxp);
+        if (nslept) atomicAdd(&block_counts[1], nslept);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && block_counts[threadIdx.x]) atomicAdd(&counters[threadIdx.x], block_counts[threadIdx.x]);
 }
 
 }  // namespace mpmx
@@ -1086,8 +1117,11 @@ int mpmx_xpbd_finish(int n, const int32_t* activeLabel, const int32_t* materialL
                                        v_int && counters)),
                  "mpmx_xpbd_finish: bad args");
     if (n == 0) return 0;
-    LAUNCH1(xpbd_finish_kernel, n, n, activeLabel, materialLabel, sleep_threshold, radius, dist_total, dt, v_max,
-            coupling, xpbd_only, x, v, x_int, v_int, counters, delta)
+    const long blocks = grid_for(n, kBlock);
+    MPMX_LAUNCH(xpbd_finish_kernel, dim3((unsigned)(blocks < kFinishBlocks ? blocks : kFinishBlocks)), dim3(kBlock), 0,
+                as_stream(stream), n, activeLabel, materialLabel, sleep_threshold, radius, dist_total, dt, v_max,
+                coupling, xpbd_only, x, v, x_int, v_int, counters, delta);
+    return check_launch("xpbd_finish_kernel", as_stream(stream));
 }
 
 }  // extern "C"
