@@ -486,20 +486,30 @@ int mpmx_solve_particle_particle_contacts_split(
     float static_threshold, float static_friction, float k_cohesion, float max_radius, float dt,
     float relaxation, int contact_lock, float* deltas, const int32_t* slot_of_id, void* stream);
 
-/* Candidate records for mpmx_solve_particle_particle_contacts_packed, one per
- * particle: xr[4n] = (x, y, z, radius), vw[4n] = (vx, vy, vz, 1/mass),
- * flags[n] = 1 (activeLabel == 1 and mass > 0) | 2 (materialLabel <= 1) |
- * 4 (materialLabel == 2); x and v are the SoA arrays the contact solve reads.
- * xr and vw need 16-byte alignment.  No counterpart in the reference: a layout
- * of the data that my_solve_particle_particle_contacts gathers per candidate
+/* Candidate records for mpmx_solve_particle_particle_contacts_packed in
+ * hash-table order: record s < n describes particle point_ids[s] of the full
+ * table, record n + s particle xpbd_point_ids[s] of the XPBD-only table (its
+ * point count is xpbd_cell_start[cells]); xr[8n] = (x, y, z, radius),
+ * vw[8n] = (vx, vy, vz, 1/mass), flags[2n] = 1 (activeLabel == 1 and mass > 0)
+ * | 2 (materialLabel <= 1) | 4 (materialLabel == 2).  x and v are the SoA
+ * arrays the contact solve reads.  With particle_xr (may be NULL; then also
+ * particle_vw, particle_flags) the records are gathered from records in
+ * particle order (mpmx_xpbd_bounds) instead of from activeLabel ... v, which
+ * may then be NULL.  Nothing is written while the XPBD-only table is empty
+ * (the contact solve then reads no record).  Records need 16-byte alignment.  No counterpart in the reference: a layout of the data
+ * that my_solve_particle_particle_contacts gathers per candidate
  * (xpbdRoutines.py:178-196). */
 int mpmx_xpbd_contact_pack(int n, const int32_t* activeLabel, const int32_t* materialLabel,
                            const float* mass, const float* radius, const float* x, const float* v,
+                           const float* particle_xr, const float* particle_vw,
+                           const int32_t* particle_flags, const int32_t* point_ids,
+                           const int32_t* xpbd_point_ids,
+                           const int32_t* xpbd_cell_start, int xpbd_dimx, int xpbd_dimy, int xpbd_dimz,
                            float* xr, float* vw, int32_t* flags, void* stream);
 
 /* mpmx_solve_particle_particle_contacts_split reading each candidate from the
- * records of mpmx_xpbd_contact_pack, packed from the same x, v, labels, mass
- * and radius after their last change (my_solve_particle_particle_contacts,
+ * records of mpmx_xpbd_contact_pack, packed from the same tables, x, v, labels,
+ * mass and radius after their last change (my_solve_particle_particle_contacts,
  * xpbdRoutines.py:138-219).  The results equal those of the split form: lock
  * writes turn label 1 into 0 only, which changes no flag. */
 int mpmx_solve_particle_particle_contacts_packed(
@@ -543,11 +553,12 @@ typedef struct {
     float minx, miny, minz, maxx, maxy, maxz; /* the XPBD box */
     float dt, relaxation;
     float* delta; /* [3][n]: every particle's delta is written */
-    /* The candidate records of the next contact solve (the layout of
-     * mpmx_xpbd_contact_pack), written from the new x_int / v_int and labels by
-     * the kernel that takes these bounds; not written when xr is NULL, nor
-     * while xpbd_count (may be NULL: the XPBD-only table's point count,
-     * cell_start[cells]) reads 0, when the packed contact solve reads none. */
+    /* Candidate records in particle order (xr[4n], vw[4n], flags[n], the record
+     * layout of mpmx_xpbd_contact_pack), written from the new x_int / v_int and
+     * labels by the kernel that takes these bounds, for mpmx_xpbd_contact_pack
+     * to gather into table order; not written when xr is NULL, nor while
+     * xpbd_count (may be NULL: the XPBD-only table's point count,
+     * cell_start[cells]) reads 0, when the contact solve reads no record. */
     float* xr;
     float* vw;
     int32_t* flags;

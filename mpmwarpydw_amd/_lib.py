@@ -93,7 +93,7 @@ SIGNATURES = {
     "mpmx_solve_particle_particle_contacts": [I, P, P, P, P, F, I, I, I, P, P, P, P, F, F, F, F, F, F, F, I, P, P],
     "mpmx_solve_particle_particle_contacts_split": [I, P, P, P, P, P, P, F, I, I, I, I, I, I, P, P, P, P, F, F, F,
                                                     F, F, F, F, I, P, P, P],
-    "mpmx_xpbd_contact_pack": [I, P, P, P, P, P, P, P, P, P, P],
+    "mpmx_xpbd_contact_pack": [I, P, P, P, P, P, P, P, P, P, P, P, P, I, I, I, P, P, P, P],
     "mpmx_solve_particle_particle_contacts_packed": [I, P, P, P, P, P, P, F, I, I, I, I, I, I, P, P, P, P, P, P, P,
                                                      F, F, F, F, F, F, F, I, P, P, P],
     "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],

@@ -213,12 +213,14 @@ __global__ void bound_contacts_kernel(int n, const int32_t* __restrict__ active,
     st3(delta, n, t, f3{da[0], da[1], da[2]});
 }
 
-// Candidate records of the contact walk, one per particle slot: a candidate
-// costs one 16-byte load of its position and radius and one flag word; its
-// velocity and inverse mass load only for a contact.  The flags hold what the
-// walk tests of a candidate's labels.  Lock writes turn label 1 into 0 only,
+// Candidate records of the contact walk in hash-table order: record s of a
+// table describes the particle at position s of that table, so the candidates
+// of a cell are contiguous records, loaded without loading their ids first and
+// whatever order the particle arrays are in.  Records [0, n) follow the full
+// table, [n, n + m) the m points of the XPBD-only table.  The flags hold what
+// the walk tests of a candidate's labels: lock writes turn label 1 into 0 only,
 // so "label <= 1" and "label == 2" cannot change while a contact solve runs,
-// and records packed before it give the tests of the live labels.
+// and records gathered before it give the tests of the live labels.
 struct ContactPack {
     const float4* xr;       // x, y, z, radius
     const float4* vw;       // vx, vy, vz, 1 / mass
@@ -228,19 +230,40 @@ constexpr int kCandValid = 1;  // active == 1 and mass > 0
 constexpr int kCandMpm = 2;    // label <= 1
 constexpr int kCandXpbd = 4;   // label == 2
 
+// thread t < n: record t of the full table; n <= t < 2n: record t of the
+// XPBD-only table; nothing while that table is empty (the walk then stops at
+// once).  From particle-order records (pk, when given: three loads per record)
+// or from the particle arrays.
 __global__ void contact_pack_kernel(int n, const int32_t* __restrict__ active, const int32_t* __restrict__ mat,
                                     const float* __restrict__ mass, const float* __restrict__ radius,
-                                    const float* __restrict__ x, const float* __restrict__ v,
-                                    float4* __restrict__ xr, float4* __restrict__ vw, int32_t* __restrict__ flags) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const float m = mass[t];
-    const int lab = mat[t];
-    const f3 xp = ld3(x, n, t), vp = ld3(v, n, t);
-    xr[t] = make_float4(xp.x, xp.y, xp.z, radius[t]);
-    vw[t] = make_float4(vp.x, vp.y, vp.z, 1.0f / m);
-    flags[t] = ((active[t] == 1 && m > 0.0f) ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) |
-               (lab == 2 ? kCandXpbd : 0);
+                                    const float* __restrict__ x, const float* __restrict__ v, ContactPack pk,
+                                    const int32_t* __restrict__ point_ids, const int32_t* __restrict__ xp_ids,
+                                    const int32_t* __restrict__ xp_count, float4* __restrict__ xr,
+                                    float4* __restrict__ vw, int32_t* __restrict__ flags) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = *xp_count;
+    if (m == 0 || t >= (long)n + m) return;
+    const int j = t < n ? point_ids[t] : xp_ids[t - n];
+    if (j < 0 || j >= n) {
+        flags[t] = 0;
+        return;
+    }
+    if (pk.xr) {
+        const int f = pk.flags[j];
+        flags[t] = f;
+        if (!(f & kCandValid)) return;
+        xr[t] = pk.xr[j];
+        vw[t] = pk.vw[j];
+        return;
+    }
+    const float pm = mass[j];
+    const int lab = mat[j];
+    const bool valid = active[j] == 1 && pm > 0.0f;
+    flags[t] = (valid ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) | (lab == 2 ? kCandXpbd : 0);
+    if (!valid) return;  // the walk reads no more of an invalid candidate
+    const f3 xp = ld3(x, n, j), vp = ld3(v, n, j);
+    xr[t] = make_float4(xp.x, xp.y, xp.z, radius[j]);
+    vw[t] = make_float4(vp.x, vp.y, vp.z, 1.0f / pm);
 }
 
 // The contact walk runs in two passes over each lane's candidates.  The scan
@@ -302,6 +325,8 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     const int32_t* ids = point_ids;
     const int32_t* starts = cell_start;
     int hx = dimx, hy = dimy, hz = dimz;
+    long rec = 0;    // the walked table's first record
+    int self = tid;  // i's position in the walked table (i is not in the XPBD-only one)
     // labels <= 1 and == 2 do not change while this kernel runs (lock turns 1 into 0)
     const int lab_i = mat[i];
     const bool i_mpm = lab_i <= 1, i_xpbd = lab_i == 2;
@@ -319,6 +344,8 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
         hx = xdimx;
         hy = xdimy;
         hz = xdimz;
+        rec = n;
+        self = -1;
     }
     const float base = fabsf(ri) + fabsf(k_coh);
     float k_mu = k_mu_in;
@@ -328,15 +355,17 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     // the exact contact test and the contact of every queued candidate, in order
     auto drain = [&]() {
         for (int k = 0; k < queued; k++) {
-            const int j = queue[k][threadIdx.x];
+            const int sq = queue[k][threadIdx.x];
             f3 xj, vj;
             float rj, w2;
             bool j_mpm, j_xpbd;
+            int j = -1;
             if constexpr (PACKED) {
-                const float4 pj = pk.xr[j];
+                const float4 pj = pk.xr[rec + sq];
                 xj = f3{pj.x, pj.y, pj.z};
                 rj = pj.w;
             } else {
+                j = ids[sq];
                 xj = ld3(x, n, j);
                 rj = radius[j];
             }
@@ -345,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
             const float err = d - ri - rj;
             if (!(err <= k_coh)) continue;
             if constexpr (PACKED) {
-                const float4 qj = pk.vw[j];
+                const float4 qj = pk.vw[rec + sq];
                 vj = f3{qj.x, qj.y, qj.z};
                 w2 = qj.w;
             } else {
@@ -356,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
             if (!(denom > 0.0f)) continue;
             if (lock == 1) {
                 if constexpr (PACKED) {
-                    const int fj = pk.flags[j];
+                    const int fj = pk.flags[rec + sq];
                     j_mpm = (fj & kCandMpm) != 0;
                     j_xpbd = (fj & kCandXpbd) != 0;
                 } else {
@@ -365,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
                     j_xpbd = lab_j == 2;
                 }
                 if (i_mpm && j_xpbd) mat[i] = 0;
-                if (i_xpbd && j_mpm) mat[j] = 0;
+                if (i_xpbd && j_mpm) mat[PACKED ? ids[sq] : j] = 0;
             }
             const float ux = nx / d, uy = ny / d, uz = nz / d;
             const float rx = vi.x - vj.x, ry = vi.y - vj.y, rz = vi.z - vj.z;
@@ -388,22 +417,22 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
             for (int cx = xs; cx <= xe; cx++) {
                 int cell = hash_index(hx, hy, hz, cx, cy, cz);
                 int b = starts[cell], e = starts[cell + 1];
-                // the next candidate's id loads while this one is tested, and a
-                // candidate's record loads with its flags (every id is a particle)
-                int j_next = b < e ? ids[b] : 0;
+                // unpacked: the next candidate's id loads while this one is tested
+                int j_next = (!PACKED && b < e) ? ids[b] : 0;
                 for (int s = b; s < e; s++) {
-                    const int j = j_next;
-                    if (s + 1 < e) j_next = ids[s + 1];
                     float rj;
                     f3 xj;
                     if constexpr (PACKED) {
-                        const int fj = pk.flags[j];
-                        const float4 pj = pk.xr[j];
-                        if (!(fj & kCandValid) || j == i) continue;
+                        // s is j's position in the walked table: s == self exactly when j == i
+                        const int fj = pk.flags[rec + s];
+                        const float4 pj = pk.xr[rec + s];
+                        if (!(fj & kCandValid) || s == self) continue;
                         if (i_mpm && (fj & kCandMpm)) continue;
                         rj = pj.w;
                         xj = f3{pj.x, pj.y, pj.z};
                     } else {
+                        const int j = j_next;
+                        if (s + 1 < e) j_next = ids[s + 1];
                         if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
                         if (i_mpm && mat[j] <= 1) continue;
                         rj = radius[j];
@@ -412,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
                     const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
                     if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
                     if (queued == kContactQueue) drain();  // a lane with many near candidates
-                    queue[queued++][threadIdx.x] = j;
+                    queue[queued++][threadIdx.x] = s;
                 }
                 if (__any(queued >= kContactFlush)) drain();
             }
@@ -730,13 +759,23 @@ int mpmx_solve_particle_particle_contacts_split(int n, const int32_t* activeLabe
 }
 
 int mpmx_xpbd_contact_pack(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* mass,
-                           const float* radius, const float* x, const float* v, float* xr, float* vw, int32_t* flags,
-                           void* stream) {
-    MPMX_REQUIRE(n >= 0 && (n == 0 || (activeLabel && materialLabel && mass && radius && x && v && xr && vw && flags)),
-                 "mpmx_xpbd_contact_pack: bad args");
-    MPMX_REQUIRE(((uintptr_t)xr | (uintptr_t)vw) % 16 == 0, "mpmx_xpbd_contact_pack: records need 16-byte alignment");
+                           const float* radius, const float* x, const float* v, const float* particle_xr,
+                           const float* particle_vw, const int32_t* particle_flags, const int32_t* point_ids,
+                           const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start, int xpbd_dimx,
+                           int xpbd_dimy, int xpbd_dimz, float* xr, float* vw, int32_t* flags, void* stream) {
+    MPMX_REQUIRE(n >= 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0, "mpmx_xpbd_contact_pack: bad sizes");
     if (n == 0) return 0;
-    LAUNCH1(contact_pack_kernel, n, n, activeLabel, materialLabel, mass, radius, x, v, (float4*)xr, (float4*)vw, flags)
+    const bool from_records = particle_xr != nullptr;
+    MPMX_REQUIRE(point_ids && xpbd_point_ids && xpbd_cell_start && xr && vw && flags &&
+                     (from_records ? (particle_vw && particle_flags)
+                                   : (activeLabel && materialLabel && mass && radius && x && v)),
+                 "mpmx_xpbd_contact_pack: null pointer");
+    MPMX_REQUIRE(((uintptr_t)xr | (uintptr_t)vw | (uintptr_t)particle_xr | (uintptr_t)particle_vw) % 16 == 0,
+                 "mpmx_xpbd_contact_pack: records need 16-byte alignment");
+    const long cells = (long)xpbd_dimx * xpbd_dimy * xpbd_dimz;
+    const ContactPack src{(const float4*)particle_xr, (const float4*)particle_vw, particle_flags};
+    LAUNCH1(contact_pack_kernel, 2L * n, n, activeLabel, materialLabel, mass, radius, x, v, src, point_ids,
+            xpbd_point_ids, xpbd_cell_start + cells, (float4*)xr, (float4*)vw, flags)
 }
 
 int mpmx_solve_particle_particle_contacts_packed(
@@ -878,7 +917,7 @@ __device__ __forceinline__ void apply_delta(f3 x0, f3 xp, f3 d, float dt, float 
     }
 }
 
-// whether the kernel taking B writes the contact records (mpmx_xpbd_bounds)
+// whether the kernel taking B writes the particle-order records (mpmx_xpbd_bounds)
 __device__ __forceinline__ bool records_wanted(const mpmx_xpbd_bounds& B) {
     return B.xr && !(B.xpbd_count && *B.xpbd_count == 0);
 }
@@ -918,7 +957,7 @@ __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int
             const float pm = B.mass[p];
             const bool valid = a == 1 && pm > 0.0f;
             B.flags[p] = (valid ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) | (lab == 2 ? kCandXpbd : 0);
-            if (valid) {  // the walk reads no record of an invalid candidate
+            if (valid) {  // no record of an invalid candidate is read
                 ((float4*)B.xr)[p] = make_float4(x0.x, x0.y, x0.z, B.radius[p]);
                 ((float4*)B.vw)[p] = make_float4(v0.x, v0.y, v0.z, 1.0f / pm);
             }
@@ -958,7 +997,7 @@ __global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ a
         if (bound) {
             const int m = mat[t];
             if (m == 2) nd = bound_delta(B, n, t, a, m, as_stored(xn), as_stored(vn));
-            // labels, mass and radius are those the records were packed from (lock
+            // labels, mass and radius are those the records were written from (lock
             // writes change no flag): only the positions and velocities are new
             if (records_wanted(B)) {
                 float* r = B.xr + 4L * t;

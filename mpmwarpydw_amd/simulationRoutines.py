@@ -289,11 +289,13 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=
 
 
 def _contact_records(xpbd, n):
-    """The candidate records of the packed contact solve (mpmx_xpbd_contact_pack),
-    kept on the XPBD state between steps."""
+    """The candidate records of the packed contact solve, kept on the XPBD state
+    between steps: in particle order (n records, written by begin and each delta
+    application) and in table order (2n, mpmx_xpbd_contact_pack)."""
     rec = getattr(xpbd, "_contact_records", None)
     if rec is None or rec[2].shape[0] != n:
-        rec = (rt.empty(4 * n, rt.float32), rt.empty(4 * n, rt.float32), rt.empty(n, rt.int32))
+        rec = (rt.empty(4 * n, rt.float32), rt.empty(4 * n, rt.float32), rt.empty(n, rt.int32),
+               rt.empty(8 * n, rt.float32), rt.empty(8 * n, rt.float32), rt.empty(2 * n, rt.int32))
         xpbd._contact_records = rec
     return rec
 
@@ -321,20 +323,20 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
                         minz=float(mn[2]), maxx=float(mx[0]), maxy=float(mx[1]), maxz=float(mx[2]),
                         dt=float(sim.dtxpbd), relaxation=float(xpbd.xpbd_relaxation),
                         delta=xpbd.particle_delta.ptr.value)
-    # the packed contact solve's candidate records, written by begin and each delta
-    # application from the x_int / v_int they produce (none while no XPBD point exists)
-    xgrid = xpbd.particle_grid_xpbd
-    xr, vw, flags = _contact_records(xpbd, n)
-    bounds.xr, bounds.vw, bounds.flags = xr.ptr.value, vw.ptr.value, flags.ptr.value
-    bounds.xpbd_count = xgrid.cell_start.ptr.value + 4 * int(np.prod(xgrid.dims))
+    grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
+    pxr, pvw, pflags, xr, vw, flags = _contact_records(xpbd, n)
+    bounds.xr, bounds.vw, bounds.flags = pxr.ptr.value, pvw.ptr.value, pflags.ptr.value
+    bounds.xpbd_count = xgrid.cell_start.ptr.value + 4 * int(np.prod(xgrid.dims))  # no records without XPBD points
     bref = C.byref(bounds)
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
         float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
         sim.numTotalXPBD.ptr, bref if iters > 0 else None, s)  # also zeroes numTotalXPBD and numSleepingXPBD
     contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
-    grid = xpbd.particle_grid
     dims = grid.dims
     for it in range(iters):
+        # this iteration's records (begin / the last delta application) into table order
+        lib("mpmx_xpbd_contact_pack", n, None, None, None, None, None, None, pxr.ptr, pvw.ptr, pflags.ptr,
+            grid.point_ids.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr, *xgrid.dims, xr.ptr, vw.ptr, flags.ptr, s)
         lib("mpmx_solve_particle_particle_contacts_packed", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
             grid.point_ids.ptr, grid.cell_start.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr,
             float(grid.cell_width), dims[0], dims[1], dims[2], *xgrid.dims, xi.ptr, vi.ptr, sim.particle_mass.ptr,

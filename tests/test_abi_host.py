@@ -104,3 +104,20 @@ def test_grid_geometry_and_time_step():
 def test_alpha_mapping():
     from mpmwarpydw_amd.scenes import alpha_from_friction_angle
     assert np.isclose(alpha_from_friction_angle(30.0), 1.0 / (np.sqrt(3) * 2.5))
+
+
+def test_ctypes_signatures_have_the_declared_arity():
+    """Every ctypes signature lists one type per parameter of its declaration in
+    include/mpmx.h (a missing or extra entry shifts every later argument)."""
+    from mpmwarpydw_amd._lib import SIGNATURES
+    src = open(os.path.join(ROOT, "include", "mpmx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    bad = []
+    for m in re.finditer(r"\b(mpmx_[A-Za-z0-9_]+)\s*\(([^;{]*?)\)\s*;", src):
+        name, params = m.group(1), m.group(2).strip()
+        if name not in SIGNATURES:
+            continue
+        arity = 0 if params in ("", "void") else params.count(",") + 1
+        if len(SIGNATURES[name]) != arity:
+            bad.append((name, arity, len(SIGNATURES[name])))
+    assert not bad, bad

@@ -257,3 +257,50 @@ def test_fused_xpbd_step_matches_launch_sequence(lock, cluster):
     for k in ("particle_x", "particle_v", "materialLabel", "particle_cumDist_xpbd"):
         assert np.array_equal(a[k], b[k]), k
     assert (ta, sa) == (tb, sb) and ta > 0
+
+
+def test_contact_pack_records():
+    """mpmx_xpbd_contact_pack writes, in table order, the flags of every
+    candidate and the (x, radius), (v, 1/mass) records of the valid ones, the
+    same from the particle arrays as from records in particle order."""
+    import torch
+    from mpmwarpydw_amd import runtime as rt
+    from mpmwarpydw_amd._lib import lib
+    S = xpbd_scene(6)
+    sim, xpbd = S.sim, S.xpbd
+    n = sim.nPoints
+    grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
+    grid.build_pair(xgrid, sim.particle_x, sim.dx, sim.materialLabel, 2, None)
+    H = host_state(S)
+    act, lab, m = H["activeLabel"], H["materialLabel"], np.float32(H["particle_mass"])
+    valid = (act == 1) & (m > 0)
+    pflags = (valid * 1 | (lab <= 1) * 2 | (lab == 2) * 4).astype(np.int32)
+    pxr = np.concatenate([np.float32(H["particle_x"]), np.float32(H["particle_radius"])[:, None]], axis=1)
+    with np.errstate(divide="ignore"):
+        pvw = np.concatenate([np.float32(H["particle_v"]), (np.float32(1) / m)[:, None]], axis=1)
+    ids = grid.point_ids.numpy()
+    nx = int(xgrid.cell_start.numpy()[-1])
+    order = np.concatenate([ids, xgrid.point_ids.numpy()[:nx]])
+    assert nx > 0 and np.array_equal(np.sort(ids), np.arange(n))
+    print(f"[contact pack] n={n} xpbd table={nx}", flush=True)
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), device=rt.device())  # noqa: E731
+    src = [dev(pxr), dev(pvw), dev(pflags)]
+    outs = []
+    for from_records in (False, True):
+        xr = torch.zeros(2 * n, 4, device=rt.device())
+        vw = torch.zeros(2 * n, 4, device=rt.device())
+        fl = torch.full((2 * n,), -1, dtype=torch.int32, device=rt.device())
+        soa = [sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_mass.ptr, sim.particle_radius.ptr,
+               sim.particle_x.ptr, sim.particle_v.ptr]
+        rec = [t.data_ptr() for t in src] if from_records else [None] * 3
+        lib("mpmx_xpbd_contact_pack", n, *([None] * 6 if from_records else soa), *rec, grid.point_ids.ptr,
+            xgrid.point_ids.ptr, xgrid.cell_start.ptr, *xgrid.dims, xr.data_ptr(), vw.data_ptr(), fl.data_ptr(),
+            rt.stream_handle())
+        torch.cuda.synchronize()
+        outs.append((xr.cpu().numpy(), vw.cpu().numpy(), fl.cpu().numpy()))
+    k = n + nx
+    for xr, vw, fl in outs:
+        assert np.array_equal(fl[:k], pflags[order])
+        assert (fl[k:] == -1).all()  # past the XPBD table's points nothing is written
+        ok = valid[order]
+        assert np.array_equal(xr[:k][ok], pxr[order][ok]) and np.array_equal(vw[:k][ok], pvw[order][ok])
