@@ -428,15 +428,18 @@ int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dim
  * materialLabel equals `label` (the split contact solve's, dims label_dim*), in
  * one set of launches: the same contents as mpmx_hashgrid_build_ordered called
  * once per table with its dims, with materialLabel NULL for the first (order may
- * be NULL).  Workspace: mpmx_hashgrid_pair_workspace_bytes. */
+ * be NULL).  point_pos / label_point_pos (both or neither may be NULL; n each):
+ * each point's position in point_ids / label_point_ids, -1 where it is not in
+ * the second table; written only when the second table holds a point (else
+ * left as they were).  Workspace: mpmx_hashgrid_pair_workspace_bytes. */
 size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz, int label_dimx,
                                           int label_dimy, int label_dimz);
 int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy,
                              int dimz, const int32_t* materialLabel, int label, int label_dimx,
                              int label_dimy, int label_dimz, const int32_t* order,
                              int32_t* point_ids, int32_t* cell_start, int32_t* label_point_ids,
-                             int32_t* label_cell_start, void* workspace, size_t ws_bytes,
-                             void* stream);
+                             int32_t* label_cell_start, int32_t* point_pos, int32_t* label_point_pos,
+                             void* workspace, size_t ws_bytes, void* stream);
 
 /* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,
@@ -553,15 +556,19 @@ typedef struct {
     float minx, miny, minz, maxx, maxy, maxz; /* the XPBD box */
     float dt, relaxation;
     float* delta; /* [3][n]: every particle's delta is written */
-    /* Candidate records in particle order (xr[4n], vw[4n], flags[n], the record
-     * layout of mpmx_xpbd_contact_pack), written from the new x_int / v_int and
-     * labels by the kernel that takes these bounds, for mpmx_xpbd_contact_pack
-     * to gather into table order; not written when xr is NULL, nor while
-     * xpbd_count (may be NULL: the XPBD-only table's point count,
-     * cell_start[cells]) reads 0, when the contact solve reads no record. */
+    /* The candidate records of the next contact solve, in table order (2n
+     * records, the layout of mpmx_xpbd_contact_pack), written from the new
+     * x_int / v_int and labels by the kernel that takes these bounds at each
+     * particle's positions pos[p] and n + xpbd_pos[p] (point_pos and
+     * label_point_pos of mpmx_hashgrid_build_pair; -1: not in that table).
+     * Not written when xr is NULL, nor while xpbd_count (may be NULL: the
+     * XPBD-only table's point count, cell_start[cells]) reads 0, when the
+     * contact solve reads no record. */
     float* xr;
     float* vw;
     int32_t* flags;
+    const int32_t* pos;
+    const int32_t* xpbd_pos;
     const int32_t* xpbd_count;
 } mpmx_xpbd_bounds;
 

@@ -84,7 +84,13 @@ struct SortSeg {
     int32_t* cell_start;
     int32_t* sorted;
     long nkeys;  // keys of this table (nkeys of counting_sort_segments when 0)
+    int32_t* pos;  // may be null: pos[i] = i's position in sorted, -1 where i has no key
+    const int32_t* pos_gate;  // may be null: pos is written only while *pos_gate != 0
 };
+// the position array of g that the scatter and the in-run sort write (null: none)
+__device__ __forceinline__ int32_t* seg_pos(const SortSeg& g) {
+    return (g.pos && (!g.pos_gate || *g.pos_gate != 0)) ? g.pos : nullptr;
+}
 struct SortSegs {
     SortSeg seg[kMaxSortSegs];
 };

@@ -305,8 +305,10 @@ __device__ __forceinline__ void cswap(int& oa, int& va, int& ob, int& vb, bool u
     }
 }
 
+// POS: also pos[id] = its new position for the ids of a run it reorders
+template <bool POS = false>
 __device__ __forceinline__ void sort_run(long c, const int32_t* __restrict__ cell_start, int32_t* __restrict__ sorted,
-                                         const int32_t* __restrict__ order) {
+                                         const int32_t* __restrict__ order, int32_t* __restrict__ pos = nullptr) {
     const int b = cell_start[c], e = cell_start[c + 1];
     const int len = e - b;
     if (len <= 1) return;
@@ -331,7 +333,10 @@ __device__ __forceinline__ void sort_run(long c, const int32_t* __restrict__ cel
                 }
 #pragma unroll
         for (int k = 0; k < kRunReg; k++)
-            if (k < len) sorted[b + k] = v[k];
+            if (k < len) {
+                sorted[b + k] = v[k];
+                if constexpr (POS) pos[v[k]] = b + k;
+            }
         return;
     }
     // heapsort in place: O(len log len) for the rare long run (an orphan bin, a
@@ -363,6 +368,8 @@ __device__ __forceinline__ void sort_run(long c, const int32_t* __restrict__ cel
         r[m] = top;
         sift(0, m);
     }
+    if constexpr (POS)
+        for (int k = 0; k < len; k++) pos[r[k]] = b + k;
 }
 
 __global__ void __launch_bounds__(kBlock) bin_sort_runs_kernel(long nkeys, const int32_t* __restrict__ cell_start,
@@ -376,7 +383,12 @@ __global__ void __launch_bounds__(kBlock) bin_sort_runs_seg_kernel(SortSegs t, l
                                                                    const int32_t* __restrict__ order) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
-    if (c < (g.nkeys ? g.nkeys : nkeys)) sort_run(c, g.cell_start, g.sorted, order);
+    if (c >= (g.nkeys ? g.nkeys : nkeys)) return;
+    int32_t* pos = seg_pos(g);
+    if (pos)
+        sort_run<true>(c, g.cell_start, g.sorted, order, pos);
+    else
+        sort_run<false>(c, g.cell_start, g.sorted, order);
 }
 
 __global__ void bin_scatter_seg_kernel(SortSegs t, int nseg, long n) {
@@ -386,7 +398,10 @@ __global__ void bin_scatter_seg_kernel(SortSegs t, int nseg, long n) {
     for (int g = 0; g < kMaxSortSegs; g++) {
         if (g >= nseg) break;
         const int k = t.seg[g].keys[i];
-        if (k >= 0) t.seg[g].sorted[t.seg[g].cell_start[k] + t.seg[g].rank[i]] = (int32_t)i;
+        const int p = k >= 0 ? t.seg[g].cell_start[k] + t.seg[g].rank[i] : -1;
+        if (k >= 0) t.seg[g].sorted[p] = (int32_t)i;
+        int32_t* pos = seg_pos(t.seg[g]);
+        if (pos) pos[i] = p;  // the in-run sort moves the points it reorders
     }
 }
 

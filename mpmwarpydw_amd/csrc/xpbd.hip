@@ -590,7 +590,8 @@ size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz, i
 static int hash_tables_build(int ntab, int n, const float* x, float cell_width, const int (*dims)[3],
                              const int32_t* const* labels, const int* label, int32_t* const* point_ids,
                              int32_t* const* cell_start, void* workspace, size_t ws_bytes, void* stream,
-                             const int32_t* order) {
+                             const int32_t* order, int32_t* const* point_pos = nullptr,
+                             const int32_t* pos_gate = nullptr) {
     MPMX_REQUIRE(n >= 0 && cell_width > 0.0f, "mpmx_hashgrid_build: bad arguments");
     for (int g = 0; g < ntab; g++) {
         MPMX_REQUIRE(dims[g][0] > 0 && dims[g][1] > 0 && dims[g][2] > 0, "mpmx_hashgrid_build: bad dims");
@@ -646,6 +647,8 @@ static int hash_tables_build(int ntab, int n, const float* x, float cell_width, 
         t.seg[g].cell_start = cell_start[g];
         t.seg[g].sorted = point_ids[g];
         t.seg[g].nkeys = h.nkeys[g];
+        t.seg[g].pos = point_pos ? point_pos[g] : nullptr;
+        t.seg[g].pos_gate = pos_gate;
     }
     return counting_sort_segments(t, ntab, n, nkeys_max, s, order);
 }
@@ -661,14 +664,20 @@ static int hashgrid_build(int n, const float* x, float cell_width, int dimx, int
 int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy, int dimz,
                              const int32_t* materialLabel, int label, int label_dimx, int label_dimy, int label_dimz,
                              const int32_t* order, int32_t* point_ids, int32_t* cell_start, int32_t* label_point_ids,
-                             int32_t* label_cell_start, void* workspace, size_t ws_bytes, void* stream) {
+                             int32_t* label_cell_start, int32_t* point_pos, int32_t* label_point_pos,
+                             void* workspace, size_t ws_bytes, void* stream) {
     MPMX_REQUIRE(n == 0 || materialLabel, "mpmx_hashgrid_build_pair: null label array");
+    MPMX_REQUIRE((point_pos == nullptr) == (label_point_pos == nullptr),
+                 "mpmx_hashgrid_build_pair: give both position arrays or neither");
     const int dims[2][3] = {{dimx, dimy, dimz}, {label_dimx, label_dimy, label_dimz}};
     const int32_t* labels[2] = {nullptr, materialLabel};
     const int lab[2] = {0, label};
     int32_t* ids[2] = {point_ids, label_point_ids};
     int32_t* starts[2] = {cell_start, label_cell_start};
-    return hash_tables_build(2, n, x, cell_width, dims, labels, lab, ids, starts, workspace, ws_bytes, stream, order);
+    int32_t* pos[2] = {point_pos, label_point_pos};
+    // the positions serve the contact records, which nothing reads while the second table is empty
+    return hash_tables_build(2, n, x, cell_width, dims, labels, lab, ids, starts, workspace, ws_bytes, stream, order,
+                             point_pos ? pos : nullptr, label_cell_start + table_keys(dims[1]));
 }
 
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
@@ -917,9 +926,20 @@ __device__ __forceinline__ void apply_delta(f3 x0, f3 xp, f3 d, float dt, float 
     }
 }
 
-// whether the kernel taking B writes the particle-order records (mpmx_xpbd_bounds)
+// whether the kernel taking B writes the contact records (mpmx_xpbd_bounds)
 __device__ __forceinline__ bool records_wanted(const mpmx_xpbd_bounds& B) {
     return B.xr && !(B.xpbd_count && *B.xpbd_count == 0);
+}
+
+// particle p's records, in the full table and (when it is in it) the XPBD-only one
+__device__ __forceinline__ void write_records(const mpmx_xpbd_bounds& B, int n, int p, float4 xr, float4 vw) {
+    const long q0 = B.pos[p], q1 = B.xpbd_pos[p];
+    ((float4*)B.xr)[q0] = xr;
+    ((float4*)B.vw)[q0] = vw;
+    if (q1 >= 0) {
+        ((float4*)B.xr)[n + q1] = xr;
+        ((float4*)B.vw)[n + q1] = vw;
+    }
 }
 
 // x_int, v_int <- x, v; integrateParticlesXPBD on active XPBD particles
@@ -956,11 +976,13 @@ __global__ void xpbd_begin_kernel(int n, const int32_t* __restrict__ active, int
             const int lab = reset ? 1 : m;
             const float pm = B.mass[p];
             const bool valid = a == 1 && pm > 0.0f;
-            B.flags[p] = (valid ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) | (lab == 2 ? kCandXpbd : 0);
-            if (valid) {  // no record of an invalid candidate is read
-                ((float4*)B.xr)[p] = make_float4(x0.x, x0.y, x0.z, B.radius[p]);
-                ((float4*)B.vw)[p] = make_float4(v0.x, v0.y, v0.z, 1.0f / pm);
-            }
+            const int f = (valid ? kCandValid : 0) | (lab <= 1 ? kCandMpm : 0) | (lab == 2 ? kCandXpbd : 0);
+            const int q1 = B.xpbd_pos[p];
+            B.flags[B.pos[p]] = f;
+            if (q1 >= 0) B.flags[n + q1] = f;
+            if (valid)  // no record of an invalid candidate is read
+                write_records(B, n, p, make_float4(x0.x, x0.y, x0.z, B.radius[p]),
+                              make_float4(v0.x, v0.y, v0.z, 1.0f / pm));
         }
     }
 }
@@ -1000,10 +1022,14 @@ __global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ a
             // labels, mass and radius are those the records were written from (lock
             // writes change no flag): only the positions and velocities are new
             if (records_wanted(B)) {
-                float* r = B.xr + 4L * t;
-                float* w = B.vw + 4L * t;
-                r[0] = xn.x, r[1] = xn.y, r[2] = xn.z;
-                w[0] = vn.x, w[1] = vn.y, w[2] = vn.z;
+                const long q[2] = {B.pos[t], B.xpbd_pos[t] >= 0 ? n + (long)B.xpbd_pos[t] : -1L};
+                for (int k = 0; k < 2; k++) {
+                    if (q[k] < 0) continue;
+                    float* r = B.xr + 4 * q[k];
+                    float* w = B.vw + 4 * q[k];
+                    r[0] = xn.x, r[1] = xn.y, r[2] = xn.z;
+                    w[0] = vn.x, w[1] = vn.y, w[2] = vn.z;
+                }
             }
         }
     }
@@ -1099,6 +1125,9 @@ extern "C" {
 
 static int check_bounds(int n, const mpmx_xpbd_bounds* B, const char* who) {
     MPMX_REQUIRE(n == 0 || (B->mass && B->radius && B->delta), "%s: null array in the bounds", who);
+    MPMX_REQUIRE(!B->xr || (B->vw && B->flags && B->pos && B->xpbd_pos), "%s: contact records without positions",
+                 who);
+    MPMX_REQUIRE(((uintptr_t)B->xr | (uintptr_t)B->vw) % 16 == 0, "%s: records need 16-byte alignment", who);
     return 0;
 }
 

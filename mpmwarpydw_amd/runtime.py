@@ -255,6 +255,7 @@ class HashGrid:
         ncell = self.dims[0] * self.dims[1] * self.dims[2]
         self.cell_start = zeros(ncell + 1, int32)
         self.point_ids = None
+        self.point_pos = None
         self.cell_width = None
         self.n = 0
         self._ws = Workspace()
@@ -293,11 +294,16 @@ class HashGrid:
                    order: Array | None = None) -> None:
         """``self.build(points, cell_width, order=order)`` and
         ``other.build(points, cell_width, labels=labels, label=label, order=order)``
-        in one set of launches (mpmx_hashgrid_build_pair); each grid keeps its own dims."""
+        in one set of launches (mpmx_hashgrid_build_pair); each grid keeps its own dims.
+        Each grid's ``point_pos`` then holds every point's position in its
+        ``point_ids`` (-1 where a point is not in ``other``), when ``other``
+        holds a point (else it is left as it was)."""
         n = points.shape[0]
         for g in (self, other):
             if g.point_ids is None or g.point_ids.shape[0] != n:
                 g.point_ids = zeros(n, int32)
+            if getattr(g, "point_pos", None) is None or g.point_pos.shape[0] != n:
+                g.point_pos = zeros(n, int32)
             g.n = n
             g.cell_width = float(cell_width)
         dx, dy, dz = self.dims
@@ -305,4 +311,4 @@ class HashGrid:
         ws, nb = self._ws.get(lib.raw("mpmx_hashgrid_pair_workspace_bytes")(n, dx, dy, dz, lx, ly, lz))
         lib("mpmx_hashgrid_build_pair", n, points.ptr, self.cell_width, dx, dy, dz, labels.ptr, int(label), lx, ly, lz,
             None if order is None else order.ptr, self.point_ids.ptr, self.cell_start.ptr, other.point_ids.ptr,
-            other.cell_start.ptr, ws, nb, stream_handle())
+            other.cell_start.ptr, self.point_pos.ptr, other.point_pos.ptr, ws, nb, stream_handle())

@@ -289,13 +289,12 @@ def xpbdSimulationStep(sim, mpm, xpbd, device=None, xpbd_only=False, split_grid=
 
 
 def _contact_records(xpbd, n):
-    """The candidate records of the packed contact solve, kept on the XPBD state
-    between steps: in particle order (n records, written by begin and each delta
-    application) and in table order (2n, mpmx_xpbd_contact_pack)."""
+    """The candidate records of the packed contact solve in table order (2n
+    records, the layout of mpmx_xpbd_contact_pack), kept on the XPBD state
+    between steps; begin and each delta application write them."""
     rec = getattr(xpbd, "_contact_records", None)
-    if rec is None or rec[2].shape[0] != n:
-        rec = (rt.empty(4 * n, rt.float32), rt.empty(4 * n, rt.float32), rt.empty(n, rt.int32),
-               rt.empty(8 * n, rt.float32), rt.empty(8 * n, rt.float32), rt.empty(2 * n, rt.int32))
+    if rec is None or rec[2].shape[0] != 2 * n:
+        rec = (rt.empty(8 * n, rt.float32), rt.empty(8 * n, rt.float32), rt.empty(2 * n, rt.int32))
         xpbd._contact_records = rec
     return rec
 
@@ -324,8 +323,9 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
                         dt=float(sim.dtxpbd), relaxation=float(xpbd.xpbd_relaxation),
                         delta=xpbd.particle_delta.ptr.value)
     grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
-    pxr, pvw, pflags, xr, vw, flags = _contact_records(xpbd, n)
-    bounds.xr, bounds.vw, bounds.flags = pxr.ptr.value, pvw.ptr.value, pflags.ptr.value
+    xr, vw, flags = _contact_records(xpbd, n)
+    bounds.xr, bounds.vw, bounds.flags = xr.ptr.value, vw.ptr.value, flags.ptr.value
+    bounds.pos, bounds.xpbd_pos = grid.point_pos.ptr.value, xgrid.point_pos.ptr.value
     bounds.xpbd_count = xgrid.cell_start.ptr.value + 4 * int(np.prod(xgrid.dims))  # no records without XPBD points
     bref = C.byref(bounds)
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
@@ -334,9 +334,7 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     contact_lock = xpbd.mpm_contact_transition_lock if not xpbd_only else 0
     dims = grid.dims
     for it in range(iters):
-        # this iteration's records (begin / the last delta application) into table order
-        lib("mpmx_xpbd_contact_pack", n, None, None, None, None, None, None, pxr.ptr, pvw.ptr, pflags.ptr,
-            grid.point_ids.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr, *xgrid.dims, xr.ptr, vw.ptr, flags.ptr, s)
+        # the records are this iteration's: begin or the last delta application wrote them
         lib("mpmx_solve_particle_particle_contacts_packed", n, sim.activeLabel.ptr, sim.materialLabel.ptr,
             grid.point_ids.ptr, grid.cell_start.ptr, xgrid.point_ids.ptr, xgrid.cell_start.ptr,
             float(grid.cell_width), dims[0], dims[1], dims[2], *xgrid.dims, xi.ptr, vi.ptr, sim.particle_mass.ptr,

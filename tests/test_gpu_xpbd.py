@@ -103,6 +103,11 @@ def test_hashgrid_pair_matches_single_builds(ordered):
         assert np.array_equal(single.point_ids.numpy(), pair.point_ids.numpy())
         assert np.array_equal(single.cell_start.numpy(), pair.cell_start.numpy())
     assert d.cell_start.numpy()[-1] == (lab == 2).sum()
+    # point_pos inverts point_ids: every point's position in each table, -1 outside it
+    for g, members in ((c, np.ones(n, bool)), (d, lab == 2)):
+        pos, ids = g.point_pos.numpy(), g.point_ids.numpy()[:members.sum()]
+        assert np.array_equal(pos[ids], np.arange(members.sum()))
+        assert (pos[~members] == -1).all()
 
 
 @pytest.mark.parametrize("lock,cluster", [(0, 0), (1, 0), (0, 64)])
