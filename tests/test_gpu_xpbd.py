@@ -309,3 +309,41 @@ def test_contact_pack_records():
         assert (fl[k:] == -1).all()  # past the XPBD table's points nothing is written
         ok = valid[order]
         assert np.array_equal(xr[:k][ok], pxr[order][ok]) and np.array_equal(vw[:k][ok], pvw[order][ok])
+
+
+@pytest.mark.parametrize("lock", [0, 1])
+def test_packed_contacts_from_gathered_records(lock):
+    """The packed contact solve over records that mpmx_xpbd_contact_pack gathers
+    from the particle arrays gives the deltas and labels of the split solve,
+    bit for bit."""
+    import torch
+    from mpmwarpydw_amd import runtime as rt
+    from mpmwarpydw_amd._lib import lib
+    outs = []
+    for packed in (False, True):
+        S = xpbd_scene(7, cluster=32)
+        sim, xpbd = S.sim, S.xpbd
+        n = sim.nPoints
+        grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
+        grid.build_pair(xgrid, sim.particle_x, sim.dx, sim.materialLabel, 2, None)
+        xpbd.particle_delta.zero_()
+        if packed:
+            xr = torch.empty(2 * n, 4, device=rt.device())
+            vw = torch.empty(2 * n, 4, device=rt.device())
+            fl = torch.empty(2 * n, dtype=torch.int32, device=rt.device())
+            lib("mpmx_xpbd_contact_pack", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_mass.ptr,
+                sim.particle_radius.ptr, sim.particle_x.ptr, sim.particle_v.ptr, None, None, None, grid.point_ids.ptr,
+                xgrid.point_ids.ptr, xgrid.cell_start.ptr, *xgrid.dims, xr.data_ptr(), vw.data_ptr(), fl.data_ptr(),
+                rt.stream_handle())
+            rec = [xr.data_ptr(), vw.data_ptr(), fl.data_ptr()]
+        name = "mpmx_solve_particle_particle_contacts_" + ("packed" if packed else "split")
+        print(f"[{name}] n={n} lock={lock}", flush=True)
+        lib(name, n, sim.activeLabel.ptr, sim.materialLabel.ptr, grid.point_ids.ptr, grid.cell_start.ptr,
+            xgrid.point_ids.ptr, xgrid.cell_start.ptr, float(grid.cell_width), *grid.dims, *xgrid.dims,
+            sim.particle_x.ptr, sim.particle_v.ptr, sim.particle_mass.ptr, sim.particle_radius.ptr,
+            *(rec if packed else []), float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
+            float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
+            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), lock, xpbd.particle_delta.ptr, None, rt.stream_handle())
+        outs.append((xpbd.particle_delta.numpy(), sim.materialLabel.numpy()))
+    assert np.abs(outs[0][0]).max() > 0
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
