@@ -431,7 +431,10 @@ int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dim
  * be NULL).  point_pos / label_point_pos (both or neither may be NULL; n each):
  * each point's position in point_ids / label_point_ids, -1 where it is not in
  * the second table; written only when the second table holds a point (else
- * left as they were).  Workspace: mpmx_hashgrid_pair_workspace_bytes. */
+ * left as they were).  lazy != 0: while no point has the label, the first
+ * table (point_ids, cell_start) is left as it was; the fused XPBD step's
+ * contact solve then reads neither table.  Workspace:
+ * mpmx_hashgrid_pair_workspace_bytes. */
 size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz, int label_dimx,
                                           int label_dimy, int label_dimz);
 int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, int dimy,
@@ -439,7 +442,7 @@ int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, 
                              int label_dimy, int label_dimz, const int32_t* order,
                              int32_t* point_ids, int32_t* cell_start, int32_t* label_point_ids,
                              int32_t* label_cell_start, int32_t* point_pos, int32_t* label_point_pos,
-                             void* workspace, size_t ws_bytes, void* stream);
+                             int lazy, void* workspace, size_t ws_bytes, void* stream);
 
 /* reset_mpm_contact_labels, xpbdRoutines.py:3-21 */
 int mpmx_reset_mpm_contact_labels(int n, const int32_t* activeLabel, int32_t* materialLabel,

@@ -86,7 +86,7 @@ SIGNATURES = {
     "mpmx_hashgrid_build_label": [I, P, F, I, I, I, P, I, P, P, P, S, P],
     "mpmx_hashgrid_build_ordered": [I, P, F, I, I, I, P, I, P, P, P, P, S, P],
     "mpmx_hashgrid_pair_workspace_bytes": [I, I, I, I, I, I, I],
-    "mpmx_hashgrid_build_pair": [I, P, F, I, I, I, P, I, I, I, I, P, P, P, P, P, P, P, P, S, P],
+    "mpmx_hashgrid_build_pair": [I, P, F, I, I, I, P, I, I, I, I, P, P, P, P, P, P, P, I, P, S, P],
     "mpmx_reset_mpm_contact_labels": [I, P, P, P],
     "mpmx_integrate_particles_xpbd": [I, P, P, P, P, F, F, F, F, P, P, F, P],
     "mpmx_solve_particle_bound_contacts": [I, P, P, P, P, P, P, F, F, F, F, F, F, F, F, F, F, F, P, P],

@@ -86,7 +86,10 @@ struct SortSeg {
     long nkeys;  // keys of this table (nkeys of counting_sort_segments when 0)
     int32_t* pos;  // may be null: pos[i] = i's position in sorted, -1 where i has no key
     const int32_t* pos_gate;  // may be null: pos is written only while *pos_gate != 0
+    const int32_t* skip_gate; // may be null: nothing of this table is scanned, scattered or sorted
+                              // while *skip_gate == 0
 };
+__device__ __forceinline__ bool seg_skipped(const SortSeg& g) { return g.skip_gate && *g.skip_gate == 0; }
 // the position array of g that the scatter and the in-run sort write (null: none)
 __device__ __forceinline__ int32_t* seg_pos(const SortSeg& g) {
     return (g.pos && (!g.pos_gate || *g.pos_gate != 0)) ? g.pos : nullptr;

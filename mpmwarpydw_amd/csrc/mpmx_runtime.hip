@@ -235,6 +235,7 @@ static_assert(kMaxSortSegs == 2, "the table kernels select seg[0] or seg[1]");
 // the same for each table of a multi-table sort (blockIdx.y = table)
 __global__ void scan_apply_chunks_seg_kernel(SortSegs t, long n, long nb) {
     const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
+    if (seg_skipped(g)) return;  // the whole block
     const long gn = g.nkeys ? g.nkeys : n, gnb = (gn + kScanTile - 1) / kScanTile;
     if ((long)blockIdx.x < gnb) scan_apply_chunks_tile(g.count, gn, g.totals, g.cell_start, gnb);
 }
@@ -383,7 +384,7 @@ __global__ void __launch_bounds__(kBlock) bin_sort_runs_seg_kernel(SortSegs t, l
                                                                    const int32_t* __restrict__ order) {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const SortSeg g = blockIdx.y == 0 ? t.seg[0] : t.seg[1];
-    if (c >= (g.nkeys ? g.nkeys : nkeys)) return;
+    if (c >= (g.nkeys ? g.nkeys : nkeys) || seg_skipped(g)) return;
     int32_t* pos = seg_pos(g);
     if (pos)
         sort_run<true>(c, g.cell_start, g.sorted, order, pos);
@@ -397,6 +398,7 @@ __global__ void bin_scatter_seg_kernel(SortSegs t, int nseg, long n) {
 #pragma unroll
     for (int g = 0; g < kMaxSortSegs; g++) {
         if (g >= nseg) break;
+        if (seg_skipped(t.seg[g])) continue;
         const int k = t.seg[g].keys[i];
         const int p = k >= 0 ? t.seg[g].cell_start[k] + t.seg[g].rank[i] : -1;
         if (k >= 0) t.seg[g].sorted[p] = (int32_t)i;

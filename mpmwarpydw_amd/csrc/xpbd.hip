@@ -44,6 +44,10 @@ struct HashTables {
     const float* x;
     float inv;
     int ntab;
+    // lazy pair build: the count kernel sets this word to 1 when the second table
+    // gets a point, and the first table is left as it was while it reads 0 (null:
+    // not lazy)
+    int32_t* label_any;
     int dimx[kMaxSortSegs], dimy[kMaxSortSegs], dimz[kMaxSortSegs];  // per table
     long nkeys[kMaxSortSegs], nchunks[kMaxSortSegs];
     long nblk0;  // transpose blocks of table 0 (table 1's follow)
@@ -70,6 +74,7 @@ __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
     }
     const int lane = (int)(threadIdx.x & 63);
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    bool in_last = false;
 #pragma unroll
     for (int g = 0; g < kMaxSortSegs; g++) {  // every lane stays for the cross-lane operations
         if (g >= h.ntab) break;
@@ -89,6 +94,14 @@ __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
         if (head && k >= 0) base = atomicAdd(&h.tcount[g][k], next_head - lane);
         base = __shfl(base, my_head);
         if (in) h.rank[g][i] = base + (lane - my_head);
+        in_last = in;
+    }
+    if (h.label_any) {  // uniform over the grid
+        // one store per wave with a point of the second table, skipped once the word
+        // (read past the L1) is set: every writer writes 1, so their order is moot
+        const unsigned long long m = __ballot(in_last);
+        if (lane == 0 && m && __hip_atomic_load(h.label_any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            __hip_atomic_store(h.label_any, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -100,6 +113,7 @@ __global__ void __launch_bounds__(kBlock) hash_counts_transpose_kernel(HashTable
     long b = blockIdx.x;
     const int g = b < h.nblk0 ? 0 : 1;
     if (g == 1) b -= h.nblk0;
+    if (g == 0 && h.label_any && *h.label_any == 0) return;  // lazy: the first table stays as it was
     const int dimx = h.dimx[g], dimy = h.dimy[g], dimz = h.dimz[g];
     const int nbx = (dimx + 63) / 64, nbz = (dimz + 63) / 64;
     const int bx = (int)(b % nbx);
@@ -567,7 +581,7 @@ static size_t hash_tables_bytes(int ntab, int n, const int (*dims)[3]) {
     size_t bytes = 2 * ntab * Carver::bytes_for(nn, 4);
     for (int g = 0; g < ntab; g++) {
         if (dims[g][0] <= 0 || dims[g][1] <= 0 || dims[g][2] <= 0) return 0;
-        bytes += Carver::bytes_for(table_keys(dims[g]) + table_chunks(dims[g]), 4) +
+        bytes += Carver::bytes_for(table_keys(dims[g]) + table_chunks(dims[g]) + 1, 4) +
                  Carver::bytes_for(table_keys(dims[g]), 4);
     }
     return bytes;
@@ -591,7 +605,7 @@ static int hash_tables_build(int ntab, int n, const float* x, float cell_width, 
                              const int32_t* const* labels, const int* label, int32_t* const* point_ids,
                              int32_t* const* cell_start, void* workspace, size_t ws_bytes, void* stream,
                              const int32_t* order, int32_t* const* point_pos = nullptr,
-                             const int32_t* pos_gate = nullptr) {
+                             const int32_t* pos_gate = nullptr, bool lazy = false) {
     MPMX_REQUIRE(n >= 0 && cell_width > 0.0f, "mpmx_hashgrid_build: bad arguments");
     for (int g = 0; g < ntab; g++) {
         MPMX_REQUIRE(dims[g][0] > 0 && dims[g][1] > 0 && dims[g][2] > 0, "mpmx_hashgrid_build: bad dims");
@@ -624,14 +638,16 @@ static int hash_tables_build(int ntab, int n, const float* x, float cell_width, 
         if (h.nkeys[g] > nkeys_max) nkeys_max = h.nkeys[g];
     }
     for (int g = 0; g < ntab; g++) {
-        h.tcount[g] = cv.take<int32_t>(h.nkeys[g] + h.nchunks[g]);  // the chunk totals follow the counts
+        h.tcount[g] = cv.take<int32_t>(h.nkeys[g] + h.nchunks[g] + 1);  // the chunk totals follow the counts
         h.totals[g] = h.tcount[g] + h.nkeys[g];
         h.count[g] = cv.take<int32_t>(h.nkeys[g]);
     }
     MPMX_REQUIRE(!cv.overflow, "mpmx_hashgrid_build: workspace carve failed");
+    MPMX_REQUIRE(!lazy || ntab == 2, "mpmx_hashgrid_build: a lazy build needs two tables");
+    if (lazy) h.label_any = h.tcount[1] + h.nkeys[1] + h.nchunks[1];  // zeroed with the counts
     int rc;
     for (int g = 0; g < ntab; g++)
-        if ((rc = fill_i32(h.tcount[g], h.nkeys[g] + h.nchunks[g], 0, s))) return rc;
+        if ((rc = fill_i32(h.tcount[g], h.nkeys[g] + h.nchunks[g] + 1, 0, s))) return rc;
     if (n > 0) {
         MPMX_LAUNCH(hash_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, h);
         if ((rc = check_launch("hash_count", s))) return rc;
@@ -649,6 +665,7 @@ static int hash_tables_build(int ntab, int n, const float* x, float cell_width, 
         t.seg[g].nkeys = h.nkeys[g];
         t.seg[g].pos = point_pos ? point_pos[g] : nullptr;
         t.seg[g].pos_gate = pos_gate;
+        t.seg[g].skip_gate = (g == 0) ? h.label_any : nullptr;
     }
     return counting_sort_segments(t, ntab, n, nkeys_max, s, order);
 }
@@ -665,7 +682,7 @@ int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, 
                              const int32_t* materialLabel, int label, int label_dimx, int label_dimy, int label_dimz,
                              const int32_t* order, int32_t* point_ids, int32_t* cell_start, int32_t* label_point_ids,
                              int32_t* label_cell_start, int32_t* point_pos, int32_t* label_point_pos,
-                             void* workspace, size_t ws_bytes, void* stream) {
+                             int lazy, void* workspace, size_t ws_bytes, void* stream) {
     MPMX_REQUIRE(n == 0 || materialLabel, "mpmx_hashgrid_build_pair: null label array");
     MPMX_REQUIRE((point_pos == nullptr) == (label_point_pos == nullptr),
                  "mpmx_hashgrid_build_pair: give both position arrays or neither");
@@ -677,7 +694,7 @@ int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, 
     int32_t* pos[2] = {point_pos, label_point_pos};
     // the positions serve the contact records, which nothing reads while the second table is empty
     return hash_tables_build(2, n, x, cell_width, dims, labels, lab, ids, starts, workspace, ws_bytes, stream, order,
-                             point_pos ? pos : nullptr, label_cell_start + table_keys(dims[1]));
+                             point_pos ? pos : nullptr, label_cell_start + table_keys(dims[1]), lazy != 0);
 }
 
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,

@@ -291,13 +291,15 @@ class HashGrid:
         return cs[:-1], cs[1:]
 
     def build_pair(self, other: "HashGrid", points: Array, cell_width: float, labels: Array, label: int = 2,
-                   order: Array | None = None) -> None:
+                   order: Array | None = None, lazy: bool = False) -> None:
         """``self.build(points, cell_width, order=order)`` and
         ``other.build(points, cell_width, labels=labels, label=label, order=order)``
         in one set of launches (mpmx_hashgrid_build_pair); each grid keeps its own dims.
         Each grid's ``point_pos`` then holds every point's position in its
         ``point_ids`` (-1 where a point is not in ``other``), when ``other``
-        holds a point (else it is left as it was)."""
+        holds a point (else it is left as it was).  With ``lazy``, while no point
+        has the label, ``self`` is left as it was too (the fused XPBD step's
+        contact solve reads neither grid then)."""
         n = points.shape[0]
         for g in (self, other):
             if g.point_ids is None or g.point_ids.shape[0] != n:
@@ -311,4 +313,4 @@ class HashGrid:
         ws, nb = self._ws.get(lib.raw("mpmx_hashgrid_pair_workspace_bytes")(n, dx, dy, dz, lx, ly, lz))
         lib("mpmx_hashgrid_build_pair", n, points.ptr, self.cell_width, dx, dy, dz, labels.ptr, int(label), lx, ly, lz,
             None if order is None else order.ptr, self.point_ids.ptr, self.cell_start.ptr, other.point_ids.ptr,
-            other.cell_start.ptr, self.point_pos.ptr, other.point_pos.ptr, ws, nb, stream_handle())
+            other.cell_start.ptr, self.point_pos.ptr, other.point_pos.ptr, 1 if lazy else 0, ws, nb, stream_handle())

@@ -307,7 +307,9 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     outside the step reads them)."""
     n = sim.nPoints
     order = getattr(sim, "particle_id", None)
-    xpbd.particle_grid.build_pair(xpbd.particle_grid_xpbd, sim.particle_x, sim.dx, sim.materialLabel, 2, order)
+    # lazy: with no XPBD particle the contact solve reads neither table, and the full one is not built
+    xpbd.particle_grid.build_pair(xpbd.particle_grid_xpbd, sim.particle_x, sim.dx, sim.materialLabel, 2, order,
+                                  lazy=True)
     g = sim.gravity
     s = stream_handle()
     reset = 1 if (xpbd.mpm_contact_transition_lock and not xpbd_only) else 0

@@ -80,10 +80,10 @@ def test_hashgrid_odd_dims_bit_exact(o64, dims):
     assert np.all(ce[~nonempty] == cs[~nonempty]) and g.cell_start.numpy()[-1] == n
 
 
-@pytest.mark.parametrize("ordered", [False, True])
-def test_hashgrid_pair_matches_single_builds(ordered):
+@pytest.mark.parametrize("ordered,lazy", [(False, False), (True, False), (True, True)])
+def test_hashgrid_pair_matches_single_builds(ordered, lazy):
     """build_pair gives both tables (with different dims) bit for bit as two
-    single builds."""
+    single builds (lazy or not, with points of the label present)."""
     from mpmwarpydw_amd import runtime as rt
     S = xpbd_scene(6)
     sim = S.sim
@@ -96,7 +96,7 @@ def test_hashgrid_pair_matches_single_builds(ordered):
     a.build(sim.particle_x, sim.dx, order=order)
     b.build(sim.particle_x, sim.dx, labels=sim.materialLabel, label=2, order=order)
     c, d = rt.HashGrid(128, 128, 128), rt.HashGrid(64, 64, 64)
-    c.build_pair(d, sim.particle_x, sim.dx, sim.materialLabel, 2, order)
+    c.build_pair(d, sim.particle_x, sim.dx, sim.materialLabel, 2, order, lazy=lazy)
     lab = sim.materialLabel.numpy()
     assert 0 < (lab == 2).sum() < n
     for single, pair in ((a, c), (b, d)):
@@ -108,6 +108,28 @@ def test_hashgrid_pair_matches_single_builds(ordered):
         pos, ids = g.point_pos.numpy(), g.point_ids.numpy()[:members.sum()]
         assert np.array_equal(pos[ids], np.arange(members.sum()))
         assert (pos[~members] == -1).all()
+
+
+def test_lazy_pair_build_without_label_points():
+    """With no point of the label, a lazy pair build leaves the first table as it
+    was and empties the second; a build that is not lazy fills the first."""
+    from mpmwarpydw_amd import runtime as rt
+    S = xpbd_scene(8)
+    sim = S.sim
+    n = sim.nPoints
+    sim.materialLabel.assign(np.ones(n, np.int32))
+    print(f"[lazy pair] n={n}", flush=True)
+    c, d = rt.HashGrid(128, 128, 128), rt.HashGrid(64, 64, 64)
+    c.point_ids = rt.array(np.full(n, -7, np.int32), rt.int32)
+    c.cell_start.assign(np.full(c.cell_start.shape[0], -5, np.int32))
+    c.build_pair(d, sim.particle_x, sim.dx, sim.materialLabel, 2, None, lazy=True)
+    assert (c.point_ids.numpy() == -7).all() and (c.cell_start.numpy() == -5).all()
+    assert (d.cell_start.numpy() == 0).all()
+    c.build_pair(d, sim.particle_x, sim.dx, sim.materialLabel, 2, None, lazy=False)
+    a = rt.HashGrid(128, 128, 128)
+    a.build(sim.particle_x, sim.dx)
+    assert np.array_equal(a.point_ids.numpy(), c.point_ids.numpy())
+    assert np.array_equal(a.cell_start.numpy(), c.cell_start.numpy())
 
 
 @pytest.mark.parametrize("lock,cluster", [(0, 0), (1, 0), (0, 64)])
