@@ -566,7 +566,10 @@ typedef struct {
      * label_point_pos of mpmx_hashgrid_build_pair; -1: not in that table).
      * Not written when xr is NULL, nor while xpbd_count (may be NULL: the
      * XPBD-only table's point count, cell_start[cells]) reads 0, when the
-     * contact solve reads no record. */
+     * contact solve reads no record.  While xpbd_count reads 0,
+     * mpmx_xpbd_apply_deltas_inplace given these bounds applies nothing: with
+     * no label-2 particle every delta is 0, and mpmx_xpbd_finish gives the same
+     * x and v whether the zero deltas were applied or not. */
     float* xr;
     float* vw;
     int32_t* flags;

@@ -267,15 +267,20 @@ def test_split_table_is_bit_identical(lock, xdims):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("lock,cluster", [(0, 0), (1, 0), (1, 64)])
-def test_fused_xpbd_step_matches_launch_sequence(lock, cluster):
-    """The fused XPBD step (packed candidate records) leaves x, v, labels, the
+@pytest.mark.parametrize("lock,cluster,mpm_only", [(0, 0, False), (1, 0, False), (1, 64, False), (0, 0, True)])
+def test_fused_xpbd_step_matches_launch_sequence(lock, cluster, mpm_only):
+    """The fused XPBD step (packed candidate records; with no XPBD particle the
+    lazy tables and skipped delta applications) leaves x, v, labels, the
     cumulative distance and the sleep counters bit-identical to the
     launch-for-launch step."""
     from mpmwarpydw_amd import simulationRoutines as SR
     outs = []
     for fused in (False, True):
         S = xpbd_scene(5, cluster=cluster)
+        if mpm_only:  # labels 0 and 1 only, some inactive particles, velocities above v_max
+            n = S.sim.nPoints
+            S.sim.materialLabel.assign(np.where(np.arange(n) % 3 == 0, 0, 1).astype(np.int32))
+            S.sim.particle_v.assign(np.random.default_rng(5).normal(0, 30.0, (n, 3)))
         S.xpbd.mpm_contact_transition_lock = lock
         (SR.xpbdSimulationStepFused if fused else SR.xpbdSimulationStep)(S.sim, S.mpm, S.xpbd)
         h = host_state(S)
@@ -283,7 +288,7 @@ def test_fused_xpbd_step_matches_launch_sequence(lock, cluster):
     (a, ta, sa), (b, tb, sb) = outs
     for k in ("particle_x", "particle_v", "materialLabel", "particle_cumDist_xpbd"):
         assert np.array_equal(a[k], b[k]), k
-    assert (ta, sa) == (tb, sb) and ta > 0
+    assert (ta, sa) == (tb, sb) and (ta == 0) == mpm_only
 
 
 def test_contact_pack_records():
