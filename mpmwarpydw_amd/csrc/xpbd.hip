@@ -44,7 +44,7 @@ struct HashTables {
     const float* x;
     float inv;
     int ntab;
-    // lazy pair build: the count kernel sets this word to 1 when the second table
+    // lazy pair build: label_any_kernel sets this word to 1 when the second table
     // gets a point, and the first table is left as it was while it reads 0 (null:
     // not lazy)
     int32_t* label_any;
@@ -74,10 +74,10 @@ __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
     }
     const int lane = (int)(threadIdx.x & 63);
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
-    bool in_last = false;
 #pragma unroll
     for (int g = 0; g < kMaxSortSegs; g++) {  // every lane stays for the cross-lane operations
         if (g >= h.ntab) break;
+        if (g == 0 && h.label_any && *h.label_any == 0) continue;  // lazy: the same word for every lane
         const int dx = h.dimx[g], dy = h.dimy[g], dz = h.dimz[g];
         int cx, cy, cz;
         hash_cell(dx, dy, dz, ux, uy, uz, cx, cy, cz);
@@ -94,15 +94,27 @@ __global__ void __launch_bounds__(kBlock) hash_count_kernel(HashTables h) {
         if (head && k >= 0) base = atomicAdd(&h.tcount[g][k], next_head - lane);
         base = __shfl(base, my_head);
         if (in) h.rank[g][i] = base + (lane - my_head);
-        in_last = in;
     }
-    if (h.label_any) {  // uniform over the grid
-        // one store per wave with a point of the second table, skipped once the word
-        // (read past the L1) is set: every writer writes 1, so their order is moot
-        const unsigned long long m = __ballot(in_last);
-        if (lane == 0 && m && __hip_atomic_load(h.label_any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-            __hip_atomic_store(h.label_any, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+}
+
+// *flag = 1 when some labels[i] == label (the word is zeroed before; kernels after
+// this one read it): one check per block that finds one, and a store unless the
+// check already reads 1.  Every writer writes 1, so their order is moot, and the
+// accesses need no coherence beyond the kernel's end: workgroup-scope relaxed
+// atomics are plain cached loads and stores (agent scope bypassed the caches,
+// and thousands of blocks then queued on the one word in memory).
+__global__ void __launch_bounds__(kBlock) label_any_kernel(int n, const int32_t* __restrict__ labels, int label,
+                                                           int32_t* flag) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int any = __syncthreads_or(i < n && labels[i] == label);
+    if (threadIdx.x == 0 && any && __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+        __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// dst[0..n) = 0 unless *gate reads 0
+__global__ void fill_zero_gated_kernel(int32_t* __restrict__ dst, long n, const int32_t* __restrict__ gate) {
+    if (*gate == 0) return;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = 0;
 }
 
 // tcount -> count, per table and y plane, through a 64 x 64 (x, z) LDS tile;
@@ -646,8 +658,24 @@ static int hash_tables_build(int ntab, int n, const float* x, float cell_width, 
     MPMX_REQUIRE(!lazy || ntab == 2, "mpmx_hashgrid_build: a lazy build needs two tables");
     if (lazy) h.label_any = h.tcount[1] + h.nkeys[1] + h.nchunks[1];  // zeroed with the counts
     int rc;
-    for (int g = 0; g < ntab; g++)
+    for (int g = ntab - 1; g >= 0; g--) {  // the second table (and the flag word) first
+        if (g == 0 && lazy) {
+            // lazy: first whether any point has the label; the first table's
+            // counts are then zeroed and counted only if one has
+            if (n > 0) {
+                MPMX_LAUNCH(label_any_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, labels[1], label[1],
+                            h.label_any);
+                if ((rc = check_launch("label_any", s))) return rc;
+            }
+            const long cnt = h.nkeys[0] + h.nchunks[0] + 1;
+            const long blocks = (cnt + kBlock - 1) / kBlock;
+            MPMX_LAUNCH(fill_zero_gated_kernel, dim3((unsigned)(blocks < 2048 ? blocks : 2048)), dim3(kBlock), 0, s,
+                        h.tcount[0], cnt, (const int32_t*)h.label_any);
+            if ((rc = check_launch("fill_i32", s))) return rc;
+            continue;
+        }
         if ((rc = fill_i32(h.tcount[g], h.nkeys[g] + h.nchunks[g] + 1, 0, s))) return rc;
+    }
     if (n > 0) {
         MPMX_LAUNCH(hash_count_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, h);
         if ((rc = check_launch("hash_count", s))) return rc;
