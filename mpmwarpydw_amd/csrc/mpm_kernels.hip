@@ -173,8 +173,8 @@ __global__ void vlc_accumulate_kernel(int n, const int32_t* __restrict__ active,
     if (!vlc_cell(ld3(x, n, p), minx, miny, minz, inv_dx, gx, gy, gz, k)) return;
     const float div = (C[p] + C[4L * n + p]) + C[8L * n + p];
     const long G = (long)gx * gy * gz;
-    atomicAdd(&acc[k], vlc_fixed((double)div * (double)vol[p]));
-    atomicAdd(&acc[G + k], vlc_fixed((double)vol[p]));
+    atomicAdd(&acc[k], vlc_fixed(div, vol[p]));
+    atomicAdd(&acc[G + k], vlc_fixed(vol[p], 1.0f));
 }
 
 __global__ void vlc_finalize_kernel(long G, unsigned long long* __restrict__ acc, float* __restrict__ cell_div,

@@ -9,7 +9,7 @@
 // functions of the left stretch, so they are evaluated here without a
 // decomposition whenever F_trial is close to a rotation:
 //
-//   b - I = F F^T - I                          (in FP64, then FP32)
+//   b - I = F F^T - I                          (compensated FP32 dot products)
 //   eps_m = 1/2 log(I + (b - I))               (Horner series, terms chosen
 //                                                from |b - I|_F)
 //   elastic:  F = F_trial,  tau = 2 mu eps_m + lam tr(eps_m) I
@@ -89,18 +89,17 @@ MPMX_HD void sym_exp(const float S[6], float ns, float X[6]) {
 }
 
 // b - I = F F^T - I (symmetric, {xx,yy,zz,xy,xz,yz}) and its Frobenius norm.
-// Formed in FP64, where the products of FP32 entries are exact and the
-// cancellation against I (F holds rotations of order 1) costs nothing, so b - I
-// keeps the relative precision of the strain.  Returns whether the series
-// route applies: det F > 0 and |b - I|_F <= 0.25.
+// Formed by the compensated dot product (dot3_plus), so the cancellation
+// against I (F holds rotations of order 1) costs nothing and b - I keeps the
+// relative precision of the strain, in FP32 arithmetic.  Returns whether the
+// series route applies: det F > 0 and |b - I|_F <= 0.25.
 MPMX_HD bool left_strain(const float Ft[9], float Bs[6], float& nb) {
     const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
     for (int q = 0; q < 6; q++) {
         const int i = ii[q], j = jj[q];
-        const double bb = (double)Ft[i * 3 + 0] * (double)Ft[j * 3 + 0] + (double)Ft[i * 3 + 1] * (double)Ft[j * 3 + 1] +
-                          (double)Ft[i * 3 + 2] * (double)Ft[j * 3 + 2];
-        Bs[q] = (float)(i == j ? bb - 1.0 : bb);
+        Bs[q] = dot3_plus(Ft[i * 3 + 0], Ft[j * 3 + 0], Ft[i * 3 + 1], Ft[j * 3 + 1], Ft[i * 3 + 2], Ft[j * 3 + 2],
+                          i == j ? -1.0f : 0.0f);
     }
     nb = sym_norm(Bs);
     const float det = Ft[0] * (Ft[4] * Ft[8] - Ft[5] * Ft[7]) - Ft[1] * (Ft[3] * Ft[8] - Ft[5] * Ft[6]) +

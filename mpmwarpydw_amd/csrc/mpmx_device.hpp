@@ -60,15 +60,45 @@ MPMX_HD f3 normalize3(f3 v) {
     return f3{0.0f, 0.0f, 0.0f};
 }
 
+// a0 b0 + a1 b1 + a2 b2 + c in FP32 arithmetic with about twice FP32's
+// precision, rounded once at the end: the compensated dot product "Dot2" of
+// Ogita, Rump & Oishi (SIAM J. Sci. Comput. 26(6), 2005).  Each product is
+// split exactly into p + e with an FMA (TwoProduct), the running sum is kept
+// exact as hi + err with Knuth's TwoSum, and the error terms are added at the
+// end.  The error is at most one rounding of the result plus about 9 u^2
+// (u = 2^-24) of the sum of the magnitudes of the terms, so F F^T - I keeps the
+// relative precision of a small strain when F holds a rotation of order 1 (the
+// terms are of order 1 and cancel to the strain).  Contraction is off so that
+// the compiler cannot fuse a product into a TwoSum.
+MPMX_HD float dot3_plus(float a0, float b0, float a1, float b1, float a2, float b2, float c) {
+#pragma clang fp contract(off)
+    float p = a0 * b0;
+    float s = fmaf(a0, b0, -p);
+    float h = a1 * b1, r = fmaf(a1, b1, -h);
+    float t = p + h, z = t - p;
+    s += ((p - (t - z)) + (h - z)) + r;
+    p = t;
+    h = a2 * b2;
+    r = fmaf(a2, b2, -h);
+    t = p + h;
+    z = t - p;
+    s += ((p - (t - z)) + (h - z)) + r;
+    p = t;
+    t = p + c;
+    z = t - p;
+    s += (p - (t - z)) + (c - z);
+    return t + s;
+}
+
 // ---------------------------------------------------------------------------
 // 3x3 SVD  F = U diag(s) V^T,  U, V in SO(3), |s| descending, sign(det F) on
 // s[2] (the McAdams et al. 2011 convention of the reference's wp.svd3,
 // mpmRoutines.py:288,339).
 //
 // Method (written for near-identity deformation gradients):
-//  1. A = F^T F - I formed in FP64 (the products of FP32 entries are exact
-//     there) and rounded once, so A carries the strain with relative rather
-//     than absolute FP32 error even when F holds a large rotation.
+//  1. A = F^T F - I formed by the compensated dot product above and rounded
+//     once, so A carries the strain with relative rather than absolute FP32
+//     error even when F holds a large rotation or a reflection.
 //  2. Cyclic Jacobi on A with exact rotations (Golub & Van Loan, sym.schur2),
 //     sweeping until the off-diagonal mass is below FP32 resolution.
 //  3. Sort descending with a sign flip per column swap (V stays in SO(3)).
@@ -136,9 +166,8 @@ MPMX_HD void svd3(const float F[9], Svd3& r) {
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = i; j < 3; j++) {
-            const double ftf = (double)F[0 * 3 + i] * (double)F[0 * 3 + j] + (double)F[1 * 3 + i] * (double)F[1 * 3 + j] +
-                               (double)F[2 * 3 + i] * (double)F[2 * 3 + j];
-            const float v = (float)(i == j ? ftf - 1.0 : ftf);
+            const float v = dot3_plus(F[0 * 3 + i], F[0 * 3 + j], F[1 * 3 + i], F[1 * 3 + j], F[2 * 3 + i],
+                                      F[2 * 3 + j], i == j ? -1.0f : 0.0f);
             A[i * 3 + j] = v;
             A[j * 3 + i] = v;
         }
@@ -190,9 +219,23 @@ MPMX_HD void svd3(const float F[9], Svd3& r) {
     r.s[2] = (B[8] < 0.0f) ? -m2 : m2;
 }
 
+// log(1 + x) to a few ulps from logf alone (Goldberg, "What every computer
+// scientist should know about floating-point arithmetic", 1991, Thm. 4): with
+// u = fl(1 + x), log(u) x / (u - 1) cancels the rounding of 1 + x.  The device
+// library's log1pf extracts exponents through FP64 instructions; this keeps the
+// per-particle kernels free of them.
+MPMX_HD float log1p_f32(float x) {
+#pragma clang fp contract(off)
+    const float u = 1.0f + x;
+    if (u == 1.0f) return x;
+    const float d = u - 1.0f;
+    if (!(fabsf(x) < 1.0e30f)) return logf(u);  // inf, NaN and huge x: u - 1 carries no rounding to undo
+    return logf(u) * (x / d);
+}
+
 // ln(max(s, floor)) with the precision of log1p for s >= floor.
 MPMX_HD float log_clamped(float s, float lam, float floorv) {
-    return (s >= floorv) ? 0.5f * log1pf(lam) : logf(floorv);
+    return (s >= floorv) ? 0.5f * log1p_f32(lam) : logf(floorv);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,16 +487,26 @@ MPMX_HD f3 grid_update_node(float m, f3 vin, int ix, int iy, int iz, int gx, int
 // of tr(C) V and of V are kept in 64-bit fixed point with 2^-40 units, so that
 // integer atomics make them independent of the order of the additions.
 // ---------------------------------------------------------------------------
-constexpr double kVlcScale = 1099511627776.0;  // 2^40
+constexpr float kVlcScale = 1099511627776.0f;     // 2^40
+constexpr float kVlcUnscale = 9.094947017729282e-13f;  // 2^-40
 
-MPMX_HD unsigned long long vlc_fixed(double v) {
-    double q = rint(v * kVlcScale);
-    const double lim = 4.0e18;
-    q = q > lim ? lim : (q < -lim ? -lim : q);
-    return (unsigned long long)(long long)q;
+// a * b in 2^-40 units, rounded to the nearest integer, in FP32 arithmetic:
+// the product is split exactly into p + e (TwoProduct), both parts are scaled
+// by 2^40 (exact), and the integer part of p and the rounded remainder
+// (p - rint p) + e are summed as 64-bit integers.  Clamped to +-4e18.
+MPMX_HD unsigned long long vlc_fixed(float a, float b) {
+#pragma clang fp contract(off)
+    const float p = a * b, e = fmaf(a, b, -p);
+    const float P = p * kVlcScale;
+    const float lim = 4.0e18f;
+    if (!(fabsf(P) <= lim)) return (unsigned long long)(P > 0.0f ? (long long)4000000000000000000LL
+                                                                   : (long long)-4000000000000000000LL);
+    const float qh = rintf(P);
+    const float ql = rintf((P - qh) + e * kVlcScale);
+    return (unsigned long long)((long long)qh + (long long)ql);
 }
 
-MPMX_HD float vlc_unfixed(unsigned long long q) { return (float)((double)(long long)q / kVlcScale); }
+MPMX_HD float vlc_unfixed(unsigned long long q) { return (float)(long long)q * kVlcUnscale; }
 
 // the particle's cell: wp.int truncation of (x - min) / dx, as :841-844
 MPMX_HD bool vlc_cell(f3 x, float minx, float miny, float minz, float inv_dx, int gx, int gy, int gz, long& k) {

@@ -368,8 +368,8 @@ __global__ void vlc_accumulate_substep(mpmx_particles P, Geo geo, float inv_dx, 
     k += d.node0;
     const long n = P.n;
     const float div = (P.C[p] + P.C[4 * n + p]) + P.C[8 * n + p];
-    atomicAdd(&acc[k], vlc_fixed((double)div * (double)P.vol[p]));
-    atomicAdd(&acc[geo.nnodes + k], vlc_fixed((double)P.vol[p]));
+    atomicAdd(&acc[k], vlc_fixed(div, P.vol[p]));
+    atomicAdd(&acc[geo.nnodes + k], vlc_fixed(P.vol[p], 1.0f));
 }
 
 __global__ void vlc_apply_substep(mpmx_particles P, Geo geo, float inv_dx, const unsigned long long* __restrict__ acc) {

@@ -76,7 +76,9 @@ def _state(rng, n, kind):
     """Random particle state in SoA float32 for the host update and AoS float64 for the oracle."""
     Q, _ = np.linalg.qr(rng.normal(size=(n, 3, 3)))
     Q *= np.sign(np.linalg.det(Q))[:, None, None]
-    if kind == "small":
+    if kind == "tiny":  # strains far below the rotation: F F^T - I cancels to 1e-8 of its terms
+        U = np.eye(3)[None] + _random_sym(rng, n, 10 ** rng.uniform(-8, -6, n))
+    elif kind == "small":
         U = np.eye(3)[None] + _random_sym(rng, n, 10 ** rng.uniform(-6, -2.5, n))
     elif kind == "moderate":
         U = np.eye(3)[None] + _random_sym(rng, n, rng.uniform(0.003, 0.1, n))
@@ -145,15 +147,15 @@ def _run_oracle(st, eff):
     return {"F": F, "stress": stress, "acc": acc, "damage": D, "ys": ys, "label": lab, "v": v}
 
 
-@pytest.mark.parametrize("kind", ["small", "moderate", "large", "inverted"])
+@pytest.mark.parametrize("kind", ["tiny", "small", "moderate", "large", "inverted"])
 def test_constitutive_update_matches_oracle(hm, kind):
-    rng = np.random.default_rng({"small": 1, "moderate": 2, "large": 3, "inverted": 4}[kind])
+    rng = np.random.default_rng({"tiny": 5, "small": 1, "moderate": 2, "large": 3, "inverted": 4}[kind])
     n = 3000
     st = _state(rng, n, kind)
     route = np.zeros(n, np.int32)
     Ft = np.ascontiguousarray(np.moveaxis(st["F_trial"].reshape(n, 9), 0, 1), np.float32)
     hm.host_route(C.c_long(n), _p(Ft), _p(route))
-    want_fast = {"small": 1.0, "moderate": 1.0, "large": 0.0, "inverted": 0.0}[kind]
+    want_fast = {"tiny": 1.0, "small": 1.0, "moderate": 1.0, "large": 0.0, "inverted": 0.0}[kind]
     assert abs(route.mean() - want_fast) < 0.02, route.mean()
     eff = 0.3
     h = _run_host(hm, st, eff)
@@ -161,11 +163,13 @@ def test_constitutive_update_matches_oracle(hm, kind):
     assert np.array_equal(h["label"], o["label"])
     moved = o["acc"] != st["acc"]
     assert np.array_equal(h["acc"] != st["acc"].astype(np.float32), moved)
-    scale_tau = np.max(np.abs(o["stress"]), axis=(1, 2))
-    err_tau = np.max(np.abs(h["stress"] - o["stress"]), axis=(1, 2)) / np.maximum(scale_tau, 1.0)
+    # FP32 against FP64, particle by particle: 1e-5 of the larger of the
+    # particle's largest stress component and its yield stress (1 Pa floor);
+    # F to a few FP32 ulps of 1
+    scale_tau = np.maximum(np.max(np.abs(o["stress"]), axis=(1, 2)), np.maximum(np.abs(o["ys"]), 1.0))
+    err_tau = np.max(np.abs(h["stress"] - o["stress"]), axis=(1, 2)) / scale_tau
     err_F = np.max(np.abs(h["F"] - o["F"]), axis=(1, 2))
-    # FP32 against FP64: 1e-5 of each particle's stress; F to a few FP32 ulps of 1
-    assert np.quantile(err_tau, 0.999) < 1e-5 and err_tau.max() < 1e-4, (np.quantile(err_tau, 0.999), err_tau.max())
+    assert err_tau.max() < 1e-5, err_tau.max()
     assert err_F.max() < 2e-6, err_F.max()
     # plastic increments, relative to the increment
     dacc = np.abs((h["acc"] - st["acc"]) - (o["acc"] - st["acc"]))
