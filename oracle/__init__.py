@@ -229,15 +229,18 @@ class Oracle:
 
     def my_solve_particle_particle_contacts(self, activeLabel, materialLabel, grid, x, v, mass, radius, k_mu,
                                             thr, static_mu, k_cohesion, max_radius, dt, relaxation, lock,
-                                            deltas):
-        """``grid`` = (point_ids, cell_starts, cell_ends, cell_width, dims)."""
+                                            deltas, order=0):
+        """``grid`` = (point_ids, cell_starts, cell_ends, cell_width, dims).
+        ``order`` 0 is wp.hash_grid_query's candidate order (x fastest, ids
+        ascending); 1 is the reverse and 2 has z fastest (for bounding what the
+        unpinned order changes)."""
         ids, cs, ce, width, dims = grid
         self._chk(activeLabel, materialLabel, x, v, mass, radius, deltas)
-        self.lib.o_solve_particle_particle_contacts(
+        self.lib.o_solve_particle_particle_contacts_ordered(
             _I(len(activeLabel)), _ptr(activeLabel), _ptr(materialLabel), _ptr(ids), _ptr(cs), _ptr(ce),
             self.r(width), _I(dims[0]), _I(dims[1]), _I(dims[2]), _ptr(x), _ptr(v), _ptr(mass), _ptr(radius),
             self.r(k_mu), self.r(thr), self.r(static_mu), self.r(k_cohesion), self.r(max_radius), self.r(dt),
-            self.r(relaxation), _I(int(lock)), _ptr(deltas))
+            self.r(relaxation), _I(int(lock)), _ptr(deltas), _I(int(order)))
 
     def my_apply_particle_deltas(self, activeLabel, materialLabel, x_orig, x_pred, delta, dt, v_max, x_out,
                                  v_out):

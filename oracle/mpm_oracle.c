@@ -806,14 +806,14 @@ EXPORT int o_hashgrid_query(const R *pos, R radius, const int *point_ids, const 
 
 /* X6 my_solve_particle_particle_contacts (xpbdRoutines.py:138-219),
  * candidates in wp.hash_grid_query order: x fastest, then y, then z, and the
- * sorted ids of each cell. */
-EXPORT void o_solve_particle_particle_contacts(int n, const int *active, int *mat,
-                                               const int *point_ids, const int *cell_starts,
-                                               const int *cell_ends, R cell_width, int dimx,
-                                               int dimy, int dimz, const R *x, const R *v,
-                                               const R *mass, const R *radius, R k_mu_in, R thr,
-                                               R static_mu, R k_cohesion, R max_radius, R dt,
-                                               R relax, int lock, R *deltas) {
+ * sorted ids of each cell (order 0).  The order is not pinned by any test or
+ * fixture of the reference; order 1 visits the same candidates in reverse
+ * (cells z, y, x descending, ids descending) and order 2 with z fastest, so
+ * that a test can bound what the order changes. */
+static void solve_contacts(int n, const int *active, int *mat, const int *point_ids, const int *cell_starts,
+                           const int *cell_ends, R cell_width, int dimx, int dimy, int dimz, const R *x,
+                           const R *v, const R *mass, const R *radius, R k_mu_in, R thr, R static_mu,
+                           R k_cohesion, R max_radius, R dt, R relax, int lock, R *deltas, int order) {
     R inv = (R)1 / cell_width;
     for (int tid = 0; tid < n; tid++) {
         if (active[tid] != 1 || mat[tid] < 0) continue;
@@ -832,43 +832,74 @@ EXPORT void o_solve_particle_particle_contacts(int n, const int *active, int *ma
         R k_mu = k_mu_in;
         R delta[3] = {0, 0, 0};
         const R alpha = 0;
-        for (int cz = zs; cz <= ze; cz++)
-            for (int cy = ys; cy <= ye; cy++)
-                for (int cx = xs; cx <= xe; cx++) {
-                    int cell = hash_index(dimx, dimy, dimz, cx, cy, cz);
-                    for (int s = cell_starts[cell]; s < cell_ends[cell]; s++) {
-                        int j = point_ids[s];
-                        if (!(active[j] == 1 && j != i && mass[j] > 0)) continue;
-                        if (mat[i] <= 1 && mat[j] <= 1) continue;
-                        R nv[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
-                        R d = vlen(nv);
-                        R err = d - radius_i - radius[j];
-                        R w2 = (R)1 / mass[j];
-                        R denom = w1 + w2;
-                        if (err <= k_cohesion && denom > 0) {
-                            if (lock == 1) {
-                                if (mat[i] <= 1 && mat[j] == 2) mat[i] = 0;
-                                if (mat[i] == 2 && mat[j] <= 1) mat[j] = 0;
-                            }
-                        }
-                        if (err <= k_cohesion && denom > 0) {
-                            R nn[3] = {nv[0] / d, nv[1] / d, nv[2] / d};
-                            R vrel[3] = {vi[0] - v[3 * j], vi[1] - v[3 * j + 1], vi[2] - v[3 * j + 2]};
-                            R lambda_n = err / ((R)1 + (alpha * dt * dt));
-                            R dn[3] = {nn[0] * lambda_n, nn[1] * lambda_n, nn[2] * lambda_n};
-                            R vn = nn[0] * vrel[0] + nn[1] * vrel[1] + nn[2] * vrel[2];
-                            if (fabs((double)vn) < thr * radius_i) k_mu = static_mu;
-                            R vt[3] = {vrel[0] - nn[0] * vn, vrel[1] - nn[1] * vn, vrel[2] - nn[2] * vn};
-                            R a_ = k_mu * lambda_n, b_ = -vlen(vt) * dt;
-                            R lf = a_ > b_ ? a_ : b_;
-                            R nt[3];
-                            vnormalize(vt, nt);
-                            for (int a = 0; a < 3; a++) delta[a] += (nt[a] * lf - dn[a]) / denom * w1;
-                        }
+        const int nx = xe - xs + 1, ny = ye - ys + 1, nz = ze - zs + 1;
+        const int ncell = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+        for (int q = 0; q < ncell; q++) {
+            int cx, cy, cz;
+            if (order == 2) {
+                cz = zs + q % nz; cy = ys + (q / nz) % ny; cx = xs + q / (nz * ny);
+            } else {
+                const int r = order == 1 ? ncell - 1 - q : q;
+                cx = xs + r % nx; cy = ys + (r / nx) % ny; cz = zs + r / (nx * ny);
+            }
+            int cell = hash_index(dimx, dimy, dimz, cx, cy, cz);
+            const int c0 = cell_starts[cell], c1 = cell_ends[cell];
+            for (int u = c0; u < c1; u++) {
+                const int s = order == 1 ? c0 + c1 - 1 - u : u;
+                int j = point_ids[s];
+                if (!(active[j] == 1 && j != i && mass[j] > 0)) continue;
+                if (mat[i] <= 1 && mat[j] <= 1) continue;
+                R nv[3] = {xi[0] - x[3 * j], xi[1] - x[3 * j + 1], xi[2] - x[3 * j + 2]};
+                R d = vlen(nv);
+                R err = d - radius_i - radius[j];
+                R w2 = (R)1 / mass[j];
+                R denom = w1 + w2;
+                if (err <= k_cohesion && denom > 0) {
+                    if (lock == 1) {
+                        if (mat[i] <= 1 && mat[j] == 2) mat[i] = 0;
+                        if (mat[i] == 2 && mat[j] <= 1) mat[j] = 0;
                     }
                 }
+                if (err <= k_cohesion && denom > 0) {
+                    R nn[3] = {nv[0] / d, nv[1] / d, nv[2] / d};
+                    R vrel[3] = {vi[0] - v[3 * j], vi[1] - v[3 * j + 1], vi[2] - v[3 * j + 2]};
+                    R lambda_n = err / ((R)1 + (alpha * dt * dt));
+                    R dn[3] = {nn[0] * lambda_n, nn[1] * lambda_n, nn[2] * lambda_n};
+                    R vn = nn[0] * vrel[0] + nn[1] * vrel[1] + nn[2] * vrel[2];
+                    if (fabs((double)vn) < thr * radius_i) k_mu = static_mu;
+                    R vt[3] = {vrel[0] - nn[0] * vn, vrel[1] - nn[1] * vn, vrel[2] - nn[2] * vn};
+                    R a_ = k_mu * lambda_n, b_ = -vlen(vt) * dt;
+                    R lf = a_ > b_ ? a_ : b_;
+                    R nt[3];
+                    vnormalize(vt, nt);
+                    for (int a = 0; a < 3; a++) delta[a] += (nt[a] * lf - dn[a]) / denom * w1;
+                }
+            }
+        }
         for (int a = 0; a < 3; a++) deltas[3 * i + a] += delta[a] * relax;
     }
+}
+
+EXPORT void o_solve_particle_particle_contacts(int n, const int *active, int *mat,
+                                               const int *point_ids, const int *cell_starts,
+                                               const int *cell_ends, R cell_width, int dimx,
+                                               int dimy, int dimz, const R *x, const R *v,
+                                               const R *mass, const R *radius, R k_mu_in, R thr,
+                                               R static_mu, R k_cohesion, R max_radius, R dt,
+                                               R relax, int lock, R *deltas) {
+    solve_contacts(n, active, mat, point_ids, cell_starts, cell_ends, cell_width, dimx, dimy, dimz, x, v, mass,
+                   radius, k_mu_in, thr, static_mu, k_cohesion, max_radius, dt, relax, lock, deltas, 0);
+}
+
+EXPORT void o_solve_particle_particle_contacts_ordered(int n, const int *active, int *mat,
+                                                       const int *point_ids, const int *cell_starts,
+                                                       const int *cell_ends, R cell_width, int dimx,
+                                                       int dimy, int dimz, const R *x, const R *v,
+                                                       const R *mass, const R *radius, R k_mu_in, R thr,
+                                                       R static_mu, R k_cohesion, R max_radius, R dt,
+                                                       R relax, int lock, R *deltas, int order) {
+    solve_contacts(n, active, mat, point_ids, cell_starts, cell_ends, cell_width, dimx, dimy, dimz, x, v, mass,
+                   radius, k_mu_in, thr, static_mu, k_cohesion, max_radius, dt, relax, lock, deltas, order);
 }
 
 /* X7 my_apply_particle_deltas (xpbdRoutines.py:221-257) */

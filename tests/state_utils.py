@@ -129,3 +129,209 @@ def oracle_g2p(o, H, S):
     return o.g2p(np.float32(sim.dt), H["activeLabel"], H["materialLabel"], H["particle_x"], H["particle_v"],
                  H["particle_C"], H["particle_F"], H["particle_F_trial"], H["particle_cov"], np.float32(sim.invdx),
                  H["grid_v_out"], bool(mpm.update_cov), np.float32(sim.minBounds))
+
+
+def oracle_xpbd_step(o, H, S, xpbd_only=False):
+    """xpbdSimulationStep (simulationRoutines.py:295-526) on the host copy H
+    with the oracle's kernels, in the reference's launch order: hash grid of the
+    pre-step positions, integration, the label reset when the transition lock
+    is on (not in xpbd_only mode), the iterations of wall contacts, particle
+    contacts and delta application, sleeping, the MPM reset, the commit and,
+    when swellingRatio > 0, swelling.  Updates H in place; returns the sleep
+    counters (total, sleeping)."""
+    sim, xpbd = S.sim, S.xpbd
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    n = sim.nPoints
+    x0, v0 = H["particle_x"].copy(), H["particle_v"].copy()
+    xi, vi = x0.copy(), v0.copy()
+    dims = xpbd.particle_grid.dims
+    ids, cs, ce = o.hashgrid_build(x0, np.float32(sim.dx), dims)
+    o.integrateParticlesXPBD(H["activeLabel"], H["materialLabel"], xi, vi, np.float32(sim.gravity), f32(sim.dtxpbd),
+                             xi, vi, f32(xpbd.particle_v_max))
+    if xpbd.mpm_contact_transition_lock and not xpbd_only:
+        o.reset_mpm_contact_labels(H["activeLabel"], H["materialLabel"])
+    lock = 0 if xpbd_only else int(xpbd.mpm_contact_transition_lock)
+    ft = H["particle_x"].dtype
+    for _ in range(xpbd.xpbd_iterations):
+        d = np.zeros((n, 3), ft)
+        o.my_solve_particle_bound_contacts(H["activeLabel"], H["materialLabel"], xi, vi, H["particle_mass"],
+                                           H["particle_radius"], f32(xpbd.dynamicParticleFriction),
+                                           f32(xpbd.staticVelocityThreshold), f32(xpbd.staticParticleFriction),
+                                           np.float32(xpbd.minBoundsXPBD), np.float32(xpbd.maxBoundsXPBD),
+                                           f32(sim.dtxpbd), f32(xpbd.xpbd_relaxation), d)
+        o.my_solve_particle_particle_contacts(H["activeLabel"], H["materialLabel"],
+                                              (ids, cs, ce, np.float32(sim.dx), dims), xi, vi, H["particle_mass"],
+                                              H["particle_radius"], f32(xpbd.dynamicParticleFriction),
+                                              f32(xpbd.staticVelocityThreshold), f32(xpbd.staticParticleFriction),
+                                              f32(xpbd.particle_cohesion), f32(xpbd.max_radius), f32(sim.dtxpbd),
+                                              f32(xpbd.xpbd_relaxation), lock, d)
+        xo, vo = xi.copy(), vi.copy()
+        o.my_apply_particle_deltas(H["activeLabel"], H["materialLabel"], x0, xi, d, f32(sim.dtxpbd),
+                                   f32(xpbd.particle_v_max), xo, vo)
+        xi, vi = xo, vo
+    counts = o.sleepParticles(H["activeLabel"], H["materialLabel"], f32(xpbd.sleepThreshold), H["particle_radius"],
+                              x0, xi, H["particle_cumDist_xpbd"], f32(sim.dtxpbd))
+    o.apply_mpm_contact_impulses(H["activeLabel"], H["materialLabel"], x0, v0, xi, vi, f32(sim.dtxpbd),
+                                 f32(xpbd.particle_v_max), f32(xpbd.xpbd_mpm_coupling_strength), int(xpbd_only), vi)
+    H["particle_x"], H["particle_v"] = xi, vi
+    if xpbd.swellingRatio > 0:
+        o.swellParticlesType2(H["activeLabel"], H["materialLabel"], H["particle_x_initial_xpbd"], H["particle_x"],
+                              H["particle_radius"], f32(xpbd.swellingActivationFactor), f32(xpbd.swellingMaxFactor),
+                              np.ascontiguousarray(xpbd.particleMaxRadius.numpy(), ft),
+                              np.ascontiguousarray(xpbd.particleBaseRadius.numpy(), ft))
+    return counts
+
+
+def large_and_inverted_F(S, seed, large=0.15, inverted=0.1):
+    """Overwrite F_trial of a fraction of the particles with states that take
+    the SVD route of the constitutive update: rotated stretches of 20-60% (so
+    |F F^T - I|_F > 0.25) and reflections (det F < 0, where the reference's
+    clamps of ln s at 0.01 and 1e-6 act, mpmRoutines.py:341,460-462).  Returns
+    the masks (large, inverted)."""
+    rng = np.random.default_rng(seed)
+    n = S.sim.nPoints
+    Ft = S.mpm.particle_F_trial.numpy().astype(np.float64)
+    r = rng.random(n)
+    big = r < large
+    inv = (r >= large) & (r < large + inverted)
+    m = int(big.sum() + inv.sum())
+    Q, _ = np.linalg.qr(rng.normal(size=(m, 3, 3)))
+    Q *= np.sign(np.linalg.det(Q))[:, None, None]
+    A = rng.normal(size=(m, 3, 3))
+    A = A + A.transpose(0, 2, 1)
+    A /= np.linalg.norm(A, axis=(1, 2))[:, None, None]
+    nb = int(big.sum())
+    mag = np.concatenate([rng.uniform(0.2, 0.6, nb), rng.uniform(0.0, 0.05, m - nb)])
+    U = np.eye(3)[None] + A * mag[:, None, None]
+    U[nb:, 2, :] *= -1.0  # reflections
+    # a few inverted states squashed below the 0.01 clamp on one axis
+    sq = np.flatnonzero(rng.random(m - nb) < 0.3) + nb
+    U[sq, 2, :] *= 0.005
+    sel = np.concatenate([np.flatnonzero(big), np.flatnonzero(inv)])
+    Ft[sel] = Q @ U
+    S.mpm.particle_F_trial.assign(Ft)
+    return big, inv
+
+
+# ---- error scales of the sums the substep takes ------------------------------
+# A per-particle (per-node) error bound relative to the item's own value fails
+# wherever the item is a sum whose terms cancel: the FP32 rounding of each term
+# stays, the sum shrinks.  The scales below are the sums of the magnitudes of
+# the terms (the condition of each sum), computed in FP64 from the same inputs
+# with the reference's weights (mpmRoutines.py:474-569, :715-776).  They only
+# scale errors; the expected values come from the oracle.
+
+def _stencil(x, minb, inv_dx):
+    gp = (x - np.asarray(minb, np.float64)) * inv_dx
+    base = np.trunc(gp - 0.5).astype(np.int64)
+    f = gp - base
+    w = np.stack([0.5 * (1.5 - f) ** 2, 0.75 - (f - 1.0) ** 2, 0.5 * (f - 0.5) ** 2], -1)  # [n][axis][q]
+    dw = np.stack([f - 1.5, -2.0 * (f - 1.0), f - 0.5], -1)
+    return base, f, w, dw
+
+
+def p2g_term_scale(H, S):
+    """Per grid node: (sum of |mass terms|, largest over components of the sum
+    of |momentum terms|) of P2G, over the particles that transfer (active,
+    label <= 1 after the constitutive update, stencil inside the grid)."""
+    sim, mpm = S.sim, S.mpm
+    gx, gy, gz = mpm.gridDims
+    dx, inv_dx, dt = float(np.float32(sim.dx)), float(np.float32(sim.invdx)), float(np.float32(sim.dt))
+    x = H["particle_x"]
+    base, f, w, dw = _stencil(x, sim.minBounds, inv_dx)
+    sel = (H["activeLabel"] == 1) & (H["materialLabel"] <= 1) & np.all(base >= 0, 1) & \
+        (base[:, 0] <= gx - 3) & (base[:, 1] <= gy - 3) & (base[:, 2] <= gz - 3)
+    b, f, w, dw = base[sel], f[sel], w[sel], dw[sel]
+    m, vol = H["particle_mass"][sel], H["particle_vol"][sel]
+    v, C_, tau = H["particle_v"][sel], H["particle_C"][sel], H["particle_stress"][sel]
+    r = float(mpm.rpic_damping)
+    Cm = np.zeros_like(C_) if r < -0.001 else (1 - r) * C_ + 0.5 * r * (C_ - C_.transpose(0, 2, 1))
+    sm = np.zeros(gx * gy * gz)
+    sv = np.zeros((gx * gy * gz, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                wt = w[:, 0, i] * w[:, 1, j] * w[:, 2, k]
+                g = np.stack([dw[:, 0, i] * w[:, 1, j] * w[:, 2, k], w[:, 0, i] * dw[:, 1, j] * w[:, 2, k],
+                              w[:, 0, i] * w[:, 1, j] * dw[:, 2, k]], 1) * inv_dx
+                dpos = (np.array([i, j, k]) - f) * dx
+                t = (wt * m)[:, None] * (np.abs(v) + np.einsum("nij,nj->ni", np.abs(Cm), np.abs(dpos))) + \
+                    (dt * vol)[:, None] * np.einsum("nij,nj->ni", np.abs(tau), np.abs(g))
+                node = ((b[:, 0] + i) * gy + (b[:, 1] + j)) * gz + (b[:, 2] + k)
+                np.add.at(sm, node, wt * m)
+                np.add.at(sv, node, t)
+    return sm, sv.max(axis=1)
+
+
+def g2p_term_scale(H, S):
+    """Per particle: the sums of the magnitudes of the G2P terms of v
+    (sum w |v_i|), C (4 inv_dx sum w |v_i| |q - f|) and grad v
+    (sum |v_i| |grad w|), each its largest over components, from the grid
+    velocities H['grid_v_out']."""
+    sim, mpm = S.sim, S.mpm
+    gx, gy, gz = mpm.gridDims
+    inv_dx = float(np.float32(sim.invdx))
+    base, f, w, dw = _stencil(H["particle_x"], sim.minBounds, inv_dx)
+    n = len(base)
+    ok = np.all(base >= 0, 1) & (base[:, 0] <= gx - 3) & (base[:, 1] <= gy - 3) & (base[:, 2] <= gz - 3)
+    bb = np.where(ok[:, None], base, 0)
+    gv = H["grid_v_out"].reshape(-1, 3)
+    sv, sc, sg = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                wt = w[:, 0, i] * w[:, 1, j] * w[:, 2, k]
+                g = np.stack([dw[:, 0, i] * w[:, 1, j] * w[:, 2, k], w[:, 0, i] * dw[:, 1, j] * w[:, 2, k],
+                              w[:, 0, i] * w[:, 1, j] * dw[:, 2, k]], 1) * inv_dx
+                node = ((bb[:, 0] + i) * gy + (bb[:, 1] + j)) * gz + (bb[:, 2] + k)
+                av = np.abs(gv[node])
+                sv += wt[:, None] * av
+                sc += 4 * inv_dx * wt[:, None] * av * np.abs(np.array([i, j, k]) - f).max(1)[:, None]
+                sg += av * np.abs(g).max(1)[:, None]
+    return sv.max(1) * ok, sc.max(1) * ok, sg.max(1) * ok
+
+
+def dev_strain_norm(F):
+    """|dev ln V| of each F (principal stretches clamped at 0.01 like the return
+    map, mpmRoutines.py:341): the scale of the plastic multiplier, which the
+    return map forms as a difference of near-equal strains."""
+    s = np.linalg.svd(F, compute_uv=False)
+    e = np.log(np.maximum(s, 0.01))
+    d = e - e.mean(axis=1, keepdims=True)
+    return np.linalg.norm(d, axis=1)
+
+
+def oracle_substep(o, H, S):
+    """One mpmSimulationStep on the host copy H (stress, P2G, grid update, G2P,
+    VLC when on); returns the error scales of its sums (p2g_term_scale before
+    G2P moves the particles, g2p_term_scale of the new grid velocities)."""
+    oracle_stress(o, H, S.mpm.eff)
+    oracle_p2g(o, H, S)
+    oracle_grid(o, H, S)
+    sc = {"p2g": p2g_term_scale(H, S), "g2p": g2p_term_scale(H, S)}
+    oracle_g2p(o, H, S)
+    oracle_vlc(o, H, S)
+    return sc
+
+
+def oracle_transfers32(o32, Hin, G, S):
+    """P2G, grid update, G2P and VLC of the FP32 oracle -- the reference's own
+    FP32 arithmetic, whose grid coordinates (x - min) / dx and stencil offsets
+    round exactly as the GPU's -- from the state before the substep, with the
+    GPU's constitutive results (stress, F, labels, and the velocity of the
+    particles that turned XPBD) in place of its own, so that only the transfers
+    are compared.  Returns the result as float64 arrays and the error scales of
+    its sums (oracle_substep)."""
+    H = {k: (np.ascontiguousarray(v, np.float32) if v.dtype == np.float64 else v.copy()) for k, v in Hin.items()}
+    H["particle_stress"] = np.ascontiguousarray(G["particle_stress"], np.float32)
+    H["particle_F"] = np.ascontiguousarray(G["particle_F"], np.float32)
+    H["materialLabel"] = G["materialLabel"].copy()
+    turned = (Hin["materialLabel"] <= 1) & (G["materialLabel"] == 2)
+    H["particle_v"][turned] = G["particle_v"][turned]
+    oracle_p2g(o32, H, S)
+    oracle_grid(o32, H, S)
+    H64 = {k: v.astype(np.float64) if v.dtype == np.float32 else v for k, v in H.items()}
+    sc = {"p2g": p2g_term_scale(H64, S), "g2p": g2p_term_scale(H64, S)}
+    oracle_g2p(o32, H, S)
+    oracle_vlc(o32, H, S)
+    return {k: v.astype(np.float64) if v.dtype == np.float32 else v for k, v in H.items()}, sc

@@ -13,14 +13,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def xpbd_scene(seed=0, scale=0.12, radius=None, cluster=0):
+def xpbd_scene(seed=0, scale=0.12, radius=None, cluster=0, **over):
     """Half-XPBD lattice block (contact_heavy at reduced size), jittered.  The
     default radius 0.25 m at 0.25 m spacing overlaps every neighbour by up to
     0.25 m (a stress case for single-kernel parity); radius=0.135 gives the
     few-centimetre overlaps of a settled pile.  cluster=k packs k XPBD particles
     into a 0.1 m cube inside one hash cell (far more near candidates per cell
     than a lattice has)."""
-    S = make_sim("contact_heavy", scale=scale, seed=seed, perturb=False)
+    S = make_sim("contact_heavy", scale=scale, seed=seed, perturb=False, **over)
     rng = np.random.default_rng(seed)
     n = S.sim.nPoints
     x = S.sim.particle_x.numpy().astype(np.float64) + 0.03 * rng.standard_normal((n, 3))
@@ -177,58 +177,69 @@ def test_contact_kernels_parity(o64, lock, cluster):
         assert (lab_gpu == 0).sum() > 0
 
 
-def test_xpbd_step_parity(o64):
-    """The whole xpbdSimulationStep (integrate, 4 iterations of bound + particle
-    contacts and delta application, sleep, MPM reset, commit) from one state."""
+XPBD_STEP_CASES = [  # (fused, lock, xpbd_only, swelling)
+    (False, 0, False, False), (True, 0, False, False), (False, 1, False, False), (True, 1, False, False),
+    (False, 1, True, False), (True, 0, True, False), (False, 0, False, True), (True, 1, False, True)]
+
+
+@pytest.mark.parametrize("fused,lock,xpbd_only,swell", XPBD_STEP_CASES)
+def test_xpbd_step_parity(o64, fused, lock, xpbd_only, swell):
+    """The whole xpbdSimulationStep (integrate, label reset under the transition
+    lock, 4 iterations of bound + particle contacts and delta application,
+    sleep, MPM reset, commit, swelling) from one state, launch for launch and
+    fused, with the transition lock, in xpbd_only mode (MPM frozen: no reset,
+    no lock, velocities not clamped) and with swelling, against the oracle."""
     from mpmwarpydw_amd import simulationRoutines as SR
-    S = xpbd_scene(3, radius=0.135)
+    from state_utils import oracle_xpbd_step
+    S = xpbd_scene(3, radius=0.135, **({"swellingRatio": 0.6} if swell else {}))
     sim, xpbd = S.sim, S.xpbd
-    H = host_state(S)
-    print(f"[xpbd step] n={sim.nPoints}", flush=True)
-    SR.xpbdSimulationStep(sim, S.mpm, xpbd)
-    G = host_state(S)
-    f32 = lambda v: float(np.float32(v))
+    xpbd.mpm_contact_transition_lock = lock
     n = sim.nPoints
-    x0, v0 = H["particle_x"].copy(), H["particle_v"].copy()
-    xi, vi = x0.copy(), v0.copy()
-    dims = xpbd.particle_grid.dims
-    ids, cs, ce = o64.hashgrid_build(x0, np.float32(sim.dx), dims)
-    o64.integrateParticlesXPBD(H["activeLabel"], H["materialLabel"], xi, vi, np.float32(sim.gravity),
-                               f32(sim.dtxpbd), xi, vi, f32(xpbd.particle_v_max))
-    for _ in range(xpbd.xpbd_iterations):
-        d = np.zeros((n, 3))
-        o64.my_solve_particle_bound_contacts(H["activeLabel"], H["materialLabel"], xi, vi, H["particle_mass"],
-                                             H["particle_radius"], f32(xpbd.dynamicParticleFriction),
-                                             f32(xpbd.staticVelocityThreshold), f32(xpbd.staticParticleFriction),
-                                             np.float32(xpbd.minBoundsXPBD), np.float32(xpbd.maxBoundsXPBD),
-                                             f32(sim.dtxpbd), f32(xpbd.xpbd_relaxation), d)
-        o64.my_solve_particle_particle_contacts(H["activeLabel"], H["materialLabel"],
-                                                (ids, cs, ce, np.float32(sim.dx), dims), xi, vi, H["particle_mass"],
-                                                H["particle_radius"], f32(xpbd.dynamicParticleFriction),
-                                                f32(xpbd.staticVelocityThreshold), f32(xpbd.staticParticleFriction),
-                                                f32(xpbd.particle_cohesion), f32(xpbd.max_radius), f32(sim.dtxpbd),
-                                                f32(xpbd.xpbd_relaxation), 0, d)
-        xo, vo = xi.copy(), vi.copy()
-        o64.my_apply_particle_deltas(H["activeLabel"], H["materialLabel"], x0, xi, d, f32(sim.dtxpbd),
-                                     f32(xpbd.particle_v_max), xo, vo)
-        xi, vi = xo, vo
-    dist = H["particle_cumDist_xpbd"].copy()
-    o64.sleepParticles(H["activeLabel"], H["materialLabel"], f32(xpbd.sleepThreshold), H["particle_radius"], x0,
-                       xi, dist, f32(sim.dtxpbd))
-    o64.apply_mpm_contact_impulses(H["activeLabel"], H["materialLabel"], x0, v0, xi, vi, f32(sim.dtxpbd),
-                                   f32(xpbd.particle_v_max), f32(xpbd.xpbd_mpm_coupling_strength), 0, vi)
+    rng = np.random.default_rng(33)
+    if swell:  # XPBD birth positions 0-3 m away: below, inside and beyond the swelling ramp (1 m to 2 m)
+        off = rng.standard_normal((n, 3))
+        off *= (3.0 * rng.random(n) / np.linalg.norm(off, axis=1))[:, None]
+        xpbd.particle_x_initial_xpbd.assign(sim.particle_x.numpy().astype(np.float64) - off)
+    if lock:  # some MPM particles start locked (label 0); the reset must free those out of contact
+        lab = sim.materialLabel.numpy()
+        lab[(lab == 1) & (rng.random(n) < 0.5)] = 0
+        sim.materialLabel.assign(lab)
+    # velocities above particle_v_max on some MPM particles: the impulse kernel clamps them unless xpbd_only
+    v = sim.particle_v.numpy()
+    fast = (sim.materialLabel.numpy() <= 1) & (rng.random(n) < 0.05)
+    v[fast] *= 4.0 * xpbd.particle_v_max / np.maximum(np.linalg.norm(v[fast], axis=1), 1e-6)[:, None]
+    sim.particle_v.assign(v)
+    H = host_state(S)
+    print(f"[xpbd step] n={n} fused={fused} lock={lock} xpbd_only={xpbd_only} swell={swell}", flush=True)
+    (SR.xpbdSimulationStepFused if fused else SR.xpbdSimulationStep)(sim, S.mpm, xpbd, xpbd_only=xpbd_only)
+    G = host_state(S)
+    total, sleeping = oracle_xpbd_step(o64, H, S, xpbd_only=xpbd_only)
     # Positions are FP32 coordinates of a several-metre domain: one rounding of x
     # is ulp(max|x|), and the step's velocities are (x_new - x_orig) / dt
     # (xpbdRoutines.py:246), so FP32 against FP64 differs by a few ulp(x) in x
     # and a few ulp(x) / dt in v after 4 iterations -- the resolution of the
     # reference's own FP32 state.  Bound: 16 ulp.
-    ulp = float(np.spacing(np.float32(np.max(np.abs(x0)))))
-    ex = np.max(np.abs(G["particle_x"] - xi))
-    ev = np.max(np.abs(G["particle_v"] - vi))
+    ulp = float(np.spacing(np.float32(np.max(np.abs(H["particle_x"])))))
+    ex = np.max(np.abs(G["particle_x"] - H["particle_x"]))
+    ev = np.max(np.abs(G["particle_v"] - H["particle_v"]))
     print(f"  x err {ex:.2e} (ulp {ulp:.2e}), v err {ev:.2e} (ulp/dt {ulp / sim.dtxpbd:.2e})")
     assert ex <= 16 * ulp
-    assert ev <= 16 * ulp / float(np.float32(sim.dtxpbd)) + TOL * np.max(np.abs(vi))
-    assert np.max(np.abs(G["particle_cumDist_xpbd"] - dist)) <= 16 * ulp
+    assert ev <= 16 * ulp / float(np.float32(sim.dtxpbd)) + TOL * np.max(np.abs(H["particle_v"]))
+    assert np.max(np.abs(G["particle_cumDist_xpbd"] - H["particle_cumDist_xpbd"])) <= 16 * ulp
+    assert np.array_equal(G["materialLabel"], H["materialLabel"])
+    assert (int(sim.numTotalXPBD.numpy()[0]), int(sim.numSleepingXPBD.numpy()[0])) == (total, sleeping)
+    lab = H["materialLabel"]
+    if lock and not xpbd_only:
+        assert ((lab == 0) & (H["activeLabel"] == 1)).sum() > 0  # contacts re-locked some
+    if swell:
+        grown = H["particle_radius"] > S.xpbd.particleBaseRadius.numpy() * (1 + 1e-6)
+        assert grown.sum() > 0
+        assert rel_err(G["particle_radius"], H["particle_radius"]) < TOL
+        r = np.asarray(G["particle_radius"]); h = np.asarray(H["particle_radius"])
+        assert np.all(np.abs(r - h) <= TOL * h)
+    if not xpbd_only:  # MPM velocities above v_max were clamped
+        mv = np.linalg.norm(H["particle_v"][(lab <= 1) & (H["activeLabel"] == 1)], axis=1)
+        assert mv.max() <= xpbd.particle_v_max * (1 + 1e-6)
 
 
 @pytest.mark.parametrize("lock,xdims", [(0, 64), (1, 64), (1, 8), (0, 4)])

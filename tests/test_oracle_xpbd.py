@@ -162,3 +162,75 @@ def test_reductions(o64):
     c = o64.countParticlesByType(act, lab)
     assert c == (int(((lab != 2) & (act == 1)).sum()), int(((lab == 2) & (act == 1)).sum()),
                  int(((lab != 2) & (act != 1)).sum()), int(((lab == 2) & (act != 1)).sum()))
+
+
+def _contact_state(seed, vscale, n_side=10):
+    """A contact_heavy-like block: 0.25 m lattice, 0.03 m jitter, radius
+    0.25 m (every neighbour overlaps), XPBD (label 2) in one half and MPM
+    (label 1) in the other, 2% inactive; velocities of scale vscale."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = (g + 0.5) * 0.25 + 0.03 * rng.standard_normal(g.shape) + 1.0
+    n = len(x)
+    lab = np.where(g[:, 0] < n_side // 2, 2, 1).astype(np.int32)
+    act = (rng.random(n) >= 0.02).astype(np.int32)
+    v = vscale * rng.standard_normal((n, 3))
+    return x, v, np.full(n, 39.0625), np.full(n, 0.25), lab, act
+
+
+def _solve(o, st, order, static_mu, dyn_mu, thr):
+    x, v, m, r, lab, act = st
+    ids, cs, ce = o.hashgrid_build(x, 0.5, (128, 128, 128))
+    d = np.zeros_like(x)
+    o.my_solve_particle_particle_contacts(act, lab.copy(), (ids, cs, ce, 0.5, (128, 128, 128)), x, v, m, r, dyn_mu,
+                                          thr, static_mu, 0.0, 0.25, 5e-4, 1.0, 0, d, order=order)
+    return d
+
+
+def _sliding_state(seed, radius, n_side=10):
+    """Exact 0.25 m lattice, shallow overlaps (radius just above 0.125 m),
+    alternate z-layers sliding at +-0.8 m/s along x, and half of the particles
+    with a normal velocity: contacts with |v_n| below the static threshold and
+    contacts above it meet in the same query."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = (g + 0.5) * 0.25 + 1.0
+    n = len(x)
+    lab = np.where(g[:, 0] < n_side // 2, 2, 1).astype(np.int32)
+    v = np.zeros((n, 3))
+    v[:, 0] = np.where(g[:, 2] % 2 == 0, 0.8, -0.8)
+    half = rng.random(n) < 0.5
+    v[half, 2] = 0.2 * rng.standard_normal(half.sum())
+    return x, v, np.full(n, 39.0625), np.full(n, radius), lab, np.ones(n, np.int32)
+
+
+def test_candidate_order_bound(o64):
+    """What the unpinned wp.HashGrid candidate order can change.  The
+    candidates are the same in every order (the neighbour set is pinned), so
+    with equal static and dynamic friction only the summation order of the
+    contact deltas differs (FP64 rounding).  With 0.6 static / 0.4 dynamic the
+    sticky static switch (xpbdRoutines.py:211-212) makes k_mu static for every
+    later candidate once one contact has |v_n| < 1e-5 r; but the friction is
+    max(k_mu lambda_n, -|v_t| dt), so k_mu matters only where the tangential
+    cap does not bind: |v_t| dt > k_mu |err|.  On contact_heavy-like states
+    (every neighbour overlapping by centimetres, at rest or moving) the order
+    changes nothing beyond rounding; it shows on sliding contacts whose
+    overlap is under a millimetre.  Reported: the largest change of a
+    particle's delta relative to the largest delta, for the reverse sweep
+    (order 1) and for z-fastest (order 2)."""
+    cases = {"c2_rest": _contact_state(5, 1e-6), "c2_moving": _contact_state(5, 0.5),
+             "sliding_0.5mm": _sliding_state(7, 0.1255), "sliding_1mm": _sliding_state(7, 0.126)}
+    report = {}
+    for name, st in cases.items():
+        for frictions in ((0.4, 0.4), (0.6, 0.4)):
+            d0 = _solve(o64, st, 0, *frictions, 1e-5)
+            assert np.abs(d0).max() > 0
+            for order in (1, 2):
+                d = _solve(o64, st, order, *frictions, 1e-5)
+                rel = float(np.abs(d - d0).max() / np.abs(d0).max())
+                report[(name, frictions, order)] = rel
+                print(f"  {name} friction {frictions} order {order}: max |change| / max |delta| = {rel:.2e}")
+                if frictions[0] == frictions[1] or not name.startswith("sliding_0.5"):
+                    assert rel < 1e-12, (name, frictions, order, rel)  # summation order only
+    # shallow sliding contacts: the order decides which contacts are static
+    assert report[("sliding_0.5mm", (0.6, 0.4), 1)] > 1e-2
