@@ -384,6 +384,16 @@ size_t mpmx_reduce_workspace_bytes(int n);
 int mpmx_velocity_convergence(int n, const int32_t* activeLabel, const float* v,
                               const float* radius, double* out_sum, int32_t* out_count,
                               void* workspace, size_t ws_bytes, void* stream);
+/* No counterpart in the reference: mpmx_velocity_convergence without a
+ * workspace, for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels, same
+ * results bit for bit.  Its block partials live in a 12 KB device array of
+ * the library (one per GPU, held until the process ends); it allocates
+ * nothing.  Any n >= 0 fits (n beyond the scratch would return MPMX_ERANGE).
+ * Calls that overlap on one GPU race on the array. */
+int mpmx_velocity_convergence_noalloc(int n, const int32_t* activeLabel, const float* v,
+                                      const float* radius, double* out_sum, int32_t* out_count,
+                                      void* stream);
 /* sum_float_array_active :598-610 (compute_mean_gpu filtered=True, :633-675) */
 int mpmx_sum_float_array_active(int n, const float* arr, const int32_t* activeLabel,
                                 double* out_sum, int32_t* out_count, void* workspace,

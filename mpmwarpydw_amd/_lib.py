@@ -79,6 +79,7 @@ SIGNATURES = {
     "mpmx_g2p": [F, I, P, P, P, P, P, P, P, P, F, P, I, I, I, I, F, F, F, P],
     "mpmx_reduce_workspace_bytes": [I],
     "mpmx_velocity_convergence": [I, P, P, P, P, P, P, S, P],
+    "mpmx_velocity_convergence_noalloc": [I, P, P, P, P, P, P],
     "mpmx_sum_float_array_active": [I, P, P, P, P, P, S, P],
     "mpmx_count_particles_by_type": [I, P, P, P, P, S, P],
     "mpmx_hashgrid_workspace_bytes": [I, I, I, I],

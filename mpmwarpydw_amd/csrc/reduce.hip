@@ -3,6 +3,8 @@
 // one address; here a fixed grid of blocks each reduces a fixed grid-stride
 // subset in a fixed tree order and one block sums the block partials in order,
 // so the result is identical from run to run.
+#include <atomic>
+
 #include "mpmx_device.hpp"
 #include "mpmx_internal.hpp"
 
@@ -122,6 +124,46 @@ static int run_reduce(int n, const int32_t* active, const int32_t* mat, const fl
     return check_launch("reduce_pass2", s);
 }
 
+// ---- the allocation-free residual (mpmx_velocity_convergence_noalloc) ------
+// Its block partials live in a file-scope device array instead of a caller's
+// workspace: kRedBlocks doubles and kRedBlocks x 4 counts, 12 KB per GPU, held
+// until the process ends.  Host code never passes the array's name to a kernel
+// (on the host it is a stand-in, not a device address): the address comes from
+// hipGetSymbolAddress, taken once per device with that device current, and only
+// that pointer reaches the kernels.  Every call on one GPU shares the array;
+// calls that overlap on one GPU race.
+constexpr size_t kNoAllocPsumBytes = (size_t)kRedBlocks * sizeof(double);
+constexpr size_t kNoAllocScratchBytes = kNoAllocPsumBytes + (size_t)kRedBlocks * 4 * sizeof(int32_t);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+static int scratch_for_current_device(unsigned char** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("mpmx_velocity_convergence_noalloc: hipGetDevice failed: %s", hipGetErrorString(e));
+        return (int)e;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("mpmx_velocity_convergence_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return MPMX_ERANGE;
+    }
+    void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("mpmx_velocity_convergence_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return e != hipSuccess ? (int)e : MPMX_EINVAL;
+        }
+        g_scratch_ptr[dev].store(p, std::memory_order_release);  // the same address whichever thread wins
+    }
+    *out = static_cast<unsigned char*>(p);
+    return 0;
+}
+
 }  // namespace mpmx
 
 using namespace mpmx;
@@ -138,6 +180,27 @@ int mpmx_velocity_convergence(int n, const int32_t* activeLabel, const float* v,
     MPMX_REQUIRE(n == 0 || (activeLabel && v && radius && out_sum), "mpmx_velocity_convergence: null pointer");
     return run_reduce<RED_VCONV>(n, activeLabel, nullptr, v, radius, out_sum, out_count, 1, workspace, ws_bytes,
                                  as_stream(stream));
+}
+
+int mpmx_velocity_convergence_noalloc(int n, const int32_t* activeLabel, const float* v, const float* radius,
+                                      double* out_sum, int32_t* out_count, void* stream) {
+    MPMX_REQUIRE(n >= 0, "mpmx_velocity_convergence_noalloc: n < 0");
+    MPMX_REQUIRE(n == 0 || (activeLabel && v && radius), "mpmx_velocity_convergence_noalloc: null input");
+    MPMX_REQUIRE(out_sum && out_count, "mpmx_velocity_convergence_noalloc: null output");
+    // the partials of nblocks(n) <= kRedBlocks blocks: the scratch fits every int n
+    MPMX_REQUIRE(nblocks(n) <= kRedBlocks, "mpmx_velocity_convergence_noalloc: n needs more than the scratch");
+    unsigned char* base = nullptr;
+    const int rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    double* psum = reinterpret_cast<double*>(base);
+    int32_t* pcnt = reinterpret_cast<int32_t*>(base + kNoAllocPsumBytes);
+    const hipStream_t s = as_stream(stream);
+    const int nb = nblocks(n);
+    MPMX_LAUNCH(reduce_pass1<RED_VCONV>, dim3(nb), dim3(kBlock), 0, s, n, activeLabel, nullptr, v, radius, psum, pcnt);
+    int r = check_launch("reduce_pass1", s);
+    if (r) return r;
+    MPMX_LAUNCH(reduce_pass2, dim3(1), dim3(kBlock), 0, s, nb, psum, pcnt, out_sum, out_count, 1);
+    return check_launch("reduce_pass2", s);
 }
 
 int mpmx_sum_float_array_active(int n, const float* arr, const int32_t* activeLabel, double* out_sum,
