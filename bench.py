@@ -6,22 +6,32 @@ A step is one MPM substep of the plain method over the whole scene: the fused
 substep (constitutive update + deterministic P2G, grid update, G2P with
 re-binning), plus the XPBD contact step every ``mpmStepsPerXpbdStep`` substeps
 as the reference's loop runs it (runMPMYDW.py:677-682).  The default workload
-is the undercut slab: 960,000 particles, grid 104x54x31, dt = 5e-5 s, XPBD
-every 10 substeps.  Inputs are resident in HBM before the timed region; at
-setup the particle arrays are permuted into the substep's tile order
-(Simulation.reorder, a layout change that leaves the results bit for bit the
-same; --no-reorder keeps the input order).
+is the undercut slab as configured: 960,000 particles, grid 104x54x31,
+dt = 5e-5 s, XPBD every 10 substeps, 2,000 substeps from rest with damage on
+(K = 2000 times the whole run).  Inputs are resident in HBM before the timed
+region; at setup the particle arrays are permuted into the substep's tile
+order (Simulation.reorder, a layout change that leaves the results bit for bit
+the same; --no-reorder keeps the input order).
 
 One XPBD period (10 substeps, the XPBD step included) is captured once in a
-HIP graph; W untimed substeps run first, then the graph is replayed K/10
-times between two device synchronisations.  Per-kernel durations come from
-the library's launch profiler (HIP events just before and after every kernel,
-on the launch stream) over one more period launched eagerly right after the
-timed region.
+HIP graph.  W untimed substeps run first on the same state (kernel loading,
+clocks, graph capture); then the device state is restored to t = 0 from a
+device copy taken at setup and binned, and the graph is replayed K/10 times
+between two device synchronisations: substeps 1..K of the run.
 
-Prints one JSON line: the metric (MPM particle-substeps/s), per-kernel device
-times, the roofline entry of the dominant kernel and a CPU baseline (the FP32
-restatement under oracle/, one core, on a bounded sample).
+After the timed region:
+  * matched error: one more substep whose constitutive update is checked
+    against the FP64 oracle on every MPM particle (per particle, relative to
+    the larger of its largest stress component and its yield stress);
+  * per-kernel durations from the library's launch profiler (HIP events
+    stamped by each kernel's dispatch) over one eagerly launched period;
+  * the driver loop's rate: the same substeps launched one by one with the
+    per-substep residual and damage readback of runMPMYDW.py:684-709;
+  * the CPU baseline: the FP32 restatement under oracle/ with OpenMP on all
+    host threads from the same t = 0 state over a bounded number of substeps
+    (XPBD steps at their ratio included), and on one core over fewer.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -63,7 +73,7 @@ PROFILE_HOLD_US = 50000  # long enough for the host to queue a whole profiled pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=400)
+    p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=40)
     p.add_argument("--scene", default="undercut_slab", help="a scene of scenes.SCENES, or 'ensemble'")
     p.add_argument("--scale", type=float, default=1.0)
@@ -71,59 +81,214 @@ def parse():
     p.add_argument("--no-reorder", action="store_true",
                    help="keep the particle arrays in input order (no spatial reorder at setup)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of each CPU baseline sample")
+    p.add_argument("--no-error-sample", action="store_true",
+                   help="skip the FP64 check of the constitutive update after the timed region")
     p.add_argument("--ensemble-scenes", type=int, default=64, help="scenes of the batched ensemble (--scene ensemble)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the 256 MiB copy that calibrates the HBM byte counters")
     return p.parse_args()
 
 
-def cpu_baseline(S, budget_s):
-    """The FP32 restatement (oracle/, reference semantics, P2G sequential in
-    particle order) timed on one host core over a bounded number of substeps of
-    the same state."""
-    from oracle import Oracle
-    o = Oracle(32)
-    sim, mpm = S.sim, S.mpm
-    f = lambda a: np.ascontiguousarray(a.numpy().astype(np.float32))
+def _host_state(S):
+    """The t = 0 state of a scene as FP32 host arrays in the reference's AoS
+    layout, for the CPU baseline."""
+    sim, mpm, xpbd = S.sim, S.mpm, S.xpbd
+    f = lambda a: np.ascontiguousarray(a.numpy().astype(np.float32))  # noqa: E731
     H = {k: f(getattr(sim, k)) for k in ("particle_x", "particle_v", "particle_mass", "particle_vol",
-                                         "particle_density")}
+                                         "particle_density", "particle_radius")}
     for k in ("particle_F", "particle_F_trial", "particle_stress", "particle_accumulated_strain", "particle_damage",
               "particle_C", "particle_cov", "mu", "lam", "ys", "alpha", "hardening", "softening", "eta_shear",
               "eta_bulk", "strainCriteria"):
         H[k] = f(getattr(mpm, k))
-    xi = np.ascontiguousarray(S.xpbd.particle_x_initial_xpbd.numpy().astype(np.float32))
-    vi = np.ascontiguousarray(S.xpbd.particle_v_initial_xpbd.numpy().astype(np.float32))
+    for k in ("particle_x_initial_xpbd", "particle_v_initial_xpbd", "particle_cumDist_xpbd"):
+        H[k] = f(getattr(xpbd, k))
+    H["activeLabel"] = np.ascontiguousarray(sim.activeLabel.numpy())
+    H["materialLabel"] = np.ascontiguousarray(sim.materialLabel.numpy())
+    return H
+
+
+def _cpu_substep(o, H, S, counter, grids):
+    """One substep of the reference's loop on the oracle's kernels: stress, P2G,
+    grid update, G2P, and every ratio-th substep the XPBD step
+    (simulationRoutines.py:6-526) with the reference's full candidate walk."""
+    sim, mpm, xpbd = S.sim, S.mpm, S.xpbd
+    act, lab = H["activeLabel"], H["materialLabel"]
+    gm, gv, go = grids
+    o.compute_stress_from_F_trial(act, lab, H["particle_x"], H["particle_v"], H["particle_x_initial_xpbd"],
+                                  H["particle_v_initial_xpbd"], H["particle_F"], H["particle_F_trial"], H["mu"],
+                                  H["lam"], H["ys"], H["hardening"], H["softening"], H["particle_density"],
+                                  H["strainCriteria"], mpm.eff, H["eta_shear"], H["eta_bulk"], H["particle_C"],
+                                  H["particle_stress"], H["particle_accumulated_strain"], H["particle_damage"], 1,
+                                  H["alpha"])
+    gm[:] = 0
+    gv[:] = 0
+    go[:] = 0
+    o.p2g_apic_with_stress(act, lab, H["particle_stress"], H["particle_x"], H["particle_v"], H["particle_C"],
+                           H["particle_vol"], H["particle_mass"], sim.dx, sim.invdx, sim.minBounds, mpm.rpic_damping,
+                           gm, gv, sim.dt)
+    o.grid_normalization_and_gravity(gm, gv, go, sim.gravity, sim.dt)
+    o.collideBounds(go, *mpm.gridDims, mpm.boundFriction, sim.dt, abs(sim.gravity[2]), mpm.boundaryPadding)
+    o.g2p(sim.dt, act, lab, H["particle_x"], H["particle_v"], H["particle_C"], H["particle_F"],
+          H["particle_F_trial"], H["particle_cov"], sim.invdx, go, bool(mpm.update_cov), sim.minBounds)
+    if counter % sim.mpmStepsPerXpbdStep or counter <= 0:
+        return
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    n = sim.nPoints
+    x0, v0 = H["particle_x"].copy(), H["particle_v"].copy()
+    xi, vi = x0.copy(), v0.copy()
+    dims = xpbd.particle_grid.dims
+    ids, cs, ce = o.hashgrid_build(x0, np.float32(sim.dx), dims)
+    o.integrateParticlesXPBD(act, lab, xi, vi, np.float32(sim.gravity), f32(sim.dtxpbd), xi, vi,
+                             f32(xpbd.particle_v_max))
+    if xpbd.mpm_contact_transition_lock:
+        o.reset_mpm_contact_labels(act, lab)
+    for _ in range(xpbd.xpbd_iterations):
+        d = np.zeros((n, 3), np.float32)
+        o.my_solve_particle_bound_contacts(act, lab, xi, vi, H["particle_mass"], H["particle_radius"],
+                                           f32(xpbd.dynamicParticleFriction), f32(xpbd.staticVelocityThreshold),
+                                           f32(xpbd.staticParticleFriction), np.float32(xpbd.minBoundsXPBD),
+                                           np.float32(xpbd.maxBoundsXPBD), f32(sim.dtxpbd),
+                                           f32(xpbd.xpbd_relaxation), d)
+        o.my_solve_particle_particle_contacts(act, lab, (ids, cs, ce, np.float32(sim.dx), dims), xi, vi,
+                                              H["particle_mass"], H["particle_radius"],
+                                              f32(xpbd.dynamicParticleFriction), f32(xpbd.staticVelocityThreshold),
+                                              f32(xpbd.staticParticleFriction), f32(xpbd.particle_cohesion),
+                                              f32(xpbd.max_radius), f32(sim.dtxpbd), f32(xpbd.xpbd_relaxation),
+                                              int(xpbd.mpm_contact_transition_lock), d)
+        xo, vo = xi.copy(), vi.copy()
+        o.my_apply_particle_deltas(act, lab, x0, xi, d, f32(sim.dtxpbd), f32(xpbd.particle_v_max), xo, vo)
+        xi, vi = xo, vo
+    o.sleepParticles(act, lab, f32(xpbd.sleepThreshold), H["particle_radius"], x0, xi, H["particle_cumDist_xpbd"],
+                     f32(sim.dtxpbd))
+    o.apply_mpm_contact_impulses(act, lab, x0, v0, xi, vi, f32(sim.dtxpbd), f32(xpbd.particle_v_max),
+                                 f32(xpbd.xpbd_mpm_coupling_strength), 0, vi)
+    H["particle_x"], H["particle_v"] = xi, vi
+
+
+def cpu_baseline(S, H0, budget_s):
+    """The FP32 restatement under oracle/ (reference semantics) from the t = 0
+    state: with OpenMP on every host thread for up to ``budget_s`` seconds of
+    substeps (whole XPBD periods when the budget allows), then on one core for
+    a shorter sample.  Returns the cpu_baseline object of the JSON line."""
+    from oracle import Oracle
+    sim, mpm = S.sim, S.mpm
+    ratio = sim.mpmStepsPerXpbdStep
+    act, lab = H0["activeLabel"], H0["materialLabel"]
+    n_mpm = int(((act == 1) & (lab <= 1)).sum())
+    res = {}
+    for name, o, budget, cap in (("all", Oracle(32, omp=True), budget_s, 10 * ratio),
+                                 ("one", Oracle(32), budget_s / 3, ratio)):
+        H = {k: v.copy() for k, v in H0.items()}
+        grids = (np.zeros(mpm.gridDims, np.float32), np.zeros(mpm.gridDims + (3,), np.float32),
+                 np.zeros(mpm.gridDims + (3,), np.float32))
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            steps += 1
+            _cpu_substep(o, H, S, steps, grids)
+            el = time.perf_counter() - t0
+            if el > budget or steps >= cap:
+                break
+        res[name] = (n_mpm * steps / el, steps, el, o.threads())
+    v, steps, el, threads = res["all"]
+    v1, steps1, el1, _ = res["one"]
+    xp = sum(1 for c in range(1, steps + 1) if c % ratio == 0)
+    return {"value": v, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"substeps 1-{steps} of the same run from t = 0 ({xp} XPBD steps with the reference's full "
+                      f"candidate walk), FP32 restatement (oracle/, OpenMP) on {threads} host threads, {el:.1f} s",
+            "single_core_value": v1,
+            "single_core_sample": f"substeps 1-{steps1}, serial FP32 build on one core, {el1:.1f} s"}
+
+
+def stress_error_sample(S, eff):
+    """The constitutive update of one substep against the FP64 oracle on every
+    active MPM particle, from the device state it starts from: per particle
+    |tau_gpu - tau_64| / max(|tau_64|, ys_64, 1 Pa) (largest component)."""
+    from mpmwarpydw_amd import simulationRoutines as SR
+    from oracle import Oracle
+    sim, mpm, xpbd = S.sim, S.mpm, S.xpbd
+    g = lambda a: np.ascontiguousarray(a.numpy().astype(np.float64))  # noqa: E731
+    H = {k: g(getattr(sim, k)) for k in ("particle_x", "particle_v", "particle_density")}
+    for k in ("particle_F", "particle_F_trial", "particle_stress", "particle_accumulated_strain", "particle_damage",
+              "particle_C", "mu", "lam", "ys", "alpha", "hardening", "softening", "eta_shear", "eta_bulk",
+              "strainCriteria"):
+        H[k] = g(getattr(mpm, k))
+    for k in ("particle_x_initial_xpbd", "particle_v_initial_xpbd"):
+        H[k] = g(getattr(xpbd, k))
     act = np.ascontiguousarray(sim.activeLabel.numpy())
     lab = np.ascontiguousarray(sim.materialLabel.numpy())
-    gm = np.zeros(mpm.gridDims, np.float32)
-    gv = np.zeros(mpm.gridDims + (3,), np.float32)
-    go = np.zeros(mpm.gridDims + (3,), np.float32)
+    sel = (act == 1) & (lab <= 1)
+    H0_Ft = H["particle_F_trial"].copy()
+    mu0, lam0, ys0, al0, D0 = (H[k].copy() for k in ("mu", "lam", "ys", "alpha", "particle_damage"))
+    SR.mpmSimulationStepFused(sim, mpm, xpbd)
+    tau_gpu = mpm.particle_stress.numpy().astype(np.float64)
+    lab_gpu = sim.materialLabel.numpy()
+    t0 = time.perf_counter()
+    acc0 = H["particle_accumulated_strain"].copy()
+    Oracle(64).compute_stress_from_F_trial(act, lab, H["particle_x"], H["particle_v"], H["particle_x_initial_xpbd"],
+                                           H["particle_v_initial_xpbd"], H["particle_F"], H["particle_F_trial"],
+                                           H["mu"], H["lam"], H["ys"], H["hardening"], H["softening"],
+                                           H["particle_density"], H["strainCriteria"], eff, H["eta_shear"],
+                                           H["eta_bulk"], H["particle_C"], H["particle_stress"],
+                                           H["particle_accumulated_strain"], H["particle_damage"], 1, H["alpha"])
+    el = time.perf_counter() - t0
+    tau = H["particle_stress"][sel]
+    scale = np.maximum(np.abs(tau).max(axis=(1, 2)), np.maximum(np.abs(H["ys"][sel]), 1.0))
+    err = np.abs(tau_gpu[sel] - tau).max(axis=(1, 2)) / scale
+    plastic = (H["particle_accumulated_strain"] != acc0)[sel]
+    margin = yield_margin(H0_Ft, mu0, lam0, ys0, al0, D0)[sel]
+    # the reference's projection jumps at the yield surface (dgam = |dev e| + 1e-6 - Y / (2 mu) is 1e-6,
+    # not 0, there): where the FP64 margin is below FP32 resolution, FP32 may decide either way
+    marginal = np.abs(margin) < MARGIN
+    ok = err[~marginal]
+    return {"max": float(ok.max()) if ok.size else 0.0, "p99_9": float(np.quantile(ok, 0.999)) if ok.size else 0.0,
+            "mean": float(ok.mean()) if ok.size else 0.0, "particles": int(sel.sum()),
+            "plastic_in_sample": int(plastic.sum()), "labels_identical": bool(np.array_equal(lab_gpu, lab)),
+            "marginal_particles": int(marginal.sum()),
+            "marginal_max": float(err[marginal].max()) if marginal.any() else 0.0,
+            "definition": "constitutive update of one substep after the timed run, every active MPM particle, "
+                          "against the FP64 oracle: max over particles of |tau - tau_fp64|_max / "
+                          "max(|tau_fp64|_max, ys, 1 Pa); 'marginal' particles, whose FP64 yield margin "
+                          f"|sigma_eq - Y| / |Y| is below {MARGIN:g}, are counted apart: there the FP32 yield test "
+                          "can decide either way and the reference's map jumps by 2 mu 1e-6 in strain",
+            "oracle_seconds": round(el, 1)}
+
+
+MARGIN = 1e-6
+
+
+def yield_margin(Ft, mu, lam, ys, alpha, D):
+    """(sigma_eq - Y) / |Y| of the return map's yield test (mpmRoutines.py:341-361)
+    in FP64, from the principal stretches of F_trial clamped at 0.01 (the
+    negative one of an inverted F included)."""
+    s = np.linalg.svd(Ft, compute_uv=False)
+    s[:, 2] *= np.sign(np.linalg.det(Ft))
+    e = np.log(np.maximum(s, 0.01))
+    dev = e - e.mean(axis=1, keepdims=True)
+    seq = 2 * mu * np.linalg.norm(dev, axis=1)
+    Y = (1 - D) * (ys - alpha * (2 * mu / 3 + lam) * e.sum(axis=1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return (seq - Y) / np.abs(Y)
+
+
+def driver_loop_rate(S, counter0, substeps):
+    """Particle-substeps/s of the driver loop as runMPMYDW.py:677-709 runs it:
+    substeps launched one by one, each followed by the FP64 residual and mean
+    damage reductions and their readback (a host synchronisation)."""
+    import torch
+    sim = S.sim
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(substeps):
+        S.substep(counter0 + k)
+        S.loop_scalars()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lab = sim.materialLabel.numpy()
+    act = sim.activeLabel.numpy()
     n_mpm = int(((act == 1) & (lab <= 1)).sum())
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        o.compute_stress_from_F_trial(act, lab, H["particle_x"], H["particle_v"], xi, vi, H["particle_F"],
-                                      H["particle_F_trial"], H["mu"], H["lam"], H["ys"], H["hardening"],
-                                      H["softening"], H["particle_density"], H["strainCriteria"], mpm.eff,
-                                      H["eta_shear"], H["eta_bulk"], H["particle_C"], H["particle_stress"],
-                                      H["particle_accumulated_strain"], H["particle_damage"], 1, H["alpha"])
-        gm[:] = 0
-        gv[:] = 0
-        go[:] = 0
-        o.p2g_apic_with_stress(act, lab, H["particle_stress"], H["particle_x"], H["particle_v"], H["particle_C"],
-                               H["particle_vol"], H["particle_mass"], sim.dx, sim.invdx, sim.minBounds,
-                               mpm.rpic_damping, gm, gv, sim.dt)
-        o.grid_normalization_and_gravity(gm, gv, go, sim.gravity, sim.dt)
-        o.collideBounds(go, *mpm.gridDims, mpm.boundFriction, sim.dt, abs(sim.gravity[2]), mpm.boundaryPadding)
-        o.g2p(sim.dt, act, lab, H["particle_x"], H["particle_v"], H["particle_C"], H["particle_F"],
-              H["particle_F_trial"], H["particle_cov"], sim.invdx, go, bool(mpm.update_cov), sim.minBounds)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or steps >= 50:
-            break
-    return {"value": n_mpm * steps / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} MPM substeps (stress+P2G+grid+G2P, FP32, no XPBD step) of the same "
-                      f"{n_mpm}-particle state, {el:.1f} s on one host core"}
+    return {"value": n_mpm * substeps / el, "substeps": substeps, "ms_per_step": el * 1e3 / substeps,
+            "what": "substeps launched one by one, each with the residual and mean-damage readback"}
 
 
 def profile_report():
@@ -188,6 +353,7 @@ def main():
         return bench_ensemble(a)
     import torch
     from mpmwarpydw_amd import driver, getArgs, scenes
+    from mpmwarpydw_amd import simulationRoutines as SR
     from mpmwarpydw_amd._lib import lib
     from mpmwarpydw_amd.runtime import stream_handle
 
@@ -203,6 +369,8 @@ def main():
     steps = periods * ratio
     print(f"[bench] scene={a.scene} n={n} grid={mpm.gridDims} ratio={ratio} steps={steps} warmup={a.warmup}",
           file=sys.stderr, flush=True)
+    snap = snapshot(S)  # the t = 0 device state, restored before the timed run
+    H0 = None if a.no_cpu_baseline else _host_state(S)
 
     counter = 1  # start inside a period so that warmup + replays keep the reference's XPBD phase
     stream = torch.cuda.Stream()
@@ -225,19 +393,40 @@ def main():
         graph.replay()  # the captured period runs once before timing (untimed)
         torch.cuda.synchronize()
 
+    # back to t = 0: the device arrays from the copy taken at setup, then the bins of that state
+    with torch.cuda.stream(stream):
+        restore(S, snap)
+        SR.mpmSimulationBin(sim, mpm, S.xpbd)
+    torch.cuda.synchronize()
+    del snap
+
     t0 = time.perf_counter()
     with torch.cuda.stream(stream):
-        for _ in range(periods):
+        for p in range(periods):
             if graph is not None:
                 graph.replay()
             else:
-                for c in period:
-                    S.substep(c)
+                for k in range(ratio):
+                    S.substep(p * ratio + k + 1)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     lab = sim.materialLabel.numpy()
     act = sim.activeLabel.numpy()
     n_mpm = int(((act == 1) & (lab <= 1)).sum())  # MPM particles at the end of the timed region
+    state_end = {"mpm_particles": n_mpm, "xpbd_particles": int(((act == 1) & (lab == 2)).sum()),
+                 "plastic_particles": int((mpm.particle_accumulated_strain.numpy() > 0).sum()),
+                 "mean_damage": float(mpm.particle_damage.numpy()[act == 1].mean())}
+    counter = steps + 1
+
+    err = None
+    if not a.no_error_sample:  # one more substep (counter steps + 1: no XPBD step), checked against FP64
+        with torch.cuda.stream(stream):
+            err = stress_error_sample(S, mpm.eff)
+        counter += 1
+        while counter % ratio != 1:
+            S.substep(counter)
+            counter += 1
+    period = [counter + k for k in range(ratio)]
 
     # per-kernel device time: one more period launched eagerly on the same stream, every
     # library kernel carrying dispatch-stamped events (graph launches cannot carry them)
@@ -255,6 +444,9 @@ def main():
     kernels = profile_report()
     how = ("HIP start/stop events stamped by each kernel's own dispatch (hipExtLaunchKernelGGL) over one eager "
            "period after the timed region; it and a warm-up period are queued behind a hold kernel")
+    counter = period[-1] + 1
+    with torch.cuda.stream(stream):
+        loop = driver_loop_rate(S, counter, 5 * ratio)
 
     value = n_mpm * steps / wall
     G = int(np.prod(mpm.gridDims))
@@ -276,9 +468,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{a.scene}: {n} particles ({n_mpm} MPM), grid {mpm.gridDims[0]}x{mpm.gridDims[1]}x"
-                               f"{mpm.gridDims[2]}, dt {sim.dt:g} s, XPBD every {ratio} substeps",
+                               f"{mpm.gridDims[2]}, dt {sim.dt:g} s, XPBD every {ratio} substeps, substeps 1-{steps} "
+                               f"of the run from t = 0",
                    "particles": n, "grid_nodes": G, "hip_graph": graph is not None,
                    "particle_order": "input order" if a.no_reorder else "tile order (reordered once at setup)"},
+        "scope": ("the timed region replays the substeps (MPM substep + XPBD step at its ratio) as HIP graphs with "
+                  "no host synchronisation; the driver loop's per-substep readback is measured separately in "
+                  "driver_loop"),
+        "state_at_end": state_end,
+        "stress_error_vs_fp64": err,
         "roofline": {"bound": "hbm", "kernel": rk, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
@@ -289,10 +487,28 @@ def main():
         "dominant_kernel": dom_name,
         "kernel_timing": how,
         "kernels_us_per_launch": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
+        "driver_loop": loop,
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(S, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(S, H0, a.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def snapshot(S):
+    """Device copies of every array of the simulation state."""
+    from mpmwarpydw_amd import runtime as rt
+    snap, seen = [], set()
+    for obj in (S.sim, S.mpm, S.xpbd):
+        for name, arr in vars(obj).items():
+            if isinstance(arr, rt.Array) and id(arr) not in seen:
+                seen.add(id(arr))
+                snap.append((arr, arr.t.clone()))
+    return snap
+
+
+def restore(S, snap):
+    for arr, t in snap:
+        arr.t.copy_(t)
 
 
 def bench_ensemble(a):

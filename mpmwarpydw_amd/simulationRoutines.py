@@ -209,6 +209,15 @@ def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
     mpm._bins = key
 
 
+def mpmSimulationBin(sim, mpm, xpbd, device=None):
+    """Bin the current state for the fused substep (what its first call does
+    for a fresh state), e.g. after the device arrays were overwritten in place."""
+    P, G, prm = step_structs(sim, mpm, xpbd)
+    ws, nb = substep_workspace(sim, mpm)
+    lib("mpmx_substep_bin", C.byref(P), C.byref(G), C.byref(prm), ws, nb, stream_handle())
+    mpm._bins = _bin_key(sim, mpm)
+
+
 def misbinned_count(sim, mpm):
     """Particles found outside their bin since the last rebin (must be 0)."""
     ws, nb = substep_workspace(sim, mpm)

@@ -29,7 +29,8 @@ _P = C.c_void_p
 
 
 def build() -> None:
-    """Compile both oracle precisions with the Makefile in this folder."""
+    """Compile both oracle precisions and the OpenMP FP32 build with the
+    Makefile in this folder."""
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
@@ -40,8 +41,11 @@ def _ptr(a):
 class Oracle:
     """ctypes view of one precision of the C oracle."""
 
-    def __init__(self, bits: int = 64, svd_from_ftf: bool | None = None):
-        path = os.path.join(HERE, f"liboracle_f{bits}.so")
+    def __init__(self, bits: int = 64, svd_from_ftf: bool | None = None, omp: bool = False):
+        """``omp``: the FP32 build with OpenMP loops (the all-cores CPU baseline)."""
+        if omp and bits != 32:
+            raise ValueError("the OpenMP build is FP32")
+        path = os.path.join(HERE, f"liboracle_f{bits}{'_omp' if omp else ''}.so")
         if not os.path.exists(path):
             build()
         self.lib = C.CDLL(path)
@@ -54,6 +58,10 @@ class Oracle:
         self.lib.o_set_svd_mode(int(svd_from_ftf))
         self.lib.o_velocity_convergence.restype = C.c_double
         self.lib.o_sum_active.restype = C.c_double
+
+    def threads(self) -> int:
+        """Threads of the OpenMP build's loops (1 for the serial builds)."""
+        return int(self.lib.o_threads())
 
     # ---- helpers ---------------------------------------------------------
     def f(self, a):

@@ -8,10 +8,15 @@
  * The same source is compiled twice:
  *   liboracle_f64.so  REAL=double  - the FP64 oracle of record
  *   liboracle_f32.so  REAL=float   - an FP32 restatement (reference semantics,
- *                                    sequential P2G in particle order), used as
- *                                    the timed CPU baseline and to show how far
- *                                    the reference's own FP32 arithmetic sits
- *                                    from FP64.
+ *                                    sequential P2G in particle order), used to
+ *                                    show how far the reference's own FP32
+ *                                    arithmetic sits from FP64, and timed on one
+ *                                    core as a CPU baseline.
+ *   liboracle_f32_omp.so  the FP32 restatement with OpenMP over particles and
+ *                                    grid nodes (P2G into one grid per thread,
+ *                                    summed afterwards): the all-cores CPU
+ *                                    baseline.  The pragmas are inert in the
+ *                                    two builds above.
  * Every function restates one reference kernel; the citation in front of it
  * names the file:line under utils/ in the MPMWarpYDW reference.
  *
@@ -25,6 +30,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef REAL
 #define REAL double
@@ -39,6 +47,14 @@ typedef REAL R;
 static int g_svd_from_ftf = 0;
 EXPORT void o_set_svd_mode(int from_ftf) { g_svd_from_ftf = from_ftf; }
 EXPORT int o_real_size(void) { return (int)sizeof(R); }
+/* threads the OpenMP build runs its loops on (1 in the other builds) */
+EXPORT int o_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------ */
 /* small linear algebra                                                */
@@ -346,6 +362,7 @@ EXPORT void o_compute_stress_from_F_trial(int n, const int *activeLabel, int *ma
                                           R *stress, R *acc, R *damage, int constitutive_model,
                                           R *alpha) {
     (void)constitutive_model; /* passed but never read by the reference kernel */
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int p = 0; p < n; p++) {
         if (materialLabel[p] > 1 || activeLabel[p] != 1) continue;
         dp_return_map(F_trial + 9 * p, x, v, x_init, v_init, materialLabel, acc, damage, mu, lam,
@@ -395,44 +412,81 @@ static int stencil(const R *xp, const R *minB, R inv_dx, int *base, R *fx, R w[3
 /* returns the number of particles whose stencil left the grid (the    */
 /* reference has no bounds check; they are skipped here).              */
 /* ------------------------------------------------------------------ */
+static int p2g_particle(int p, const int *activeLabel, const int *materialLabel, const R *stress, const R *x,
+                        const R *v, const R *C, const R *vol, const R *mass, R dx, R inv_dx, const R *minB, R rpic,
+                        int gx, int gy, int gz, R *grid_m, R *grid_v_in, R dt) {
+    if (!(activeLabel[p] == 1 && materialLabel[p] <= 1)) return 0;
+    int base[3];
+    R fx[3], w[3][3], dw[3][3];
+    stencil(x + 3 * p, minB, inv_dx, base, fx, w, dw);
+    if (base[0] < 0 || base[1] < 0 || base[2] < 0 || base[0] + 2 >= gx || base[1] + 2 >= gy || base[2] + 2 >= gz)
+        return 1;
+    const R *Cp = C + 9 * p, *S = stress + 9 * p;
+    R Cm[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            Cm[i * 3 + j] = ((R)1 - rpic) * Cp[i * 3 + j] + rpic / (R)2 * (Cp[i * 3 + j] - Cp[j * 3 + i]);
+    if (rpic < (R)-0.001) memset(Cm, 0, sizeof(Cm));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                R dpos[3] = {((R)i - fx[0]) * dx, ((R)j - fx[1]) * dx, ((R)k - fx[2]) * dx};
+                R weight = w[0][i] * w[1][j] * w[2][k];
+                R dwt[3] = {dw[0][i] * w[1][j] * w[2][k] * inv_dx, w[0][i] * dw[1][j] * w[2][k] * inv_dx,
+                            w[0][i] * w[1][j] * dw[2][k] * inv_dx};
+                R add[3];
+                for (int a = 0; a < 3; a++) {
+                    R cd = Cm[a * 3 + 0] * dpos[0] + Cm[a * 3 + 1] * dpos[1] + Cm[a * 3 + 2] * dpos[2];
+                    R ef = -vol[p] * (S[a * 3 + 0] * dwt[0] + S[a * 3 + 1] * dwt[1] + S[a * 3 + 2] * dwt[2]);
+                    add[a] = weight * mass[p] * (v[3 * p + a] + cd) + dt * ef;
+                }
+                long idx = ((long)(base[0] + i) * gy + (base[1] + j)) * gz + (base[2] + k);
+                for (int a = 0; a < 3; a++) grid_v_in[3 * idx + a] += add[a];
+                grid_m[idx] += weight * mass[p];
+            }
+    return 0;
+}
+
 EXPORT int o_p2g(int n, const int *activeLabel, const int *materialLabel, const R *stress,
                  const R *x, const R *v, const R *C, const R *vol, const R *mass, R dx, R inv_dx,
                  const R *minB, R rpic, int gx, int gy, int gz, R *grid_m, R *grid_v_in, R dt) {
     int bad = 0;
-    for (int p = 0; p < n; p++) {
-        if (!(activeLabel[p] == 1 && materialLabel[p] <= 1)) continue;
-        int base[3];
-        R fx[3], w[3][3], dw[3][3];
-        stencil(x + 3 * p, minB, inv_dx, base, fx, w, dw);
-        if (base[0] < 0 || base[1] < 0 || base[2] < 0 || base[0] + 2 >= gx || base[1] + 2 >= gy ||
-            base[2] + 2 >= gz) {
-            bad++;
-            continue;
+#ifdef _OPENMP
+    /* the all-cores baseline: each thread scatters a contiguous range of
+     * particles into a grid of its own, then the grids are summed node by node */
+    const long G = (long)gx * gy * gz;
+    const int T = omp_get_max_threads();
+    R *priv = (R *)calloc((size_t)T * 4 * (size_t)G, sizeof(R));
+#pragma omp parallel reduction(+ : bad)
+    {
+        const int t = omp_get_thread_num();
+        R *gm = priv + (size_t)t * 4 * G, *gv = gm + G;
+#pragma omp for schedule(static)
+        for (int p = 0; p < n; p++)
+            bad += p2g_particle(p, activeLabel, materialLabel, stress, x, v, C, vol, mass, dx, inv_dx, minB, rpic,
+                                gx, gy, gz, gm, gv, dt);
+#pragma omp for schedule(static)
+        for (long i = 0; i < G; i++) {
+            R m = grid_m[i], a0 = grid_v_in[3 * i], a1 = grid_v_in[3 * i + 1], a2 = grid_v_in[3 * i + 2];
+            for (int u = 0; u < T; u++) {
+                const R *um = priv + (size_t)u * 4 * G, *uv = um + G;
+                m += um[i];
+                a0 += uv[3 * i];
+                a1 += uv[3 * i + 1];
+                a2 += uv[3 * i + 2];
+            }
+            grid_m[i] = m;
+            grid_v_in[3 * i] = a0;
+            grid_v_in[3 * i + 1] = a1;
+            grid_v_in[3 * i + 2] = a2;
         }
-        const R *Cp = C + 9 * p, *S = stress + 9 * p;
-        R Cm[9];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                Cm[i * 3 + j] = ((R)1 - rpic) * Cp[i * 3 + j] + rpic / (R)2 * (Cp[i * 3 + j] - Cp[j * 3 + i]);
-        if (rpic < (R)-0.001) memset(Cm, 0, sizeof(Cm));
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                for (int k = 0; k < 3; k++) {
-                    R dpos[3] = {((R)i - fx[0]) * dx, ((R)j - fx[1]) * dx, ((R)k - fx[2]) * dx};
-                    R weight = w[0][i] * w[1][j] * w[2][k];
-                    R dwt[3] = {dw[0][i] * w[1][j] * w[2][k] * inv_dx, w[0][i] * dw[1][j] * w[2][k] * inv_dx,
-                                w[0][i] * w[1][j] * dw[2][k] * inv_dx};
-                    R add[3];
-                    for (int a = 0; a < 3; a++) {
-                        R cd = Cm[a * 3 + 0] * dpos[0] + Cm[a * 3 + 1] * dpos[1] + Cm[a * 3 + 2] * dpos[2];
-                        R ef = -vol[p] * (S[a * 3 + 0] * dwt[0] + S[a * 3 + 1] * dwt[1] + S[a * 3 + 2] * dwt[2]);
-                        add[a] = weight * mass[p] * (v[3 * p + a] + cd) + dt * ef;
-                    }
-                    long idx = ((long)(base[0] + i) * gy + (base[1] + j)) * gz + (base[2] + k);
-                    for (int a = 0; a < 3; a++) grid_v_in[3 * idx + a] += add[a];
-                    grid_m[idx] += weight * mass[p];
-                }
     }
+    free(priv);
+#else
+    for (int p = 0; p < n; p++)
+        bad += p2g_particle(p, activeLabel, materialLabel, stress, x, v, C, vol, mass, dx, inv_dx, minB, rpic, gx,
+                            gy, gz, grid_m, grid_v_in, dt);
+#endif
     return bad;
 }
 
@@ -440,6 +494,7 @@ EXPORT int o_p2g(int n, const int *activeLabel, const int *materialLabel, const 
  * hold zeros where m <= 1e-15 (simulationRoutines.py:13 zeroes it). */
 EXPORT void o_grid_normalization_and_gravity(long G, const R *grid_m, const R *v_in, R *v_out,
                                              const R *gravity, R dt) {
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < G; i++) {
         if (grid_m[i] > (R)1e-15) {
             R inv = (R)1 / grid_m[i];
@@ -628,6 +683,7 @@ EXPORT int o_g2p(R dt, int n, const int *activeLabel, const int *materialLabel, 
                  const R *F, R *F_trial, R *cov, R inv_dx, const R *v_out, int gx, int gy, int gz,
                  int update_cov, const R *minB) {
     int bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
     for (int p = 0; p < n; p++) {
         if (activeLabel[p] != 1 || materialLabel[p] > 1) continue;
         int base[3];
@@ -692,6 +748,7 @@ EXPORT void o_reset_mpm_contact_labels(int n, const int *active, int *mat) {
 /* X3 integrateParticlesXPBD (xpbdRoutines.py:23-48) */
 EXPORT void o_integrate_particles_xpbd(int n, const int *active, const int *mat, const R *x,
                                        const R *v, const R *g, R dt, R *x_new, R *v_new, R v_max) {
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < n; p++) {
         if (active[p] != 1 || mat[p] != 2) continue;
         R v1[3] = {v[3 * p] + g[0] * dt, v[3 * p + 1] + g[1] * dt, v[3 * p + 2] + g[2] * dt};
@@ -710,6 +767,7 @@ EXPORT void o_solve_particle_bound_contacts(int n, const int *active, const int 
                                             const R *v, const R *mass, const R *radius, R mu,
                                             R thr, R static_mu, const R *bmin, const R *bmax, R dt,
                                             R relax, R *delta) {
+#pragma omp parallel for schedule(static)
     for (int tid = 0; tid < n; tid++) {
         if (active[tid] != 1 || mat[tid] != 2 || mass[tid] == 0) continue;
         R wi = (R)1 / mass[tid];
@@ -815,6 +873,9 @@ static void solve_contacts(int n, const int *active, int *mat, const int *point_
                            const R *v, const R *mass, const R *radius, R k_mu_in, R thr, R static_mu,
                            R k_cohesion, R max_radius, R dt, R relax, int lock, R *deltas, int order) {
     R inv = (R)1 / cell_width;
+    /* every thread adds to its own particle's delta only; the lock's label
+     * writes go to neighbours, so with the lock on the loop stays serial */
+#pragma omp parallel for schedule(dynamic, 256) if (lock == 0)
     for (int tid = 0; tid < n; tid++) {
         if (active[tid] != 1 || mat[tid] < 0) continue;
         int i = point_ids[tid];
@@ -907,6 +968,7 @@ EXPORT void o_apply_particle_deltas(int n, const int *active, const int *mat, co
                                     const R *x_pred, const R *delta, R dt, R v_max, R *x_out,
                                     R *v_out) {
     (void)mat;
+#pragma omp parallel for schedule(static)
     for (int t = 0; t < n; t++) {
         if (active[t] != 1) continue;
         R d[3] = {delta[3 * t], delta[3 * t + 1], delta[3 * t + 2]};
@@ -930,6 +992,7 @@ EXPORT void o_apply_mpm_contact_impulses(int n, const int *active, const int *ma
                                          const R *v_orig, R *x_final, const R *v_final, R dt,
                                          R v_max, R k, int xpbd_only, R *v_out) {
     (void)v_final;
+#pragma omp parallel for schedule(static)
     for (int t = 0; t < n; t++) {
         if (!(active[t] == 1 && mat[t] <= 1)) continue;
         for (int a = 0; a < 3; a++) x_final[3 * t + a] = x_orig[3 * t + a];

@@ -353,3 +353,47 @@ def test_deactivate_and_teleport_hand_case(o64):
     assert act.tolist() == [0, 1, 0, 1, 0]
     assert x[:, 2].tolist() == [-31.0, -1.0, -1.0, 1.0, -31.0]
     assert v.tolist() == [[0, 0, 0], [1, 1, 1], [1, 1, 1], [1, 1, 1], [0, 0, 0]]
+
+
+def test_openmp_build_matches_serial_fp32():
+    """The all-cores CPU baseline (FP32 with OpenMP, P2G into a grid per
+    thread) computes what the serial FP32 restatement computes: the particle
+    kernels bit for bit, the grid sums up to their order (FP32 rounding)."""
+    from oracle import Oracle
+    s32, omp = Oracle(32), Oracle(32, omp=True)
+    assert omp.threads() >= 1
+    rng = np.random.default_rng(12)
+    n, g = 3000, (12, 11, 10)
+    x = (rng.uniform(1.2, 3.8, (n, 3)) * np.array([1.0, 0.9, 0.8])).astype(np.float32)
+    v = rng.standard_normal((n, 3)).astype(np.float32)
+    C_ = (0.3 * rng.standard_normal((n, 3, 3))).astype(np.float32)
+    S = (1e4 * rng.standard_normal((n, 3, 3))).astype(np.float32)
+    vol = np.full(n, 0.015625, np.float32)
+    mass = np.full(n, 39.0625, np.float32)
+    act = np.ones(n, np.int32)
+    lab = np.ones(n, np.int32)
+    out = []
+    for o in (s32, omp):
+        gm = np.zeros(g, np.float32)
+        gv = np.zeros(g + (3,), np.float32)
+        bad = o.p2g_apic_with_stress(act, lab, S, x, v, C_, vol, mass, 0.5, 2.0, np.zeros(3, np.float32), 0.0, gm,
+                                     gv, 1e-4)
+        go = np.zeros(g + (3,), np.float32)
+        o.grid_normalization_and_gravity(gm, gv, go, np.array([0, 0, -9.81], np.float32), 1e-4)
+        xo, vo, Co = x.copy(), v.copy(), C_.copy()
+        Ft = np.zeros((n, 3, 3), np.float32)
+        cov = np.zeros((n, 6), np.float32)
+        F = np.broadcast_to(np.eye(3, dtype=np.float32), (n, 3, 3)).copy()
+        o.g2p(1e-4, act, lab, xo, vo, Co, F, Ft, cov, 2.0, go, True, np.zeros(3, np.float32))
+        out.append((bad, gm, gv, go, xo, vo, Co, Ft))
+    assert out[0][0] == out[1][0] == 0
+    for a, b in zip(out[0][1:4], out[1][1:4]):
+        assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(a))
+    # G2P of the same grid is per particle: identical
+    o = out[1]
+    xo, vo, Co = x.copy(), v.copy(), C_.copy()
+    Ft = np.zeros((n, 3, 3), np.float32)
+    F = np.broadcast_to(np.eye(3, dtype=np.float32), (n, 3, 3)).copy()
+    s32.g2p(1e-4, act, lab, xo, vo, Co, F, Ft, np.zeros((n, 6), np.float32), 2.0, o[3], True,
+            np.zeros(3, np.float32))
+    assert np.array_equal(xo, o[4]) and np.array_equal(vo, o[5]) and np.array_equal(Ft, o[7])
