@@ -135,7 +135,10 @@ const char* mpmx_last_error(void);
 #define MPMX_TUNE_XCD_TILES 0
 #define MPMX_TUNE_SPLIT_STRESS 1
 #define MPMX_TUNE_G2P_TILES 2
-#define MPMX_TUNE_COUNT 3
+/* MPMX_TUNE_P2G_CHUNK: particles the P2G tile kernel stages per LDS round,
+ * 0 = 512, 1 = 384, 2 = 256 (the smaller, the more workgroups per CU) */
+#define MPMX_TUNE_P2G_CHUNK 3
+#define MPMX_TUNE_COUNT 4
 int mpmx_set_tuning(int knob, int value);
 int mpmx_get_tuning(int knob);
 
@@ -535,7 +538,37 @@ int mpmx_solve_particle_particle_contacts_packed(
     const float* x, const float* v, const float* mass, const float* radius, const float* xr,
     const float* vw, const int32_t* flags, float k_mu, float static_threshold, float static_friction,
     float k_cohesion, float max_radius, float dt, float relaxation, int contact_lock, float* deltas,
-    const int32_t* slot_of_id, void* stream);
+    const int32_t* slot_of_id, const uint64_t* xpbd_near, const int32_t* contact_list,
+    const int32_t* contact_count, void* stream);
+
+/* The table positions whose contact walk can meet a contact in this XPBD step,
+ * for mpmx_solve_particle_particle_contacts_packed (contact_list, contact_count;
+ * both NULL solves every position).  No counterpart in the reference.  It drops
+ * the MPM particles that walk the XPBD-only table from a cell whose
+ * neighbourhood bitmap bit (mpmx_hashgrid_near_cells) is clear: such a walk
+ * meets no XPBD point in any iteration, so the particle gets no delta and its
+ * query box stays put.  point_pos is the full table's position of every
+ * particle (mpmx_hashgrid_build_pair); list holds up to n positions, count gets
+ * their number (0 while the XPBD-only table is empty).  Build it after the
+ * tables of the step and before its first contact solve. */
+int mpmx_xpbd_contact_list(int n, const int32_t* materialLabel, const int32_t* point_pos,
+                           const int32_t* xpbd_cell_start, float cell_width, int xpbd_dimx,
+                           int xpbd_dimy, int xpbd_dimz, const float* x, const float* radius,
+                           float k_cohesion, float max_radius, const uint64_t* xpbd_near,
+                           int32_t* list, int32_t* count, void* stream);
+
+/* Neighbourhood bitmap of a hash table for the contact solve (no counterpart
+ * in the reference): bit c of near (cells / 64 words, mpmx_hashgrid_near_words)
+ * is set when table cell c or one of its 26 neighbours in the table's wrapped
+ * cell coordinates holds a point; occ is scratch of the same size.  Given to
+ * mpmx_solve_particle_particle_contacts_packed as xpbd_near (built on the
+ * XPBD-only table, may be NULL), it lets an MPM particle whose query box lies
+ * within one cell of its own cell skip a walk that the bitmap shows to meet no
+ * XPBD point (the walk would solve no contact and write no lock label).
+ * Nothing is written while the table is empty. */
+size_t mpmx_hashgrid_near_words(int dimx, int dimy, int dimz);
+int mpmx_hashgrid_near_cells(const int32_t* cell_start, int dimx, int dimy, int dimz, uint64_t* occ,
+                             uint64_t* near, void* stream);
 
 /* my_apply_particle_deltas, xpbdRoutines.py:221-257 (x_out/v_out may alias x_pred) */
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel,

@@ -96,7 +96,10 @@ SIGNATURES = {
                                                     F, F, F, F, I, P, P, P],
     "mpmx_xpbd_contact_pack": [I, P, P, P, P, P, P, P, P, P, P, P, P, I, I, I, P, P, P, P],
     "mpmx_solve_particle_particle_contacts_packed": [I, P, P, P, P, P, P, F, I, I, I, I, I, I, P, P, P, P, P, P, P,
-                                                     F, F, F, F, F, F, F, I, P, P, P],
+                                                     F, F, F, F, F, F, F, I, P, P, P, P, P, P],
+    "mpmx_xpbd_contact_list": [I, P, P, P, F, I, I, I, P, P, F, F, P, P, P, P],
+    "mpmx_hashgrid_near_words": [I, I, I],
+    "mpmx_hashgrid_near_cells": [P, I, I, I, P, P, P],
     "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],
     "mpmx_sleep_particles": [I, P, P, F, P, P, P, P, F, P, P],
     "mpmx_apply_mpm_contact_impulses": [I, P, P, P, P, P, P, F, F, F, I, P, P],

@@ -36,12 +36,18 @@ constexpr int kTN = kTB + 2;                 // nodes per tile edge
 constexpr int kTileCells = kTB * kTB * kTB;  // 64
 constexpr int kTileNodes = kTN * kTN * kTN;  // 216
 constexpr int kRec = 25;                     // floats per staged particle record
-constexpr int kChunk = 512;                  // particles staged per LDS round (two per thread)
 constexpr int kTasks = kTileCells * 3;       // (cell, i): each task owns the 9 nodes (i, j, k) of its cell
+// Particles staged per LDS round: 512 (two per thread; a whole tile of 8 per
+// cell in one round, 51 KB, 3 workgroups per CU), 384 (38 KB) or 256 (27.6 KB,
+// the size of the final node-sum slots).  The smaller chunks are compiled for 4
+// and 5 waves per SIMD, so that their LDS saving turns into occupancy.  The
+// chunk changes no bit of the result: every task sums its cell's particles in
+// slot order, in registers, across rounds.  Chosen at launch (MPMX_TUNE_P2G_CHUNK).
+constexpr int kChunkMax = 512;
 // Record slot s lives at column s + s/8: the particles of neighbouring cells
 // (about 8 apart in the list) then fall in different LDS banks when the tasks
 // of a wavefront read their cells' records in step.
-constexpr int kRecCols = kChunk + kChunk / 8;
+__host__ __device__ constexpr int rec_cols(int chunk) { return chunk + chunk / 8; }
 __host__ __device__ constexpr int rcol(int s) { return s + (s >> 3); }
 
 // Grid geometry of one scene: its nodes, its 4x4x4-cell tiles, where its tiles
@@ -200,8 +206,8 @@ __device__ __forceinline__ void p2g_load(const TileArgs& a, long n, int id, P2GI
 }
 
 // the P2G record of one particle in LDS column r (slot: its place in the bins)
-template <bool FUSED, bool SYM>
-__device__ __forceinline__ void p2g_record(const TileArgs& a, const P2GIn& q, float (*rec)[kRecCols], int r, int slot,
+template <bool FUSED, bool SYM, int COLS>
+__device__ __forceinline__ void p2g_record(const TileArgs& a, const P2GIn& q, float (*rec)[COLS], int r, int slot,
                                            int t, const int* cs, const SceneDesc& sd, float sfac) {
     constexpr int kRows = (SYM || FUSED) ? kRec - 3 : kRec;
     if (!q.mpm) {
@@ -263,11 +269,12 @@ __device__ __forceinline__ void p2g_record(const TileArgs& a, const P2GIn& q, fl
     }
 }
 
-template <bool FUSED, bool SYM>
+template <bool FUSED, bool SYM, int CHUNK = kChunkMax>
 __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     // a symmetric stress takes 6 record rows instead of 9: 22 rows x 576 columns
     // (51 KB) stage a whole tile of up to 512 particles in one round
     constexpr int kRows = (SYM || FUSED) ? kRec - 3 : kRec;
+    constexpr int kRecCols = rec_cols(CHUNK);
     constexpr int kLds = kRows * kRecCols > kPartFloats ? kRows * kRecCols : kPartFloats;
     __shared__ float4 smem4[(kLds + 3) / 4];
     __shared__ int cs[kTileCells + 1];
@@ -301,15 +308,15 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     for (int q = 0; q < 9; q++) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0f;
 
     const float sfac = -a.dt * a.inv_dx;
-    for (int c0 = 0; c0 < np; c0 += kChunk) {
-        const int cn = min(kChunk, np - c0);
+    for (int c0 = 0; c0 < np; c0 += CHUNK) {
+        const int cn = min(CHUNK, np - c0);
         // the previous round's readers are done with rec; the misbinned check reads cs
         if (c0 > 0 || a.misbinned) __syncthreads();
         if constexpr (FUSED) {
             for (int s = tid; s < cn; s += kBlock) {
                 P2GIn q;
                 p2g_load<FUSED, SYM>(a, n, a.sorted[p0 + c0 + s], q);
-                p2g_record<FUSED, SYM>(a, q, rec, rcol(s), p0 + c0 + s, t, cs, sd, sfac);
+                p2g_record<FUSED, SYM, kRecCols>(a, q, rec, rcol(s), p0 + c0 + s, t, cs, sd, sfac);
             }
         } else {
             // two particles per thread and pass: both load before either is used
@@ -321,8 +328,8 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
                 P2GIn q0, q1;
                 p2g_load<FUSED, SYM>(a, n, id0, q0);
                 p2g_load<FUSED, SYM>(a, n, id1, q1);
-                p2g_record<FUSED, SYM>(a, q0, rec, rcol(s), p0 + c0 + s, t, cs, sd, sfac);
-                if (two) p2g_record<FUSED, SYM>(a, q1, rec, rcol(s2), p0 + c0 + s2, t, cs, sd, sfac);
+                p2g_record<FUSED, SYM, kRecCols>(a, q0, rec, rcol(s), p0 + c0 + s, t, cs, sd, sfac);
+                if (two) p2g_record<FUSED, SYM, kRecCols>(a, q1, rec, rcol(s2), p0 + c0 + s2, t, cs, sd, sfac);
             }
         }
         __syncthreads();
@@ -345,14 +352,14 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
                 bspline(rec[1][r], wy, dwy);
                 bspline(rec[2][r], wz, dwz);
                 const float m = rec[3][r];
-                // momentum of node (i, j, k): w (A + Q (i, j, k)) with Q = m dx Cm, plus the
-                // stress impulse S grad w.  With a = wx wy_j, b = dwx wy_j, c = wx dwy_j the
-                // node gets wz_k (E + k F) + dwz_k G, E = a (A + Q (i, j, 0)) + S_0 b + S_1 c,
-                // F = a Q_2, G = a S_2 (S_d: column d of S), and the mass wz_k a m.
-                const float b0 = rec[4][r] + rec[7][r] * fi, b1 = rec[5][r] + rec[10][r] * fi,
-                            b2 = rec[6][r] + rec[13][r] * fi;
-                const float q10 = rec[8][r], q11 = rec[11][r], q12 = rec[14][r];
-                const float q20 = rec[9][r], q21 = rec[12][r], q22 = rec[15][r];
+                // Momentum of node (i, j, k): w (A + Q (i, j, k)) + S grad w, with w the
+                // product of the axis weights, A the staged momentum, Q = m dx Cm and S
+                // the stress impulse (columns S_d).  Factored along the axes, every FMA
+                // landing on an accumulator:
+                //   node (j, k) += wz_k E(j) + a_j H(k),   mass += wz_k wy_j wx m
+                //   E(j) = wy_j (P + j Qy) + dwy_j R,      a_j = wx wy_j
+                //   P = wx (A + i Q_0) + dwx S_0,  Qy = wx Q_1,  R = wx S_1
+                //   H(k) = k wz_k Q_2 + dwz_k S_2
                 float s00, s01, s02, s10, s11, s12, s20, s21, s22;
                 if (SYM || FUSED) {
                     s00 = rec[16][r];
@@ -366,29 +373,33 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
                     s10 = rec[19][r], s11 = rec[20][r], s12 = rec[21][r];
                     s20 = rec[22][r], s21 = rec[23][r], s22 = rec[24][r];
                 }
+                const float P[3] = {fmaf(wx, fmaf(rec[7][r], fi, rec[4][r]), dwx * s00),
+                                    fmaf(wx, fmaf(rec[10][r], fi, rec[5][r]), dwx * s10),
+                                    fmaf(wx, fmaf(rec[13][r], fi, rec[6][r]), dwx * s20)};
+                const float Qy[3] = {wx * rec[8][r], wx * rec[11][r], wx * rec[14][r]};
+                const float R[3] = {wx * s01, wx * s11, wx * s21};
+                const float q2[3] = {rec[9][r], rec[12][r], rec[15][r]};
+                const float sz[3] = {s02, s12, s22};
+                float H[3][3];
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        H[k][c] = k == 0 ? dwz[0] * sz[c] : fmaf((float)k * wz[k], q2[c], dwz[k] * sz[c]);
+                const float wxm = wx * m;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    const float a = wx * wy[j], b = dwx * wy[j], c = wx * dwy[j];
-                    const float M = a * m;
-                    const float c0 = j == 0 ? b0 : (j == 1 ? b0 + q10 : b0 + 2.0f * q10);
-                    const float c1 = j == 0 ? b1 : (j == 1 ? b1 + q11 : b1 + 2.0f * q11);
-                    const float c2 = j == 0 ? b2 : (j == 1 ? b2 + q12 : b2 + 2.0f * q12);
-                    const float E0 = a * c0 + s00 * b + s01 * c;
-                    const float E1 = a * c1 + s10 * b + s11 * c;
-                    const float E2 = a * c2 + s20 * b + s21 * c;
-                    const float F0 = a * q20, F1 = a * q21, F2 = a * q22;
-                    const float G0 = a * s02, G1 = a * s12, G2 = a * s22;
+                    const float aj = wx * wy[j], Mj = wxm * wy[j];
+                    float E[3];
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        E[c] = fmaf(wy[j], j == 0 ? P[c] : fmaf((float)j, Qy[c], P[c]), dwy[j] * R[c]);
 #pragma unroll
                     for (int k = 0; k < 3; k++) {
-                        // E + k F, with the k = 0 and k = 1 products folded away
-                        const float t0 = k == 0 ? E0 : (k == 1 ? E0 + F0 : E0 + 2.0f * F0);
-                        const float t1 = k == 0 ? E1 : (k == 1 ? E1 + F1 : E1 + 2.0f * F1);
-                        const float t2 = k == 0 ? E2 : (k == 1 ? E2 + F2 : E2 + 2.0f * F2);
                         float* A = acc[j * 3 + k];
-                        A[0] += wz[k] * M;
-                        A[1] += wz[k] * t0 + dwz[k] * G0;
-                        A[2] += wz[k] * t1 + dwz[k] * G1;
-                        A[3] += wz[k] * t2 + dwz[k] * G2;
+                        A[0] = fmaf(wz[k], Mj, A[0]);
+#pragma unroll
+                        for (int c = 0; c < 3; c++) A[c + 1] = fmaf(wz[k], E[c], fmaf(aj, H[k][c], A[c + 1]));
                     }
                 }
             }
@@ -427,10 +438,13 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     if (tid == 0) a.tile_flag[t] = 1;
 }
 
-// P2G only (the stress is read); SYM as for p2g_tile_body
-template <bool SYM>
-__global__ void __launch_bounds__(kBlock) p2g_tile_kernel(TileArgs a) {
-    p2g_tile_body<false, SYM>(a);
+// P2G only (the stress is read); SYM as for p2g_tile_body.  The waves-per-SIMD
+// hint asks the register allocator for the occupancy the chunk's LDS allows.
+template <bool SYM, int CHUNK = kChunkMax>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(CHUNK >= 512 ? 1 : (CHUNK >= 384 ? 4 : 5))))
+    p2g_tile_kernel(TileArgs a) {
+    p2g_tile_body<false, SYM, CHUNK>(a);
 }
 
 // the constitutive update of every particle inside the tile pass

@@ -578,7 +578,12 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
             if ((rc = check_launch("substep_stress", s))) return rc;
         }
         ta.stress = P->stress;
-        MPMX_LAUNCH(p2g_tile_kernel<true>, dim3(ntile_blocks - 1), dim3(kBlock), 0, s, ta);  // no orphan block
+        const dim3 tgrid(ntile_blocks - 1);  // no orphan block
+        switch (tuning(MPMX_TUNE_P2G_CHUNK)) {
+            case 1: MPMX_LAUNCH((p2g_tile_kernel<true, 384>), tgrid, dim3(kBlock), 0, s, ta); break;
+            case 2: MPMX_LAUNCH((p2g_tile_kernel<true, 256>), tgrid, dim3(kBlock), 0, s, ta); break;
+            default: MPMX_LAUNCH((p2g_tile_kernel<true, 512>), tgrid, dim3(kBlock), 0, s, ta); break;
+        }
         if ((rc = check_launch("substep_tile(p2g)", s))) return rc;
     } else {
         MPMX_LAUNCH(p2g_tile_fused_kernel<>, dim3(ntile_blocks), dim3(kBlock), 0, s, ta);

@@ -251,7 +251,46 @@ struct ContactPack {
     const float4* xr;       // x, y, z, radius
     const float4* vw;       // vx, vy, vz, 1 / mass
     const int32_t* flags;   // kCand* bits
+    const uint64_t* near;   // optional: bit c = a cell of the XPBD-only table within one cell of c holds a point
+    const int32_t* list;        // optional: the table positions to solve (mpmx_xpbd_contact_list)
+    const int32_t* list_count;  // their number
 };
+
+// Neighbourhood bitmap of a hash table, one bit per cell: set when the cell or
+// one of its 26 neighbours (the table's wrapped cell coordinates) holds a point.
+// occ first gets the occupancy bits; both hold cells / 64 rounded up words.
+__global__ void __launch_bounds__(kBlock) table_occupancy_kernel(const int32_t* __restrict__ cell_start, long ncell,
+                                                                 uint64_t* __restrict__ occ) {
+    if (cell_start[ncell] == 0) return;  // empty table: the contact solve reads no bitmap
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool full = c < ncell && cell_start[c + 1] > cell_start[c];
+    const uint64_t m = __ballot(full);
+    if ((threadIdx.x & 63) == 0 && c < ncell) occ[c >> 6] = m;
+}
+
+__global__ void __launch_bounds__(kBlock) table_near_kernel(const int32_t* __restrict__ cell_start, int dimx, int dimy,
+                                                            int dimz, const uint64_t* __restrict__ occ,
+                                                            uint64_t* __restrict__ near) {
+    const long ncell = (long)dimx * dimy * dimz;
+    if (cell_start[ncell] == 0) return;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool any = false;
+    if (c < ncell) {
+        const int cx = (int)(c % dimx), cy = (int)((c / dimx) % dimy), cz = (int)(c / ((long)dimx * dimy));
+        // the wrapped neighbour coordinates, as hash_index wraps cell coordinates
+        const int xs[3] = {cx == 0 ? dimx - 1 : cx - 1, cx, cx + 1 == dimx ? 0 : cx + 1};
+        const int ys[3] = {cy == 0 ? dimy - 1 : cy - 1, cy, cy + 1 == dimy ? 0 : cy + 1};
+        const int zs[3] = {cz == 0 ? dimz - 1 : cz - 1, cz, cz + 1 == dimz ? 0 : cz + 1};
+        for (int a = 0; a < 3 && !any; a++)
+            for (int b = 0; b < 3 && !any; b++)
+                for (int d = 0; d < 3 && !any; d++) {
+                    const long q = ((long)zs[a] * dimy + ys[b]) * dimx + xs[d];
+                    any = (occ[q >> 6] >> (q & 63)) & 1ull;
+                }
+    }
+    const uint64_t m = __ballot(any);
+    if ((threadIdx.x & 63) == 0 && c < ncell) near[c >> 6] = m;
+}
 constexpr int kCandValid = 1;  // active == 1 and mass > 0
 constexpr int kCandMpm = 2;    // label <= 1
 constexpr int kCandXpbd = 4;   // label == 2
@@ -310,25 +349,71 @@ __device__ __forceinline__ bool maybe_in_reach(float d2, float base, float rj) {
     return !(d2 > fmaxf(r * r, 1.0e-30f));
 }
 
+// The table positions whose contact walk can find a contact in this XPBD step:
+// every position the contact solve does not return from at once, except MPM
+// particles that walk the XPBD-only table from a cell the neighbourhood bitmap
+// shows to be beside no XPBD point.  Those find no candidate in any iteration:
+// the table is fixed during the step, and such a particle gets no delta, so its
+// position (and query box) stays where it is.  Each wave appends its kept
+// positions with one atomic; the list order is free, because every position
+// writes only its own particle's delta and the lock writes only turn label 1
+// into 0, which no test of the solve distinguishes.
+__global__ void __launch_bounds__(kBlock) contact_list_kernel(
+    int n, const int32_t* __restrict__ mat, const int32_t* __restrict__ point_pos, const int32_t* __restrict__ xp_start,
+    float inv, int xdimx, int xdimy, int xdimz, const float* __restrict__ x, const float* __restrict__ radius,
+    float k_coh, float max_radius, const uint64_t* __restrict__ near, int32_t* __restrict__ list,
+    int32_t* __restrict__ count) {
+    // one thread per particle in array order (coalesced); the list holds table positions
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool keep = false;
+    if (i < n && xp_start[(long)xdimx * xdimy * xdimz] > 0) {
+        keep = true;
+        if (mat[i] <= 1) {  // an MPM particle: does its walk meet an XPBD point?
+            const f3 xi = ld3(x, n, i);
+            const float ri = radius[i];
+            const float qr = ri + max_radius + k_coh * ri;
+            const int xs = (int)((xi.x - qr) * inv), ys = (int)((xi.y - qr) * inv), zs = (int)((xi.z - qr) * inv);
+            const int xe = min((int)((xi.x + qr) * inv), xs + xdimx - 1);
+            const int ye = min((int)((xi.y + qr) * inv), ys + xdimy - 1);
+            const int ze = min((int)((xi.z + qr) * inv), zs + xdimz - 1);
+            // (the solve's own test for walking the XPBD-only table, on the same box)
+            const float reach = (ri + max_radius + k_coh) * inv;
+            const int gapx = xdimx - (xe - xs + 1) - 1, gapy = xdimy - (ye - ys + 1) - 1;
+            const int gapz = xdimz - (ze - zs + 1) - 1;
+            const int cx = (int)(xi.x * inv), cy = (int)(xi.y * inv), cz = (int)(xi.z * inv);
+            if ((float)gapx > reach && (float)gapy > reach && (float)gapz > reach && xs >= cx - 1 && xe <= cx + 1 &&
+                ys >= cy - 1 && ye <= cy + 1 && zs >= cz - 1 && ze <= cz + 1) {
+                const int c = hash_index(xdimx, xdimy, xdimz, cx, cy, cz);
+                keep = (near[c >> 6] >> (c & 63)) & 1ull;
+            }
+        }
+    }
+    const uint64_t m = __ballot(keep);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, first);
+    if (keep) list[base + __popcll(m & ((1ull << lane) - 1ull))] = point_pos[i];
+}
+
 // my_solve_particle_particle_contacts (xpbdRoutines.py:138-219).
 // With xp_ids/xp_start (a table of the label-2 points only, same hashing and
 // stable order) an MPM particle i walks that table instead of the full one: it
 // skips every MPM candidate anyway (:185-186), and the XPBD candidates come in
 // the same relative order, so deltas and lock writes are unchanged.  PACKED
 // reads the candidates from ContactPack records instead of the particle arrays.
+// the contact walk of table position tid (one per thread); queue is the
+// workgroup's LDS queue of candidate positions, [kContactQueue][kBlock]
 template <bool PACKED>
-__global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
-    int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
-    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids, const int32_t* __restrict__ xp_start,
-    float inv, int dimx, int dimy, int dimz, int xdimx, int xdimy, int xdimz, const float* __restrict__ x,
-    const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
-    float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
-    const int32_t* __restrict__ slot_of_id, ContactPack pk) {
-    __shared__ int32_t queue[kContactQueue][kBlock];
-    // table positions in XCD-contiguous ranges: the cells a block walks are
-    // those of the blocks beside it in the table, so they share one L2
-    int tid = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (tid >= n) return;
+__device__ __forceinline__ void contact_one(
+    int tid, int32_t (*queue)[kBlock], int n, const int32_t* __restrict__ active, int32_t* mat,
+    const int32_t* __restrict__ point_ids, const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids,
+    const int32_t* __restrict__ xp_start, float inv, int dimx, int dimy, int dimz, int xdimx, int xdimy, int xdimz,
+    const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ mass,
+    const float* __restrict__ radius, float k_mu_in, float thr, float smu, float k_coh, float max_radius, float dt,
+    float relax, int lock, float* __restrict__ deltas, const int32_t* __restrict__ slot_of_id, const ContactPack& pk) {
     // with no XPBD particle at all there is no pair to solve (MPM-MPM pairs are skipped)
     if (xp_ids && xp_start[(long)xdimx * xdimy * xdimz] == 0) return;
     // The reference tests the labels and the mass of array index tid, the thread's
@@ -372,6 +457,16 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
         hz = xdimz;
         rec = n;
         self = -1;
+        if constexpr (PACKED) {
+            // A query box within one cell of the particle's own cell, whose cell is not
+            // beside any XPBD point, walks only empty cells: no contact, no lock write.
+            const int cx = (int)(xi.x * inv), cy = (int)(xi.y * inv), cz = (int)(xi.z * inv);
+            if (pk.near && xs >= cx - 1 && xe <= cx + 1 && ys >= cy - 1 && ye <= cy + 1 && zs >= cz - 1 &&
+                ze <= cz + 1) {
+                const int c = hash_index(hx, hy, hz, cx, cy, cz);
+                if (!((pk.near[c >> 6] >> (c & 63)) & 1ull)) return;
+            }
+        }
     }
     const float base = fabsf(ri) + fabsf(k_coh);
     float k_mu = k_mu_in;
@@ -476,6 +571,38 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     deltas[i] += d0 * relax;
     deltas[n + i] += d1 * relax;
     deltas[2L * n + i] += d2 * relax;
+}
+
+template <bool PACKED>
+__global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
+    int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
+    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids, const int32_t* __restrict__ xp_start,
+    float inv, int dimx, int dimy, int dimz, int xdimx, int xdimy, int xdimz, const float* __restrict__ x,
+    const float* __restrict__ v, const float* __restrict__ mass, const float* __restrict__ radius, float k_mu_in,
+    float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
+    const int32_t* __restrict__ slot_of_id, ContactPack pk) {
+    __shared__ int32_t queue[kContactQueue][kBlock];
+    if (PACKED && pk.list) {
+        // the listed table positions over a bounded grid (the list is short while
+        // few particles are in the XPBD phase): each workgroup takes a contiguous
+        // range of the list, and the ranges go to the XCDs in contiguous runs
+        const int cnt = *pk.list_count;
+        const int per = (cnt + (int)gridDim.x - 1) / (int)gridDim.x;
+        const int b = xcd_tile(blockIdx.x, gridDim.x);
+        const int t1 = min(cnt, (b + 1) * per);
+        for (int t = b * per + (int)threadIdx.x; t < t1; t += blockDim.x)
+            contact_one<PACKED>(pk.list[t], queue, n, active, mat, point_ids, cell_start, xp_ids, xp_start, inv, dimx,
+                                dimy, dimz, xdimx, xdimy, xdimz, x, v, mass, radius, k_mu_in, thr, smu, k_coh,
+                                max_radius, dt, relax, lock, deltas, slot_of_id, pk);
+        return;
+    }
+    // table positions in XCD-contiguous ranges: the cells a block walks are
+    // those of the blocks beside it in the table, so they share one L2
+    const int tid = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    contact_one<PACKED>(tid, queue, n, active, mat, point_ids, cell_start, xp_ids, xp_start, inv, dimx, dimy, dimz,
+                        xdimx, xdimy, xdimz, x, v, mass, radius, k_mu_in, thr, smu, k_coh, max_radius, dt, relax, lock,
+                        deltas, slot_of_id, pk);
 }
 
 // my_apply_particle_deltas (xpbdRoutines.py:221-257)
@@ -725,6 +852,43 @@ int mpmx_hashgrid_build_pair(int n, const float* x, float cell_width, int dimx, 
                              point_pos ? pos : nullptr, label_cell_start + table_keys(dims[1]), lazy != 0);
 }
 
+size_t mpmx_hashgrid_near_words(int dimx, int dimy, int dimz) {
+    const long ncell = (long)dimx * dimy * dimz;
+    return (size_t)((ncell + 63) / 64);
+}
+
+int mpmx_hashgrid_near_cells(const int32_t* cell_start, int dimx, int dimy, int dimz, uint64_t* occ, uint64_t* near,
+                             void* stream) {
+    MPMX_REQUIRE(dimx > 0 && dimy > 0 && dimz > 0 && (long)dimx * dimy * dimz < (1L << 31),
+                 "mpmx_hashgrid_near_cells: bad dims");
+    MPMX_REQUIRE(cell_start && occ && near, "mpmx_hashgrid_near_cells: null pointer");
+    const long ncell = (long)dimx * dimy * dimz;
+    hipStream_t s = as_stream(stream);
+    MPMX_LAUNCH(table_occupancy_kernel, dim3(grid_for(ncell, kBlock)), dim3(kBlock), 0, s, cell_start, ncell, occ);
+    int rc = check_launch("table_occupancy", s);
+    if (rc) return rc;
+    MPMX_LAUNCH(table_near_kernel, dim3(grid_for(ncell, kBlock)), dim3(kBlock), 0, s, cell_start, dimx, dimy, dimz,
+                (const uint64_t*)occ, near);
+    return check_launch("table_near", s);
+}
+
+int mpmx_xpbd_contact_list(int n, const int32_t* materialLabel, const int32_t* point_pos,
+                           const int32_t* xpbd_cell_start, float cell_width, int xpbd_dimx, int xpbd_dimy,
+                           int xpbd_dimz, const float* x, const float* radius, float k_cohesion, float max_radius,
+                           const uint64_t* xpbd_near, int32_t* list, int32_t* count, void* stream) {
+    MPMX_REQUIRE(n >= 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0 && cell_width > 0.0f,
+                 "mpmx_xpbd_contact_list: bad sizes");
+    MPMX_REQUIRE(count && xpbd_cell_start, "mpmx_xpbd_contact_list: null pointer");
+    hipStream_t s = as_stream(stream);
+    int rc = fill_i32(count, 1, 0, s);
+    if (rc || n == 0) return rc;
+    MPMX_REQUIRE(materialLabel && point_pos && x && radius && xpbd_near && list, "mpmx_xpbd_contact_list: null pointer");
+    MPMX_LAUNCH(contact_list_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, materialLabel, point_pos,
+                xpbd_cell_start, 1.0f / cell_width, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, radius, k_cohesion, max_radius,
+                xpbd_near, list, count);
+    return check_launch("contact_list", s);
+}
+
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
                         int32_t* cell_start, void* workspace, size_t ws_bytes, void* stream) {
     return hashgrid_build(n, x, cell_width, dimx, dimy, dimz, nullptr, 0, point_ids, cell_start, workspace, ws_bytes,
@@ -838,20 +1002,28 @@ int mpmx_solve_particle_particle_contacts_packed(
     int xpbd_dimx, int xpbd_dimy, int xpbd_dimz, const float* x, const float* v, const float* mass,
     const float* radius, const float* xr, const float* vw, const int32_t* flags, float k_mu, float static_threshold,
     float static_friction, float k_cohesion, float max_radius, float dt, float relaxation, int contact_lock,
-    float* deltas, const int32_t* slot_of_id, void* stream) {
+    float* deltas, const int32_t* slot_of_id, const uint64_t* xpbd_near, const int32_t* contact_list,
+    const int32_t* contact_count, void* stream) {
     MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0 &&
                      cell_width > 0.0f,
                  "mpmx_solve_particle_particle_contacts_packed: bad sizes");
+    MPMX_REQUIRE((contact_list == nullptr) == (contact_count == nullptr),
+                 "mpmx_solve_particle_particle_contacts_packed: give the contact list with its count");
     if (n == 0) return 0;
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && xpbd_point_ids && xpbd_cell_start && x &&
                      v && mass && radius && xr && vw && flags && deltas,
                  "mpmx_solve_particle_particle_contacts_packed: null pointer");
     MPMX_REQUIRE(((uintptr_t)xr | (uintptr_t)vw) % 16 == 0,
                  "mpmx_solve_particle_particle_contacts_packed: records need 16-byte alignment");
-    LAUNCH1(particle_contacts_kernel<true>, n, n, activeLabel, materialLabel, point_ids, cell_start, xpbd_point_ids,
-            xpbd_cell_start, 1.0f / cell_width, dimx, dimy, dimz, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, v, mass, radius,
-            k_mu, static_threshold, static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas,
-            slot_of_id, (ContactPack{(const float4*)xr, (const float4*)vw, flags}))
+    // with a list, a bounded grid strides over it (8 workgroups per CU at most)
+    const unsigned blocks = contact_list ? (grid_for(n, kBlock) < 2048u ? grid_for(n, kBlock) : 2048u)
+                                         : grid_for(n, kBlock);
+    MPMX_LAUNCH(particle_contacts_kernel<true>, dim3(blocks), dim3(kBlock), 0, as_stream(stream), n, activeLabel,
+                materialLabel, point_ids, cell_start, xpbd_point_ids, xpbd_cell_start, 1.0f / cell_width, dimx, dimy,
+                dimz, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, v, mass, radius, k_mu, static_threshold, static_friction,
+                k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, slot_of_id,
+                (ContactPack{(const float4*)xr, (const float4*)vw, flags, xpbd_near, contact_list, contact_count}));
+    return check_launch("particle_contacts_kernel<true>", as_stream(stream));
 }
 
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,

@@ -290,6 +290,18 @@ class HashGrid:
         cs = self.cell_start.numpy()
         return cs[:-1], cs[1:]
 
+    def near_cells(self) -> Array:
+        """The grid's neighbourhood bitmap (mpmx_hashgrid_near_cells): bit c set
+        when cell c or one of its 26 neighbours holds a point.  Kept on the grid
+        (64-bit words stored as pairs of int32) and returned."""
+        words = int(lib.raw("mpmx_hashgrid_near_words")(*self.dims))
+        if getattr(self, "near", None) is None or self.near.shape[0] != 2 * words:
+            self.near = zeros(2 * words, int32)
+            self._occ = zeros(2 * words, int32)
+        lib("mpmx_hashgrid_near_cells", self.cell_start.ptr, *self.dims, self._occ.ptr, self.near.ptr,
+            stream_handle())
+        return self.near
+
     def build_pair(self, other: "HashGrid", points: Array, cell_width: float, labels: Array, label: int = 2,
                    order: Array | None = None, lazy: bool = False) -> None:
         """``self.build(points, cell_width, order=order)`` and

@@ -308,6 +308,15 @@ def _contact_records(xpbd, n):
     return rec
 
 
+def _contact_list(xpbd, n):
+    """The fused step's contact list (mpmx_xpbd_contact_list) and its count."""
+    cl = getattr(xpbd, "_contact_list", None)
+    if cl is None or cl[0].shape[0] != n:
+        cl = (rt.empty(n, rt.int32), rt.zeros(1, rt.int32))
+        xpbd._contact_list = cl
+    return cl
+
+
 def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     """xpbdSimulationStep with the copies folded into its kernels: x, v, the
     labels, particle_cumDist_xpbd and the sleep counters end exactly as after
@@ -334,11 +343,17 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
                         dt=float(sim.dtxpbd), relaxation=float(xpbd.xpbd_relaxation),
                         delta=xpbd.particle_delta.ptr.value)
     grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
+    near = xgrid.near_cells()  # MPM particles with no XPBD point beside their cell skip their walk
     xr, vw, flags = _contact_records(xpbd, n)
+    clist, ccount = _contact_list(xpbd, n)
     bounds.xr, bounds.vw, bounds.flags = xr.ptr.value, vw.ptr.value, flags.ptr.value
     bounds.pos, bounds.xpbd_pos = grid.point_pos.ptr.value, xgrid.point_pos.ptr.value
     bounds.xpbd_count = xgrid.cell_start.ptr.value + 4 * int(np.prod(xgrid.dims))  # no records without XPBD points
     bref = C.byref(bounds)
+    dims = grid.dims
+    lib("mpmx_xpbd_contact_list", n, sim.materialLabel.ptr, grid.point_pos.ptr, xgrid.cell_start.ptr,
+        float(grid.cell_width), *xgrid.dims, sim.particle_x.ptr, sim.particle_radius.ptr,
+        float(xpbd.particle_cohesion), float(xpbd.max_radius), near.ptr, clist.ptr, ccount.ptr, s)
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
         float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
         sim.numTotalXPBD.ptr, bref if iters > 0 else None, s)  # also zeroes numTotalXPBD and numSleepingXPBD
@@ -352,7 +367,7 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
             sim.particle_radius.ptr, xr.ptr, vw.ptr, flags.ptr, float(xpbd.dynamicParticleFriction),
             float(xpbd.staticVelocityThreshold), float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion),
             float(xpbd.max_radius), float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock),
-            xpbd.particle_delta.ptr, _aptr(getattr(sim, "_slot_of_id", None)), s)
+            xpbd.particle_delta.ptr, _aptr(getattr(sim, "_slot_of_id", None)), near.ptr, clist.ptr, ccount.ptr, s)
         if it + 1 < iters:
             lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
                 float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, sim.materialLabel.ptr, bref, s)

@@ -349,19 +349,27 @@ def test_contact_pack_records():
         assert np.array_equal(xr[:k][ok], pxr[order][ok]) and np.array_equal(vw[:k][ok], pvw[order][ok])
 
 
-@pytest.mark.parametrize("lock", [0, 1])
-def test_packed_contacts_from_gathered_records(lock):
+@pytest.mark.parametrize("lock,few", [(0, False), (1, False), (1, True)])
+def test_packed_contacts_from_gathered_records(lock, few):
     """The packed contact solve over records that mpmx_xpbd_contact_pack gathers
     from the particle arrays gives the deltas and labels of the split solve,
-    bit for bit."""
+    bit for bit -- with the XPBD table's neighbourhood bitmap letting MPM
+    particles far from every XPBD point skip their walk (few: only a handful of
+    XPBD particles, as just after the first transitions of a run)."""
     import torch
     from mpmwarpydw_amd import runtime as rt
     from mpmwarpydw_amd._lib import lib
     outs = []
     for packed in (False, True):
-        S = xpbd_scene(7, cluster=32)
+        S = xpbd_scene(7, cluster=0 if few else 32)
         sim, xpbd = S.sim, S.xpbd
         n = sim.nPoints
+        if few:
+            lab = sim.materialLabel.numpy()
+            keep = np.flatnonzero(lab == 2)[::97][:6]
+            lab[lab == 2] = 1
+            lab[keep] = 2
+            sim.materialLabel.assign(lab)
         grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
         grid.build_pair(xgrid, sim.particle_x, sim.dx, sim.materialLabel, 2, None)
         xpbd.particle_delta.zero_()
@@ -374,6 +382,16 @@ def test_packed_contacts_from_gathered_records(lock):
                 xgrid.point_ids.ptr, xgrid.cell_start.ptr, *xgrid.dims, xr.data_ptr(), vw.data_ptr(), fl.data_ptr(),
                 rt.stream_handle())
             rec = [xr.data_ptr(), vw.data_ptr(), fl.data_ptr()]
+            near = xgrid.near_cells().ptr  # MPM particles far from every XPBD point are not listed
+            clist = torch.empty(n, dtype=torch.int32, device=rt.device())
+            ccount = torch.empty(1, dtype=torch.int32, device=rt.device())
+            lib("mpmx_xpbd_contact_list", n, sim.materialLabel.ptr, grid.point_pos.ptr, xgrid.cell_start.ptr,
+                float(grid.cell_width), *xgrid.dims, sim.particle_x.ptr, sim.particle_radius.ptr,
+                float(xpbd.particle_cohesion), float(xpbd.max_radius), near, clist.data_ptr(), ccount.data_ptr(),
+                rt.stream_handle())
+            listed = int(ccount.item())
+            print(f"  contact list: {listed} of {n} table positions", flush=True)
+            assert 0 < listed <= n and (listed < n // 4 or not few)
         name = "mpmx_solve_particle_particle_contacts_" + ("packed" if packed else "split")
         print(f"[{name}] n={n} lock={lock}", flush=True)
         lib(name, n, sim.activeLabel.ptr, sim.materialLabel.ptr, grid.point_ids.ptr, grid.cell_start.ptr,
@@ -381,7 +399,8 @@ def test_packed_contacts_from_gathered_records(lock):
             sim.particle_x.ptr, sim.particle_v.ptr, sim.particle_mass.ptr, sim.particle_radius.ptr,
             *(rec if packed else []), float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
             float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
-            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), lock, xpbd.particle_delta.ptr, None, rt.stream_handle())
+            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), lock, xpbd.particle_delta.ptr, None,
+            *([near, clist.data_ptr(), ccount.data_ptr()] if packed else []), rt.stream_handle())
         outs.append((xpbd.particle_delta.numpy(), sim.materialLabel.numpy()))
     assert np.abs(outs[0][0]).max() > 0
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
