@@ -682,12 +682,15 @@ int mpmx_mlp_loss_grad(int B, const float* params, const float* X, const float* 
 int mpmx_mlp_adam_step(int n, float* params, const float* grad, float* m, float* v, float lr,
                        float beta1, float beta2, float eps, int t, void* stream);
 /* Training pairs from the simulator's own state after G2P: for every particle,
- * flag = 1 (active MPM, plastic trial state), 0 (active MPM, elastic) or -1;
- * for MPM particles X[p] = the 10 features (principal trial log strains over the
- * yield strain eref = max(ys,1)/(2 mu), log10 E - 8, 4 nu - 1, log10 cohesion
- * - 5, tan friction angle, tan dilation angle, 20 acc, damage), T[p] = (e' - e)
- * / eref of the exact projection, e_trial[p] = the principal trial log
- * strains.  The state is not modified. */
+ * flag = 1 (active MPM, plastic trial state), 0 (active MPM, elastic) or -1.
+ * With c = max(ys, 1), eref = c / (2 mu), d the deviator of the principal
+ * trial log strains e, Y the yield value, the overshoot phi = (2 mu |d| - Y)/c,
+ * y0 = Y / c and kap = 2 mu 1e-6 / c, X[p] = the 10 features 0.1 ln(phi + kap),
+ * 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d/|d| (3), tan friction angle, 4 nu - 1,
+ * log10 E - 8, damage; T[p] = (s, alpha, 0) for plastic states (zeros for
+ * elastic ones), with s = Y / (2 mu (|d| + 1e-6)), so that the exact projection
+ * is e' = e - (1 - Y/(2 mu |d|)) d - eref kap s d/|d| + 0.3 alpha eref
+ * (phi + kap); e_trial[p] = e.  The state is not modified. */
 int mpmx_lrm_harvest(const mpmx_particles* particles, float* X, float* T, float* e_trial,
                      int32_t* flag, void* stream);
 /* compute_stress_from_F_trial (mpmRoutines.py:229-312) with the projection of
@@ -699,6 +702,16 @@ int mpmx_lrm_harvest(const mpmx_particles* particles, float* X, float* T, float*
  * accepted predictions, particles updated}. */
 int mpmx_compute_stress_learned(const mpmx_particles* particles, float eff, const float* params,
                                 float tol, int32_t* counters, void* stream);
+/* mpmx_mpm_substep and mpmx_ensemble_substep with the constitutive update of
+ * mpmx_compute_stress_learned (network mlp_params, tolerance tol, counters as
+ * there) run as its own kernel before the P2G tile kernel. */
+int mpmx_mpm_substep_learned(const mpmx_particles* particles, const mpmx_grid* grid,
+                             const mpmx_step_params* params, const float* mlp_params, float tol,
+                             int32_t* counters, void* workspace, size_t ws_bytes, void* stream);
+int mpmx_ensemble_substep_learned(const mpmx_particles* particles, const mpmx_scene* scenes,
+                                  int nscenes, float* grid_m, float* grid_v_in, float* grid_v_out,
+                                  const mpmx_step_params* params, const float* mlp_params, float tol,
+                                  int32_t* counters, void* workspace, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

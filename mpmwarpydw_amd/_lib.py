@@ -115,6 +115,8 @@ SIGNATURES = {
     "mpmx_mlp_adam_step": [I, P, P, P, P, F, F, F, F, I, P],
     "mpmx_lrm_harvest": [P, P, P, P, P, P],
     "mpmx_compute_stress_learned": [P, F, P, F, P, P],
+    "mpmx_mpm_substep_learned": [P, P, P, P, F, P, P, S, P],
+    "mpmx_ensemble_substep_learned": [P, P, I, P, P, P, P, P, F, P, P, S, P],
 }
 _RESTYPE = {
     "mpmx_substep_workspace_bytes": S,

@@ -7,13 +7,20 @@
 // pressure p = mean(2 mu e + lam sum e) and Y = (1 - D)(ys - alpha p), a state
 // is plastic when 2 mu |d| > Y or ys < 0 or Y < 0, and the exact projection is
 //     e' = e - (dgam / edn) d + 0.3 alpha dgam,   dgam = edn - Y / (2 mu).
-// The network sees 10 features (e / eref, log10 E - 8, 4 nu - 1,
-// log10 cohesion - 5, tan of the friction and dilation angles, 20 acc, D, with
-// eref = max(ys, 1) / (2 mu)) and predicts (e' - e) / eref.  The exact map is
-// the judge: a prediction is used only when both residuals
+// With c = max(ys, 1), eref = c / (2 mu), the overshoot phi = (2 mu |d| - Y)/c,
+// y0 = Y / c and kap = 2 mu 1e-6 / c this is the classical radial return
+// e0' = e - (1 - Y / (2 mu |d|)) d plus two corrections,
+//     e' = e0' - eref kap s d/|d| + 0.3 alpha eref (phi + kap) (1, 1, 1),
+// where s = y0 / (phi + y0 + kap) is the part of the deviator that the 1e-6
+// keeps and alpha the cone coefficient that sets the dilatancy.  The network
+// predicts y = (s, alpha, 0) from 10 features: 0.1 ln(phi + kap),
+// 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d / |d|, tan(friction angle), 4 nu - 1,
+// log10 E - 8 and D.  The exact map is the judge: a prediction is used only
+// when both residuals
 //     2 mu |dev e' - Y/(2 mu edn) d|              (yield surface and flow direction)
 //     K |tr e' - tr e - 0.9 alpha dgam|           (dilatancy)
-// are within tol * max(cohesion, 1); otherwise the exact projection replaces it.
+// are within tol * max(cohesion, 1); otherwise, and for Y < 0 (tension past
+// the cone's apex), the exact projection replaces it.
 // Everything else (plastic strain, damage, hardening, phase transition,
 // Kirchhoff stress, viscosity) is the exact update.
 #include "mlp.hpp"
@@ -51,18 +58,26 @@ MPMX_HD Principal principal_state(const float eps[3], float mu, float lam, float
 }
 
 MPMX_HD void lrm_features(const Principal& s, float mu, float lam, float ys, float al, float acc, float D, float x[10]) {
+    (void)acc;
     const float E = mu * (3.0f * lam + 2.0f * mu) / (lam + mu);
     const float nu = lam / (2.0f * (lam + mu));
     const float c = ys > 1.0f ? ys : 1.0f;
-    const float er = c / (2.0f * mu);
-    for (int k = 0; k < 3; k++) x[k] = s.e[k] / er;
-    x[3] = log10f(E) - 8.0f;
-    x[4] = 4.0f * nu - 1.0f;
-    x[5] = log10f(c) - 5.0f;
+    const float kap = 2.0f * mu * 1.0e-6f / c, y0 = s.Y / c;
+    const float pk = (2.0f * mu * s.dn - s.Y) / c + kap;
+    x[0] = 0.1f * logf(pk > 1.0e-30f ? pk : 1.0e-30f);
+    x[1] = 0.1f * logf(y0 > 1.0e-12f ? y0 : 1.0e-12f);
+    x[2] = 0.1f * logf(kap);
+    for (int k = 0; k < 3; k++) x[3 + k] = s.dn > 0.0f ? s.d[k] / s.dn : 0.0f;
     x[6] = lrm_tan_of_alpha(al);
-    x[7] = lrm_tan_of_alpha(0.3f * al);
-    x[8] = 20.0f * acc;
+    x[7] = 4.0f * nu - 1.0f;
+    x[8] = log10f(E) - 8.0f;
     x[9] = D;
+}
+
+// the radial return e0' = e - (1 - Y / (2 mu |d|)) d (e for |d| = 0)
+MPMX_HD void radial_return(const Principal& s, float mu, float e0[3]) {
+    const float r0 = s.dn > 0.0f ? 1.0f - s.Y / (2.0f * mu * s.dn) : 0.0f;
+    for (int k = 0; k < 3; k++) e0[k] = s.e[k] - r0 * s.d[k];
 }
 
 // exact projection e' of a plastic state and its dgam
@@ -86,9 +101,10 @@ MPMX_HD bool lrm_judge(const Principal& s, const float q[3], float mu, float lam
     return res1 <= lim && res2 <= lim;
 }
 
-// Harvest: features, exact targets (e' - e)/eref and the state of every particle
-// (1 plastic MPM, 0 elastic MPM, -1 not an active MPM particle); e_trial gets
-// the principal trial log strains.  Reads the state, writes nothing of it.
+// Harvest: features, exact targets (s, alpha, 0) and the state of every
+// particle (1 plastic MPM, 0 elastic MPM, -1 not an active MPM particle);
+// e_trial gets the principal trial log strains.  Reads the state, writes
+// nothing of it.
 __global__ void lrm_harvest_kernel(mpmx_particles P, float* __restrict__ X, float* __restrict__ Tg,
                                    float* __restrict__ E3, int32_t* __restrict__ flag) {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,13 +124,10 @@ __global__ void lrm_harvest_kernel(mpmx_particles P, float* __restrict__ X, floa
     float x[10];
     lrm_features(s, mu, lam, ys, al, P.acc_strain[p], D, x);
     for (int k = 0; k < 10; k++) X[p * 10 + k] = x[k];
-    const float er = (ys > 1.0f ? ys : 1.0f) / (2.0f * mu);
-    float en[3] = {eps[0], eps[1], eps[2]}, dg = 0.0f;
-    if (s.plastic) exact_projection(s, mu, al, en, dg);
-    for (int k = 0; k < 3; k++) {
-        Tg[p * 3 + k] = (en[k] - eps[k]) / er;
-        E3[p * 3 + k] = eps[k];
-    }
+    Tg[p * 3 + 0] = s.plastic ? s.Y / (2.0f * mu * (s.dn + 1.0e-6f)) : 0.0f;
+    Tg[p * 3 + 1] = s.plastic ? al : 0.0f;
+    Tg[p * 3 + 2] = 0.0f;
+    for (int k = 0; k < 3; k++) E3[p * 3 + k] = eps[k];
     flag[p] = s.plastic ? 1 : 0;
 }
 
@@ -269,7 +282,8 @@ __global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P
                         if (f0 + j == q) xv[j] = v;
                 }
                 MlpTile tt;
-                mlp_tile_forward(w, pack_bf16(xv[0], xv[1], xv[2], xv[3]), lane, tt);
+                mlp_tile_forward(w, pack_bf16(xv[0], xv[1], xv[2], xv[3]), pack_bf16_lo(xv[0], xv[1], xv[2], xv[3]),
+                                 lane, tt);
                 // lanes 0..15 hold the outputs of particles t*16 + 0..15
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
@@ -284,10 +298,14 @@ __global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P
             if (plastic) {
                 n_plastic++;
                 bool ok = false;
-                if (use_net) {
-                    const float er = (ys > 1.0f ? ys : 1.0f) / (2.0f * mu);
+                if (use_net && s.Y >= 0.0f) {
+                    const float c = ys > 1.0f ? ys : 1.0f;
+                    const float er = c / (2.0f * mu), kap = 2.0f * mu * 1.0e-6f / c;
+                    const float dil = 0.3f * yo[1] * (er * ((2.0f * mu * s.dn - s.Y) / c + kap));
+                    const float shr = s.dn > 0.0f ? er * kap * yo[0] / s.dn : 0.0f;
                     float q[3];
-                    for (int k = 0; k < 3; k++) q[k] = eps[k] + er * yo[k];
+                    radial_return(s, mu, q);
+                    for (int k = 0; k < 3; k++) q[k] = (q[k] - shr * s.d[k]) + dil;
                     ok = lrm_judge(s, q, mu, lam, ys, al, tol);
                     if (ok) {
                         for (int k = 0; k < 3; k++) en[k] = q[k];
@@ -324,6 +342,17 @@ static int check_particles(const mpmx_particles* P) {
     return 0;
 }
 
+int launch_stress_learned(const mpmx_particles& P, float eff, const float* params, float tol, int32_t* counters,
+                          hipStream_t s) {
+    if (P.n == 0) return 0;
+    const long waves = ((long)P.n + 63) / 64;
+    long nb = (waves + 3) / 4;
+    if (nb > 8192) nb = 8192;
+    MPMX_LAUNCH(stress_learned_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, P, eff, params, tol, params ? 1 : 0,
+                counters);
+    return check_launch("stress_learned", s);
+}
+
 }  // namespace mpmx
 
 using namespace mpmx;
@@ -349,13 +378,7 @@ int mpmx_compute_stress_learned(const mpmx_particles* P, float eff, const float*
                      P->softening && P->density && P->strainCriteria && P->eta_shear && P->eta_bulk,
                  "mpmx_compute_stress_learned: null particle array");
     MPMX_REQUIRE(tol >= 0.0f, "mpmx_compute_stress_learned: negative tolerance");
-    hipStream_t s = as_stream(stream);
-    const long waves = ((long)P->n + 63) / 64;
-    long nb = (waves + 3) / 4;
-    if (nb > 8192) nb = 8192;
-    MPMX_LAUNCH(stress_learned_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, *P, eff, params, tol,
-                params ? 1 : 0, counters);
-    return check_launch("stress_learned", s);
+    return launch_stress_learned(*P, eff, params, tol, counters, as_stream(stream));
 }
 
 }  // extern "C"

@@ -18,14 +18,17 @@ namespace mpmx {
 
 constexpr int kMlpBlocks = 256;  // workgroups of the training kernel (at most)
 
-__device__ __forceinline__ bf16x4 x_frag(const float* __restrict__ X, int B, int tile, int lane) {
-    // B operand: lane l holds features 4(l/16)..+3 of sample tile*16 + l%16
+// B operands of the inputs, high and low bfloat16 halves: lane l holds features
+// 4(l/16)..+3 of sample tile*16 + l%16
+__device__ __forceinline__ void x_frag(const float* __restrict__ X, int B, int tile, int lane, bf16x4& hi,
+                                       bf16x4& lo) {
     const int b = tile * 16 + (lane & 15);
     const int f0 = 4 * (lane >> 4);
     float v[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = (b < B && f0 + j < kNI) ? X[(long)b * kNI + f0 + j] : 0.0f;
-    return pack_bf16(v[0], v[1], v[2], v[3]);
+    hi = pack_bf16(v[0], v[1], v[2], v[3]);
+    lo = pack_bf16_lo(v[0], v[1], v[2], v[3]);
 }
 
 __global__ void __launch_bounds__(kBlock) mlp_forward_kernel(int B, const float* __restrict__ prm,
@@ -38,7 +41,9 @@ __global__ void __launch_bounds__(kBlock) mlp_forward_kernel(int B, const float*
     const int ntiles = (B + 15) / 16;
     for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {  // no barrier inside
         MlpTile t;
-        mlp_tile_forward(w, x_frag(X, B, tile, lane), lane, t);
+        bf16x4 xh, xl;
+        x_frag(X, B, tile, lane, xh, xl);
+        mlp_tile_forward(w, xh, xl, lane, t);
         const int b = tile * 16 + lane;
         if (lane < 16 && b < B) {
             Y[(long)b * kNO + 0] = t.y[0];
@@ -117,8 +122,10 @@ __global__ void __launch_bounds__(kBlock) mlp_train_kernel(int B, const float* _
         const int tile = (rd * gridDim.x + blockIdx.x) * 4 + wave;
         const int nvalid = max(0, min(16, B - tile * 16));  // 0 for a tile past the end: all terms vanish
         MlpTile t;
-        mlp_tile_forward(w, x_frag(X, B, tile, lane), lane, t);
-        // X^T fragment for the W1 gradient (k = sample 4(l/16)+j, column = feature l%16)
+        bf16x4 xh, xl;
+        x_frag(X, B, tile, lane, xh, xl);
+        mlp_tile_forward(w, xh, xl, lane, t);
+        // X^T fragments (high and low halves) for the W1 gradient (k = sample 4(l/16)+j, column = feature l%16)
         float xt[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -126,6 +133,7 @@ __global__ void __launch_bounds__(kBlock) mlp_train_kernel(int B, const float* _
             xt[j] = (b < nvalid && col < kNI) ? X[(long)(tile * 16 + b) * kNI + col] : 0.0f;
         }
         const bf16x4 xtb = pack_bf16(xt[0], xt[1], xt[2], xt[3]);
+        const bf16x4 xtl = pack_bf16_lo(xt[0], xt[1], xt[2], xt[3]);
 #pragma unroll
         for (int mb = 0; mb < 4; mb++)
 #pragma unroll
@@ -182,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) mlp_train_kernel(int B, const float* _
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
             const bf16x4 ag1 = a_frag<16>(tw.G1, mb, 0, lane);
-            gW1[mb] = mfma16(ag1, xtb, gW1[mb]);
+            gW1[mb] = mfma16(ag1, xtl, mfma16(ag1, xtb, gW1[mb]));
             const bf16x4 ag2 = a_frag<16>(tw.G2, mb, 0, lane);
 #pragma unroll
             for (int nb = 0; nb < 4; nb++) gW2[mb][nb] = mfma16(ag2, a_frag<16>(tw.H1, nb, 0, lane), gW2[mb][nb]);

@@ -2,9 +2,12 @@
 //
 // Parameters are one FP32 vector (master copy, updated by Adam):
 //   W1[64][10], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]   (out x in)
-// The kernels round weights, inputs, hidden activations and back-propagated
-// gradients to bfloat16 (round to nearest even) where they enter
-// v_mfma_f32_16x16x16_bf16, and accumulate in FP32.
+// The kernels round weights, hidden activations and back-propagated gradients
+// to bfloat16 (round to nearest even) where they enter
+// v_mfma_f32_16x16x16_bf16, and accumulate in FP32.  The inputs enter as two
+// bfloat16 halves, hi = bf16(x) and lo = bf16(x - hi), in two MFMA passes
+// over the first layer: 16 significant bits of every feature, so that the
+// network can resolve small corrections of well-scaled features.
 //
 // Every product is computed "transposed", features x samples: a layer's
 // output Z^T[o][b] = sum_i W[o][i] X^T[i][b] takes W as the A operand (16 rows
@@ -33,12 +36,24 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
 }
 __device__ __forceinline__ float bf16_value(float f) { return __uint_as_float(((uint32_t)bf16_bits(f)) << 16); }
 
+// the low half x - bf16(x) of a split input, as bfloat16 bits
+__device__ __forceinline__ unsigned short bf16_lo_bits(float f) { return bf16_bits(f - bf16_value(f)); }
+
 __device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c, float d) {
     bf16x4 r;
     r[0] = (short)bf16_bits(a);
     r[1] = (short)bf16_bits(b);
     r[2] = (short)bf16_bits(c);
     r[3] = (short)bf16_bits(d);
+    return r;
+}
+
+__device__ __forceinline__ bf16x4 pack_bf16_lo(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)bf16_lo_bits(a);
+    r[1] = (short)bf16_lo_bits(b);
+    r[2] = (short)bf16_lo_bits(c);
+    r[3] = (short)bf16_lo_bits(d);
     return r;
 }
 
@@ -90,8 +105,9 @@ __device__ __forceinline__ bf16x4 a_frag(const unsigned short (*M)[COLS], int mb
     return *reinterpret_cast<const bf16x4*>(&M[mb * 16 + (lane & 15)][kb * 16 + 4 * (lane >> 4)]);
 }
 
-// Forward pass of one tile of 16 samples.  xb is the B fragment of the inputs
-// (lane l: features 4(l/16)..+3 of sample l%16, bfloat16).  Outputs: z1, z2
+// Forward pass of one tile of 16 samples.  xb, xl are the B fragments of the
+// inputs' high and low bfloat16 halves (lane l: features 4(l/16)..+3 of sample
+// l%16).  Outputs: z1, z2
 // (pre-activations, D layout per 16-row block), h1b, h2b (bfloat16
 // activations, B layout) and y (lanes 0..15 hold y[0..2] of sample l%16 in
 // y[0..2]).
@@ -101,12 +117,14 @@ struct MlpTile {
     f32x4 y;
 };
 
-__device__ __forceinline__ void mlp_tile_forward(const MlpWeights& w, bf16x4 xb, int lane, MlpTile& t) {
+__device__ __forceinline__ void mlp_tile_forward(const MlpWeights& w, bf16x4 xb, bf16x4 xl, int lane, MlpTile& t) {
     const int r0 = 4 * (lane >> 4);
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         f32x4 c = {w.b1[mb * 16 + r0], w.b1[mb * 16 + r0 + 1], w.b1[mb * 16 + r0 + 2], w.b1[mb * 16 + r0 + 3]};
-        c = mfma16(a_frag<16>(w.W1, mb, 0, lane), xb, c);
+        const bf16x4 a = a_frag<16>(w.W1, mb, 0, lane);
+        c = mfma16(a, xb, c);
+        c = mfma16(a, xl, c);
         t.z1[mb] = c;
         t.h1b[mb] = pack_bf16(gelu_f(c[0]), gelu_f(c[1]), gelu_f(c[2]), gelu_f(c[3]));
     }

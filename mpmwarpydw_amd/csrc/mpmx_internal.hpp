@@ -101,6 +101,12 @@ int counting_sort_segments(const SortSegs& t, int nseg, long n, long nkeys, hipS
 // dst[0..n) = value on stream s (a kernel: safe inside a captured graph)
 int fill_i32(int32_t* dst, long n, int32_t value, hipStream_t s);
 
+// the constitutive update of every active MPM particle with the learned
+// projection judged by the exact map (params NULL: exact projection for all);
+// counters[3] (may be NULL) += {plastic states, accepted predictions, updated}
+int launch_stress_learned(const mpmx_particles& P, float eff, const float* params, float tol, int32_t* counters,
+                          hipStream_t s);
+
 }  // namespace mpmx
 
 // every kernel launch of the library goes through this macro: with the

@@ -551,8 +551,16 @@ static int reorder_geo(const mpmx_particles* P, const Geo& geo, float inv_dx, Su
     return counting_sort_ids(w.keys, P->n, reorder_nkeys(geo, P->n), perm, w.bin, s, P->order);
 }
 
+// the learned projection of the constitutive update (mpmx_*_substep_learned)
+struct LearnedStress {
+    const float* params;
+    float tol;
+    int32_t* counters;
+};
+
 static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in, float* grid_v_out,
-                       const mpmx_step_params* prm, const Geo& geo, SubstepWork& w, hipStream_t s) {
+                       const mpmx_step_params* prm, const Geo& geo, SubstepWork& w, hipStream_t s,
+                       const LearnedStress* ls = nullptr) {
     int rc;
     const long nkeys = substep_nkeys(geo);
     const unsigned ntile_blocks = (unsigned)geo.ntiles() + 1;  // + orphan bin
@@ -572,8 +580,10 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ta.geo = geo;
     ta.xcd = tuning(MPMX_TUNE_XCD_TILES);
     ta.misbinned = w.misbinned;
-    if (tuning(MPMX_TUNE_SPLIT_STRESS)) {
-        if (P->n > 0) {
+    if (ls || tuning(MPMX_TUNE_SPLIT_STRESS)) {  // the learned update runs as its own kernel
+        if (ls) {
+            if ((rc = launch_stress_learned(*P, prm->eff, ls->params, ls->tol, ls->counters, s))) return rc;
+        } else if (P->n > 0) {
             MPMX_LAUNCH(substep_stress_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->eff);
             if ((rc = check_launch("substep_stress", s))) return rc;
         }
@@ -714,6 +724,20 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
     return substep_geo(P, G->m, G->v_in, G->v_out, prm, geo, w, as_stream(stream));
 }
 
+int mpmx_mpm_substep_learned(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm,
+                             const float* mlp_params, float tol, int32_t* counters, void* workspace, size_t ws_bytes,
+                             void* stream) {
+    int rc = check_args(P, G, prm);
+    if (rc) return rc;
+    MPMX_REQUIRE(tol >= 0.0f, "mpmx_mpm_substep_learned: negative tolerance");
+    Geo geo = make_geom(G->gx, G->gy, G->gz, prm->minx, prm->miny, prm->minz);
+    MPMX_REQUIRE(workspace && ws_bytes >= substep_ws_bytes(P->n, geo), "substep: workspace too small");
+    SubstepWork w;
+    MPMX_REQUIRE(carve_substep(workspace, ws_bytes, P->n, geo, w), "substep: workspace carve failed");
+    const LearnedStress ls{mlp_params, tol, counters};
+    return substep_geo(P, G->m, G->v_in, G->v_out, prm, geo, w, as_stream(stream), &ls);
+}
+
 size_t mpmx_ensemble_workspace_bytes(int n, const mpmx_scene* scenes, int nscenes) {
     Geo geo;
     if (n < 0 || ensemble_geo(scenes, nscenes, geo)) return 0;
@@ -767,6 +791,21 @@ int mpmx_ensemble_substep(const mpmx_particles* P, const mpmx_scene* scenes, int
     int rc = ensemble_setup(P, scenes, nscenes, prm, workspace, ws_bytes, s, geo, w);
     if (rc) return rc;
     return substep_geo(P, grid_m, grid_v_in, grid_v_out, prm, geo, w, s);
+}
+
+int mpmx_ensemble_substep_learned(const mpmx_particles* P, const mpmx_scene* scenes, int nscenes, float* grid_m,
+                                  float* grid_v_in, float* grid_v_out, const mpmx_step_params* prm,
+                                  const float* mlp_params, float tol, int32_t* counters, void* workspace,
+                                  size_t ws_bytes, void* stream) {
+    MPMX_REQUIRE(grid_m && grid_v_in && grid_v_out, "ensemble: null grid array");
+    MPMX_REQUIRE(tol >= 0.0f, "mpmx_ensemble_substep_learned: negative tolerance");
+    Geo geo;
+    SubstepWork w;
+    hipStream_t s = as_stream(stream);
+    int rc = ensemble_setup(P, scenes, nscenes, prm, workspace, ws_bytes, s, geo, w);
+    if (rc) return rc;
+    const LearnedStress ls{mlp_params, tol, counters};
+    return substep_geo(P, grid_m, grid_v_in, grid_v_out, prm, geo, w, s, &ls);
 }
 
 }  // extern "C"

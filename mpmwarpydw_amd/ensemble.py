@@ -108,14 +108,30 @@ class Ensemble:
         lib("mpmx_ensemble_bin", C.byref(P), self.table, len(self.table), C.byref(prm), ws, nb, rt.stream_handle())
         self._binned = True
 
-    def substep(self) -> None:
-        """One fused MPM substep of every scene (one launch per kernel)."""
+    def substep(self, learned=None, tol: float = 1e-4, counters: torch.Tensor | None = None,
+                use_net: bool = True) -> None:
+        """One fused MPM substep of every scene (one launch per kernel).  With
+        ``learned`` (a learned.LearnedReturnMap) the constitutive update uses
+        its projection, judged by the exact map at ``tol`` (``use_net`` False:
+        the same kernel with the exact projection for every state);
+        ``counters`` (int32 device tensor of 3, optional) accumulates
+        {plastic, accepted, updated}."""
         ws, nb = self._workspace()
         if not self._binned:
             self.bin()
         P, prm = self._structs()
-        lib("mpmx_ensemble_substep", C.byref(P), self.table, len(self.table), self.grid_m.ptr, self.grid_v_in.ptr,
-            self.grid_v_out.ptr, C.byref(prm), ws, nb, rt.stream_handle())
+        if learned is None:
+            lib("mpmx_ensemble_substep", C.byref(P), self.table, len(self.table), self.grid_m.ptr,
+                self.grid_v_in.ptr, self.grid_v_out.ptr, C.byref(prm), ws, nb, rt.stream_handle())
+            return
+        cp = None
+        if counters is not None:
+            if counters.dtype != torch.int32 or counters.numel() < 3 or not counters.is_cuda:
+                raise ValueError("counters: an int32 device tensor of 3 elements")
+            cp = C.c_void_p(counters.data_ptr())
+        lib("mpmx_ensemble_substep_learned", C.byref(P), self.table, len(self.table), self.grid_m.ptr,
+            self.grid_v_in.ptr, self.grid_v_out.ptr, C.byref(prm),
+            C.c_void_p(learned.params.data_ptr()) if use_net else None, float(tol), cp, ws, nb, rt.stream_handle())
 
     def reorder(self) -> None:
         """Permute every per-particle array into tile order (as driver.Simulation.reorder)."""

@@ -20,6 +20,11 @@ from ._lib import lib
 from .simulationRoutines import step_structs
 
 NI, NH, NO = 10, 64, 3
+# features that the exact targets do not depend on (4 nu - 1, log10 E - 8,
+# damage) and the ranges they are redrawn from in training
+INVARIANT_COLS = [7, 8, 9]
+INVARIANT_LO = [-0.6, -2.0, 0.0]
+INVARIANT_HI = [0.6, 2.0, 1.0]
 LAYERS = ((NH, NI), (NH, NH), (NO, NH))  # (out, in)
 
 
@@ -78,16 +83,31 @@ class LearnedReturnMap:
             float(beta1), float(beta2), float(eps), self.t, rt.stream_handle())
 
     def train(self, X: torch.Tensor, T: torch.Tensor, steps: int, batch: int = 4096, lr: float = 1e-3,
-              lr_final: float | None = None, seed: int = 0, log_every: int = 0) -> list[float]:
+              lr_final: float | None = None, seed: int = 0, log_every: int = 0,
+              redraw_invariant: bool = True) -> list[float]:
         """Adam on random minibatches of (X, T) (device tensors); returns the
-        loss of every ``log_every``-th step (one synchronisation each)."""
+        loss of every ``log_every``-th step (one synchronisation each).
+
+        The exact targets (s, alpha) are functions of the overshoot, yield
+        ratio, kap and friction features alone: Poisson's ratio, E and damage
+        enter them only through those normalised features.  With
+        ``redraw_invariant`` those three columns are redrawn in every minibatch
+        (4 nu - 1 in [-0.6, 0.6], log10 E - 8 in [-2, 2], damage in [0, 1]), so
+        that the network learns that they do not matter instead of meeting
+        combinations the training runs never produced."""
         g = torch.Generator(device=X.device)
         g.manual_seed(seed)
         n = X.shape[0]
         hist = []
+        lo = torch.tensor(INVARIANT_LO, device=X.device)
+        span = torch.tensor(INVARIANT_HI, device=X.device) - lo
         for k in range(steps):
             idx = torch.randint(0, n, (min(batch, n),), device=X.device, generator=g)
-            self.loss_grad(X[idx], T[idx])
+            xb = X[idx]
+            if redraw_invariant:
+                xb[:, INVARIANT_COLS] = lo + span * torch.rand((xb.shape[0], len(INVARIANT_COLS)), device=X.device,
+                                                              generator=g)
+            self.loss_grad(xb, T[idx])
             frac = k / max(1, steps - 1)
             cur = lr if lr_final is None else lr * (lr_final / lr) ** frac
             self.adam(cur)
@@ -132,18 +152,27 @@ class LearnedReturnMap:
         return obj
 
 
-def harvest_run(sim_obj, substeps: int, start: int = 1, every: int = 1, max_pairs: int | None = None):
+def harvest_run(sim_obj, substeps: int, start: int = 1, every: int = 1, max_pairs: int | None = None,
+                sample: int | None = None, seed: int = 0):
     """Advance a driver.Simulation ``substeps`` substeps (the reference's loop
     order, XPBD steps included) and collect the plastic trial states the return
-    map meets after each ``every``-th substep: (X, T) device tensors."""
+    map meets after each ``every``-th substep (at most ``sample`` of them per
+    substep, drawn at random): (X, T) device tensors."""
     xs, ts = [], []
     total = 0
+    g = None
+    if sample is not None:
+        g = torch.Generator(device=rt.device())
+        g.manual_seed(seed)
     for k in range(substeps):
         counter = start + k
         sim_obj.substep(counter)
         if k % every:
             continue
         X, T, _ = LearnedReturnMap.harvest(sim_obj.sim, sim_obj.mpm, sim_obj.xpbd)
+        if sample is not None and X.shape[0] > sample:
+            keep = torch.randperm(X.shape[0], device=X.device, generator=g)[:sample]
+            X, T = X[keep], T[keep]
         if X.shape[0]:
             xs.append(X)
             ts.append(T)
@@ -154,3 +183,51 @@ def harvest_run(sim_obj, substeps: int, start: int = 1, every: int = 1, max_pair
     if not xs:
         return torch.zeros((0, NI), device=dev), torch.zeros((0, NO), device=dev)
     return torch.cat(xs), torch.cat(ts)
+
+
+def draw_block_material(rng: np.random.Generator) -> tuple[float, float, float]:
+    """(E, cohesion, friction angle in degrees) of a block-collapse run:
+    E log-uniform in [1e7, 1e9] Pa, cohesion log-uniform in [1e4, 1e6] Pa,
+    friction angle U(20, 40) degrees (the batched-ensemble ranges)."""
+    E = float(np.exp(rng.uniform(np.log(1e7), np.log(1e9))))
+    c = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+    phi = float(rng.uniform(20.0, 40.0))
+    return E, c, phi
+
+
+def harvest_blocks(target_pairs: int, max_runs: int = 64, substeps: int = 500, side_particles: int = 32768,
+                   seed: int = 0, per_run: int | None = None, every: int = 1, log=None):
+    """Training pairs from block-collapse runs (a cube of ``side_particles``
+    particles on the 0.25 m lattice, dt 1e-4 s, ``substeps`` substeps with the
+    XPBD steps at their ratio), one material per run drawn by
+    draw_block_material from default_rng(seed), the plastic trial states of
+    every ``every``-th substep kept, at most ``per_run`` from one run (spread
+    evenly over its substeps, drawn at random within each): runs are added
+    until ``target_pairs`` pairs are collected or ``max_runs`` ran.
+    Returns (X, T, stats) with X, T device tensors."""
+    from . import driver, getArgs
+    from .ensemble import block_scene
+    rng = np.random.default_rng(seed)
+    xs, ts, runs = [], [], []
+    total = 0
+    while total < target_pairs and len(runs) < max_runs:
+        E, c, phi = draw_block_material(rng)
+        dom, ov = block_scene(side_particles, E, c, phi, seed=seed * 1000 + len(runs))
+        S = driver.setup(getArgs.get_args([], ov), dom)
+        cap = target_pairs - total if per_run is None else min(per_run, target_pairs - total)
+        sample = None if per_run is None else max(1, per_run // max(1, substeps // every))
+        X, T = harvest_run(S, substeps=substeps, start=1, every=every, max_pairs=cap, sample=sample,
+                           seed=seed * 1000 + len(runs))
+        X, T = X[:cap], T[:cap]
+        runs.append({"E": E, "cohesion": c, "friction_deg": phi, "pairs": int(X.shape[0])})
+        if log:
+            log(f"run {len(runs)}: E {E:.3g} cohesion {c:.3g} friction {phi:.1f} deg -> {X.shape[0]} pairs")
+        if X.shape[0]:
+            xs.append(X)
+            ts.append(T)
+            total += X.shape[0]
+        del S
+    dev = rt.device()
+    X = torch.cat(xs) if xs else torch.zeros((0, NI), device=dev)
+    T = torch.cat(ts) if ts else torch.zeros((0, NO), device=dev)
+    return X[:target_pairs], T[:target_pairs], {"runs": runs, "pairs": int(min(total, target_pairs))}

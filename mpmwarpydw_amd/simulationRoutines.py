@@ -193,19 +193,27 @@ def substep_workspace(sim, mpm):
     return ws, nb
 
 
-def mpmSimulationStepFused(sim, mpm, xpbd, device=None):
+def mpmSimulationStepFused(sim, mpm, xpbd, device=None, *, learned=None, tol=1e-4, counters=None):
     """mpmSimulationStep (simulationRoutines.py:6-293) as libmpmx's fused substep:
     a tile pass (constitutive update + deterministic P2G), a node pass (grid
     update) and a tile pass (G2P, which also bins the particles for the next
     substep), then the volumetric locking correction when it is on.  The bins
     are rebuilt first whenever x, the labels or the activity were written from
-    the host since the last substep."""
+    the host since the last substep.  With ``learned`` (a
+    learned.LearnedReturnMap) the constitutive update uses the network's
+    projection, judged by the exact map at ``tol``; ``counters`` (int32 device
+    tensor of 3) accumulates {plastic, accepted, updated}."""
     P, G, prm = step_structs(sim, mpm, xpbd)
     ws, nb = substep_workspace(sim, mpm)
     key = _bin_key(sim, mpm)
     if mpm._bins != key:
         lib("mpmx_substep_bin", C.byref(P), C.byref(G), C.byref(prm), ws, nb, stream_handle())
-    lib("mpmx_mpm_substep", C.byref(P), C.byref(G), C.byref(prm), ws, nb, stream_handle())
+    if learned is None:
+        lib("mpmx_mpm_substep", C.byref(P), C.byref(G), C.byref(prm), ws, nb, stream_handle())
+    else:
+        cp = None if counters is None else C.c_void_p(counters.data_ptr())
+        lib("mpmx_mpm_substep_learned", C.byref(P), C.byref(G), C.byref(prm), C.c_void_p(learned.params.data_ptr()),
+            float(tol), cp, ws, nb, stream_handle())
     mpm._bins = key
 
 

@@ -316,6 +316,8 @@ class MLPOracle:
         self.lib = C.CDLL(path)
         self.lib.o_bf16.restype = C.c_double
         self.lib.o_bf16.argtypes = [C.c_double]
+        self.lib.o_bf16_split.restype = C.c_double
+        self.lib.o_bf16_split.argtypes = [C.c_double]
         self.lib.o_gelu.restype = C.c_double
         self.lib.o_gelu.argtypes = [C.c_double]
         self.lib.o_gelu_grad.restype = C.c_double
@@ -329,6 +331,10 @@ class MLPOracle:
 
     def bf16(self, v):
         return self.lib.o_bf16(float(v))
+
+    def bf16_split(self, v):
+        """An input as the kernels feed it: bf16(v) + bf16(v - bf16(v))."""
+        return self.lib.o_bf16_split(float(v))
 
     def gelu(self, v):
         return self.lib.o_gelu(float(v))
@@ -381,6 +387,14 @@ class MLPOracle:
         self.lib.o_lrm_targets(_I(n), *[_ptr(self._d(a)) for a in (e, mu, lam, ys, alpha, damage)], _ptr(T),
                                _ptr(plastic))
         return T, plastic.astype(bool)
+
+    def apply(self, e, Y, mu, lam, ys, alpha, damage):
+        """The predicted projection e0' + eref Y / 10 (radial return plus the
+        network's correction) of trial strains e from network outputs Y."""
+        n = len(mu)
+        out = np.zeros((n, 3))
+        self.lib.o_lrm_apply(_I(n), *[_ptr(self._d(a)) for a in (e, mu, lam, ys, alpha, damage, Y)], _ptr(out))
+        return out
 
     def judge(self, e, e_pred, mu, lam, ys, alpha, damage, tol=1e-4):
         n = len(mu)

@@ -7,10 +7,12 @@
  * W3[3][64], b3[3] (row-major, out x in).
  *
  * With quant = 1 the oracle rounds to bfloat16 (round to nearest even) at the
- * points where the GPU kernels feed their MFMA units: the inputs, the weights,
- * the hidden activations and, in the backward pass, the back-propagated
- * gradients; sums and everything else stay in REAL.  The loss is the mean
- * squared error over all B*3 outputs.
+ * points where the GPU kernels feed their MFMA units: the weights, the hidden
+ * activations and, in the backward pass, the back-propagated gradients; the
+ * inputs enter as the sum of two bfloat16 values, hi = bf16(x) and
+ * lo = bf16(x - hi) (two MFMA passes), which keeps 16 bits of each feature.
+ * Sums and everything else stay in REAL.  The loss is the mean squared error
+ * over all B*3 outputs.
  */
 #include <math.h>
 #include <stdint.h>
@@ -45,6 +47,13 @@ static R bf16(R v) {
 EXPORT double o_bf16(double v) { return (double)bf16((R)v); }
 
 static R q(R v, int quant) { return quant ? bf16(v) : v; }
+/* an input as the GPU feeds it: bf16(x) + bf16(x - bf16(x)) */
+static R qin(R v, int quant) {
+    if (!quant) return v;
+    const R hi = bf16(v);
+    return hi + bf16(v - hi);
+}
+EXPORT double o_bf16_split(double v) { return (double)qin((R)v, 1); }
 
 static R gelu(R x) {
     const R c = (R)0.7978845608028654; /* sqrt(2/pi) */
@@ -78,7 +87,7 @@ static struct Params split(const R *p) {
 /* forward of one sample; z1, z2 pre-activations, h1, h2 activations (as fed on) */
 static void forward1(const struct Params *P, const R *x, R *z1, R *h1, R *z2, R *h2, R *y, int quant) {
     R xq[NI];
-    for (int i = 0; i < NI; i++) xq[i] = q(x[i], quant);
+    for (int i = 0; i < NI; i++) xq[i] = qin(x[i], quant);
     for (int o = 0; o < NH; o++) {
         R s = 0;
         for (int i = 0; i < NI; i++) s += q(P->W1[o * NI + i], quant) * xq[i];
@@ -147,7 +156,7 @@ EXPORT double o_mlp_loss_grad(int B, const R *params, const R *X, const R *T, R 
         }
         for (int o = 0; o < NH; o++) {
             R g1q = q(g1[o], quant);
-            for (int i = 0; i < NI; i++) gW1[o * NI + i] += g1q * q(x[i], quant);
+            for (int i = 0; i < NI; i++) gW1[o * NI + i] += g1q * qin(x[i], quant);
         }
     }
     return loss / (double)((long)B * NO);
@@ -175,14 +184,22 @@ EXPORT void o_adam_step(int n, R *params, const R *grad, R *m, R *v, R lr, R bet
 /* is e' = e - (dgam / edn) d + 0.3 alpha dgam with dgam = edn - Y / (2 mu)  */
 /* (mpmRoutines.py:360-413).                                                 */
 /*                                                                          */
-/* Features (10): e / eref (3), log10 E - 8, 4 nu - 1, log10 cohesion - 5,   */
-/* tan(friction angle), tan(dilation angle), 20 acc, damage, where           */
-/* eref = max(ys, 1) / (2 mu) is the particle's yield strain, E and nu come  */
-/* from mu and lam, the friction angle from alpha through                    */
-/* alpha = 2 sin(phi) / (sqrt(3) (3 - sin(phi))) and the dilation angle from */
-/* the reference's dilatancy 0.3 alpha through the same relation.  The       */
-/* target is (e' - e) / eref.                                                */
-/* ------------------------------------------------------------------------ */
+/* With c = max(ys, 1), eref = c / (2 mu) (the yield strain), the overshoot  */
+/* phi = (2 mu |d| - Y) / c, y0 = Y / c and kap = 2 mu 1e-6 / c, the exact   */
+/* projection is the classical radial return e0' = e - (1 - Y/(2 mu |d|)) d  */
+/* (von Mises' projection onto the yield surface at the trial pressure)      */
+/* plus two corrections, in units of eref:                                   */
+/*   e' = e0' - eref kap s d/|d| + 0.3 alpha eref (phi + kap) (1, 1, 1),     */
+/*   s = y0 / (phi + y0 + kap)   (the part of the deviator that the 1e-6 in  */
+/*                                edn keeps: s = Y / (2 mu edn)),            */
+/*   alpha                       (the Drucker-Prager cone coefficient, which */
+/*                                sets the dilatancy).                       */
+/* The network predicts y = (s, alpha, 0) from 10 features: 0.1 ln(phi+kap), */
+/* 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d/|d| (3), tan(friction angle),       */
+/* 4 nu - 1, log10 E - 8, damage.  The tolerance of the judge then asks for  */
+/* kap |ds| and 0.9 (phi + kap) |dalpha| (2 mu / K) below tol: loose where   */
+/* the overshoot is small, as it is within one substep.  States with Y < 0   */
+/* (tension past the cone's apex) are left to the exact projection.          */
 static R eref_of(R mu, R ys) { return (ys > (R)1 ? ys : (R)1) / ((R)2 * mu); }
 
 static R tan_of_alpha(R a) {
@@ -193,52 +210,98 @@ static R tan_of_alpha(R a) {
     return s / (R)sqrt((double)((R)1 - s * s));
 }
 
+struct Trial {
+    R d[3], dn, sum, Y;
+};
+static struct Trial trial(const R *e, R mu, R lam, R ys, R alpha, R D) {
+    struct Trial t;
+    R mean = (e[0] + e[1] + e[2]) / (R)3;
+    for (int k = 0; k < 3; k++) t.d[k] = e[k] - mean;
+    t.sum = e[0] + e[1] + e[2];
+    R p = ((R)2 * mu * e[0] + lam * t.sum + (R)2 * mu * e[1] + lam * t.sum + (R)2 * mu * e[2] + lam * t.sum) / (R)3;
+    t.dn = (R)sqrt((double)(t.d[0] * t.d[0] + t.d[1] * t.d[1] + t.d[2] * t.d[2]));
+    t.Y = ((R)1 - D) * (ys - alpha * p);
+    return t;
+}
+
 /* exact principal projection; returns 1 when the trial state is plastic */
 static int exact_principal(const R *e, R mu, R lam, R ys, R alpha, R D, R *e_new, R *dgam_out) {
-    R mean = (e[0] + e[1] + e[2]) / (R)3;
-    R d[3] = {e[0] - mean, e[1] - mean, e[2] - mean};
-    R sum = e[0] + e[1] + e[2];
-    R p = ((R)2 * mu * e[0] + lam * sum + (R)2 * mu * e[1] + lam * sum + (R)2 * mu * e[2] + lam * sum) / (R)3;
-    R dn = (R)sqrt((double)(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
-    R Y = ((R)1 - D) * (ys - alpha * p);
+    struct Trial t = trial(e, mu, lam, ys, alpha, D);
     for (int k = 0; k < 3; k++) e_new[k] = e[k];
     *dgam_out = 0;
-    if (!((R)2 * mu * dn > Y || ys < 0 || Y < 0)) return 0;
-    R edn = dn + (R)1e-6;
-    R dgam = edn - Y / ((R)2 * mu);
+    if (!((R)2 * mu * t.dn > t.Y || ys < 0 || t.Y < 0)) return 0;
+    R edn = t.dn + (R)1e-6;
+    R dgam = edn - t.Y / ((R)2 * mu);
     R beta = (R)0.3 * alpha;
-    for (int k = 0; k < 3; k++) e_new[k] = e[k] - (dgam / edn) * d[k] + beta * dgam;
+    for (int k = 0; k < 3; k++) e_new[k] = e[k] - (dgam / edn) * t.d[k] + beta * dgam;
     *dgam_out = dgam;
     return 1;
 }
 
+/* the radial return e0' = e - r0 d, r0 = 1 - Y / (2 mu |d|) (0 for |d| = 0) */
+static void radial_return(const R *e, const struct Trial *t, R mu, R *e0) {
+    R r0 = t->dn > 0 ? (R)1 - t->Y / ((R)2 * mu * t->dn) : (R)0;
+    for (int k = 0; k < 3; k++) e0[k] = e[k] - r0 * t->d[k];
+}
+
 EXPORT void o_lrm_features(int n, const R *e, const R *mu, const R *lam, const R *ys, const R *alpha,
                            const R *acc, const R *damage, R *X) {
+    (void)acc;
     for (int p = 0; p < n; p++) {
         R m = mu[p], l = lam[p];
         R E = m * ((R)3 * l + (R)2 * m) / (l + m);
         R nu = l / ((R)2 * (l + m));
-        R er = eref_of(m, ys[p]);
+        R c = ys[p] > (R)1 ? ys[p] : (R)1;
+        struct Trial t = trial(e + 3L * p, m, l, ys[p], alpha[p], damage[p]);
+        R phi = ((R)2 * m * t.dn - t.Y) / c, y0 = t.Y / c, kap = (R)2 * m * (R)1e-6 / c;
         R *x = X + (long)p * NI;
-        for (int k = 0; k < 3; k++) x[k] = e[3L * p + k] / er;
-        x[3] = (R)log10((double)E) - (R)8;
-        x[4] = (R)4 * nu - (R)1;
-        x[5] = (R)log10((double)(ys[p] > (R)1 ? ys[p] : (R)1)) - (R)5;
+        R pk = phi + kap;
+        x[0] = (R)0.1 * (R)log((double)(pk > (R)1e-30 ? pk : (R)1e-30));
+        x[1] = (R)0.1 * (R)log((double)(y0 > (R)1e-12 ? y0 : (R)1e-12));
+        x[2] = (R)0.1 * (R)log((double)kap);
+        for (int k = 0; k < 3; k++) x[3 + k] = t.dn > 0 ? t.d[k] / t.dn : (R)0;
         x[6] = tan_of_alpha(alpha[p]);
-        x[7] = tan_of_alpha((R)0.3 * alpha[p]);
-        x[8] = (R)20 * acc[p];
+        x[7] = (R)4 * nu - (R)1;
+        x[8] = (R)log10((double)E) - (R)8;
         x[9] = damage[p];
     }
 }
 
-/* exact targets (e' - e) / eref and the plastic flag */
+/* exact targets (s, alpha, 0) of plastic states (zeros for elastic ones) and
+ * the plastic flag */
 EXPORT void o_lrm_targets(int n, const R *e, const R *mu, const R *lam, const R *ys, const R *alpha,
                           const R *damage, R *T, int *plastic) {
     for (int p = 0; p < n; p++) {
         R en[3], dg;
-        plastic[p] = exact_principal(e + 3L * p, mu[p], lam[p], ys[p], alpha[p], damage[p], en, &dg);
+        const R *ep = e + 3L * p;
+        plastic[p] = exact_principal(ep, mu[p], lam[p], ys[p], alpha[p], damage[p], en, &dg);
+        struct Trial t = trial(ep, mu[p], lam[p], ys[p], alpha[p], damage[p]);
+        R *tp = T + 3L * p;
+        tp[0] = tp[1] = tp[2] = 0;
+        if (plastic[p]) {
+            tp[0] = t.Y / ((R)2 * mu[p] * (t.dn + (R)1e-6));
+            tp[1] = alpha[p];
+        }
+    }
+}
+
+/* the prediction e0' - eref kap y0 d/|d| + 0.3 y1 eref (phi + kap) from
+ * network outputs y */
+EXPORT void o_lrm_apply(int n, const R *e, const R *mu, const R *lam, const R *ys, const R *alpha,
+                        const R *damage, const R *Y, R *e_pred) {
+    for (int p = 0; p < n; p++) {
+        const R *ep = e + 3L * p;
+        struct Trial t = trial(ep, mu[p], lam[p], ys[p], alpha[p], damage[p]);
+        R e0[3];
+        radial_return(ep, &t, mu[p], e0);
         R er = eref_of(mu[p], ys[p]);
-        for (int k = 0; k < 3; k++) T[3L * p + k] = (en[k] - e[3L * p + k]) / er;
+        R c = ys[p] > (R)1 ? ys[p] : (R)1;
+        R kap = (R)2 * mu[p] * (R)1e-6 / c, phi = ((R)2 * mu[p] * t.dn - t.Y) / c;
+        const R *y = Y + 3L * p;
+        for (int k = 0; k < 3; k++) {
+            R dh = t.dn > 0 ? t.d[k] / t.dn : (R)0;
+            e_pred[3L * p + k] = e0[k] - er * kap * y[0] * dh + (R)0.3 * y[1] * er * (phi + kap);
+        }
     }
 }
 

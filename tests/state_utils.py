@@ -335,3 +335,41 @@ def oracle_transfers32(o32, Hin, G, S):
     oracle_g2p(o32, H, S)
     oracle_vlc(o32, H, S)
     return {k: v.astype(np.float64) if v.dtype == np.float32 else v for k, v in H.items()}, sc
+
+
+def c3_state(scale=1.61, seed=3, plastic_frac=0.5):
+    """The learned-map workload: an undercut slab of about 4.0M particles whose
+    materials follow the slab's distribution (E lognormal, median 5e8 Pa,
+    sigma_ln 0.3; cohesion lognormal, median 2e5 Pa, sigma_ln 0.4; drawn from
+    default_rng(seed)), with trial deformation gradients placed around the
+    yield surface: compressive volumetric strain up to two cohesions of
+    pressure, damage 0 (80%) or U(0, 0.9), and a deviator that is inside the
+    cone (the elastic share) or past it by an overshoot log-uniform in
+    [1e-5, 0.3] cohesions (``plastic_frac``), in random principal frames."""
+    from mpmwarpydw_amd import driver, getArgs, scenes
+    dom, ov = scenes.undercut_slab(scale=scale, seed=seed)
+    S = driver.setup(getArgs.get_args([], ov), dom)
+    n = S.sim.nPoints
+    rng = np.random.default_rng(seed + 100)
+    mu = S.mpm.mu.numpy().astype(np.float64)
+    lam = S.mpm.lam.numpy().astype(np.float64)
+    ys = S.mpm.ys.numpy().astype(np.float64)
+    al = S.mpm.alpha.numpy().astype(np.float64)
+    K = lam + 2 * mu / 3
+    D = np.where(rng.random(n) < 0.8, 0.0, rng.uniform(0, 0.9, n))
+    tr = -rng.uniform(0, 2, n) * ys / K
+    Y = (1 - D) * (ys - al * K * tr)
+    c = np.maximum(ys, 1.0)
+    plastic = rng.random(n) < plastic_frac
+    over = 10 ** rng.uniform(-5, np.log10(0.3), n)
+    dmag = np.where(plastic, Y + c * over, Y * rng.uniform(0, 0.99, n)) / (2 * mu)
+    w = rng.standard_normal((n, 3))
+    w -= w.mean(1, keepdims=True)
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    e = tr[:, None] / 3 + dmag[:, None] * w
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, 3, 3)))
+    Ft = np.einsum("nij,nj,nkj->nik", Q1, np.exp(e), Q2)
+    S.mpm.particle_F_trial.assign(Ft)
+    S.mpm.particle_damage.assign(D)
+    return S

@@ -93,19 +93,75 @@ def test_harvest_matches_oracle(mo):
     args = [H[k][sel] for k in ("mu", "lam", "ys", "alpha")]
     Xo = mo.features(E, *args, H["particle_accumulated_strain"][sel], H["particle_damage"][sel])
     To, plastic = mo.targets(E, *args, H["particle_damage"][sel])
-    assert np.abs(X - Xo).max() < 1e-4 * np.maximum(1.0, np.abs(Xo)).max()
-    assert np.abs(T - To).max() < 1e-4 * np.maximum(1.0, np.abs(To).max())
+    # The log features are compared in linear space, each against the rounding
+    # of the FP32 sums it comes from: the overshoot phi + kap = (2 mu |d| - Y)/c
+    # + kap cancels, and so does Y = (1 - D)(ys - alpha p) near the cone's apex.
+    mu, lam, ys, al = args
+    D = H["particle_damage"][sel]
+    c = np.maximum(ys, 1.0)
+    d = E - E.mean(1, keepdims=True)
+    dn = np.linalg.norm(d, axis=1)
+    pr = (2 * mu / 3 + lam) * E.sum(1)
+    y_scale = (1 - D) * (np.abs(ys) + al * np.abs(pr)) / c  # magnitude of the terms of y0
+    pk_scale = 2 * mu * dn / c + y_scale + 2 * mu * 1e-6 / c
+    pl = plastic.astype(bool)
+    assert np.all(np.abs(np.exp(10 * X[pl, 0]) - np.exp(10 * Xo[pl, 0])) <= 2e-6 * pk_scale[pl])
+    assert np.all(np.abs(np.exp(10 * X[:, 1]) - np.exp(10 * Xo[:, 1])) <= 2e-6 * y_scale + 1e-12)
+    assert np.allclose(np.exp(10 * X[:, 2]), np.exp(10 * Xo[:, 2]), rtol=2e-6)
+    assert np.abs(X[:, 3:] - Xo[:, 3:]).max() < 1e-5 * np.maximum(1.0, np.abs(Xo[:, 3:])).max()
+    # targets: s = Y / (2 mu edn) to the rounding of Y, alpha exactly
+    assert np.all(np.abs(T[:, 0] - To[:, 0]) <= 2e-6 * y_scale * c / (2 * mu * (dn + 1e-6)) + 1e-6)
+    assert np.array_equal(T[:, 1], To[:, 1]) and np.all(T[:, 2] == 0)
     assert plastic.sum() > 100 and (~plastic).sum() > 5
-    assert np.array_equal(np.any(T != 0, axis=1), plastic)
+    assert np.array_equal(T[:, 1] != 0, pl)
+
+
+def _run_stress(S, net, use_net, tol=1e-4):
+    """One learned constitutive update on S's current trial state; the state is
+    restored afterwards.  Returns (updated arrays, counters, seconds)."""
+    keys = ("particle_F", "particle_stress", "particle_accumulated_strain", "particle_damage", "ys", "mu", "lam",
+            "alpha")
+    state = {k: getattr(S.mpm, k).t.clone() for k in keys}
+    lab, v = S.sim.materialLabel.t.clone(), S.sim.particle_v.t.clone()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    c = net.stress_update(S.sim, S.mpm, S.xpbd, tol=tol, use_net=use_net)
+    t1.record()
+    torch.cuda.synchronize()
+    A = {k: getattr(S.mpm, k).t.clone() for k in keys}
+    for k in keys:
+        getattr(S.mpm, k).t.copy_(state[k])
+    A["label"] = S.sim.materialLabel.t.clone()
+    S.sim.materialLabel.t.copy_(lab)
+    S.sim.particle_v.t.copy_(v)
+    return A, c.cpu().numpy(), t0.elapsed_time(t1) * 1e-3
+
+
+def _check_against_exact(A, ex):
+    """Accepted predictions are within the judge's bound of the exact update;
+    everything the exact map computes anyway (plastic strain, damage,
+    hardening, phase flags) is identical."""
+    assert torch.equal(A["label"], ex["label"])
+    for k in ("particle_accumulated_strain", "particle_damage", "ys"):
+        assert torch.equal(A[k], ex[k]), k
+    ys = np.maximum(ex["ys"].cpu().numpy(), 1.0)
+    d = (A["particle_stress"] - ex["particle_stress"]).abs().amax(0).cpu().numpy()  # SoA: [9, n]
+    scale = ex["particle_stress"].abs().amax(0).cpu().numpy()
+    # per particle: the judge's tol * cohesion on the principal stresses, a few
+    # more for the rotation back to the grid frame, plus FP32 rounding
+    assert np.all(d <= 3e-4 * ys + 1e-6 * scale), np.max((d - 1e-6 * scale) / ys)
+    return float(np.max(d / ys))
 
 
 def test_learned_stress_update_gain():
     """Train on trial states harvested from a block-collapse run, then run the
     constitutive update three ways on the same held-out state: exact projection,
-    untrained network and trained network, each judged by the exact map.  The
-    judged results stay within the judge's bound of the exact update; what the
-    trained network gains over the untrained one is measured (prediction error
-    and accepted fraction)."""
+    untrained network and trained network, each judged by the exact map.  What
+    the trained network gains: held-out prediction error, accepted fraction at
+    the 1e-4 tolerance, and the stress error against the exact update; the
+    update's time with and without the network is reported."""
     from mpmwarpydw_amd import driver, getArgs, scenes
     from mpmwarpydw_amd.learned import LearnedReturnMap, harvest_run
     dom, ov = scenes.block_collapse(scale=1.0)
@@ -118,37 +174,166 @@ def test_learned_stress_update_gain():
     net0 = LearnedReturnMap(seed=3)
     net = LearnedReturnMap(seed=3)
     err0 = float(((net0.forward(Xh) - Th) ** 2).mean())
-    hist = net.train(X, T, steps=1500, batch=2048, lr=3e-3, lr_final=1e-4, seed=0, log_every=500)
+    hist = net.train(X, T, steps=6000, batch=4096, lr=3e-3, lr_final=3e-5, seed=0, log_every=1000)
     err1 = float(((net.forward(Xh) - Th) ** 2).mean())
     print(f"[learned] held-out MSE untrained {err0:.3e} trained {err1:.3e}; train loss {hist}", flush=True)
-    assert err1 < 0.1 * err0
+    assert err1 < 1e-3 * err0
 
-    def run(net_, use_net):
-        A = {}
-        state = {k: getattr(S.mpm, k).t.clone() for k in ("particle_F", "particle_stress", "particle_accumulated_strain",
-                                                          "particle_damage", "ys", "mu", "lam", "alpha")}
-        lab, v = S.sim.materialLabel.t.clone(), S.sim.particle_v.t.clone()
-        c = (net_ or net).stress_update(S.sim, S.mpm, S.xpbd, tol=1e-4, use_net=use_net).cpu().numpy()
-        for k in state:
-            A[k] = getattr(S.mpm, k).t.clone()
-            getattr(S.mpm, k).t.copy_(state[k])
-        A["label"] = S.sim.materialLabel.t.clone()
-        S.sim.materialLabel.t.copy_(lab)
-        S.sim.particle_v.t.copy_(v)
-        return A, c
-
-    ex, c_ex = run(net, False)
-    un, c_un = run(net0, True)
-    tr, c_tr = run(net, True)
-    print(f"[learned] plastic {c_tr[0]}, accepted untrained {c_un[1]}, trained {c_tr[1]}", flush=True)
+    for use in (False, True):  # first launches (code loading) outside the timed calls
+        _run_stress(S, net, use)
+    ex, c_ex, t_ex = _run_stress(S, net, False)
+    un, c_un, _ = _run_stress(S, net0, True)
+    tr, c_tr, t_tr = _run_stress(S, net, True)
+    f_un, f_tr = c_un[1] / c_un[0], c_tr[1] / c_tr[0]
+    print(f"[learned] plastic {c_tr[0]} of {c_tr[2]}: accepted untrained {c_un[1]} ({f_un:.3f}), trained "
+          f"{c_tr[1]} ({f_tr:.3f}); update {t_ex * 1e6:.0f} us exact, {t_tr * 1e6:.0f} us learned", flush=True)
     assert c_ex[1] == 0 and c_ex[0] == c_tr[0] == c_un[0] > 0
-    assert c_tr[1] >= c_un[1]
-    ys = np.maximum(ex["ys"].cpu().numpy(), 1.0)
+    assert f_tr > 0.9 and f_un < 0.05
     for A in (un, tr):
-        assert torch.equal(A["label"], ex["label"])
-        for k in ("particle_accumulated_strain", "particle_damage", "ys"):
-            assert torch.equal(A[k], ex[k]), k  # plastic increments come from the exact map
-        d = (A["particle_stress"] - ex["particle_stress"]).abs().amax(0).cpu().numpy()
-        scale = ex["particle_stress"].abs().amax(0).cpu().numpy()
-        # accepted predictions: within the judge's bound; others: the same route as the exact run
-        assert np.all(d <= 3e-4 * np.maximum(ys, 1.0) + 1e-5 * scale), np.max(d / np.maximum(ys, 1.0))
+        e = _check_against_exact(A, ex)
+    print(f"[learned] max stress error of the trained update: {e:.2e} x cohesion", flush=True)
+
+
+def _report_prediction_errors(S, net, tag):
+    """The network's errors on S's plastic states in the judge's units (1 = the
+    tolerance 1e-4), by overshoot decade: the deviatoric part kap |ds| and the
+    dilatancy part 0.9 (phi + kap) |d alpha| (2 mu / K)."""
+    from mpmwarpydw_amd.learned import LearnedReturnMap
+    X, T, _ = LearnedReturnMap.harvest(S.sim, S.mpm, S.xpbd)
+    Y = net.forward(X)
+    X, T, Y = (a.double().cpu().numpy() for a in (X, T, Y))
+    pk, kap = np.exp(10 * X[:, 0]), np.exp(10 * X[:, 2])
+    nu = (X[:, 7] + 1) / 4
+    r = (1 - 2 * nu) / (1 + nu) * 3  # 2 mu / K
+    dev = kap * np.abs(Y[:, 0] - T[:, 0]) / 1e-4
+    vol = 0.9 * pk * np.abs(Y[:, 1] - T[:, 1]) / r / 1e-4
+    bad = (dev > 1) | (vol > 1)
+    if bad.any():
+        q = lambda a: "/".join(f"{v:.3g}" for v in np.quantile(a, [0.05, 0.5, 0.95]))
+        names = ["0.1ln(phi+kap)", "0.1ln(y0)", "0.1ln(kap)", "d0", "d1", "d2", "tan", "4nu-1", "lgE-8", "D"]
+        for j, nm in enumerate(names):
+            print(f"[{tag}] {nm:>15}: rejected {q(X[bad, j])}  accepted {q(X[~bad, j])}", flush=True)
+        print(f"[{tag}] s target rejected {q(T[bad, 0])} accepted {q(T[~bad, 0])}", flush=True)
+    for lo in range(-7, 1):
+        m = (pk >= 10.0 ** lo) & (pk < 10.0 ** (lo + 1))
+        if m.sum():
+            print(f"[{tag}] phi+kap in [1e{lo}, 1e{lo + 1}): n {m.sum()}, dev med {np.median(dev[m]):.2f} "
+                  f"ok {np.mean(dev[m] <= 1):.3f}, vol med {np.median(vol[m]):.2f} ok {np.mean(vol[m] <= 1):.3f}",
+                  flush=True)
+
+
+@pytest.fixture(scope="module")
+def block_net():
+    """The network of the learned-map workloads: trained on 10M plastic trial
+    states harvested from 32,768-particle block-collapse runs of 500 substeps
+    whose materials span the batched-ensemble ranges (E 1e7-1e9 Pa, cohesion
+    1e4-1e6 Pa, friction 20-40 degrees)."""
+    import time
+    from mpmwarpydw_amd.learned import LearnedReturnMap, harvest_blocks
+    t0 = time.perf_counter()
+    X, T, stats = harvest_blocks(10_000_000, max_runs=100, substeps=500, seed=0, per_run=400_000, every=5,
+                                 log=lambda m: print(f"[learned] {m}", flush=True))
+    t1 = time.perf_counter()
+    print(f"[learned] harvested {stats['pairs']} pairs from {len(stats['runs'])} runs in {t1 - t0:.1f} s", flush=True)
+    assert X.shape[0] > 1_000_000
+    net = LearnedReturnMap(seed=5)
+    hist = net.train(X, T, steps=30000, batch=8192, lr=3e-3, lr_final=2e-5, seed=1, log_every=5000)
+    torch.cuda.synchronize()
+    print(f"[learned] trained 30000 steps in {time.perf_counter() - t1:.1f} s; loss {hist}", flush=True)
+    del X, T
+    return net
+
+
+def test_c3_learned_map_on_4m_particles(block_net):
+    """The learned-map workload: 4.0M particles of the undercut slab's material
+    distribution (default_rng(3)) around the yield surface, one constitutive
+    update with the network (trained on block-collapse pairs only) judged by
+    the exact map, against the exact update: every particle within the judge's
+    bound, and the accepted fraction and both update times reported."""
+    from mpmwarpydw_amd.learned import LearnedReturnMap
+    from state_utils import c3_state
+    S = c3_state()
+    n = S.sim.nPoints
+    print(f"[c3] {n} particles", flush=True)
+    assert 3_900_000 < n < 4_100_000
+    for use in (False, True):
+        _run_stress(S, block_net, use)
+    ex, c_ex, t_ex = _run_stress(S, block_net, False)
+    tr, c_tr, t_tr = _run_stress(S, block_net, True)
+    _, c_un, _ = _run_stress(S, LearnedReturnMap(seed=5), True)
+    frac = c_tr[1] / c_tr[0]
+    print(f"[c3] plastic {c_tr[0]} of {c_tr[2]}: accepted {c_tr[1]} ({frac:.4f}), untrained {c_un[1]}; update "
+          f"{t_ex * 1e6:.0f} us exact, {t_tr * 1e6:.0f} us learned", flush=True)
+    assert c_ex[0] == c_tr[0] > 1_000_000 and c_tr[2] == n
+    _report_prediction_errors(S, block_net, "c3")
+    e = _check_against_exact(tr, ex)
+    print(f"[c3] max stress error {e:.2e} x cohesion", flush=True)
+    assert frac > 0.9 and c_un[1] < 0.05 * c_un[0]
+
+
+def test_c4_ensemble_learned_matches_exact(block_net):
+    """The batched-ensemble workload (64 block scenes, log-uniform 20k-500k
+    particles, seeded materials) with the learned map in the ensemble substep:
+    after 60 exact substeps, five substeps each run from the same state with
+    the learned map and with the exact map (the same kernel without the
+    network); every particle's stress is within the judge's bound of the exact
+    one (per scene, relative to its cohesion), and labels, plastic strain and
+    damage are identical.  A 20-substep free run
+    of each form from the same state is reported."""
+    import time
+    from mpmwarpydw_amd import driver, getArgs
+    from mpmwarpydw_amd.ensemble import Ensemble, c4_scene_list
+    t0 = time.perf_counter()
+    specs = c4_scene_list(count=64)
+    E = Ensemble([driver.setup(getArgs.get_args([], ov), dom) for dom, ov in specs])
+    print(f"[c4] {len(E.table)} scenes, {E.n} particles, setup {time.perf_counter() - t0:.1f} s", flush=True)
+    E.reorder()
+    for _ in range(60):
+        E.substep()
+    keys = [("mpm", "particle_stress"), ("mpm", "particle_accumulated_strain"), ("mpm", "particle_damage"),
+            ("sim", "materialLabel"), ("sim", "particle_x"), ("sim", "particle_v")]
+    arrays = {}
+    for obj in (E.sim, E.mpm, E.xpbd):
+        for name, a in vars(obj).items():
+            if hasattr(a, "t") and isinstance(getattr(a, "t"), torch.Tensor):
+                arrays[id(a)] = a
+    # the whole device state: particle and grid arrays and the workspace (which
+    # holds the bins that the G2P pass writes for the next substep)
+    tensors = [a.t for a in arrays.values()] + [E.grid_m.t, E.grid_v_in.t, E.grid_v_out.t, E._ws.t]
+    scene = E.scene.t.long()
+    ys = E.mpm.ys.t  # per-particle cohesion (the same in both forms: the exact map computes hardening)
+    cnt = torch.zeros(3, dtype=torch.int32, device="cuda")
+    worst = 0.0
+    for step in range(5):
+        snap = [t.clone() for t in tensors]
+        E.substep(learned=block_net, tol=1e-4, counters=cnt)
+        got = {k: getattr(getattr(E, o), k).t.clone() for o, k in keys}
+        for t, t0 in zip(tensors, snap):
+            t.copy_(t0)
+        E.substep(learned=block_net, use_net=False)
+        for o, k in keys[1:4]:
+            assert torch.equal(got[k], getattr(getattr(E, o), k).t), (step, k)
+        d = (got["particle_stress"] - E.mpm.particle_stress.t).abs().amax(0)
+        scale = E.mpm.particle_stress.t.abs().amax(0)
+        c = torch.clamp(ys, min=1.0)
+        ratio = ((d - 1e-6 * scale) / c).cpu().numpy()
+        sc = scene.cpu().numpy()
+        per_scene = np.zeros(len(E.table))
+        np.maximum.at(per_scene, sc, ratio)
+        worst = max(worst, float(per_scene.max()))
+        assert np.all(per_scene <= 3e-4), (step, np.argmax(per_scene), per_scene.max())
+    c = cnt.cpu().numpy()
+    print(f"[c4] 5 substeps: plastic {c[0]}, accepted {c[1]} ({c[1] / max(c[0], 1):.4f}), updated {c[2]}; "
+          f"worst per-scene stress error {worst:.2e} x cohesion", flush=True)
+    assert c[0] > 0 and c[1] > 0.5 * c[0]
+    # free run of 20 substeps of each form from the same state
+    snap = [t.clone() for t in tensors]
+    for _ in range(20):
+        E.substep(learned=block_net)
+    sl = E.mpm.particle_stress.t.clone()
+    for t, t0 in zip(tensors, snap):
+        t.copy_(t0)
+    for _ in range(20):
+        E.substep(learned=block_net, use_net=False)
+    drift = ((sl - E.mpm.particle_stress.t).abs().amax(0) / torch.clamp(ys, min=1.0)).amax().item()
+    print(f"[c4] 20-substep free run: max stress difference {drift:.2e} x cohesion", flush=True)

@@ -146,6 +146,30 @@ def test_stress_return_map_parity(o64, seed, svd):
             assert (np.abs(H["particle_F"] - Hin["particle_F_trial"]).max(axis=(1, 2))[mask & tr] > 0).any()
 
 
+@pytest.mark.parametrize("seed,svd", [(0, False), (3, True)])
+def test_learned_kernel_exact_projection_parity(o64, seed, svd):
+    """The constitutive update of the learned-map kernel
+    (mpmx_compute_stress_learned) with no network, i.e. the exact projection
+    for every plastic state through its own SVD-only code, against the FP64
+    oracle at the same per-particle bounds as the plain update."""
+    from mpmwarpydw_amd.learned import LearnedReturnMap
+    S = make_sim(seed=seed)
+    if svd:
+        large_and_inverted_F(S, seed)
+    H = host_state(S)
+    Hin = {k: v.copy() for k, v in H.items()}
+    print(f"[learned-kernel parity] seed={seed} svd={svd} n={S.sim.nPoints}", flush=True)
+    c = LearnedReturnMap(seed=0).stress_update(S.sim, S.mpm, S.xpbd, use_net=False).cpu().numpy()
+    G = host_state(S)
+    oracle_stress(o64, H, S.mpm.eff)
+    assert c[1] == 0 and c[0] > 100
+    assert np.array_equal(G["materialLabel"], H["materialLabel"])
+    assert_pp(G, H, Hin, S, keys={"particle_stress", "particle_F", "particle_damage", "particle_accumulated_strain",
+                               "ys"})
+    for k in ("mu", "lam", "alpha", "particle_v", "particle_v_initial_xpbd", "particle_x_initial_xpbd"):
+        assert pp_err(G[k], H[k]) < TOL, k
+
+
 def test_fused_substep_mixed_viscosity(o64, o32):
     """Viscosities zero on some particles (the update then skips reading C) and
     not on others (shear only, bulk only, both), in one fused substep."""

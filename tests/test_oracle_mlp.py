@@ -25,7 +25,10 @@ def init_params(mo, rng, scale=0.3):
 def torch_forward(mo, params, X, quant=False):
     W1, b1, W2, b2, W3, b3 = [torch.tensor(a, dtype=torch.float64) for a in mo.split(params)]
     q = (lambda t: t.to(torch.float32).to(torch.bfloat16).to(torch.float64)) if quant else (lambda t: t)
-    x = q(torch.tensor(X, dtype=torch.float64))
+    x = torch.tensor(X, dtype=torch.float64)
+    if quant:  # inputs enter the matrix units as hi + lo bfloat16 halves
+        hi = q(x)
+        x = hi + q(x - hi)
     h1 = q(torch.nn.functional.gelu(x @ q(W1).T + b1, approximate="tanh"))
     h2 = q(torch.nn.functional.gelu(h1 @ q(W2).T + b2, approximate="tanh"))
     return (h2 @ q(W3).T + b3).numpy()
@@ -39,6 +42,12 @@ def test_bf16_rounding_matches_torch(mo):
     got = np.array([mo.bf16(float(a)) for a in v])
     assert np.array_equal(got, want)
     assert mo.bf16(1 + 2 ** -8) == 1.0  # a tie goes to the even mantissa
+    # the split input keeps two 8-bit significands: 1 + 2^-12 survives (bf16 alone drops it), the
+    # 2^-21 bit of 1 + 2^-12 + 2^-21 lies beyond the low half's significand and rounds away
+    assert mo.bf16(1 + 2 ** -12) == 1.0 and mo.bf16_split(1 + 2 ** -12) == 1 + 2 ** -12
+    assert mo.bf16_split(1 + 2 ** -12 + 2 ** -21) == 1 + 2 ** -12
+    rel = np.abs(np.array([mo.bf16_split(float(a)) for a in v[:4000]]) / v[:4000].astype(np.float64) - 1)
+    assert rel.max() <= 2.0 ** -16
 
 
 def test_gelu_and_derivative(mo):
@@ -134,6 +143,15 @@ def _material(n, rng):
     return E, nu, mu, lam, ys, alpha, phi
 
 
+def _trial(e, mu, lam, ys, alpha, D):
+    """Deviator, its norm and the yield value of principal trial strains."""
+    d = e - e.mean(1, keepdims=True)
+    dn = np.linalg.norm(d, axis=1)
+    p = (2 * mu / 3 + lam) * e.sum(1)
+    Y = (1 - D) * (ys - alpha * p)
+    return d, dn, Y
+
+
 def test_features_closed_form(mo):
     rng = np.random.default_rng(5)
     n = 64
@@ -142,21 +160,26 @@ def test_features_closed_form(mo):
     acc = rng.uniform(0, 0.05, n)
     D = rng.uniform(0, 1, n)
     X = mo.features(e, mu, lam, ys, alpha, acc, D)
-    eref = ys / (2 * mu)
-    assert np.allclose(X[:, :3], e / eref[:, None], rtol=1e-12)
-    assert np.allclose(X[:, 3], np.log10(E) - 8, atol=1e-12)
-    assert np.allclose(X[:, 4], 4 * nu - 1, atol=1e-12)
-    assert np.allclose(X[:, 5], np.log10(ys) - 5, atol=1e-12)
+    d, dn, Y = _trial(e, mu, lam, ys, alpha, D)
+    over, y0, kap = (2 * mu * dn - Y) / ys, Y / ys, 2 * mu * 1e-6 / ys
+    assert np.allclose(X[:, 0], 0.1 * np.log(np.maximum(over + kap, 1e-30)), rtol=1e-10, atol=1e-12)
+    assert np.allclose(X[:, 1], 0.1 * np.log(np.maximum(y0, 1e-12)), rtol=1e-12)
+    assert np.allclose(X[:, 2], 0.1 * np.log(kap), rtol=1e-12)
+    assert (over + kap > 0).sum() > 5 and (over + kap < 0).sum() > 5  # both branches of the clamp
+    assert np.allclose(X[:, 3:6], d / dn[:, None], rtol=1e-10)
     assert np.allclose(X[:, 6], np.tan(phi), rtol=1e-10)
-    assert np.allclose(X[:, 8], 20 * acc) and np.allclose(X[:, 9], D)
-    # dilation angle: the reference's dilatancy 0.3 alpha mapped back through the same relation
-    s = np.sin(np.arctan(X[:, 7]))
-    assert np.allclose(2 * s / (np.sqrt(3) * (3 - s)), 0.3 * alpha, rtol=1e-10)
+    assert np.allclose(X[:, 7], 4 * nu - 1, atol=1e-12)
+    assert np.allclose(X[:, 8], np.log10(E) - 8, atol=1e-12)
+    assert np.allclose(X[:, 9], D)
 
 
 def test_targets_match_full_return_map():
-    """(e' - e) / eref from the principal-space projection equals the principal
-    log strains of F after the full FP64 constitutive update."""
+    """The radial return plus the two predicted corrections,
+    e - (1 - Y/(2 mu |d|)) d - eref kap s d/|d| + 0.3 alpha eref (phi + kap),
+    with the exact targets (s, alpha) equals the principal log strains of F after
+    the full FP64 constitutive update (so a dropped 1e-6 or dilatancy term, or a
+    wrong sign, fails here); and s = y0 / (phi + y0 + kap) in terms of the
+    features."""
     o = Oracle(64)
     mo = MLPOracle()
     rng = np.random.default_rng(6)
@@ -178,14 +201,27 @@ def test_targets_match_full_return_map():
     e_trial = np.log(np.linalg.svd(F_trial, compute_uv=False))
     e_new = np.log(np.linalg.svd(F, compute_uv=False))
     T, plastic = mo.targets(e_trial, mu, lam, ys0, alpha, D)
-    eref = ys0 / (2 * mu)
+    c = np.maximum(ys0, 1.0)
+    eref = c / (2 * mu)
     assert plastic.sum() > 50 and (~plastic).sum() > 10
     assert np.array_equal(plastic, acc > 0)
-    got = e_trial + T * eref[:, None]
+    d, dn, Y = _trial(e_trial, mu, lam, ys0, alpha, D)
+    phi, y0, kap = (2 * mu * dn - Y) / c, Y / c, 2 * mu * 1e-6 / c
+    dh = d / dn[:, None]
+    got = (e_trial - (1 - Y / (2 * mu * dn))[:, None] * d - (eref * kap * T[:, 0])[:, None] * dh
+           + (0.3 * T[:, 1] * eref * (phi + kap))[:, None])
     stay = lab == 1  # a particle that reaches damage 1 turns XPBD and keeps F_trial instead
     assert stay.sum() > 100
-    assert np.max(np.abs(got - e_new)[stay]) < 1e-9
+    assert np.max(np.abs(got - e_new)[stay & plastic]) < 1e-9
     assert np.all(T[~plastic] == 0)
+    assert np.array_equal(T[plastic, 1], alpha[plastic]) and np.all(T[:, 2] == 0)
+    assert np.allclose(mo.apply(e_trial, T, mu, lam, ys0, alpha, D)[plastic], got[plastic], atol=1e-15)
+    X = mo.features(e_trial, mu, lam, ys0, alpha, acc, D)
+    pk, y0f = np.exp(10 * X[:, 0]), np.exp(10 * X[:, 1])
+    kapf = np.exp(10 * X[:, 2])
+    closed = y0f / (pk + y0f)
+    assert np.max(np.abs(closed - T[:, 0])[plastic & (Y > 0)]) < 1e-9
+    assert np.allclose(kapf, kap, rtol=1e-12)
 
 
 def test_judge_accepts_exact_and_rejects_each_kind_of_error(mo):
@@ -197,8 +233,7 @@ def test_judge_accepts_exact_and_rejects_each_kind_of_error(mo):
     D = rng.uniform(0, 0.5, n)
     T, plastic = mo.targets(e, mu, lam, ys, alpha, D)
     e, T, mu, lam, ys, alpha, D = (a[plastic] for a in (e, T, mu, lam, ys, alpha, D))
-    eref = ys / (2 * mu)
-    exact = e + T * eref[:, None]
+    exact = mo.apply(e, T, mu, lam, ys, alpha, D)
     assert mo.judge(e, exact, mu, lam, ys, alpha, D).all()
     K = lam + 2 * mu / 3
     m = exact.mean(1, keepdims=True)
