@@ -687,7 +687,7 @@ int mpmx_mlp_adam_step(int n, float* params, const float* grad, float* m, float*
  * trial log strains e, Y the yield value, the overshoot phi = (2 mu |d| - Y)/c,
  * y0 = Y / c and kap = 2 mu 1e-6 / c, X[p] = the 10 features 0.1 ln(phi + kap),
  * 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d/|d| (3), tan friction angle, 4 nu - 1,
- * log10 E - 8, damage; T[p] = (s, alpha, 0) for plastic states (zeros for
+ * log10 E - 8, damage; T[p] = (s, 20 (alpha - 0.3), 0) for plastic states (zeros for
  * elastic ones), with s = Y / (2 mu (|d| + 1e-6)), so that the exact projection
  * is e' = e - (1 - Y/(2 mu |d|)) d - eref kap s d/|d| + 0.3 alpha eref
  * (phi + kap); e_trial[p] = e.  The state is not modified. */

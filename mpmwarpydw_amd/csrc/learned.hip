@@ -13,16 +13,18 @@
 //     e' = e0' - eref kap s d/|d| + 0.3 alpha eref (phi + kap) (1, 1, 1),
 // where s = y0 / (phi + y0 + kap) is the part of the deviator that the 1e-6
 // keeps and alpha the cone coefficient that sets the dilatancy.  The network
-// predicts y = (s, alpha, 0) from 10 features: 0.1 ln(phi + kap),
-// 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d / |d|, tan(friction angle), 4 nu - 1,
-// log10 E - 8 and D.  The exact map is the judge: a prediction is used only
-// when both residuals
+// predicts y = (s, 20 (alpha - 0.3), 0) (alpha centred and scaled, so that the
+// loss weighs its error as the dilatancy needs) from 10 features:
+// 0.1 ln(phi + kap), 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d / |d|, tan(friction
+// angle), 4 nu - 1, log10 E - 8 and D.  The exact map is the judge: a
+// prediction is used only when both residuals
 //     2 mu |dev e' - Y/(2 mu edn) d|              (yield surface and flow direction)
 //     K |tr e' - tr e - 0.9 alpha dgam|           (dilatancy)
 // are within tol * max(cohesion, 1); otherwise, and for Y < 0 (tension past
-// the cone's apex), the exact projection replaces it.
-// Everything else (plastic strain, damage, hardening, phase transition,
-// Kirchhoff stress, viscosity) is the exact update.
+// the cone's apex), the plain update's projection stands.  Everything else
+// (which particles are plastic, plastic strain, damage, hardening, phase
+// transition, Kirchhoff stress, viscosity) is the plain update of
+// mpmx_constitutive.hpp.
 #include "mlp.hpp"
 #include "mpmx_constitutive.hpp"
 #include "mpmx_internal.hpp"
@@ -80,14 +82,6 @@ MPMX_HD void radial_return(const Principal& s, float mu, float e0[3]) {
     for (int k = 0; k < 3; k++) e0[k] = s.e[k] - r0 * s.d[k];
 }
 
-// exact projection e' of a plastic state and its dgam
-MPMX_HD void exact_projection(const Principal& s, float mu, float al, float en[3], float& dgam) {
-    const float edn = s.dn + 1.0e-6f;
-    dgam = edn - s.Y / (2.0f * mu);
-    const float r = dgam / edn, volc = (0.3f * al) * dgam;
-    for (int k = 0; k < 3; k++) en[k] = (s.e[k] - r * s.d[k]) + volc;
-}
-
 MPMX_HD bool lrm_judge(const Principal& s, const float q[3], float mu, float lam, float ys, float al, float tol) {
     const float edn = s.dn + 1.0e-6f;
     const float c = s.Y / (2.0f * mu * edn);
@@ -101,7 +95,7 @@ MPMX_HD bool lrm_judge(const Principal& s, const float q[3], float mu, float lam
     return res1 <= lim && res2 <= lim;
 }
 
-// Harvest: features, exact targets (s, alpha, 0) and the state of every
+// Harvest: features, exact targets (s, 20 (alpha - 0.3), 0) and the state of every
 // particle (1 plastic MPM, 0 elastic MPM, -1 not an active MPM particle);
 // e_trial gets the principal trial log strains.  Reads the state, writes
 // nothing of it.
@@ -125,114 +119,21 @@ __global__ void lrm_harvest_kernel(mpmx_particles P, float* __restrict__ X, floa
     lrm_features(s, mu, lam, ys, al, P.acc_strain[p], D, x);
     for (int k = 0; k < 10; k++) X[p * 10 + k] = x[k];
     Tg[p * 3 + 0] = s.plastic ? s.Y / (2.0f * mu * (s.dn + 1.0e-6f)) : 0.0f;
-    Tg[p * 3 + 1] = s.plastic ? al : 0.0f;
+    Tg[p * 3 + 1] = s.plastic ? 20.0f * (al - 0.3f) : 0.0f;
     Tg[p * 3 + 2] = 0.0f;
     for (int k = 0; k < 3; k++) E3[p * 3 + k] = eps[k];
     flag[p] = s.plastic ? 1 : 0;
 }
 
-// The constitutive update of one particle with the projection supplied: the
-// SVD route of constitutive_update (mpmx_constitutive.hpp) with e' = en.
-__device__ __forceinline__ void finish_update(const mpmx_particles& P, long p, float eff, const Svd3& sv,
-                                              const float eps[3], const Principal& s, bool plastic, const float en_in[3]) {
-    const long n = P.n;
-    const int label = P.materialLabel[p];
-    float mu = P.mu[p], lam = P.lam[p];
-    const float ys = P.ys[p], D = P.damage[p];
-    int branch = 0;
-    float en[3] = {en_in[0], en_in[1], en_in[2]};
-    if (plastic) {
-        const float edn = s.dn + 1.0e-6f;
-        const float dgam = edn - s.Y / (2.0f * mu);
-        const float dep = sqrtf(2.0f / 3.0f) * dgam;
-        P.acc_strain[p] = P.acc_strain[p] + dep;
-        const float sc = P.strainCriteria[p];
-        float Dn = (sc > 0.0f) ? wmin(1.0f, D + dep / sc) : 1.0f;
-        if (ys < 0.0f || s.Y < 0.0f) Dn = 1.0f;
-        P.damage[p] = Dn;
-        if (Dn >= 1.0f && label == 1) {
-            branch = 2;
-            P.materialLabel[p] = 2;
-            f3 xp = ld3(P.x, n, p);
-            st3(P.x_init_xpbd, n, p, xp);
-            const float tau0 = 2.0f * mu * eps[0] + lam * s.esum, tau1 = 2.0f * mu * eps[1] + lam * s.esum,
-                        tau2 = 2.0f * mu * eps[2] + lam * s.esum;
-            const float u = 0.5f * (tau0 * eps[0] + tau1 * eps[1] + tau2 * eps[2]);
-            const float v_expected = sqrtf(2.0f * eff * u / P.density[p]);
-            f3 vp = ld3(P.v, n, p);
-            f3 vdir = normalize3(f3{vp.x + 1.0e-12f, vp.y + 1.0e-12f, vp.z + 1.0e-12f});
-            f3 vn = f3{vp.x + v_expected * vdir.x, vp.y + v_expected * vdir.y, vp.z + v_expected * vdir.z};
-            st3(P.v, n, p, vn);
-            st3(P.v_init_xpbd, n, p, vn);
-            P.ys[p] = 0.0f;
-            P.mu[p] = 0.0f;
-            P.lam[p] = 0.0f;
-            P.alpha[p] = 0.0f;
-            mu = 0.0f;
-            lam = 0.0f;
-        } else {
-            branch = 1;
-            P.ys[p] = wmax(0.0f, ys + 2.0f * mu * (P.hardening[p] - P.softening[p]) * dgam);
-        }
-    }
-    float d[3];
-    float Ft[9];
-    if (branch == 1) {
-        float e[3], l[3];
-        for (int k = 0; k < 3; k++) {
-            e[k] = expf(en[k]);
-            l[k] = (e[k] >= 1.0e-6f) ? en[k] : logf(1.0e-6f);
-        }
-        const float ls = l[0] + l[1] + l[2];
-        for (int k = 0; k < 3; k++) {
-            const float dk = 2.0f * mu * l[k] + lam * ls;
-            d[k] = (e[k] >= 1.0e-6f) ? dk : dk * (e[k] / 1.0e-6f);
-        }
-        float Fo[9];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                Fo[i * 3 + j] = sv.U[i * 3 + 0] * e[0] * sv.V[j * 3 + 0] + sv.U[i * 3 + 1] * e[1] * sv.V[j * 3 + 1] +
-                                sv.U[i * 3 + 2] * e[2] * sv.V[j * 3 + 2];
-        st9(P.F, n, p, Fo);
-    } else {
-        ld9(P.F_trial, n, p, Ft);
-        float l[3];
-        for (int k = 0; k < 3; k++) l[k] = log_clamped(sv.s[k], sv.lam[k], 1.0e-6f);
-        const float ls = l[0] + l[1] + l[2];
-        for (int k = 0; k < 3; k++) {
-            const float dk = 2.0f * mu * l[k] + lam * ls;
-            d[k] = (sv.s[k] >= 1.0e-6f) ? dk : dk * (sv.s[k] / 1.0e-6f);
-        }
-        st9(P.F, n, p, Ft);
-    }
-    float G[9];
-    ld9(P.C, n, p, G);
-    const float s00 = G[0], s11 = G[4], s22 = G[8];
-    const float s01 = 0.5f * (G[1] + G[3]), s02 = 0.5f * (G[2] + G[6]), s12 = 0.5f * (G[5] + G[7]);
-    const float vr = s00 + s11 + s22;
-    const float es = 2.0f * P.eta_shear[p], eb = P.eta_bulk[p];
-    const float third = vr / 3.0f;
-    float T[9];
-    const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
-    float tau[6];
-    for (int q = 0; q < 6; q++) {
-        const int i = ii[q], j = jj[q];
-        tau[q] = sv.U[i * 3 + 0] * d[0] * sv.U[j * 3 + 0] + sv.U[i * 3 + 1] * d[1] * sv.U[j * 3 + 1] +
-                 sv.U[i * 3 + 2] * d[2] * sv.U[j * 3 + 2];
-    }
-    T[0] = tau[0] + (es * (s00 - third) + eb * vr);
-    T[4] = tau[1] + (es * (s11 - third) + eb * vr);
-    T[8] = tau[2] + (es * (s22 - third) + eb * vr);
-    T[1] = T[3] = tau[3] + es * s01;
-    T[2] = T[6] = tau[4] + es * s02;
-    T[5] = T[7] = tau[5] + es * s12;
-    st9(P.stress, n, p, T);
-}
-
-// One wavefront handles 64 consecutive particles per step; the network runs on
-// the matrix cores for the wavefront's 4 tiles of 16 whenever one of its
-// particles is plastic.  counters: [0] += plastic states, [1] += accepted
-// predictions, [2] += particles updated.
+// One wavefront handles 64 consecutive particles per step.  Every particle
+// runs the plain update's first phase (cu_begin: invariants and branch, as
+// compute_stress_from_F_trial decides them).  Plastic particles that stay MPM
+// are the network's candidates: they take the SVD of F_trial (if the series
+// route skipped it), their principal state and features; the network runs on
+// the matrix cores for the wavefront's 4 tiles of 16 whenever one lane is a
+// candidate.  A judged prediction is the projection cu_finish uses; every other
+// particle finishes exactly as the plain update, bit for bit.  counters:
+// [0] += plastic states, [1] += accepted predictions, [2] += particles updated.
 __global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P, float eff, const float* __restrict__ prm,
                                                                 float tol, int use_net, int32_t* __restrict__ counters) {
     __shared__ MlpWeights w;
@@ -245,29 +146,23 @@ __global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P
     for (long base = wave * 64; base < P.n; base += nwaves * 64) {  // uniform per wavefront
         const long p = base + lane;
         const bool mpm = p < P.n && P.activeLabel[p] == 1 && P.materialLabel[p] <= 1;
-        Svd3 sv;
+        CuState c;
+        Principal s;
         float eps[3] = {0.0f, 0.0f, 0.0f}, x[10];
         for (int k = 0; k < 10; k++) x[k] = 0.0f;
-        Principal s;
-        bool plastic = false;
-        float mu = 1.0f, lam = 0.0f, ys = 0.0f, al = 0.0f, D = 0.0f;
+        bool cand = false;
         if (mpm) {
-            float Ft[9];
-            ld9(P.F_trial, P.n, p, Ft);
-            svd3(Ft, sv);
-            for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
-            mu = P.mu[p];
-            lam = P.lam[p];
-            ys = P.ys[p];
-            al = P.alpha[p];
-            D = P.damage[p];
-            s = principal_state(eps, mu, lam, ys, al, D);
-            plastic = s.plastic;
-            if (plastic) lrm_features(s, mu, lam, ys, al, P.acc_strain[p], D, x);
+            cu_begin(P, p, c);
+            if (use_net && c.branch == 1) {
+                if (c.fast) svd3(c.Ft, c.sv);
+                for (int k = 0; k < 3; k++) eps[k] = log_clamped(c.sv.s[k], c.sv.lam[k], 0.01f);
+                s = principal_state(eps, c.mu, c.lam, c.ys, c.al, c.D);
+                cand = s.plastic && s.Y >= 0.0f;
+                if (cand) lrm_features(s, c.mu, c.lam, c.ys, c.al, P.acc_strain[p], c.D, x);
+            }
         }
         float yo[3] = {0.0f, 0.0f, 0.0f};
-        const unsigned long long pm = __ballot(plastic);
-        if (use_net && pm) {  // uniform: every lane runs the matrix-core tiles
+        if (__ballot(cand)) {  // uniform: every lane runs the matrix-core tiles
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 // B fragment: lane l takes features 4(l/16)..+3 of particle t*16 + l%16
@@ -294,30 +189,22 @@ __global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P
         }
         if (mpm) {
             n_done++;
-            float en[3] = {eps[0], eps[1], eps[2]};
-            if (plastic) {
-                n_plastic++;
-                bool ok = false;
-                if (use_net && s.Y >= 0.0f) {
-                    const float c = ys > 1.0f ? ys : 1.0f;
-                    const float er = c / (2.0f * mu), kap = 2.0f * mu * 1.0e-6f / c;
-                    const float dil = 0.3f * yo[1] * (er * ((2.0f * mu * s.dn - s.Y) / c + kap));
-                    const float shr = s.dn > 0.0f ? er * kap * yo[0] / s.dn : 0.0f;
-                    float q[3];
-                    radial_return(s, mu, q);
-                    for (int k = 0; k < 3; k++) q[k] = (q[k] - shr * s.d[k]) + dil;
-                    ok = lrm_judge(s, q, mu, lam, ys, al, tol);
-                    if (ok) {
-                        for (int k = 0; k < 3; k++) en[k] = q[k];
-                        n_accept++;
-                    }
-                }
-                if (!ok) {
-                    float dg;
-                    exact_projection(s, mu, al, en, dg);
-                }
+            if (c.branch != 0) n_plastic++;
+            float q[3];
+            bool ok = false;
+            if (cand) {
+                const float mu = c.mu, ys = c.ys;
+                const float cc = ys > 1.0f ? ys : 1.0f;
+                const float er = cc / (2.0f * mu), kap = 2.0f * mu * 1.0e-6f / cc;
+                const float dil = 0.3f * (0.3f + 0.05f * yo[1]) * (er * ((2.0f * mu * s.dn - s.Y) / cc + kap));
+                const float shr = s.dn > 0.0f ? er * kap * yo[0] / s.dn : 0.0f;
+                radial_return(s, mu, q);
+                for (int k = 0; k < 3; k++) q[k] = (q[k] - shr * s.d[k]) + dil;
+                ok = lrm_judge(s, q, mu, c.lam, ys, c.al, tol);
+                n_accept += ok ? 1 : 0;
             }
-            finish_update(P, p, eff, sv, eps, s, plastic, en);
+            float T[9];
+            cu_finish(P, p, eff, c, T, ok ? q : nullptr);
         }
     }
     // one atomic per wavefront and counter

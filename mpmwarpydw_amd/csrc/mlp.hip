@@ -55,8 +55,10 @@ __global__ void __launch_bounds__(kBlock) mlp_forward_kernel(int B, const float*
 
 // per-wavefront LDS of the training kernel (bfloat16 bits)
 struct TrainWave {
-    unsigned short H1[kNH][16];  // [feature][sample]
+    unsigned short H1[kNH][16];  // [feature][sample], high halves
+    unsigned short H1L[kNH][16];  // low halves
     unsigned short H2[kNH][16];
+    unsigned short H2L[kNH][16];
     unsigned short G1[kNH][16];
     unsigned short G2[kNH][16];
     unsigned short G3[16][16];
@@ -139,7 +141,9 @@ __global__ void __launch_bounds__(kBlock) mlp_train_kernel(int B, const float* _
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 tw.H1[mb * 16 + r0 + j][col] = (unsigned short)t.h1b[mb][j];
+                tw.H1L[mb * 16 + r0 + j][col] = (unsigned short)t.h1l[mb][j];
                 tw.H2[mb * 16 + r0 + j][col] = (unsigned short)t.h2b[mb][j];
+                tw.H2L[mb * 16 + r0 + j][col] = (unsigned short)t.h2l[mb][j];
             }
         // output error: lanes 0..15 hold outputs 0..2 of sample l
         float g3[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -193,11 +197,14 @@ __global__ void __launch_bounds__(kBlock) mlp_train_kernel(int B, const float* _
             gW1[mb] = mfma16(ag1, xtl, mfma16(ag1, xtb, gW1[mb]));
             const bf16x4 ag2 = a_frag<16>(tw.G2, mb, 0, lane);
 #pragma unroll
-            for (int nb = 0; nb < 4; nb++) gW2[mb][nb] = mfma16(ag2, a_frag<16>(tw.H1, nb, 0, lane), gW2[mb][nb]);
+            for (int nb = 0; nb < 4; nb++)
+                gW2[mb][nb] = mfma16(ag2, a_frag<16>(tw.H1L, nb, 0, lane),
+                                     mfma16(ag2, a_frag<16>(tw.H1, nb, 0, lane), gW2[mb][nb]));
         }
         const bf16x4 ag3 = a_frag<16>(tw.G3, 0, 0, lane);
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) gW3[nb] = mfma16(ag3, a_frag<16>(tw.H2, nb, 0, lane), gW3[nb]);
+        for (int nb = 0; nb < 4; nb++)
+            gW3[nb] = mfma16(ag3, a_frag<16>(tw.H2L, nb, 0, lane), mfma16(ag3, a_frag<16>(tw.H2, nb, 0, lane), gW3[nb]));
         __syncthreads();  // before the next tile overwrites the transposes
     }
 

@@ -2,12 +2,13 @@
 //
 // Parameters are one FP32 vector (master copy, updated by Adam):
 //   W1[64][10], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]   (out x in)
-// The kernels round weights, hidden activations and back-propagated gradients
-// to bfloat16 (round to nearest even) where they enter
-// v_mfma_f32_16x16x16_bf16, and accumulate in FP32.  The inputs enter as two
-// bfloat16 halves, hi = bf16(x) and lo = bf16(x - hi), in two MFMA passes
-// over the first layer: 16 significant bits of every feature, so that the
-// network can resolve small corrections of well-scaled features.
+// The kernels round weights and back-propagated gradients to bfloat16 (round
+// to nearest even) where they enter v_mfma_f32_16x16x16_bf16, and accumulate
+// in FP32.  Inputs and hidden activations enter as two bfloat16 halves,
+// hi = bf16(x) and lo = bf16(x - hi), in two MFMA passes per layer: 16
+// significant bits of every value, so that the outputs are not limited by
+// the 8 bits of one bfloat16 (an output made of 64 terms would carry their
+// rounding errors).
 //
 // Every product is computed "transposed", features x samples: a layer's
 // output Z^T[o][b] = sum_i W[o][i] X^T[i][b] takes W as the A operand (16 rows
@@ -107,13 +108,12 @@ __device__ __forceinline__ bf16x4 a_frag(const unsigned short (*M)[COLS], int mb
 
 // Forward pass of one tile of 16 samples.  xb, xl are the B fragments of the
 // inputs' high and low bfloat16 halves (lane l: features 4(l/16)..+3 of sample
-// l%16).  Outputs: z1, z2
-// (pre-activations, D layout per 16-row block), h1b, h2b (bfloat16
-// activations, B layout) and y (lanes 0..15 hold y[0..2] of sample l%16 in
-// y[0..2]).
+// l%16).  Outputs: z1, z2 (pre-activations, D layout per 16-row block), h1b,
+// h1l, h2b, h2l (high and low bfloat16 halves of the activations, B layout)
+// and y (lanes 0..15 hold y[0..2] of sample l%16 in y[0..2]).
 struct MlpTile {
     f32x4 z1[4], z2[4];
-    bf16x4 h1b[4], h2b[4];
+    bf16x4 h1b[4], h1l[4], h2b[4], h2l[4];
     f32x4 y;
 };
 
@@ -126,19 +126,31 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& w, bf16x4 xb,
         c = mfma16(a, xb, c);
         c = mfma16(a, xl, c);
         t.z1[mb] = c;
-        t.h1b[mb] = pack_bf16(gelu_f(c[0]), gelu_f(c[1]), gelu_f(c[2]), gelu_f(c[3]));
+        const float g0 = gelu_f(c[0]), g1 = gelu_f(c[1]), g2 = gelu_f(c[2]), g3 = gelu_f(c[3]);
+        t.h1b[mb] = pack_bf16(g0, g1, g2, g3);
+        t.h1l[mb] = pack_bf16_lo(g0, g1, g2, g3);
     }
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         f32x4 c = {w.b2[mb * 16 + r0], w.b2[mb * 16 + r0 + 1], w.b2[mb * 16 + r0 + 2], w.b2[mb * 16 + r0 + 3]};
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) c = mfma16(a_frag<kNH>(w.W2, mb, kb, lane), t.h1b[kb], c);
+        for (int kb = 0; kb < 4; kb++) {
+            const bf16x4 a = a_frag<kNH>(w.W2, mb, kb, lane);
+            c = mfma16(a, t.h1b[kb], c);
+            c = mfma16(a, t.h1l[kb], c);
+        }
         t.z2[mb] = c;
-        t.h2b[mb] = pack_bf16(gelu_f(c[0]), gelu_f(c[1]), gelu_f(c[2]), gelu_f(c[3]));
+        const float g0 = gelu_f(c[0]), g1 = gelu_f(c[1]), g2 = gelu_f(c[2]), g3 = gelu_f(c[3]);
+        t.h2b[mb] = pack_bf16(g0, g1, g2, g3);
+        t.h2l[mb] = pack_bf16_lo(g0, g1, g2, g3);
     }
     f32x4 c = {w.b3[r0], w.b3[r0 + 1], w.b3[r0 + 2], w.b3[r0 + 3]};
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) c = mfma16(a_frag<kNH>(w.W3, 0, kb, lane), t.h2b[kb], c);
+    for (int kb = 0; kb < 4; kb++) {
+        const bf16x4 a = a_frag<kNH>(w.W3, 0, kb, lane);
+        c = mfma16(a, t.h2b[kb], c);
+        c = mfma16(a, t.h2l[kb], c);
+    }
     t.y = c;
 }
 

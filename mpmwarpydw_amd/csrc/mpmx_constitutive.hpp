@@ -107,72 +107,101 @@ MPMX_HD bool left_strain(const float Ft[9], float Bs[6], float& nb) {
     return det > 0.0f && nb <= 0.25f;
 }
 
-// Runs the update for particle p, which the caller has checked to be active
-// with materialLabel <= 1.  Writes F, stress, accumulated strain, damage, ys
-// and, on a phase transition, the label, velocity, XPBD birth state and zeroed
-// material.  Returns the stress in T (row-major) and whether the particle is
-// still MPM (label <= 1) afterwards.
-MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, float T[9]) {
-    const long n = P.n;
-    const int label = P.materialLabel[p];
+// State of one particle's update between its two phases: the strain
+// invariants and the branch (cu_begin), then the writes (cu_finish).
+struct CuState {
     float Ft[9];
-    ld9(P.F_trial, n, p, Ft);
+    bool fast;     // series route (eps_m in E6) or SVD route (eps, sv)
+    float E6[6];   // fast route: eps_m
+    float eps[3];  // SVD route: principal log strains
+    Svd3 sv;
+    float esum, devn, enorm2;
+    float mu, lam, ys, al, D;
+    int label;
+    int branch;  // 0 elastic, 1 plastic projection, 2 phase transition
+    float yield_eff, dgam, Dn;
+};
+
+// First phase of the update of particle p (active, materialLabel <= 1): reads
+// F_trial and the material, computes the strain invariants and decides the
+// branch.  Writes nothing.
+MPMX_HD void cu_begin(const mpmx_particles& P, long p, CuState& c) {
+    const long n = P.n;
+    c.label = P.materialLabel[p];
+    ld9(P.F_trial, n, p, c.Ft);
 
     float Bs[6], nb;
-    const bool fast = left_strain(Ft, Bs, nb);
+    c.fast = left_strain(c.Ft, Bs, nb);
 
-    float mu = P.mu[p], lam = P.lam[p], ys = P.ys[p], al = P.alpha[p], D = P.damage[p];
+    c.mu = P.mu[p];
+    c.lam = P.lam[p];
+    c.ys = P.ys[p];
+    c.al = P.alpha[p];
+    c.D = P.damage[p];
 
     // strain invariants: trace, |dev eps| and |eps|^2
-    Svd3 sv;
-    float E6[6];     // fast route: eps_m
-    float eps[3];    // SVD route: principal log strains
-    float esum, devn, enorm2;
-    if (fast) {
+    if (c.fast) {
         float L[6];
         sym_log1p(Bs, nb, L);
 #pragma unroll
-        for (int q = 0; q < 6; q++) E6[q] = 0.5f * L[q];
-        esum = E6[0] + E6[1] + E6[2];
-        const float m3 = esum / 3.0f;
-        const float a = E6[0] - m3, b = E6[1] - m3, c = E6[2] - m3;
-        const float off = E6[3] * E6[3] + E6[4] * E6[4] + E6[5] * E6[5];
-        devn = sqrtf(a * a + b * b + c * c + 2.0f * off);
-        enorm2 = E6[0] * E6[0] + E6[1] * E6[1] + E6[2] * E6[2] + 2.0f * off;
+        for (int q = 0; q < 6; q++) c.E6[q] = 0.5f * L[q];
+        c.esum = c.E6[0] + c.E6[1] + c.E6[2];
+        const float m3 = c.esum / 3.0f;
+        const float a = c.E6[0] - m3, b = c.E6[1] - m3, d = c.E6[2] - m3;
+        const float off = c.E6[3] * c.E6[3] + c.E6[4] * c.E6[4] + c.E6[5] * c.E6[5];
+        c.devn = sqrtf(a * a + b * b + d * d + 2.0f * off);
+        c.enorm2 = c.E6[0] * c.E6[0] + c.E6[1] * c.E6[1] + c.E6[2] * c.E6[2] + 2.0f * off;
     } else {
-        svd3(Ft, sv);
+        svd3(c.Ft, c.sv);
 #pragma unroll
-        for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
-        esum = eps[0] + eps[1] + eps[2];
-        const float m3 = esum / 3.0f;
-        const float a = eps[0] - m3, b = eps[1] - m3, c = eps[2] - m3;
-        devn = sqrtf(a * a + b * b + c * c);
-        enorm2 = eps[0] * eps[0] + eps[1] * eps[1] + eps[2] * eps[2];
+        for (int k = 0; k < 3; k++) c.eps[k] = log_clamped(c.sv.s[k], c.sv.lam[k], 0.01f);
+        c.esum = c.eps[0] + c.eps[1] + c.eps[2];
+        const float m3 = c.esum / 3.0f;
+        const float a = c.eps[0] - m3, b = c.eps[1] - m3, d = c.eps[2] - m3;
+        c.devn = sqrtf(a * a + b * b + d * d);
+        c.enorm2 = c.eps[0] * c.eps[0] + c.eps[1] * c.eps[1] + c.eps[2] * c.eps[2];
     }
     // mean Kirchhoff stress and its deviatoric norm: tau_k = 2 mu eps_k + lam sum(eps)
-    const float p_stress = (2.0f * mu / 3.0f + lam) * esum;
-    const float sigma_eq = 2.0f * mu * devn;
-    const float yield_eff = (1.0f - D) * (ys - al * p_stress);
+    const float p_stress = (2.0f * c.mu / 3.0f + c.lam) * c.esum;
+    const float sigma_eq = 2.0f * c.mu * c.devn;
+    c.yield_eff = (1.0f - c.D) * (c.ys - c.al * p_stress);
 
-    int branch = 0;  // 0 elastic, 1 plastic projection, 2 phase transition
-    float r = 0.0f, volc = 0.0f;
-    if (sigma_eq > yield_eff || ys < 0.0f || yield_eff < 0.0f) {
-        const float edn = devn + 1.0e-6f;
-        const float dgam = edn - yield_eff / (2.0f * mu);
-        const float dep = sqrtf(2.0f / 3.0f) * dgam;
-        P.acc_strain[p] = P.acc_strain[p] + dep;
+    c.branch = 0;
+    c.dgam = 0.0f;
+    c.Dn = c.D;
+    if (sigma_eq > c.yield_eff || c.ys < 0.0f || c.yield_eff < 0.0f) {
+        const float edn = c.devn + 1.0e-6f;
+        c.dgam = edn - c.yield_eff / (2.0f * c.mu);
+        const float dep = sqrtf(2.0f / 3.0f) * c.dgam;
         const float sc = P.strainCriteria[p];
-        float Dn = (sc > 0.0f) ? wmin(1.0f, D + dep / sc) : 1.0f;
-        if (ys < 0.0f || yield_eff < 0.0f) Dn = 1.0f;
-        P.damage[p] = Dn;
-        if (Dn >= 1.0f && label == 1) {
+        c.Dn = (sc > 0.0f) ? wmin(1.0f, c.D + dep / sc) : 1.0f;
+        if (c.ys < 0.0f || c.yield_eff < 0.0f) c.Dn = 1.0f;
+        c.branch = (c.Dn >= 1.0f && c.label == 1) ? 2 : 1;
+    }
+}
+
+// Second phase: writes F, stress, accumulated strain, damage, ys and, on a
+// phase transition, the label, velocity, XPBD birth state and zeroed material.
+// With e_proj (plastic branch only; c.sv must hold the SVD of F_trial) the
+// principal projected log strains are e_proj instead of the reference's
+// projection of the principal trial strains.  Returns the stress in T
+// (row-major) and whether the particle is still MPM (label <= 1) afterwards.
+MPMX_HD bool cu_finish(const mpmx_particles& P, long p, float eff, CuState& c, float T[9],
+                       const float* e_proj = nullptr) {
+    const long n = P.n;
+    float mu = c.mu, lam = c.lam;
+    const int branch = c.branch;
+    float r = 0.0f, volc = 0.0f;
+    if (branch != 0) {
+        P.acc_strain[p] = P.acc_strain[p] + sqrtf(2.0f / 3.0f) * c.dgam;
+        P.damage[p] = c.Dn;
+        if (branch == 2) {
             // phase transition to XPBD (mpmRoutines.py:379-408); u = tau : eps / 2
-            branch = 2;
             P.materialLabel[p] = 2;
             f3 xp = ld3(P.x, n, p);
             st3(P.x_init_xpbd, n, p, xp);
             const float rho = P.density[p];
-            const float u = mu * enorm2 + 0.5f * lam * esum * esum;
+            const float u = mu * c.enorm2 + 0.5f * lam * c.esum * c.esum;
             const float v_expected = sqrtf(2.0f * eff * u / rho);
             f3 vp = ld3(P.v, n, p);
             f3 vdir = normalize3(f3{vp.x + 1.0e-12f, vp.y + 1.0e-12f, vp.z + 1.0e-12f});
@@ -186,16 +215,19 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
             mu = 0.0f;
             lam = 0.0f;
         } else {
-            branch = 1;
-            r = dgam / edn;
-            volc = (0.3f * al) * dgam;
-            P.ys[p] = wmax(0.0f, ys + 2.0f * mu * (P.hardening[p] - P.softening[p]) * dgam);
+            const float edn = c.devn + 1.0e-6f;
+            r = c.dgam / edn;
+            volc = (0.3f * c.al) * c.dgam;
+            P.ys[p] = wmax(0.0f, c.ys + 2.0f * mu * (P.hardening[p] - P.softening[p]) * c.dgam);
         }
     }
 
     // ---- deformation gradient and Kirchhoff stress ----
+    const bool proj = e_proj && branch == 1;
     float tau[6];
-    if (fast) {
+    if (c.fast && !proj) {
+        float* E6 = c.E6;
+        float esum = c.esum;
         if (branch == 1) {
             const float m3 = esum / 3.0f;
             float S[6] = {-r * (E6[0] - m3), -r * (E6[1] - m3), -r * (E6[2] - m3), -r * E6[3], -r * E6[4], -r * E6[5]};
@@ -203,6 +235,7 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
             sym_exp(S, sym_norm(S), X);
             const float ec = expf(volc);
             const float Xf[9] = {X[0], X[3], X[4], X[3], X[1], X[5], X[4], X[5], X[2]};
+            const float* Ft = c.Ft;
             float Fo[9];
 #pragma unroll
             for (int i = 0; i < 3; i++)
@@ -216,7 +249,7 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
             E6[0] += volc; E6[1] += volc; E6[2] += volc;
             esum += 3.0f * volc;
         } else {
-            st9(P.F, n, p, Ft);
+            st9(P.F, n, p, c.Ft);
         }
         const float ls = lam * esum;
         tau[0] = 2.0f * mu * E6[0] + ls;
@@ -226,12 +259,18 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
         tau[4] = 2.0f * mu * E6[4];
         tau[5] = 2.0f * mu * E6[5];
     } else {
+        const Svd3& sv = c.sv;
         float d[3];
         if (branch == 1) {
             float epsn[3], e[3], l[3];
-            const float mean_eps = esum / 3.0f;
+            if (proj) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) epsn[k] = (eps[k] - r * (eps[k] - mean_eps)) + volc;
+                for (int k = 0; k < 3; k++) epsn[k] = e_proj[k];
+            } else {
+                const float mean_eps = c.esum / 3.0f;
+#pragma unroll
+                for (int k = 0; k < 3; k++) epsn[k] = (c.eps[k] - r * (c.eps[k] - mean_eps)) + volc;
+            }
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 e[k] = expf(epsn[k]);
@@ -261,7 +300,7 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
                 const float dk = 2.0f * mu * l[k] + lam * ls;
                 d[k] = (sv.s[k] >= 1.0e-6f) ? dk : dk * (sv.s[k] / 1.0e-6f);
             }
-            st9(P.F, n, p, Ft);
+            st9(P.F, n, p, c.Ft);
         }
         const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
@@ -297,6 +336,14 @@ MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, flo
     T[5] = T[7] = tau[5] + es * s12;
     st9(P.stress, n, p, T);
     return branch != 2;
+}
+
+// The whole update of particle p, which the caller has checked to be active
+// with materialLabel <= 1 (see cu_finish for what it writes).
+MPMX_HD bool constitutive_update(const mpmx_particles& P, long p, float eff, float T[9]) {
+    CuState c;
+    cu_begin(P, p, c);
+    return cu_finish(P, p, eff, c, T);
 }
 
 }  // namespace mpmx

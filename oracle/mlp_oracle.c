@@ -7,10 +7,10 @@
  * W3[3][64], b3[3] (row-major, out x in).
  *
  * With quant = 1 the oracle rounds to bfloat16 (round to nearest even) at the
- * points where the GPU kernels feed their MFMA units: the weights, the hidden
- * activations and, in the backward pass, the back-propagated gradients; the
- * inputs enter as the sum of two bfloat16 values, hi = bf16(x) and
- * lo = bf16(x - hi) (two MFMA passes), which keeps 16 bits of each feature.
+ * points where the GPU kernels feed their MFMA units: the weights and, in the
+ * backward pass, the back-propagated gradients; the inputs and the hidden
+ * activations enter as the sum of two bfloat16 values, hi = bf16(x) and
+ * lo = bf16(x - hi) (two MFMA passes), which keeps 16 bits of each value.
  * Sums and everything else stay in REAL.  The loss is the mean squared error
  * over all B*3 outputs.
  */
@@ -92,13 +92,13 @@ static void forward1(const struct Params *P, const R *x, R *z1, R *h1, R *z2, R 
         R s = 0;
         for (int i = 0; i < NI; i++) s += q(P->W1[o * NI + i], quant) * xq[i];
         z1[o] = s + P->b1[o];
-        h1[o] = q(gelu(z1[o]), quant);
+        h1[o] = qin(gelu(z1[o]), quant);
     }
     for (int o = 0; o < NH; o++) {
         R s = 0;
         for (int i = 0; i < NH; i++) s += q(P->W2[o * NH + i], quant) * h1[i];
         z2[o] = s + P->b2[o];
-        h2[o] = q(gelu(z2[o]), quant);
+        h2[o] = qin(gelu(z2[o]), quant);
     }
     for (int o = 0; o < NO; o++) {
         R s = 0;
@@ -194,7 +194,9 @@ EXPORT void o_adam_step(int n, R *params, const R *grad, R *m, R *v, R lr, R bet
 /*                                edn keeps: s = Y / (2 mu edn)),            */
 /*   alpha                       (the Drucker-Prager cone coefficient, which */
 /*                                sets the dilatancy).                       */
-/* The network predicts y = (s, alpha, 0) from 10 features: 0.1 ln(phi+kap), */
+/* The network predicts y = (s, 20 (alpha - 0.3), 0) (alpha centred and     */
+/* scaled, so that the loss weighs its error as the dilatancy needs) from 10 */
+/* features: 0.1 ln(phi+kap),                                                */
 /* 0.1 ln(max(y0, 1e-12)), 0.1 ln kap, d/|d| (3), tan(friction angle),       */
 /* 4 nu - 1, log10 E - 8, damage.  The tolerance of the judge then asks for  */
 /* kap |ds| and 0.9 (phi + kap) |dalpha| (2 mu / K) below tol: loose where   */
@@ -267,7 +269,7 @@ EXPORT void o_lrm_features(int n, const R *e, const R *mu, const R *lam, const R
     }
 }
 
-/* exact targets (s, alpha, 0) of plastic states (zeros for elastic ones) and
+/* exact targets (s, 20 (alpha - 0.3), 0) of plastic states (zeros for elastic ones) and
  * the plastic flag */
 EXPORT void o_lrm_targets(int n, const R *e, const R *mu, const R *lam, const R *ys, const R *alpha,
                           const R *damage, R *T, int *plastic) {
@@ -280,13 +282,13 @@ EXPORT void o_lrm_targets(int n, const R *e, const R *mu, const R *lam, const R 
         tp[0] = tp[1] = tp[2] = 0;
         if (plastic[p]) {
             tp[0] = t.Y / ((R)2 * mu[p] * (t.dn + (R)1e-6));
-            tp[1] = alpha[p];
+            tp[1] = (R)20 * (alpha[p] - (R)0.3);
         }
     }
 }
 
-/* the prediction e0' - eref kap y0 d/|d| + 0.3 y1 eref (phi + kap) from
- * network outputs y */
+/* the prediction e0' - eref kap y0 d/|d| + 0.3 (0.3 + y1 / 20) eref (phi + kap)
+ * from network outputs y */
 EXPORT void o_lrm_apply(int n, const R *e, const R *mu, const R *lam, const R *ys, const R *alpha,
                         const R *damage, const R *Y, R *e_pred) {
     for (int p = 0; p < n; p++) {
@@ -300,7 +302,7 @@ EXPORT void o_lrm_apply(int n, const R *e, const R *mu, const R *lam, const R *y
         const R *y = Y + 3L * p;
         for (int k = 0; k < 3; k++) {
             R dh = t.dn > 0 ? t.d[k] / t.dn : (R)0;
-            e_pred[3L * p + k] = e0[k] - er * kap * y[0] * dh + (R)0.3 * y[1] * er * (phi + kap);
+            e_pred[3L * p + k] = e0[k] - er * kap * y[0] * dh + (R)0.3 * ((R)0.3 + y[1] / (R)20) * er * (phi + kap);
         }
     }
 }

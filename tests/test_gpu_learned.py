@@ -109,11 +109,10 @@ def test_harvest_matches_oracle(mo):
     assert np.all(np.abs(np.exp(10 * X[:, 1]) - np.exp(10 * Xo[:, 1])) <= 2e-6 * y_scale + 1e-12)
     assert np.allclose(np.exp(10 * X[:, 2]), np.exp(10 * Xo[:, 2]), rtol=2e-6)
     assert np.abs(X[:, 3:] - Xo[:, 3:]).max() < 1e-5 * np.maximum(1.0, np.abs(Xo[:, 3:])).max()
-    # targets: s = Y / (2 mu edn) to the rounding of Y, alpha exactly
+    # targets: s = Y / (2 mu edn) to the rounding of Y, 20 (alpha - 0.3) to FP32 rounding
     assert np.all(np.abs(T[:, 0] - To[:, 0]) <= 2e-6 * y_scale * c / (2 * mu * (dn + 1e-6)) + 1e-6)
-    assert np.array_equal(T[:, 1], To[:, 1]) and np.all(T[:, 2] == 0)
+    assert np.abs(T[:, 1] - To[:, 1]).max() < 1e-6 and np.all(T[:, 2] == 0)
     assert plastic.sum() > 100 and (~plastic).sum() > 5
-    assert np.array_equal(T[:, 1] != 0, pl)
 
 
 def _run_stress(S, net, use_net, tol=1e-4):
@@ -206,7 +205,7 @@ def _report_prediction_errors(S, net, tag):
     nu = (X[:, 7] + 1) / 4
     r = (1 - 2 * nu) / (1 + nu) * 3  # 2 mu / K
     dev = kap * np.abs(Y[:, 0] - T[:, 0]) / 1e-4
-    vol = 0.9 * pk * np.abs(Y[:, 1] - T[:, 1]) / r / 1e-4
+    vol = 0.9 * pk * np.abs(Y[:, 1] - T[:, 1]) / 20 / r / 1e-4
     bad = (dev > 1) | (vol > 1)
     if bad.any():
         q = lambda a: "/".join(f"{v:.3g}" for v in np.quantile(a, [0.05, 0.5, 0.95]))
@@ -335,5 +334,22 @@ def test_c4_ensemble_learned_matches_exact(block_net):
         t.copy_(t0)
     for _ in range(20):
         E.substep(learned=block_net, use_net=False)
-    drift = ((sl - E.mpm.particle_stress.t).abs().amax(0) / torch.clamp(ys, min=1.0)).amax().item()
-    print(f"[c4] 20-substep free run: max stress difference {drift:.2e} x cohesion", flush=True)
+    drift = ((sl - E.mpm.particle_stress.t).abs().amax(0) / torch.clamp(ys, min=1.0)).cpu().numpy()
+    print(f"[c4] 20-substep free run: stress difference x cohesion max {drift.max():.2e}, 99.99% "
+          f"{np.quantile(drift, 0.9999):.2e}, median {np.median(drift):.2e}; above 1e-3: {(drift > 1e-3).sum()} "
+          f"of {drift.size}", flush=True)
+    # control: the plain substep from the same state with every velocity moved by
+    # 1e-6 of itself (random sign), i.e. how far the flow itself carries a tiny
+    # perturbation in 20 substeps
+    sp = E.mpm.particle_stress.t.clone()
+    for t, t0 in zip(tensors, snap):
+        t.copy_(t0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    v = E.sim.particle_v.t
+    v.mul_(1.0 + 1e-6 * (2 * torch.randint(0, 2, v.shape, device="cuda", generator=g) - 1))
+    for _ in range(20):
+        E.substep()
+    ctl = ((sp - E.mpm.particle_stress.t).abs().amax(0) / torch.clamp(ys, min=1.0)).cpu().numpy()
+    print(f"[c4] control, velocities moved by 1e-6: stress difference x cohesion max {ctl.max():.2e}, 99.99% "
+          f"{np.quantile(ctl, 0.9999):.2e}; above 1e-3: {(ctl > 1e-3).sum()} of {ctl.size}", flush=True)
