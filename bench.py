@@ -84,7 +84,12 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of each CPU baseline sample")
     p.add_argument("--no-error-sample", action="store_true",
                    help="skip the FP64 check of the constitutive update after the timed region")
-    p.add_argument("--ensemble-scenes", type=int, default=64, help="scenes of the batched ensemble (--scene ensemble)")
+    p.add_argument("--ensemble-scenes", type=int, default=64, help="scenes of the batched ensemble")
+    p.add_argument("--ensemble-steps", type=int, default=1000, help="substeps of each batched-ensemble run")
+    p.add_argument("--no-learned", action="store_true", help="skip the learned+batched measurement")
+    p.add_argument("--learned-tol", type=float, default=5e-5,
+                   help="judge tolerance of the learned map (residuals against tol * cohesion)")
+    p.add_argument("--no-autotune", action="store_true", help="default knobs instead of the autotuner's")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the 256 MiB copy that calibrates the HBM byte counters")
     return p.parse_args()
@@ -200,14 +205,21 @@ def cpu_baseline(S, H0, budget_s):
             "single_core_sample": f"substeps 1-{steps1}, serial FP32 build on one core, {el1:.1f} s"}
 
 
-def stress_error_sample(S, eff):
-    """The constitutive update of one substep against the FP64 oracle on every
-    active MPM particle, from the device state it starts from: per particle
-    |tau_gpu - tau_64| / max(|tau_64|, ys_64, 1 Pa) (largest component)."""
+def stress_error_sample(S, eff, step=None, sample=None, seed=0, what="constitutive update of one substep after "
+                        "the timed run, every active MPM particle"):
+    """The constitutive update of one substep (``step()``, by default the fused
+    substep of S) against the FP64 oracle on every active MPM particle (or on
+    ``sample`` particles drawn at random), from the device state it starts
+    from: per particle |tau_gpu - tau_64| / max(|tau_64|, ys_64, 1 Pa) (largest
+    component)."""
     from mpmwarpydw_amd import simulationRoutines as SR
     from oracle import Oracle
     sim, mpm, xpbd = S.sim, S.mpm, S.xpbd
-    g = lambda a: np.ascontiguousarray(a.numpy().astype(np.float64))  # noqa: E731
+    idx = None
+    if sample is not None and sample < sim.nPoints:
+        idx = np.sort(np.random.default_rng(seed).choice(sim.nPoints, sample, replace=False))
+    pick = (lambda a: a) if idx is None else (lambda a: a[idx])  # noqa: E731
+    g = lambda a: np.ascontiguousarray(pick(a.numpy()).astype(np.float64))  # noqa: E731
     H = {k: g(getattr(sim, k)) for k in ("particle_x", "particle_v", "particle_density")}
     for k in ("particle_F", "particle_F_trial", "particle_stress", "particle_accumulated_strain", "particle_damage",
               "particle_C", "mu", "lam", "ys", "alpha", "hardening", "softening", "eta_shear", "eta_bulk",
@@ -215,14 +227,17 @@ def stress_error_sample(S, eff):
         H[k] = g(getattr(mpm, k))
     for k in ("particle_x_initial_xpbd", "particle_v_initial_xpbd"):
         H[k] = g(getattr(xpbd, k))
-    act = np.ascontiguousarray(sim.activeLabel.numpy())
-    lab = np.ascontiguousarray(sim.materialLabel.numpy())
+    act = np.ascontiguousarray(pick(sim.activeLabel.numpy()))
+    lab = np.ascontiguousarray(pick(sim.materialLabel.numpy()))
     sel = (act == 1) & (lab <= 1)
     H0_Ft = H["particle_F_trial"].copy()
     mu0, lam0, ys0, al0, D0 = (H[k].copy() for k in ("mu", "lam", "ys", "alpha", "particle_damage"))
-    SR.mpmSimulationStepFused(sim, mpm, xpbd)
-    tau_gpu = mpm.particle_stress.numpy().astype(np.float64)
-    lab_gpu = sim.materialLabel.numpy()
+    if step is None:
+        SR.mpmSimulationStepFused(sim, mpm, xpbd)
+    else:
+        step()
+    tau_gpu = pick(mpm.particle_stress.numpy()).astype(np.float64)
+    lab_gpu = pick(sim.materialLabel.numpy())
     t0 = time.perf_counter()
     acc0 = H["particle_accumulated_strain"].copy()
     Oracle(64).compute_stress_from_F_trial(act, lab, H["particle_x"], H["particle_v"], H["particle_x_initial_xpbd"],
@@ -246,7 +261,7 @@ def stress_error_sample(S, eff):
             "plastic_in_sample": int(plastic.sum()), "labels_identical": bool(np.array_equal(lab_gpu, lab)),
             "marginal_particles": int(marginal.sum()),
             "marginal_max": float(err[marginal].max()) if marginal.any() else 0.0,
-            "definition": "constitutive update of one substep after the timed run, every active MPM particle, "
+            "definition": f"{what}, "
                           "against the FP64 oracle: max over particles of |tau - tau_fp64|_max / "
                           "max(|tau_fp64|_max, ys, 1 Pa); 'marginal' particles, whose FP64 yield margin "
                           f"|sigma_eq - Y| / |Y| is below {MARGIN:g}, are counted apart: there the FP32 yield test "
@@ -357,12 +372,31 @@ def main():
     from mpmwarpydw_amd._lib import lib
     from mpmwarpydw_amd.runtime import stream_handle
 
+    from mpmwarpydw_amd import autotune
     dom, ov = scenes.SCENES[a.scene](scale=a.scale)
     args = getArgs.get_args([], ov)
     S = driver.setup(args, dom)
     sim, mpm = S.sim, S.mpm
-    if not a.no_reorder:
-        S.reorder()  # part of setup: tile order in HBM (results are the same bits either way)
+    tuning = None
+    if not a.no_autotune:
+        # launch shape and particle order (part of setup): every candidate timed over substeps 1-23 from the
+        # t = 0 state, which the tuner restores before each candidate and leaves in the order it keeps
+        tc = [1]
+
+        def tstep():
+            S.substep(tc[0])
+            tc[0] += 1
+
+        with torch.cuda.stream(torch.cuda.Stream()):
+            tuning = autotune.tune(S, tstep, orders=(False,) if a.no_reorder else (False, True))
+        torch.cuda.synchronize()
+        print(f"[bench] autotune: {tuning['best']} ({len(tuning['candidates'])} candidates, "
+              f"{tuning['tuning_s']} s)", file=sys.stderr, flush=True)
+        reorder = tuning["kept_reordered"]
+    else:
+        reorder = not a.no_reorder
+        if reorder:
+            S.reorder()  # part of setup: tile order in HBM (results are the same bits either way)
     n = sim.nPoints
     ratio = sim.mpmStepsPerXpbdStep
     periods = max(1, a.steps // ratio)
@@ -471,7 +505,8 @@ def main():
                                f"{mpm.gridDims[2]}, dt {sim.dt:g} s, XPBD every {ratio} substeps, substeps 1-{steps} "
                                f"of the run from t = 0",
                    "particles": n, "grid_nodes": G, "hip_graph": graph is not None,
-                   "particle_order": "input order" if a.no_reorder else "tile order (reordered once at setup)"},
+                   "particle_order": "tile order (reordered once at setup)" if reorder else "input order",
+                   "knobs": autotune.current()},
         "scope": ("the timed region replays the substeps (MPM substep + XPBD step at its ratio) as HIP graphs with "
                   "no host synchronisation; the driver loop's per-substep readback is measured separately in "
                   "driver_loop"),
@@ -489,8 +524,17 @@ def main():
         "kernels_us_per_launch": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
         "driver_loop": loop,
     }
+    if tuning is not None:
+        out["autotune"] = {"best": tuning["best"], "kept_reordered": tuning["kept_reordered"],
+                           "candidates": tuning["candidates"], "tuning_s": tuning["tuning_s"],
+                           "tuned_on": "the scene itself at setup: each candidate over substeps 1-23 from the "
+                                       "t = 0 state (XPBD steps at their ratio), restored before each candidate"}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(S, H0, a.cpu_seconds)
+    if not a.no_learned:
+        del S
+        torch.cuda.empty_cache()
+        out["learned_batched"] = bench_learned_batched(a)
     print(json.dumps(out), flush=True)
 
 
@@ -509,6 +553,171 @@ def snapshot(S):
 def restore(S, snap):
     for arr, t in snap:
         arr.t.copy_(t)
+
+
+def train_block_network(log=None):
+    """The learned return map used by the batched measurement: trained on the
+    GPU (the library's own forward, backward and Adam kernels) on 10M plastic
+    trial states harvested from 32,768-particle block-collapse runs of 500
+    substeps whose materials span the ensemble's ranges."""
+    import torch
+    from mpmwarpydw_amd.learned import LearnedReturnMap, harvest_blocks
+    t0 = time.perf_counter()
+    X, T, stats = harvest_blocks(10_000_000, max_runs=100, substeps=500, seed=0, per_run=400_000, every=5)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    net = LearnedReturnMap(seed=5)
+    hist = net.train(X, T, steps=30000, batch=8192, lr=3e-3, lr_final=2e-5, seed=1, log_every=10000)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    info = {"pairs": stats["pairs"], "runs": len(stats["runs"]),
+            "runs_with_plastic_states": sum(1 for r in stats["runs"] if r["pairs"]), "harvest_s": round(t1 - t0, 2),
+            "train_steps": 30000, "batch": 8192, "train_s": round(t2 - t1, 2), "final_loss": hist[-1],
+            "what": "10M plastic trial states from 32,768-particle block-collapse runs of 500 substeps, materials "
+                    "drawn from the ensemble's ranges; Adam on the library's bfloat16 MFMA forward/backward"}
+    if log:
+        log(f"[bench] network: {info}")
+    del X, T
+    return net, info
+
+
+def _ensemble_graph(E, stream, per_graph, learned=None, counters=None, tol=1e-4):
+    """A HIP graph of ``per_graph`` ensemble substeps (plain, or with the
+    learned map), captured after one eager substep (workspace and bins
+    exist).  Replays use the arrays as they are now: nothing may reallocate
+    them afterwards."""
+    import torch
+
+    def sub():
+        if learned is None:
+            E.substep()
+        else:
+            E.substep(learned=learned, counters=counters, tol=tol)
+
+    with torch.cuda.stream(stream):
+        sub()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        for _ in range(per_graph):
+            sub()
+    return graph
+
+
+def bench_learned_batched(a, net=None, net_info=None):
+    """The batched ensemble (64 block scenes of 20k-500k particles, seeded
+    materials) for ``a.ensemble_steps`` substeps from t = 0, once with the plain
+    constitutive update and once with the learned map (judged by the exact
+    map), each replayed from HIP graphs of 10
+    substeps; launch shape and particle order from the autotuner on the
+    ensemble's own state.  Then one learned substep is checked against the
+    FP64 oracle on 2M particles drawn at random.  The judge's tolerance
+    (a.learned_tol, 5e-5 of the cohesion on each residual) keeps the learned
+    stresses within 1e-4 of the FP64 ones, the plain method's bound."""
+    import torch
+    from mpmwarpydw_amd import autotune, driver, getArgs
+    from mpmwarpydw_amd.ensemble import Ensemble, c4_scene_list
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if net is None:
+        net, net_info = train_block_network(log)
+    t0 = time.perf_counter()
+    specs = c4_scene_list(count=a.ensemble_scenes)
+    E = Ensemble([driver.setup(getArgs.get_args([], ov), dom) for dom, ov in specs])
+    setup_s = time.perf_counter() - t0
+    n, G = E.n, E.node_offsets[-1]
+    log(f"[bench] ensemble: {len(E.table)} scenes, {n} particles, {G} nodes, setup {setup_s:.1f} s")
+    stream = torch.cuda.Stream()
+    knobs_before = autotune.current()
+    state = lambda: E._state_arrays() + [E.grid_m, E.grid_v_in, E.grid_v_out]  # noqa: E731
+    # the tuner times every candidate from the t = 0 state and leaves it there, in the order it keeps
+    # (reordering allocates new arrays: it happens before any graph capture)
+    with torch.cuda.stream(stream):
+        tuning = autotune.tune(E, lambda: E.substep(learned=net, tol=a.learned_tol), split_forced=True)
+    torch.cuda.synchronize()
+    log(f"[bench] ensemble autotune: {tuning['best']}")
+    with torch.cuda.stream(stream):
+        E.bin()
+    torch.cuda.synchronize()
+    snap = [(arr, arr.t.clone()) for arr in state()]
+    per_graph = 10
+    reps = max(1, a.ensemble_steps // per_graph)
+    steps = reps * per_graph
+    out = {}
+    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
+    for form in ("plain", "learned"):
+        g = _ensemble_graph(E, stream, per_graph, learned=None if form == "plain" else net, counters=cnt,
+                            tol=a.learned_tol)
+        for arr, t in snap:  # back to t = 0 in place: the graph's pointers stay valid
+            arr.t.copy_(t)
+        with torch.cuda.stream(stream):
+            E.bin()
+        cnt.zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for _ in range(reps):
+                g.replay()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        lab, act = E.sim.materialLabel.numpy(), E.sim.activeLabel.numpy()
+        n_mpm = int(((act == 1) & (lab <= 1)).sum())
+        out[form] = {"value": n_mpm * steps / wall, "ms_per_step": wall * 1e3 / steps, "mpm_particles_at_end": n_mpm}
+        if form == "learned":
+            c = cnt.cpu().numpy()  # plastic states and accepted predictions over the run
+            out[form].update({"plastic_states": int(c[0]), "accepted": int(c[1]),
+                              "accepted_fraction": float(c[1] / max(c[0], 1))})
+        log(f"[bench] ensemble {form}: {out[form]}")
+        del g
+    del snap
+    # per-kernel device time of one eager substep of each form (dispatch-stamped events)
+    from mpmwarpydw_amd._lib import lib
+    from mpmwarpydw_amd.runtime import stream_handle
+    kernels = {}
+    for form in ("plain", "learned"):
+        with torch.cuda.stream(stream):
+            lib("mpmx_profile_hold", stream_handle(), PROFILE_HOLD_US)
+            for _ in range(2):
+                E.substep() if form == "plain" else E.substep(learned=net, tol=a.learned_tol)
+            lib("mpmx_profile_begin", 4096)
+            for _ in range(2):
+                E.substep() if form == "plain" else E.substep(learned=net, tol=a.learned_tol)
+        torch.cuda.synchronize()
+        lib("mpmx_profile_end")
+        kernels[form] = {k: round(v["avg_us"], 2) for k, v in profile_report().items()}
+    log(f"[bench] ensemble kernels: {kernels}")
+    # matched error: one more learned substep against the FP64 oracle on a random sample
+    err = stress_error_sample(E, E.mpm.eff, step=lambda: E.substep(learned=net, tol=a.learned_tol),
+                              sample=2_000_000, seed=7,
+                              what=f"constitutive update (learned map, judge tolerance {a.learned_tol:g} of the "
+                                   f"cohesion) of substep {steps + 1} of the batched ensemble, 2,000,000 particles "
+                                   "drawn at random")
+    autotune.apply(knobs_before)
+    res = {
+        "metric": "MPM particle-substeps/s (learned+batched)",
+        "value": out["learned"]["value"],
+        "unit": "particle-substeps/s",
+        "ms_per_step": out["learned"]["ms_per_step"],
+        "steps": steps,
+        "workload": f"batched ensemble: {len(E.table)} block-collapse scenes, {n} particles (log-uniform 20k-500k "
+                    f"per scene, friction 20-40 deg, cohesion 1e4-1e6 Pa, E 1e7-1e9 Pa, seeded), {G} grid nodes, "
+                    f"dt 1e-4 s, substeps 1-{steps} from t = 0, MPM substep (the XPBD step is per scene and not "
+                    "part of this measurement), HIP graphs of 10 substeps",
+        "learned": out["learned"],
+        "plain_batched": out["plain"],
+        "learned_vs_plain_batched": out["learned"]["value"] / out["plain"]["value"],
+        "judge_tolerance": a.learned_tol,
+        "stress_error_vs_fp64": err,
+        "kernels_us_per_launch": kernels,
+        "network": net_info,
+        "autotune": {"best": tuning["best"], "kept_reordered": tuning["kept_reordered"],
+                     "candidates": len(tuning["candidates"]), "tuning_s": tuning["tuning_s"],
+                     "tuned_on": "the learned ensemble substep, each candidate over substeps 1-23 from the "
+                                 "t = 0 state"},
+        "setup_s": round(setup_s, 2),
+    }
+    del E
+    torch.cuda.empty_cache()
+    return res
 
 
 def bench_ensemble(a):

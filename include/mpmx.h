@@ -698,20 +698,23 @@ int mpmx_lrm_harvest(const mpmx_particles* particles, float* X, float* T, float*
  * all) and judged by the exact map: a prediction whose yield-surface/flow
  * residual 2 mu |dev e' - Y/(2 mu edn) d| or dilatancy residual
  * K |tr e' - tr e - 0.9 alpha dgam| exceeds tol * max(cohesion, 1) is replaced
- * by the exact projection.  counters[3] (may be NULL) += {plastic states,
- * accepted predictions, particles updated}. */
+ * by the exact projection.  counters[3] (64-bit, may be NULL) += {plastic states,
+ * accepted predictions, particles updated}.  Workspace (with params):
+ * mpmx_lrm_workspace_bytes(n). */
+size_t mpmx_lrm_workspace_bytes(int n);
 int mpmx_compute_stress_learned(const mpmx_particles* particles, float eff, const float* params,
-                                float tol, int32_t* counters, void* stream);
+                                float tol, int64_t* counters, void* workspace, size_t ws_bytes,
+                                void* stream);
 /* mpmx_mpm_substep and mpmx_ensemble_substep with the constitutive update of
  * mpmx_compute_stress_learned (network mlp_params, tolerance tol, counters as
  * there) run as its own kernel before the P2G tile kernel. */
 int mpmx_mpm_substep_learned(const mpmx_particles* particles, const mpmx_grid* grid,
                              const mpmx_step_params* params, const float* mlp_params, float tol,
-                             int32_t* counters, void* workspace, size_t ws_bytes, void* stream);
+                             int64_t* counters, void* workspace, size_t ws_bytes, void* stream);
 int mpmx_ensemble_substep_learned(const mpmx_particles* particles, const mpmx_scene* scenes,
                                   int nscenes, float* grid_m, float* grid_v_in, float* grid_v_out,
                                   const mpmx_step_params* params, const float* mlp_params, float tol,
-                                  int32_t* counters, void* workspace, size_t ws_bytes, void* stream);
+                                  int64_t* counters, void* workspace, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -114,7 +114,8 @@ SIGNATURES = {
     "mpmx_mlp_loss_grad": [I, P, P, P, P, P, P, S, P],
     "mpmx_mlp_adam_step": [I, P, P, P, P, F, F, F, F, I, P],
     "mpmx_lrm_harvest": [P, P, P, P, P, P],
-    "mpmx_compute_stress_learned": [P, F, P, F, P, P],
+    "mpmx_lrm_workspace_bytes": [I],
+    "mpmx_compute_stress_learned": [P, F, P, F, P, P, S, P],
     "mpmx_mpm_substep_learned": [P, P, P, P, F, P, P, S, P],
     "mpmx_ensemble_substep_learned": [P, P, I, P, P, P, P, P, F, P, P, S, P],
 }
@@ -127,6 +128,7 @@ _RESTYPE = {
     "mpmx_hashgrid_workspace_bytes": S,
     "mpmx_hashgrid_pair_workspace_bytes": S,
     "mpmx_mlp_workspace_bytes": S,
+    "mpmx_lrm_workspace_bytes": S,
     "mpmx_ctx_stream": P,
     "mpmx_ensemble_workspace_bytes": S,
 }

@@ -59,40 +59,41 @@ MPMX_HD Principal principal_state(const float eps[3], float mu, float lam, float
     return s;
 }
 
-MPMX_HD void lrm_features(const Principal& s, float mu, float lam, float ys, float al, float acc, float D, float x[10]) {
-    (void)acc;
+// the 10 features of a plastic trial state from its invariants: |dev e| = dn,
+// the yield value Y and the unit principal deviator dh (descending)
+MPMX_HD void lrm_features(float dn, float Y, const float dh[3], float mu, float lam, float ys, float al, float D,
+                          float x[10]) {
     const float E = mu * (3.0f * lam + 2.0f * mu) / (lam + mu);
     const float nu = lam / (2.0f * (lam + mu));
     const float c = ys > 1.0f ? ys : 1.0f;
-    const float kap = 2.0f * mu * 1.0e-6f / c, y0 = s.Y / c;
-    const float pk = (2.0f * mu * s.dn - s.Y) / c + kap;
+    const float kap = 2.0f * mu * 1.0e-6f / c, y0 = Y / c;
+    const float pk = (2.0f * mu * dn - Y) / c + kap;
     x[0] = 0.1f * logf(pk > 1.0e-30f ? pk : 1.0e-30f);
     x[1] = 0.1f * logf(y0 > 1.0e-12f ? y0 : 1.0e-12f);
     x[2] = 0.1f * logf(kap);
-    for (int k = 0; k < 3; k++) x[3 + k] = s.dn > 0.0f ? s.d[k] / s.dn : 0.0f;
+    for (int k = 0; k < 3; k++) x[3 + k] = dh[k];
     x[6] = lrm_tan_of_alpha(al);
     x[7] = 4.0f * nu - 1.0f;
     x[8] = log10f(E) - 8.0f;
     x[9] = D;
 }
 
-// the radial return e0' = e - (1 - Y / (2 mu |d|)) d (e for |d| = 0)
-MPMX_HD void radial_return(const Principal& s, float mu, float e0[3]) {
-    const float r0 = s.dn > 0.0f ? 1.0f - s.Y / (2.0f * mu * s.dn) : 0.0f;
-    for (int k = 0; k < 3; k++) e0[k] = s.e[k] - r0 * s.d[k];
-}
-
-MPMX_HD bool lrm_judge(const Principal& s, const float q[3], float mu, float lam, float ys, float al, float tol) {
-    const float edn = s.dn + 1.0e-6f;
-    const float c = s.Y / (2.0f * mu * edn);
-    const float qm = (q[0] + q[1] + q[2]) / 3.0f;
-    const float r0 = q[0] - qm - c * s.d[0], r1 = q[1] - qm - c * s.d[1], r2 = q[2] - qm - c * s.d[2];
-    const float res1 = 2.0f * mu * sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
-    const float dgam = edn - s.Y / (2.0f * mu);
-    const float K = lam + 2.0f * mu / 3.0f;
-    const float res2 = K * fabsf((q[0] + q[1] + q[2]) - s.esum - 3.0f * 0.3f * al * dgam);
-    const float lim = tol * (ys > 1.0f ? ys : 1.0f);
-    return res1 <= lim && res2 <= lim;
+// Eigenvalues, descending, of the unit deviator A = dev(E) / |dev E| of a
+// symmetric E = {xx, yy, zz, xy, xz, yz} with trace esum and |dev E| = devn > 0
+// (trigonometric solution of the characteristic cubic, Smith 1961: with
+// J2 = 1/2 and cos 3t = 3 sqrt6 det A, the roots are sqrt(2/3) cos(t - 2 pi k / 3)).
+MPMX_HD void unit_deviator_eigenvalues(const float E[6], float esum, float devn, float dh[3]) {
+    const float m = esum / 3.0f, inv = 1.0f / devn;
+    const float a = (E[0] - m) * inv, b = (E[1] - m) * inv, c = (E[2] - m) * inv;
+    const float d = E[3] * inv, e = E[4] * inv, f = E[5] * inv;
+    const float det = a * (b * c - f * f) - d * (d * c - f * e) + e * (d * f - b * e);
+    float x = 7.348469228349534f * det;  // 3 sqrt6
+    x = x > 1.0f ? 1.0f : (x < -1.0f ? -1.0f : x);
+    const float t = acosf(x) / 3.0f;
+    const float r = 0.816496580927726f;  // sqrt(2/3)
+    dh[0] = r * cosf(t);
+    dh[2] = r * cosf(t + 2.0943951023931953f);
+    dh[1] = -dh[0] - dh[2];
 }
 
 // Harvest: features, exact targets (s, 20 (alpha - 0.3), 0) and the state of every
@@ -115,8 +116,9 @@ __global__ void lrm_harvest_kernel(mpmx_particles P, float* __restrict__ X, floa
     for (int k = 0; k < 3; k++) eps[k] = log_clamped(sv.s[k], sv.lam[k], 0.01f);
     const float mu = P.mu[p], lam = P.lam[p], ys = P.ys[p], al = P.alpha[p], D = P.damage[p];
     const Principal s = principal_state(eps, mu, lam, ys, al, D);
-    float x[10];
-    lrm_features(s, mu, lam, ys, al, P.acc_strain[p], D, x);
+    float x[10], dh[3];
+    for (int k = 0; k < 3; k++) dh[k] = s.dn > 0.0f ? s.d[k] / s.dn : 0.0f;
+    lrm_features(s.dn, s.Y, dh, mu, lam, ys, al, D, x);
     for (int k = 0; k < 10; k++) X[p * 10 + k] = x[k];
     Tg[p * 3 + 0] = s.plastic ? s.Y / (2.0f * mu * (s.dn + 1.0e-6f)) : 0.0f;
     Tg[p * 3 + 1] = s.plastic ? 20.0f * (al - 0.3f) : 0.0f;
@@ -125,100 +127,178 @@ __global__ void lrm_harvest_kernel(mpmx_particles P, float* __restrict__ X, floa
     flag[p] = s.plastic ? 1 : 0;
 }
 
-// One wavefront handles 64 consecutive particles per step.  Every particle
-// runs the plain update's first phase (cu_begin: invariants and branch, as
-// compute_stress_from_F_trial decides them).  Plastic particles that stay MPM
-// are the network's candidates: they take the SVD of F_trial (if the series
-// route skipped it), their principal state and features; the network runs on
-// the matrix cores for the wavefront's 4 tiles of 16 whenever one lane is a
-// candidate.  A judged prediction is the projection cu_finish uses; every other
-// particle finishes exactly as the plain update, bit for bit.  counters:
-// [0] += plastic states, [1] += accepted predictions, [2] += particles updated.
-__global__ void __launch_bounds__(kBlock) stress_learned_kernel(mpmx_particles P, float eff, const float* __restrict__ prm,
-                                                                float tol, int use_net, int32_t* __restrict__ counters) {
-    __shared__ MlpWeights w;
-    if (use_net) load_weights(prm, w);
-    __syncthreads();
+// a plastic state the network may project: stays MPM, Y >= 0 (not past the
+// cone's apex) and a deviator to scale
+MPMX_HD bool lrm_candidate(const CuState& c) { return c.branch == 1 && c.yield_eff >= 0.0f && c.devn > 0.0f; }
+
+// wave-level sums of the counters, one 64-bit atomic per wavefront and counter
+__device__ __forceinline__ void add_counters(int64_t* counters, int lane, int a, int b, int c) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+        c += __shfl_xor(c, o);
+    }
+    if (lane == 0 && counters) {
+        unsigned long long* cu = reinterpret_cast<unsigned long long*>(counters);
+        if (a) atomicAdd(&cu[0], (unsigned long long)a);
+        if (b) atomicAdd(&cu[1], (unsigned long long)b);
+        if (c) atomicAdd(&cu[2], (unsigned long long)c);
+    }
+}
+
+// Pass 1, the plain update for every particle that is not the network's: each
+// active MPM particle runs cu_begin (invariants and branch, as
+// compute_stress_from_F_trial decides them); with use_net the candidates are
+// flagged and left for pass 2, everything else finishes here.  Wavefronts
+// take 64 consecutive particles per round.  counters: [0] += plastic states,
+// [2] += particles updated.
+__global__ void __launch_bounds__(kBlock) stress_learned_pass1_kernel(mpmx_particles P, float eff, int use_net,
+                                                                      uint8_t* __restrict__ flag,
+                                                                      int64_t* __restrict__ counters) {
     const int lane = threadIdx.x & 63;
     const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long nwaves = (long)gridDim.x * 4;
-    int n_plastic = 0, n_accept = 0, n_done = 0;
-    for (long base = wave * 64; base < P.n; base += nwaves * 64) {  // uniform per wavefront
+    int n_plastic = 0, n_done = 0;
+    for (long base = wave * 64; base < P.n; base += nwaves * 64) {
         const long p = base + lane;
-        const bool mpm = p < P.n && P.activeLabel[p] == 1 && P.materialLabel[p] <= 1;
-        CuState c;
-        Principal s;
-        float eps[3] = {0.0f, 0.0f, 0.0f}, x[10];
-        for (int k = 0; k < 10; k++) x[k] = 0.0f;
-        bool cand = false;
-        if (mpm) {
+        if (p >= P.n) continue;
+        uint8_t f = 0;
+        if (P.activeLabel[p] == 1 && P.materialLabel[p] <= 1) {
+            CuState c;
             cu_begin(P, p, c);
-            if (use_net && c.branch == 1) {
-                if (c.fast) svd3(c.Ft, c.sv);
-                for (int k = 0; k < 3; k++) eps[k] = log_clamped(c.sv.s[k], c.sv.lam[k], 0.01f);
-                s = principal_state(eps, c.mu, c.lam, c.ys, c.al, c.D);
-                cand = s.plastic && s.Y >= 0.0f;
-                if (cand) lrm_features(s, c.mu, c.lam, c.ys, c.al, P.acc_strain[p], c.D, x);
+            if (use_net && lrm_candidate(c)) {
+                f = 1;
+            } else {
+                n_done++;
+                n_plastic += c.branch != 0 ? 1 : 0;
+                float T[9];
+                cu_finish(P, p, eff, c, T);
             }
         }
-        float yo[3] = {0.0f, 0.0f, 0.0f};
-        if (__ballot(cand)) {  // uniform: every lane runs the matrix-core tiles
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                // B fragment: lane l takes features 4(l/16)..+3 of particle t*16 + l%16
-                const int src = t * 16 + (lane & 15);
-                const int f0 = 4 * (lane >> 4);
-                float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int q = 0; q < 10; q++) {
-                    const float v = __shfl(x[q], src);
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        if (f0 + j == q) xv[j] = v;
-                }
-                MlpTile tt;
-                mlp_tile_forward(w, pack_bf16(xv[0], xv[1], xv[2], xv[3]), pack_bf16_lo(xv[0], xv[1], xv[2], xv[3]),
-                                 lane, tt);
-                // lanes 0..15 hold the outputs of particles t*16 + 0..15
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const float v = __shfl(tt.y[j], lane & 15);
-                    if ((lane >> 4) == t) yo[j] = v;
-                }
+        if (use_net) flag[p] = f;
+    }
+    add_counters(counters, lane, n_plastic, 0, n_done);
+}
+
+// The k <= 64 candidates q[0..k) of one wavefront, one per lane: features,
+// the network on the matrix cores (ceil(k / 16) tiles of 16), the judge and
+// the finish.  cu_begin runs twice per candidate (before and after the
+// tiles), so that no update state is live across them.
+__device__ __forceinline__ void lrm_process(const mpmx_particles& P, float eff, const MlpWeights& w, const int* q,
+                                            int k, int lane, float tol, int& n_plastic, int& n_accept, int& n_done) {
+    const bool act = lane < k;
+    const long p = act ? q[lane] : 0;
+    float x[10];
+    for (int j = 0; j < 10; j++) x[j] = 0.0f;
+    if (act) {
+        CuState c;
+        cu_begin(P, p, c);
+        if (lrm_candidate(c)) {
+            float dh[3];
+            if (c.fast) {
+                unit_deviator_eigenvalues(c.E6, c.esum, c.devn, dh);
+            } else {
+                const float m = c.esum / 3.0f;
+                for (int j = 0; j < 3; j++) dh[j] = (c.eps[j] - m) / c.devn;
             }
-        }
-        if (mpm) {
-            n_done++;
-            if (c.branch != 0) n_plastic++;
-            float q[3];
-            bool ok = false;
-            if (cand) {
-                const float mu = c.mu, ys = c.ys;
-                const float cc = ys > 1.0f ? ys : 1.0f;
-                const float er = cc / (2.0f * mu), kap = 2.0f * mu * 1.0e-6f / cc;
-                const float dil = 0.3f * (0.3f + 0.05f * yo[1]) * (er * ((2.0f * mu * s.dn - s.Y) / cc + kap));
-                const float shr = s.dn > 0.0f ? er * kap * yo[0] / s.dn : 0.0f;
-                radial_return(s, mu, q);
-                for (int k = 0; k < 3; k++) q[k] = (q[k] - shr * s.d[k]) + dil;
-                ok = lrm_judge(s, q, mu, c.lam, ys, c.al, tol);
-                n_accept += ok ? 1 : 0;
-            }
-            float T[9];
-            cu_finish(P, p, eff, c, T, ok ? q : nullptr);
+            lrm_features(c.devn, c.yield_eff, dh, c.mu, c.lam, c.ys, c.al, c.D, x);
         }
     }
-    // one atomic per wavefront and counter
+    float yo[3] = {0.0f, 0.0f, 0.0f};
+    const int ntiles = (k + 15) >> 4;  // uniform over the wavefront
+    for (int t = 0; t < ntiles; t++) {
+        // B fragment: lane l takes features 4(l/16)..+3 of candidate t*16 + l%16
+        const int src = t * 16 + (lane & 15);
+        const int f0 = 4 * (lane >> 4);
+        float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        n_plastic += __shfl_xor(n_plastic, o);
-        n_accept += __shfl_xor(n_accept, o);
-        n_done += __shfl_xor(n_done, o);
+        for (int j = 0; j < 10; j++) {
+            const float v = __shfl(x[j], src);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (f0 + i == j) xv[i] = v;
+        }
+        MlpTile tt;
+        mlp_tile_forward(w, pack_bf16(xv[0], xv[1], xv[2], xv[3]), pack_bf16_lo(xv[0], xv[1], xv[2], xv[3]), lane, tt);
+        // lanes 0..15 hold the outputs of candidates t*16 + 0..15
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float v = __shfl(tt.y[j], lane & 15);
+            if ((lane >> 4) == t) yo[j] = v;
+        }
     }
-    if (lane == 0 && counters) {
-        atomicAdd(&counters[0], n_plastic);
-        atomicAdd(&counters[1], n_accept);
-        atomicAdd(&counters[2], n_done);
+    if (!act) return;
+    CuState c;
+    cu_begin(P, p, c);
+    n_done++;
+    n_plastic += c.branch != 0 ? 1 : 0;
+    float T[9];
+    if (!lrm_candidate(c)) {  // pass 1 flagged it with the same state: not reached
+        cu_finish(P, p, eff, c, T);
+        return;
     }
+    // e' = e - r dev(e) + volc with the network's flow split and cone coefficient
+    const float mu = c.mu, Y = c.yield_eff, dn = c.devn;
+    const float cc = c.ys > 1.0f ? c.ys : 1.0f;
+    const float er = cc / (2.0f * mu), kap = 2.0f * mu * 1.0e-6f / cc;
+    float rv[2];
+    rv[0] = (1.0f - Y / (2.0f * mu * dn)) + er * kap * yo[0] / dn;
+    rv[1] = 0.3f * (0.3f + 0.05f * yo[1]) * (er * ((2.0f * mu * dn - Y) / cc + kap));
+    // the judge: dev e' against Y / (2 mu edn) dev e, tr e' against tr e + 0.9 alpha dgam
+    const float edn = dn + 1.0e-6f;
+    const float res1 = 2.0f * mu * fabsf((1.0f - rv[0]) - Y / (2.0f * mu * edn)) * dn;
+    const float K = c.lam + 2.0f * mu / 3.0f;
+    const float res2 = K * fabsf(3.0f * rv[1] - 0.9f * c.al * c.dgam);
+    const float lim = tol * cc;
+    const bool ok = res1 <= lim && res2 <= lim;
+    n_accept += ok ? 1 : 0;
+    cu_finish(P, p, eff, c, T, ok ? rv : nullptr);
+}
+
+// Pass 2, the flagged candidates: each wavefront scans its rounds of 64
+// particles and queues the flagged ones in LDS; every 64 queued (and the rest
+// at the end) go through lrm_process with one candidate per lane, so the
+// network's tiles are full.  counters: [0] += plastic, [1] += accepted,
+// [2] += updated.
+__global__ void __launch_bounds__(kBlock) stress_learned_pass2_kernel(mpmx_particles P, float eff,
+                                                                      const float* __restrict__ prm, float tol,
+                                                                      const uint8_t* __restrict__ flag,
+                                                                      int64_t* __restrict__ counters) {
+    __shared__ MlpWeights w;
+    __shared__ int queue[4][128];
+    load_weights(prm, w);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    int* q = queue[threadIdx.x >> 6];
+    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long nwaves = (long)gridDim.x * 4;
+    const long nchunks = ((long)P.n + 63) / 64;
+    int n_plastic = 0, n_accept = 0, n_done = 0;
+    int qn = 0;  // queued candidates, uniform over the wavefront, at most 63 between rounds
+    for (long ch = wave;; ch += nwaves) {
+        const bool more = ch < nchunks;  // uniform
+        if (more) {
+            const long p = ch * 64 + lane;
+            const bool f = p < P.n && flag[p];
+            const unsigned long long m = __ballot(f);
+            if (f) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = (int)p;
+            qn += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+        }
+        while (qn >= 64 || (!more && qn > 0)) {
+            const int k = qn < 64 ? qn : 64;
+            lrm_process(P, eff, w, q, k, lane, tol, n_plastic, n_accept, n_done);
+            const int rest = qn - k;  // move the rest to the front
+            const int v = lane < rest ? q[k + lane] : 0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) q[lane] = v;
+            __builtin_amdgcn_wave_barrier();
+            qn = rest;
+        }
+        if (!more) break;
+    }
+    add_counters(counters, lane, n_plastic, n_accept, n_done);
 }
 
 static int check_particles(const mpmx_particles* P) {
@@ -229,15 +309,20 @@ static int check_particles(const mpmx_particles* P) {
     return 0;
 }
 
-int launch_stress_learned(const mpmx_particles& P, float eff, const float* params, float tol, int32_t* counters,
-                          hipStream_t s) {
+int launch_stress_learned(const mpmx_particles& P, float eff, const float* params, float tol, int64_t* counters,
+                          uint8_t* flag, hipStream_t s) {
     if (P.n == 0) return 0;
     const long waves = ((long)P.n + 63) / 64;
     long nb = (waves + 3) / 4;
     if (nb > 8192) nb = 8192;
-    MPMX_LAUNCH(stress_learned_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, P, eff, params, tol, params ? 1 : 0,
+    const int use_net = params ? 1 : 0;
+    MPMX_REQUIRE(!use_net || flag, "learned stress update: no flag buffer");
+    MPMX_LAUNCH(stress_learned_pass1_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, P, eff, use_net, flag, counters);
+    int rc = check_launch("stress_learned(plain)", s);
+    if (rc || !use_net) return rc;
+    MPMX_LAUNCH(stress_learned_pass2_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, P, eff, params, tol, flag,
                 counters);
-    return check_launch("stress_learned", s);
+    return check_launch("stress_learned(network)", s);
 }
 
 }  // namespace mpmx
@@ -256,8 +341,10 @@ int mpmx_lrm_harvest(const mpmx_particles* P, float* X, float* T, float* e_trial
     return check_launch("lrm_harvest", s);
 }
 
+size_t mpmx_lrm_workspace_bytes(int n) { return n > 0 ? (((size_t)n + 255) / 256) * 256 : 256; }
+
 int mpmx_compute_stress_learned(const mpmx_particles* P, float eff, const float* params, float tol,
-                                int32_t* counters, void* stream) {
+                                int64_t* counters, void* workspace, size_t ws_bytes, void* stream) {
     int rc = check_particles(P);
     if (rc) return rc;
     if (P->n == 0) return 0;
@@ -265,7 +352,10 @@ int mpmx_compute_stress_learned(const mpmx_particles* P, float eff, const float*
                      P->softening && P->density && P->strainCriteria && P->eta_shear && P->eta_bulk,
                  "mpmx_compute_stress_learned: null particle array");
     MPMX_REQUIRE(tol >= 0.0f, "mpmx_compute_stress_learned: negative tolerance");
-    return launch_stress_learned(*P, eff, params, tol, counters, as_stream(stream));
+    MPMX_REQUIRE(!params || (workspace && ws_bytes >= mpmx_lrm_workspace_bytes(P->n)),
+                 "mpmx_compute_stress_learned: workspace smaller than mpmx_lrm_workspace_bytes(n)");
+    return launch_stress_learned(*P, eff, params, tol, counters, static_cast<uint8_t*>(workspace),
+                                 as_stream(stream));
 }
 
 }  // extern "C"

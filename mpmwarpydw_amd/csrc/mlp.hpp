@@ -62,23 +62,33 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// tanh form of GELU (Hendrycks & Gimpel 2016) and its derivative
-__device__ __forceinline__ float gelu_f(float x) {
+// tanh form of GELU (Hendrycks & Gimpel 2016), 0.5 x (1 + tanh u) with
+// u = sqrt(2/pi) (x + 0.044715 x^3), and its derivative, written with the
+// logistic function: 0.5 (1 + tanh u) = sig(2 u) = 1 / (1 + exp(-2 u)) and
+// 1 - tanh^2 u = 4 sig (1 - sig).  One hardware exponential and one
+// reciprocal (about 1 ulp each); exp(-2u) overflowing to inf gives sig = 0.
+__device__ __forceinline__ float sig2u(float x) {
     const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    return 0.5f * x * (1.0f + tanhf(u));
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
+__device__ __forceinline__ float gelu_f(float x) { return x * sig2u(x); }
 __device__ __forceinline__ float gelu_grad_f(float x) {
-    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    const float t = tanhf(u);
+    const float s = sig2u(x);
     const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
+    return s + 2.0f * x * (s * (1.0f - s)) * du;
 }
 
 // Weights of the forward pass in LDS as bfloat16, zero-padded to 16-wide blocks.
+// Rows are padded (16 -> 20 and 64 -> 68 entries) so that the 16 rows one A
+// fragment reads fall in distinct LDS banks: a row of 16 or 64 bfloat16 is 8
+// or 32 banks long, which puts rows r and r + 4 (or every row) in the same
+// bank; rows of 40 or 136 bytes spread them over all 32 banks, and keep the
+// 8-byte alignment of the fragments.
+constexpr int kS16 = 20, kS64 = 68;
 struct MlpWeights {
-    unsigned short W1[kNH][16];  // [o][i], i >= 10 zero
-    unsigned short W2[kNH][kNH];
-    unsigned short W3[16][kNH];  // [o][i], o >= 3 zero
+    unsigned short W1[kNH][kS16];  // [o][i], i >= 10 zero
+    unsigned short W2[kNH][kS64];
+    unsigned short W3[16][kS64];  // [o][i], o >= 3 zero
     float b1[kNH], b2[kNH], b3[16];
 };
 
@@ -100,7 +110,8 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ prm, MlpW
     for (int k = threadIdx.x; k < 16; k += blockDim.x) w.b3[k] = k < kNO ? prm[kOB3 + k] : 0.0f;
 }
 
-// A fragment of a [rows][cols] bfloat16 matrix for output block mb, input block kb
+// A fragment of a bfloat16 matrix with rows of COLS entries (the padded
+// stride) for output block mb, input block kb
 template <int COLS>
 __device__ __forceinline__ bf16x4 a_frag(const unsigned short (*M)[COLS], int mb, int kb, int lane) {
     return *reinterpret_cast<const bf16x4*>(&M[mb * 16 + (lane & 15)][kb * 16 + 4 * (lane >> 4)]);
@@ -122,7 +133,7 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& w, bf16x4 xb,
 #pragma unroll
     for (int mb = 0; mb < 4; mb++) {
         f32x4 c = {w.b1[mb * 16 + r0], w.b1[mb * 16 + r0 + 1], w.b1[mb * 16 + r0 + 2], w.b1[mb * 16 + r0 + 3]};
-        const bf16x4 a = a_frag<16>(w.W1, mb, 0, lane);
+        const bf16x4 a = a_frag<kS16>(w.W1, mb, 0, lane);
         c = mfma16(a, xb, c);
         c = mfma16(a, xl, c);
         t.z1[mb] = c;
@@ -135,7 +146,7 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& w, bf16x4 xb,
         f32x4 c = {w.b2[mb * 16 + r0], w.b2[mb * 16 + r0 + 1], w.b2[mb * 16 + r0 + 2], w.b2[mb * 16 + r0 + 3]};
 #pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-            const bf16x4 a = a_frag<kNH>(w.W2, mb, kb, lane);
+            const bf16x4 a = a_frag<kS64>(w.W2, mb, kb, lane);
             c = mfma16(a, t.h1b[kb], c);
             c = mfma16(a, t.h1l[kb], c);
         }
@@ -147,7 +158,7 @@ __device__ __forceinline__ void mlp_tile_forward(const MlpWeights& w, bf16x4 xb,
     f32x4 c = {w.b3[r0], w.b3[r0 + 1], w.b3[r0 + 2], w.b3[r0 + 3]};
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) {
-        const bf16x4 a = a_frag<kNH>(w.W3, 0, kb, lane);
+        const bf16x4 a = a_frag<kS64>(w.W3, 0, kb, lane);
         c = mfma16(a, t.h2b[kb], c);
         c = mfma16(a, t.h2l[kb], c);
     }

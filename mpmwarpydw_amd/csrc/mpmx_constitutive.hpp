@@ -182,12 +182,12 @@ MPMX_HD void cu_begin(const mpmx_particles& P, long p, CuState& c) {
 
 // Second phase: writes F, stress, accumulated strain, damage, ys and, on a
 // phase transition, the label, velocity, XPBD birth state and zeroed material.
-// With e_proj (plastic branch only; c.sv must hold the SVD of F_trial) the
-// principal projected log strains are e_proj instead of the reference's
-// projection of the principal trial strains.  Returns the stress in T
-// (row-major) and whether the particle is still MPM (label <= 1) afterwards.
+// The plastic branch projects e' = e - r dev(e) + volc with the reference's
+// r = dgam / edn and volc = 0.3 alpha dgam, or with proj = {r, volc} when
+// given (plastic branch only).  Returns the stress in T (row-major) and
+// whether the particle is still MPM (label <= 1) afterwards.
 MPMX_HD bool cu_finish(const mpmx_particles& P, long p, float eff, CuState& c, float T[9],
-                       const float* e_proj = nullptr) {
+                       const float* proj = nullptr) {
     const long n = P.n;
     float mu = c.mu, lam = c.lam;
     const int branch = c.branch;
@@ -218,14 +218,17 @@ MPMX_HD bool cu_finish(const mpmx_particles& P, long p, float eff, CuState& c, f
             const float edn = c.devn + 1.0e-6f;
             r = c.dgam / edn;
             volc = (0.3f * c.al) * c.dgam;
+            if (proj) {
+                r = proj[0];
+                volc = proj[1];
+            }
             P.ys[p] = wmax(0.0f, c.ys + 2.0f * mu * (P.hardening[p] - P.softening[p]) * c.dgam);
         }
     }
 
     // ---- deformation gradient and Kirchhoff stress ----
-    const bool proj = e_proj && branch == 1;
     float tau[6];
-    if (c.fast && !proj) {
+    if (c.fast) {
         float* E6 = c.E6;
         float esum = c.esum;
         if (branch == 1) {
@@ -263,14 +266,9 @@ MPMX_HD bool cu_finish(const mpmx_particles& P, long p, float eff, CuState& c, f
         float d[3];
         if (branch == 1) {
             float epsn[3], e[3], l[3];
-            if (proj) {
+            const float mean_eps = c.esum / 3.0f;
 #pragma unroll
-                for (int k = 0; k < 3; k++) epsn[k] = e_proj[k];
-            } else {
-                const float mean_eps = c.esum / 3.0f;
-#pragma unroll
-                for (int k = 0; k < 3; k++) epsn[k] = (c.eps[k] - r * (c.eps[k] - mean_eps)) + volc;
-            }
+            for (int k = 0; k < 3; k++) epsn[k] = (c.eps[k] - r * (c.eps[k] - mean_eps)) + volc;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 e[k] = expf(epsn[k]);

@@ -103,9 +103,10 @@ int fill_i32(int32_t* dst, long n, int32_t value, hipStream_t s);
 
 // the constitutive update of every active MPM particle with the learned
 // projection judged by the exact map (params NULL: exact projection for all);
-// counters[3] (may be NULL) += {plastic states, accepted predictions, updated}
-int launch_stress_learned(const mpmx_particles& P, float eff, const float* params, float tol, int32_t* counters,
-                          hipStream_t s);
+// counters[3] (may be NULL) += {plastic states, accepted predictions, updated};
+// flag: n bytes of scratch (needed with params)
+int launch_stress_learned(const mpmx_particles& P, float eff, const float* params, float tol, int64_t* counters,
+                          uint8_t* flag, hipStream_t s);
 
 }  // namespace mpmx
 

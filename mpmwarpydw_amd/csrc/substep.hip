@@ -32,6 +32,7 @@ struct SubstepWork {
     int32_t* tile_flag;
     SceneDesc* table;    // [nscenes + 1] for an ensemble
     unsigned long long* vlc;  // [2 * nodes] cell sums of the volumetric locking correction
+    uint8_t* lrm_flag;        // [n] candidates of the learned projection (mpmx_*_substep_learned)
     BinWork bin;
 };
 
@@ -57,7 +58,8 @@ static size_t substep_ws_bytes(long n, const Geo& g) {
     long nn = n > 0 ? n : 1;
     return Carver::bytes_for(1, 4) + Carver::bytes_for(nn, 4) * 2 + Carver::bytes_for(g.ntiles() * kTileNodes * 4, 4) +
            Carver::bytes_for(g.ntiles(), 4) + Carver::bytes_for(g.nscenes + 1, sizeof(SceneDesc)) +
-           Carver::bytes_for(2 * g.nnodes, sizeof(unsigned long long)) + bin_workspace_bytes(nn, bin_nkeys(g, nn));
+           Carver::bytes_for(2 * g.nnodes, sizeof(unsigned long long)) + Carver::bytes_for(nn, 1) +
+           bin_workspace_bytes(nn, bin_nkeys(g, nn));
 }
 
 static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepWork& w) {
@@ -70,6 +72,7 @@ static bool carve_substep(void* ws, size_t bytes, long n, const Geo& g, SubstepW
     w.tile_flag = cv.take<int32_t>(g.ntiles());
     w.table = cv.take<SceneDesc>(g.nscenes + 1);
     w.vlc = cv.take<unsigned long long>(2 * g.nnodes);
+    w.lrm_flag = cv.take<uint8_t>(nn);
     carve_bin(cv, nn, bin_nkeys(g, nn), w.bin);
     return !cv.overflow;
 }
@@ -555,7 +558,7 @@ static int reorder_geo(const mpmx_particles* P, const Geo& geo, float inv_dx, Su
 struct LearnedStress {
     const float* params;
     float tol;
-    int32_t* counters;
+    int64_t* counters;
 };
 
 static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in, float* grid_v_out,
@@ -582,7 +585,8 @@ static int substep_geo(const mpmx_particles* P, float* grid_m, float* grid_v_in,
     ta.misbinned = w.misbinned;
     if (ls || tuning(MPMX_TUNE_SPLIT_STRESS)) {  // the learned update runs as its own kernel
         if (ls) {
-            if ((rc = launch_stress_learned(*P, prm->eff, ls->params, ls->tol, ls->counters, s))) return rc;
+            if ((rc = launch_stress_learned(*P, prm->eff, ls->params, ls->tol, ls->counters, w.lrm_flag, s)))
+                return rc;
         } else if (P->n > 0) {
             MPMX_LAUNCH(substep_stress_kernel, dim3(grid_for(P->n, kBlock)), dim3(kBlock), 0, s, *P, prm->eff);
             if ((rc = check_launch("substep_stress", s))) return rc;
@@ -725,7 +729,7 @@ int mpmx_mpm_substep(const mpmx_particles* P, const mpmx_grid* G, const mpmx_ste
 }
 
 int mpmx_mpm_substep_learned(const mpmx_particles* P, const mpmx_grid* G, const mpmx_step_params* prm,
-                             const float* mlp_params, float tol, int32_t* counters, void* workspace, size_t ws_bytes,
+                             const float* mlp_params, float tol, int64_t* counters, void* workspace, size_t ws_bytes,
                              void* stream) {
     int rc = check_args(P, G, prm);
     if (rc) return rc;
@@ -795,7 +799,7 @@ int mpmx_ensemble_substep(const mpmx_particles* P, const mpmx_scene* scenes, int
 
 int mpmx_ensemble_substep_learned(const mpmx_particles* P, const mpmx_scene* scenes, int nscenes, float* grid_m,
                                   float* grid_v_in, float* grid_v_out, const mpmx_step_params* prm,
-                                  const float* mlp_params, float tol, int32_t* counters, void* workspace,
+                                  const float* mlp_params, float tol, int64_t* counters, void* workspace,
                                   size_t ws_bytes, void* stream) {
     MPMX_REQUIRE(grid_m && grid_v_in && grid_v_out, "ensemble: null grid array");
     MPMX_REQUIRE(tol >= 0.0f, "mpmx_ensemble_substep_learned: negative tolerance");

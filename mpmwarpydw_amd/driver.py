@@ -137,6 +137,31 @@ class Simulation:
         lib("mpmx_invert_permutation", sim.particle_id.ptr, sim._slot_of_id.ptr, n, rt.stream_handle())
         mpm._bins = None
 
+    def restore_input_order(self) -> None:
+        """Undo reorder(): permute every per-particle array back to input order
+        (slot i holds input index i again)."""
+        SR = simulationRoutines
+        sim, mpm, xpbd = self.sim, self.mpm, self.xpbd
+        n = sim.nPoints
+        back = rt.zeros(n, rt.int32)  # slot of every input index
+        lib("mpmx_invert_permutation", sim.particle_id.ptr, back.ptr, n, rt.stream_handle())
+        seen = set()
+        for obj in (sim, mpm, xpbd):
+            for name, a in list(vars(obj).items()):
+                if (not isinstance(a, rt.Array) or name.startswith("_") or id(a) in seen or a.dtype == rt.float64
+                        or len(a.shape) != 1):
+                    continue
+                if a.shape[0] == n:
+                    a.permute_(back, 1)
+                elif a.shape[0] == 6 * n:
+                    a.permute_(back, 6)
+                else:
+                    continue
+                seen.add(id(a))
+        sim._slot_of_id = None
+        mpm._bins = None
+        SR.mpmSimulationBin(sim, mpm, xpbd)
+
     # ---- what the loop reads back (runMPMYDW.py:686-722) ----
     def loop_scalars(self) -> tuple[float, float]:
         """(velocity residual, mean damage) after a substep: sum |v|/r over

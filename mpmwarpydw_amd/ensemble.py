@@ -114,7 +114,7 @@ class Ensemble:
         ``learned`` (a learned.LearnedReturnMap) the constitutive update uses
         its projection, judged by the exact map at ``tol`` (``use_net`` False:
         the same kernel with the exact projection for every state);
-        ``counters`` (int32 device tensor of 3, optional) accumulates
+        ``counters`` (int64 device tensor of 3, optional) accumulates
         {plastic, accepted, updated}."""
         ws, nb = self._workspace()
         if not self._binned:
@@ -126,8 +126,8 @@ class Ensemble:
             return
         cp = None
         if counters is not None:
-            if counters.dtype != torch.int32 or counters.numel() < 3 or not counters.is_cuda:
-                raise ValueError("counters: an int32 device tensor of 3 elements")
+            if counters.dtype != torch.int64 or counters.numel() < 3 or not counters.is_cuda:
+                raise ValueError("counters: an int64 device tensor of 3 elements")
             cp = C.c_void_p(counters.data_ptr())
         lib("mpmx_ensemble_substep_learned", C.byref(P), self.table, len(self.table), self.grid_m.ptr,
             self.grid_v_in.ptr, self.grid_v_out.ptr, C.byref(prm),
@@ -140,19 +140,27 @@ class Ensemble:
         perm = rt.zeros(self.n, rt.int32)
         lib("mpmx_ensemble_reorder_permutation", C.byref(P), self.table, len(self.table), C.byref(prm), perm.ptr, ws,
             nb, rt.stream_handle())
-        arrays = [self.scene]
+        for a in self._state_arrays():
+            words = 6 if a.shape[0] == 6 * self.n else 1
+            a.permute_(perm, words, None if a is self.sim.particle_id else self.sim.particle_id)
+        self._binned = False
+
+    def _state_arrays(self):
+        arrays, seen = [self.scene], {id(self.scene)}
         for obj in (self.sim, self.mpm, self.xpbd):
             for name, a in vars(obj).items():
                 if isinstance(a, rt.Array) and not name.startswith("_") and len(a.shape) == 1 \
-                        and a.dtype != rt.float64 and a.shape[0] in (self.n, 6 * self.n):
+                        and a.dtype != rt.float64 and a.shape[0] in (self.n, 6 * self.n) and id(a) not in seen:
+                    seen.add(id(a))
                     arrays.append(a)
-        seen = set()
-        for a in arrays:
-            if id(a) in seen:
-                continue
-            seen.add(id(a))
-            words = 6 if a.shape[0] == 6 * self.n else 1
-            a.permute_(perm, words, None if a is self.sim.particle_id else self.sim.particle_id)
+        return arrays
+
+    def restore_input_order(self) -> None:
+        """Undo reorder(): every per-particle array back in input order."""
+        back = rt.zeros(self.n, rt.int32)  # slot of every input index
+        lib("mpmx_invert_permutation", self.sim.particle_id.ptr, back.ptr, self.n, rt.stream_handle())
+        for a in self._state_arrays():
+            a.permute_(back, 6 if a.shape[0] == 6 * self.n else 1)
         self._binned = False
 
     # ---- views ---------------------------------------------------------------

@@ -135,9 +135,10 @@ class LearnedReturnMap:
         """compute_stress_from_F_trial with the learned projection judged by the
         exact map; returns the device counters {plastic, accepted, updated}."""
         P, _, _ = step_structs(sim, mpm, xpbd)
-        counters = torch.zeros(3, dtype=torch.int32, device=rt.device())
+        counters = torch.zeros(3, dtype=torch.int64, device=rt.device())
+        ws, nb = self._ws.get(lib.raw("mpmx_lrm_workspace_bytes")(sim.nPoints))
         lib("mpmx_compute_stress_learned", C.byref(P), float(mpm.eff), _p(self.params) if use_net else None,
-            float(tol), _p(counters), rt.stream_handle())
+            float(tol), _p(counters), ws, nb, rt.stream_handle())
         return counters
 
     # ---- persistence ------------------------------------------------------

@@ -201,7 +201,7 @@ def mpmSimulationStepFused(sim, mpm, xpbd, device=None, *, learned=None, tol=1e-
     are rebuilt first whenever x, the labels or the activity were written from
     the host since the last substep.  With ``learned`` (a
     learned.LearnedReturnMap) the constitutive update uses the network's
-    projection, judged by the exact map at ``tol``; ``counters`` (int32 device
+    projection, judged by the exact map at ``tol``; ``counters`` (int64 device
     tensor of 3) accumulates {plastic, accepted, updated}."""
     P, G, prm = step_structs(sim, mpm, xpbd)
     ws, nb = substep_workspace(sim, mpm)

@@ -8,7 +8,9 @@ from state_utils import host_state, make_sim
 
 pytestmark = pytest.mark.gpu
 
-COMBOS = [(1, 1, 1), (1, 2, 1), (1, 2, 0), (1, 0, 1), (1, 0, 0), (0, 1, 1), (0, 2, 1), (0, 0, 1)]  # (split_stress, g2p_tiles, xcd_tiles)
+# (split_stress, g2p_tiles, xcd_tiles, p2g_chunk knob)
+COMBOS = [(1, 1, 1, 0), (1, 2, 1, 0), (1, 2, 0, 0), (1, 0, 1, 0), (1, 0, 0, 0), (0, 1, 1, 0), (0, 2, 1, 0),
+          (0, 0, 1, 0), (1, 1, 1, 1), (1, 2, 0, 2)]
 
 
 def test_knobs_leave_results_unchanged():
@@ -16,12 +18,13 @@ def test_knobs_leave_results_unchanged():
     from mpmwarpydw_amd import simulationRoutines as SR
     ref = None
     try:
-        for split, g2p, xcd in COMBOS:
+        for split, g2p, xcd, chunk in COMBOS:
             autotune.set_split_stress(bool(split))
             autotune.set_g2p_tiles(g2p)
             autotune.set_xcd_tiles(bool(xcd))
+            autotune.set_p2g_chunk(chunk)
             S = make_sim(seed=50)
-            print(f"[knobs] split={split} g2p_tiles={g2p} xcd={xcd}", flush=True)
+            print(f"[knobs] split={split} g2p_tiles={g2p} xcd={xcd} p2g_chunk={chunk}", flush=True)
             for counter in range(1, 13):
                 S.substep(counter)
             assert SR.misbinned_count(S.sim, S.mpm) == 0
@@ -30,8 +33,47 @@ def test_knobs_leave_results_unchanged():
                 ref = st
                 continue
             for k in ref:
-                assert np.array_equal(st[k], ref[k]), (split, g2p, xcd, k)
+                assert np.array_equal(st[k], ref[k]), (split, g2p, xcd, chunk, k)
     finally:
         autotune.set_split_stress(True)
         autotune.set_g2p_tiles(True)
         autotune.set_xcd_tiles(True)
+        autotune.set_p2g_chunk(0)
+
+
+@pytest.mark.parametrize("orders", [(False, True), (False,)])
+def test_autotune_leaves_state_at_start_in_its_order(orders):
+    """tune() times every candidate from the same starting state, in input
+    order and then in tile order, and leaves the state as it started in the
+    order of the best candidate with that candidate's knobs in force: the
+    state then runs on to the bits of a state that never went through the
+    tuner."""
+    from mpmwarpydw_amd import autotune
+    try:
+        S = make_sim(seed=51)
+        R = make_sim(seed=51)
+        counter = [1]
+
+        def step():
+            S.substep(counter[0])
+            counter[0] += 1
+
+        log = autotune.tune(S, step, steps=2, warmup=1, trials=1, orders=orders)
+        n_cand = len(log["candidates"])
+        print(f"[autotune] orders {orders}: {n_cand} candidates, best {log['best']}", flush=True)
+        assert n_cand == len(orders) * (3 * 3 * 2 + 3 * 2)
+        assert log["kept_reordered"] == log["best"]["reorder"]
+        cur = autotune.current()
+        for k in ("xcd_tiles", "split_stress", "g2p_tiles"):
+            assert cur[k] == log["best"][k], k
+        if log["kept_reordered"]:  # and the way back to input order keeps the state too
+            S.restore_input_order()
+            S.reorder()
+        for c in range(1, 13):
+            S.substep(c)
+            R.substep(c)
+        a, b = host_state(S), host_state(R)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+    finally:
+        autotune.apply({"xcd_tiles": 1, "split_stress": 1, "g2p_tiles": 1, "p2g_chunk": 512})

@@ -301,7 +301,7 @@ def test_c4_ensemble_learned_matches_exact(block_net):
     tensors = [a.t for a in arrays.values()] + [E.grid_m.t, E.grid_v_in.t, E.grid_v_out.t, E._ws.t]
     scene = E.scene.t.long()
     ys = E.mpm.ys.t  # per-particle cohesion (the same in both forms: the exact map computes hardening)
-    cnt = torch.zeros(3, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
     worst = 0.0
     for step in range(5):
         snap = [t.clone() for t in tensors]
