@@ -87,6 +87,7 @@ def parse():
     p.add_argument("--ensemble-scenes", type=int, default=64, help="scenes of the batched ensemble")
     p.add_argument("--ensemble-steps", type=int, default=1000, help="substeps of each batched-ensemble run")
     p.add_argument("--no-learned", action="store_true", help="skip the learned+batched measurement")
+    p.add_argument("--learned-only", action="store_true", help="only the learned+batched measurement")
     p.add_argument("--learned-tol", type=float, default=5e-5,
                    help="judge tolerance of the learned map (residuals against tol * cohesion)")
     p.add_argument("--no-autotune", action="store_true", help="default knobs instead of the autotuner's")
@@ -366,6 +367,9 @@ def main():
         sys.exit(2)
     if a.scene == "ensemble":
         return bench_ensemble(a)
+    if a.learned_only:
+        print(json.dumps(bench_learned_batched(a)), flush=True)
+        return
     import torch
     from mpmwarpydw_amd import driver, getArgs, scenes
     from mpmwarpydw_amd import simulationRoutines as SR
@@ -432,7 +436,6 @@ def main():
         restore(S, snap)
         SR.mpmSimulationBin(sim, mpm, S.xpbd)
     torch.cuda.synchronize()
-    del snap
 
     t0 = time.perf_counter()
     with torch.cuda.stream(stream):
@@ -450,6 +453,30 @@ def main():
     state_end = {"mpm_particles": n_mpm, "xpbd_particles": int(((act == 1) & (lab == 2)).sum()),
                  "plastic_particles": int((mpm.particle_accumulated_strain.numpy() > 0).sum()),
                  "mean_damage": float(mpm.particle_damage.numpy()[act == 1].mean())}
+
+    # per-kernel device time over the same substeps: back to t = 0 and substeps 1..K again, launched
+    # eagerly with every library kernel carrying dispatch-stamped events (graph launches cannot carry
+    # them), in chunks of two periods queued behind a hold kernel so that each chunk runs back to back
+    # at full clocks; the run repeats the timed one bit for bit
+    with torch.cuda.stream(stream):
+        restore(S, snap)
+        SR.mpmSimulationBin(sim, mpm, S.xpbd)
+    torch.cuda.synchronize()
+    del snap
+    chunk = 2 * ratio
+    lib("mpmx_profile_begin", min(1 << 22, 2 * 64 * (steps + ratio)))
+    for c0 in range(1, steps + 1, chunk):
+        with torch.cuda.stream(stream):
+            lib("mpmx_profile_hold", stream_handle(), PROFILE_HOLD_US)
+            for c in range(c0, min(c0 + chunk, steps + 1)):
+                S.substep(c)
+        torch.cuda.synchronize()
+    lib("mpmx_profile_end")
+    kernels = profile_report()
+    kernels.pop("hold", None)
+    how = ("HIP start/stop events stamped by each kernel's own dispatch (hipExtLaunchKernelGGL) over substeps "
+           f"1-{steps} run again from t = 0 after the timed region, eagerly, in chunks of {chunk} substeps queued "
+           "behind a hold kernel")
     counter = steps + 1
 
     err = None
@@ -461,23 +488,6 @@ def main():
             S.substep(counter)
             counter += 1
     period = [counter + k for k in range(ratio)]
-
-    # per-kernel device time: one more period launched eagerly on the same stream, every
-    # library kernel carrying dispatch-stamped events (graph launches cannot carry them)
-    with torch.cuda.stream(stream):
-        # the periods queue up behind the hold and then run back to back; the
-        # first brings the clocks back up after the idle hold, the second is timed
-        lib("mpmx_profile_hold", stream_handle(), PROFILE_HOLD_US)
-        for c in period:
-            S.substep(c)
-        lib("mpmx_profile_begin", 2 * (64 * ratio + 64))
-        for c in period:
-            S.substep(c)
-    torch.cuda.synchronize()
-    lib("mpmx_profile_end")
-    kernels = profile_report()
-    how = ("HIP start/stop events stamped by each kernel's own dispatch (hipExtLaunchKernelGGL) over one eager "
-           "period after the timed region; it and a warm-up period are queued behind a hold kernel")
     counter = period[-1] + 1
     with torch.cuda.stream(stream):
         loop = driver_loop_rate(S, counter, 5 * ratio)

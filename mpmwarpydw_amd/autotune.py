@@ -133,6 +133,8 @@ def tune(state, step, steps: int = 10, warmup: int = 3, trials: int = 2, orders=
     snaps = {}
     for reorder in orders:
         if reorder:
+            if snaps:  # back to the starting state before it is permuted
+                _restore(state, *snaps[False])
             state.reorder()
         arrays = _state_arrays(state)
         snap = [a.t.clone() for a in arrays]
