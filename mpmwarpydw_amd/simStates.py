@@ -168,10 +168,11 @@ class XPBDState:
         self.particle_delta = rt.zeros(n, rt.vec3)
         self.particle_v_before_contact = rt.zeros(n, rt.vec3)
         self.particle_grid = rt.HashGrid(128, 128, 128)
-        # the label-2 points only, same cell width (the MPM side of the contact solve walks
-        # it); 64^3 cells keep its per-cell passes 8x cheaper than the full table's, and
-        # the cells that wrap onto each other lie far beyond any contact reach
-        self.particle_grid_xpbd = rt.HashGrid(64, 64, 64)
+        # the label-2 points only, same cell width and cell count as the full table (the
+        # MPM side of the contact solve walks it): with fewer cells, XPBD cells 64 cells
+        # apart would wrap onto one another, and the neighbourhood bitmap would then list
+        # the MPM particles beside every alias of an XPBD cell for a walk
+        self.particle_grid_xpbd = rt.HashGrid(128, 128, 128)
         self.minBoundsXPBD = None
         self.maxBoundsXPBD = None
         self.max_radius = None
