@@ -530,15 +530,7 @@ int mpmx_xpbd_contact_pack(int n, const int32_t* activeLabel, const int32_t* mat
  * records of mpmx_xpbd_contact_pack, packed from the same tables, x, v, labels,
  * mass and radius after their last change (my_solve_particle_particle_contacts,
  * xpbdRoutines.py:138-219).  The results equal those of the split form: lock
- * writes turn label 1 into 0 only, which changes no flag.  With brick_ws (the
- * brick form; only with a contact list built with skip_xpbd = 1) the walks of
- * the label-2 particles run brick by brick: each walk goes to the brick (4x4x2
- * of the full table's wrapped cells) of its own cell at x, and a workgroup
- * stages the candidate records of the brick's 6x6x4-cell halo in LDS once for
- * all the brick's walks, which read them in the same order as from the table,
- * so the results are the same bits; the contact list then holds the other
- * walks.  brick_ws holds mpmx_xpbd_brick_workspace_bytes(n, dims) bytes.  No
- * counterpart in the reference for the brick form. */
+ * writes turn label 1 into 0 only, which changes no flag. */
 int mpmx_solve_particle_particle_contacts_packed(
     int n, const int32_t* activeLabel, int32_t* materialLabel, const int32_t* point_ids,
     const int32_t* cell_start, const int32_t* xpbd_point_ids, const int32_t* xpbd_cell_start,
@@ -547,8 +539,7 @@ int mpmx_solve_particle_particle_contacts_packed(
     const float* vw, const int32_t* flags, float k_mu, float static_threshold, float static_friction,
     float k_cohesion, float max_radius, float dt, float relaxation, int contact_lock, float* deltas,
     const int32_t* slot_of_id, const uint64_t* xpbd_near, const int32_t* contact_list,
-    const int32_t* contact_count, void* brick_ws, size_t brick_ws_bytes, void* stream);
-size_t mpmx_xpbd_brick_workspace_bytes(int n, int dimx, int dimy, int dimz);
+    const int32_t* contact_count, void* stream);
 
 /* The table positions whose contact walk can meet a contact in this XPBD step,
  * for mpmx_solve_particle_particle_contacts_packed (contact_list, contact_count;
@@ -558,15 +549,16 @@ size_t mpmx_xpbd_brick_workspace_bytes(int n, int dimx, int dimy, int dimz);
  * meets no XPBD point in any iteration, so the particle gets no delta and its
  * query box stays put.  point_pos is the full table's position of every
  * particle (mpmx_hashgrid_build_pair); list holds up to n positions, count gets
- * their number (0 while the XPBD-only table is empty).  With skip_xpbd the
- * label-2 particles are left out (the brick form solves them).  Build it after
- * the tables of the step and before its first contact solve. */
+ * their number (0 while the XPBD-only table is empty).  With more than n/16
+ * XPBD points no list is written and count gets n + 1; the solve walks every
+ * table position whenever count exceeds n/8 (a long list in particle-array
+ * order walks slower than the table order; the results are the same bits).
+ * Build it after the tables of the step and before its first contact solve. */
 int mpmx_xpbd_contact_list(int n, const int32_t* materialLabel, const int32_t* point_pos,
                            const int32_t* xpbd_cell_start, float cell_width, int xpbd_dimx,
                            int xpbd_dimy, int xpbd_dimz, const float* x, const float* radius,
                            float k_cohesion, float max_radius, const uint64_t* xpbd_near,
-                           int skip_xpbd, int32_t* list, int32_t* count, void* stream);
-
+                           int32_t* list, int32_t* count, void* stream);
 
 /* Neighbourhood bitmap of a hash table for the contact solve (no counterpart
  * in the reference): bit c of near (cells / 64 words, mpmx_hashgrid_near_words)

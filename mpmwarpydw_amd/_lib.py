@@ -96,9 +96,8 @@ SIGNATURES = {
                                                     F, F, F, F, I, P, P, P],
     "mpmx_xpbd_contact_pack": [I, P, P, P, P, P, P, P, P, P, P, P, P, I, I, I, P, P, P, P],
     "mpmx_solve_particle_particle_contacts_packed": [I, P, P, P, P, P, P, F, I, I, I, I, I, I, P, P, P, P, P, P, P,
-                                                     F, F, F, F, F, F, F, I, P, P, P, P, P, P, S, P],
-    "mpmx_xpbd_contact_list": [I, P, P, P, F, I, I, I, P, P, F, F, P, I, P, P, P],
-    "mpmx_xpbd_brick_workspace_bytes": [I, I, I, I],
+                                                     F, F, F, F, F, F, F, I, P, P, P, P, P, P],
+    "mpmx_xpbd_contact_list": [I, P, P, P, F, I, I, I, P, P, F, F, P, P, P, P],
     "mpmx_hashgrid_near_words": [I, I, I],
     "mpmx_hashgrid_near_cells": [P, I, I, I, P, P, P],
     "mpmx_apply_particle_deltas": [I, P, P, P, P, P, F, F, P, P, P],
@@ -130,7 +129,6 @@ _RESTYPE = {
     "mpmx_hashgrid_pair_workspace_bytes": S,
     "mpmx_mlp_workspace_bytes": S,
     "mpmx_lrm_workspace_bytes": S,
-    "mpmx_xpbd_brick_workspace_bytes": S,
     "mpmx_ctx_stream": P,
     "mpmx_ensemble_workspace_bytes": S,
 }

@@ -361,12 +361,19 @@ __device__ __forceinline__ bool maybe_in_reach(float d2, float base, float rj) {
 __global__ void __launch_bounds__(kBlock) contact_list_kernel(
     int n, const int32_t* __restrict__ mat, const int32_t* __restrict__ point_pos, const int32_t* __restrict__ xp_start,
     float inv, int xdimx, int xdimy, int xdimz, const float* __restrict__ x, const float* __restrict__ radius,
-    float k_coh, float max_radius, const uint64_t* __restrict__ near, int skip_xpbd, int32_t* __restrict__ list,
+    float k_coh, float max_radius, const uint64_t* __restrict__ near, int32_t* __restrict__ list,
     int32_t* __restrict__ count) {
     // one thread per particle in array order (coalesced); the list holds table positions
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxpbd = xp_start[(long)xdimx * xdimy * xdimz];
+    if (nxpbd > (n >> 4)) {
+        // every XPBD point walks, so the list would be long and the solve walks every
+        // table position instead: no list is written, and a count above n says so
+        if (i == 0) *count = n + 1;
+        return;
+    }
     bool keep = false;
-    if (i < n && xp_start[(long)xdimx * xdimy * xdimz] > 0 && !(skip_xpbd && mat[i] == 2)) {
+    if (i < n && nxpbd > 0) {
         keep = true;
         if (mat[i] <= 1) {  // an MPM particle: does its walk meet an XPBD point?
             const f3 xi = ld3(x, n, i);
@@ -404,42 +411,9 @@ __global__ void __launch_bounds__(kBlock) contact_list_kernel(
 // skips every MPM candidate anyway (:185-186), and the XPBD candidates come in
 // the same relative order, so deltas and lock writes are unchanged.  PACKED
 // reads the candidates from ContactPack records instead of the particle arrays.
-// Bricks of the full table for the label-2 walks: 4 x 4 x 2 hash cells (a
-// brick's wrapped cell coordinates start at multiples of 4, 4, 2), and the
-// 6 x 6 x 4 halo of cells around a brick, whose candidate records a workgroup
-// stages in LDS once for all the brick's walks.
-constexpr int kBrX = 4, kBrY = 4, kBrZ = 2;
-constexpr int kHaX = kBrX + 2, kHaY = kBrY + 2, kHaZ = kBrZ + 2;
-constexpr int kHaloCells = kHaX * kHaY * kHaZ;  // 144
-constexpr int kBrickCells = kBrX * kBrY * kBrZ;  // 32
-
-constexpr int kBrickRecords = 1280;  // staged records of one halo at most
-constexpr int kBrickQueue = kContactQueue, kBrickFlush = kContactFlush;
-
-// A staged halo: cell h = (az * kHaY + ay) * kHaX + ax of the halo whose wrapped
-// origin is (ox, oy, oz) holds records off[h] .. off[h+1] (the cell's table
-// positions in table order): xr as in ContactPack, pf = position * 8 + flags
-// (vw, read for the few candidates in contact, stays in the table).
-struct Halo {
-    int ox, oy, oz;
-    const int* off;
-    const float4* xr;
-    const int32_t* pf;
-};
-
-// halo coordinate of wrapped cell coordinate c along an axis of dim cells with
-// halo origin o, or -1 beyond the halo (dim >= halo length, checked by the caller)
-__device__ __forceinline__ int halo_axis(int c, int o, int dim, int len) {
-    int a = c - o;
-    if (a < 0) a += dim;
-    if (a >= dim) a -= dim;
-    return a < len ? a : -1;
-}
-
 // the contact walk of table position tid (one per thread); queue is the
-// workgroup's LDS queue of candidate positions, [QD][kBlock] (a wave drains once
-// one of its lanes holds QF)
-template <bool PACKED, int QD = kContactQueue, int QF = kContactFlush>
+// workgroup's LDS queue of candidate positions, [kContactQueue][kBlock]
+template <bool PACKED>
 __device__ __forceinline__ void contact_one(
     int tid, int32_t (*queue)[kBlock], int n, const int32_t* __restrict__ active, int32_t* mat,
     const int32_t* __restrict__ point_ids, const int32_t* __restrict__ cell_start, const int32_t* __restrict__ xp_ids,
@@ -594,145 +568,16 @@ __device__ __forceinline__ void contact_one(
                     }
                     const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
                     if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
-                    if (queued == QD) drain();  // a lane with many near candidates
+                    if (queued == kContactQueue) drain();  // a lane with many near candidates
                     queue[queued++][threadIdx.x] = s;
                 }
-                if (__any(queued >= QF)) drain();
+                if (__any(queued >= kContactFlush)) drain();
             }
     drain();
     // point_ids is a permutation, so particle i has exactly one writer here
     deltas[i] += d0 * relax;
     deltas[n + i] += d1 * relax;
     deltas[2L * n + i] += d2 * relax;
-}
-
-// The walk of a label-2 particle at table position tid (contact_one for a
-// particle walking the full table) from a staged halo: the same candidates in
-// the same order with the same arithmetic, read from LDS; the queue holds the
-// candidates' record indices.  Returns false, having done nothing, when the
-// query box leaves the halo (the caller walks the table instead).
-__device__ __forceinline__ bool contact_one_halo(int tid, int32_t (*queue)[kBlock], int n,
-                                                 const int32_t* __restrict__ active, int32_t* mat,
-                                                 const int32_t* __restrict__ point_ids, float inv, int dimx, int dimy,
-                                                 int dimz, const float* __restrict__ x, const float* __restrict__ v,
-                                                 const float* __restrict__ mass, const float* __restrict__ radius,
-                                                 float k_mu_in, float thr, float smu, float k_coh, float max_radius,
-                                                 float dt, float relax, int lock, float* __restrict__ deltas,
-                                                 const int32_t* __restrict__ slot_of_id, const Halo& halo,
-                                                 const float4* __restrict__ vw) {
-    const int q = slot_of_id ? slot_of_id[tid] : tid;
-    if (active[q] != 1 || mat[q] < 0) return true;
-    const int i = point_ids[tid];
-    if (i == -1) return true;
-    const f3 xi = ld3(x, n, i), vi = ld3(v, n, i);
-    const float ri = radius[i];
-    if (mass[q] == 0.0f) return true;
-    const float w1 = 1.0f / mass[i];
-    const float qr = ri + max_radius + k_coh * ri;
-    const int xs = (int)((xi.x - qr) * inv), ys = (int)((xi.y - qr) * inv), zs = (int)((xi.z - qr) * inv);
-    const int xe = min((int)((xi.x + qr) * inv), xs + dimx - 1);
-    const int ye = min((int)((xi.y + qr) * inv), ys + dimy - 1);
-    const int ze = min((int)((xi.z + qr) * inv), zs + dimz - 1);
-    int wx, wy, wz;
-    hash_cell(dimx, dimy, dimz, xs, ys, zs, wx, wy, wz);
-    const int hx0 = halo_axis(wx, halo.ox, dimx, kHaX), hy0 = halo_axis(wy, halo.oy, dimy, kHaY),
-              hz0 = halo_axis(wz, halo.oz, dimz, kHaZ);
-    if (hx0 < 0 || hy0 < 0 || hz0 < 0 || hx0 + (xe - xs) >= kHaX || hy0 + (ye - ys) >= kHaY ||
-        hz0 + (ze - zs) >= kHaZ)
-        return false;
-    const int self = tid;
-    const float base = fabsf(ri) + fabsf(k_coh);
-    float k_mu = k_mu_in;
-    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
-    const float alpha = 0.0f;  // XPBD compliance of the reference
-    int queued = 0;
-    // the exact contact test and the contact of every queued candidate, in order;
-    // four at a time, their records (LDS) and velocities (table) loaded together
-    auto drain = [&]() {
-        for (int k0 = 0; k0 < queued; k0 += 4) {
-            float4 pj[4], qj[4];
-            int pfv[4];
-            bool in[4];
-            float nxv[4], nyv[4], nzv[4], dv[4], errv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const bool valid = k0 + u < queued;
-                const int l = valid ? queue[k0 + u][threadIdx.x] : 0;
-                pj[u] = halo.xr[l];
-                pfv[u] = halo.pf[l];
-                nxv[u] = xi.x - pj[u].x;
-                nyv[u] = xi.y - pj[u].y;
-                nzv[u] = xi.z - pj[u].z;
-                dv[u] = len3(f3{nxv[u], nyv[u], nzv[u]});
-                errv[u] = dv[u] - ri - pj[u].w;
-                in[u] = valid && (errv[u] <= k_coh);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) qj[u] = in[u] ? vw[pfv[u] >> 3] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (!in[u]) continue;
-                const float nx = nxv[u], ny = nyv[u], nz = nzv[u], d = dv[u], err = errv[u];
-                const f3 vj = f3{qj[u].x, qj[u].y, qj[u].z};
-                const float w2 = qj[u].w;
-                const float denom = w1 + w2;
-                if (!(denom > 0.0f)) continue;
-                if (lock == 1) {  // i is label 2: an MPM candidate takes label 0
-                    if (pfv[u] & kCandMpm) mat[point_ids[pfv[u] >> 3]] = 0;
-                }
-                const float ux = nx / d, uy = ny / d, uz = nz / d;
-                const float rx = vi.x - vj.x, ry = vi.y - vj.y, rz = vi.z - vj.z;
-                const float lambda_n = err / (1.0f + (alpha * dt * dt));
-                const float vn = ux * rx + uy * ry + uz * rz;
-                if (fabsf(vn) < thr * ri) k_mu = smu;
-                const float tx = rx - ux * vn, ty = ry - uy * vn, tz = rz - uz * vn;
-                const float tl = len3(f3{tx, ty, tz});
-                const float a_ = k_mu * lambda_n, b_ = -tl * dt;
-                const float lf = a_ > b_ ? a_ : b_;  // wp.max
-                const f3 nt = normalize3(f3{tx, ty, tz});
-                d0 += (nt.x * lf - ux * lambda_n) / denom * w1;
-                d1 += (nt.y * lf - uy * lambda_n) / denom * w1;
-                d2 += (nt.z * lf - uz * lambda_n) / denom * w1;
-            }
-        }
-        queued = 0;
-    };
-    for (int cz = zs; cz <= ze; cz++)
-        for (int cy = ys; cy <= ye; cy++)
-            for (int cx = xs; cx <= xe; cx++) {
-                const int h = ((hz0 + cz - zs) * kHaY + (hy0 + cy - ys)) * kHaX + (hx0 + cx - xs);
-                const int l1 = halo.off[h + 1];
-                // four candidates at a time: their records load together
-                for (int l = halo.off[h]; l < l1; l += 4) {
-                    int pfv[4];
-                    float4 pjv[4];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int lu = l + u < l1 ? l + u : l;
-                        pfv[u] = halo.pf[lu];
-                        pjv[u] = halo.xr[lu];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        if (l + u >= l1) break;
-                        const int pf = pfv[u];
-                        if (!(pf & kCandValid) || (pf >> 3) == self) continue;
-                        const float4 pj = pjv[u];
-                        const float rj = pj.w;
-                        const float nx = xi.x - pj.x, ny = xi.y - pj.y, nz = xi.z - pj.z;
-                        if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
-                        if (queued == kBrickQueue) drain();  // a lane with many near candidates
-                        queue[queued++][threadIdx.x] = l + u;
-                    }
-                }
-                if (__any(queued >= kBrickFlush)) drain();
-            }
-    drain();
-    // point_ids is a permutation, so particle i has exactly one writer here
-    deltas[i] += d0 * relax;
-    deltas[n + i] += d1 * relax;
-    deltas[2L * n + i] += d2 * relax;
-    return true;
 }
 
 template <bool PACKED>
@@ -744,139 +589,27 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     float thr, float smu, float k_coh, float max_radius, float dt, float relax, int lock, float* __restrict__ deltas,
     const int32_t* __restrict__ slot_of_id, ContactPack pk) {
     __shared__ int32_t queue[kContactQueue][kBlock];
-    if (PACKED && pk.list) {
-        // the listed table positions over a bounded grid (the list is short while
-        // few particles are in the XPBD phase): each workgroup takes a contiguous
-        // range of the list, and the ranges go to the XCDs in contiguous runs
-        const int cnt = *pk.list_count;
-        const int per = (cnt + (int)gridDim.x - 1) / (int)gridDim.x;
-        const int b = xcd_tile(blockIdx.x, gridDim.x);
-        const int t1 = min(cnt, (b + 1) * per);
-        for (int t = b * per + (int)threadIdx.x; t < t1; t += blockDim.x)
-            contact_one<PACKED>(pk.list[t], queue, n, active, mat, point_ids, cell_start, xp_ids, xp_start, inv, dimx,
-                                dimy, dimz, xdimx, xdimy, xdimz, x, v, mass, radius, k_mu_in, thr, smu, k_coh,
-                                max_radius, dt, relax, lock, deltas, slot_of_id, pk);
-        return;
-    }
-    // table positions in XCD-contiguous ranges: the cells a block walks are
-    // those of the blocks beside it in the table, so they share one L2
+    // one thread per table position, or per listed position while the list is
+    // short (it holds the walks that can meet a contact); the positions go to
+    // the XCDs in contiguous ranges, so the cells a block walks are those of the
+    // blocks beside it in the table and share one L2.  A list holding more than
+    // an eighth of the particles is not used: its order is that of the particle
+    // arrays, and table order walks neighbouring cells from neighbouring lanes
+    // (2.0x faster on a 2M-particle scene with 1M XPBD points).  The results are
+    // the same either way: each walk writes only its own particle's delta.
     const int tid = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (tid >= n) return;
-    contact_one<PACKED>(tid, queue, n, active, mat, point_ids, cell_start, xp_ids, xp_start, inv, dimx, dimy, dimz,
+    int pos = tid;
+    if (PACKED && pk.list) {
+        const int cnt = *pk.list_count;
+        if (cnt <= (n >> 3)) {
+            if (tid >= cnt) return;
+            pos = pk.list[tid];
+        }
+    }
+    if (pos >= n) return;
+    contact_one<PACKED>(pos, queue, n, active, mat, point_ids, cell_start, xp_ids, xp_start, inv, dimx, dimy, dimz,
                         xdimx, xdimy, xdimz, x, v, mass, radius, k_mu_in, thr, smu, k_coh, max_radius, dt, relax, lock,
                         deltas, slot_of_id, pk);
-}
-
-// The label-2 walks of one contact solve grouped by brick: a walk goes to the
-// brick of its particle's cell at the walk's own position x (the cell of the
-// query box's centre), so its box lies in that brick's halo whenever the box
-// reaches at most one cell around the centre cell.  Keyed per table position
-// and counting-sorted (positions ascending inside a brick, so the lanes of a
-// wavefront walk neighbouring cells from the same staged records).
-__global__ void brick_key_kernel(int n, const int32_t* __restrict__ point_ids, const int32_t* __restrict__ mat,
-                                 const float* __restrict__ x, float inv, int dimx, int dimy, int dimz, int nbx,
-                                 int nby, int32_t* __restrict__ keys) {
-    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= n) return;
-    const int i = point_ids[pos];
-    int key = -1;
-    if (i >= 0 && mat[i] == 2) {
-        int cx, cy, cz;
-        hash_cell(dimx, dimy, dimz, (int)(x[i] * inv), (int)(x[n + i] * inv), (int)(x[2L * n + i] * inv), cx, cy,
-                  cz);
-        key = ((cz / kBrZ) * nby + cy / kBrY) * nbx + cx / kBrX;
-    }
-    keys[pos] = key;
-}
-
-// exclusive prefix over cnt[0..len) (len <= 192) by wave 0 of the block: off[k]
-// = sum of cnt[0..k), off[len] = the total
-__device__ __forceinline__ void block_prefix(const int* cnt, int* off, int len) {
-    if (threadIdx.x >= 64) return;
-    const int lane = threadIdx.x;
-    int v[3], t = 0;
-    for (int j = 0; j < 3; j++) {
-        const int k = lane * 3 + j;
-        v[j] = k < len ? cnt[k] : 0;
-        t += v[j];
-    }
-    int inc = t;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(inc, o);
-        if (lane >= o) inc += u;
-    }
-    int run = inc - t;
-    for (int j = 0; j < 3; j++) {
-        const int k = lane * 3 + j;
-        if (k < len) off[k] = run;
-        run += v[j];
-    }
-    if (lane == 63) off[len] = inc;
-}
-
-// The label-2 walks, brick by brick: a workgroup stages the candidate records
-// (flags, x and radius) of the brick's halo in LDS once, then every walk of
-// the brick's bucket reads its query box from LDS (contact_one_halo; a box
-// leaving the halo, or a halo beyond kBrickRecords records, reads the table).
-// The candidates, their order and the arithmetic are those of the table walk,
-// so the deltas and lock writes are the same bits.  A bounded grid strides over
-// the bricks.
-__global__ void __launch_bounds__(kBlock) particle_contacts_brick_kernel(
-    int n, const int32_t* __restrict__ active, int32_t* mat, const int32_t* __restrict__ point_ids,
-    const int32_t* __restrict__ cell_start, float inv, int dimx, int dimy, int dimz, int nbx, int nby,
-    const float* __restrict__ x, const float* __restrict__ v, const float* __restrict__ mass,
-    const float* __restrict__ radius, float k_mu_in, float thr, float smu, float k_coh, float max_radius, float dt,
-    float relax, int lock, float* __restrict__ deltas, const int32_t* __restrict__ slot_of_id, ContactPack pk,
-    int nb, const int32_t* __restrict__ boff, const int32_t* __restrict__ bucket) {
-    __shared__ int32_t queue[kBrickQueue][kBlock];
-    __shared__ int hcnt[kHaloCells], hbeg[kHaloCells], hoff[kHaloCells + 1];
-    __shared__ float4 rxr[kBrickRecords];
-    __shared__ int32_t rpf[kBrickRecords];
-    const int t = threadIdx.x;
-    for (int b = blockIdx.x; b < nb; b += gridDim.x) {  // uniform per workgroup
-        const int p0 = boff[b], np = boff[b + 1] - p0;
-        if (np == 0) continue;
-        const int bx = (b % nbx) * kBrX, by = ((b / nbx) % nby) * kBrY, bz = (b / (nbx * nby)) * kBrZ;
-        const int ox = bx == 0 ? dimx - 1 : bx - 1, oy = by == 0 ? dimy - 1 : by - 1, oz = bz == 0 ? dimz - 1 : bz - 1;
-        if (t < kHaloCells) {
-            const int ax = t % kHaX, ay = (t / kHaX) % kHaY, az = t / (kHaX * kHaY);
-            const int c = ((oz + az) % dimz * dimy + (oy + ay) % dimy) * dimx + (ox + ax) % dimx;
-            hbeg[t] = cell_start[c];
-            hcnt[t] = cell_start[c + 1] - hbeg[t];
-        }
-        __syncthreads();
-        if (t < 64) block_prefix(hcnt, hoff, kHaloCells);
-        __syncthreads();
-        const int total = hoff[kHaloCells];
-        const bool fits = total <= kBrickRecords && dimx >= kHaX && dimy >= kHaY && dimz >= kHaZ;
-        if (fits) {
-            for (int r = t; r < total; r += blockDim.x) {
-                int lo = 0, hi = kHaloCells - 1;  // the halo cell holding record r: last off <= r
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (hoff[mid] <= r) lo = mid; else hi = mid - 1;
-                }
-                const int sidx = hbeg[lo] + (r - hoff[lo]);
-                const int f = pk.flags[sidx];
-                rpf[r] = sidx * 8 + (f & 7);
-                if (f & kCandValid) rxr[r] = pk.xr[sidx];  // the walk reads no more of an invalid candidate
-            }
-        }
-        __syncthreads();
-        const Halo halo{ox, oy, oz, hoff, rxr, rpf};
-        for (int k = t; k < np; k += blockDim.x) {
-            const int pos = bucket[p0 + k];  // a label-2 walk whose centre cell is in the brick
-            if (fits && contact_one_halo(pos, queue, n, active, mat, point_ids, inv, dimx, dimy, dimz, x, v, mass,
-                                         radius, k_mu_in, thr, smu, k_coh, max_radius, dt, relax, lock, deltas,
-                                         slot_of_id, halo, pk.vw))
-                continue;
-            contact_one<true, kBrickQueue, kBrickFlush>(pos, queue, n, active, mat, point_ids, cell_start, nullptr,
-                                                        nullptr, inv, dimx, dimy, dimz, 1, 1, 1, x, v, mass, radius,
-                                                        k_mu_in, thr, smu, k_coh, max_radius, dt, relax, lock,
-                                                        deltas, slot_of_id, pk);
-        }
-        __syncthreads();  // the next brick overwrites the staged halo
-    }
 }
 
 // my_apply_particle_deltas (xpbdRoutines.py:221-257)
@@ -1149,7 +882,7 @@ int mpmx_hashgrid_near_cells(const int32_t* cell_start, int dimx, int dimy, int 
 int mpmx_xpbd_contact_list(int n, const int32_t* materialLabel, const int32_t* point_pos,
                            const int32_t* xpbd_cell_start, float cell_width, int xpbd_dimx, int xpbd_dimy,
                            int xpbd_dimz, const float* x, const float* radius, float k_cohesion, float max_radius,
-                           const uint64_t* xpbd_near, int skip_xpbd, int32_t* list, int32_t* count, void* stream) {
+                           const uint64_t* xpbd_near, int32_t* list, int32_t* count, void* stream) {
     MPMX_REQUIRE(n >= 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0 && cell_width > 0.0f,
                  "mpmx_xpbd_contact_list: bad sizes");
     MPMX_REQUIRE(count && xpbd_cell_start, "mpmx_xpbd_contact_list: null pointer");
@@ -1159,31 +892,8 @@ int mpmx_xpbd_contact_list(int n, const int32_t* materialLabel, const int32_t* p
     MPMX_REQUIRE(materialLabel && point_pos && x && radius && xpbd_near && list, "mpmx_xpbd_contact_list: null pointer");
     MPMX_LAUNCH(contact_list_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, materialLabel, point_pos,
                 xpbd_cell_start, 1.0f / cell_width, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, radius, k_cohesion, max_radius,
-                xpbd_near, skip_xpbd, list, count);
+                xpbd_near, list, count);
     return check_launch("contact_list", s);
-}
-
-static long brick_total(int dimx, int dimy, int dimz) {
-    return (long)((dimx + kBrX - 1) / kBrX) * ((dimy + kBrY - 1) / kBrY) * ((dimz + kBrZ - 1) / kBrZ);
-}
-
-size_t mpmx_xpbd_brick_workspace_bytes(int n, int dimx, int dimy, int dimz) {
-    if (n < 0 || dimx <= 0 || dimy <= 0 || dimz <= 0) return 0;
-    const long nb = brick_total(dimx, dimy, dimz), nn = n > 0 ? n : 1;
-    return 2 * Carver::bytes_for(nn, 4) + bin_workspace_bytes(nn, nb);
-}
-
-// the workspace of the brick form: per-position keys, the positions by brick and the sort's scratch
-struct BrickWork {
-    int32_t *keys, *bucket;
-    BinWork bin;
-};
-static bool carve_bricks(void* ws, size_t bytes, int n, long nb, BrickWork& w) {
-    Carver cv(ws, bytes);
-    const long nn = n > 0 ? n : 1;
-    w.keys = cv.take<int32_t>(nn);
-    w.bucket = cv.take<int32_t>(nn);
-    return carve_bin(cv, nn, nb, w.bin) && !cv.overflow;
 }
 
 int mpmx_hashgrid_build(int n, const float* x, float cell_width, int dimx, int dimy, int dimz, int32_t* point_ids,
@@ -1300,50 +1010,26 @@ int mpmx_solve_particle_particle_contacts_packed(
     const float* radius, const float* xr, const float* vw, const int32_t* flags, float k_mu, float static_threshold,
     float static_friction, float k_cohesion, float max_radius, float dt, float relaxation, int contact_lock,
     float* deltas, const int32_t* slot_of_id, const uint64_t* xpbd_near, const int32_t* contact_list,
-    const int32_t* contact_count, void* brick_ws, size_t brick_ws_bytes, void* stream) {
+    const int32_t* contact_count, void* stream) {
     MPMX_REQUIRE(n >= 0 && dimx > 0 && dimy > 0 && dimz > 0 && xpbd_dimx > 0 && xpbd_dimy > 0 && xpbd_dimz > 0 &&
                      cell_width > 0.0f,
                  "mpmx_solve_particle_particle_contacts_packed: bad sizes");
     MPMX_REQUIRE((contact_list == nullptr) == (contact_count == nullptr),
                  "mpmx_solve_particle_particle_contacts_packed: give the contact list with its count");
-    MPMX_REQUIRE(!brick_ws || (contact_list && brick_ws_bytes >= mpmx_xpbd_brick_workspace_bytes(n, dimx, dimy, dimz)),
-                 "mpmx_solve_particle_particle_contacts_packed: the brick form needs a contact list and "
-                 "mpmx_xpbd_brick_workspace_bytes of workspace");
     if (n == 0) return 0;
     MPMX_REQUIRE(activeLabel && materialLabel && point_ids && cell_start && xpbd_point_ids && xpbd_cell_start && x &&
                      v && mass && radius && xr && vw && flags && deltas,
                  "mpmx_solve_particle_particle_contacts_packed: null pointer");
     MPMX_REQUIRE(((uintptr_t)xr | (uintptr_t)vw) % 16 == 0,
                  "mpmx_solve_particle_particle_contacts_packed: records need 16-byte alignment");
-    // with a list, a bounded grid strides over it (8 workgroups per CU at most)
-    const unsigned blocks = contact_list ? (grid_for(n, kBlock) < 2048u ? grid_for(n, kBlock) : 2048u)
-                                         : grid_for(n, kBlock);
+    // one thread per table position (a short contact list is read by the first threads)
+    const unsigned blocks = grid_for(n, kBlock);
     MPMX_LAUNCH(particle_contacts_kernel<true>, dim3(blocks), dim3(kBlock), 0, as_stream(stream), n, activeLabel,
                 materialLabel, point_ids, cell_start, xpbd_point_ids, xpbd_cell_start, 1.0f / cell_width, dimx, dimy,
                 dimz, xpbd_dimx, xpbd_dimy, xpbd_dimz, x, v, mass, radius, k_mu, static_threshold, static_friction,
                 k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, slot_of_id,
                 (ContactPack{(const float4*)xr, (const float4*)vw, flags, xpbd_near, contact_list, contact_count}));
-    int rc = check_launch("particle_contacts_kernel<true>", as_stream(stream));
-    if (rc || !brick_ws) return rc;
-    // the label-2 walks (the contact list then holds the others), brick by brick
-    hipStream_t s = as_stream(stream);
-    const long nb = brick_total(dimx, dimy, dimz);
-    const int nbx = (dimx + kBrX - 1) / kBrX, nby = (dimy + kBrY - 1) / kBrY;
-    MPMX_REQUIRE(nb < (1L << 31), "mpmx_solve_particle_particle_contacts_packed: too many bricks");
-    BrickWork w;
-    MPMX_REQUIRE(carve_bricks(brick_ws, brick_ws_bytes, n, nb, w), "contacts: brick workspace carve failed");
-    const float inv = 1.0f / cell_width;
-    MPMX_LAUNCH(brick_key_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, n, point_ids, materialLabel, x, inv,
-                dimx, dimy, dimz, nbx, nby, w.keys);
-    if ((rc = check_launch("brick_key", s))) return rc;
-    if ((rc = counting_sort_ids(w.keys, n, nb, w.bucket, w.bin, s))) return rc;
-    const unsigned bblocks = nb < 768 ? (unsigned)nb : 768u;  // three workgroups per CU (LDS)
-    MPMX_LAUNCH(particle_contacts_brick_kernel, dim3(bblocks), dim3(kBlock), 0, s, n, activeLabel, materialLabel,
-                point_ids, cell_start, inv, dimx, dimy, dimz, nbx, nby, x, v, mass, radius, k_mu, static_threshold,
-                static_friction, k_cohesion, max_radius, dt, relaxation, contact_lock, deltas, slot_of_id,
-                (ContactPack{(const float4*)xr, (const float4*)vw, flags, xpbd_near, nullptr, nullptr}), (int)nb,
-                w.bin.cell_start, w.bucket);
-    return check_launch("particle_contacts_brick_kernel", as_stream(stream));
+    return check_launch("particle_contacts_kernel<true>", as_stream(stream));
 }
 
 int mpmx_apply_particle_deltas(int n, const int32_t* activeLabel, const int32_t* materialLabel, const float* x_orig,

@@ -325,22 +325,6 @@ def _contact_list(xpbd, n):
     return cl
 
 
-# the label-2 contact walks run brick by brick from LDS-staged candidates (the
-# brick form of mpmx_solve_particle_particle_contacts_packed); False walks them
-# from the contact list like the others (the same bits)
-CONTACT_BRICKS = True
-
-
-def _brick_workspace(xpbd, n, dims):
-    """Workspace of the brick form."""
-    nbytes = int(lib.raw("mpmx_xpbd_brick_workspace_bytes")(n, *dims))
-    ws = getattr(xpbd, "_brick_ws", None)
-    if ws is None or ws.numel() < nbytes:
-        ws = torch.empty(nbytes, dtype=torch.uint8, device=rt.device())
-        xpbd._brick_ws = ws
-    return C.c_void_p(ws.data_ptr()), ws.numel()
-
-
 def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     """xpbdSimulationStep with the copies folded into its kernels: x, v, the
     labels, particle_cumDist_xpbd and the sleep counters end exactly as after
@@ -375,12 +359,9 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
     bounds.xpbd_count = xgrid.cell_start.ptr.value + 4 * int(np.prod(xgrid.dims))  # no records without XPBD points
     bref = C.byref(bounds)
     dims = grid.dims
-    bricks = CONTACT_BRICKS
     lib("mpmx_xpbd_contact_list", n, sim.materialLabel.ptr, grid.point_pos.ptr, xgrid.cell_start.ptr,
         float(grid.cell_width), *xgrid.dims, sim.particle_x.ptr, sim.particle_radius.ptr,
-        float(xpbd.particle_cohesion), float(xpbd.max_radius), near.ptr, 1 if bricks else 0, clist.ptr, ccount.ptr,
-        s)
-    bws, bnb = _brick_workspace(xpbd, n, grid.dims) if bricks else (None, 0)
+        float(xpbd.particle_cohesion), float(xpbd.max_radius), near.ptr, clist.ptr, ccount.ptr, s)
     lib("mpmx_xpbd_begin", n, sim.activeLabel.ptr, sim.materialLabel.ptr, sim.particle_x.ptr, sim.particle_v.ptr,
         float(g[0]), float(g[1]), float(g[2]), float(sim.dtxpbd), float(xpbd.particle_v_max), reset, xi.ptr, vi.ptr,
         sim.numTotalXPBD.ptr, bref if iters > 0 else None, s)  # also zeroes numTotalXPBD and numSleepingXPBD
@@ -394,8 +375,7 @@ def xpbdSimulationStepFused(sim, mpm, xpbd, device=None, xpbd_only=False):
             sim.particle_radius.ptr, xr.ptr, vw.ptr, flags.ptr, float(xpbd.dynamicParticleFriction),
             float(xpbd.staticVelocityThreshold), float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion),
             float(xpbd.max_radius), float(sim.dtxpbd), float(xpbd.xpbd_relaxation), int(contact_lock),
-            xpbd.particle_delta.ptr, _aptr(getattr(sim, "_slot_of_id", None)), near.ptr, clist.ptr, ccount.ptr,
-            bws, bnb, s)
+            xpbd.particle_delta.ptr, _aptr(getattr(sim, "_slot_of_id", None)), near.ptr, clist.ptr, ccount.ptr, s)
         if it + 1 < iters:
             lib("mpmx_xpbd_apply_deltas_inplace", n, sim.activeLabel.ptr, sim.particle_x.ptr, xpbd.particle_delta.ptr,
                 float(sim.dtxpbd), float(xpbd.particle_v_max), xi.ptr, vi.ptr, sim.materialLabel.ptr, bref, s)

@@ -355,27 +355,25 @@ def test_packed_contacts_from_gathered_records(lock, few):
     from the particle arrays gives the deltas and labels of the split solve,
     bit for bit -- with the XPBD table's neighbourhood bitmap letting MPM
     particles far from every XPBD point skip their walk (few: only a handful of
-    XPBD particles, as just after the first transitions of a run), and with the
-    label-2 walks run brick by brick from LDS-staged halos (bricks)."""
+    XPBD particles, as just after the first transitions of a run; with half the
+    points in the XPBD phase no list is written and every position walks)."""
     import torch
     from mpmwarpydw_amd import runtime as rt
     from mpmwarpydw_amd._lib import lib
     outs = []
-    for mode in ("split", "packed", "bricks"):
-        packed = mode != "split"
+    for packed in (False, True):
         S = xpbd_scene(7, cluster=0 if few else 32)
         sim, xpbd = S.sim, S.xpbd
         n = sim.nPoints
         if few:
             lab = sim.materialLabel.numpy()
-            keep = np.flatnonzero(lab == 2)[::97][:6]
+            keep = np.flatnonzero(lab == 2)[::97][:2]
             lab[lab == 2] = 1
             lab[keep] = 2
             sim.materialLabel.assign(lab)
         grid, xgrid = xpbd.particle_grid, xpbd.particle_grid_xpbd
         grid.build_pair(xgrid, sim.particle_x, sim.dx, sim.materialLabel, 2, None)
         xpbd.particle_delta.zero_()
-        extra = []
         if packed:
             xr = torch.empty(2 * n, 4, device=rt.device())
             vw = torch.empty(2 * n, 4, device=rt.device())
@@ -390,27 +388,23 @@ def test_packed_contacts_from_gathered_records(lock, few):
             ccount = torch.empty(1, dtype=torch.int32, device=rt.device())
             lib("mpmx_xpbd_contact_list", n, sim.materialLabel.ptr, grid.point_pos.ptr, xgrid.cell_start.ptr,
                 float(grid.cell_width), *xgrid.dims, sim.particle_x.ptr, sim.particle_radius.ptr,
-                float(xpbd.particle_cohesion), float(xpbd.max_radius), near, 1 if mode == "bricks" else 0,
-                clist.data_ptr(), ccount.data_ptr(), rt.stream_handle())
+                float(xpbd.particle_cohesion), float(xpbd.max_radius), near, clist.data_ptr(), ccount.data_ptr(),
+                rt.stream_handle())
             listed = int(ccount.item())
             print(f"  contact list: {listed} of {n} table positions", flush=True)
-            assert 0 < listed <= n and (listed < n // 4 or not few)
-            bricks = [None, 0]
-            if mode == "bricks":
-                nbytes = int(lib.raw("mpmx_xpbd_brick_workspace_bytes")(n, *grid.dims))
-                bws = torch.empty(nbytes, dtype=torch.uint8, device=rt.device())
-                bricks = [bws.data_ptr(), nbytes]
-            extra = [near, clist.data_ptr(), ccount.data_ptr(), *bricks]
+            if few:
+                assert 0 < listed <= n // 8  # the solve walks the listed positions only
+            else:
+                assert listed == n + 1  # half the points are XPBD: no list, every table position walks
         name = "mpmx_solve_particle_particle_contacts_" + ("packed" if packed else "split")
-        print(f"[{name}] mode={mode} n={n} lock={lock}", flush=True)
+        print(f"[{name}] n={n} lock={lock}", flush=True)
         lib(name, n, sim.activeLabel.ptr, sim.materialLabel.ptr, grid.point_ids.ptr, grid.cell_start.ptr,
             xgrid.point_ids.ptr, xgrid.cell_start.ptr, float(grid.cell_width), *grid.dims, *xgrid.dims,
             sim.particle_x.ptr, sim.particle_v.ptr, sim.particle_mass.ptr, sim.particle_radius.ptr,
             *(rec if packed else []), float(xpbd.dynamicParticleFriction), float(xpbd.staticVelocityThreshold),
             float(xpbd.staticParticleFriction), float(xpbd.particle_cohesion), float(xpbd.max_radius),
-            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), lock, xpbd.particle_delta.ptr, None, *extra,
-            rt.stream_handle())
+            float(sim.dtxpbd), float(xpbd.xpbd_relaxation), lock, xpbd.particle_delta.ptr, None,
+            *([near, clist.data_ptr(), ccount.data_ptr()] if packed else []), rt.stream_handle())
         outs.append((xpbd.particle_delta.numpy(), sim.materialLabel.numpy()))
     assert np.abs(outs[0][0]).max() > 0
-    for d, lab in outs[1:]:
-        assert np.array_equal(outs[0][0], d) and np.array_equal(outs[0][1], lab)
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
