@@ -614,6 +614,48 @@ def _ensemble_graph(E, stream, per_graph, learned=None, counters=None, tol=1e-4)
     return graph
 
 
+def plain_per_scene(E, stream, steps, per_graph=10):
+    """The plain GPU path over the ensemble's scenes one at a time, as the
+    reference runs a scene per process: each scene alone, in tile order, from
+    its t = 0 state, one eager substep, then ``steps`` MPM substeps replayed
+    from a HIP graph of ``per_graph`` fused substeps.  Returns the aggregate
+    particle-substeps/s (the scenes' particle-substeps over the sum of their
+    wall times) and the per-scene figures of the smallest and largest scene."""
+    import torch
+    from mpmwarpydw_amd import simulationRoutines as SR
+    reps = max(1, steps // per_graph)
+    walls, work, rows = 0.0, 0, []
+    for S in E.scenes_sims:
+        S.reorder()
+        sub = lambda: SR.mpmSimulationStepFused(S.sim, S.mpm, S.xpbd)  # noqa: E731
+        with torch.cuda.stream(stream):
+            sub()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            for _ in range(per_graph):
+                sub()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for _ in range(reps):
+                g.replay()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        del g
+        lab, act = S.sim.materialLabel.numpy(), S.sim.activeLabel.numpy()
+        n_mpm = int(((act == 1) & (lab <= 1)).sum())
+        walls += wall
+        work += n_mpm * reps * per_graph
+        rows.append((S.sim.nPoints, wall * 1e3 / (reps * per_graph)))
+    rows.sort()
+    return {"value": work / walls, "ms_per_step": walls * 1e3 / (reps * per_graph), "steps": reps * per_graph,
+            "smallest_scene": {"particles": rows[0][0], "ms_per_step": round(rows[0][1], 4)},
+            "largest_scene": {"particles": rows[-1][0], "ms_per_step": round(rows[-1][1], 4)},
+            "what": "each scene alone through the plain fused substep (substeps 2 to steps + 1 from t = 0, HIP "
+                    f"graphs of {per_graph} substeps), one scene after another; ms_per_step sums the scenes"}
+
+
 def bench_learned_batched(a, net=None, net_info=None):
     """The batched ensemble (64 block scenes of 20k-500k particles, seeded
     materials) for ``a.ensemble_steps`` substeps from t = 0, once with the plain
@@ -702,6 +744,9 @@ def bench_learned_batched(a, net=None, net_info=None):
                                    f"cohesion) of substep {steps + 1} of the batched ensemble, 2,000,000 particles "
                                    "drawn at random")
     autotune.apply(knobs_before)
+    # the plain GPU path without batching: every scene alone, one after another
+    per_scene = plain_per_scene(E, stream, steps)
+    log(f"[bench] plain, scene by scene: {per_scene}")
     res = {
         "metric": "MPM particle-substeps/s (learned+batched)",
         "value": out["learned"]["value"],
@@ -714,7 +759,10 @@ def bench_learned_batched(a, net=None, net_info=None):
                     "part of this measurement), HIP graphs of 10 substeps",
         "learned": out["learned"],
         "plain_batched": out["plain"],
+        "plain_per_scene": per_scene,
         "learned_vs_plain_batched": out["learned"]["value"] / out["plain"]["value"],
+        "learned_batched_vs_plain_per_scene": out["learned"]["value"] / per_scene["value"],
+        "plain_batched_vs_plain_per_scene": out["plain"]["value"] / per_scene["value"],
         "judge_tolerance": a.learned_tol,
         "stress_error_vs_fp64": err,
         "kernels_us_per_launch": kernels,

@@ -320,7 +320,11 @@ int launch_stress_learned(const mpmx_particles& P, float eff, const float* param
     MPMX_LAUNCH(stress_learned_pass1_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, P, eff, use_net, flag, counters);
     int rc = check_launch("stress_learned(plain)", s);
     if (rc || !use_net) return rc;
-    MPMX_LAUNCH(stress_learned_pass2_kernel, dim3((unsigned)nb), dim3(kBlock), 0, s, P, eff, params, tol, flag,
+    // pass 2 with a quarter of the waves per SIMD of pass 1 at most: every wavefront then
+    // scans enough rounds to fill its batches of 64 candidates (with 8192 workgroups a
+    // wavefront met about 15 candidates, and three quarters of its lanes idled)
+    const long nb2 = nb < 1024 ? nb : 1024;
+    MPMX_LAUNCH(stress_learned_pass2_kernel, dim3((unsigned)nb2), dim3(kBlock), 0, s, P, eff, params, tol, flag,
                 counters);
     return check_launch("stress_learned(network)", s);
 }
