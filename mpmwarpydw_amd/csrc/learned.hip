@@ -183,16 +183,16 @@ __global__ void __launch_bounds__(kBlock) stress_learned_pass1_kernel(mpmx_parti
 
 // The k <= 64 candidates q[0..k) of one wavefront, one per lane: features,
 // the network on the matrix cores (ceil(k / 16) tiles of 16), the judge and
-// the finish.  cu_begin runs twice per candidate (before and after the
-// tiles), so that no update state is live across them.
+// the finish.  The update state of cu_begin stays in registers across the
+// tiles.
 __device__ __forceinline__ void lrm_process(const mpmx_particles& P, float eff, const MlpWeights& w, const int* q,
                                             int k, int lane, float tol, int& n_plastic, int& n_accept, int& n_done) {
     const bool act = lane < k;
     const long p = act ? q[lane] : 0;
     float x[10];
     for (int j = 0; j < 10; j++) x[j] = 0.0f;
+    CuState c;
     if (act) {
-        CuState c;
         cu_begin(P, p, c);
         if (lrm_candidate(c)) {
             float dh[3];
@@ -229,8 +229,6 @@ __device__ __forceinline__ void lrm_process(const mpmx_particles& P, float eff, 
         }
     }
     if (!act) return;
-    CuState c;
-    cu_begin(P, p, c);
     n_done++;
     n_plastic += c.branch != 0 ? 1 : 0;
     float T[9];
