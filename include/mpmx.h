@@ -446,7 +446,10 @@ int mpmx_hashgrid_build_ordered(int n, const float* x, float cell_width, int dim
  * the second table; written only when the second table holds a point (else
  * left as they were).  lazy != 0: while no point has the label, the first
  * table (point_ids, cell_start) is left as it was; the fused XPBD step's
- * contact solve then reads neither table.  Workspace:
+ * contact solve then reads neither table.  After a lazy build, read the first
+ * table only when the second holds a point (label_cell_start[label_dimx *
+ * label_dimy * label_dimz] > 0): otherwise it may list the ids of an earlier
+ * build, stale after a reorder of the particle arrays.  Workspace:
  * mpmx_hashgrid_pair_workspace_bytes. */
 size_t mpmx_hashgrid_pair_workspace_bytes(int n, int dimx, int dimy, int dimz, int label_dimx,
                                           int label_dimy, int label_dimz);

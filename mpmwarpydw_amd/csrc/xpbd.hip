@@ -1232,9 +1232,10 @@ __global__ void apply_deltas_inplace_kernel(int n, const int32_t* __restrict__ a
                                             int bound, mpmx_xpbd_bounds B) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    // With no XPBD point every delta is 0 (only label-2 particles get wall or contact
-    // deltas): the application would leave x_int and set v_int to 0, which the finish
-    // kernel recomputes or overwrites for every particle, so it is skipped.
+    // With no XPBD point every delta is 0: wall deltas go to label-2 particles only,
+    // and there is no MPM-XPBD pair.  The application would then leave x_int and set
+    // v_int to 0, which the finish kernel recomputes or overwrites for every
+    // particle, so it is skipped.
     if (bound && B.xpbd_count && *B.xpbd_count == 0) return;
     const int a = active[t];
     f3 nd{0.0f, 0.0f, 0.0f};

@@ -278,12 +278,17 @@ def test_split_table_is_bit_identical(lock, xdims):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("lock,cluster,mpm_only", [(0, 0, False), (1, 0, False), (1, 64, False), (0, 0, True)])
-def test_fused_xpbd_step_matches_launch_sequence(lock, cluster, mpm_only):
+@pytest.mark.parametrize("lock,cluster,mpm_only,xpbd_only", [(0, 0, False, False), (1, 0, False, False),
+                                                              (1, 64, False, False), (0, 0, True, False),
+                                                              (1, 0, True, False), (0, 0, False, True),
+                                                              (1, 64, False, True)])
+def test_fused_xpbd_step_matches_launch_sequence(lock, cluster, mpm_only, xpbd_only):
     """The fused XPBD step (packed candidate records; with no XPBD particle the
     lazy tables and skipped delta applications) leaves x, v, labels, the
     cumulative distance and the sleep counters bit-identical to the
-    launch-for-launch step."""
+    launch-for-launch step, in the combined phase (with and without the
+    contact lock) and in the XPBD-only phase, where MPM velocities come from
+    the last delta application instead of the contact impulses."""
     from mpmwarpydw_amd import simulationRoutines as SR
     outs = []
     for fused in (False, True):
@@ -293,7 +298,9 @@ def test_fused_xpbd_step_matches_launch_sequence(lock, cluster, mpm_only):
             S.sim.materialLabel.assign(np.where(np.arange(n) % 3 == 0, 0, 1).astype(np.int32))
             S.sim.particle_v.assign(np.random.default_rng(5).normal(0, 30.0, (n, 3)))
         S.xpbd.mpm_contact_transition_lock = lock
-        (SR.xpbdSimulationStepFused if fused else SR.xpbdSimulationStep)(S.sim, S.mpm, S.xpbd)
+        print(f"[fused xpbd] fused={fused} lock={lock} cluster={cluster} mpm_only={mpm_only} xpbd_only={xpbd_only}",
+              flush=True)
+        (SR.xpbdSimulationStepFused if fused else SR.xpbdSimulationStep)(S.sim, S.mpm, S.xpbd, xpbd_only=xpbd_only)
         h = host_state(S)
         outs.append((h, S.sim.numTotalXPBD.numpy()[0], S.sim.numSleepingXPBD.numpy()[0]))
     (a, ta, sa), (b, tb, sb) = outs
