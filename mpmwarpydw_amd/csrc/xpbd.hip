@@ -338,6 +338,7 @@ __global__ void contact_pack_kernel(int n, const int32_t* __restrict__ active, c
 // scan in step, and the drain's divisions run for near candidates only (solved
 // inline, they ran for the whole wave whenever one of its 64 lanes had a contact).
 constexpr int kContactQueue = 24;  // queued candidates per lane
+constexpr int kScanBatch = 4;  // candidates whose records one lane loads together in the scan (8: slower)
 constexpr int kContactFlush = 12;  // a wave drains once one of its lanes holds this many
 
 // False only where the exact test d - ri - rj <= k_coh fails: the squared distance
@@ -550,6 +551,35 @@ __device__ __forceinline__ void contact_one(
             for (int cx = xs; cx <= xe; cx++) {
                 int cell = hash_index(hx, hy, hz, cx, cy, cz);
                 int b = starts[cell], e = starts[cell + 1];
+                if constexpr (PACKED) {
+                    // kScanBatch candidates per batch: their records load together
+                    for (int s0 = b; s0 < e; s0 += kScanBatch) {
+                        int fv[kScanBatch];
+                        float4 pv[kScanBatch];
+#pragma unroll
+                        for (int u = 0; u < kScanBatch; u++) {
+                            const int su = s0 + u < e ? s0 + u : s0;
+                            fv[u] = pk.flags[rec + su];
+                            pv[u] = pk.xr[rec + su];
+                        }
+#pragma unroll
+                        for (int u = 0; u < kScanBatch; u++) {
+                            const int s = s0 + u;
+                            if (s >= e) break;
+                            // s is j's position in the walked table: s == self exactly when j == i
+                            const int fj = fv[u];
+                            if (!(fj & kCandValid) || s == self) continue;
+                            if (i_mpm && (fj & kCandMpm)) continue;
+                            const float4 pj = pv[u];
+                            const float nx = xi.x - pj.x, ny = xi.y - pj.y, nz = xi.z - pj.z;
+                            if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, pj.w)) continue;
+                            if (queued == kContactQueue) drain();  // a lane with many near candidates
+                            queue[queued++][threadIdx.x] = s;
+                        }
+                    }
+                    if (__any(queued >= kContactFlush)) drain();
+                    continue;
+                }
                 // unpacked: the next candidate's id loads while this one is tested
                 int j_next = (!PACKED && b < e) ? ids[b] : 0;
                 for (int s = b; s < e; s++) {
