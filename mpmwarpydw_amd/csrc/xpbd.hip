@@ -546,68 +546,81 @@ __device__ __forceinline__ void contact_one(
         }
         queued = 0;
     };
-    for (int cz = zs; cz <= ze; cz++)
-        for (int cy = ys; cy <= ye; cy++)
-            for (int cx = xs; cx <= xe; cx++) {
-                int cell = hash_index(hx, hy, hz, cx, cy, cz);
-                int b = starts[cell], e = starts[cell + 1];
-                if constexpr (PACKED) {
-                    // kScanBatch candidates per batch: their records load together
+    if constexpr (PACKED) {
+        // The cells of one x-row of the query box are consecutive keys, so their
+        // candidates are one contiguous range of the table, or two where the row
+        // wraps round the hash dims (the wrapped part follows, as the walk visits
+        // it).  A row costs two cell-start loads instead of two per cell, and its
+        // candidates are scanned kScanBatch at a time with their records loaded
+        // together; the order of the queued candidates is the walk's.
+        const float4* __restrict__ xr = pk.xr + rec;
+        const int32_t* __restrict__ fl = pk.flags + rec;
+        int wx0, wy0, wz0;
+        hash_cell(hx, hy, hz, xs, ys, zs, wx0, wy0, wz0);
+        const int wx1 = wx0 + (xe - xs);  // past hx-1 where the row wraps
+        const int npiece = wx1 < hx ? 1 : 2;
+        for (int cz = zs, wz = wz0; cz <= ze; cz++, wz = wz + 1 == hz ? 0 : wz + 1)
+            for (int cy = ys, wy = wy0; cy <= ye; cy++, wy = wy + 1 == hy ? 0 : wy + 1) {
+                const int row = (wz * hy + wy) * hx;
+                for (int piece = 0; piece < npiece; piece++) {
+                    const int lo = piece == 0 ? wx0 : 0;
+                    const int hi = piece == 0 ? (npiece == 1 ? wx1 : hx - 1) : wx1 - hx;
+                    const int b = starts[row + lo], e = starts[row + hi + 1];
                     for (int s0 = b; s0 < e; s0 += kScanBatch) {
                         int fv[kScanBatch];
                         float4 pv[kScanBatch];
 #pragma unroll
                         for (int u = 0; u < kScanBatch; u++) {
                             const int su = s0 + u < e ? s0 + u : s0;
-                            fv[u] = pk.flags[rec + su];
-                            pv[u] = pk.xr[rec + su];
+                            fv[u] = fl[su];
+                            pv[u] = xr[su];
                         }
+                        // every test evaluated for the whole batch (no branch per candidate)
+                        unsigned keep = 0;
 #pragma unroll
                         for (int u = 0; u < kScanBatch; u++) {
-                            const int s = s0 + u;
-                            if (s >= e) break;
-                            // s is j's position in the walked table: s == self exactly when j == i
+                            const int s = s0 + u;  // j's position in the walked table: s == self exactly when j == i
                             const int fj = fv[u];
-                            if (!(fj & kCandValid) || s == self) continue;
-                            if (i_mpm && (fj & kCandMpm)) continue;
-                            const float4 pj = pv[u];
-                            const float nx = xi.x - pj.x, ny = xi.y - pj.y, nz = xi.z - pj.z;
-                            if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, pj.w)) continue;
-                            if (queued == kContactQueue) drain();  // a lane with many near candidates
-                            queue[queued++][threadIdx.x] = s;
+                            const float nx = xi.x - pv[u].x, ny = xi.y - pv[u].y, nz = xi.z - pv[u].z;
+                            const bool k = (s < e) & ((fj & kCandValid) != 0) & (s != self) &
+                                           !(i_mpm & ((fj & kCandMpm) != 0)) &
+                                           maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, pv[u].w);
+                            keep |= (unsigned)k << u;
+                        }
+                        if (keep) {
+                            // room for the whole batch (a lane with many near candidates)
+                            if (queued > kContactQueue - kScanBatch) drain();
+#pragma unroll
+                            for (int u = 0; u < kScanBatch; u++)
+                                if (keep & (1u << u)) queue[queued++][threadIdx.x] = s0 + u;
                         }
                     }
-                    if (__any(queued >= kContactFlush)) drain();
-                    continue;
                 }
-                // unpacked: the next candidate's id loads while this one is tested
-                int j_next = (!PACKED && b < e) ? ids[b] : 0;
-                for (int s = b; s < e; s++) {
-                    float rj;
-                    f3 xj;
-                    if constexpr (PACKED) {
-                        // s is j's position in the walked table: s == self exactly when j == i
-                        const int fj = pk.flags[rec + s];
-                        const float4 pj = pk.xr[rec + s];
-                        if (!(fj & kCandValid) || s == self) continue;
-                        if (i_mpm && (fj & kCandMpm)) continue;
-                        rj = pj.w;
-                        xj = f3{pj.x, pj.y, pj.z};
-                    } else {
+                if (__any(queued >= kContactFlush)) drain();
+            }
+    } else {
+        for (int cz = zs; cz <= ze; cz++)
+            for (int cy = ys; cy <= ye; cy++)
+                for (int cx = xs; cx <= xe; cx++) {
+                    const int cell = hash_index(hx, hy, hz, cx, cy, cz);
+                    const int b = starts[cell], e = starts[cell + 1];
+                    // the next candidate's id loads while this one is tested
+                    int j_next = b < e ? ids[b] : 0;
+                    for (int s = b; s < e; s++) {
                         const int j = j_next;
                         if (s + 1 < e) j_next = ids[s + 1];
                         if (!(active[j] == 1 && j != i && mass[j] > 0.0f)) continue;
                         if (i_mpm && mat[j] <= 1) continue;
-                        rj = radius[j];
-                        xj = ld3(x, n, j);
+                        const float rj = radius[j];
+                        const f3 xj = ld3(x, n, j);
+                        const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
+                        if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
+                        if (queued == kContactQueue) drain();  // a lane with many near candidates
+                        queue[queued++][threadIdx.x] = s;
                     }
-                    const float nx = xi.x - xj.x, ny = xi.y - xj.y, nz = xi.z - xj.z;
-                    if (!maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, rj)) continue;
-                    if (queued == kContactQueue) drain();  // a lane with many near candidates
-                    queue[queued++][threadIdx.x] = s;
+                    if (__any(queued >= kContactFlush)) drain();
                 }
-                if (__any(queued >= kContactFlush)) drain();
-            }
+    }
     drain();
     // point_ids is a permutation, so particle i has exactly one writer here
     deltas[i] += d0 * relax;
