@@ -488,7 +488,11 @@ __device__ __forceinline__ void contact_one(
             float rj, w2;
             bool j_mpm, j_xpbd;
             int j = -1;
+            int fj = 0;
             if constexpr (PACKED) {
+                // the scan's candidate filters: valid, not i itself, no MPM-MPM pair
+                fj = pk.flags[rec + sq];
+                if (!(fj & kCandValid) || sq == self || (i_mpm && (fj & kCandMpm))) continue;
                 const float4 pj = pk.xr[rec + sq];
                 xj = f3{pj.x, pj.y, pj.z};
                 rj = pj.w;
@@ -513,7 +517,6 @@ __device__ __forceinline__ void contact_one(
             if (!(denom > 0.0f)) continue;
             if (lock == 1) {
                 if constexpr (PACKED) {
-                    const int fj = pk.flags[rec + sq];
                     j_mpm = (fj & kCandMpm) != 0;
                     j_xpbd = (fj & kCandXpbd) != 0;
                 } else {
@@ -552,9 +555,12 @@ __device__ __forceinline__ void contact_one(
         // wraps round the hash dims (the wrapped part follows, as the walk visits
         // it).  A row costs two cell-start loads instead of two per cell, and its
         // candidates are scanned kScanBatch at a time with their records loaded
-        // together; the order of the queued candidates is the walk's.
+        // together; the order of the queued candidates is the walk's.  The scan
+        // reads only the (x, radius) records: the flag tests and the self test
+        // are filters like the reach test, so the drain makes them on the few
+        // queued candidates (an invalid candidate's record is stale, and any
+        // value there only queues it).
         const float4* __restrict__ xr = pk.xr + rec;
-        const int32_t* __restrict__ fl = pk.flags + rec;
         int wx0, wy0, wz0;
         hash_cell(hx, hy, hz, xs, ys, zs, wx0, wy0, wz0);
         const int wx1 = wx0 + (xe - xs);  // past hx-1 where the row wraps
@@ -567,23 +573,15 @@ __device__ __forceinline__ void contact_one(
                     const int hi = piece == 0 ? (npiece == 1 ? wx1 : hx - 1) : wx1 - hx;
                     const int b = starts[row + lo], e = starts[row + hi + 1];
                     for (int s0 = b; s0 < e; s0 += kScanBatch) {
-                        int fv[kScanBatch];
                         float4 pv[kScanBatch];
 #pragma unroll
-                        for (int u = 0; u < kScanBatch; u++) {
-                            const int su = s0 + u < e ? s0 + u : s0;
-                            fv[u] = fl[su];
-                            pv[u] = xr[su];
-                        }
+                        for (int u = 0; u < kScanBatch; u++) pv[u] = xr[s0 + u < e ? s0 + u : s0];
                         // every test evaluated for the whole batch (no branch per candidate)
                         unsigned keep = 0;
 #pragma unroll
                         for (int u = 0; u < kScanBatch; u++) {
-                            const int s = s0 + u;  // j's position in the walked table: s == self exactly when j == i
-                            const int fj = fv[u];
                             const float nx = xi.x - pv[u].x, ny = xi.y - pv[u].y, nz = xi.z - pv[u].z;
-                            const bool k = (s < e) & ((fj & kCandValid) != 0) & (s != self) &
-                                           !(i_mpm & ((fj & kCandMpm) != 0)) &
+                            const bool k = (s0 + u < e) &
                                            maybe_in_reach(fmaf(nz, nz, fmaf(ny, ny, nx * nx)), base, pv[u].w);
                             keep |= (unsigned)k << u;
                         }
