@@ -489,11 +489,15 @@ __device__ __forceinline__ void contact_one(
             bool j_mpm, j_xpbd;
             int j = -1;
             int fj = 0;
+            float4 qj;  // PACKED: (v, 1 / mass)
             if constexpr (PACKED) {
-                // the scan's candidate filters: valid, not i itself, no MPM-MPM pair
+                // the three records load together (sq is a table position, so all are
+                // in bounds); then the scan's candidate filters: valid, not i itself,
+                // no MPM-MPM pair
                 fj = pk.flags[rec + sq];
-                if (!(fj & kCandValid) || sq == self || (i_mpm && (fj & kCandMpm))) continue;
                 const float4 pj = pk.xr[rec + sq];
+                qj = pk.vw[rec + sq];
+                if (!(fj & kCandValid) || sq == self || (i_mpm && (fj & kCandMpm))) continue;
                 xj = f3{pj.x, pj.y, pj.z};
                 rj = pj.w;
             } else {
@@ -506,7 +510,6 @@ __device__ __forceinline__ void contact_one(
             const float err = d - ri - rj;
             if (!(err <= k_coh)) continue;
             if constexpr (PACKED) {
-                const float4 qj = pk.vw[rec + sq];
                 vj = f3{qj.x, qj.y, qj.z};
                 w2 = qj.w;
             } else {
