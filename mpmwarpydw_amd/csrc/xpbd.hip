@@ -482,8 +482,21 @@ __device__ __forceinline__ void contact_one(
     int queued = 0;
     // the exact contact test and the contact of every queued candidate, in order
     auto drain = [&]() {
+        // PACKED: the three records of a queued candidate load together, one
+        // candidate ahead of the one being solved (table positions, all in bounds)
+        int next_sq = 0, next_f = 0;
+        float4 next_p = {}, next_q = {};
+        auto fetch = [&](int k) {
+            next_sq = queue[k][threadIdx.x];
+            next_f = pk.flags[rec + next_sq];
+            next_p = pk.xr[rec + next_sq];
+            next_q = pk.vw[rec + next_sq];
+        };
+        if constexpr (PACKED) {
+            if (queued > 0) fetch(0);
+        }
         for (int k = 0; k < queued; k++) {
-            const int sq = queue[k][threadIdx.x];
+            int sq;
             f3 xj, vj;
             float rj, w2;
             bool j_mpm, j_xpbd;
@@ -491,16 +504,17 @@ __device__ __forceinline__ void contact_one(
             int fj = 0;
             float4 qj;  // PACKED: (v, 1 / mass)
             if constexpr (PACKED) {
-                // the three records load together (sq is a table position, so all are
-                // in bounds); then the scan's candidate filters: valid, not i itself,
-                // no MPM-MPM pair
-                fj = pk.flags[rec + sq];
-                const float4 pj = pk.xr[rec + sq];
-                qj = pk.vw[rec + sq];
+                sq = next_sq;
+                fj = next_f;
+                const float4 pj = next_p;
+                qj = next_q;
+                if (k + 1 < queued) fetch(k + 1);
+                // the scan's candidate filters: valid, not i itself, no MPM-MPM pair
                 if (!(fj & kCandValid) || sq == self || (i_mpm && (fj & kCandMpm))) continue;
                 xj = f3{pj.x, pj.y, pj.z};
                 rj = pj.w;
             } else {
+                sq = queue[k][threadIdx.x];
                 j = ids[sq];
                 xj = ld3(x, n, j);
                 rj = radius[j];
