@@ -332,9 +332,11 @@ __global__ void contact_pack_kernel(int n, const int32_t* __restrict__ active, c
 }
 
 // The contact walk runs in two passes over each lane's candidates.  The scan
-// applies the reference's filters and keeps the candidates within a margin of
-// contact reach, queued per lane in LDS; the drain takes them in the same order,
-// makes the exact contact test and solves the contacts.  The lanes of a wave
+// keeps the candidates within a margin of contact reach, queued per lane in LDS
+// (the unpacked walk applies the reference's validity and label filters there
+// too, the packed one leaves them to the drain); the drain takes them in the
+// same order, makes the filters and the exact contact test and solves the
+// contacts.  The lanes of a wave
 // scan in step, and the drain's divisions run for near candidates only (solved
 // inline, they ran for the whole wave whenever one of its 64 lanes had a contact).
 constexpr int kContactQueue = 24;  // queued candidates per lane
