@@ -124,13 +124,14 @@ const char* mpmx_last_error(void);
 
 /* Launch-shape knobs, process-wide (no reference counterpart), set by the
  * autotuner.  MPMX_TUNE_XCD_TILES = 1 deals the tiles of the tile kernels to
- * the 8 XCDs in contiguous ranges (neighbouring tiles share an L2), 0 in
- * block order.  MPMX_TUNE_SPLIT_STRESS = 1 runs the constitutive update of the
+ * the 8 XCDs in contiguous ranges (neighbouring tiles share an L2), 2 in runs
+ * of 16 tiles in turn (each XCD gets tiles from the whole range), 0 in block
+ * order.  MPMX_TUNE_SPLIT_STRESS = 1 runs the constitutive update of the
  * fused substep as its own particle-parallel kernel before a P2G-only tile
  * kernel, 0 inside the tile kernel.  MPMX_TUNE_G2P_TILES = 1 runs G2P per tile
  * with the tile's grid velocities staged in LDS, 2 likewise with two blocks
  * sharing each tile's particles, 0 one thread per particle with the grid read
- * through the caches.  The other knobs take 0 or 1.  Results do not depend on
+ * through the caches.  MPMX_TUNE_SPLIT_STRESS takes 0 or 1.  Results do not depend on
  * any knob. */
 #define MPMX_TUNE_XCD_TILES 0
 #define MPMX_TUNE_SPLIT_STRESS 1

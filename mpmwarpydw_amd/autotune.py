@@ -1,8 +1,9 @@
 """Autotuner of the fused substep's launch shape and particle order.
 
 Knobs (none changes results, only speed):
-  xcd_tiles   deal tiles to the chip's 8 XCDs in contiguous ranges (1) or in
-              block order (0)                      (mpmx_set_tuning)
+  xcd_tiles   deal tiles to the chip's 8 XCDs in contiguous ranges (1), in
+              runs of 16 tiles in turn (2) or in block order (0)
+                                                   (mpmx_set_tuning)
   split_stress run the constitutive update as its own particle kernel before
               a P2G-only tile kernel (1) or inside the tile kernel (0)
                                                    (mpmx_set_tuning)
@@ -37,8 +38,10 @@ MPMX_TUNE_P2G_CHUNK = 3
 P2G_CHUNKS = (512, 384, 256)  # knob value -> particles per LDS pass
 
 
-def set_xcd_tiles(on: bool) -> None:
-    lib("mpmx_set_tuning", MPMX_TUNE_XCD_TILES, 1 if on else 0)
+def set_xcd_tiles(mode: int) -> None:
+    """0: block order; 1: 8 contiguous ranges; 2: runs of 16 tiles in turn
+    (True and False stand for 1 and 0)."""
+    lib("mpmx_set_tuning", MPMX_TUNE_XCD_TILES, int(mode))
 
 
 def set_split_stress(on: bool) -> None:
@@ -71,7 +74,7 @@ def current() -> dict:
 
 def apply(cfg: dict) -> None:
     """Set the knobs of a candidate (as tune() logs them)."""
-    set_xcd_tiles(bool(cfg["xcd_tiles"]))
+    set_xcd_tiles(int(cfg["xcd_tiles"]))
     set_split_stress(bool(cfg["split_stress"]))
     set_g2p_tiles(cfg["g2p_tiles"])
     set_p2g_chunk(P2G_CHUNKS.index(cfg.get("p2g_chunk", 512)))
@@ -145,8 +148,8 @@ def tune(state, step, steps: int = 10, warmup: int = 3, trials: int = 2, orders=
                 set_p2g_chunk(chunk)
                 for g2p in (2, 1, 0):
                     set_g2p_tiles(g2p)
-                    for xcd in (1, 0):
-                        set_xcd_tiles(bool(xcd))
+                    for xcd in (1, 2, 0):
+                        set_xcd_tiles(xcd)
                         _restore(state, arrays, snap)
                         ms = _time(step, steps, warmup, trials)
                         log.append({"reorder": reorder, "split_stress": split,

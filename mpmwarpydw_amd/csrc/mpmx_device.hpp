@@ -26,6 +26,25 @@ MPMX_HD int xcd_tile(int b, int nb) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
+// Runs of kXcdRun consecutive tiles dealt to the XCDs in turn: neighbouring
+// tiles still share an L2 within a run, and a scene that fills only part of
+// the tile range (a block beside an empty region) spreads over every XCD,
+// where 8 contiguous ranges left some XCDs with empty tiles only.  A bijection
+// of [0, nb): blocks past the last whole round of 8 runs keep their own tile.
+constexpr int kXcdRun = 16;
+MPMX_HD int xcd_run_tile(int b, int nb) {
+    const int round = 8 * kXcdRun, full = nb / round * round;
+    if (b >= full) return b;
+    const int x = b % 8, j = b / 8;  // the XCD and the block's turn on it
+    return (j / kXcdRun) * round + x * kXcdRun + j % kXcdRun;
+}
+
+// the tile of block b under tuning mode xcd (0: block order, 1: 8 contiguous
+// ranges, 2: runs of kXcdRun tiles in turn)
+MPMX_HD int tile_of_block(int xcd, int b, int nb) {
+    return xcd == 1 ? xcd_tile(b, nb) : xcd == 2 ? xcd_run_tile(b, nb) : b;
+}
+
 constexpr int kBlock = 256;
 
 struct f3 {

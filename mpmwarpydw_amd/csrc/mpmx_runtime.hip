@@ -16,8 +16,8 @@
 namespace mpmx {
 
 static thread_local char g_err[512] = "";
-// XCD tile dealing on, split constitutive update on, G2P per tile
-static int g_tuning[MPMX_TUNE_COUNT] = {1, 1, 1, 0};
+// tiles dealt to the XCDs in runs of 16, split constitutive update on, G2P per tile
+static int g_tuning[MPMX_TUNE_COUNT] = {2, 1, 1, 0};
 
 int tuning(int knob) { return (knob >= 0 && knob < MPMX_TUNE_COUNT) ? g_tuning[knob] : 0; }
 
@@ -635,7 +635,8 @@ int mpmx_version(void) { return 1; }
 
 int mpmx_set_tuning(int knob, int value) {
     MPMX_REQUIRE(knob >= 0 && knob < MPMX_TUNE_COUNT, "mpmx_set_tuning: unknown knob %d", knob);
-    const int vmax = (knob == MPMX_TUNE_G2P_TILES || knob == MPMX_TUNE_P2G_CHUNK) ? 2 : 1;
+    const int vmax =
+        (knob == MPMX_TUNE_G2P_TILES || knob == MPMX_TUNE_P2G_CHUNK || knob == MPMX_TUNE_XCD_TILES) ? 2 : 1;
     MPMX_REQUIRE(value >= 0 && value <= vmax, "mpmx_set_tuning: knob %d takes 0 to %d", knob, vmax);
     g_tuning[knob] = value;
     return 0;

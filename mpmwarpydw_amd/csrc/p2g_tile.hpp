@@ -281,7 +281,7 @@ __device__ __forceinline__ void p2g_tile_body(const TileArgs& a) {
     float (*rec)[kRecCols] = reinterpret_cast<float (*)[kRecCols]>(smem4);
     float4* part = smem4;
 
-    const int t = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int t = tile_of_block(a.xcd, (int)blockIdx.x, (int)gridDim.x);
     const int tid = threadIdx.x;
     if (FUSED && t == a.ntiles) {
         orphan_bin(a);

@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(kBlock) g2p_tile_kernel(G2PArgs a) {
     const mpmx_particles& P = a.P;
     const long n = P.n;
     // a.segs blocks share a tile, block h of them taking rounds h, h + segs, ...
-    const int b = a.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int b = tile_of_block(a.xcd, (int)blockIdx.x, (int)gridDim.x);
     const int t = b / a.segs, h = b - t * a.segs;
     const long ntiles = a.geo.ntiles();
     const int orphan_key = (int)(ntiles * kTileCells);

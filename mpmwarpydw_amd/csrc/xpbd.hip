@@ -656,13 +656,14 @@ __global__ void __launch_bounds__(kBlock) particle_contacts_kernel(
     __shared__ int32_t queue[kContactQueue][kBlock];
     // one thread per table position, or per listed position while the list is
     // short (it holds the walks that can meet a contact); the positions go to
-    // the XCDs in contiguous ranges, so the cells a block walks are those of the
-    // blocks beside it in the table and share one L2.  A list holding more than
+    // the XCDs in runs of 16 blocks in turn, so the cells a block walks are
+    // mostly those of the blocks beside it in the table and share one L2, and a
+    // crowded part of the table spreads over every XCD.  A list holding more than
     // an eighth of the particles is not used: its order is that of the particle
     // arrays, and table order walks neighbouring cells from neighbouring lanes
     // (2.0x faster on a 2M-particle scene with 1M XPBD points).  The results are
     // the same either way: each walk writes only its own particle's delta.
-    const int tid = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int tid = xcd_run_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int pos = tid;
     if (PACKED && pk.list) {
         const int cnt = *pk.list_count;

@@ -10,21 +10,25 @@ pytestmark = pytest.mark.gpu
 
 # (split_stress, g2p_tiles, xcd_tiles, p2g_chunk knob)
 COMBOS = [(1, 1, 1, 0), (1, 2, 1, 0), (1, 2, 0, 0), (1, 0, 1, 0), (1, 0, 0, 0), (0, 1, 1, 0), (0, 2, 1, 0),
-          (0, 0, 1, 0), (1, 1, 1, 1), (1, 2, 0, 2)]
+          (0, 0, 1, 0), (1, 1, 1, 1), (1, 2, 0, 2), (1, 1, 2, 0), (0, 2, 2, 0)]
+# the slab at half size has more than 200 tiles (50 x 25 x 12 cells and more), so
+# runs of 16 tiles in turn, which need at least 128, move blocks there
+SLAB_COMBOS = [(1, 1, 1, 0), (1, 1, 2, 0), (1, 2, 2, 1), (0, 1, 2, 0), (1, 1, 0, 0)]
 
 
-def test_knobs_leave_results_unchanged():
+@pytest.mark.parametrize("scene,combos", [("block_collapse", COMBOS), ("undercut_slab", SLAB_COMBOS)])
+def test_knobs_leave_results_unchanged(scene, combos):
     from mpmwarpydw_amd import autotune
     from mpmwarpydw_amd import simulationRoutines as SR
     ref = None
     try:
-        for split, g2p, xcd, chunk in COMBOS:
+        for split, g2p, xcd, chunk in combos:
             autotune.set_split_stress(bool(split))
             autotune.set_g2p_tiles(g2p)
-            autotune.set_xcd_tiles(bool(xcd))
+            autotune.set_xcd_tiles(xcd)
             autotune.set_p2g_chunk(chunk)
-            S = make_sim(seed=50)
-            print(f"[knobs] split={split} g2p_tiles={g2p} xcd={xcd} p2g_chunk={chunk}", flush=True)
+            S = make_sim(scene, seed=50)
+            print(f"[knobs] {scene} split={split} g2p_tiles={g2p} xcd={xcd} p2g_chunk={chunk}", flush=True)
             for counter in range(1, 13):
                 S.substep(counter)
             assert SR.misbinned_count(S.sim, S.mpm) == 0
@@ -37,7 +41,7 @@ def test_knobs_leave_results_unchanged():
     finally:
         autotune.set_split_stress(True)
         autotune.set_g2p_tiles(True)
-        autotune.set_xcd_tiles(True)
+        autotune.set_xcd_tiles(2)
         autotune.set_p2g_chunk(0)
 
 
@@ -61,7 +65,8 @@ def test_autotune_leaves_state_at_start_in_its_order(orders):
         log = autotune.tune(S, step, steps=2, warmup=1, trials=1, orders=orders)
         n_cand = len(log["candidates"])
         print(f"[autotune] orders {orders}: {n_cand} candidates, best {log['best']}", flush=True)
-        assert n_cand == len(orders) * (3 * 3 * 2 + 3 * 2)
+        # split: 3 chunk sizes x 3 G2P forms x 3 XCD mappings; fused: 3 G2P forms x 3 XCD mappings
+        assert n_cand == len(orders) * (3 * 3 * 3 + 3 * 3)
         assert log["kept_reordered"] == log["best"]["reorder"]
         cur = autotune.current()
         for k in ("xcd_tiles", "split_stress", "g2p_tiles"):
@@ -76,4 +81,4 @@ def test_autotune_leaves_state_at_start_in_its_order(orders):
         for k in a:
             assert np.array_equal(a[k], b[k]), k
     finally:
-        autotune.apply({"xcd_tiles": 1, "split_stress": 1, "g2p_tiles": 1, "p2g_chunk": 512})
+        autotune.apply({"xcd_tiles": 2, "split_stress": 1, "g2p_tiles": 1, "p2g_chunk": 512})
